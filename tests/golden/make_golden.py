@@ -1,0 +1,293 @@
+"""Write golden input/output vectors from the reference PulseSuite SBE path.
+
+Runs ONLY in the build container, where the reference tree is readable at
+``/root/reference``.  It is never imported by the tests: the tests read the
+``.npz`` fixtures this script writes.
+
+The reference imports ``numba`` and ``pyfftw`` unguarded (``coulomb.py:18``,
+``usefulsubs.py:18``); neither is installed here.  They are stubbed before the
+import: ``numba.jit`` becomes the identity decorator and ``prange`` is
+``range`` (so the JIT kernels run as plain Python, IEEE arithmetic, no
+fastmath), and ``pyfftw.interfaces.numpy_fft`` is ``numpy.fft`` (pinned equal
+at rtol 1e-12 by ``tests/PSTD3D/test_usefulsubs.py:1104-1141``).
+``numba.cuda`` is left absent, which the reference catches
+(``SBEs.py:18-23``).
+
+Only functions that no closed form, library routine or invariant pins are
+recorded (composed and index-heavy behaviour):
+
+* the grid and every init table of ``SBESolver.__init__`` (``SBEs.py:4623``),
+  including ``CalcCoulombArrays``/``Vint``/``_K03_jit`` (``coulomb.py:38-177``)
+  and ``CalcChi1D``;
+* ``SBESolver._calc_H`` -> ``_CalcH_jit`` (``SBEs.py:808-924``);
+* ``CalcGammaE``/``CalcGammaH`` (``dephasing.py:541-625``);
+* ``OffDiagDephasing2`` (``dephasing.py:714-743``);
+* ``MBCE``/``MBCH`` (``coulomb.py:1298-1326``);
+* ``QWOptics.QWRho5`` (``qwoptics.py:942-1071``);
+* ``SBESolver.QWCalculator`` trajectories (``SBEs.py:5826-5985``), with the
+  shipped ``mb.params`` and with the many-body relaxation/transport switched
+  off so the leapfrog RHS can be read back from the state.
+
+Usage: ``python tests/golden/make_golden.py`` (writes ``tests/golden/*.npz``).
+"""
+
+import contextlib
+import io
+import os
+import sys
+import tempfile
+import types
+
+import numpy as np
+
+REF_SRC = "/root/reference/src"
+OUT_DIR = os.path.dirname(os.path.abspath(__file__))
+
+MB_SHIPPED = [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 0, 1, 0]
+# Optics, Excitons, EHs, Screened, Phonon, DCTrans, LF, FreePot, DiagDph,
+# OffDiagDph, Recomb, PLSpec, ignorewire, Xqwparams, LorentzDelta
+MB_RHS_ONLY = [1, 1, 0, 1, 0, 0, 1, 0, 1, 1, 0, 0, 0, 0, 0]
+
+
+def _install_shims():
+    numba = types.ModuleType("numba")
+
+    def jit(*args, **kwargs):
+        if args and callable(args[0]) and len(args) == 1 and not kwargs:
+            return args[0]
+
+        def deco(f):
+            return f
+
+        return deco
+
+    numba.jit = jit
+    numba.njit = jit
+    numba.prange = range
+    sys.modules["numba"] = numba
+
+    pyfftw = types.ModuleType("pyfftw")
+    interfaces = types.ModuleType("pyfftw.interfaces")
+    cache = types.ModuleType("pyfftw.interfaces.cache")
+    cache.enable = lambda *a, **k: None
+    interfaces.cache = cache
+    interfaces.numpy_fft = np.fft
+    pyfftw.interfaces = interfaces
+    sys.modules["pyfftw"] = pyfftw
+    sys.modules["pyfftw.interfaces"] = interfaces
+    sys.modules["pyfftw.interfaces.cache"] = cache
+    sys.modules["pyfftw.interfaces.numpy_fft"] = np.fft
+
+
+def wire_length(nk, me_rel=0.067, gap_ev=1.5):
+    """Wire length L whose derived grid has exactly ``nk`` points."""
+    from scipy.constants import e, hbar, m_e
+
+    kmax = np.sqrt(1.2 * gap_ev * e * 2.0 * me_rel * m_e / hbar**2)
+    return (nk / 2 + 0.5) * np.pi / kmax
+
+
+def write_params(d, L, mb, me_rel=0.067):
+    os.makedirs(os.path.join(d, "params"), exist_ok=True)
+    qw = [L, 5e-9, 1.5, me_rel, 0.45, 100e-3, 1e12, 1e12, 1e12, 9.1,
+          36e-3, 3e-3, 0.0, 500, 20000]
+    with open(os.path.join(d, "params", "qw.params"), "w") as f:
+        for v in qw:
+            f.write(f"{v if isinstance(v, int) else float(v)!r}\n")
+    with open(os.path.join(d, "params", "mb.params"), "w") as f:
+        for v in mb:
+            f.write(f"{v}\n")
+
+
+def random_state(rng, n, occ=0.3, coh=0.05, pol=0.02):
+    """Hermitian C, D with diagonal occupations and off-diagonal coherences,
+    and a dense complex p."""
+    def herm(scale_d, scale_o):
+        a = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        a = (a + a.conj().T) * 0.5 * scale_o
+        np.fill_diagonal(a, rng.uniform(0.0, scale_d, n))
+        return a
+
+    C = herm(occ, coh)
+    D = herm(occ, coh)
+    p = pol * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    return C, D, p
+
+
+def pulse(t, E0, tw, tp, w0):
+    """sbetestprop.py:187-192 analytic pulse."""
+    x = w0 * (t - tp)
+    return (E0 * np.exp(-(x**2) / (w0 * tw) ** 2) * np.cos(x)
+            * np.exp(-(x**20) / (2 * tw * w0) ** 20))
+
+
+def run_case(nk, seed, mb, nsteps, tag, nx=64, drr=10e-9):
+    rng = np.random.default_rng(seed)
+    L = wire_length(nk)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            write_params(d, L, mb)
+            buf = io.StringIO()
+            with contextlib.redirect_stdout(buf):
+                from pulsesuite.PSTD3D import (SBEs, coulomb, dephasing,
+                                               emission, phonons)
+                from pulsesuite.PSTD3D.typespace import (GetKArray,
+                                                         GetSpaceArray)
+
+                rr = GetSpaceArray(nx, (nx - 1) * drr)
+                qrr = GetKArray(nx, nx * drr)
+                lam = 800e-9
+                E0x, E0y = 2e7, 4e7
+                s = SBEs.SBESolver(qrr, rr, 0.0, np.hypot(E0x, E0y), lam, 2, True)
+            n = s.Nk
+            assert n == nk, (n, nk)
+            ci = coulomb._instance
+            out.update(
+                L=L, mb=np.array(mb), nk=n, nr=s.Nr, kr=s.kr, r=s.r, Qr=s.Qr,
+                NQ0=s.NQ0, kkp=s.kkp, Ee=s.Ee, Eh=s.Eh, dcv=s.dcv,
+                ehint=s.ehint, area=s.area, alphae=s.alphae, alphah=s.alphah,
+                qc=s.qc, dkr=s.dkr, dr=s.dr, rr=rr, qrr=qrr,
+                Veh0=ci.Veh0, Vee0=ci.Vee0, Vhh0=ci.Vhh0, qe=ci.qe, qh=ci.qh,
+                Chi1De=ci.Chi1De, Chi1Dh=ci.Chi1Dh,
+                EP=phonons._EP, HP=phonons._HP, Vscale=phonons._Vscale,
+                NO=phonons._NO,
+                k_p_q=dephasing._k_p_q, k_m_q=dephasing._k_m_q,
+                k1_m_q=dephasing._k1_m_q, k1p_m_q=dephasing._k1p_m_q,
+                k1=dephasing._k1, k1p=dephasing._k1p,
+                xe=dephasing._xe, xh=dephasing._xh,
+                Expikr=s.qw._Expikr, QWWindow=s.qw._QWWindow,
+            )
+            if s.Recomb:
+                out.update(HOmega=emission._HOmega, square=emission._square,
+                           RScale=emission._RScale)
+            if nk <= 16:
+                out.update(Ceh=ci.Ceh, Cee=ci.Cee, Chh=ci.Chh, k3=ci.k3)
+
+            # ---- per-function vectors on a random many-body state ----
+            C, D, p = random_state(rng, n)
+            ne = np.diag(C).copy()
+            nh = np.diag(D).copy()
+            out.update(fC=C, fD=D, fp=p)
+            VC = np.zeros((n, n, 3))
+            E1D = np.zeros((n, n))
+            with contextlib.redirect_stdout(buf):
+                coulomb.CalcScreenedArrays(True, s.L, ne, nh, VC, E1D)
+            out.update(fVC=VC.copy(), fE1D=E1D.copy())
+            # Hamiltonians with a random dipole-coupling matrix
+            Meh = 1e-22 * (rng.standard_normal((n, n))
+                           + 1j * rng.standard_normal((n, n)))
+            Z = np.zeros((n, n), dtype=complex)
+            Heh = np.zeros((n, n), dtype=complex)
+            Hee = np.zeros((n, n), dtype=complex)
+            Hhh = np.zeros((n, n), dtype=complex)
+            s._calc_H(Meh, Z, Z, C, D, p, VC, Heh, Hee, Hhh)
+            out.update(fMeh=Meh, fHeh=Heh, fHee=Hee, fHhh=Hhh)
+            GamE = np.zeros(n)
+            GamH = np.zeros(n)
+            dephasing.CalcGammaE(s.kr, ne, nh, VC, GamE)
+            dephasing.CalcGammaH(s.kr, ne, nh, VC, GamH)
+            out.update(fGamE=GamE, fGamH=GamH)
+            g = np.array([s.gam_eh, s.gam_e, s.gam_h])
+            OffG = np.zeros((n, n), dtype=complex)
+            dephasing.OffDiagDephasing2(ne, nh, p, s.kr, s.Ee, s.Eh, g, VC,
+                                        0.0, OffG)
+            out.update(fOffP=OffG)
+            WinE = np.zeros(n)
+            WoutE = np.zeros(n)
+            WinH = np.zeros(n)
+            WoutH = np.zeros(n)
+            coulomb.MBCE(np.real(ne), np.real(nh), s.kr, s.Ee, s.Eh, VC,
+                         s.gam_eh, s.gam_e, WinE, WoutE)
+            coulomb.MBCH(np.real(ne), np.real(nh), s.kr, s.Ee, s.Eh, VC,
+                         s.gam_eh, s.gam_h, WinH, WoutH)
+            out.update(fWinE=WinE, fWoutE=WoutE, fWinH=WinH, fWoutH=WoutH)
+            re = np.zeros(s.Nr, dtype=complex)
+            rh = np.zeros(s.Nr, dtype=complex)
+            s.qw.QWRho5(s.Qr, s.kr, s.r, s.L, s.kkp, p, C, D, ne, nh, re, rh,
+                        1, 1)
+            out.update(fre=re, frh=rh)
+
+            # ---- QWCalculator trajectory from a seeded many-body state ----
+            for w in range(2):
+                C0, D0, p0 = random_state(rng, n, occ=0.2, coh=0.02, pol=0.01)
+                C1, D1, p1 = random_state(rng, n, occ=0.2, coh=0.02, pol=0.01)
+                s.CC2[:, :, w] = C0
+                s.DD2[:, :, w] = D0
+                s.YY2[:, :, w] = p0
+                s.CC3[:, :, w] = C1
+                s.DD3[:, :, w] = D1
+                s.YY3[:, :, w] = p1
+            out.update(s0_C2=s.CC2.copy(), s0_D2=s.DD2.copy(),
+                       s0_p2=s.YY2.copy(), s0_C3=s.CC3.copy(),
+                       s0_D3=s.DD3.copy(), s0_p3=s.YY3.copy())
+            dt = 10e-18
+            w0 = 2 * np.pi * 299792458.0 / lam
+            Exx = np.zeros(nx, dtype=complex)
+            Eyy = np.zeros(nx, dtype=complex)
+            Ezz = np.zeros(nx, dtype=complex)
+            Vrr = np.zeros(nx, dtype=complex)
+            traj = {k: [] for k in ("Pxx", "Pyy", "Pzz", "Rho", "Exx", "Eyy",
+                                    "C2", "D2", "p2", "C3", "D3", "p3", "I0")}
+            t = 40e-15
+            for it in range(nsteps):
+                # a spatially varying field so Prop2QW's spline is exercised
+                env = np.cos(2 * np.pi * rr / (nx * drr)) + 0.3
+                Exx[:] = pulse(t, E0x, 10e-15, 50e-15, w0) * env
+                Eyy[:] = pulse(t, E0y, 10e-15, 50e-15, w0) * env
+                for w in (1, 2):
+                    Pxx = np.zeros(nx, dtype=complex)
+                    Pyy = np.zeros(nx, dtype=complex)
+                    Pzz = np.zeros(nx, dtype=complex)
+                    Rho = np.zeros(nx, dtype=complex)
+                    with contextlib.redirect_stdout(buf):
+                        s.QWCalculator(Exx, Eyy, Ezz, Vrr, rr, qrr, dt, w, Pxx,
+                                       Pyy, Pzz, Rho, [True], [False])
+                    traj["Pxx"].append(Pxx)
+                    traj["Pyy"].append(Pyy)
+                    traj["Pzz"].append(Pzz)
+                    traj["Rho"].append(Rho)
+                traj["Exx"].append(Exx.copy())
+                traj["Eyy"].append(Eyy.copy())
+                traj["C2"].append(s.CC2.copy())
+                traj["D2"].append(s.DD2.copy())
+                traj["p2"].append(s.YY2.copy())
+                traj["C3"].append(s.CC3.copy())
+                traj["D3"].append(s.DD3.copy())
+                traj["p3"].append(s.YY3.copy())
+                traj["I0"].append(s.I0.copy())
+                t += dt
+            for k, v in traj.items():
+                out["t_" + k] = np.array(v)
+            out.update(dt=dt, t0=40e-15, nsteps=nsteps, E0x=E0x, E0y=E0y,
+                       lam=lam, Emax0=np.hypot(E0x, E0y))
+            info = {}
+            for w in (1, 2):
+                fn = os.path.join("dataQW", f"info.{w:02d}.t.dat")
+                with open(fn) as f:
+                    info[w] = np.array([[complex(tok) for tok in line.split()]
+                                        for line in f if line.strip()])
+            out["info1"] = info[1]
+            out["info2"] = info[2]
+            s.close()
+        finally:
+            os.chdir(cwd)
+    path = os.path.join(OUT_DIR, f"ref_{tag}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "N =", nk)
+
+
+def main():
+    if not os.path.isdir(REF_SRC):
+        raise SystemExit("reference tree not found; run in the build container")
+    _install_shims()
+    sys.path.insert(0, REF_SRC)
+    run_case(16, seed=1, mb=MB_SHIPPED, nsteps=4, tag="n16_mb")
+    run_case(16, seed=2, mb=MB_RHS_ONLY, nsteps=3, tag="n16_rhs")
+    run_case(32, seed=3, mb=MB_SHIPPED, nsteps=3, tag="n32_mb")
+
+
+if __name__ == "__main__":
+    main()
