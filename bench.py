@@ -1,0 +1,265 @@
+"""Benchmark: batched quantum-wire SBE stepping on one MI355X.
+
+Workload (the metric's configuration): 64 wires, Nk = 256 each (wire length
+from the k-grid relation, me = 0.067, gap 1.5 eV), full shipped mb.params,
+fields on a 4096-point propagation grid (drr = 1 nm), per-wire pulse
+amplitudes log-uniform in [1e5, 1e7] V/m from a seed, dt = 10 as.  A step is
+one QWCalculator step of every wire (Prop2QW, Hamiltonians, dephasing, RHS
++ leapfrog, relaxation, transport, averaging, sources, QW2Prop) with the
+fields already resident in HBM.
+
+Prints one JSON line (see --help).  --gpus > 1 is not supported: the
+product runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SBE steps/s (Nk=256, 64 wires) mixed vs FP64 @<=1e-6 err; p50/p99 sim latency"
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def wire_length(nk, me_rel=0.067, gap_ev=1.5):
+    from scipy.constants import e, hbar, m_e
+    kmax = np.sqrt(1.2 * gap_ev * e * 2.0 * me_rel * m_e / hbar**2)
+    return (nk / 2 + 0.5) * np.pi / kmax
+
+
+def seeded_state(rng, n, nw):
+    """Levels 2 and 3 in the reference layout [n, n, nw]: Hermitian C, D
+    with occupations up to 0.2 and coherences 0.02, p of size 0.01."""
+    lv = []
+    for _ in range(2):
+        for kind in ("C", "D", "p"):
+            a = np.zeros((n, n, nw), dtype=complex)
+            for w in range(nw):
+                m = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+                if kind == "p":
+                    a[:, :, w] = 0.01 * m
+                else:
+                    h = (m + m.conj().T) * 0.01
+                    np.fill_diagonal(h, rng.uniform(0.0, 0.2, n))
+                    a[:, :, w] = h
+            lv.append(a)
+    return lv
+
+
+def make_fields(rng, rr, nw, L):
+    w0 = 2 * np.pi * 299792458.0 / 815.7e-9
+    E0 = 10 ** rng.uniform(5, 7, nw)
+    env = np.exp(-((rr / (4 * L)) ** 2))
+    t = 0.0
+    tw = 100e-15 / (2 * np.sqrt(np.log(2)))
+    base = np.exp(-(t / tw) ** 2) * np.cos(w0 * t)
+    Exx = (E0[:, None] * base * env[None, :]).astype(complex)
+    return Exx, 2.0 * Exx, np.zeros_like(Exx)
+
+
+def build_group(prec, nw, N, nx, seed):
+    from pulsesuite_amd.engine import WireGroup
+    from pulsesuite_amd.tables import QWParams
+    L = wire_length(N)
+    prm = QWParams(L=L, me_rel=0.067)
+    rr = np.linspace(-(nx - 1) * 1e-9 / 2, (nx - 1) * 1e-9 / 2, nx)
+    wg = WireGroup(prm, nw, rr, prec=prec)
+    rng = np.random.default_rng(seed)
+    wg.set_state(*seeded_state(rng, wg.N, nw))
+    Exx, Eyy, Ezz = make_fields(rng, rr, nw, L)
+    wg.set_fields(Exx, Eyy, Ezz)
+    return wg, prm, rr
+
+
+def time_steps(wg, steps, warmup, dt):
+    import torch
+    for _ in range(warmup):
+        wg.step(dt)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        wg.step(dt)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for a, b in evs])
+    return wall, lat
+
+
+def time_rhs_kernel(wg, reps):
+    """Average duration of the RHS GEMM launch (the dominant kernel), HIP
+    events on the stream it is launched on."""
+    import torch
+    wg.prop2qw()
+    wg.prep()
+    wg.calc_h()
+    wg.offdiag()
+    wg.rhs(1, 1e-17)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        wg.rhs(1, 1e-17)
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps / 1e3  # seconds
+
+
+def accuracy_vs_fp64(N, nx, nw_err, steps, dt, seed):
+    """Density and polarization error of the mixed path against the FP64
+    path after `steps` steps from the same state."""
+    import torch
+    a, _, _ = build_group("fp64", nw_err, N, nx, seed)
+    b, _, _ = build_group("fp32", nw_err, N, nx, seed)
+    for _ in range(steps):
+        a.step(dt)
+        b.step(dt)
+    torch.cuda.synchronize()
+    Sa, Sb = a.get_state(), b.get_state()
+    ne_a = np.real(np.einsum("iiw->iw", Sa[3]))
+    ne_b = np.real(np.einsum("iiw->iw", Sb[3]))
+    Pa, Pb = a.outputs()[0], b.outputs()[0]
+    rel = lambda x, y: float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))  # noqa: E731
+    out = dict(density=rel(ne_b, ne_a), polarization=rel(Pb, Pa),
+               p_matrix=rel(Sb[5], Sa[5]))
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(wg, prm, rr, dt, budget_s=20.0):
+    """The oracle (NumPy FP64 restatement of the reference) on the host:
+    one wire-step of the same workload, repeated until ~budget_s."""
+    from oracle import sbe_oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:  # pragma: no cover
+        threads = 1
+    oprm = O.Params(L=prm.L, me_rel=0.067)
+    g = O.Grid(oprm)
+    vt = np.zeros((3, g.Nr))
+    vt[:, :g.Nk] = wg.tab["vtab"]
+    t = O.Tables(g, vtab=vt)
+    s = O.Solver(oprm, 1, rr, 1.0, tables=t, grid=g)
+    S = wg.get_state()
+    for dst, src in zip((s.CC2, s.DD2, s.YY2, s.CC3, s.DD3, s.YY3), S):
+        dst[:, :, 0] = src[:, :, 0]
+    Exx = wg.s["efield"][0, 0].cpu().numpy()
+    Eyy = wg.s["efield"][1, 0].cpu().numpy()
+    Ezz = wg.s["efield"][2, 0].cpu().numpy()
+    s._mb()  # the many-body tables are built once, as in the reference init
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        s.qw_calculator(Exx, Eyy, Ezz, dt, 1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 3:
+            break
+    per_wire_step = el / n
+    return dict(value=1.0 / (per_wire_step * wg.nw), unit="steps/s", cores=int(threads),
+                kind="port",
+                sample=f"{n} single-wire QWCalculator step(s) of the Nk=256 workload with the "
+                       f"NumPy oracle ({per_wire_step:.2f} s/wire-step), scaled to 64 wires")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--wires", type=int, default=64)
+    ap.add_argument("--nk", type=int, default=256)
+    ap.add_argument("--nx", type=int, default=4096)
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only (the SBE wire batch is stepped on a single "
+              "MI355X); multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    dt = 10e-18
+    seed = 1234
+    print(f"[bench] building {args.wires} wires Nk={args.nk} Nx={args.nx} fp64", file=sys.stderr, flush=True)
+    wg, prm, rr = build_group("fp64", args.wires, args.nk, args.nx, seed)
+    N = wg.N
+    print("[bench] timing fp64", file=sys.stderr, flush=True)
+    wall64, lat64 = time_steps(wg, args.steps, args.warmup, dt)
+    sps64 = args.steps / wall64
+    t_rhs = time_rhs_kernel(wg, 20)
+    flops_rhs = args.wires * 104.0 * N**3
+    achieved = flops_rhs / t_rhs / 1e12
+    res = dict(fp64_steps_per_s=sps64, fp64_ms_per_step=1e3 * wall64 / args.steps,
+               fp64_p50_ms=float(np.percentile(lat64, 50)),
+               fp64_p99_ms=float(np.percentile(lat64, 99)))
+    value, ms, lat, dtype = sps64, 1e3 * wall64 / args.steps, lat64, "f64"
+    if not args.no_mixed:
+        try:
+            print("[bench] timing fp32", file=sys.stderr, flush=True)
+            del wg
+            torch.cuda.empty_cache()
+            wg32, _, _ = build_group("fp32", args.wires, args.nk, args.nx, seed)
+            wall32, lat32 = time_steps(wg32, args.steps, args.warmup, dt)
+            res.update(mixed_steps_per_s=args.steps / wall32,
+                       mixed_p50_ms=float(np.percentile(lat32, 50)),
+                       mixed_p99_ms=float(np.percentile(lat32, 99)),
+                       mixed_speedup_vs_fp64=(args.steps / wall32) / sps64)
+            del wg32
+            torch.cuda.empty_cache()
+            err = accuracy_vs_fp64(args.nk, args.nx, 4, 20, dt, seed)
+            res["mixed_error_vs_fp64_after_20_steps"] = err
+            if max(err["density"], err["polarization"]) <= 1e-6:
+                value, ms, lat, dtype = args.steps / wall32, 1e3 * wall32 / args.steps, lat32, "f32+f64"
+            wg, _, _ = build_group("fp64", args.wires, args.nk, args.nx, seed)
+        except Exception as ex:  # report, keep the FP64 number
+            res["mixed_error"] = repr(ex)
+            wg, _, _ = build_group("fp64", args.wires, args.nk, args.nx, seed)
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(wg, prm, rr, dt)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded many-body state, seeded pulse amplitudes)",
+        "config": {"workload": f"{args.wires} quantum wires x Nk={args.nk}, Nx={args.nx} "
+                               "propagation grid, shipped mb.params, dt=10 as, "
+                               "one QWCalculator step of every wire per step",
+                   "wires": args.wires, "Nk": N, "Nx": args.nx},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "roofline": {"bound": "mfma", "kernel": "rhs gemm (dp/dC/dD + leapfrog), FP64 MFMA",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "avg_launch_ms": t_rhs * 1e3,
+                     "flops_per_launch": flops_rhs},
+        "cpu_baseline": cpu,
+        "extra": res,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+/*
+ * pulsesuite_amd: MI355X (gfx950) quantum-wire SBE time step, C ABI.
+ *
+ * The reference (rahulranjansah/pulsesuite) is a Python module API with no
+ * FFI; each entry point below replaces one reference function (or one
+ * composed step) and is what a ctypes/cffi binding of that function binds.
+ * Citations are relative to src/pulsesuite/PSTD3D/ of the reference.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success, a negative PSA_E* code on
+ *    failure, never throws or aborts; psa_last_error() gives the message
+ *    (thread-local).
+ *  - Pointers inside psa_group are DEVICE pointers (HBM), complex values are
+ *    interleaved (re, im) doubles, matrices row-major [N][N], per-wire
+ *    blocks wire-major ([Nw][...]).  `stream` is a hipStream_t (NULL = the
+ *    default stream).  Calls are asynchronous on that stream.
+ *  - A psa_group describes Nw wires that share one k-grid size N (one
+ *    SBESolver of the reference).  It is a plain struct the caller fills;
+ *    the library keeps no state between calls except lazily-built kernels.
+ *  - Calls that write the same group's buffers must not overlap.
+ */
+#ifndef PULSESUITE_AMD_H
+#define PULSESUITE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSA_ABI_VERSION 1
+
+#define PSA_OK 0
+#define PSA_E_INVALID (-1)   /* bad argument / shape */
+#define PSA_E_HIP (-2)       /* HIP runtime error */
+#define PSA_E_UNSUPPORTED (-3)
+
+/* precision modes of the dense contractions */
+#define PSA_PREC_FP64 0      /* FP64 MFMA, reference-equivalent */
+#define PSA_PREC_FP32 1      /* FP32 MFMA operands, FP32 accumulate */
+#define PSA_PREC_BF16X3 2    /* BF16 hi/lo split, 3 products, FP32 accumulate */
+
+/* mb.params flags (SBEs.py:5004-5022), bit i = line i+1 */
+#define PSA_F_OPTICS (1 << 0)
+#define PSA_F_EXCITONS (1 << 1)
+#define PSA_F_EHS (1 << 2)
+#define PSA_F_SCREENED (1 << 3)
+#define PSA_F_PHONON (1 << 4)
+#define PSA_F_DCTRANS (1 << 5)
+#define PSA_F_LF (1 << 6)
+#define PSA_F_FREEPOT (1 << 7)
+#define PSA_F_DIAGDPH (1 << 8)
+#define PSA_F_OFFDIAGDPH (1 << 9)
+#define PSA_F_RECOMB (1 << 10)
+
+typedef struct psa_group {
+  /* ---- sizes ---- */
+  int32_t N;        /* k points per wire (Nk) */
+  int32_t Nr;       /* QW real-space points, 2N+2 */
+  int32_t Nw;       /* wires in the group */
+  int32_t Nx;       /* propagation-grid points (rr) */
+  int32_t Nhw;      /* emission photon-energy points */
+  int32_t flags;    /* PSA_F_* */
+  int32_t prec;     /* PSA_PREC_* */
+  int32_t gx0, gnx; /* support of the Prop2QW transform in rr: [gx0, gx0+gnx) */
+  int32_t bx0, bnx; /* rows of rr inside the wire for QW2Prop: [bx0, bx0+bnx) */
+  int32_t nnz_mb[8];/* entries in each many-body scattering list */
+  int32_t pad0;
+  /* ---- scalars (SI units) ---- */
+  double hbar, e0, gap, L, ehint, dcv, area, dkr, dr_r, drr;
+  double gam_e, gam_h, gam_eh, Edc, Vscale, n1d_max, dhw;
+  /* ---- per-N tables (read only) ---- */
+  const double *kr, *Ee, *Eh;         /* [N] */
+  const double *vtab;                 /* [3][N] unscreened V0(d), d = k-k' >= 0 (eh, ee, hh) */
+  const double *chi;                  /* [2][N] Chi1De(d), Chi1Dh(d) */
+  const double *qeh;                  /* [2][N] qe(d), qh(d) */
+  const double *EP, *HP;              /* [N][N] phonon tables */
+  const double *xe, *xh;              /* [N] dephasing weights */
+  const int32_t *didx;                /* [6][N][N] k_p_q, k_m_q, k1p_m_q, k1p, k1_m_q, k1 */
+  const int32_t *iq;                  /* [N] Vxx2 index rint(|ky/dq|) */
+  const double *homega, *hsq;         /* [Nhw] emission grid and weights */
+  const double *window;               /* [Nr] QW window */
+  const double *rq;                   /* [Nr] QW real-space grid r */
+  const double *ytr;                  /* [N] Transport phase ramp Y */
+  const double *gmat;                 /* [2N-1][gnx] complex: Prop2QW (spline, window, FFTG) */
+  const double *m2;                   /* [bnx][Nr] real: QW2Prop spline r -> rr */
+  const int32_t *mb_off[8];           /* CSR offsets [N+1] per many-body list */
+  const uint32_t *mb_idx[8];          /* packed (i, j, l) per entry */
+  const double *mb_val[8];            /* table value per entry */
+  /* ---- per-wire data ---- */
+  const int32_t *wire_id;             /* [Nw] 1-based reference wire index w (host-visible copy in wid_host) */
+  double *pA, *pB, *pX;               /* [Nw][N][N] complex: YY2, YY3, new level */
+  double *CA, *CB, *CX;
+  double *DA, *DB, *DX;
+  double *Heh, *Heo, *Hho;            /* [Nw][N][N] complex: Heh, Hee/Hhh without diagonal */
+  double *hdiag;                      /* [Nw][2][N] complex: diagonals of Hee, Hhh */
+  double *prep;                       /* [Nw][PSA_PREP_STRIDE(N)] doubles, see PSA_PREP_* */
+  double *dedh;                       /* [Nw][2][2N-1][N] off-diagonal dephasing tables */
+  double *wmb;                        /* [Nw][6][N] many-body Win_e, Wout_e, Win_h, Wout_h, |Re ne3|, |Re nh3| */
+  double *exq;                        /* [Nw][3][2N-1] complex: FFTG'd Ex, Ey, Ez at q = -(N-1)..N-1 */
+  const double *efield;               /* [3][Nw][Nx] complex fields on rr (Exx, Eyy, Ezz) */
+  double *src;                        /* [Nw][PSA_SRC_STRIDE] real-space Px, Py, Pz, re, rh + diagonal sums */
+  double *out;                        /* [4][Nw][Nx] complex: Pxx, Pyy, Pzz, Rho */
+  double *obs;                        /* [Nw][PSA_OBS_N] observables (I0, totals, ...) */
+  const int32_t *wid_host;            /* [Nw] host copy of wire_id */
+} psa_group;
+
+/* layout of the per-wire prep block (doubles) */
+#define PSA_PREP_NE 0                         /* [N] complex */
+#define PSA_PREP_NH(N) (2 * (N))              /* [N] complex */
+#define PSA_PREP_E1D(N) (4 * (N))             /* [N] */
+#define PSA_PREP_VC(N) (5 * (N))              /* [3][N] screened V(d) */
+#define PSA_PREP_DSC(N) (8 * (N))             /* [2N-1] complex diagonal sums of C */
+#define PSA_PREP_DSD(N) (8 * (N) + 2 * (2 * (N) - 1))
+#define PSA_PREP_GAME(N) (8 * (N) + 4 * (2 * (N) - 1))
+#define PSA_PREP_GAMH(N) (PSA_PREP_GAME(N) + (N))
+#define PSA_PREP_RSP(N) (PSA_PREP_GAME(N) + 2 * (N))
+#define PSA_PREP_STRIDE(N) (PSA_PREP_GAME(N) + 3 * (N))
+
+#define PSA_OBS_N 8 /* I0, sum ne2, sum nh2, total(rho), ... */
+#define PSA_SRC_STRIDE(N, Nr) (5 * (Nr) + 8 * (2 * (N) - 1) + 8)
+
+/* ---- library ---- */
+int psa_abi_version(void);
+const char *psa_last_error(void);
+/* sizeof(psa_group), so bindings can check their struct mirror */
+int psa_group_size(void);
+
+/* Unscreened Coulomb table: out[c][k] = pref * Vint(Qy[k], y, a1_c, a2_c,
+ * Delta0) for c in (eh, ee, hh) with the reference's crude K0.
+ * Replaces coulomb.CalcCoulombArrays's Vint loop (coulomb.py:502-565,
+ * Vint coulomb.py:107-177, _Vint_jit 69-104, _K03_jit 38-66). */
+int psa_vint_table(const double *Qy, int32_t NQ, const double *y, int32_t Ny,
+                   double alphae, double alphah, double Delta0, double pref,
+                   double *out, void *stream);
+
+/* Per-wire preparation: diag of C2/D2, CalcScreenedArrays (coulomb.py:1195-1230),
+ * diagonal sums for the Hartree terms, CalcGammaE/H (dephasing.py:541-625),
+ * SpontEmission (emission.py:94-130).  SBEs.py:5173-5226 minus CalcH. */
+int psa_prep(const psa_group *g, int32_t w0, int32_t nw, void *stream);
+
+/* Hartree-Fock Hamiltonians incl. the dipole coupling Meh:
+ * _CalcH_jit (SBEs.py:808-924) + _calc_Meh (SBEs.py:5153-5163). */
+int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream);
+
+/* Off-diagonal dephasing tables De, Dh: CalcOffDiagDeph_E2/H2
+ * (dephasing.py:226-317, 381-414) as used by OffDiagDephasing2 (714-743). */
+int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream);
+
+/* dpdt/dCdt/dDdt (SBEs.py:1941-1965, 2308-2327, 2606-2625) incl. the
+ * off-diagonal dephasing OffP (_OffDiagDephasing2_jit, dephasing.py:320-356).
+ * mode 0: write dp/dt, dC/dt, dD/dt into pX, CX, DX.
+ * mode 1: leapfrog X3 = X1 + 2 dt dX/dt(X2) into pX, CX, DX (SBEs.py:5673-5675). */
+int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode,
+            double dt, void *stream);
+
+/* Relaxation of the new level's diagonal: MBCE/MBCH (coulomb.py:1298-1326),
+ * MBPE/MBPH (phonons.py:157-268), the exponential in/out update and the
+ * Recomb factor (SBEs.py:5276-5325). */
+int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt,
+                   void *stream);
+
+/* Transport of CX, DX, pX (dcfield.py:1176-1236, DCTrans and LF set). */
+int psa_transport(const psa_group *g, int32_t w0, int32_t nw, double dt,
+                  void *stream);
+
+/* Diagonal renormalisation and averaging X2 = (X1+X3)/2 (SBEs.py:5692-5706);
+ * leaves the new YY2 in *A and the new level YY3 in *X (the caller swaps
+ * B and X afterwards). */
+int psa_finish(const psa_group *g, int32_t w0, int32_t nw, void *stream);
+
+/* QWPolarization3 (qwoptics.py:809-940), QWRho5 (942-1071) with the
+ * 2 e0 ehint/area scaling (SBEs.py:5749-5750), QW2Prop's LF renormalisation
+ * (qwoptics.py:785-790) and CalcI0 (dcfield.py:1110-1152), in real space. */
+int psa_sources(const psa_group *g, int32_t w0, int32_t nw, void *stream);
+
+/* Prop2QW (qwoptics.py:667-715) for Ex, Ey, Ez of every wire. */
+int psa_prop2qw(const psa_group *g, int32_t w0, int32_t nw, void *stream);
+
+/* QW2Prop splines (qwoptics.py:792-805) and Rho = RhoH - RhoE (SBEs.py:5917). */
+int psa_qw2prop(const psa_group *g, int32_t w0, int32_t nw, void *stream);
+
+/* One full SBESolver.QWCalculator step (SBEs.py:5826-5985 minus file I/O)
+ * for wires [w0, w0+nw): prop2qw, prep, calc_h, offdiag, rhs (leapfrog),
+ * relaxation, transport, finish, sources, qw2prop. */
+int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

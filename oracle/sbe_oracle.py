@@ -293,7 +293,10 @@ def vint(Qyk, y, ae, ah, Delta0):
 class Tables:
     """Module-level tables of coulomb/phonons/dephasing/emission/qwoptics."""
 
-    def __init__(self, g):
+    def __init__(self, g, vtab=None):
+        """``vtab`` (3, Nr) may supply precomputed unscreened Coulomb
+        integrals (only the CPU-baseline timing uses it, to skip the slow
+        table build that is not part of a step)."""
         prm = g.prm
         N, Nr = g.Nk, g.Nr
         ky = g.kr
@@ -301,13 +304,15 @@ class Tables:
         # --- coulomb.py:502-577 CalcCoulombArrays
         pref = e0**2 / (twopi * eps0 * prm.epsr * prm.L)
         NQ = len(g.Qr)
-        eh = np.array([pref * vint(g.Qr[k], g.r, ae, ah, prm.Delta0)
-                       for k in range(NQ)])
-        ee = np.array([pref * vint(g.Qr[k], g.r, ae, ae, prm.Delta0)
-                       for k in range(NQ)])
-        hh = np.array([pref * vint(g.Qr[k], g.r, ah, ah, prm.Delta0)
-                       for k in range(NQ)])
-        self.vtab = np.stack([eh, ee, hh])  # (3, Nr), Qr order
+        if vtab is None:
+            eh = np.array([pref * vint(g.Qr[k], g.r, ae, ah, prm.Delta0)
+                           for k in range(NQ)])
+            ee = np.array([pref * vint(g.Qr[k], g.r, ae, ae, prm.Delta0)
+                           for k in range(NQ)])
+            hh = np.array([pref * vint(g.Qr[k], g.r, ah, ah, prm.Delta0)
+                           for k in range(NQ)])
+            vtab = np.stack([eh, ee, hh])
+        self.vtab = np.asarray(vtab)  # (3, Nr), Qr order
         self.V0 = np.zeros((3, N, N))
         for k in range(N):
             for q in range(N):

@@ -1,0 +1,98 @@
+"""ctypes binding of the C ABI in include/pulsesuite_amd.h.
+
+The library is loaded from the package's ``_lib`` directory.  Loading fails
+loudly when it is missing: there is no CPU fallback for the device path.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libpulsesuite_amd.so")
+
+PSA_ABI_VERSION = 1
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+c_i32 = ctypes.c_int32
+c_dbl = ctypes.c_double
+c_ptr = ctypes.c_void_p
+
+_INTS = ["N", "Nr", "Nw", "Nx", "Nhw", "flags", "prec", "gx0", "gnx", "bx0", "bnx"]
+_DBLS = ["hbar", "e0", "gap", "L", "ehint", "dcv", "area", "dkr", "dr_r", "drr",
+         "gam_e", "gam_h", "gam_eh", "Edc", "Vscale", "n1d_max", "dhw"]
+_TABS = ["kr", "Ee", "Eh", "vtab", "chi", "qeh", "EP", "HP", "xe", "xh", "didx",
+         "iq", "homega", "hsq", "window", "rq", "ytr", "gmat", "m2"]
+_WIRE = ["wire_id", "pA", "pB", "pX", "CA", "CB", "CX", "DA", "DB", "DX", "Heh",
+         "Heo", "Hho", "hdiag", "prep", "dedh", "wmb", "exq", "efield", "src",
+         "out", "obs", "wid_host"]
+
+
+class PsaGroup(ctypes.Structure):
+    """Mirror of ``psa_group``; the size is checked against the library."""
+
+    _fields_ = ([(n, c_i32) for n in _INTS]
+                + [("nnz_mb", c_i32 * 8), ("pad0", c_i32)]
+                + [(n, c_dbl) for n in _DBLS]
+                + [(n, c_ptr) for n in _TABS]
+                + [("mb_off", c_ptr * 8), ("mb_idx", c_ptr * 8), ("mb_val", c_ptr * 8)]
+                + [(n, c_ptr) for n in _WIRE])
+
+
+_GP = ctypes.POINTER(PsaGroup)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "psa_abi_version": (ctypes.c_int, []),
+    "psa_last_error": (ctypes.c_char_p, []),
+    "psa_group_size": (ctypes.c_int, []),
+    "psa_vint_table": (ctypes.c_int, [c_ptr, c_i32, c_ptr, c_i32, c_dbl, c_dbl,
+                                      c_dbl, c_dbl, c_ptr, c_ptr]),
+    "psa_prep": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
+    "psa_calc_h": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
+    "psa_offdiag_tables": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
+    "psa_rhs": (ctypes.c_int, [_GP, c_i32, c_i32, c_i32, c_dbl, c_ptr]),
+    "psa_relaxation": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
+    "psa_transport": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
+    "psa_finish": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
+    "psa_sources": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
+    "psa_prop2qw": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
+    "psa_qw2prop": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
+    "psa_step": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
+}
+
+_lib = None
+
+
+class PsaError(RuntimeError):
+    pass
+
+
+def load(path=None):
+    """Load the HIP library (once) and declare every entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise PsaError(
+            f"{path} is missing: build it with `python -m pulsesuite_amd._build` "
+            "(the device path has no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.psa_abi_version() != PSA_ABI_VERSION:
+        raise PsaError("ABI version mismatch")
+    if lib.psa_group_size() != ctypes.sizeof(PsaGroup):
+        raise PsaError(f"psa_group size mismatch: C {lib.psa_group_size()} vs "
+                       f"ctypes {ctypes.sizeof(PsaGroup)}")
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = _lib.psa_last_error().decode() if _lib is not None else ""
+        raise PsaError(f"pulsesuite_amd error {rc}: {msg}")
+    return rc

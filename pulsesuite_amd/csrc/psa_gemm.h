@@ -1,0 +1,545 @@
+// MFMA GEMM engine for the SBE contractions (gfx950, wave64).
+//
+// One workgroup (256 threads = 4 waves in a 2x2 arrangement) computes a
+// 64x64 output tile of a batched "sum of products" C = sum_t op(A_t) op(B_t).
+// Operands are produced tile by tile by the problem's loaders (dense
+// matrices with transpose/conjugate, or generated on the fly: Toeplitz
+// Coulomb matrices, skewed diagonals, Lorentzian kernels, occupation
+// vectors) into LDS as separate real/imaginary planes laid out [k][x].
+// Each wave owns a 32x32 sub-tile = 2x2 MFMA 16x16 blocks; a complex
+// product costs four real 16x16x4 MFMAs.  The epilogue hands every output
+// element with its real/imaginary accumulator to the problem's store().
+//
+// MFMA 16x16x4 lane layout (f64 and f32 alike): lane l supplies
+// A[l%16][l/16] and B[l/16][l%16]; accumulator element v of lane l is
+// D[4*(l/16)+v][l%16].
+#pragma once
+
+#include "psa_dev.h"
+
+namespace psa {
+
+typedef double dv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef dv4 acc;
+  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef fv4 acc;
+  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int GBM = 64, GBN = 64, GBK = 16, GLD = GBM + 2;
+
+template <typename T>
+struct Tile {
+  T v[2][GBK][GLD];  // [re/im][k][x]
+};
+
+// s[k][x] = sgn * op(X)(x0+x, k0+k); op(X)(x, k) = X[x][k] (xmajor) or
+// X[k][x]; optionally conjugated; zero outside [0,xext) x [0,kext).
+// Thread mapping follows the memory layout so global reads coalesce.
+template <typename T>
+__device__ __forceinline__ void load_dense(Tile<T> &s, const double *X, int ldx,
+                                           int xext, int kext, bool xmajor,
+                                           bool conj, double sgn, int x0, int k0) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int idx = tid + 256 * e;
+    int x, k;
+    if (xmajor) {
+      x = idx >> 4;
+      k = idx & 15;
+    } else {
+      k = idx >> 6;
+      x = idx & 63;
+    }
+    const int gx = x0 + x, gk = k0 + k;
+    cd v = {0.0, 0.0};
+    if (gx < xext && gk < kext)
+      v = ld(X, xmajor ? (long)gx * ldx + gk : (long)gk * ldx + gx);
+    s.v[0][k][x] = (T)(sgn * v.re);
+    s.v[1][k][x] = (T)(conj ? -sgn * v.im : sgn * v.im);
+  }
+}
+
+template <typename T, class P>
+__global__ __launch_bounds__(256) void gemm_kernel(P p) {
+  __shared__ Tile<T> sA, sB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int b = blockIdx.z, m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+  using acc_t = typename Mfma<T>::acc;
+  acc_t aR[2][2], aI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      aR[i][j] = acc_t{0, 0, 0, 0};
+      aI[i][j] = acc_t{0, 0, 0, 0};
+    }
+  const int kl = lane >> 4, xl = lane & 15;
+  const int nt = p.nterms(b);
+  for (int t = 0; t < nt; ++t) {
+    const int K = p.kext(t, b);
+    for (int k0 = 0; k0 < K; k0 += GBK) {
+      p.loadA(sA, t, b, m0, k0);
+      p.loadB(sB, t, b, k0, n0);
+      __syncthreads();
+#pragma unroll
+      for (int kq = 0; kq < GBK; kq += 4) {
+        const int kk = kq + kl;
+        T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          ar[s] = sA.v[0][kk][wm * 32 + s * 16 + xl];
+          br[s] = sB.v[0][kk][wn * 32 + s * 16 + xl];
+          if (!P::A_REAL) ai[s] = sA.v[1][kk][wm * 32 + s * 16 + xl];
+          if (!P::B_REAL) bi[s] = sB.v[1][kk][wn * 32 + s * 16 + xl];
+        }
+#pragma unroll
+        for (int si = 0; si < 2; ++si)
+#pragma unroll
+          for (int sj = 0; sj < 2; ++sj) {
+            aR[si][sj] = Mfma<T>::mma(ar[si], br[sj], aR[si][sj]);
+            if (!P::A_REAL && !P::B_REAL)
+              aR[si][sj] = Mfma<T>::mma(-ai[si], bi[sj], aR[si][sj]);
+            if (!P::B_REAL) aI[si][sj] = Mfma<T>::mma(ar[si], bi[sj], aI[si][sj]);
+            if (!P::A_REAL) aI[si][sj] = Mfma<T>::mma(ai[si], br[sj], aI[si][sj]);
+          }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int si = 0; si < 2; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 2; ++sj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + wm * 32 + si * 16 + 4 * kl + v;
+        const int col = n0 + wn * 32 + sj * 16 + xl;
+        p.store(b, row, col, (double)aR[si][sj][v], (double)aI[si][sj][v]);
+      }
+}
+
+// ===========================================================================
+// Problem: the SBE right-hand sides (SBEs.py:1941-1965, 2308-2327,
+// 2606-2625) with the Hamiltonian diagonals taken out of the products and
+// added exactly in the epilogue; for dp the off-diagonal dephasing
+// (dephasing.py:320-356) enters as two banded products.
+// batch b -> equation e = b % 3 (p, C, D), wire w = w0 + b / 3.
+// ===========================================================================
+struct RhsProb {
+  static constexpr bool A_REAL = false, B_REAL = false;
+  psa_group g;
+  int w0, mode;
+  double dt;
+
+  __device__ int nterms(int b) const {
+    return (b % 3 == 0 && (g.flags & PSA_F_OFFDIAGDPH)) ? 6 : 4;
+  }
+  __device__ int kext(int, int) const { return g.N; }
+
+  template <typename T>
+  __device__ void loadA(Tile<T> &s, int t, int b, int m0, int k0) const {
+    const int e = b % 3, w = w0 + b / 3, N = g.N;
+    const long o = mat_off(g, w);
+    const double *P = g.pB + o, *C = g.CB + o, *D = g.DB + o;
+    const double *He = g.Heh + o, *Ho = g.Heo + o, *Hh = g.Hho + o;
+    if (e == 0) {
+      switch (t) {
+        case 0: load_dense(s, Hh, N, N, N, true, false, 1.0, m0, k0); break;    // Hhh_o
+        case 1: load_dense(s, P, N, N, N, true, false, 1.0, m0, k0); break;     // p
+        case 2: load_dense(s, He, N, N, N, false, false, -1.0, m0, k0); break;  // -Heh^T
+        case 3: load_dense(s, D, N, N, N, false, false, -1.0, m0, k0); break;   // -D^T
+        case 4: load_band(s, w, 1, m0, k0, true); break;                        // i hbar Dh band
+        default: load_dense(s, P, N, N, N, true, false, 1.0, m0, k0); break;    // p
+      }
+    } else if (e == 1) {
+      switch (t) {
+        case 0: load_dense(s, C, N, N, N, true, false, 1.0, m0, k0); break;     // C
+        case 1: load_dense(s, Ho, N, N, N, false, false, -1.0, m0, k0); break;  // -Hee_o^T
+        case 2: load_dense(s, P, N, N, N, false, true, 1.0, m0, k0); break;     // p^H
+        default: load_dense(s, He, N, N, N, true, true, -1.0, m0, k0); break;   // -conj(Heh)
+      }
+    } else {
+      switch (t) {
+        case 0: load_dense(s, D, N, N, N, true, false, 1.0, m0, k0); break;     // D
+        case 1: load_dense(s, Hh, N, N, N, false, false, -1.0, m0, k0); break;  // -Hhh_o^T
+        case 2: load_dense(s, P, N, N, N, true, true, 1.0, m0, k0); break;      // conj(p)
+        default: load_dense(s, He, N, N, N, false, true, -1.0, m0, k0); break;  // -Heh^H
+      }
+    }
+  }
+
+  template <typename T>
+  __device__ void loadB(Tile<T> &s, int t, int b, int k0, int n0) const {
+    const int e = b % 3, w = w0 + b / 3, N = g.N;
+    const long o = mat_off(g, w);
+    const double *P = g.pB + o, *C = g.CB + o, *D = g.DB + o;
+    const double *He = g.Heh + o, *Ho = g.Heo + o, *Hh = g.Hho + o;
+    // B operand: s[k][x] = op(B)(k0+k, n0+x); "xmajor" means op(B)[k][x] = X[x][k]
+    if (e == 0) {
+      switch (t) {
+        case 0: load_dense(s, P, N, N, N, false, false, 1.0, n0, k0); break;   // p
+        case 1: load_dense(s, Ho, N, N, N, true, false, 1.0, n0, k0); break;   // Hee_o^T
+        case 2: load_dense(s, C, N, N, N, false, false, 1.0, n0, k0); break;   // C
+        case 3: load_dense(s, He, N, N, N, true, false, 1.0, n0, k0); break;   // Heh^T
+        case 4: load_dense(s, P, N, N, N, false, false, 1.0, n0, k0); break;   // p
+        default: load_band(s, w, 0, n0, k0, false); break;                     // i hbar De band
+      }
+    } else if (e == 1) {
+      switch (t) {
+        case 0: load_dense(s, Ho, N, N, N, true, false, 1.0, n0, k0); break;   // Hee_o^T
+        case 1: load_dense(s, C, N, N, N, false, false, 1.0, n0, k0); break;   // C
+        case 2: load_dense(s, He, N, N, N, true, false, 1.0, n0, k0); break;   // Heh^T
+        default: load_dense(s, P, N, N, N, false, false, 1.0, n0, k0); break;  // p
+      }
+    } else {
+      switch (t) {
+        case 0: load_dense(s, Hh, N, N, N, true, false, 1.0, n0, k0); break;   // Hhh_o^T
+        case 1: load_dense(s, D, N, N, N, false, false, 1.0, n0, k0); break;   // D
+        case 2: load_dense(s, He, N, N, N, false, false, 1.0, n0, k0); break;  // Heh
+        default: load_dense(s, P, N, N, N, true, false, 1.0, n0, k0); break;   // p^T
+      }
+    }
+  }
+
+  // Banded off-diagonal dephasing operand, value i*hbar*table.
+  //  kind 1 (Dh, A side): A[kh][m] = Dh[m-kh][kh],  x = kh, k = m
+  //  kind 0 (De, B side): B[m][ke] = De[m-ke][ke],  x = ke, k = m
+  // tables stored [w][kind][k][q + N - 1]; q = 0 excluded (undel(0) = 0).
+  template <typename T>
+  __device__ void load_band(Tile<T> &s, int w, int kind, int x0, int k0, bool) const {
+    const int N = g.N, Q = 2 * N - 1;
+    const double *tab = g.dedh + ((long)w * 2 + kind) * (long)N * Q;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int x = idx >> 4, k = idx & 15;
+      const int gx = x0 + x, gk = k0 + k;
+      double v = 0.0;
+      if (gx < N && gk < N && gk != gx) v = tab[(long)gx * Q + (gk - gx + N - 1)];
+      s.v[0][k][x] = (T)0;
+      s.v[1][k][x] = (T)(g.hbar * v);
+    }
+  }
+
+  __device__ void store(int b, int row, int col, double re, double im) const {
+    const int N = g.N;
+    if (row >= N || col >= N) return;
+    const int e = b % 3, w = w0 + b / 3;
+    const long o = mat_off(g, w);
+    const long ij = (long)row * N + col;
+    const double hbar = g.hbar;
+    cd acc = {re, im};
+    cd d;
+    const double *hd = g.hdiag + (long)w * 4 * N;  // [2][N] complex
+    if (e == 0) {
+      // row = kh, col = ke
+      const cd p = ld(g.pB + o, ij);
+      const cd he = ld(hd, col), hh = ld(hd, N + row);
+      const cd heh = ld(g.Heh + o, (long)col * N + row);
+      cd z = cadd(acc, cmul(cadd(he, hh), p));
+      z = cadd(z, heh);
+      d = cdiv_ih(z, hbar);
+      const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+      const double gam = pr[PSA_PREP_GAME(N) + col] + pr[PSA_PREP_GAMH(N) + row];
+      d = csub(d, cscale(p, gam));
+    } else {
+      const double *X = (e == 1 ? g.CB : g.DB) + o;
+      const cd x = ld(X, ij);
+      const int sl = (e == 1) ? 0 : N;
+      const cd h2 = ld(hd, sl + col), h1 = ld(hd, sl + row);
+      d = cdiv_ih(cadd(acc, cmul(csub(h2, h1), x)), hbar);
+    }
+    double *out = (e == 0 ? g.pX : (e == 1 ? g.CX : g.DX)) + o;
+    if (mode == 0) {
+      st(out, ij, d);
+    } else {
+      const double *X1 = (e == 0 ? g.pA : (e == 1 ? g.CA : g.DA)) + o;
+      const cd x1 = ld(X1, ij);
+      st(out, ij, cd{x1.re + d.re * dt * 2.0, x1.im + d.im * dt * 2.0});
+    }
+  }
+};
+
+// ===========================================================================
+// Problem: Hartree-Fock exchange as a skewed Toeplitz product
+// (_CalcH_jit, SBEs.py:808-924).  For diagonal offset d = k1 - k2 and
+// a = k2:  O[a][d] = sum_{m != a} V(|m-a|) X[m][m+d]  with X = p (eh),
+// C (ee), D (hh); the epilogue adds Meh, the Hartree (direct) terms and
+// the band energies and scatters to (k1, k2) = (a+d, a).
+// batch b -> which = b % 3 (eh, ee, hh), wire w = w0 + b / 3.
+// ===========================================================================
+struct CalcHProb {
+  static constexpr bool A_REAL = true, B_REAL = false;
+  psa_group g;
+  int w0;
+
+  __device__ int nterms(int) const { return 1; }
+  __device__ int kext(int, int) const { return g.N; }
+
+  template <typename T>
+  __device__ void loadA(Tile<T> &s, int, int b, int m0, int k0) const {
+    const int which = b % 3, w = w0 + b / 3, N = g.N;
+    const double *vc = g.prep + (long)w * PSA_PREP_STRIDE(N) + PSA_PREP_VC(N) + which * N;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int k = idx >> 6, x = idx & 63;
+      const int a = m0 + x, m = k0 + k;
+      double v = 0.0;
+      if (a < N && m < N && m != a) v = vc[m > a ? m - a : a - m];
+      s.v[0][k][x] = (T)v;
+      s.v[1][k][x] = (T)0;
+    }
+  }
+
+  template <typename T>
+  __device__ void loadB(Tile<T> &s, int, int b, int k0, int n0) const {
+    const int which = b % 3, w = w0 + b / 3, N = g.N;
+    const long o = mat_off(g, w);
+    const double *X = (which == 0 ? g.pB : (which == 1 ? g.CB : g.DB)) + o;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int k = idx >> 6, x = idx & 63;
+      const int m = k0 + k, dd = n0 + x;
+      const int c = m + dd - (N - 1);  // column m + d
+      cd v = {0.0, 0.0};
+      if (m < N && dd < 2 * N - 1 && c >= 0 && c < N) v = ld(X, (long)m * N + c);
+      s.v[0][k][x] = (T)v.re;
+      s.v[1][k][x] = (T)v.im;
+    }
+  }
+
+  __device__ void store(int b, int a, int dd, double re, double im) const {
+    const int N = g.N;
+    if (a >= N || dd >= 2 * N - 1) return;
+    const int d = dd - (N - 1);
+    const int k1 = a + d, k2 = a;
+    if (k1 < 0 || k1 >= N) return;
+    const int which = b % 3, w = w0 + b / 3;
+    const long o = mat_off(g, w);
+    const long ij = (long)k1 * N + k2;
+    const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+    const bool lf = (g.flags & PSA_F_LF) != 0;
+    double *hd = g.hdiag + (long)w * 4 * N;
+    if (which == 0) {
+      cd meh = {0.0, 0.0};
+      if (g.flags & PSA_F_OPTICS) {
+        const int wid = g.wire_id[w];
+        const double *eq = g.exq + (long)w * 3 * 2 * (2 * N - 1);
+        const cd ex = ld(eq, dd), ey = ld(eq, (2 * N - 1) + dd), ez = ld(eq, 2 * (2 * N - 1) + dd);
+        const double xcv = (k2 & 1) ? -g.dcv : g.dcv;  // Xcv0[ke,kh] = dcv (-1)^kh
+        const double yv = yw_of(wid), zs = sgn_pow(wid);
+        // -ehint*Xcv*Ex - ehint*Ycv*Ey*yw - ehint*Zcv*Ez*(-1)^w, Ycv = dcv, Zcv = -dcv
+        const double a1 = -g.ehint * xcv, a2 = -g.ehint * g.dcv, a3 = -g.ehint * (-g.dcv);
+        meh.re = a1 * ex.re + (a2 * ey.re) * yv + (a3 * ez.re) * zs;
+        meh.im = a1 * ex.im + (a2 * ey.im) * yv + (a3 * ez.im) * zs;
+      }
+      st(g.Heh + o, ij, cd{meh.re + re, meh.im + im});
+    } else {
+      const bool ee = (which == 1);
+      if (d == 0) {
+        const double e0 = ee ? g.Ee[a] + g.gap : g.Eh[a];
+        st(hd, (ee ? 0 : N) + a, cd{e0 - re, -im});
+        return;
+      }
+      double *Hout = (ee ? g.Heo : g.Hho) + o;
+      if (!lf) {
+        st(Hout, ij, cd{0.0, 0.0});
+        return;
+      }
+      const int ad = d > 0 ? d : -d;
+      const double *vc = pr + PSA_PREP_VC(N);
+      const double vself = vc[(ee ? 1 : 2) * N + ad], veh = vc[ad];
+      const cd dsC = ld(pr + PSA_PREP_DSC(N), d + N - 1);
+      const cd dsD = ld(pr + PSA_PREP_DSD(N), d + N - 1);
+      const cd dsCm = ld(pr + PSA_PREP_DSC(N), -d + N - 1);
+      const cd dsDm = ld(pr + PSA_PREP_DSD(N), -d + N - 1);
+      cd h;
+      if (ee)  // - exchange + Vee(qd) sum_k C[k,k+qd] - Veh(qd) sum_k D[k+qd,k]
+        h = cd{-re + vself * dsC.re - veh * dsDm.re, -im + vself * dsC.im - veh * dsDm.im};
+      else     // - exchange + Vhh(qd) sum_k D[k,k+qd] - Veh(qd) sum_k C[k+qd,k]
+        h = cd{-re + vself * dsD.re - veh * dsCm.re, -im + vself * dsD.im - veh * dsCm.im};
+      st(Hout, ij, h);
+    }
+  }
+};
+
+// ===========================================================================
+// Problem: off-diagonal dephasing tables (CalcOffDiagDeph_E2/H2,
+// dephasing.py:226-317) batched over wires.  For fixed q the sums over p
+// are a product of a wire-independent Lorentzian kernel L_q[k][p] with
+// occupation vectors of every wire:
+//   De[q][k] = pi/hbar ( (1-ne_k) R_A + ne_k R_B ),
+//   R_s = sum_p Lee(q,k,p) Vee2(q) a_s(p) + sum_p Leh(q,k,p) Veh2(q) b_s(p).
+// K runs over [same-carrier p | other-carrier p] (2N); columns are
+// (wire, s) pairs, s in {A, B}.  batch b -> qi = b / 2, kind = b % 2
+// (0 = De for electrons, 1 = Dh for holes).
+// ===========================================================================
+struct OffdiagProb {
+  static constexpr bool A_REAL = true, B_REAL = true;
+  psa_group g;
+  int w0, nw;
+
+  __device__ int nterms(int) const { return 1; }
+  __device__ int kext(int, int) const { return 2 * g.N; }
+
+  template <typename T>
+  __device__ void loadA(Tile<T> &s, int, int b, int m0, int k0) const {
+    const int N = g.N, q = b / 2 - (N - 1), kind = b % 2;
+    const double *Es = kind == 0 ? g.Ee : g.Eh;  // carrier of row k
+    const double *Eo = kind == 0 ? g.Eh : g.Ee;  // the other carrier
+    const double hg_same = g.hbar * (kind == 0 ? g.gam_eh : g.gam_e);
+    const double hg_x = g.hbar * g.gam_h;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int k = idx >> 6, x = idx & 63;
+      const int kr = m0 + x, pp = k0 + k;
+      double v = 0.0;
+      const int kq = kr + q;
+      if (kr < N && kq >= 0 && kq < N && pp < 2 * N) {
+        double Ed, hg;
+        if (pp < N) {
+          const int p = pp, pm = p - q;
+          if (pm >= 1 && pm < N) {
+            Ed = Es[kq] + Es[pm] - Es[p] - Es[kr];
+            hg = hg_same;
+            v = (hg / PI) / (Ed * Ed + hg * hg);
+          }
+        } else {
+          const int p = pp - N, pq = p + q;
+          if (pq >= 0 && pq < N) {
+            Ed = Es[kq] + Eo[pq] - Eo[p] - Es[kr];
+            hg = hg_x;
+            v = (hg / PI) / (Ed * Ed + hg * hg);
+          }
+        }
+      }
+      s.v[0][k][x] = (T)v;
+      s.v[1][k][x] = (T)0;
+    }
+  }
+
+  template <typename T>
+  __device__ void loadB(Tile<T> &s, int, int b, int k0, int n0) const {
+    const int N = g.N, q = b / 2 - (N - 1), kind = b % 2;
+    const int aq = q > 0 ? q : -q;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int k = idx >> 6, x = idx & 63;
+      const int pp = k0 + k, col = n0 + x;
+      double v = 0.0;
+      if (pp < 2 * N && col < 2 * nw && aq > 0) {
+        const int w = w0 + (col >> 1), sA = col & 1;
+        const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+        const double *ne = pr + PSA_PREP_NE, *nh = pr + PSA_PREP_NH(N);
+        const double *vc = pr + PSA_PREP_VC(N);
+        // same-carrier occupations: n[p-q] (1-n[p]) | (1-n[p-q]) n[p]
+        // other-carrier:            o[p+q] (1-o[p]) | (1-o[p+q]) o[p]
+        const double *ns = kind == 0 ? ne : nh;
+        const double *no = kind == 0 ? nh : ne;
+        if (pp < N) {
+          const int p = pp, pm = p - q;
+          if (pm >= 1 && pm < N) {
+            const double a = hypot(ns[2 * pm], ns[2 * pm + 1]);
+            const double c = hypot(ns[2 * p], ns[2 * p + 1]);
+            const double V = vc[(kind == 0 ? 1 : 2) * N + aq];
+            v = V * V * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
+          }
+        } else {
+          const int p = pp - N, pq = p + q;
+          if (pq >= 0 && pq < N) {
+            const double a = hypot(no[2 * pq], no[2 * pq + 1]);
+            const double c = hypot(no[2 * p], no[2 * p + 1]);
+            const double V = vc[aq];
+            v = V * V * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
+          }
+        }
+      }
+      s.v[0][k][x] = (T)v;
+      s.v[1][k][x] = (T)0;
+    }
+  }
+
+  __device__ void store(int b, int kr, int col, double re, double) const {
+    const double other = __shfl_xor(re, 1, 64);
+    const int N = g.N, kind = b % 2, qi = b / 2;
+    if ((col & 1) || kr >= N || col >= 2 * nw) return;
+    const int w = w0 + (col >> 1);
+    const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+    const double *n = pr + (kind == 0 ? PSA_PREP_NE : PSA_PREP_NH(N));
+    const double nk = hypot(n[2 * kr], n[2 * kr + 1]);
+    const double D = ((1.0 - nk) * re + nk * other) * PI / g.hbar;
+    g.dedh[(((long)w * 2 + kind) * N + kr) * (2 * N - 1) + qi] = D;
+  }
+};
+
+// ===========================================================================
+// Problem: Prop2QW for Ex, Ey, Ez (qwoptics.py:667-715): spline rr -> r,
+// window, real part and FFTG are linear, so the FFT bins the wire needs
+// (q = -(N-1)..N-1) are one complex matrix G [2N-1][gnx] applied to the
+// real parts of the fields.  Columns = (wire, component).
+// ===========================================================================
+struct Prop2qwProb {
+  static constexpr bool A_REAL = false, B_REAL = true;
+  psa_group g;
+  int w0, nw;
+
+  __device__ int nterms(int) const { return 1; }
+  __device__ int kext(int, int) const { return g.gnx; }
+
+  template <typename T>
+  __device__ void loadA(Tile<T> &s, int, int, int m0, int k0) const {
+    load_dense(s, g.gmat, g.gnx, 2 * g.N - 1, g.gnx, true, false, 1.0, m0, k0);
+  }
+
+  template <typename T>
+  __device__ void loadB(Tile<T> &s, int, int, int k0, int n0) const {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int x = idx >> 4, k = idx & 15;
+      const int col = n0 + x, xi = k0 + k;
+      double v = 0.0;
+      if (col < 3 * nw && xi < g.gnx) {
+        const int w = w0 + col / 3, f = col % 3;
+        v = g.efield[(((long)f * g.Nw + w) * g.Nx + g.gx0 + xi) * 2];
+      }
+      s.v[0][k][x] = (T)v;
+      s.v[1][k][x] = (T)0;
+    }
+  }
+
+  __device__ void store(int, int j, int col, double re, double im) const {
+    const int N = g.N;
+    if (j >= 2 * N - 1 || col >= 3 * nw) return;
+    const int w = w0 + col / 3, f = col % 3;
+    st(g.exq + (long)w * 3 * 2 * (2 * N - 1), (long)f * (2 * N - 1) + j, cd{re, im});
+  }
+};
+
+}  // namespace psa

@@ -1,0 +1,906 @@
+// pulsesuite_amd: HIP kernels and C ABI for the quantum-wire SBE step.
+// Target: gfx950 (MI355X).  See include/pulsesuite_amd.h for the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "psa_gemm.h"
+
+using namespace psa;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return PSA_OK;
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return PSA_E_HIP;
+}
+
+#define PSA_LAUNCH_CHECK(what)                              \
+  do {                                                      \
+    int _rc = hip_check(hipGetLastError(), what);           \
+    if (_rc) return _rc;                                    \
+  } while (0)
+
+constexpr double C0_LIGHT = 299792458.0;
+
+int check_group(const psa_group *g, int w0, int nw) {
+  if (!g) return fail(PSA_E_INVALID, "null group");
+  if (g->N < 4 || g->N > 1023) return fail(PSA_E_INVALID, "N must be in [4, 1023]");
+  if (g->Nr != 2 * g->N + 2) return fail(PSA_E_INVALID, "Nr must be 2N+2");
+  if (g->Nw < 1) return fail(PSA_E_INVALID, "Nw must be >= 1");
+  if (w0 < 0 || nw < 1 || w0 + nw > g->Nw) return fail(PSA_E_INVALID, "wire range outside the group");
+  if (g->prec < 0 || g->prec > 2) return fail(PSA_E_INVALID, "unknown precision mode");
+  return PSA_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Vint table (coulomb.py:38-177, 502-565): crude _K03_jit
+// ===========================================================================
+__device__ double k03_crude(double x) {
+  if (x > 100.0) return 0.0;
+  if (x < 1e-10) return 23.0;
+  if (x < 2.0) {
+    const double x2 = x * x;
+    double res = -log(0.5 * x) - 0.5772156649015329;
+    double term = 1.0;
+    for (int k = 1; k < 20; ++k) {
+      term = term * 0.25 * x2 / (double)(k * k);
+      res = res + term * (1.0 / k - 0.5772156649015329);
+      if (fabs(term) < 1e-15) break;
+    }
+    return res;
+  }
+  return sqrt(PI / (2.0 * x)) * exp(-x);
+}
+
+// one block per (component, Qy index); threads split the (i, j) double sum
+__global__ __launch_bounds__(256) void k_vint(const double *Qy, int NQ, const double *y, int Ny,
+                                              double ae, double ah, double Delta0, double pref,
+                                              double *out) {
+  __shared__ double sh[8];
+  const int kq = blockIdx.x, c = blockIdx.y;
+  const double a1 = (c == 2) ? ah : ae;  // (eh: ae,ah) (ee: ae,ae) (hh: ah,ah)
+  const double a2 = (c == 1) ? ae : ah;
+  const int N1 = Ny / 4, N2 = 3 * Ny / 4, M = N2 - N1 + 1;
+  const double kmin = (a1 + a2) / 4.0;
+  const double dk = fmax(fabs(Qy[kq]), kmin);
+  const double dy = y[1] - y[0];
+  const double mult = a1 * a2 / PI * dy * dy;
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < M * M; t += blockDim.x) {
+    const int i = N1 + t / M, j = N1 + t % M;
+    const double yi = y[i], yj = y[j];
+    const double r = sqrt((yi - yj) * (yi - yj) + Delta0 * Delta0);
+    const double ex = exp(-(a1 * yi) * (a1 * yi) - (a2 * yj) * (a2 * yj));
+    acc += ex * mult * k03_crude(dk * r);
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[(long)c * NQ + kq] = pref * s;
+}
+
+// ===========================================================================
+// prep: occupations, screening, Hartree diagonal sums, diagonal dephasing,
+// spontaneous emission.  One block per wire.
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_prep(psa_group g, int w0) {
+  __shared__ double sh[8];
+  __shared__ double s_ner[1024], s_nhr[1024];
+  const int w = w0 + blockIdx.x, N = g.N, tid = threadIdx.x;
+  const long o = mat_off(g, w);
+  const double *C2 = g.CB + o, *D2 = g.DB + o;
+  double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+  double *ne = pr + PSA_PREP_NE, *nh = pr + PSA_PREP_NH(N);
+  double part = 0.0;
+  for (int k = tid; k < N; k += blockDim.x) {
+    const cd a = ld(C2, (long)k * N + k), b = ld(D2, (long)k * N + k);
+    st(ne, k, a);
+    st(nh, k, b);
+    s_ner[k] = a.re;
+    s_nhr[k] = b.re;
+    part += a.re + b.re;
+  }
+  const double tot = block_sum(part, sh);
+  // CalcScreenedArrays (coulomb.py:1195-1230), Eps1D (1172-1193)
+  const bool screened = (g.flags & PSA_F_SCREENED) != 0;
+  double n1 = tot / 2.0 / g.L;
+  n1 = fmin(n1, g.n1d_max);
+  double *E1D = pr + PSA_PREP_E1D(N), *vc = pr + PSA_PREP_VC(N);
+  for (int d = tid; d < N; d += blockDim.x) {
+    double e = 1.0;
+    if (screened) {
+      const double qe = g.qeh[d], qh = g.qeh[N + d];
+      e = 1.0 - g.chi[d] * 2.0 * log(fabs((qe - PI * n1) / (qe + n1))) -
+          g.chi[N + d] * 2.0 * log(fabs((qh - PI * n1) / (qh + n1)));
+    }
+    E1D[d] = e;
+    for (int c = 0; c < 3; ++c) vc[c * N + d] = screened ? g.vtab[c * N + d] / e : g.vtab[c * N + d];
+  }
+  // diagonal sums dsC[s] = sum_k C[k][k+s], dsD likewise, s = -(N-1)..N-1
+  double *dsC = pr + PSA_PREP_DSC(N), *dsD = pr + PSA_PREP_DSD(N);
+  for (int si = tid; si < 2 * N - 1; si += blockDim.x) {
+    const int s = si - (N - 1);
+    const int k0 = s < 0 ? -s : 0, k1 = s < 0 ? N : N - s;
+    cd ac = {0, 0}, ad = {0, 0};
+    for (int k = k0; k < k1; ++k) {
+      ac = cadd(ac, ld(C2, (long)k * N + k + s));
+      ad = cadd(ad, ld(D2, (long)k * N + k + s));
+    }
+    st(dsC, si, ac);
+    st(dsD, si, ad);
+  }
+  __syncthreads();
+  // CalcGammaE/H (dephasing.py:28-103) with Vxx2 (362-378)
+  double *GE = pr + PSA_PREP_GAME(N), *GH = pr + PSA_PREP_GAMH(N);
+  const bool diag = (g.flags & PSA_F_DIAGDPH) != 0;
+  const long NN = (long)N * N;
+  const int *kpq = g.didx, *kmq = g.didx + NN, *k1pmq = g.didx + 2 * NN, *k1p = g.didx + 3 * NN,
+            *k1mq = g.didx + 4 * NN, *k1 = g.didx + 5 * NN;  // stored [q][k]
+  const double ph = PI / g.hbar;
+  for (int k = tid; k < N; k += blockDim.x) {
+    if (!diag) {
+      GE[k] = g.gam_e;
+      GH[k] = g.gam_h;
+      continue;
+    }
+    double ge = 0.0, gh = 0.0;
+    for (int q = 0; q < N; ++q) {
+      const double V = vc[N + g.iq[q]];
+      const int i = kpq[(long)q * N + k];
+      if (i >= 0 && i < N) ge = ge + ph * (V * V) * s_ner[i] * (1.0 - s_ner[i]) * fabs(g.xe[q]);
+    }
+    for (int q = 0; q < N; ++q) {
+      const double V = vc[g.iq[q]];
+      const int a = k1pmq[(long)q * N + k], b = k1p[(long)q * N + k], c = kmq[(long)q * N + k];
+      if (a >= 0 && a < N && b >= 0 && b < N && c >= 0 && c < N)
+        ge = ge + ph * (V * V) *
+                      (s_nhr[a] * (1.0 - s_nhr[b]) * s_ner[c] + (1.0 - s_nhr[a]) * s_nhr[b] * (1.0 - s_ner[c])) *
+                      fabs(g.xh[q]);
+    }
+    for (int q = 0; q < N; ++q) {
+      const double V = vc[2 * N + g.iq[q]];
+      const int i = kpq[(long)q * N + k];
+      if (i >= 0 && i < N) gh = gh + ph * (V * V) * s_nhr[i] * (1.0 - s_nhr[i]) * fabs(g.xh[q]);
+    }
+    for (int q = 0; q < N; ++q) {
+      const double V = vc[g.iq[q]];
+      const int a = k1mq[(long)q * N + k], b = k1[(long)q * N + k], c = kmq[(long)q * N + k];
+      if (a >= 0 && a < N && b >= 0 && b < N && c >= 0 && c < N)
+        gh = gh + ph * (V * V) *
+                      (s_ner[a] * (1.0 - s_ner[b]) * s_nhr[c] + (1.0 - s_ner[a]) * s_ner[b] * (1.0 - s_nhr[c])) *
+                      fabs(g.xe[q]);
+    }
+    GE[k] = ge;
+    GH[k] = gh;
+  }
+  // SpontEmission (emission.py:94-198): Ec with the lower-triangular VC
+  double *Rsp = pr + PSA_PREP_RSP(N);
+  const bool recomb = (g.flags & PSA_F_RECOMB) != 0;
+  const double rho_c = 1.0 / (C0_LIGHT * C0_LIGHT * C0_LIGHT * PI * PI * g.hbar * g.hbar * g.hbar);
+  for (int k = tid; k < N; k += blockDim.x) {
+    if (!recomb) {
+      Rsp[k] = 0.0;
+      continue;
+    }
+    const double veh0 = vc[0], vee0 = vc[N], vhh0 = vc[2 * N];
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < N; ++i) {
+      const double vee_ik = i >= k ? vc[N + i - k] : 0.0;
+      const double vhh_ik = i >= k ? vc[2 * N + i - k] : 0.0;
+      s1 += s_ner[i] * (vee0 - vee_ik) + s_nhr[i] * (vhh0 - vhh_ik);
+      if (i != k) s2 += s_ner[i] * veh0 - s_nhr[i] * veh0;
+    }
+    const double Ec = s1 + s2 - veh0;
+    const double Ek = g.gap + g.Ee[k] + g.Eh[k] + Ec;
+    double acc = 0.0;
+    for (int n = 0; n < g.Nhw; ++n) {
+      const double hw = g.homega[n] + Ek;
+      acc += hw * (hw * hw * rho_c) * g.hsq[n];
+    }
+    Rsp[k] = acc * g.dhw;
+  }
+}
+
+// Hamiltonians without excitons (SBEs.py:826-835): Heh = Meh, diagonal energies
+__global__ __launch_bounds__(256) void k_h_simple(psa_group g, int w0) {
+  const int w = w0 + blockIdx.y, N = g.N;
+  const long o = mat_off(g, w);
+  const int wid = g.wire_id[w];
+  const double yv = yw_of(wid), zs = sgn_pow(wid);
+  const double *eq = g.exq + (long)w * 3 * 2 * (2 * N - 1);
+  for (long ij = blockIdx.x * (long)blockDim.x + threadIdx.x; ij < (long)N * N; ij += (long)gridDim.x * blockDim.x) {
+    const int k1 = ij / N, k2 = ij % N;
+    cd meh = {0, 0};
+    if (g.flags & PSA_F_OPTICS) {
+      const int dd = k1 - k2 + N - 1;
+      const cd ex = ld(eq, dd), ey = ld(eq, (2 * N - 1) + dd), ez = ld(eq, 2 * (2 * N - 1) + dd);
+      const double xcv = (k2 & 1) ? -g.dcv : g.dcv;
+      const double a1 = -g.ehint * xcv, a2 = -g.ehint * g.dcv, a3 = g.ehint * g.dcv;
+      meh.re = a1 * ex.re + (a2 * ey.re) * yv + (a3 * ez.re) * zs;
+      meh.im = a1 * ex.im + (a2 * ey.im) * yv + (a3 * ez.im) * zs;
+    }
+    st(g.Heh + o, ij, meh);
+    st(g.Heo + o, ij, cd{0, 0});
+    st(g.Hho + o, ij, cd{0, 0});
+    if (k1 == k2) {
+      double *hd = g.hdiag + (long)w * 4 * N;
+      st(hd, k1, cd{g.Ee[k1] + g.gap, 0.0});
+      st(hd, N + k1, cd{g.Eh[k1], 0.0});
+    }
+  }
+}
+
+// ===========================================================================
+// Many-body Coulomb scattering (MBCE/MBCH, coulomb.py:956-1071) over the
+// sparse lists of non-negligible Gaussian-delta entries.  List l holds
+// entries (i, j, l) of the stored table, grouped (CSR) by the rate index it
+// feeds (see the slot mapping below).  One block per
+// (slot, target k); wires in chunks of 8 accumulated in registers.
+// ===========================================================================
+__device__ __forceinline__ double vsq_lower(const double *vc, int a, int b) {
+  // VC[a][b]^2 with the lower-triangular table (zero for a < b)
+  if (a < b) return 0.0;
+  const double v = vc[a - b];
+  return v * v;
+}
+
+__global__ __launch_bounds__(256) void k_mb(psa_group g, int w0, int nw) {
+  __shared__ double red[4][8];
+  const int slot = blockIdx.y, k = blockIdx.x, N = g.N, tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  for (int wc = 0; wc < nw; wc += 8) {
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+    for (int half = 0; half < 2; ++half) {
+      // slot 0 WinE <- lists 0, 2; 1 WoutE <- 1, 3; 2 WinH <- 4, 6; 3 WoutH <- 5, 7
+      const int L = (slot >> 1) * 4 + (slot & 1) + 2 * half;
+      if (g.nnz_mb[L] == 0) continue;
+      const int beg = g.mb_off[L][k], end = g.mb_off[L][k + 1];
+      for (int e = beg + tid; e < end; e += blockDim.x) {
+        const uint32_t pk = g.mb_idx[L][e];
+        const int i = pk & 1023, j = (pk >> 10) & 1023, l = (pk >> 20) & 1023;
+        const double val = g.mb_val[L][e];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int w = w0 + wc + u;
+          if (wc + u >= nw) break;
+          const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+          const double *vc = pr + PSA_PREP_VC(N);
+          const double *nn = g.wmb + (long)w * 6 * N;
+          const double *ne = nn + 4 * N, *nh = nn + 5 * N;
+          double t = 0.0;
+          switch (L) {
+            case 0: t = vsq_lower(vc, i, l) * (1.0 - nh[j]) * nh[j + l - i] * ne[l]; break;
+            case 1: t = vsq_lower(vc, i, l) * (1.0 - ne[i]) * (1.0 - nh[j]) * nh[j + l - i]; break;
+            case 2: t = vsq_lower(vc + N, i, l) * (1.0 - ne[j]) * ne[i + j - l] * ne[l]; break;
+            case 3: t = vsq_lower(vc + N, i, l) * (1.0 - ne[i]) * (1.0 - ne[j]) * ne[i + j - l]; break;
+            case 4: t = vsq_lower(vc, i, l) * (1.0 - ne[i]) * nh[j + l - i] * ne[l]; break;
+            case 5: t = vsq_lower(vc, l, i) * (1.0 - ne[i]) * (1.0 - nh[j]) * ne[l]; break;
+            case 6: t = vsq_lower(vc + 2 * N, i, l) * (1.0 - nh[j]) * nh[i + j - l] * nh[l]; break;
+            default: t = vsq_lower(vc + 2 * N, i, l) * (1.0 - nh[i]) * (1.0 - nh[j]) * nh[i + j - l]; break;
+          }
+          acc[u] += t * val;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = wave_sum(acc[u]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) red[wid][u] = acc[u];
+    __syncthreads();
+    if (tid < 8 && wc + tid < nw) {
+      const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+      g.wmb[(long)(w0 + wc + tid) * 6 * N + slot * N + k] = s;
+    }
+  }
+}
+
+// |Re| of the new level's diagonals for the many-body sums (coulomb.py:1307-1310)
+__global__ void k_diag_gather(psa_group g, int w0) {
+  const int w = w0 + blockIdx.x, N = g.N;
+  const long o = mat_off(g, w);
+  double *nn = g.wmb + (long)w * 6 * N;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    nn[4 * N + k] = fabs(ld(g.CX + o, (long)k * N + k).re);
+    nn[5 * N + k] = fabs(ld(g.DX + o, (long)k * N + k).re);
+    for (int s = 0; s < 4; ++s) nn[s * N + k] = 0.0;
+  }
+}
+
+// Phonon rates (phonons.py:139-268), in/out update and Recomb
+// (SBEs.py:5276-5325) on the diagonal of the new level.  One block per wire.
+__global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
+  __shared__ double s_ner[1024], s_nhr[1024];
+  const int w = w0 + blockIdx.x, N = g.N, tid = threadIdx.x;
+  const long o = mat_off(g, w);
+  double *C3 = g.CX + o, *D3 = g.DX + o;
+  const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+  const double *E1D = pr + PSA_PREP_E1D(N), *vc = pr + PSA_PREP_VC(N), *Rsp = pr + PSA_PREP_RSP(N);
+  const double *mb = g.wmb + (long)w * 6 * N;
+  const bool ehs = (g.flags & PSA_F_EHS) != 0, phon = (g.flags & PSA_F_PHONON) != 0;
+  const bool recomb = (g.flags & PSA_F_RECOMB) != 0;
+  for (int k = tid; k < N; k += blockDim.x) {
+    s_ner[k] = ld(C3, (long)k * N + k).re;
+    s_nhr[k] = ld(D3, (long)k * N + k).re;
+  }
+  __syncthreads();
+  for (int k = tid; k < N; k += blockDim.x) {
+    double wie = 0, woe = 0, wih = 0, woh = 0;
+    if (ehs) {
+      wie = mb[k];
+      woe = mb[N + k];
+      wih = mb[2 * N + k];
+      woh = mb[3 * N + k];
+    }
+    if (phon) {
+      // Vep[k][i] = VC[k,i,1] / E1D[k,i] * Vscale, VC lower-triangular
+      double a = 0, b = 0, c = 0, d = 0;
+      for (int i = 0; i <= k; ++i) {
+        const double vep = vc[N + k - i] / E1D[k - i] * g.Vscale;
+        a += vep * s_ner[i] * g.EP[(long)k * N + i];
+        b += vep * (1.0 - s_ner[i]) * g.EP[(long)i * N + k];
+      }
+      for (int i = k; i < N; ++i) {
+        const double vhp = vc[2 * N + i - k] / E1D[i - k] * g.Vscale;
+        c += vhp * s_nhr[i] * g.HP[(long)k * N + i];
+        d += vhp * (1.0 - s_nhr[i]) * g.HP[(long)i * N + k];
+      }
+      wie += a;
+      woe += b;
+      wih += c;
+      woh += d;
+    }
+    cd ne = ld(C3, (long)k * N + k), nh = ld(D3, (long)k * N + k);
+    if (ehs || phon) {
+      const double We = wie + woe, Wh = wih + woh;
+      const double ee = exp(-We * dt), eep = exp(We * dt);
+      const double eh = exp(-Wh * dt), ehp = exp(Wh * dt);
+      cd t1 = {(-1.0 + eep + ne.re) * wie + ne.re * woe, ne.im * wie + ne.im * woe};
+      ne = cd{ee * t1.re / (We + 1e-200), ee * t1.im / (We + 1e-200)};
+      cd t2 = {(-1.0 + ehp + nh.re) * wih + nh.re * woh, nh.im * wih + nh.im * woh};
+      nh = cd{eh * t2.re / (Wh + 1e-200), eh * t2.im / (Wh + 1e-200)};
+      if (recomb) {
+        // ne *= exp(-Rsp nh dt); nh *= exp(-Rsp ne dt) with the updated ne
+        const double r = Rsp[k];
+        cd z = {-r * nh.re * dt, -r * nh.im * dt};
+        double sz, cz;
+        sincos(z.im, &sz, &cz);
+        const double m = exp(z.re);
+        ne = cmul(ne, cd{m * cz, m * sz});
+        z = cd{-r * ne.re * dt, -r * ne.im * dt};
+        sincos(z.im, &sz, &cz);
+        const double m2 = exp(z.re);
+        nh = cmul(nh, cd{m2 * cz, m2 * sz});
+      }
+    }
+    st(C3, (long)k * N + k, ne);
+    st(D3, (long)k * N + k, nh);
+  }
+}
+
+// ===========================================================================
+// Transport (dcfield.py:1176-1236): X <- ifft2(fft2(X) exp(-i(Y_a+Y_b) dk)).
+// The phase is separable, so it is a row pass and a column pass of
+// FFT -> phase -> inverse FFT, each in LDS (Stockham radix 2/3, N = 2^a 3^b).
+// One wave per line, four lines per block.
+// ===========================================================================
+__device__ void fft_line(cd *x, cd *y, int N, int sign, int lane) {
+  // returns with the result in x
+  int Ns = 1;
+  cd *src = x, *dst = y;
+  int rem = N;
+  while (rem > 1) {
+    const int R = (rem & 1) ? 3 : 2;
+    const int M = N / R;
+    for (int j = lane; j < M; j += 64) {
+      const int k = j % Ns;
+      const double ang = (double)sign * 2.0 * (double)k / (double)(Ns * R);  // in units of pi
+      cd v[3];
+      for (int t = 0; t < R; ++t) {
+        cd a = src[j + t * M];
+        if (t > 0) {
+          double s, c;
+          sincospi(ang * t, &s, &c);
+          a = cmul(a, cd{c, s});
+        }
+        v[t] = a;
+      }
+      const int base = (j / Ns) * Ns * R + k;
+      if (R == 2) {
+        dst[base] = cadd(v[0], v[1]);
+        dst[base + Ns] = csub(v[0], v[1]);
+      } else {
+        // w = exp(sign * 2 pi i / 3)
+        const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
+        const cd w1 = {c3, s3}, w2 = {c3, -s3};
+        dst[base] = cadd(v[0], cadd(v[1], v[2]));
+        dst[base + Ns] = cadd(v[0], cadd(cmul(v[1], w1), cmul(v[2], w2)));
+        dst[base + 2 * Ns] = cadd(v[0], cadd(cmul(v[1], w2), cmul(v[2], w1)));
+      }
+    }
+    __syncthreads();
+    cd *tmp = src;
+    src = dst;
+    dst = tmp;
+    Ns *= R;
+    rem /= R;
+  }
+  if (src != x) {
+    for (int j = lane; j < N; j += 64) x[j] = src[j];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_transport(psa_group g, int w0, double dt, int pass) {
+  extern __shared__ double smem[];
+  const int N = g.N, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lines_per_block = 4;
+  const int mat = blockIdx.y;  // 0 C, 1 D, 2 p
+  const int w = w0 + blockIdx.z;
+  const int line = blockIdx.x * lines_per_block + wv;
+  cd *x = reinterpret_cast<cd *>(smem) + (long)wv * 2 * N;
+  cd *y = x + N;
+  const long o = mat_off(g, w);
+  double *X = (mat == 0 ? g.CX : (mat == 1 ? g.DX : g.pX)) + o;
+  const double *eq = g.exq + (long)w * 3 * 2 * (2 * N - 1);
+  const double eac = eq[2 * (N - 1)];  // Re Ex at q = 0
+  const double dk = -g.e0 * (g.Edc + eac) / g.hbar * dt;
+  const bool valid = line < N;
+  if (valid)
+    for (int j = lane; j < N; j += 64)
+      x[j] = pass == 0 ? ld(X, (long)line * N + j) : ld(X, (long)j * N + line);
+  __syncthreads();
+  fft_line(x, y, N, -1, lane);
+  if (valid)
+    for (int j = lane; j < N; j += 64) {
+      double s, c;
+      sincos(-g.ytr[j] * dk, &s, &c);
+      x[j] = cmul(x[j], cd{c, s});
+    }
+  __syncthreads();
+  fft_line(x, y, N, +1, lane);
+  if (valid)
+    for (int j = lane; j < N; j += 64) {
+      const cd v = cscale(x[j], 1.0 / N);
+      if (pass == 0)
+        st(X, (long)line * N + j, v);
+      else
+        st(X, (long)j * N + line, v);
+    }
+}
+
+// ===========================================================================
+// finish: diagonal |.| and e/h renormalisation (SBEs.py:5692-5702)
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_finish_diag(psa_group g, int w0) {
+  __shared__ double sh[8];
+  const int w = w0 + blockIdx.x, N = g.N, tid = threadIdx.x;
+  const long o = mat_off(g, w);
+  double *C3 = g.CX + o, *D3 = g.DX + o;
+  double se = 0, shh = 0;
+  for (int k = tid; k < N; k += blockDim.x) {
+    se += cabs_(ld(C3, (long)k * N + k));
+    shh += cabs_(ld(D3, (long)k * N + k));
+  }
+  const double Se = block_sum(se, sh), Sh = block_sum(shh, sh);
+  const double total = (Se + Sh) / 2.0;
+  for (int k = tid; k < N; k += blockDim.x) {
+    const double a = cabs_(ld(C3, (long)k * N + k)), b = cabs_(ld(D3, (long)k * N + k));
+    st(C3, (long)k * N + k, cd{a * total / (Se + 1e-200), 0.0});
+    st(D3, (long)k * N + k, cd{b * total / (Sh + 1e-200), 0.0});
+  }
+}
+
+// X2 = (X1 + X3) / 2 for p, C, D of every wire (SBEs.py:5704-5706)
+__global__ __launch_bounds__(256) void k_average(psa_group g, int w0, int nw) {
+  const long per = (long)g.N * g.N;
+  const long total = per * nw * 3;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int m = i / (per * nw);
+    const long r = i % (per * nw);
+    const long off = (long)w0 * per + r;
+    double *A = m == 0 ? g.pA : (m == 1 ? g.CA : g.DA);
+    const double *X = m == 0 ? g.pX : (m == 1 ? g.CX : g.DX);
+    const double2 a = reinterpret_cast<double2 *>(A)[off];
+    const double2 x = reinterpret_cast<const double2 *>(X)[off];
+    reinterpret_cast<double2 *>(A)[off] = make_double2((a.x + x.x) / 2.0, (a.y + x.y) / 2.0);
+  }
+}
+
+// ===========================================================================
+// sources: QWPolarization3 + QWRho5 in real space via diagonal sums,
+// the 2 e0 ehint / area scaling, QW2Prop's LF renormalisation, CalcI0.
+// One block per wire.  src layout per wire: [5][Nr] then 4 (2N-1) complex
+// diagonal sums, then sums.
+// ===========================================================================
+#define SRC_STRIDE(N, Nr) PSA_SRC_STRIDE(N, Nr)
+
+__global__ __launch_bounds__(256) void k_sources(psa_group g, int w0) {
+  __shared__ double sh[8];
+  const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, tid = threadIdx.x;
+  const int Q = 2 * N - 1;
+  const long o = mat_off(g, w);
+  const double *p3 = g.pX + o, *C3 = g.CX + o, *D3 = g.DX + o;
+  double *src = g.src + (long)w * SRC_STRIDE(N, Nr);
+  double *SX = src + 5 * Nr, *SY = SX + 2 * Q, *TC = SY + 2 * Q, *TD = TC + 2 * Q;
+  const int wid = g.wire_id[w];
+  for (int si = tid; si < Q; si += blockDim.x) {
+    const int s = si - (N - 1);
+    const int k0 = s < 0 ? -s : 0, k1 = s < 0 ? N : N - s;
+    cd ax = {0, 0}, ay = {0, 0}, ac = {0, 0}, ad = {0, 0};
+    for (int k = k0; k < k1; ++k) {
+      const cd pv = ld(p3, (long)k * N + k + s);
+      ay = cadd(ay, pv);
+      ax = (k & 1) ? csub(ax, pv) : cadd(ax, pv);
+      ac = cadd(ac, ld(C3, (long)k * N + k + s));
+      ad = cadd(ad, cconj(ld(D3, (long)k * N + k + s)));
+    }
+    st(SX, si, cscale(ax, g.dcv));
+    st(SY, si, cscale(ay, g.dcv));
+    st(TC, si, ac);
+    st(TD, si, ad);
+  }
+  __syncthreads();
+  const double *rq = g.rq, *W = g.window;
+  const double sP = 2.0 * g.ehint / g.area / g.L;
+  const double yv = yw_of(wid), zs = sgn_pow(wid);
+  double *Px = src, *Py = src + Nr, *Pz = src + 2 * Nr, *re_ = src + 3 * Nr, *rh_ = src + 4 * Nr;
+  // real-space sums; complex re/rh kept as (re, im) in registers then stored
+  extern __shared__ double dyn[];  // [2][Nr] imaginary parts of re, rh
+  double *re_im = dyn, *rh_im = dyn + Nr;
+  for (int r = tid; r < Nr; r += blockDim.x) {
+    cd px = {0, 0}, py = {0, 0}, rc = {0, 0}, rd = {0, 0};
+    const double rr = rq[r];
+    for (int si = 0; si < Q; ++si) {
+      const int d = si - (N - 1);
+      double s, c;
+      sincos(((double)d * g.dkr) * rr, &s, &c);
+      const cd ph = {c, s};
+      px = cadd(px, cmul(ld(SX, si), ph));
+      py = cadd(py, cmul(ld(SY, si), ph));
+      rc = cadd(rc, cmul(ld(TC, si), ph));
+      rd = cadd(rd, cmul(ld(TD, si), ph));
+    }
+    const bool optics = (g.flags & PSA_F_OPTICS) != 0;
+    Px[r] = optics ? px.re * W[r] * sP : 0.0;
+    Py[r] = optics ? py.re * W[r] * yv * sP : 0.0;
+    Pz[r] = optics ? -py.re * W[r] * zs * sP : 0.0;
+    const double f = (g.flags & PSA_F_LF) ? W[r] / (2.0 * g.L) : 0.0;
+    re_[r] = rc.re * f;
+    re_im[r] = rc.im * f;
+    rh_[r] = rd.re * f;
+    rh_im[r] = rd.im * f;
+  }
+  __syncthreads();
+  // boundary mean (qwoptics.py:1062-1064)
+  double be = 0, bh = 0;
+  if (tid == 0) {
+    for (int r = 0; r < 10; ++r) {
+      be += re_[r];
+      bh += rh_[r];
+    }
+    for (int r = Nr - 10; r < Nr; ++r) {
+      be += re_[r];
+      bh += rh_[r];
+    }
+    sh[4] = be / 20.0;
+    sh[5] = bh / 20.0;
+  }
+  __syncthreads();
+  be = sh[4];
+  bh = sh[5];
+  __syncthreads();
+  // totals of |ne3|, |nh3| (already real after renormalisation)
+  double ne_t = 0, nh_t = 0;
+  for (int k = tid; k < N; k += blockDim.x) {
+    ne_t += cabs_(ld(C3, (long)k * N + k));
+    nh_t += cabs_(ld(D3, (long)k * N + k));
+  }
+  const double NeT = block_sum(ne_t, sh) + 1e-50, NhT = block_sum(nh_t, sh) + 1e-50;
+  const double dr = rq[1] - rq[0];
+  double ae = 0, ah = 0;
+  for (int r = tid; r < Nr; r += blockDim.x) {
+    re_[r] -= be;
+    rh_[r] -= bh;
+    ae += hypot(re_[r], re_im[r]);
+    ah += hypot(rh_[r], rh_im[r]);
+  }
+  const double Ae = block_sum(ae, sh), Ah = block_sum(ah, sh);
+  const double fe = NeT / (Ae * dr + 1e-100), fh = NhT / (Ah * dr + 1e-100);
+  const double sc = 2.0 * g.e0 / g.area * g.ehint;
+  // after the 2 e0 ehint / area scaling: |.| and the common-total renormalisation
+  double s1 = 0, s2 = 0;
+  for (int r = tid; r < Nr; r += blockDim.x) {
+    const double a = hypot(re_[r] * fe * sc, re_im[r] * fe * sc);
+    const double b = hypot(rh_[r] * fh * sc, rh_im[r] * fh * sc);
+    re_[r] = a;
+    rh_[r] = b;
+    s1 += a;
+    s2 += b;
+  }
+  const double S1 = block_sum(s1, sh), S2 = block_sum(s2, sh);
+  const double total = (S1 * dr + S2 * dr + 1e-100) / 2.0;
+  for (int r = tid; r < Nr; r += blockDim.x) {
+    re_[r] = re_[r] * total / (S1 * dr + 1e-100);
+    rh_[r] = rh_[r] * total / (S2 * dr + 1e-100);
+  }
+  // CalcI0 (dcfield.py:1110-1152) on ne2, nh2 = |diag of the averaged level|
+  __syncthreads();
+  const double *CA = g.CA + o, *DA = g.DA + o;
+  const double *vc = g.prep + (long)w * PSA_PREP_STRIDE(N) + PSA_PREP_VC(N);
+  double *ecE = dyn, *ecH = dyn + N;  // reuse (Nr >= 2N)
+  __shared__ double s_ne2[1024], s_nh2[1024];
+  for (int k = tid; k < N; k += blockDim.x) {
+    s_ne2[k] = cabs_(ld(CA, (long)k * N + k));
+    s_nh2[k] = cabs_(ld(DA, (long)k * N + k));
+  }
+  __syncthreads();
+  for (int k = tid; k < N; k += blockDim.x) {
+    double se = 0, shh = 0;
+    for (int i = 0; i <= k; ++i) {  // V[k][i] nonzero for i <= k
+      se += s_ne2[i] * (vc[N] - vc[N + k - i]);
+      shh += s_nh2[i] * (vc[2 * N] - vc[2 * N + k - i]);
+    }
+    for (int i = k + 1; i < N; ++i) {
+      se += s_ne2[i] * vc[N];
+      shh += s_nh2[i] * vc[2 * N];
+    }
+    ecE[k] = g.Ee[k] + se / 2.0;
+    ecH[k] = g.Eh[k] + shh / 2.0;
+  }
+  __syncthreads();
+  double pe = 0, phh = 0, sne = 0, snh = 0;
+  for (int i = tid; i < N; i += blockDim.x) {
+    double de, dh;
+    if (i >= 1 && i <= N - 2) {
+      de = (ecE[i + 1] - ecE[i - 1]) / 2.0 / g.dkr;
+      dh = (ecH[i + 1] - ecH[i - 1]) / 2.0 / g.dkr;
+    } else if (i == 0) {
+      de = 2.0 * ((ecE[2] - ecE[0]) / 2.0 / g.dkr) - (ecE[3] - ecE[1]) / 2.0 / g.dkr;
+      dh = 2.0 * ((ecH[2] - ecH[0]) / 2.0 / g.dkr) - (ecH[3] - ecH[1]) / 2.0 / g.dkr;
+    } else {
+      de = 2.0 * ((ecE[N - 1] - ecE[N - 3]) / 2.0 / g.dkr) - (ecE[N - 2] - ecE[N - 4]) / 2.0 / g.dkr;
+      dh = 2.0 * ((ecH[N - 1] - ecH[N - 3]) / 2.0 / g.dkr) - (ecH[N - 2] - ecH[N - 4]) / 2.0 / g.dkr;
+    }
+    pe += de * s_ne2[i];
+    phh += dh * s_nh2[i];
+    sne += s_ne2[i];
+    snh += s_nh2[i];
+  }
+  const double Pe = block_sum(pe, sh), Ph = block_sum(phh, sh);
+  const double Sne = block_sum(sne, sh), Snh = block_sum(snh, sh);
+  if (tid == 0) {
+    const double ve = Pe / (1e-100 + Sne) / g.hbar, vh = Ph / (1e-100 + Snh) / g.hbar;
+    double *ob = g.obs + (long)w * PSA_OBS_N;
+    ob[0] = -g.e0 * (ve + vh) * Sne * g.dkr * 2.0;
+    ob[1] = Sne;
+    ob[2] = Snh;
+    ob[3] = total;
+  }
+}
+
+// ===========================================================================
+// QW2Prop splines r -> rr (qwoptics.py:792-805) and Rho = RhoH - RhoE.
+// One block per wire.
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_qw2prop(psa_group g, int w0) {
+  __shared__ double sh[8];
+  const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, Nx = g.Nx, tid = threadIdx.x;
+  const double *src = g.src + (long)w * SRC_STRIDE(N, Nr);
+  const long NW = g.Nw;
+  double *Pxx = g.out + ((0 * NW + w) * (long)Nx) * 2;
+  double *Pyy = g.out + ((1 * NW + w) * (long)Nx) * 2;
+  double *Pzz = g.out + ((2 * NW + w) * (long)Nx) * 2;
+  double *Rho = g.out + ((3 * NW + w) * (long)Nx) * 2;
+  const bool lf = (g.flags & PSA_F_LF) != 0;
+  double se = 0, shh = 0;
+  for (int x = tid; x < Nx; x += blockDim.x) {
+    const int b = x - g.bx0;
+    double v[5] = {0, 0, 0, 0, 0};
+    if (b >= 0 && b < g.bnx) {
+      const double *m = g.m2 + (long)b * Nr;
+      for (int r = 0; r < Nr; ++r) {
+        const double c = m[r];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) v[f] += c * src[f * Nr + r];
+      }
+    }
+    st(Pxx, x, cd{v[0], 0});
+    st(Pyy, x, cd{v[1], 0});
+    st(Pzz, x, cd{v[2], 0});
+    const double a = lf ? fabs(v[3]) : v[3], b2 = lf ? fabs(v[4]) : v[4];
+    Rho[2 * x] = a;       // RhoE stash
+    Rho[2 * x + 1] = b2;  // RhoH stash
+    se += a;
+    shh += b2;
+  }
+  const double SE = block_sum(se, sh), SH = block_sum(shh, sh);
+  const double total = g.obs[(long)w * PSA_OBS_N + 3];
+  const double dRR = g.drr;
+  const double fe = lf ? total / (SE * dRR + 1e-100) : 1.0;
+  const double fh = lf ? total / (SH * dRR + 1e-100) : 1.0;
+  for (int x = tid; x < Nx; x += blockDim.x) {
+    const double a = Rho[2 * x] * fe, b = Rho[2 * x + 1] * fh;
+    st(Rho, x, cd{b - a, 0.0});
+  }
+}
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+namespace {
+
+template <class P>
+int launch_gemm(const P &p, int prec, int M, int Ncols, int batches, hipStream_t s, const char *what) {
+  dim3 grid((Ncols + GBN - 1) / GBN, (M + GBM - 1) / GBM, batches);
+  if (prec == PSA_PREC_FP64)
+    hipLaunchKernelGGL((gemm_kernel<double, P>), grid, dim3(256), 0, s, p);
+  else
+    hipLaunchKernelGGL((gemm_kernel<float, P>), grid, dim3(256), 0, s, p);
+  PSA_LAUNCH_CHECK(what);
+  return PSA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psa_abi_version(void) { return PSA_ABI_VERSION; }
+
+const char *psa_last_error(void) { return g_err.c_str(); }
+
+int psa_group_size(void) { return (int)sizeof(psa_group); }
+
+int psa_vint_table(const double *Qy, int32_t NQ, const double *y, int32_t Ny, double alphae,
+                   double alphah, double Delta0, double pref, double *out, void *stream) {
+  if (!Qy || !y || !out || NQ < 1 || Ny < 4) return fail(PSA_E_INVALID, "psa_vint_table: bad arguments");
+  hipLaunchKernelGGL(k_vint, dim3(NQ, 3), dim3(256), 0, (hipStream_t)stream, Qy, NQ, y, Ny, alphae,
+                     alphah, Delta0, pref, out);
+  PSA_LAUNCH_CHECK("k_vint");
+  return PSA_OK;
+}
+
+int psa_prep(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  hipLaunchKernelGGL(k_prep, dim3(nw), dim3(256), 0, (hipStream_t)stream, *g, w0);
+  PSA_LAUNCH_CHECK("k_prep");
+  return PSA_OK;
+}
+
+int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  if (!(g->flags & PSA_F_EXCITONS)) {
+    hipLaunchKernelGGL(k_h_simple, dim3((g->N * g->N + 255) / 256, nw), dim3(256), 0, s, *g, w0);
+    PSA_LAUNCH_CHECK("k_h_simple");
+    return PSA_OK;
+  }
+  CalcHProb p;
+  p.g = *g;
+  p.w0 = w0;
+  return launch_gemm(p, g->prec, g->N, 2 * g->N - 1, 3 * nw, s, "calc_h gemm");
+}
+
+int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  OffdiagProb p;
+  p.g = *g;
+  p.w0 = w0;
+  p.nw = nw;
+  return launch_gemm(p, g->prec, g->N, 2 * nw, 2 * (2 * g->N - 1), (hipStream_t)stream,
+                     "offdiag gemm");
+}
+
+int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  if (mode != 0 && mode != 1) return fail(PSA_E_INVALID, "psa_rhs: mode must be 0 or 1");
+  RhsProb p;
+  p.g = *g;
+  p.w0 = w0;
+  p.mode = mode;
+  p.dt = dt;
+  return launch_gemm(p, g->prec, g->N, g->N, 3 * nw, (hipStream_t)stream, "rhs gemm");
+}
+
+int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  if (!(g->flags & (PSA_F_EHS | PSA_F_PHONON))) return PSA_OK;
+  hipLaunchKernelGGL(k_diag_gather, dim3(nw), dim3(256), 0, s, *g, w0);
+  PSA_LAUNCH_CHECK("k_diag_gather");
+  if (g->flags & PSA_F_EHS) {
+    hipLaunchKernelGGL(k_mb, dim3(g->N, 4), dim3(256), 0, s, *g, w0, nw);
+    PSA_LAUNCH_CHECK("k_mb");
+  }
+  hipLaunchKernelGGL(k_relax, dim3(nw), dim3(256), 0, s, *g, w0, dt);
+  PSA_LAUNCH_CHECK("k_relax");
+  return PSA_OK;
+}
+
+int psa_transport(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  if (!(g->flags & PSA_F_DCTRANS)) return PSA_OK;
+  if (!(g->flags & PSA_F_LF)) return fail(PSA_E_UNSUPPORTED, "Transport with LF = 0 (diagonal-only shift) is not supported");
+  if (g->N > 512) return fail(PSA_E_UNSUPPORTED, "Transport needs N <= 512 (LDS lines)");
+  int n = g->N;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  if (n != 1) return fail(PSA_E_UNSUPPORTED, "Transport needs N = 2^a 3^b");
+  const size_t shm = 4 * 2 * (size_t)g->N * sizeof(cd);
+  const hipStream_t s = (hipStream_t)stream;
+  for (int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL(k_transport, dim3((g->N + 3) / 4, 3, nw), dim3(256), shm, s, *g, w0, dt, pass);
+    PSA_LAUNCH_CHECK("k_transport");
+  }
+  return PSA_OK;
+}
+
+int psa_finish(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_finish_diag, dim3(nw), dim3(256), 0, s, *g, w0);
+  PSA_LAUNCH_CHECK("k_finish_diag");
+  const long total = (long)g->N * g->N * nw * 3;
+  const int blocks = (int)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
+  hipLaunchKernelGGL(k_average, dim3(blocks), dim3(256), 0, s, *g, w0, nw);
+  PSA_LAUNCH_CHECK("k_average");
+  return PSA_OK;
+}
+
+int psa_sources(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  const size_t shm = 2 * (size_t)g->Nr * sizeof(double);
+  hipLaunchKernelGGL(k_sources, dim3(nw), dim3(256), shm, (hipStream_t)stream, *g, w0);
+  PSA_LAUNCH_CHECK("k_sources");
+  return PSA_OK;
+}
+
+int psa_prop2qw(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  Prop2qwProb p;
+  p.g = *g;
+  p.w0 = w0;
+  p.nw = nw;
+  // the field transform always runs in FP64 (it sets the drive amplitude)
+  return launch_gemm(p, PSA_PREC_FP64, 2 * g->N - 1, 3 * nw, 1, (hipStream_t)stream, "prop2qw gemm");
+}
+
+int psa_qw2prop(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  hipLaunchKernelGGL(k_qw2prop, dim3(nw), dim3(256), 0, (hipStream_t)stream, *g, w0);
+  PSA_LAUNCH_CHECK("k_qw2prop");
+  return PSA_OK;
+}
+
+int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  int rc;
+  if ((rc = psa_prop2qw(g, w0, nw, stream))) return rc;
+  if ((rc = psa_prep(g, w0, nw, stream))) return rc;
+  if ((rc = psa_calc_h(g, w0, nw, stream))) return rc;
+  if (g->flags & PSA_F_OFFDIAGDPH)
+    if ((rc = psa_offdiag_tables(g, w0, nw, stream))) return rc;
+  if ((rc = psa_rhs(g, w0, nw, 1, dt, stream))) return rc;
+  if ((rc = psa_relaxation(g, w0, nw, dt, stream))) return rc;
+  if ((rc = psa_transport(g, w0, nw, dt, stream))) return rc;
+  if ((rc = psa_finish(g, w0, nw, stream))) return rc;
+  if ((rc = psa_sources(g, w0, nw, stream))) return rc;
+  if ((rc = psa_qw2prop(g, w0, nw, stream))) return rc;
+  return PSA_OK;
+}
+
+}  // extern "C"

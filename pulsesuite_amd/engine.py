@@ -1,0 +1,273 @@
+"""Device engine: a group of quantum wires sharing one k-grid, stepped on
+one GPU through the C ABI.
+
+A ``WireGroup`` owns every device buffer as a torch tensor (PyTorch is the
+allocator and the stream provider only) and fills the ``psa_group`` struct
+the kernels read.  State levels follow the reference names: level 2
+(``YY2/CC2/DD2``, the averaged level) and level 3 (``YY3/CC3/DD3``, the
+newest level) of ``SBESolver`` (``SBEs.py:4770-4778``); the reference
+layout [N, N, Nw] is converted to the device's wire-major [Nw, N, N].
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+from scipy.constants import e as e0
+from scipy.constants import hbar
+
+from . import capi, tables
+
+F64 = torch.float64
+C128 = torch.complex128
+
+
+def _prep_stride(N):
+    return 8 * N + 4 * (2 * N - 1) + 3 * N
+
+
+def _src_stride(N, Nr):
+    return 5 * Nr + 8 * (2 * N - 1) + 8
+
+
+class WireGroup:
+    """Nw wires of one k-grid size on one device.
+
+    prm    -- tables.QWParams (material, numerics, flags)
+    nw     -- number of wires
+    rr     -- propagation grid the fields live on (QWCalculator's rr)
+    wire_ids -- 1-based reference wire index per wire (sets yw(w), (-1)^w)
+    prec   -- "fp64" (reference-equivalent), "fp32" or "bf16x3"
+    """
+
+    def __init__(self, prm, nw, rr, wire_ids=None, prec="fp64", device="cuda"):
+        self.lib = capi.load()
+        if not torch.cuda.is_available():
+            raise capi.PsaError("no GPU visible: the device path has no CPU fallback")
+        self.dev = torch.device(device)
+        self.prm = prm
+        self.nw = int(nw)
+        self.rr = np.asarray(rr, dtype=float)
+        self.prec = prec
+        if prec not in capi.PREC:
+            raise ValueError(f"unknown precision {prec!r}")
+        if prm.flags.get("FreePot"):
+            # _preparation never fills Wee/Whh (SBEs.py:5173-5226): FreePot
+            # adds zero in the reference; nothing to do here either.
+            pass
+        self.grid, tb = tables.build_tables(prm, self.rr, self._vint)
+        self.tab = tb
+        g = self.grid
+        N, Nr, Nx = g["N"], g["Nr"], len(self.rr)
+        self.N, self.Nr, self.Nx = N, Nr, Nx
+        d = self.dev
+        T = lambda a, dt=F64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(d)  # noqa: E731
+        self.t = {
+            "kr": T(g["kr"]), "Ee": T(g["Ee"]), "Eh": T(g["Eh"]),
+            "vtab": T(tb["vtab"]), "chi": T(tb["chi"]), "qeh": T(tb["qeh"]),
+            "EP": T(tb["EP"]), "HP": T(tb["HP"]), "xe": T(tb["xe"]), "xh": T(tb["xh"]),
+            "didx": T(tb["didx"], torch.int32), "iq": T(tb["iq"], torch.int32),
+            "homega": T(tb["homega"]), "hsq": T(tb["hsq"]), "window": T(tb["window"]),
+            "rq": T(g["r"]), "ytr": T(tb["ytr"]),
+            "gmat": T(tb["gmat"], C128), "m2": T(tb["m2"] if tb["bnx"] else np.zeros((1, Nr))),
+        }
+        self.mb = []
+        for off, idx, val in tb["mb"]:
+            self.mb.append((T(off, torch.int32),
+                            T(idx.astype(np.int64).astype(np.int32) if idx.size else np.zeros(1, np.int32), torch.int32),
+                            T(val if val.size else np.zeros(1))))
+        nwz = self.nw
+        z = lambda *s, dt=C128: torch.zeros(s, dtype=dt, device=d)  # noqa: E731
+        self.s = {}
+        for k in ("pA", "pB", "pX", "CA", "CB", "CX", "DA", "DB", "DX", "Heh", "Heo", "Hho"):
+            self.s[k] = z(nwz, N, N)
+        self.s["hdiag"] = z(nwz, 2, N)
+        self.s["prep"] = z(nwz, _prep_stride(N), dt=F64)
+        self.s["dedh"] = z(nwz, 2, N, 2 * N - 1, dt=F64)
+        self.s["wmb"] = z(nwz, 6, N, dt=F64)
+        self.s["exq"] = z(nwz, 3, 2 * N - 1)
+        self.s["efield"] = z(3, nwz, Nx)
+        self.s["src"] = z(nwz, _src_stride(N, Nr), dt=F64)
+        self.s["out"] = z(4, nwz, Nx)
+        self.s["obs"] = z(nwz, 8, dt=F64)
+        if wire_ids is None:
+            wire_ids = np.arange(1, nwz + 1)
+        self.wire_ids = np.asarray(wire_ids, dtype=np.int32)
+        if self.wire_ids.shape != (nwz,):
+            raise ValueError("wire_ids must have one entry per wire")
+        self.s["wire_id"] = T(self.wire_ids, torch.int32)
+        self._wid_host = self.wire_ids.copy()
+        self.init_state()
+        self._sync_struct()
+
+    # ------------------------------------------------------------------
+    def _vint(self, Qr, r, ae, ah, Delta0, pref):
+        d = self.dev
+        Q = torch.as_tensor(Qr, dtype=F64, device=d)
+        y = torch.as_tensor(r, dtype=F64, device=d)
+        out = torch.zeros(3, len(Qr), dtype=F64, device=d)
+        capi.check(self.lib.psa_vint_table(Q.data_ptr(), len(Qr), y.data_ptr(), len(r),
+                                           ae, ah, Delta0, pref, out.data_ptr(),
+                                           self._stream()))
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def _sync_struct(self):
+        g, tb, prm = self.grid, self.tab, self.prm
+        S = capi.PsaGroup()
+        S.N, S.Nr, S.Nw, S.Nx = self.N, self.Nr, self.nw, self.Nx
+        S.Nhw = len(tb["homega"])
+        S.flags = prm.flag_bits()
+        S.prec = capi.PREC[self.prec]
+        S.gx0, S.gnx, S.bx0, S.bnx = tb["gx0"], tb["gnx"], tb["bx0"], tb["bnx"]
+        for i, (off, idx, val) in enumerate(tb["mb"]):
+            S.nnz_mb[i] = int(val.size)
+        S.hbar, S.e0, S.gap, S.L = hbar, e0, prm.gap, prm.L
+        S.ehint, S.dcv, S.area, S.dkr = g["ehint"], g["dcv"], g["area"], g["dkr"]
+        S.dr_r = g["r"][1] - g["r"][0]
+        S.drr = self.rr[1] - self.rr[0] if len(self.rr) > 1 else 1.0
+        S.gam_e, S.gam_h, S.gam_eh, S.Edc = prm.gam_e, prm.gam_h, g["gam_eh"], prm.Edc
+        S.Vscale, S.n1d_max, S.dhw = tb["Vscale"], tb["n1d_max"], tb["dhw"]
+        for k, v in self.t.items():
+            setattr(S, k, v.data_ptr())
+        for i, (off, idx, val) in enumerate(self.mb):
+            S.mb_off[i] = off.data_ptr()
+            S.mb_idx[i] = idx.data_ptr()
+            S.mb_val[i] = val.data_ptr()
+        for k, v in self.s.items():
+            setattr(S, k, v.data_ptr())
+        S.wid_host = self._wid_host.ctypes.data
+        self.S = S
+
+    # ------------------------------------------------------------------
+    # state
+    # ------------------------------------------------------------------
+    def init_state(self):
+        """Initial populations FermiDistr(Ee + gap/2) for both carriers
+        (SBEs.py:4829-4840), coherences zero."""
+        N = self.N
+        f = tables.fermi(self.grid["Ee"] + self.prm.gap / 2.0)
+        diag = torch.as_tensor(np.diag(f).astype(np.complex128), device=self.dev)
+        for k in ("pA", "pB", "pX"):
+            self.s[k].zero_()
+        for k in ("CA", "CB", "DA", "DB"):
+            self.s[k].copy_(diag.expand(self.nw, N, N))
+        self.s["CX"].zero_()
+        self.s["DX"].zero_()
+
+    def set_state(self, C2, D2, p2, C3, D3, p3):
+        """Load levels 2 and 3 from arrays in the reference layout [N, N, Nw]."""
+        for k, a in (("CA", C2), ("DA", D2), ("pA", p2), ("CB", C3), ("DB", D3), ("pB", p3)):
+            a = np.asarray(a, dtype=np.complex128)
+            if a.ndim == 2:
+                a = a[:, :, None]
+            self.s[k].copy_(torch.as_tensor(np.ascontiguousarray(np.moveaxis(a, 2, 0)), device=self.dev))
+
+    def get_state(self):
+        """(C2, D2, p2, C3, D3, p3) in the reference layout [N, N, Nw]."""
+        torch.cuda.synchronize()
+        out = []
+        for k in ("CA", "DA", "pA", "CB", "DB", "pB"):
+            out.append(np.moveaxis(self.s[k].cpu().numpy(), 0, 2).copy())
+        return tuple(out)
+
+    # ------------------------------------------------------------------
+    # stages (each one C-ABI call on the current stream)
+    # ------------------------------------------------------------------
+    def _range(self, w0, nw):
+        nw = self.nw - w0 if nw is None else nw
+        return int(w0), int(nw)
+
+    def set_fields(self, Exx, Eyy, Ezz, w0=0, nw=None):
+        """Fields on rr for wires [w0, w0+nw): arrays of shape [Nx] (same
+        for every wire) or [nw, Nx]; numpy or device tensors."""
+        w0, nw = self._range(w0, nw)
+        for f, E in enumerate((Exx, Eyy, Ezz)):
+            t = torch.as_tensor(E, dtype=C128).to(self.dev)
+            if t.ndim == 1:
+                t = t.expand(nw, self.Nx)
+            self.s["efield"][f, w0:w0 + nw].copy_(t)
+
+    def prop2qw(self, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_prop2qw(ctypes.byref(self.S), w0, nw, self._stream()))
+
+    def prep(self, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_prep(ctypes.byref(self.S), w0, nw, self._stream()))
+
+    def calc_h(self, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_calc_h(ctypes.byref(self.S), w0, nw, self._stream()))
+
+    def offdiag(self, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_offdiag_tables(ctypes.byref(self.S), w0, nw, self._stream()))
+
+    def rhs(self, mode=0, dt=0.0, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_rhs(ctypes.byref(self.S), w0, nw, mode, dt, self._stream()))
+
+    def relaxation(self, dt, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_relaxation(ctypes.byref(self.S), w0, nw, dt, self._stream()))
+
+    def transport(self, dt, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_transport(ctypes.byref(self.S), w0, nw, dt, self._stream()))
+
+    def finish(self, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_finish(ctypes.byref(self.S), w0, nw, self._stream()))
+
+    def sources(self, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_sources(ctypes.byref(self.S), w0, nw, self._stream()))
+
+    def qw2prop(self, w0=0, nw=None):
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_qw2prop(ctypes.byref(self.S), w0, nw, self._stream()))
+
+    def rhs_eval(self, w0=0, nw=None):
+        """One RHS evaluation on level 3 (the reference's p2/C2/D2 at
+        checkout): fills pX/CX/DX with dp/dt, dC/dt, dD/dt."""
+        if self.prm.flags.get("Optics"):
+            self.prop2qw(w0, nw)
+        self.prep(w0, nw)
+        self.calc_h(w0, nw)
+        if self.prm.flags.get("OffDiagDph"):
+            self.offdiag(w0, nw)
+        self.rhs(0, 0.0, w0, nw)
+
+    def advance_levels(self, w0=0, nw=None):
+        """After a step: level 3 <- the new level (pointer swap for the whole
+        group, device copy for a sub-range)."""
+        w0, nw = self._range(w0, nw)
+        if w0 == 0 and nw == self.nw:
+            for a, b in (("pB", "pX"), ("CB", "CX"), ("DB", "DX")):
+                self.s[a], self.s[b] = self.s[b], self.s[a]
+            self._sync_struct()
+        else:
+            for a, b in (("pB", "pX"), ("CB", "CX"), ("DB", "DX")):
+                self.s[a][w0:w0 + nw].copy_(self.s[b][w0:w0 + nw])
+
+    def step(self, dt, w0=0, nw=None):
+        """One full QWCalculator step (without the host bookkeeping) for
+        wires [w0, w0+nw) with the fields already in ``efield``."""
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_step(ctypes.byref(self.S), w0, nw, dt, self._stream()))
+        self.advance_levels(w0, nw)
+
+    def outputs(self, w0=0, nw=None):
+        """(Pxx, Pyy, Pzz, Rho) as numpy [nw, Nx]."""
+        w0, nw = self._range(w0, nw)
+        o = self.s["out"][:, w0:w0 + nw].cpu().numpy()
+        return o[0], o[1], o[2], o[3]
+
+    def observables(self):
+        """Per-wire I0 (CalcI0), sum ne2, sum nh2 as numpy."""
+        ob = self.s["obs"].cpu().numpy()
+        return dict(I0=ob[:, 0], sum_ne2=ob[:, 1], sum_nh2=ob[:, 2])

@@ -1,0 +1,450 @@
+"""Host-side, init-time construction of the per-N tables of one wire group.
+
+Everything here runs once per k-grid size (``SBESolver.__init__``,
+``SBEs.py:4623-4972``); nothing runs per step.  O(N) grids and O(N^2) index
+tables are built with NumPy; the expensive unscreened Coulomb integral
+(about Nr^3/4 Bessel evaluations) runs on the GPU through
+``psa_vint_table``.  Citations are relative to ``src/pulsesuite/PSTD3D/``
+of the reference.
+"""
+
+import math
+
+import numpy as np
+from scipy.constants import e as e0
+from scipy.constants import epsilon_0 as eps0
+from scipy.constants import hbar
+from scipy.constants import k as kB
+from scipy.constants import m_e as me0
+
+TWOPI = 2.0 * np.pi
+TEMP = 77.0  # phonons.py:22, emission.py:29
+EULER = 0.5772156649015329
+
+FLAG_NAMES = ("Optics", "Excitons", "EHs", "Screened", "Phonon", "DCTrans",
+              "LF", "FreePot", "DiagDph", "OffDiagDph", "Recomb", "PLSpec",
+              "ignorewire", "Xqwparams", "LorentzDelta")
+# bit positions of the flags the device path reads (include/pulsesuite_amd.h)
+FLAG_BITS = {"Optics": 0, "Excitons": 1, "EHs": 2, "Screened": 3, "Phonon": 4,
+             "DCTrans": 5, "LF": 6, "FreePot": 7, "DiagDph": 8,
+             "OffDiagDph": 9, "Recomb": 10}
+
+
+class QWParams:
+    """The values ``_read_qw_params``/``_read_mb_params`` read
+    (SBEs.py:4978-5022) in SI units, plus the flags."""
+
+    def __init__(self, L=100e-9, Delta0=5e-9, gap_eV=1.5, me_rel=0.07,
+                 mh_rel=0.45, HO_eV=100e-3, gam_e=1e12, gam_h=1e12,
+                 gam_eh=1e12, epsr=9.1, Oph_eV=36e-3, Gph_eV=3e-3, Edc=0.0,
+                 jmax=500, ntmax=20000, flags=None):
+        self.L = float(L)
+        self.Delta0 = float(Delta0)
+        self.gap = gap_eV * e0
+        self.me = me_rel * me0
+        self.mh = mh_rel * me0
+        self.HO = HO_eV * e0
+        self.gam_e = float(gam_e)
+        self.gam_h = float(gam_h)
+        self.gam_eh = float(gam_eh)
+        self.epsr = float(epsr)
+        self.Oph = Oph_eV * e0 / hbar
+        self.Gph = Gph_eV * e0 / hbar
+        self.Edc = float(Edc)
+        self.jmax = int(jmax)
+        self.ntmax = int(ntmax)
+        if flags is None:
+            flags = (1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 0, 1, 0)
+        self.flags = {n: bool(int(v)) for n, v in zip(FLAG_NAMES, flags)}
+
+    @classmethod
+    def from_files(cls, qw_path, mb_path):
+        """Parse params/qw.params and params/mb.params (first token per
+        line, SBEs.py:4978-5022)."""
+        def first(path, n):
+            vals = []
+            with open(path, encoding="utf-8") as f:
+                for line in f:
+                    if line.strip():
+                        vals.append(line.split()[0])
+                    if len(vals) == n:
+                        break
+            if len(vals) < n:
+                raise ValueError(f"{path}: expected {n} values, found {len(vals)}")
+            return vals
+
+        q = first(qw_path, 15)
+        m = first(mb_path, 15)
+        return cls(L=float(q[0]), Delta0=float(q[1]), gap_eV=float(q[2]),
+                   me_rel=float(q[3]), mh_rel=float(q[4]), HO_eV=float(q[5]),
+                   gam_e=float(q[6]), gam_h=float(q[7]), gam_eh=float(q[8]),
+                   epsr=float(q[9]), Oph_eV=float(q[10]), Gph_eV=float(q[11]),
+                   Edc=float(q[12]), jmax=int(q[13]), ntmax=int(q[14]),
+                   flags=[int(v) for v in m])
+
+    def flag_bits(self):
+        b = 0
+        for n, bit in FLAG_BITS.items():
+            if self.flags.get(n):
+                b |= 1 << bit
+        return b
+
+
+def derive_grid(prm):
+    """Grid and material constants (SBEs.py:4754-4823, 5028-5051)."""
+    kmax = np.sqrt(1.2 * prm.gap * 2.0 * prm.me / (hbar**2))
+    dkr = TWOPI / (2.0 * prm.L)
+    nk = int(np.floor(kmax / dkr) * 2 + 1)
+    Nr = nk * 2
+    N = nk - 1
+    dnk = (N - 1) // 2
+    # dkr * (-dnk + k - 0.5): exact half-integer times dkr, one rounding
+    kr = np.array([dkr * (-dnk + k - 0.5) for k in range(N)])
+    r = np.linspace(-(2 * prm.L) / 2.0, (2 * prm.L) / 2.0, Nr)
+    qd = TWOPI / (2 * prm.L)
+    Qr = np.arange(Nr, dtype=float) * qd
+    Qr = Qr - Qr[Nr // 2]
+    Qr = np.roll(Qr, Nr // 2)
+    Qr[0] = -Qr[0]
+    dx = Qr[2] - Qr[1]
+    xi = np.round((Qr + 0.1 * dx) / dx).astype(np.int32)
+    NQ0 = int(np.nonzero(xi == 0)[0][0])
+    ae = np.sqrt(prm.me * prm.HO) / hbar
+    ah = np.sqrt(prm.mh * prm.HO) / hbar
+    g = dict(
+        N=N, Nr=Nr, dkr=dkr, kr=kr, r=r, Qr=Qr, NQ0=NQ0,
+        dr=(r[2] - r[1]) * (Nr - 1) / float(Nr),
+        dcv=np.sqrt((e0 * hbar) ** 2 / (6.0 * me0 * prm.gap) * (me0 / prm.me - 1.0)),
+        alphae=ae, alphah=ah,
+        ehint=np.sqrt(2.0 * ae * ah / (ae**2 + ah**2)),
+        gam_eh=(prm.gam_e + prm.gam_h) / 2.0,
+        qc=2.0 * ae * ah / (ae + ah),
+        area=np.sqrt(2.0 * np.pi) / np.sqrt(ae**2 + ah**2) * prm.Delta0,
+        Ee=hbar**2 * kr**2 / (2.0 * prm.me),
+        Eh=hbar**2 * kr**2 / (2.0 * prm.mh),
+    )
+    return g
+
+
+def fermi(E):
+    """phonons.py:317-340."""
+    return 1.0 / (np.exp(E / kB / TEMP) + 1.0)
+
+
+# ---------------------------------------------------------------------------
+# accurate K0 (Zhang & Jin, "Computation of Special Functions", CIK01) for
+# Chi1D; vectorised.  The reference evaluates it through usefulsubs.K03.
+# ---------------------------------------------------------------------------
+
+def _i0_asym_coef(n):
+    c, num = [], 1.0
+    for k in range(1, n + 1):
+        num *= (2 * k - 1) ** 2
+        c.append(num / (math.factorial(k) * 8.0**k))
+    return c
+
+
+def _i0k0_asym_coef(n):
+    c, v = [], 1.0
+    for k in range(1, n + 1):
+        v *= (2 * k - 1) ** 3 / (8.0 * k)
+        c.append(v)
+    return c
+
+
+_CI0 = _i0_asym_coef(12)
+_CIK = _i0k0_asym_coef(10)
+
+
+def k0_zhang_jin(x):
+    x = np.asarray(x, dtype=float)
+    out = np.zeros_like(x)
+    it = np.nditer(x, flags=["multi_index"])
+    for xv in it:
+        xv = float(xv)
+        if xv > 100.0:
+            continue
+        z2 = xv * xv
+        if xv <= 18.0:
+            i0, cr = 1.0, 1.0
+            for k in range(1, 51):
+                cr = 0.25 * cr * z2 / (k * k)
+                i0 += cr
+                if abs(cr / i0) < 1e-15:
+                    break
+        else:
+            k0n = 12 if xv < 35.0 else (9 if xv < 50.0 else 7)
+            s = 1.0
+            for k in range(1, k0n + 1):
+                s += _CI0[k - 1] * xv ** (-k)
+            i0 = np.exp(xv) / np.sqrt(2.0 * np.pi * xv) * s
+        if xv <= 9.0:
+            cs, cw, w0, cr = 0.0, 0.0, 0.0, 1.0
+            ct = -np.log(0.5 * xv) - EULER
+            for k in range(1, 51):
+                w0 += 1.0 / k
+                cr = 0.25 * cr / (k * k) * z2
+                cs += cr * (w0 + ct)
+                if abs((cs - cw) / cs) < 1e-15:
+                    break
+                cw = cs
+            out[it.multi_index] = ct + cs
+        else:
+            s = 1.0
+            for k in range(1, 11):
+                s += _CIK[k - 1] * z2 ** (-k)
+            out[it.multi_index] = 0.5 / xv * s / i0
+    return out
+
+
+# ---------------------------------------------------------------------------
+# FMM cubic spline as a linear map (spliner.py:153-188, 1287-1396)
+# ---------------------------------------------------------------------------
+
+def spline_matrix(x0, x1):
+    """Matrix M (len(x1) x len(x0)) with rescale_1D(x0, y, x1) == M @ y:
+    the spline coefficients are linear in y, so they are computed for the
+    identity (all unit vectors at once); rows for x1 outside
+    [min x0, max x0] are zero."""
+    x0 = np.asarray(x0, dtype=float)
+    n = len(x0)
+    y = np.eye(n)
+    b = np.zeros((n, n))
+    c = np.zeros((n, n))
+    d = np.zeros((n, n))
+    dd = np.zeros(n)
+    bb = np.zeros(n)
+    dd[0] = x0[1] - x0[0]
+    c[1] = (y[1] - y[0]) / dd[0]
+    for i in range(1, n - 1):
+        dd[i] = x0[i + 1] - x0[i]
+        bb[i] = 2.0 * (dd[i - 1] + dd[i])
+        c[i + 1] = (y[i + 1] - y[i]) / dd[i]
+        c[i] = c[i + 1] - c[i]
+    bb[0] = -dd[0]
+    bb[n - 1] = -dd[n - 2]
+    c[0] = 0.0
+    c[n - 1] = 0.0
+    if n != 3:
+        c[0] = c[2] / (x0[3] - x0[1]) - c[1] / (x0[2] - x0[0])
+        c[n - 1] = c[n - 2] / (x0[n - 1] - x0[n - 3]) - c[n - 3] / (x0[n - 2] - x0[n - 4])
+        c[0] = c[0] * dd[0] ** 2 / (x0[3] - x0[0])
+        c[n - 1] = -c[n - 1] * dd[n - 2] ** 2 / (x0[n - 1] - x0[n - 4])
+    for i in range(1, n):
+        t = dd[i - 1] / bb[i - 1]
+        bb[i] = bb[i] - t * dd[i - 1]
+        c[i] = c[i] - t * c[i - 1]
+    c[n - 1] = c[n - 1] / bb[n - 1]
+    for i in range(n - 2, -1, -1):
+        c[i] = (c[i] - dd[i] * c[i + 1]) / bb[i]
+    b[n - 1] = (y[n - 1] - y[n - 2]) / dd[n - 2] + dd[n - 2] * (c[n - 2] + 2.0 * c[n - 1])
+    for i in range(n - 1):
+        b[i] = (y[i + 1] - y[i]) / dd[i] - dd[i] * (c[i + 1] + 2.0 * c[i])
+        d[i] = (c[i + 1] - c[i]) / dd[i]
+        c[i] = 3.0 * c[i]
+    c[n - 1] = 3.0 * c[n - 1]
+    d[n - 1] = d[n - 2]
+    x1 = np.asarray(x1, dtype=float)
+    M = np.zeros((len(x1), n))
+    lo, hi = x0.min(), x0.max()
+    j = np.searchsorted(x0, x1, side="right") - 1  # spliner.locate
+    j = np.clip(j, 0, n - 1)
+    inside = (x1 >= lo) & (x1 <= hi)
+    for r in np.nonzero(inside)[0]:
+        jj = j[r]
+        dx = x1[r] - x0[jj]
+        M[r] = y[jj] + dx * (b[jj] + dx * (c[jj] + dx * d[jj]))
+    return M
+
+
+# ---------------------------------------------------------------------------
+# Many-body scattering lists (coulomb.py:367-499 tables, as read by
+# _MBCE_core/_MBCH_core, coulomb.py:956-1071)
+# ---------------------------------------------------------------------------
+
+def _stored_table_chunk(kind, i, N, Ee, Eh, ge, gh):
+    """Row i (stored index, 1..N) of the (N+1)^3 Gaussian table:
+    S[i][j][l] for j, l in [0, N] (index 0 planes are zero)."""
+    geh = (ge + gh) / 2.0
+    out = np.zeros((N + 1, N + 1))
+    if i == 0:
+        return out
+    a = i - 1
+    b = np.arange(N)[:, None]
+    c = np.arange(N)[None, :]
+    if kind == "eh":
+        k30 = c + b - a
+        hb = hbar * geh
+        ok = (k30 >= 0) & (k30 < N)
+        kk = np.clip(k30, 0, N - 1)
+        Ed = Ee[a] + Eh[b] - Eh[kk] - Ee[c]
+    else:
+        E = Ee if kind == "ee" else Eh
+        hb = hbar * (ge if kind == "ee" else gh)
+        k30 = a + b - c
+        ok = (k30 >= 0) & (k30 < N)
+        kk = np.clip(k30, 0, N - 1)
+        Ed = E[a] + E[b] - E[kk] - E[c]
+    val = (TWOPI / hbar * (a != c) * (b != k30)
+           * (1.0 / (np.sqrt(np.pi) * hb) * np.exp(-((Ed / hb) ** 2))))
+    out[1:, 1:] = np.where(ok, val, 0.0)
+    return out
+
+
+def many_body_lists(N, Ee, Eh, ge, gh, rel_cut=0.0):
+    """Eight CSR lists of the non-negligible stored-table entries (i, j, l)
+    grouped by the rate index they feed:
+      0 eh->WinE[i]   1 eh->WoutE[l]   2 ee->WinE[i]   3 ee->WoutE[l]
+      4 eh->WinH[j]   5 eh->WoutH[j+l-i]   6 hh->WinH[i]   7 hh->WoutH[l]
+    Entries whose Gaussian weight is below rel_cut times the peak weight
+    2 pi/hbar /(sqrt(pi) hbar g) are dropped (their total is below
+    N^3 rel_cut of the largest term)."""
+    geh = (ge + gh) / 2.0
+    peak = {k: TWOPI / hbar / (np.sqrt(np.pi) * hbar * g)
+            for k, g in (("eh", geh), ("ee", ge), ("hh", gh))}
+    ent = {k: [] for k in ("eh", "ee", "hh")}
+    J, L = np.meshgrid(np.arange(N + 1), np.arange(N + 1), indexing="ij")
+    for kind in ("eh", "ee", "hh"):
+        cut = rel_cut * peak[kind]
+        for i in range(1, N):
+            S = _stored_table_chunk(kind, i, N, Ee, Eh, ge, gh)[:N, :N]
+            m = S > cut
+            if m.any():
+                jj = J[:N, :N][m]
+                ll = L[:N, :N][m]
+                ent[kind].append((np.full(jj.shape, i), jj, ll, S[m]))
+    def cat(kind):
+        if not ent[kind]:
+            z = np.zeros(0, dtype=np.int64)
+            return z, z, z, np.zeros(0)
+        return tuple(np.concatenate([e[t] for e in ent[kind]]) for t in range(4))
+
+    lists = []
+    for li, (kind, tgt, cond) in enumerate((
+            ("eh", "i", "jli_il"), ("eh", "l", "jli_il"),
+            ("ee", "i", "ijl_il"), ("ee", "l", "ijl_il"),
+            ("eh", "j", "jli_il"), ("eh", "jli", "jli_li"),
+            ("hh", "i", "ijl_il"), ("hh", "l", "ijl_il"))):
+        i, j, l, v = cat(kind)
+        if cond.startswith("jli"):
+            idx3 = j + l - i
+        else:
+            idx3 = i + j - l
+        ok = (idx3 >= 0) & (idx3 < N)
+        ok &= (i >= l) if cond.endswith("_il") else (l >= i)
+        i, j, l, v, idx3 = i[ok], j[ok], l[ok], v[ok], idx3[ok]
+        t = {"i": i, "l": l, "j": j, "jli": j + l - i}[tgt]
+        order = np.lexsort((l, j, i, t))
+        i, j, l, v, t = i[order], j[order], l[order], v[order], t[order]
+        off = np.zeros(N + 1, dtype=np.int32)
+        np.add.at(off, t + 1, 1)
+        off = np.cumsum(off).astype(np.int32)
+        packed = (i.astype(np.uint32) | (j.astype(np.uint32) << 10)
+                  | (l.astype(np.uint32) << 20))
+        lists.append((off, packed, v.astype(np.float64)))
+    return lists
+
+
+# ---------------------------------------------------------------------------
+# all per-N tables
+# ---------------------------------------------------------------------------
+
+def build_tables(prm, rr, vint_fn):
+    """Per-N tables for the device path.  ``rr`` is the propagation grid;
+    ``vint_fn(Qr, r, ae, ah, Delta0, pref) -> (3, Nr)`` evaluates the
+    unscreened Coulomb integrals (on the GPU)."""
+    g = derive_grid(prm)
+    N, Nr = g["N"], g["Nr"]
+    if g["NQ0"] != 0:
+        raise ValueError("the Qr grid must have its zero at index 0")
+    kr, Ee, Eh = g["kr"], g["Ee"], g["Eh"]
+    ae, ah = g["alphae"], g["alphah"]
+    t = {}
+    # coulomb.py:502-577: V0[k][k'] = tab[kkp] for kkp >= 0, kkp = k - k'
+    pref = e0**2 / (TWOPI * eps0 * prm.epsr * prm.L)
+    vt = np.asarray(vint_fn(g["Qr"], g["r"], ae, ah, prm.Delta0, pref))
+    t["vtab"] = np.ascontiguousarray(vt[:, :N])
+    # coulomb.py:302-333 (qe[d,0]), 580-630 (Chi1D)
+    d0 = np.abs(kr[0] - kr)  # |ky[0] - ky[d]|
+    qe = np.maximum(d0, ae / 2.0)
+    qh = np.maximum(d0, ah / 2.0)
+    Re = np.sqrt((2.0 / ae) ** 2 + prm.Delta0**2)
+    Rh = np.sqrt((2.0 / ah) ** 2 + prm.Delta0**2)
+    sc = e0**2 / (TWOPI * eps0 * prm.epsr * hbar**2)
+    t["chi"] = np.stack([prm.me * k0_zhang_jin(qe * Re) / qe * sc,
+                         prm.mh * k0_zhang_jin(qh * Rh) / qh * sc])
+    t["qeh"] = np.stack([qe, qh])
+    # density cap (coulomb.py:1216-1218): min(qe[1,1], qh[1,1]) / pi * 0.99
+    t["n1d_max"] = min(ae / 2.0, ah / 2.0) / np.pi * 0.99
+    # phonons.py:55-136
+    NO = 1.0 / (np.exp(hbar * prm.Oph / kB / TEMP) - 1.0)
+    hO, hG = hbar * prm.Oph, hbar * prm.Gph
+    dEe = Ee[:, None] - Ee[None, :]
+    dEh = Eh[:, None] - Eh[None, :]
+    idel = 1.0 - np.eye(N)
+    t["EP"] = (NO / ((dEe - hO) ** 2 + hG**2) + (NO + 1.0) / ((dEe + hO) ** 2 + hG**2)) * 2.0 * prm.Gph * idel
+    t["HP"] = (NO / ((dEh - hO) ** 2 + hG**2) + (NO + 1.0) / ((dEh + hO) ** 2 + hG**2)) * 2.0 * prm.Gph * idel
+    t["Vscale"] = hbar * prm.Oph * prm.epsr * (1.0 / 8.2 - 1.0 / 10.0)
+    # dephasing.py:472-539, stored transposed [q][k]
+    dk = kr[1] - kr[0]
+    kmaxd, kmind = kr[N - 1] + dk, kr[0] - dk
+    nk0 = (N - 1) // 2 + 1
+
+    def idx(v):
+        return (np.round(np.clip(v, kmind, kmaxd) / dk)).astype(np.int64) + nk0 - 1
+
+    K = kr[:, None]
+    Qv = kr[None, :]
+    k1p0 = ((prm.me + prm.mh) * Qv - 2 * prm.mh * K) / 2.0 / prm.me
+    k10 = ((prm.me + prm.mh) * Qv - 2 * prm.me * K) / 2.0 / prm.mh
+    tabs = [idx(K + Qv), idx(K - Qv), idx(k1p0 - Qv), idx(k1p0), idx(k10 - Qv), idx(k10)]
+    t["didx"] = np.ascontiguousarray(np.stack([a.T for a in tabs]).astype(np.int32))
+    t["xe"] = prm.me / hbar**2 * np.abs(kr) / (np.abs(kr) + dk * 1e-5) ** 2 / dk
+    t["xh"] = prm.mh / hbar**2 * np.abs(kr) / (np.abs(kr) + dk * 1e-5) ** 2 / dk
+    t["iq"] = np.round(np.abs(kr / dk)).astype(np.int32)
+    # emission.py:35-91, 220-255
+    kBT, hg = kB * TEMP, hbar * g["gam_eh"]
+    dhw = min(kBT, hg) / 20.0
+    nhw = int(np.ceil((kBT + hg) * 4.0 / dhw))
+    hom = (np.arange(nhw) + 0.5) * dhw
+    dcv = abs(g["dcv"])
+    t["homega"] = hom
+    t["hsq"] = (3.0 * dcv**2 * g["ehint"] / eps0 / np.sqrt(prm.epsr) * g["ehint"] / hbar
+                * ((hg / np.pi) / (hom**2 + hg**2)) * np.exp(-hom / kB / TEMP))
+    t["dhw"] = hom[1] - hom[0]
+    # qwoptics.py:630-650
+    t["window"] = np.exp(-((g["r"] / (prm.L / 2.0)) ** 150))
+    # dcfield.py:767-796 Transport phase ramp Y = GetKArray(N, (N-1) dky)
+    dky = kr[2] - kr[1]
+    Lw = (N - 1) * dky
+    y = np.arange(N, dtype=float) * (TWOPI / Lw)
+    t["ytr"] = y - y[N // 2]
+    # Prop2QW as one matrix on the real parts (qwoptics.py:667-715)
+    rr = np.asarray(rr, dtype=float)
+    M1 = spline_matrix(rr, g["r"])                       # [Nr][Nx]
+    W = t["window"]
+    qbins = np.arange(-(N - 1), N) % Nr
+    F = np.exp(-2j * np.pi * np.outer(qbins, np.arange(Nr)) / Nr)
+    G = -(F * W[None, :]) @ M1 / Nr                       # [2N-1][Nx]
+    colmax = np.max(np.abs(G), axis=0)
+    keep = np.nonzero(colmax > 1e-25 * colmax.max())[0]
+    gx0, gx1 = int(keep[0]), int(keep[-1]) + 1
+    t["gx0"], t["gnx"] = gx0, gx1 - gx0
+    t["gmat"] = np.ascontiguousarray(G[:, gx0:gx1])
+    # QW2Prop spline r -> rr (rows inside the wire only)
+    M2 = spline_matrix(g["r"], rr)                       # [Nx][Nr]
+    rows = np.nonzero((rr >= g["r"].min()) & (rr <= g["r"].max()))[0]
+    bx0, bx1 = (int(rows[0]), int(rows[-1]) + 1) if rows.size else (0, 0)
+    t["bx0"], t["bnx"] = bx0, bx1 - bx0
+    t["m2"] = np.ascontiguousarray(M2[bx0:bx1])
+    # many-body scattering lists
+    if prm.flags.get("EHs"):
+        if prm.flags.get("LorentzDelta"):
+            raise NotImplementedError(
+                "LorentzDelta = 1 (Lorentzian many-body tables) is not supported "
+                "by the device path")
+        t["mb"] = many_body_lists(N, Ee, Eh, prm.gam_e, prm.gam_h)
+    else:
+        z = np.zeros(N + 1, dtype=np.int32)
+        t["mb"] = [(z, np.zeros(0, np.uint32), np.zeros(0))] * 8
+    return g, t

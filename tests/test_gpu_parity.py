@@ -1,0 +1,216 @@
+"""GPU parity: every stage of the HIP path through the C ABI against the
+oracle (FP64, tolerance stated per check), and full QWCalculator
+trajectories against the reference's golden vectors."""
+
+import numpy as np
+import pytest
+
+from oracle import sbe_oracle as O
+from tests.helpers import (load_golden, params_pair, pulse, random_state, rel,
+                           wire_length)
+
+pytestmark = pytest.mark.gpu
+
+RHS_TOL = 1e-12     # per RHS evaluation, normwise (max |diff| / max |ref|)
+TRAJ_TOL = 1e-9     # trajectories
+
+
+def _engine(prm, nw, rr, prec="fp64", wire_ids=None):
+    from pulsesuite_amd.engine import WireGroup
+    return WireGroup(prm, nw, rr, wire_ids=wire_ids, prec=prec)
+
+
+class Case:
+    """Matched device group and oracle solver on the same random state."""
+
+    def __init__(self, N, nw, seed, flags=O.SHIPPED_FLAGS, prec="fp64", nx=96):
+        import torch
+        torch.cuda.synchronize()
+        self.L = wire_length(N)
+        prm, oprm = params_pair(self.L, flags)
+        self.rr = np.linspace(-1.2 * self.L, 1.2 * self.L, nx)
+        self.nw = nw
+        print(f"[case] N={N} nw={nw} seed={seed} prec={prec}", flush=True)
+        self.wg = _engine(prm, nw, self.rr, prec=prec)
+        self.og = O.Grid(oprm)
+        self.ot = O.Tables(self.og)
+        self.os = O.Solver(oprm, nw, self.rr, 1.0, tables=self.ot, grid=self.og)
+        rng = np.random.default_rng(seed)
+        st = random_state(rng, N, nw)
+        self.state = st
+        self.wg.set_state(*st)
+        for dst, src in zip((self.os.CC2, self.os.DD2, self.os.YY2,
+                             self.os.CC3, self.os.DD3, self.os.YY3), st):
+            dst[:] = src
+        w0 = 2 * np.pi * 299792458.0 / 800e-9
+        env = np.cos(2 * np.pi * self.rr / (3 * self.L)) + 0.4
+        self.Exx = pulse(45e-15, 3e7, 10e-15, 50e-15, w0) * env + 0j
+        self.Eyy = 2.0 * self.Exx
+        self.Ezz = 0.3 * self.Exx
+        self.wg.set_fields(self.Exx, self.Eyy, self.Ezz)
+
+    def oracle_fields(self):
+        g, t = self.og, self.ot
+        return tuple(O.prop2qw(g, t, self.rr, E) for E in (self.Exx, self.Eyy, self.Ezz))
+
+
+@pytest.fixture(scope="module")
+def case64():
+    return Case(64, 3, seed=11)
+
+
+def test_library_loads_on_gpu():
+    from pulsesuite_amd import capi
+    lib = capi.load()
+    assert lib.psa_abi_version() == capi.PSA_ABI_VERSION
+
+
+def test_vint_table(case64):
+    c = case64
+    for i in range(3):
+        assert rel(c.wg.tab["vtab"][i], c.ot.V0[i][:, 0]) < 1e-13
+
+
+def test_prop2qw_fields(case64):
+    import torch
+    c = case64
+    c.wg.prop2qw()
+    torch.cuda.synchronize()
+    N, Nr = c.wg.N, c.wg.Nr
+    qb = np.arange(-(N - 1), N) % Nr
+    exq = c.wg.s["exq"].cpu().numpy()
+    for f, E in enumerate(c.oracle_fields()):
+        assert rel(exq[0, f], E[qb]) < 1e-13
+
+
+def test_prep_stage(case64):
+    import torch
+    c = case64
+    c.wg.prep()
+    torch.cuda.synchronize()
+    N = c.wg.N
+    pr = c.wg.s["prep"].cpu().numpy()
+    for w in range(c.nw):
+        C3, D3 = c.state[3][:, :, w], c.state[4][:, :, w]
+        ne, nh = np.diag(C3).copy(), np.diag(D3).copy()
+        VC, E1D = O.screened_arrays(c.og, c.ot, True, ne, nh)
+        assert rel(pr[w, 4 * N:5 * N], E1D[:, 0]) < 1e-13
+        for ch in range(3):
+            assert rel(pr[w, (5 + ch) * N:(6 + ch) * N], VC[:, 0, ch]) < 1e-13
+        g0 = 8 * N + 4 * (2 * N - 1)
+        assert rel(pr[w, g0:g0 + N], O.gamma_e(c.og, c.ot, ne, nh, VC)) < 1e-12
+        assert rel(pr[w, g0 + N:g0 + 2 * N], O.gamma_h(c.og, c.ot, ne, nh, VC)) < 1e-12
+        assert rel(pr[w, g0 + 2 * N:g0 + 3 * N], O.spont_emission(c.og, c.ot, ne, nh, VC)) < 1e-12
+
+
+def test_calc_h_stage(case64):
+    import torch
+    c = case64
+    c.wg.prop2qw()
+    c.wg.prep()
+    c.wg.calc_h()
+    torch.cuda.synchronize()
+    Ex, Ey, Ez = c.oracle_fields()
+    Heh_g = c.wg.s["Heh"].cpu().numpy()
+    Heo = c.wg.s["Heo"].cpu().numpy()
+    Hho = c.wg.s["Hho"].cpu().numpy()
+    hd = c.wg.s["hdiag"].cpu().numpy()
+    for w in range(c.nw):
+        C3, D3, p3 = c.state[3][:, :, w], c.state[4][:, :, w], c.state[5][:, :, w]
+        VC, _ = O.screened_arrays(c.og, c.ot, True, np.diag(C3), np.diag(D3))
+        Meh = O.calc_meh(c.og, c.ot, Ex, Ey, Ez, w + 1)
+        Heh, Hee, Hhh = O.calc_h(c.og, Meh, C3, D3, p3, VC, True, True)
+        assert rel(Heh_g[w], Heh) < 1e-12
+        off = lambda m: m - np.diag(np.diag(m))  # noqa: E731
+        assert rel(Heo[w], off(Hee)) < 1e-12
+        assert rel(Hho[w], off(Hhh)) < 1e-12
+        assert rel(hd[w, 0], np.diag(Hee)) < 1e-14
+        assert rel(hd[w, 1], np.diag(Hhh)) < 1e-14
+
+
+def test_offdiag_tables(case64):
+    import torch
+    c = case64
+    c.wg.prep()
+    c.wg.offdiag()
+    torch.cuda.synchronize()
+    N = c.wg.N
+    dd = c.wg.s["dedh"].cpu().numpy()
+    g = c.og
+    for w in range(c.nw):
+        C3, D3 = c.state[3][:, :, w], c.state[4][:, :, w]
+        ne, nh = np.diag(C3).copy(), np.diag(D3).copy()
+        VC, _ = O.screened_arrays(c.og, c.ot, True, ne, nh)
+        De = O.offdiag_deph_e2(g, ne, nh, VC, g.gam_eh, g.prm.gam_h)
+        Dh = O.offdiag_deph_h2(g, ne, nh, VC, g.prm.gam_e, g.prm.gam_h)
+        assert rel(dd[w, 0], De[1:2 * N, :].T) < 1e-12
+        assert rel(dd[w, 1], Dh[1:2 * N, :].T) < 1e-12
+
+
+def test_rhs_fp64(case64):
+    import torch
+    c = case64
+    c.wg.rhs_eval()
+    torch.cuda.synchronize()
+    Ex, Ey, Ez = c.oracle_fields()
+    dp_g = c.wg.s["pX"].cpu().numpy()
+    dC_g = c.wg.s["CX"].cpu().numpy()
+    dD_g = c.wg.s["DX"].cpu().numpy()
+    for w in range(c.nw):
+        C3, D3, p3 = c.state[3][:, :, w], c.state[4][:, :, w], c.state[5][:, :, w]
+        dp, dC, dD, _ = c.os.rhs(p3, C3, D3, Ex, Ey, Ez, w + 1)
+        assert rel(dp_g[w], dp) < RHS_TOL
+        assert rel(dC_g[w], dC) < RHS_TOL
+        assert rel(dD_g[w], dD) < RHS_TOL
+
+
+@pytest.mark.parametrize("tag", ["n16_mb", "n16_rhs", "n32_mb"])
+def test_trajectory_vs_reference_golden(tag):
+    """Full QWCalculator steps from the reference's own seeded state and
+    fields (golden vectors written from the reference)."""
+    import torch
+    z = load_golden(tag)
+    from pulsesuite_amd.tables import QWParams
+    prm = QWParams(L=float(z["L"]), me_rel=0.067, flags=tuple(z["mb"]))
+    print(f"[golden] {tag}", flush=True)
+    wg = _engine(prm, 2, z["rr"])
+    wg.set_state(z["s0_C2"], z["s0_D2"], z["s0_p2"], z["s0_C3"], z["s0_D3"], z["s0_p3"])
+    dt = float(z["dt"])
+    for it in range(int(z["nsteps"])):
+        Exx = z["t_Exx"][it]
+        wg.set_fields(Exx, z["t_Eyy"][it], np.zeros_like(Exx))
+        wg.step(dt)
+        torch.cuda.synchronize()
+        Pxx, Pyy, Pzz, Rho = wg.outputs()
+        for w in range(2):
+            j = 2 * it + w
+            assert rel(Pxx[w], z["t_Pxx"][j]) < TRAJ_TOL, (it, w)
+            assert rel(Pyy[w], z["t_Pyy"][j]) < TRAJ_TOL, (it, w)
+            assert rel(Pzz[w], z["t_Pzz"][j]) < TRAJ_TOL, (it, w)
+            assert rel(Rho[w], z["t_Rho"][j]) < TRAJ_TOL, (it, w)
+        C2, D2, p2, C3, D3, p3 = wg.get_state()
+        for v, nm in ((C2, "C2"), (D2, "D2"), (p2, "p2"), (C3, "C3"), (D3, "D3"), (p3, "p3")):
+            assert rel(v, z["t_" + nm][it]) < TRAJ_TOL, (it, nm)
+        assert rel(wg.observables()["I0"], z["t_I0"][it]) < TRAJ_TOL
+
+
+def test_step_vs_oracle_n64():
+    """Five full steps, three wires (signs yw(w), (-1)^w differ per wire),
+    N = 64, all mb.params terms on."""
+    import torch
+    c = Case(64, 3, seed=5)
+    dt = 10e-18
+    for it in range(5):
+        c.wg.step(dt)
+        torch.cuda.synchronize()
+        Pxx, Pyy, Pzz, Rho = c.wg.outputs()
+        for w in range(c.nw):
+            ref = c.os.qw_calculator(c.Exx, c.Eyy, c.Ezz, dt, w + 1)
+            assert ref is not None
+            assert rel(Pxx[w], ref[0]) < TRAJ_TOL, (it, w)
+            assert rel(Pyy[w], ref[1]) < TRAJ_TOL, (it, w)
+            assert rel(Pzz[w], ref[2]) < TRAJ_TOL, (it, w)
+            assert rel(Rho[w], ref[3]) < TRAJ_TOL, (it, w)
+        got = c.wg.get_state()
+        for v, r in zip(got, (c.os.CC2, c.os.DD2, c.os.YY2, c.os.CC3, c.os.DD3, c.os.YY3)):
+            assert rel(v, r) < TRAJ_TOL, it
