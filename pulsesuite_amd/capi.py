@@ -73,6 +73,11 @@ def load(path=None):
     if _lib is not None:
         return _lib
     path = path or LIB_PATH
+    # One HIP runtime per process: torch ships its own libamdhip64
+    # (soname libamdhip64.so.7).  Importing torch first makes this library's
+    # libamdhip64.so.7 dependency bind to that already-loaded copy instead of
+    # pulling in a second runtime that would not see the device.
+    import torch  # noqa: F401
     if not os.path.exists(path):
         raise PsaError(
             f"{path} is missing: build it with `python -m pulsesuite_amd._build` "

@@ -10,9 +10,9 @@
 // product costs four real 16x16x4 MFMAs.  The epilogue hands every output
 // element with its real/imaginary accumulator to the problem's store().
 //
-// MFMA 16x16x4 lane layout (f64 and f32 alike): lane l supplies
-// A[l%16][l/16] and B[l/16][l%16]; accumulator element v of lane l is
-// D[4*(l/16)+v][l%16].
+// MFMA 16x16x4 lane layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]
+// for both f32 and f64.  Accumulator element v of lane l is D[4*(l/16)+v][l%16]
+// for f32 but D[l/16+4*v][l%16] for f64 (checked on gfx950).
 #pragma once
 
 #include "psa_dev.h"
@@ -27,6 +27,7 @@ struct Mfma;
 template <>
 struct Mfma<double> {
   typedef dv4 acc;
+  static __device__ __forceinline__ int row(int kl, int v) { return kl + 4 * v; }
   static __device__ __forceinline__ acc mma(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -34,6 +35,7 @@ struct Mfma<double> {
 template <>
 struct Mfma<float> {
   typedef fv4 acc;
+  static __device__ __forceinline__ int row(int kl, int v) { return 4 * kl + v; }
   static __device__ __forceinline__ acc mma(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(P p) {
     for (int sj = 0; sj < 2; ++sj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = m0 + wm * 32 + si * 16 + 4 * kl + v;
+        const int row = m0 + wm * 32 + si * 16 + Mfma<T>::row(kl, v);
         const int col = n0 + wn * 32 + sj * 16 + xl;
         p.store(b, row, col, (double)aR[si][sj][v], (double)aI[si][sj][v]);
       }
