@@ -83,7 +83,9 @@ typedef struct psa_group {
   const double *rq;                   /* [Nr] QW real-space grid r */
   const double *ytr;                  /* [N] Transport phase ramp Y */
   const double *gmat;                 /* [2N-1][gnx] complex: Prop2QW (spline, window, FFTG) */
-  const double *m2;                   /* [bnx][Nr] real: QW2Prop spline r -> rr */
+  const double *m2;                   /* [Nr][bnx] real: QW2Prop spline r -> rr (transposed) */
+  const double *phi;                  /* [2N-1][Nr] complex exp(i d dkr r) */
+  const double *twid;                 /* [N] complex exp(-2 pi i j / N) */
   const int32_t *mb_off[8];           /* CSR offsets [N+1] per many-body list */
   const uint32_t *mb_idx[8];          /* packed (i, j, l) per entry */
   const double *mb_val[8];            /* table value per entry */
@@ -102,6 +104,8 @@ typedef struct psa_group {
   double *src;                        /* [Nw][PSA_SRC_STRIDE] real-space Px, Py, Pz, re, rh + diagonal sums */
   double *out;                        /* [4][Nw][Nx] complex: Pxx, Pyy, Pzz, Rho */
   double *obs;                        /* [Nw][PSA_OBS_N] observables (I0, totals, ...) */
+  double *tphase;                     /* [Nw][N] complex Transport phase exp(-i Y dk) */
+  double *part;                       /* [Nw][64] partial sums (QW2Prop normalisation) */
   const int32_t *wid_host;            /* [Nw] host copy of wire_id */
 } psa_group;
 
@@ -115,10 +119,17 @@ typedef struct psa_group {
 #define PSA_PREP_GAME(N) (8 * (N) + 4 * (2 * (N) - 1))
 #define PSA_PREP_GAMH(N) (PSA_PREP_GAME(N) + (N))
 #define PSA_PREP_RSP(N) (PSA_PREP_GAME(N) + 2 * (N))
-#define PSA_PREP_STRIDE(N) (PSA_PREP_GAME(N) + 3 * (N))
+#define PSA_PREP_ANE(N) (PSA_PREP_GAME(N) + 3 * (N))  /* [N] |ne| */
+#define PSA_PREP_ANH(N) (PSA_PREP_GAME(N) + 4 * (N))  /* [N] |nh| */
+#define PSA_PREP_STRIDE(N) (PSA_PREP_GAME(N) + 5 * (N))
 
 #define PSA_OBS_N 8 /* I0, sum ne2, sum nh2, total(rho), ... */
-#define PSA_SRC_STRIDE(N, Nr) (5 * (Nr) + 8 * (2 * (N) - 1) + 8)
+/* per-wire src block: [5][Nr] Px, Py, Pz, re, rh (real space, final);
+ * [2][Nr] complex re, rh before the LF normalisation; [4][2N-1] complex
+ * diagonal sums; 8 scalars */
+#define PSA_SRC_RC(Nr) (5 * (Nr))
+#define PSA_SRC_DS(Nr) (9 * (Nr))
+#define PSA_SRC_STRIDE(N, Nr) (9 * (Nr) + 8 * (2 * (N) - 1) + 8)
 
 /* ---- library ---- */
 int psa_abi_version(void);

@@ -21,10 +21,10 @@ _INTS = ["N", "Nr", "Nw", "Nx", "Nhw", "flags", "prec", "gx0", "gnx", "bx0", "bn
 _DBLS = ["hbar", "e0", "gap", "L", "ehint", "dcv", "area", "dkr", "dr_r", "drr",
          "gam_e", "gam_h", "gam_eh", "Edc", "Vscale", "n1d_max", "dhw"]
 _TABS = ["kr", "Ee", "Eh", "vtab", "chi", "qeh", "EP", "HP", "xe", "xh", "didx",
-         "iq", "homega", "hsq", "window", "rq", "ytr", "gmat", "m2"]
+         "iq", "homega", "hsq", "window", "rq", "ytr", "gmat", "m2", "phi", "twid"]
 _WIRE = ["wire_id", "pA", "pB", "pX", "CA", "CB", "CX", "DA", "DB", "DX", "Heh",
          "Heo", "Hho", "hdiag", "prep", "dedh", "wmb", "exq", "efield", "src",
-         "out", "obs", "wid_host"]
+         "out", "obs", "tphase", "part", "wid_host"]
 
 
 class PsaGroup(ctypes.Structure):

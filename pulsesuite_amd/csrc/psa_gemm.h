@@ -48,13 +48,36 @@ struct Tile {
   T v[2][GBK][GLD];  // [re/im][k][x]
 };
 
+// Register fragment of one operand slab: each of the 256 threads holds 4
+// elements (value and LDS coordinates) between the global load and the LDS
+// store, so the next slab's loads are in flight during the MFMAs.
+template <typename T>
+struct Frag {
+  T re[4], im[4];
+  uint8_t k[4], x[4];
+  __device__ __forceinline__ void set(int e, int kk, int xx, T r, T i) {
+    re[e] = r;
+    im[e] = i;
+    k[e] = (uint8_t)kk;
+    x[e] = (uint8_t)xx;
+  }
+  __device__ __forceinline__ void commit(Tile<T> &t) const {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      t.v[0][k[e]][x[e]] = re[e];
+      t.v[1][k[e]][x[e]] = im[e];
+    }
+  }
+};
+
 // s[k][x] = sgn * op(X)(x0+x, k0+k); op(X)(x, k) = X[x][k] (xmajor) or
 // X[k][x]; optionally conjugated; zero outside [0,xext) x [0,kext).
 // Thread mapping follows the memory layout so global reads coalesce.
 template <typename T>
-__device__ __forceinline__ void load_dense(Tile<T> &s, const double *X, int ldx,
+__device__ __forceinline__ void load_dense(Frag<T> &s, const double *X, int ldx,
                                            int xext, int kext, bool xmajor,
-                                           bool conj, double sgn, int x0, int k0) {
+                                           bool conj, double sgn, int x0, int k0,
+                                           bool zero_diag = false) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -69,16 +92,16 @@ __device__ __forceinline__ void load_dense(Tile<T> &s, const double *X, int ldx,
     }
     const int gx = x0 + x, gk = k0 + k;
     cd v = {0.0, 0.0};
-    if (gx < xext && gk < kext)
+    if (gx < xext && gk < kext && !(zero_diag && gx == gk))
       v = ld(X, xmajor ? (long)gx * ldx + gk : (long)gk * ldx + gx);
-    s.v[0][k][x] = (T)(sgn * v.re);
-    s.v[1][k][x] = (T)(conj ? -sgn * v.im : sgn * v.im);
+    s.set(e, k, x, (T)(sgn * v.re), (T)(conj ? -sgn * v.im : sgn * v.im));
   }
 }
 
 template <typename T, class P>
 __global__ __launch_bounds__(256) void gemm_kernel(P p) {
-  __shared__ Tile<T> sA, sB;
+  extern __shared__ char lds_raw[];
+  Tile<T> *tiles = reinterpret_cast<Tile<T> *>(lds_raw);  // [2 buffers][A, B]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int b = blockIdx.z, m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
@@ -93,36 +116,56 @@ __global__ __launch_bounds__(256) void gemm_kernel(P p) {
     }
   const int kl = lane >> 4, xl = lane & 15;
   const int nt = p.nterms(b);
-  for (int t = 0; t < nt; ++t) {
-    const int K = p.kext(t, b);
-    for (int k0 = 0; k0 < K; k0 += GBK) {
-      p.loadA(sA, t, b, m0, k0);
-      p.loadB(sB, t, b, k0, n0);
-      __syncthreads();
-#pragma unroll
-      for (int kq = 0; kq < GBK; kq += 4) {
-        const int kk = kq + kl;
-        T ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          ar[s] = sA.v[0][kk][wm * 32 + s * 16 + xl];
-          br[s] = sB.v[0][kk][wn * 32 + s * 16 + xl];
-          if (!P::A_REAL) ai[s] = sA.v[1][kk][wm * 32 + s * 16 + xl];
-          if (!P::B_REAL) bi[s] = sB.v[1][kk][wn * 32 + s * 16 + xl];
-        }
-#pragma unroll
-        for (int si = 0; si < 2; ++si)
-#pragma unroll
-          for (int sj = 0; sj < 2; ++sj) {
-            aR[si][sj] = Mfma<T>::mma(ar[si], br[sj], aR[si][sj]);
-            if (!P::A_REAL && !P::B_REAL)
-              aR[si][sj] = Mfma<T>::mma(-ai[si], bi[sj], aR[si][sj]);
-            if (!P::B_REAL) aI[si][sj] = Mfma<T>::mma(ar[si], bi[sj], aI[si][sj]);
-            if (!P::A_REAL) aI[si][sj] = Mfma<T>::mma(ai[si], br[sj], aI[si][sj]);
-          }
-      }
-      __syncthreads();
+  // flattened (term, k-slab) iteration with a one-slab register prefetch
+  int t = 0, k0 = 0;
+  Frag<T> fa, fb;
+  p.fetchA(fa, 0, b, m0, 0);
+  p.fetchB(fb, 0, b, 0, n0);
+  fa.commit(tiles[0]);
+  fb.commit(tiles[1]);
+  __syncthreads();
+  int cur = 0;
+  while (true) {
+    int tn = t, kn = k0 + GBK;
+    if (kn >= p.kext(t, b)) {
+      tn = t + 1;
+      kn = 0;
     }
+    const bool more = tn < nt;
+    if (more) {
+      p.fetchA(fa, tn, b, m0, kn);
+      p.fetchB(fb, tn, b, kn, n0);
+    }
+    const Tile<T> &sA = tiles[2 * cur], &sB = tiles[2 * cur + 1];
+#pragma unroll
+    for (int kq = 0; kq < GBK; kq += 4) {
+      const int kk = kq + kl;
+      T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        ar[s] = sA.v[0][kk][wm * 32 + s * 16 + xl];
+        br[s] = sB.v[0][kk][wn * 32 + s * 16 + xl];
+        if (!P::A_REAL) ai[s] = sA.v[1][kk][wm * 32 + s * 16 + xl];
+        if (!P::B_REAL) bi[s] = sB.v[1][kk][wn * 32 + s * 16 + xl];
+      }
+#pragma unroll
+      for (int si = 0; si < 2; ++si)
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) {
+          aR[si][sj] = Mfma<T>::mma(ar[si], br[sj], aR[si][sj]);
+          if (!P::A_REAL && !P::B_REAL)
+            aR[si][sj] = Mfma<T>::mma(-ai[si], bi[sj], aR[si][sj]);
+          if (!P::B_REAL) aI[si][sj] = Mfma<T>::mma(ar[si], bi[sj], aI[si][sj]);
+          if (!P::A_REAL) aI[si][sj] = Mfma<T>::mma(ai[si], br[sj], aI[si][sj]);
+        }
+    }
+    if (!more) break;
+    fa.commit(tiles[2 * (cur ^ 1)]);
+    fb.commit(tiles[2 * (cur ^ 1) + 1]);
+    __syncthreads();
+    cur ^= 1;
+    t = tn;
+    k0 = kn;
   }
 #pragma unroll
   for (int si = 0; si < 2; ++si)
@@ -136,11 +179,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(P p) {
       }
 }
 
+template <typename T>
+constexpr size_t gemm_lds_bytes() {
+  return 4 * sizeof(Tile<T>);
+}
+
 // ===========================================================================
 // Problem: the SBE right-hand sides (SBEs.py:1941-1965, 2308-2327,
-// 2606-2625) with the Hamiltonian diagonals taken out of the products and
-// added exactly in the epilogue; for dp the off-diagonal dephasing
-// (dephasing.py:320-356) enters as two banded products.
+// 2606-2625).  The diagonals of Hee, Hhh (band energies) and of C, D
+// (occupations) are taken out of the products and applied exactly in the
+// FP64 epilogue, so the products only carry the Coulomb/coherence
+// corrections (this is what lets them run in reduced precision).  For dp
+// the off-diagonal dephasing (dephasing.py:320-356) enters as two banded
+// products.
 // batch b -> equation e = b % 3 (p, C, D), wire w = w0 + b / 3.
 // ===========================================================================
 struct RhsProb {
@@ -154,69 +205,76 @@ struct RhsProb {
   }
   __device__ int kext(int, int) const { return g.N; }
 
-  template <typename T>
-  __device__ void loadA(Tile<T> &s, int t, int b, int m0, int k0) const {
-    const int e = b % 3, w = w0 + b / 3, N = g.N;
-    const long o = mat_off(g, w);
+  // Operand descriptor: which matrix, how it is read (op(X)(x, k) = X[x][k]
+  // when xm, else X[k][x]), conjugation, sign, zeroed diagonal, or a band.
+  struct Op {
+    const double *X;
+    int band;  // -1 dense, else band table kind (0 De, 1 Dh)
+    bool xm, cj, zd;
+    double sg;
+  };
+
+  __device__ Op opA(int e, int t, long o) const {
     const double *P = g.pB + o, *C = g.CB + o, *D = g.DB + o;
     const double *He = g.Heh + o, *Ho = g.Heo + o, *Hh = g.Hho + o;
-    if (e == 0) {
-      switch (t) {
-        case 0: load_dense(s, Hh, N, N, N, true, false, 1.0, m0, k0); break;    // Hhh_o
-        case 1: load_dense(s, P, N, N, N, true, false, 1.0, m0, k0); break;     // p
-        case 2: load_dense(s, He, N, N, N, false, false, -1.0, m0, k0); break;  // -Heh^T
-        case 3: load_dense(s, D, N, N, N, false, false, -1.0, m0, k0); break;   // -D^T
-        case 4: load_band(s, w, 1, m0, k0, true); break;                        // i hbar Dh band
-        default: load_dense(s, P, N, N, N, true, false, 1.0, m0, k0); break;    // p
-      }
-    } else if (e == 1) {
-      switch (t) {
-        case 0: load_dense(s, C, N, N, N, true, false, 1.0, m0, k0); break;     // C
-        case 1: load_dense(s, Ho, N, N, N, false, false, -1.0, m0, k0); break;  // -Hee_o^T
-        case 2: load_dense(s, P, N, N, N, false, true, 1.0, m0, k0); break;     // p^H
-        default: load_dense(s, He, N, N, N, true, true, -1.0, m0, k0); break;   // -conj(Heh)
-      }
-    } else {
-      switch (t) {
-        case 0: load_dense(s, D, N, N, N, true, false, 1.0, m0, k0); break;     // D
-        case 1: load_dense(s, Hh, N, N, N, false, false, -1.0, m0, k0); break;  // -Hhh_o^T
-        case 2: load_dense(s, P, N, N, N, true, true, 1.0, m0, k0); break;      // conj(p)
-        default: load_dense(s, He, N, N, N, false, true, -1.0, m0, k0); break;  // -Heh^H
-      }
+    switch (e * 8 + t) {
+      case 0: return {Hh, -1, true, false, false, 1.0};     // dp: Hhh_o
+      case 1: return {P, -1, true, false, false, 1.0};      //     p
+      case 2: return {He, -1, false, false, false, -1.0};   //     -Heh^T
+      case 3: return {D, -1, false, false, true, -1.0};     //     -D_o^T
+      case 4: return {nullptr, 1, true, false, false, 1.0}; //     i hbar Dh band
+      case 5: return {P, -1, true, false, false, 1.0};      //     p
+      case 8: return {C, -1, true, false, true, 1.0};       // dC: C_o
+      case 9: return {Ho, -1, false, false, false, -1.0};   //     -Hee_o^T
+      case 10: return {P, -1, false, true, false, 1.0};     //     p^H
+      case 11: return {He, -1, true, true, false, -1.0};    //     -conj(Heh)
+      case 16: return {D, -1, true, false, true, 1.0};      // dD: D_o
+      case 17: return {Hh, -1, false, false, false, -1.0};  //     -Hhh_o^T
+      case 18: return {P, -1, true, true, false, 1.0};      //     conj(p)
+      default: return {He, -1, false, true, false, -1.0};   //     -Heh^H
+    }
+  }
+
+  __device__ Op opB(int e, int t, long o) const {
+    const double *P = g.pB + o, *C = g.CB + o, *D = g.DB + o;
+    const double *He = g.Heh + o, *Ho = g.Heo + o, *Hh = g.Hho + o;
+    // op(B)[k][x] = X[x][k] when xm
+    switch (e * 8 + t) {
+      case 0: return {P, -1, false, false, false, 1.0};     // dp: p
+      case 1: return {Ho, -1, true, false, false, 1.0};     //     Hee_o^T
+      case 2: return {C, -1, false, false, true, 1.0};      //     C_o
+      case 3: return {He, -1, true, false, false, 1.0};     //     Heh^T
+      case 4: return {P, -1, false, false, false, 1.0};     //     p
+      case 5: return {nullptr, 0, true, false, false, 1.0}; //     i hbar De band
+      case 8: return {Ho, -1, true, false, false, 1.0};     // dC: Hee_o^T
+      case 9: return {C, -1, false, false, true, 1.0};      //     C_o
+      case 10: return {He, -1, true, false, false, 1.0};    //     Heh^T
+      case 11: return {P, -1, false, false, false, 1.0};    //     p
+      case 16: return {Hh, -1, true, false, false, 1.0};    // dD: Hhh_o^T
+      case 17: return {D, -1, false, false, true, 1.0};     //     D_o
+      case 18: return {He, -1, false, false, false, 1.0};   //     Heh
+      default: return {P, -1, true, false, false, 1.0};     //     p^T
     }
   }
 
   template <typename T>
-  __device__ void loadB(Tile<T> &s, int t, int b, int k0, int n0) const {
+  __device__ void fetchA(Frag<T> &s, int t, int b, int m0, int k0) const {
     const int e = b % 3, w = w0 + b / 3, N = g.N;
-    const long o = mat_off(g, w);
-    const double *P = g.pB + o, *C = g.CB + o, *D = g.DB + o;
-    const double *He = g.Heh + o, *Ho = g.Heo + o, *Hh = g.Hho + o;
-    // B operand: s[k][x] = op(B)(k0+k, n0+x); "xmajor" means op(B)[k][x] = X[x][k]
-    if (e == 0) {
-      switch (t) {
-        case 0: load_dense(s, P, N, N, N, false, false, 1.0, n0, k0); break;   // p
-        case 1: load_dense(s, Ho, N, N, N, true, false, 1.0, n0, k0); break;   // Hee_o^T
-        case 2: load_dense(s, C, N, N, N, false, false, 1.0, n0, k0); break;   // C
-        case 3: load_dense(s, He, N, N, N, true, false, 1.0, n0, k0); break;   // Heh^T
-        case 4: load_dense(s, P, N, N, N, false, false, 1.0, n0, k0); break;   // p
-        default: load_band(s, w, 0, n0, k0, false); break;                     // i hbar De band
-      }
-    } else if (e == 1) {
-      switch (t) {
-        case 0: load_dense(s, Ho, N, N, N, true, false, 1.0, n0, k0); break;   // Hee_o^T
-        case 1: load_dense(s, C, N, N, N, false, false, 1.0, n0, k0); break;   // C
-        case 2: load_dense(s, He, N, N, N, true, false, 1.0, n0, k0); break;   // Heh^T
-        default: load_dense(s, P, N, N, N, false, false, 1.0, n0, k0); break;  // p
-      }
-    } else {
-      switch (t) {
-        case 0: load_dense(s, Hh, N, N, N, true, false, 1.0, n0, k0); break;   // Hhh_o^T
-        case 1: load_dense(s, D, N, N, N, false, false, 1.0, n0, k0); break;   // D
-        case 2: load_dense(s, He, N, N, N, false, false, 1.0, n0, k0); break;  // Heh
-        default: load_dense(s, P, N, N, N, true, false, 1.0, n0, k0); break;   // p^T
-      }
-    }
+    const Op op = opA(e, t, mat_off(g, w));
+    if (op.band < 0)
+      load_dense(s, op.X, N, N, N, op.xm, op.cj, op.sg, m0, k0, op.zd);
+    else
+      load_band(s, w, op.band, m0, k0, true);
+  }
+
+  template <typename T>
+  __device__ void fetchB(Frag<T> &s, int t, int b, int k0, int n0) const {
+    const int e = b % 3, w = w0 + b / 3, N = g.N;
+    const Op op = opB(e, t, mat_off(g, w));
+    if (op.band < 0)
+      load_dense(s, op.X, N, N, N, op.xm, op.cj, op.sg, n0, k0, op.zd);
+    else
+      load_band(s, w, op.band, n0, k0, false);
   }
 
   // Banded off-diagonal dephasing operand, value i*hbar*table.
@@ -224,7 +282,7 @@ struct RhsProb {
   //  kind 0 (De, B side): B[m][ke] = De[m-ke][ke],  x = ke, k = m
   // tables stored [w][kind][k][q + N - 1]; q = 0 excluded (undel(0) = 0).
   template <typename T>
-  __device__ void load_band(Tile<T> &s, int w, int kind, int x0, int k0, bool) const {
+  __device__ void load_band(Frag<T> &s, int w, int kind, int x0, int k0, bool) const {
     const int N = g.N, Q = 2 * N - 1;
     const double *tab = g.dedh + ((long)w * 2 + kind) * (long)N * Q;
     const int tid = threadIdx.x;
@@ -235,8 +293,7 @@ struct RhsProb {
       const int gx = x0 + x, gk = k0 + k;
       double v = 0.0;
       if (gx < N && gk < N && gk != gx) v = tab[(long)gx * Q + (gk - gx + N - 1)];
-      s.v[0][k][x] = (T)0;
-      s.v[1][k][x] = (T)(g.hbar * v);
+      s.set(e, k, x, (T)0, (T)(g.hbar * v));
     }
   }
 
@@ -250,23 +307,33 @@ struct RhsProb {
     cd acc = {re, im};
     cd d;
     const double *hd = g.hdiag + (long)w * 4 * N;  // [2][N] complex
+    const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
     if (e == 0) {
-      // row = kh, col = ke
+      // row = kh, col = ke.  Diagonal parts: (he[ke] + hh[kh]) p and the
+      // Pauli factor Heh[ke,kh] (1 - ne[ke] - nh[kh]) from -Heh^T C - (Heh D)^T.
       const cd p = ld(g.pB + o, ij);
       const cd he = ld(hd, col), hh = ld(hd, N + row);
       const cd heh = ld(g.Heh + o, (long)col * N + row);
+      const cd ne = ld(pr + PSA_PREP_NE, col), nh = ld(pr + PSA_PREP_NH(N), row);
+      const cd pauli = {1.0 - ne.re - nh.re, -ne.im - nh.im};
       cd z = cadd(acc, cmul(cadd(he, hh), p));
-      z = cadd(z, heh);
+      z = cadd(z, cmul(heh, pauli));
       d = cdiv_ih(z, hbar);
-      const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
       const double gam = pr[PSA_PREP_GAME(N) + col] + pr[PSA_PREP_GAMH(N) + row];
       d = csub(d, cscale(p, gam));
     } else {
+      // (h[k2] - h[k1]) X[k1,k2] + (n[k1] - n[k2]) H_o[k2,k1]
       const double *X = (e == 1 ? g.CB : g.DB) + o;
+      const double *Ho = (e == 1 ? g.Heo : g.Hho) + o;
       const cd x = ld(X, ij);
       const int sl = (e == 1) ? 0 : N;
       const cd h2 = ld(hd, sl + col), h1 = ld(hd, sl + row);
-      d = cdiv_ih(cadd(acc, cmul(csub(h2, h1), x)), hbar);
+      const double *nn = pr + (e == 1 ? PSA_PREP_NE : PSA_PREP_NH(N));
+      const cd n1 = ld(nn, row), n2 = ld(nn, col);
+      const cd hot = (row == col) ? cd{0.0, 0.0} : ld(Ho, (long)col * N + row);
+      cd z = cadd(acc, cmul(csub(h2, h1), x));
+      z = cadd(z, cmul(csub(n1, n2), hot));
+      d = cdiv_ih(z, hbar);
     }
     double *out = (e == 0 ? g.pX : (e == 1 ? g.CX : g.DX)) + o;
     if (mode == 0) {
@@ -296,7 +363,7 @@ struct CalcHProb {
   __device__ int kext(int, int) const { return g.N; }
 
   template <typename T>
-  __device__ void loadA(Tile<T> &s, int, int b, int m0, int k0) const {
+  __device__ void fetchA(Frag<T> &s, int, int b, int m0, int k0) const {
     const int which = b % 3, w = w0 + b / 3, N = g.N;
     const double *vc = g.prep + (long)w * PSA_PREP_STRIDE(N) + PSA_PREP_VC(N) + which * N;
     const int tid = threadIdx.x;
@@ -307,13 +374,12 @@ struct CalcHProb {
       const int a = m0 + x, m = k0 + k;
       double v = 0.0;
       if (a < N && m < N && m != a) v = vc[m > a ? m - a : a - m];
-      s.v[0][k][x] = (T)v;
-      s.v[1][k][x] = (T)0;
+      s.set(e, k, x, (T)v, (T)0);
     }
   }
 
   template <typename T>
-  __device__ void loadB(Tile<T> &s, int, int b, int k0, int n0) const {
+  __device__ void fetchB(Frag<T> &s, int, int b, int k0, int n0) const {
     const int which = b % 3, w = w0 + b / 3, N = g.N;
     const long o = mat_off(g, w);
     const double *X = (which == 0 ? g.pB : (which == 1 ? g.CB : g.DB)) + o;
@@ -326,8 +392,7 @@ struct CalcHProb {
       const int c = m + dd - (N - 1);  // column m + d
       cd v = {0.0, 0.0};
       if (m < N && dd < 2 * N - 1 && c >= 0 && c < N) v = ld(X, (long)m * N + c);
-      s.v[0][k][x] = (T)v.re;
-      s.v[1][k][x] = (T)v.im;
+      s.set(e, k, x, (T)v.re, (T)v.im);
     }
   }
 
@@ -406,7 +471,7 @@ struct OffdiagProb {
   __device__ int kext(int, int) const { return 2 * g.N; }
 
   template <typename T>
-  __device__ void loadA(Tile<T> &s, int, int b, int m0, int k0) const {
+  __device__ void fetchA(Frag<T> &s, int, int b, int m0, int k0) const {
     const int N = g.N, q = b / 2 - (N - 1), kind = b % 2;
     const double *Es = kind == 0 ? g.Ee : g.Eh;  // carrier of row k
     const double *Eo = kind == 0 ? g.Eh : g.Ee;  // the other carrier
@@ -438,13 +503,12 @@ struct OffdiagProb {
           }
         }
       }
-      s.v[0][k][x] = (T)v;
-      s.v[1][k][x] = (T)0;
+      s.set(e, k, x, (T)v, (T)0);
     }
   }
 
   template <typename T>
-  __device__ void loadB(Tile<T> &s, int, int b, int k0, int n0) const {
+  __device__ void fetchB(Frag<T> &s, int, int b, int k0, int n0) const {
     const int N = g.N, q = b / 2 - (N - 1), kind = b % 2;
     const int aq = q > 0 ? q : -q;
     const int tid = threadIdx.x;
@@ -457,7 +521,7 @@ struct OffdiagProb {
       if (pp < 2 * N && col < 2 * nw && aq > 0) {
         const int w = w0 + (col >> 1), sA = col & 1;
         const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-        const double *ne = pr + PSA_PREP_NE, *nh = pr + PSA_PREP_NH(N);
+        const double *ne = pr + PSA_PREP_ANE(N), *nh = pr + PSA_PREP_ANH(N);
         const double *vc = pr + PSA_PREP_VC(N);
         // same-carrier occupations: n[p-q] (1-n[p]) | (1-n[p-q]) n[p]
         // other-carrier:            o[p+q] (1-o[p]) | (1-o[p+q]) o[p]
@@ -466,23 +530,22 @@ struct OffdiagProb {
         if (pp < N) {
           const int p = pp, pm = p - q;
           if (pm >= 1 && pm < N) {
-            const double a = hypot(ns[2 * pm], ns[2 * pm + 1]);
-            const double c = hypot(ns[2 * p], ns[2 * p + 1]);
+            const double a = ns[pm];
+            const double c = ns[p];
             const double V = vc[(kind == 0 ? 1 : 2) * N + aq];
             v = V * V * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
           }
         } else {
           const int p = pp - N, pq = p + q;
           if (pq >= 0 && pq < N) {
-            const double a = hypot(no[2 * pq], no[2 * pq + 1]);
-            const double c = hypot(no[2 * p], no[2 * p + 1]);
+            const double a = no[pq];
+            const double c = no[p];
             const double V = vc[aq];
             v = V * V * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
           }
         }
       }
-      s.v[0][k][x] = (T)v;
-      s.v[1][k][x] = (T)0;
+      s.set(e, k, x, (T)v, (T)0);
     }
   }
 
@@ -492,8 +555,7 @@ struct OffdiagProb {
     if ((col & 1) || kr >= N || col >= 2 * nw) return;
     const int w = w0 + (col >> 1);
     const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-    const double *n = pr + (kind == 0 ? PSA_PREP_NE : PSA_PREP_NH(N));
-    const double nk = hypot(n[2 * kr], n[2 * kr + 1]);
+    const double nk = pr[(kind == 0 ? PSA_PREP_ANE(N) : PSA_PREP_ANH(N)) + kr];
     const double D = ((1.0 - nk) * re + nk * other) * PI / g.hbar;
     g.dedh[(((long)w * 2 + kind) * N + kr) * (2 * N - 1) + qi] = D;
   }
@@ -514,12 +576,12 @@ struct Prop2qwProb {
   __device__ int kext(int, int) const { return g.gnx; }
 
   template <typename T>
-  __device__ void loadA(Tile<T> &s, int, int, int m0, int k0) const {
+  __device__ void fetchA(Frag<T> &s, int, int, int m0, int k0) const {
     load_dense(s, g.gmat, g.gnx, 2 * g.N - 1, g.gnx, true, false, 1.0, m0, k0);
   }
 
   template <typename T>
-  __device__ void loadB(Tile<T> &s, int, int, int k0, int n0) const {
+  __device__ void fetchB(Frag<T> &s, int, int, int k0, int n0) const {
     const int tid = threadIdx.x;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -531,8 +593,7 @@ struct Prop2qwProb {
         const int w = w0 + col / 3, f = col % 3;
         v = g.efield[(((long)f * g.Nw + w) * g.Nx + g.gx0 + xi) * 2];
       }
-      s.v[0][k][x] = (T)v;
-      s.v[1][k][x] = (T)0;
+      s.set(e, k, x, (T)v, (T)0);
     }
   }
 

@@ -91,28 +91,29 @@ __global__ __launch_bounds__(256) void k_vint(const double *Qy, int NQ, const do
 }
 
 // ===========================================================================
-// prep: occupations, screening, Hartree diagonal sums, diagonal dephasing,
-// spontaneous emission.  One block per wire.
+// prep (SBEs.py:5173-5226 minus CalcH), in three kernels:
+//  k_prep_a   one block per wire: occupations, screening (CalcScreenedArrays,
+//             coulomb.py:1195-1230; Eps1D 1172-1193)
+//  k_diagsums diagonal sums of C, D for the Hartree terms
+//  k_prep_b   one wave per (wire, k): CalcGammaE/H (dephasing.py:28-103,
+//             Vxx2 362-378) and SpontEmission (emission.py:94-198)
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_prep(psa_group g, int w0) {
+__global__ __launch_bounds__(256) void k_prep_a(psa_group g, int w0) {
   __shared__ double sh[8];
-  __shared__ double s_ner[1024], s_nhr[1024];
   const int w = w0 + blockIdx.x, N = g.N, tid = threadIdx.x;
   const long o = mat_off(g, w);
   const double *C2 = g.CB + o, *D2 = g.DB + o;
   double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-  double *ne = pr + PSA_PREP_NE, *nh = pr + PSA_PREP_NH(N);
   double part = 0.0;
   for (int k = tid; k < N; k += blockDim.x) {
     const cd a = ld(C2, (long)k * N + k), b = ld(D2, (long)k * N + k);
-    st(ne, k, a);
-    st(nh, k, b);
-    s_ner[k] = a.re;
-    s_nhr[k] = b.re;
+    st(pr + PSA_PREP_NE, k, a);
+    st(pr + PSA_PREP_NH(N), k, b);
+    pr[PSA_PREP_ANE(N) + k] = cabs_(a);
+    pr[PSA_PREP_ANH(N) + k] = cabs_(b);
     part += a.re + b.re;
   }
   const double tot = block_sum(part, sh);
-  // CalcScreenedArrays (coulomb.py:1195-1230), Eps1D (1172-1193)
   const bool screened = (g.flags & PSA_F_SCREENED) != 0;
   double n1 = tot / 2.0 / g.L;
   n1 = fmin(n1, g.n1d_max);
@@ -127,89 +128,119 @@ __global__ __launch_bounds__(256) void k_prep(psa_group g, int w0) {
     E1D[d] = e;
     for (int c = 0; c < 3; ++c) vc[c * N + d] = screened ? g.vtab[c * N + d] / e : g.vtab[c * N + d];
   }
-  // diagonal sums dsC[s] = sum_k C[k][k+s], dsD likewise, s = -(N-1)..N-1
-  double *dsC = pr + PSA_PREP_DSC(N), *dsD = pr + PSA_PREP_DSD(N);
-  for (int si = tid; si < 2 * N - 1; si += blockDim.x) {
-    const int s = si - (N - 1);
-    const int k0 = s < 0 ? -s : 0, k1 = s < 0 ? N : N - s;
+}
+
+// mode 0: dsC[s] = sum_k C2[k][k+s], dsD likewise (prep);
+// mode 1: sums of the new level for the sources: dcv sum_kh (-1)^kh p[kh][kh+s],
+//         dcv sum p[kh][kh+s], sum C3[k][k+s], sum conj(D3[k][k+s]).
+__global__ __launch_bounds__(256) void k_diagsums(psa_group g, int w0, int mode) {
+  const int w = w0 + blockIdx.x, N = g.N, Q = 2 * N - 1;
+  const int si = blockIdx.y * blockDim.x + threadIdx.x;
+  if (si >= Q) return;
+  const int s = si - (N - 1);
+  const int k0 = s < 0 ? -s : 0, k1 = s < 0 ? N : N - s;
+  const long o = mat_off(g, w);
+  if (mode == 0) {
+    const double *C2 = g.CB + o, *D2 = g.DB + o;
     cd ac = {0, 0}, ad = {0, 0};
     for (int k = k0; k < k1; ++k) {
       ac = cadd(ac, ld(C2, (long)k * N + k + s));
       ad = cadd(ad, ld(D2, (long)k * N + k + s));
     }
-    st(dsC, si, ac);
-    st(dsD, si, ad);
+    double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+    st(pr + PSA_PREP_DSC(N), si, ac);
+    st(pr + PSA_PREP_DSD(N), si, ad);
+  } else {
+    const double *p3 = g.pX + o, *C3 = g.CX + o, *D3 = g.DX + o;
+    cd ax = {0, 0}, ay = {0, 0}, ac = {0, 0}, ad = {0, 0};
+    for (int k = k0; k < k1; ++k) {
+      const cd pv = ld(p3, (long)k * N + k + s);
+      ay = cadd(ay, pv);
+      ax = (k & 1) ? csub(ax, pv) : cadd(ax, pv);
+      ac = cadd(ac, ld(C3, (long)k * N + k + s));
+      ad = cadd(ad, cconj(ld(D3, (long)k * N + k + s)));
+    }
+    double *ds = g.src + (long)w * PSA_SRC_STRIDE(N, g.Nr) + PSA_SRC_DS(g.Nr);
+    st(ds, si, cscale(ax, g.dcv));
+    st(ds, Q + si, cscale(ay, g.dcv));
+    st(ds, 2 * Q + si, ac);
+    st(ds, 3 * Q + si, ad);
   }
-  __syncthreads();
-  // CalcGammaE/H (dephasing.py:28-103) with Vxx2 (362-378)
-  double *GE = pr + PSA_PREP_GAME(N), *GH = pr + PSA_PREP_GAMH(N);
-  const bool diag = (g.flags & PSA_F_DIAGDPH) != 0;
-  const long NN = (long)N * N;
-  const int *kpq = g.didx, *kmq = g.didx + NN, *k1pmq = g.didx + 2 * NN, *k1p = g.didx + 3 * NN,
-            *k1mq = g.didx + 4 * NN, *k1 = g.didx + 5 * NN;  // stored [q][k]
-  const double ph = PI / g.hbar;
-  for (int k = tid; k < N; k += blockDim.x) {
-    if (!diag) {
+}
+
+__global__ __launch_bounds__(256) void k_prep_b(psa_group g, int w0) {
+  const int w = w0 + blockIdx.x, N = g.N, lane = threadIdx.x & 63;
+  const int k = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (k >= N) return;  // whole wave; only wave-level reductions below
+  double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+  const double *nev = pr + PSA_PREP_NE, *nhv = pr + PSA_PREP_NH(N);
+  const double *vc = pr + PSA_PREP_VC(N);
+  double *GE = pr + PSA_PREP_GAME(N), *GH = pr + PSA_PREP_GAMH(N), *Rsp = pr + PSA_PREP_RSP(N);
+  if (!(g.flags & PSA_F_DIAGDPH)) {
+    if (lane == 0) {
       GE[k] = g.gam_e;
       GH[k] = g.gam_h;
-      continue;
     }
+  } else {
+    const long NN = (long)N * N;
+    const int *kpq = g.didx, *kmq = g.didx + NN, *k1pmq = g.didx + 2 * NN, *k1p = g.didx + 3 * NN,
+              *k1mq = g.didx + 4 * NN, *k1 = g.didx + 5 * NN;  // stored [q][k]
+    const double ph = PI / g.hbar;
     double ge = 0.0, gh = 0.0;
-    for (int q = 0; q < N; ++q) {
-      const double V = vc[N + g.iq[q]];
-      const int i = kpq[(long)q * N + k];
-      if (i >= 0 && i < N) ge = ge + ph * (V * V) * s_ner[i] * (1.0 - s_ner[i]) * fabs(g.xe[q]);
+    for (int q = lane; q < N; q += 64) {
+      const long qk = (long)q * N + k;
+      const int iq = g.iq[q];
+      const double Vee = vc[N + iq], Veh = vc[iq], Vhh = vc[2 * N + iq];
+      const int i = kpq[qk];
+      if (i >= 0 && i < N) {
+        const double ni = nev[2 * i], hi = nhv[2 * i];
+        ge += ph * (Vee * Vee) * ni * (1.0 - ni) * fabs(g.xe[q]);
+        gh += ph * (Vhh * Vhh) * hi * (1.0 - hi) * fabs(g.xh[q]);
+      }
+      const int c = kmq[qk];
+      int a = k1pmq[qk], b = k1p[qk];
+      if (a >= 0 && a < N && b >= 0 && b < N && c >= 0 && c < N) {
+        const double na = nhv[2 * a], nb = nhv[2 * b], ec = nev[2 * c];
+        ge += ph * (Veh * Veh) * (na * (1.0 - nb) * ec + (1.0 - na) * nb * (1.0 - ec)) * fabs(g.xh[q]);
+      }
+      a = k1mq[qk];
+      b = k1[qk];
+      if (a >= 0 && a < N && b >= 0 && b < N && c >= 0 && c < N) {
+        const double ea = nev[2 * a], eb = nev[2 * b], hc = nhv[2 * c];
+        gh += ph * (Veh * Veh) * (ea * (1.0 - eb) * hc + (1.0 - ea) * eb * (1.0 - hc)) * fabs(g.xe[q]);
+      }
     }
-    for (int q = 0; q < N; ++q) {
-      const double V = vc[g.iq[q]];
-      const int a = k1pmq[(long)q * N + k], b = k1p[(long)q * N + k], c = kmq[(long)q * N + k];
-      if (a >= 0 && a < N && b >= 0 && b < N && c >= 0 && c < N)
-        ge = ge + ph * (V * V) *
-                      (s_nhr[a] * (1.0 - s_nhr[b]) * s_ner[c] + (1.0 - s_nhr[a]) * s_nhr[b] * (1.0 - s_ner[c])) *
-                      fabs(g.xh[q]);
+    ge = wave_sum(ge);
+    gh = wave_sum(gh);
+    if (lane == 0) {
+      GE[k] = ge;
+      GH[k] = gh;
     }
-    for (int q = 0; q < N; ++q) {
-      const double V = vc[2 * N + g.iq[q]];
-      const int i = kpq[(long)q * N + k];
-      if (i >= 0 && i < N) gh = gh + ph * (V * V) * s_nhr[i] * (1.0 - s_nhr[i]) * fabs(g.xh[q]);
-    }
-    for (int q = 0; q < N; ++q) {
-      const double V = vc[g.iq[q]];
-      const int a = k1mq[(long)q * N + k], b = k1[(long)q * N + k], c = kmq[(long)q * N + k];
-      if (a >= 0 && a < N && b >= 0 && b < N && c >= 0 && c < N)
-        gh = gh + ph * (V * V) *
-                      (s_ner[a] * (1.0 - s_ner[b]) * s_nhr[c] + (1.0 - s_ner[a]) * s_ner[b] * (1.0 - s_nhr[c])) *
-                      fabs(g.xe[q]);
-    }
-    GE[k] = ge;
-    GH[k] = gh;
   }
-  // SpontEmission (emission.py:94-198): Ec with the lower-triangular VC
-  double *Rsp = pr + PSA_PREP_RSP(N);
-  const bool recomb = (g.flags & PSA_F_RECOMB) != 0;
+  if (!(g.flags & PSA_F_RECOMB)) {
+    if (lane == 0) Rsp[k] = 0.0;
+    return;
+  }
+  // Ec (emission.py:133-171) with the lower-triangular VC
+  const double veh0 = vc[0], vee0 = vc[N], vhh0 = vc[2 * N];
+  double s1 = 0.0;
+  for (int i = lane; i < N; i += 64) {
+    const double ne = nev[2 * i], nh = nhv[2 * i];
+    const double vee_ik = i >= k ? vc[N + i - k] : 0.0;
+    const double vhh_ik = i >= k ? vc[2 * N + i - k] : 0.0;
+    s1 += ne * (vee0 - vee_ik) + nh * (vhh0 - vhh_ik);
+    if (i != k) s1 += ne * veh0 - nh * veh0;
+  }
+  s1 = wave_sum(s1);
+  const double Ek = g.gap + g.Ee[k] + g.Eh[k] + (s1 - veh0);
   const double rho_c = 1.0 / (C0_LIGHT * C0_LIGHT * C0_LIGHT * PI * PI * g.hbar * g.hbar * g.hbar);
-  for (int k = tid; k < N; k += blockDim.x) {
-    if (!recomb) {
-      Rsp[k] = 0.0;
-      continue;
-    }
-    const double veh0 = vc[0], vee0 = vc[N], vhh0 = vc[2 * N];
-    double s1 = 0.0, s2 = 0.0;
-    for (int i = 0; i < N; ++i) {
-      const double vee_ik = i >= k ? vc[N + i - k] : 0.0;
-      const double vhh_ik = i >= k ? vc[2 * N + i - k] : 0.0;
-      s1 += s_ner[i] * (vee0 - vee_ik) + s_nhr[i] * (vhh0 - vhh_ik);
-      if (i != k) s2 += s_ner[i] * veh0 - s_nhr[i] * veh0;
-    }
-    const double Ec = s1 + s2 - veh0;
-    const double Ek = g.gap + g.Ee[k] + g.Eh[k] + Ec;
-    double acc = 0.0;
-    for (int n = 0; n < g.Nhw; ++n) {
-      const double hw = g.homega[n] + Ek;
-      acc += hw * (hw * hw * rho_c) * g.hsq[n];
-    }
-    Rsp[k] = acc * g.dhw;
+  double acc = 0.0;
+  for (int n = lane; n < g.Nhw; n += 64) {
+    const double hw = g.homega[n] + Ek;
+    acc += hw * (hw * hw * rho_c) * g.hsq[n];
   }
+  acc = wave_sum(acc);
+  if (lane == 0) Rsp[k] = acc * g.dhw;
 }
 
 // Hamiltonians without excitons (SBEs.py:826-835): Heh = Meh, diagonal energies
@@ -395,49 +426,56 @@ __global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
 // ===========================================================================
 // Transport (dcfield.py:1176-1236): X <- ifft2(fft2(X) exp(-i(Y_a+Y_b) dk)).
 // The phase is separable, so it is a row pass and a column pass of
-// FFT -> phase -> inverse FFT, each in LDS (Stockham radix 2/3, N = 2^a 3^b).
-// One wave per line, four lines per block.
+// FFT -> phase -> inverse FFT, each in LDS (Stockham radix 2/3,
+// N = 2^a 3^b, twiddles from a per-N table).  One wave per line, four lines
+// per block, cooperative (coalesced) loads and stores of the four lines.
 // ===========================================================================
-__device__ void fft_line(cd *x, cd *y, int N, int sign, int lane) {
-  // returns with the result in x
+__device__ __forceinline__ cd tw_get(const cd *tw, int idx, int sign) {
+  cd w = tw[idx];  // exp(-2 pi i idx / N)
+  if (sign > 0) w.im = -w.im;
+  return w;
+}
+
+// Stockham autosort FFT of one line held in LDS (x, ping-pong buffer y),
+// radix 2 and 3 stages, twiddles from the LDS table tw; result left in x.
+__device__ void fft_line(cd *x, cd *y, const cd *tw, int N, int sign, int lane) {
   int Ns = 1;
   cd *src = x, *dst = y;
   int rem = N;
   while (rem > 1) {
-    const int R = (rem & 1) ? 3 : 2;
-    const int M = N / R;
-    for (int j = lane; j < M; j += 64) {
-      const int k = j % Ns;
-      const double ang = (double)sign * 2.0 * (double)k / (double)(Ns * R);  // in units of pi
-      cd v[3];
-      for (int t = 0; t < R; ++t) {
-        cd a = src[j + t * M];
-        if (t > 0) {
-          double s, c;
-          sincospi(ang * t, &s, &c);
-          a = cmul(a, cd{c, s});
-        }
-        v[t] = a;
+    if ((rem & 1) == 0) {
+      const int M = N >> 1, step = N / (Ns * 2);
+      for (int j = lane; j < M; j += 64) {
+        const int k = j % Ns;
+        const cd a = src[j];
+        const cd b = cmul(src[j + M], tw_get(tw, k * step, sign));
+        const int base = (j - k) * 2 + k;
+        dst[base] = cadd(a, b);
+        dst[base + Ns] = csub(a, b);
       }
-      const int base = (j / Ns) * Ns * R + k;
-      if (R == 2) {
-        dst[base] = cadd(v[0], v[1]);
-        dst[base + Ns] = csub(v[0], v[1]);
-      } else {
-        // w = exp(sign * 2 pi i / 3)
-        const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
-        const cd w1 = {c3, s3}, w2 = {c3, -s3};
-        dst[base] = cadd(v[0], cadd(v[1], v[2]));
-        dst[base + Ns] = cadd(v[0], cadd(cmul(v[1], w1), cmul(v[2], w2)));
-        dst[base + 2 * Ns] = cadd(v[0], cadd(cmul(v[1], w2), cmul(v[2], w1)));
+      Ns *= 2;
+      rem >>= 1;
+    } else {
+      const int M = N / 3, step = N / (Ns * 3);
+      const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
+      const cd w1 = {c3, s3}, w2 = {c3, -s3};
+      for (int j = lane; j < M; j += 64) {
+        const int k = j % Ns;
+        const cd a = src[j];
+        const cd b = cmul(src[j + M], tw_get(tw, k * step, sign));
+        const cd c = cmul(src[j + 2 * M], tw_get(tw, 2 * k * step, sign));
+        const int base = (j - k) * 3 + k;
+        dst[base] = cadd(a, cadd(b, c));
+        dst[base + Ns] = cadd(a, cadd(cmul(b, w1), cmul(c, w2)));
+        dst[base + 2 * Ns] = cadd(a, cadd(cmul(b, w2), cmul(c, w1)));
       }
+      Ns *= 3;
+      rem /= 3;
     }
     __syncthreads();
     cd *tmp = src;
     src = dst;
     dst = tmp;
-    Ns *= R;
-    rem /= R;
   }
   if (src != x) {
     for (int j = lane; j < N; j += 64) x[j] = src[j];
@@ -445,42 +483,69 @@ __device__ void fft_line(cd *x, cd *y, int N, int sign, int lane) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_transport(psa_group g, int w0, double dt, int pass) {
+// per-wire phase exp(-i Y_j dk), dk = -e (Edc + Re Ex[q=0]) dt / hbar
+__global__ void k_tphase(psa_group g, int w0, double dt) {
+  const int w = w0 + blockIdx.x, N = g.N;
+  const double *eq = g.exq + (long)w * 3 * 2 * (2 * N - 1);
+  const double eac = eq[2 * (N - 1)];
+  const double dk = -g.e0 * (g.Edc + eac) / g.hbar * dt;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    double s, c;
+    sincos(-g.ytr[j] * dk, &s, &c);
+    st(g.tphase + (long)w * 2 * N, j, cd{c, s});
+  }
+}
+
+__global__ __launch_bounds__(256) void k_transport(psa_group g, int w0, int pass) {
   extern __shared__ double smem[];
-  const int N = g.N, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int lines_per_block = 4;
+  const int N = g.N, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int mat = blockIdx.y;  // 0 C, 1 D, 2 p
   const int w = w0 + blockIdx.z;
-  const int line = blockIdx.x * lines_per_block + wv;
-  cd *x = reinterpret_cast<cd *>(smem) + (long)wv * 2 * N;
-  cd *y = x + N;
+  const int line0 = blockIdx.x * 4;
+  cd *base = reinterpret_cast<cd *>(smem);
+  cd *tw = base + 4 * 2 * N;  // [N] twiddles
+  for (int j = tid; j < N; j += blockDim.x) tw[j] = ld(g.twid, j);
   const long o = mat_off(g, w);
   double *X = (mat == 0 ? g.CX : (mat == 1 ? g.DX : g.pX)) + o;
-  const double *eq = g.exq + (long)w * 3 * 2 * (2 * N - 1);
-  const double eac = eq[2 * (N - 1)];  // Re Ex at q = 0
-  const double dk = -g.e0 * (g.Edc + eac) / g.hbar * dt;
-  const bool valid = line < N;
-  if (valid)
-    for (int j = lane; j < N; j += 64)
-      x[j] = pass == 0 ? ld(X, (long)line * N + j) : ld(X, (long)j * N + line);
-  __syncthreads();
-  fft_line(x, y, N, -1, lane);
-  if (valid)
-    for (int j = lane; j < N; j += 64) {
-      double s, c;
-      sincos(-g.ytr[j] * dk, &s, &c);
-      x[j] = cmul(x[j], cd{c, s});
+  // cooperative load of 4 lines
+  for (int idx = tid; idx < 4 * N; idx += blockDim.x) {
+    int c, j;
+    if (pass == 0) {
+      c = idx / N;
+      j = idx % N;
+    } else {
+      c = idx & 3;
+      j = idx >> 2;
     }
+    const int line = line0 + c;
+    if (line < N) base[c * 2 * N + j] = pass == 0 ? ld(X, (long)line * N + j) : ld(X, (long)j * N + line);
+  }
   __syncthreads();
-  fft_line(x, y, N, +1, lane);
-  if (valid)
-    for (int j = lane; j < N; j += 64) {
-      const cd v = cscale(x[j], 1.0 / N);
+  cd *x = base + wv * 2 * N, *y = x + N;
+  fft_line(x, y, tw, N, -1, lane);
+  const double *ph = g.tphase + (long)w * 2 * N;
+  for (int j = lane; j < N; j += 64) x[j] = cmul(x[j], ld(ph, j));
+  __syncthreads();
+  fft_line(x, y, tw, N, +1, lane);
+  const double inv = 1.0 / N;
+  for (int idx = tid; idx < 4 * N; idx += blockDim.x) {
+    int c, j;
+    if (pass == 0) {
+      c = idx / N;
+      j = idx % N;
+    } else {
+      c = idx & 3;
+      j = idx >> 2;
+    }
+    const int line = line0 + c;
+    if (line < N) {
+      const cd v = cscale(base[c * 2 * N + j], inv);
       if (pass == 0)
         st(X, (long)line * N + j, v);
       else
         st(X, (long)j * N + line, v);
     }
+  }
 }
 
 // ===========================================================================
@@ -522,111 +587,82 @@ __global__ __launch_bounds__(256) void k_average(psa_group g, int w0, int nw) {
 }
 
 // ===========================================================================
-// sources: QWPolarization3 + QWRho5 in real space via diagonal sums,
-// the 2 e0 ehint / area scaling, QW2Prop's LF renormalisation, CalcI0.
-// One block per wire.  src layout per wire: [5][Nr] then 4 (2N-1) complex
-// diagonal sums, then sums.
+// sources: QWPolarization3 (qwoptics.py:809-940) and QWRho5 (942-1071) in
+// real space.  Both are sums over (k, k') of a matrix element times
+// exp(i (k'-k) r); with kr on a uniform grid that is sum_d S_d exp(i d dkr r)
+// over the diagonal sums S_d (k_diagsums mode 1), evaluated per r from the
+// static table phi[d][r].  k_src_norm then applies the boundary-mean
+// subtraction, the particle-number normalisation, the 2 e0 ehint / area
+// scaling (SBEs.py:5749-5750) and QW2Prop's |.| / common-total step
+// (qwoptics.py:785-790), and evaluates CalcI0 (dcfield.py:1110-1152).
 // ===========================================================================
-#define SRC_STRIDE(N, Nr) PSA_SRC_STRIDE(N, Nr)
-
-__global__ __launch_bounds__(256) void k_sources(psa_group g, int w0) {
-  __shared__ double sh[8];
-  const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, tid = threadIdx.x;
-  const int Q = 2 * N - 1;
-  const long o = mat_off(g, w);
-  const double *p3 = g.pX + o, *C3 = g.CX + o, *D3 = g.DX + o;
-  double *src = g.src + (long)w * SRC_STRIDE(N, Nr);
-  double *SX = src + 5 * Nr, *SY = SX + 2 * Q, *TC = SY + 2 * Q, *TD = TC + 2 * Q;
+__global__ __launch_bounds__(256) void k_src_r(psa_group g, int w0) {
+  const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, Q = 2 * N - 1;
+  const int r = blockIdx.y * blockDim.x + threadIdx.x;
+  if (r >= Nr) return;
+  double *src = g.src + (long)w * PSA_SRC_STRIDE(N, Nr);
+  const double *ds = src + PSA_SRC_DS(Nr);
+  cd px = {0, 0}, py = {0, 0}, rc = {0, 0}, rd = {0, 0};
+  for (int si = 0; si < Q; ++si) {
+    const cd ph = ld(g.phi, (long)si * Nr + r);
+    px = cadd(px, cmul(ld(ds, si), ph));
+    py = cadd(py, cmul(ld(ds, Q + si), ph));
+    rc = cadd(rc, cmul(ld(ds, 2 * Q + si), ph));
+    rd = cadd(rd, cmul(ld(ds, 3 * Q + si), ph));
+  }
   const int wid = g.wire_id[w];
-  for (int si = tid; si < Q; si += blockDim.x) {
-    const int s = si - (N - 1);
-    const int k0 = s < 0 ? -s : 0, k1 = s < 0 ? N : N - s;
-    cd ax = {0, 0}, ay = {0, 0}, ac = {0, 0}, ad = {0, 0};
-    for (int k = k0; k < k1; ++k) {
-      const cd pv = ld(p3, (long)k * N + k + s);
-      ay = cadd(ay, pv);
-      ax = (k & 1) ? csub(ax, pv) : cadd(ax, pv);
-      ac = cadd(ac, ld(C3, (long)k * N + k + s));
-      ad = cadd(ad, cconj(ld(D3, (long)k * N + k + s)));
-    }
-    st(SX, si, cscale(ax, g.dcv));
-    st(SY, si, cscale(ay, g.dcv));
-    st(TC, si, ac);
-    st(TD, si, ad);
-  }
-  __syncthreads();
-  const double *rq = g.rq, *W = g.window;
+  const double W = g.window[r];
   const double sP = 2.0 * g.ehint / g.area / g.L;
-  const double yv = yw_of(wid), zs = sgn_pow(wid);
-  double *Px = src, *Py = src + Nr, *Pz = src + 2 * Nr, *re_ = src + 3 * Nr, *rh_ = src + 4 * Nr;
-  // real-space sums; complex re/rh kept as (re, im) in registers then stored
-  extern __shared__ double dyn[];  // [2][Nr] imaginary parts of re, rh
-  double *re_im = dyn, *rh_im = dyn + Nr;
-  for (int r = tid; r < Nr; r += blockDim.x) {
-    cd px = {0, 0}, py = {0, 0}, rc = {0, 0}, rd = {0, 0};
-    const double rr = rq[r];
-    for (int si = 0; si < Q; ++si) {
-      const int d = si - (N - 1);
-      double s, c;
-      sincos(((double)d * g.dkr) * rr, &s, &c);
-      const cd ph = {c, s};
-      px = cadd(px, cmul(ld(SX, si), ph));
-      py = cadd(py, cmul(ld(SY, si), ph));
-      rc = cadd(rc, cmul(ld(TC, si), ph));
-      rd = cadd(rd, cmul(ld(TD, si), ph));
+  const bool optics = (g.flags & PSA_F_OPTICS) != 0;
+  src[r] = optics ? px.re * W * sP : 0.0;
+  src[Nr + r] = optics ? py.re * W * yw_of(wid) * sP : 0.0;
+  src[2 * Nr + r] = optics ? -py.re * W * sgn_pow(wid) * sP : 0.0;
+  const double f = (g.flags & PSA_F_LF) ? W / (2.0 * g.L) : 0.0;
+  st(src + PSA_SRC_RC(Nr), r, cscale(rc, f));
+  st(src + PSA_SRC_RC(Nr), Nr + r, cscale(rd, f));
+}
+
+__global__ __launch_bounds__(256) void k_src_norm(psa_group g, int w0) {
+  __shared__ double sh[8];
+  __shared__ double s_ne2[1024], s_nh2[1024], ecE[1024], ecH[1024];
+  const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, tid = threadIdx.x;
+  const long o = mat_off(g, w);
+  double *src = g.src + (long)w * PSA_SRC_STRIDE(N, Nr);
+  cd *rcv = reinterpret_cast<cd *>(src + PSA_SRC_RC(Nr));  // [2][Nr]
+  double *re_ = src + 3 * Nr, *rh_ = src + 4 * Nr;
+  // boundary mean of the real parts (qwoptics.py:1062-1064)
+  double bpe = 0, bph = 0;
+  for (int r = tid; r < Nr; r += blockDim.x)
+  {
+    if (r < 10) {
+      bpe += rcv[r].re;
+      bph += rcv[Nr + r].re;
     }
-    const bool optics = (g.flags & PSA_F_OPTICS) != 0;
-    Px[r] = optics ? px.re * W[r] * sP : 0.0;
-    Py[r] = optics ? py.re * W[r] * yv * sP : 0.0;
-    Pz[r] = optics ? -py.re * W[r] * zs * sP : 0.0;
-    const double f = (g.flags & PSA_F_LF) ? W[r] / (2.0 * g.L) : 0.0;
-    re_[r] = rc.re * f;
-    re_im[r] = rc.im * f;
-    rh_[r] = rd.re * f;
-    rh_im[r] = rd.im * f;
+    if (r >= Nr - 10) {
+      bpe += rcv[r].re;
+      bph += rcv[Nr + r].re;
+    }
   }
-  __syncthreads();
-  // boundary mean (qwoptics.py:1062-1064)
-  double be = 0, bh = 0;
-  if (tid == 0) {
-    for (int r = 0; r < 10; ++r) {
-      be += re_[r];
-      bh += rh_[r];
-    }
-    for (int r = Nr - 10; r < Nr; ++r) {
-      be += re_[r];
-      bh += rh_[r];
-    }
-    sh[4] = be / 20.0;
-    sh[5] = bh / 20.0;
-  }
-  __syncthreads();
-  be = sh[4];
-  bh = sh[5];
-  __syncthreads();
-  // totals of |ne3|, |nh3| (already real after renormalisation)
+  const double be = block_sum(bpe, sh) / 20.0, bh = block_sum(bph, sh) / 20.0;
   double ne_t = 0, nh_t = 0;
   for (int k = tid; k < N; k += blockDim.x) {
-    ne_t += cabs_(ld(C3, (long)k * N + k));
-    nh_t += cabs_(ld(D3, (long)k * N + k));
+    ne_t += cabs_(ld(g.CX + o, (long)k * N + k));
+    nh_t += cabs_(ld(g.DX + o, (long)k * N + k));
   }
   const double NeT = block_sum(ne_t, sh) + 1e-50, NhT = block_sum(nh_t, sh) + 1e-50;
-  const double dr = rq[1] - rq[0];
+  const double dr = g.rq[1] - g.rq[0];
   double ae = 0, ah = 0;
   for (int r = tid; r < Nr; r += blockDim.x) {
-    re_[r] -= be;
-    rh_[r] -= bh;
-    ae += hypot(re_[r], re_im[r]);
-    ah += hypot(rh_[r], rh_im[r]);
+    ae += hypot(rcv[r].re - be, rcv[r].im);
+    ah += hypot(rcv[Nr + r].re - bh, rcv[Nr + r].im);
   }
   const double Ae = block_sum(ae, sh), Ah = block_sum(ah, sh);
   const double fe = NeT / (Ae * dr + 1e-100), fh = NhT / (Ah * dr + 1e-100);
   const double sc = 2.0 * g.e0 / g.area * g.ehint;
-  // after the 2 e0 ehint / area scaling: |.| and the common-total renormalisation
   double s1 = 0, s2 = 0;
   for (int r = tid; r < Nr; r += blockDim.x) {
-    const double a = hypot(re_[r] * fe * sc, re_im[r] * fe * sc);
-    const double b = hypot(rh_[r] * fh * sc, rh_im[r] * fh * sc);
+    const double a = hypot((rcv[r].re - be) * fe * sc, rcv[r].im * fe * sc);
+    const double b = hypot((rcv[Nr + r].re - bh) * fh * sc, rcv[Nr + r].im * fh * sc);
     re_[r] = a;
     rh_[r] = b;
     s1 += a;
@@ -638,20 +674,16 @@ __global__ __launch_bounds__(256) void k_sources(psa_group g, int w0) {
     re_[r] = re_[r] * total / (S1 * dr + 1e-100);
     rh_[r] = rh_[r] * total / (S2 * dr + 1e-100);
   }
-  // CalcI0 (dcfield.py:1110-1152) on ne2, nh2 = |diag of the averaged level|
-  __syncthreads();
-  const double *CA = g.CA + o, *DA = g.DA + o;
+  // CalcI0 on ne2, nh2 = |diag of the averaged level|
   const double *vc = g.prep + (long)w * PSA_PREP_STRIDE(N) + PSA_PREP_VC(N);
-  double *ecE = dyn, *ecH = dyn + N;  // reuse (Nr >= 2N)
-  __shared__ double s_ne2[1024], s_nh2[1024];
   for (int k = tid; k < N; k += blockDim.x) {
-    s_ne2[k] = cabs_(ld(CA, (long)k * N + k));
-    s_nh2[k] = cabs_(ld(DA, (long)k * N + k));
+    s_ne2[k] = cabs_(ld(g.CA + o, (long)k * N + k));
+    s_nh2[k] = cabs_(ld(g.DA + o, (long)k * N + k));
   }
   __syncthreads();
   for (int k = tid; k < N; k += blockDim.x) {
     double se = 0, shh = 0;
-    for (int i = 0; i <= k; ++i) {  // V[k][i] nonzero for i <= k
+    for (int i = 0; i <= k; ++i) {  // V[k][i] nonzero for i <= k (EkReNorm, dcfield.py:338-345)
       se += s_ne2[i] * (vc[N] - vc[N + k - i]);
       shh += s_nh2[i] * (vc[2 * N] - vc[2 * N + k - i]);
     }
@@ -664,17 +696,18 @@ __global__ __launch_bounds__(256) void k_sources(psa_group g, int w0) {
   }
   __syncthreads();
   double pe = 0, phh = 0, sne = 0, snh = 0;
-  for (int i = tid; i < N; i += blockDim.x) {
+  const double dk = g.dkr;
+  for (int i = tid; i < N; i += blockDim.x) {  // DriftVt (dcfield.py:347-360)
     double de, dh;
     if (i >= 1 && i <= N - 2) {
-      de = (ecE[i + 1] - ecE[i - 1]) / 2.0 / g.dkr;
-      dh = (ecH[i + 1] - ecH[i - 1]) / 2.0 / g.dkr;
+      de = (ecE[i + 1] - ecE[i - 1]) / 2.0 / dk;
+      dh = (ecH[i + 1] - ecH[i - 1]) / 2.0 / dk;
     } else if (i == 0) {
-      de = 2.0 * ((ecE[2] - ecE[0]) / 2.0 / g.dkr) - (ecE[3] - ecE[1]) / 2.0 / g.dkr;
-      dh = 2.0 * ((ecH[2] - ecH[0]) / 2.0 / g.dkr) - (ecH[3] - ecH[1]) / 2.0 / g.dkr;
+      de = 2.0 * ((ecE[2] - ecE[0]) / 2.0 / dk) - (ecE[3] - ecE[1]) / 2.0 / dk;
+      dh = 2.0 * ((ecH[2] - ecH[0]) / 2.0 / dk) - (ecH[3] - ecH[1]) / 2.0 / dk;
     } else {
-      de = 2.0 * ((ecE[N - 1] - ecE[N - 3]) / 2.0 / g.dkr) - (ecE[N - 2] - ecE[N - 4]) / 2.0 / g.dkr;
-      dh = 2.0 * ((ecH[N - 1] - ecH[N - 3]) / 2.0 / g.dkr) - (ecH[N - 2] - ecH[N - 4]) / 2.0 / g.dkr;
+      de = 2.0 * ((ecE[N - 1] - ecE[N - 3]) / 2.0 / dk) - (ecE[N - 2] - ecE[N - 4]) / 2.0 / dk;
+      dh = 2.0 * ((ecH[N - 1] - ecH[N - 3]) / 2.0 / dk) - (ecH[N - 2] - ecH[N - 4]) / 2.0 / dk;
     }
     pe += de * s_ne2[i];
     phh += dh * s_nh2[i];
@@ -694,49 +727,62 @@ __global__ __launch_bounds__(256) void k_sources(psa_group g, int w0) {
 }
 
 // ===========================================================================
-// QW2Prop splines r -> rr (qwoptics.py:792-805) and Rho = RhoH - RhoE.
-// One block per wire.
+// QW2Prop splines r -> rr (qwoptics.py:792-805) and Rho = RhoH - RhoE
+// (SBEs.py:5917): k_qw2prop_a applies the spline matrix (transposed,
+// coalesced) and stores per-block partial sums of |RhoE|, |RhoH|;
+// k_qw2prop_b renormalises to the common total.
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_qw2prop(psa_group g, int w0) {
+__global__ __launch_bounds__(256) void k_qw2prop_a(psa_group g, int w0) {
   __shared__ double sh[8];
-  const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, Nx = g.Nx, tid = threadIdx.x;
-  const double *src = g.src + (long)w * SRC_STRIDE(N, Nr);
+  const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, Nx = g.Nx;
+  const int x = blockIdx.y * blockDim.x + threadIdx.x;
+  const double *src = g.src + (long)w * PSA_SRC_STRIDE(N, Nr);
   const long NW = g.Nw;
-  double *Pxx = g.out + ((0 * NW + w) * (long)Nx) * 2;
-  double *Pyy = g.out + ((1 * NW + w) * (long)Nx) * 2;
-  double *Pzz = g.out + ((2 * NW + w) * (long)Nx) * 2;
-  double *Rho = g.out + ((3 * NW + w) * (long)Nx) * 2;
   const bool lf = (g.flags & PSA_F_LF) != 0;
-  double se = 0, shh = 0;
-  for (int x = tid; x < Nx; x += blockDim.x) {
+  double a = 0.0, b2 = 0.0;
+  if (x < Nx) {
     const int b = x - g.bx0;
     double v[5] = {0, 0, 0, 0, 0};
     if (b >= 0 && b < g.bnx) {
-      const double *m = g.m2 + (long)b * Nr;
       for (int r = 0; r < Nr; ++r) {
-        const double c = m[r];
+        const double c = g.m2[(long)r * g.bnx + b];
 #pragma unroll
         for (int f = 0; f < 5; ++f) v[f] += c * src[f * Nr + r];
       }
     }
-    st(Pxx, x, cd{v[0], 0});
-    st(Pyy, x, cd{v[1], 0});
-    st(Pzz, x, cd{v[2], 0});
-    const double a = lf ? fabs(v[3]) : v[3], b2 = lf ? fabs(v[4]) : v[4];
-    Rho[2 * x] = a;       // RhoE stash
-    Rho[2 * x + 1] = b2;  // RhoH stash
-    se += a;
-    shh += b2;
+    st(g.out + ((0 * NW + w) * (long)Nx) * 2, x, cd{v[0], 0});
+    st(g.out + ((1 * NW + w) * (long)Nx) * 2, x, cd{v[1], 0});
+    st(g.out + ((2 * NW + w) * (long)Nx) * 2, x, cd{v[2], 0});
+    a = lf ? fabs(v[3]) : v[3];
+    b2 = lf ? fabs(v[4]) : v[4];
+    double *Rho = g.out + ((3 * NW + w) * (long)Nx) * 2;
+    Rho[2 * x] = a;       // RhoE, finished by k_qw2prop_b
+    Rho[2 * x + 1] = b2;  // RhoH
   }
-  const double SE = block_sum(se, sh), SH = block_sum(shh, sh);
+  const double SE = block_sum(a, sh), SH = block_sum(b2, sh);
+  if (threadIdx.x == 0) {
+    g.part[(long)w * 256 + 2 * blockIdx.y] = SE;
+    g.part[(long)w * 256 + 2 * blockIdx.y + 1] = SH;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_qw2prop_b(psa_group g, int w0) {
+  const int w = w0 + blockIdx.x, Nx = g.Nx;
+  const int nb = gridDim.y;
+  double SE = 0.0, SH = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    SE += g.part[(long)w * 256 + 2 * i];
+    SH += g.part[(long)w * 256 + 2 * i + 1];
+  }
+  const bool lf = (g.flags & PSA_F_LF) != 0;
   const double total = g.obs[(long)w * PSA_OBS_N + 3];
-  const double dRR = g.drr;
-  const double fe = lf ? total / (SE * dRR + 1e-100) : 1.0;
-  const double fh = lf ? total / (SH * dRR + 1e-100) : 1.0;
-  for (int x = tid; x < Nx; x += blockDim.x) {
-    const double a = Rho[2 * x] * fe, b = Rho[2 * x + 1] * fh;
-    st(Rho, x, cd{b - a, 0.0});
-  }
+  const double fe = lf ? total / (SE * g.drr + 1e-100) : 1.0;
+  const double fh = lf ? total / (SH * g.drr + 1e-100) : 1.0;
+  const int x = blockIdx.y * blockDim.x + threadIdx.x;
+  if (x >= Nx) return;
+  double *Rho = g.out + ((3 * (long)g.Nw + w) * (long)Nx) * 2;
+  const double a = Rho[2 * x] * fe, b = Rho[2 * x + 1] * fh;
+  st(Rho, x, cd{b - a, 0.0});
 }
 
 // ===========================================================================
@@ -744,15 +790,27 @@ __global__ __launch_bounds__(256) void k_qw2prop(psa_group g, int w0) {
 // ===========================================================================
 namespace {
 
+template <typename T, class P>
+int launch_gemm_t(const P &p, dim3 grid, hipStream_t s, const char *what) {
+  constexpr size_t shm = gemm_lds_bytes<T>();
+  static bool attr_set = false;  // per instantiation
+  if (shm > 65536 && !attr_set) {
+    int rc = hip_check(hipFuncSetAttribute((const void *)gemm_kernel<T, P>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
+                       "hipFuncSetAttribute(gemm_kernel)");
+    if (rc) return rc;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((gemm_kernel<T, P>), grid, dim3(256), shm, s, p);
+  PSA_LAUNCH_CHECK(what);
+  return PSA_OK;
+}
+
 template <class P>
 int launch_gemm(const P &p, int prec, int M, int Ncols, int batches, hipStream_t s, const char *what) {
   dim3 grid((Ncols + GBN - 1) / GBN, (M + GBM - 1) / GBM, batches);
-  if (prec == PSA_PREC_FP64)
-    hipLaunchKernelGGL((gemm_kernel<double, P>), grid, dim3(256), 0, s, p);
-  else
-    hipLaunchKernelGGL((gemm_kernel<float, P>), grid, dim3(256), 0, s, p);
-  PSA_LAUNCH_CHECK(what);
-  return PSA_OK;
+  if (prec == PSA_PREC_FP64) return launch_gemm_t<double>(p, grid, s, what);
+  return launch_gemm_t<float>(p, grid, s, what);
 }
 
 }  // namespace
@@ -776,8 +834,13 @@ int psa_vint_table(const double *Qy, int32_t NQ, const double *y, int32_t Ny, do
 
 int psa_prep(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
-  hipLaunchKernelGGL(k_prep, dim3(nw), dim3(256), 0, (hipStream_t)stream, *g, w0);
-  PSA_LAUNCH_CHECK("k_prep");
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_prep_a, dim3(nw), dim3(256), 0, s, *g, w0);
+  PSA_LAUNCH_CHECK("k_prep_a");
+  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 255) / 256), dim3(256), 0, s, *g, w0, 0);
+  PSA_LAUNCH_CHECK("k_diagsums");
+  hipLaunchKernelGGL(k_prep_b, dim3(nw, (g->N + 3) / 4), dim3(256), 0, s, *g, w0);
+  PSA_LAUNCH_CHECK("k_prep_b");
   return PSA_OK;
 }
 
@@ -792,7 +855,8 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   CalcHProb p;
   p.g = *g;
   p.w0 = w0;
-  return launch_gemm(p, g->prec, g->N, 2 * g->N - 1, 3 * nw, s, "calc_h gemm");
+  // the exchange sums feed Heh, which drives p directly: always FP64
+  return launch_gemm(p, PSA_PREC_FP64, g->N, 2 * g->N - 1, 3 * nw, s, "calc_h gemm");
 }
 
 int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
@@ -801,7 +865,8 @@ int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream)
   p.g = *g;
   p.w0 = w0;
   p.nw = nw;
-  return launch_gemm(p, g->prec, g->N, 2 * nw, 2 * (2 * g->N - 1), (hipStream_t)stream,
+  // OffP enters dp directly: always FP64
+  return launch_gemm(p, PSA_PREC_FP64, g->N, 2 * nw, 2 * (2 * g->N - 1), (hipStream_t)stream,
                      "offdiag gemm");
 }
 
@@ -840,10 +905,22 @@ int psa_transport(const psa_group *g, int32_t w0, int32_t nw, double dt, void *s
   while (n % 2 == 0) n /= 2;
   while (n % 3 == 0) n /= 3;
   if (n != 1) return fail(PSA_E_UNSUPPORTED, "Transport needs N = 2^a 3^b");
-  const size_t shm = 4 * 2 * (size_t)g->N * sizeof(cd);
   const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_tphase, dim3(nw), dim3(256), 0, s, *g, w0, dt);
+  PSA_LAUNCH_CHECK("k_tphase");
+  const size_t shm = 9 * (size_t)g->N * sizeof(cd);  // 4 lines x 2 buffers + twiddles
+  if (shm > 65536) {
+    static bool attr_set = false;
+    if (!attr_set) {
+      int rc = hip_check(hipFuncSetAttribute((const void *)k_transport,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
+                         "hipFuncSetAttribute(k_transport)");
+      if (rc) return rc;
+      attr_set = true;
+    }
+  }
   for (int pass = 0; pass < 2; ++pass) {
-    hipLaunchKernelGGL(k_transport, dim3((g->N + 3) / 4, 3, nw), dim3(256), shm, s, *g, w0, dt, pass);
+    hipLaunchKernelGGL(k_transport, dim3((g->N + 3) / 4, 3, nw), dim3(256), shm, s, *g, w0, pass);
     PSA_LAUNCH_CHECK("k_transport");
   }
   return PSA_OK;
@@ -863,9 +940,13 @@ int psa_finish(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
 
 int psa_sources(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
-  const size_t shm = 2 * (size_t)g->Nr * sizeof(double);
-  hipLaunchKernelGGL(k_sources, dim3(nw), dim3(256), shm, (hipStream_t)stream, *g, w0);
-  PSA_LAUNCH_CHECK("k_sources");
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 255) / 256), dim3(256), 0, s, *g, w0, 1);
+  PSA_LAUNCH_CHECK("k_diagsums");
+  hipLaunchKernelGGL(k_src_r, dim3(nw, (g->Nr + 255) / 256), dim3(256), 0, s, *g, w0);
+  PSA_LAUNCH_CHECK("k_src_r");
+  hipLaunchKernelGGL(k_src_norm, dim3(nw), dim3(256), 0, s, *g, w0);
+  PSA_LAUNCH_CHECK("k_src_norm");
   return PSA_OK;
 }
 
@@ -881,8 +962,13 @@ int psa_prop2qw(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
 
 int psa_qw2prop(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
-  hipLaunchKernelGGL(k_qw2prop, dim3(nw), dim3(256), 0, (hipStream_t)stream, *g, w0);
-  PSA_LAUNCH_CHECK("k_qw2prop");
+  const int nb = (g->Nx + 255) / 256;
+  if (nb > 128) return fail(PSA_E_INVALID, "Nx must be <= 32768");
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_qw2prop_a, dim3(nw, nb), dim3(256), 0, s, *g, w0);
+  PSA_LAUNCH_CHECK("k_qw2prop_a");
+  hipLaunchKernelGGL(k_qw2prop_b, dim3(nw, nb), dim3(256), 0, s, *g, w0);
+  PSA_LAUNCH_CHECK("k_qw2prop_b");
   return PSA_OK;
 }
 

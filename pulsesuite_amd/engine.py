@@ -23,11 +23,11 @@ C128 = torch.complex128
 
 
 def _prep_stride(N):
-    return 8 * N + 4 * (2 * N - 1) + 3 * N
+    return 8 * N + 4 * (2 * N - 1) + 5 * N
 
 
 def _src_stride(N, Nr):
-    return 5 * Nr + 8 * (2 * N - 1) + 8
+    return 9 * Nr + 8 * (2 * N - 1) + 8
 
 
 class WireGroup:
@@ -69,7 +69,9 @@ class WireGroup:
             "didx": T(tb["didx"], torch.int32), "iq": T(tb["iq"], torch.int32),
             "homega": T(tb["homega"]), "hsq": T(tb["hsq"]), "window": T(tb["window"]),
             "rq": T(g["r"]), "ytr": T(tb["ytr"]),
-            "gmat": T(tb["gmat"], C128), "m2": T(tb["m2"] if tb["bnx"] else np.zeros((1, Nr))),
+            "gmat": T(tb["gmat"], C128),
+            "m2": T(tb["m2"].T if tb["bnx"] else np.zeros((Nr, 1))),
+            "phi": T(tb["phi"], C128), "twid": T(tb["twid"], C128),
         }
         self.mb = []
         for off, idx, val in tb["mb"]:
@@ -90,6 +92,8 @@ class WireGroup:
         self.s["src"] = z(nwz, _src_stride(N, Nr), dt=F64)
         self.s["out"] = z(4, nwz, Nx)
         self.s["obs"] = z(nwz, 8, dt=F64)
+        self.s["tphase"] = z(nwz, N)
+        self.s["part"] = z(nwz, 256, dt=F64)
         if wire_ids is None:
             wire_ids = np.arange(1, nwz + 1)
         self.wire_ids = np.asarray(wire_ids, dtype=np.int32)

@@ -419,6 +419,12 @@ def build_tables(prm, rr, vint_fn):
     Lw = (N - 1) * dky
     y = np.arange(N, dtype=float) * (TWOPI / Lw)
     t["ytr"] = y - y[N // 2]
+    # sources: exp(i d dkr r) for d = -(N-1)..N-1 (QWPolarization3/QWRho5
+    # phases Expikr[k'] conj(Expikr[k]) with k' - k = d, qwoptics.py:652-663)
+    dd = np.arange(-(N - 1), N, dtype=float)
+    t["phi"] = np.exp(1j * ((dd * g["dkr"])[:, None] * g["r"][None, :]))
+    # FFT twiddles for the Transport kernel
+    t["twid"] = np.exp(-2j * np.pi * np.arange(N) / N)
     # Prop2QW as one matrix on the real parts (qwoptics.py:667-715)
     rr = np.asarray(rr, dtype=float)
     M1 = spline_matrix(rr, g["r"])                       # [Nr][Nx]

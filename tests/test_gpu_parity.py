@@ -214,3 +214,22 @@ def test_step_vs_oracle_n64():
         got = c.wg.get_state()
         for v, r in zip(got, (c.os.CC2, c.os.DD2, c.os.YY2, c.os.CC3, c.os.DD3, c.os.YY3)):
             assert rel(v, r) < TRAJ_TOL, it
+
+
+def test_mixed_precision_fp32_error_bounded():
+    """FP32-MFMA contractions (operands rounded to FP32, FP32 accumulation;
+    diagonals, epilogues and state in FP64) against the FP64 path after
+    five steps: the state stays within 1e-6 relative."""
+    import torch
+    a = Case(64, 2, seed=21)
+    b = Case(64, 2, seed=21, prec="fp32")
+    for _ in range(5):
+        a.wg.step(1e-17)
+        b.wg.step(1e-17)
+    torch.cuda.synchronize()
+    Sa, Sb = a.wg.get_state(), b.wg.get_state()
+    errs = [rel(x, y) for x, y in zip(Sb, Sa)]
+    pa, pb = a.wg.outputs()[0], b.wg.outputs()[0]
+    print("fp32 state errors", errs, "Pxx", rel(pb, pa))
+    assert max(errs) < 1e-6
+    assert rel(pb, pa) < 1e-6
