@@ -1,0 +1,412 @@
+"""Drop-in for ``pulsesuite.PSTD3D.SBEs`` on one MI355X.
+
+Keeps the reference API (``SBEs.py:4619-6053``):
+
+* ``SBESolver(q, rr, r0, Emaxxx, lam, Nw, QW, backend="auto")`` reading
+  ``params/qw.params`` and ``params/mb.params`` from the working directory
+  (``SBEs.py:4978-5022``);
+* ``SBESolver.QWCalculator(Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz,
+  Rho, DoQWP, DoQWDl)`` with the same bookkeeping (``SBEs.py:5837-5847``),
+  wire-off gate (``5849-5858``), in-place outputs (strided views allowed),
+  ``Rho = RhoH - RhoE`` and the P.E energy bookkeeping (``5925-5979``);
+* the module shims ``InitializeSBE``, ``QWCalculator``, ``chiqw``, ``getqc``,
+  ``QWArea``, ``ShutOffOptics``, ``WriteSBEsData``, ``ReadSBEsData``.
+
+The state of all wires lives on the GPU (``engine.WireGroup``); every call
+runs the HIP path.  ``QWCalculatorBatch`` steps all wires in one call (what
+a throughput driver should use).  Output files (``dataQW/info.WW.t.dat``,
+``dataQW/EP.WW.t.dat``, ``output/PdotE.*.dat``, backups) follow the
+reference formats when ``io=True``.
+"""
+
+import os
+
+import numpy as np
+import torch
+from scipy.constants import c as c0
+from scipy.constants import e as e0
+from scipy.constants import epsilon_0 as eps0
+from scipy.constants import hbar
+from scipy.constants import k as kB
+
+from . import tables
+from .engine import WireGroup
+
+twopi = 2.0 * np.pi
+ii = 1j
+
+
+def yw(w):
+    """qwoptics.py:1093-1106."""
+    return (-1) ** int(np.floor((w - 1) / 2.0))
+
+
+def QWChi1(lam, dky, Ee, Eh, area, geh, dcv):
+    """Linear susceptibility of the wire (qwoptics.py:1109-1146)."""
+    ww = twopi * c0 / lam
+    d1 = Ee + Eh - ii * hbar * geh - hbar * ww
+    d2 = Ee + Eh + ii * hbar * geh + hbar * ww
+    return 4 * dcv**2 / eps0 / area * dky / twopi * np.sum((Ee + Eh) / (d1 * d2))
+
+
+class SBESolver:
+    """Class-based SBE solver with the reference constructor signature.
+
+    Extra keywords (all optional): ``params`` (a ``tables.QWParams``
+    instead of reading params/*.params), ``prec`` ("fp64" default,
+    "fp32"), ``io`` (write the reference's per-step text files),
+    ``keep_level1`` (keep the YY1/CC1/DD1 level for backups)."""
+
+    def __init__(self, q, rr, r0, Emaxxx, lam, Nw, QW, backend="auto", params=None,
+                 prec="fp64", io=True, keep_level1=True):
+        self._backend = backend
+        self.Nw = int(Nw)
+        self.io = bool(io)
+        self.keep_level1 = bool(keep_level1)
+        self.start = False
+        if not QW:
+            if self.io:
+                os.makedirs("dataQW/Wire/info", exist_ok=True)
+                open("dataQW/Wire/info/ETHz.t.dat", "w").close()
+            return
+        if params is None:
+            params = tables.QWParams.from_files("params/qw.params", "params/mb.params")
+        self.prm = params
+        self.rr = np.asarray(rr, dtype=float)
+        self.Emax0 = float(Emaxxx)
+        self.lam = float(lam)
+        self.group = WireGroup(params, self.Nw, self.rr, wire_ids=np.arange(1, self.Nw + 1),
+                               prec=prec)
+        g = self.group.grid
+        for k in ("kr", "r", "Qr", "NQ0", "dkr", "dr", "dcv", "alphae", "alphah",
+                  "ehint", "qc", "area", "Ee", "Eh"):
+            setattr(self, k, g[k])
+        self.Nk, self.Nr = g["N"], g["Nr"]
+        self.L, self.gap = params.L, params.gap
+        self.me, self.mh = params.me, params.mh
+        self.gam_e, self.gam_h = params.gam_e, params.gam_h
+        self.gam_eh = g["gam_eh"]
+        self.jmax = params.jmax
+        for n, v in params.flags.items():
+            setattr(self, n, v)
+        self.t = 0.0
+        self.xxx = 1
+        self.jjj = 1
+        self.wireoff = True
+        self.I0 = np.zeros(self.Nw)
+        self.chiw = QWChi1(lam, self.dkr, self.Ee + self.gap, self.Eh, self.area,
+                           self.gam_eh, self.dcv)
+        self.EPEnergy = 0.0
+        self.EPEnergyW = 0.0
+        self._P0 = self._P1 = None
+        self._file_972 = self._file_973 = None
+        self._lvl1 = None
+        # real-space field at r[Nr//2] for the EP file: window * spline row
+        self._m1 = tables.spline_matrix(self.rr, self.r) if self.io else None
+        if self.io:
+            os.makedirs("dataQW", exist_ok=True)
+            for w in range(1, self.Nw + 1):
+                for nm in ("info", "EP", "XQ"):
+                    open(f"dataQW/{nm}.{w:02d}.t.dat", "w").close()
+            os.makedirs("dataQW/Wire/info", exist_ok=True)
+            open("dataQW/Wire/info/ETHz.t.dat", "w").close()
+        self.start = True
+
+    # ------------------------------------------------------------------
+    # reference-style state access ([N, N, Nw], host copies)
+    # ------------------------------------------------------------------
+    def _levels(self):
+        return self.group.get_state()
+
+    @property
+    def CC2(self):
+        return self._levels()[0]
+
+    @property
+    def DD2(self):
+        return self._levels()[1]
+
+    @property
+    def YY2(self):
+        return self._levels()[2]
+
+    @property
+    def CC3(self):
+        return self._levels()[3]
+
+    @property
+    def DD3(self):
+        return self._levels()[4]
+
+    @property
+    def YY3(self):
+        return self._levels()[5]
+
+    def set_levels(self, C2, D2, p2, C3, D3, p3):
+        """Load levels 2 and 3 (reference layout [N, N, Nw])."""
+        self.group.set_state(C2, D2, p2, C3, D3, p3)
+
+    # ------------------------------------------------------------------
+    def ShutOffOptics(self):
+        """Disable optical coupling (SBEs.py:5516-5518)."""
+        self.Optics = False
+        self.prm.flags["Optics"] = False
+        self.group._sync_struct()
+
+    def chiqw(self):
+        return self.chiw
+
+    def getqc(self):
+        return self.qc
+
+    def QWArea(self):
+        return self.area
+
+    # ------------------------------------------------------------------
+    def _bookkeeping(self, w, dt):
+        """SBEs.py:5837-5847."""
+        if w == 1:
+            if self.jjj == self.jmax:
+                self.jjj = 0
+            self.xxx += 1
+            self.jjj += 1
+            self.t += dt
+        return self.jjj == self.jmax
+
+    def _gate(self, Exx, Eyy, Ezz):
+        """SBEs.py:5849-5858: True while the wire stays off."""
+        mag = np.max(np.sqrt(np.abs(Exx) ** 2 + np.abs(Eyy) ** 2 + np.abs(Ezz) ** 2))
+        if self.wireoff and mag < 1e-3 * self.Emax0:
+            return True
+        self.wireoff = False
+        return False
+
+    def _snapshot_level1(self, w0, nw):
+        if not self.keep_level1:
+            return
+        s = self.group.s
+        if self._lvl1 is None:
+            self._lvl1 = {k: torch.zeros_like(s[k]) for k in ("pA", "CA", "DA")}
+        for k in ("pA", "CA", "DA"):
+            self._lvl1[k][w0:w0 + nw].copy_(s[k][w0:w0 + nw])
+
+    def QWCalculator(self, Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz, Rho, DoQWP, DoQWDl):
+        """One step of wire w (1-based) on the GPU; outputs in place."""
+        Pxx[:] = 0.0
+        Pyy[:] = 0.0
+        Pzz[:] = 0.0
+        Rho[:] = 0.0
+        self._bookkeeping(w, dt)
+        if self._gate(Exx, Eyy, Ezz):
+            return
+        DoQWP[0] = self.Optics
+        DoQWDl[0] = self.LF
+        i = w - 1
+        g = self.group
+        g.set_fields(np.asarray(Exx), np.asarray(Eyy), np.asarray(Ezz), w0=i, nw=1)
+        self._snapshot_level1(i, 1)
+        g.step(dt, w0=i, nw=1)
+        P = g.outputs(i, 1)
+        out = (P[0][0], P[1][0], P[2][0], P[3][0])
+        if self.ignorewire:
+            out = tuple(np.zeros_like(o) for o in out)
+        Pxx[:] = out[0]
+        Pyy[:] = out[1]
+        Pzz[:] = out[2]
+        Rho[:] = out[3]
+        self.I0[i] = g.observables()["I0"][i]
+        self._energy(np.asarray(Exx), np.asarray(Eyy), np.asarray(Ezz), out, i)
+        if self.io:
+            self._write_files(np.asarray(Exx), np.asarray(Eyy), np.asarray(Ezz), dt, w)
+
+    def QWCalculatorBatch(self, Exx, Eyy, Ezz, dt):
+        """Step every wire at once; fields [Nw, Nx] (or [Nx] shared) as numpy
+        or device tensors.  Returns device tensors (Pxx, Pyy, Pzz, Rho) of
+        shape [Nw, Nx].  Bookkeeping as Nw successive QWCalculator calls;
+        the gate is evaluated on the host when the fields are numpy."""
+        self._bookkeeping(1, dt)
+        if isinstance(Exx, np.ndarray) and self._gate(Exx, Eyy, Ezz):
+            z = torch.zeros(self.Nw, len(self.rr), dtype=torch.complex128, device=self.group.dev)
+            return z, z.clone(), z.clone(), z.clone()
+        self.wireoff = False
+        g = self.group
+        g.set_fields(Exx, Eyy, Ezz)
+        self._snapshot_level1(0, self.Nw)
+        g.step(dt)
+        o = g.s["out"]
+        return o[0], o[1], o[2], o[3]
+
+    # ------------------------------------------------------------------
+    def _energy(self, Exx, Eyy, Ezz, out, i):
+        """P.E bookkeeping on the propagation grid (SBEs.py:5925-5979); the
+        reference keeps one history for all wires."""
+        Pxx, Pyy, Pzz, _ = out
+        drr = self.rr[1] - self.rr[0] if len(self.rr) > 1 else 1.0
+        if self._P0 is None:
+            self._P0 = [np.zeros(len(Pxx)) for _ in range(3)]
+            self._P1 = [np.zeros(len(Pxx)) for _ in range(3)]
+            if self.io:
+                os.makedirs("output", exist_ok=True)
+                self._file_972 = open("output/PdotE.p.dat", "w", encoding="utf-8")
+                self._file_973 = open("output/PdotE.w.dat", "w", encoding="utf-8")
+        for c, (P, E) in enumerate(((Pxx, Exx), (Pyy, Eyy), (Pzz, Ezz))):
+            self.EPEnergy += np.sum((np.real(P) - self._P0[c]) * np.real(E) * 0.5) * drr * self.area
+        for c, P in enumerate((Pxx, Pyy, Pzz)):
+            self._P0[c][:] = self._P1[c]
+            self._P1[c][:] = np.real(P)
+        if self._m1 is not None:
+            # same on the wire grid with the real-space QW fields and sources
+            g = self.group
+            W = g.tab["window"]
+            src = g.s["src"][i].cpu().numpy()
+            Nr = self.Nr
+            if not hasattr(self, "_PW"):
+                self._PW = [[np.zeros(Nr) for _ in range(3)] for _ in range(2)]
+            for c, E in enumerate((Exx, Eyy, Ezz)):
+                Eq = W * np.real(self._m1 @ E)
+                Pq = src[c * Nr:(c + 1) * Nr]
+                self.EPEnergyW += np.sum((Pq - self._PW[0][c]) * Eq * 0.5) * self.dr * self.area
+                self._PW[0][c][:] = self._PW[1][c]
+                self._PW[1][c][:] = Pq
+        if self._file_972 is not None:
+            self._file_972.write(f"{self.t} {self.EPEnergy / e0}\n")
+            self._file_972.flush()
+            self._file_973.write(f"{self.t} {self.EPEnergyW / e0}\n")
+            self._file_973.flush()
+
+    def _write_files(self, Exx, Eyy, Ezz, dt, w):
+        """dataQW/info.WW.t.dat and EP.WW.t.dat lines (SBEs.py:5773-5811)."""
+        i = w - 1
+        g = self.group
+        N, Nr = self.Nk, self.Nr
+        C2 = g.s["CA"][i].diagonal().cpu().numpy()
+        D2 = g.s["DA"][i].diagonal().cpu().numpy()
+        ne2 = np.abs(C2).astype(complex)
+        nh2 = np.abs(D2).astype(complex)
+        kr, Ee, Eh, me, mh = self.kr, self.Ee, self.Eh, self.me, self.mh
+        small = 1e-200
+
+        def vd(m, n):
+            return np.sum(np.real(n) * hbar * kr / m) / np.sum(np.real(n) + small)
+
+        def temp(n, E):
+            return 2.0 * np.real(np.sum(n * E) / (np.sum(n) + 1e-100)) / kB
+
+        def i0n(n, m):
+            dk = kr[1] - kr[0]
+            return -e0 * np.sum(n * kr * hbar / m) * 2.0 * dk
+
+        src = g.s["src"][i].cpu().numpy()
+        rc = src[5 * Nr:9 * Nr].view(complex).reshape(2, Nr)
+        X3d = (g.s["CX"][i].diagonal().cpu().numpy(), g.s["DX"][i].diagonal().cpu().numpy())
+        dr = self.r[1] - self.r[0]
+        rmax = []
+        for c in range(2):
+            re = rc[c].copy()
+            re = re - np.real(np.sum(re[0:10]) + np.sum(re[Nr - 10:Nr])) / 20.0
+            tot = np.sum(np.abs(X3d[c])) + 1e-50
+            re = re * tot / (np.sum(np.abs(re)) * dr + 1e-100)
+            Rq = -np.fft.fft(re) / Nr * 2.0 * e0 / self.area * self.ehint
+            rmax.append(np.max(np.abs(Rq)))
+        cols = [self.xxx * dt, vd(me, ne2), vd(mh, nh2),
+                np.real(2 * np.sum(ne2) / self.L), np.real(2 * np.sum(nh2) / self.L),
+                np.real(np.sum(ne2 * (Ee + self.gap))) / e0, np.real(np.sum(nh2 * Eh)) / e0,
+                temp(ne2, Ee), temp(nh2, Eh), rmax[0], rmax[1], 0.0,
+                np.real(hbar * np.sum((ne2 - nh2) * kr)) * 2.0,
+                np.real(hbar * np.sum(ne2 * kr)), np.real(-hbar * np.sum(nh2 * kr)),
+                self.I0[i], np.real(i0n(ne2, me)), np.real(i0n(1 - nh2, -mh))]
+        with open(f"dataQW/info.{w:02d}.t.dat", "a", encoding="utf-8") as f:
+            f.write(" ".join(f"{c:18.6e}" for c in cols) + "\n")
+        W = g.tab["window"][Nr // 2]
+        Ef = [W * np.real(self._m1[Nr // 2] @ E) for E in (Exx, Eyy, Ezz)]
+        Pm = [src[c * Nr + Nr // 2] for c in range(3)]
+        with open(f"dataQW/EP.{w:02d}.t.dat", "a", encoding="utf-8") as f:
+            f.write(" ".join(f"{c:18.6e}" for c in [self.xxx * dt] + Ef + Pm) + "\n")
+
+    # ------------------------------------------------------------------
+    # backups (SBEs.py:5524-5569): one "re im" pair per line, order w, k2, k1
+    # ------------------------------------------------------------------
+    def _all_levels(self):
+        C2, D2, p2, C3, D3, p3 = self._levels()
+        if self._lvl1 is not None:
+            l1 = {k: np.moveaxis(v.cpu().numpy(), 0, 2) for k, v in self._lvl1.items()}
+            C1, D1, p1 = l1["CA"], l1["DA"], l1["pA"]
+        else:
+            C1, D1, p1 = np.zeros_like(C2), np.zeros_like(D2), np.zeros_like(p2)
+        return dict(CC1=C1, CC2=C2, CC3=C3, DD1=D1, DD2=D2, DD3=D3, YY1=p1, YY2=p2, YY3=p3)
+
+    def write_sbes_data(self, n):
+        d = "dataQW/backup/"
+        os.makedirs(d, exist_ok=True)
+        for key, arr in self._all_levels().items():
+            with open(f"{d}{key}.{n}.dat", "w", encoding="utf-8") as f:
+                for w in range(arr.shape[2]):
+                    blk = arr[:, :, w].T.reshape(-1)  # k2 outer, k1 inner
+                    f.write("".join(f"{np.real(v)} {np.imag(v)}\n" for v in blk))
+
+    def read_sbes_data(self, Nt):
+        d = "dataQW/backup/"
+        N, Nw = self.Nk, self.Nw
+        lv = {}
+        for key in ("CC1", "CC2", "CC3", "DD1", "DD2", "DD3", "YY1", "YY2", "YY3"):
+            v = np.loadtxt(f"{d}{key}.{Nt}.dat").reshape(Nw, N, N, 2)
+            a = v[..., 0] + 1j * v[..., 1]          # [w][k2][k1]
+            lv[key] = np.moveaxis(np.swapaxes(a, 1, 2), 0, 2)  # [k1][k2][w]
+        self.set_levels(lv["CC2"], lv["DD2"], lv["YY2"], lv["CC3"], lv["DD3"], lv["YY3"])
+        if self.keep_level1:
+            self._snapshot_level1(0, Nw)
+            for k, key in (("CA", "CC1"), ("DA", "DD1"), ("pA", "YY1")):
+                self._lvl1[k].copy_(torch.as_tensor(np.ascontiguousarray(
+                    np.moveaxis(lv[key], 2, 0)), device=self.group.dev))
+
+    def close(self):
+        for f in (self._file_972, self._file_973):
+            if f is not None:
+                f.close()
+        self._file_972 = self._file_973 = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# module-level shims (SBEs.py:6008-6053)
+# ---------------------------------------------------------------------------
+_default_solver = None
+
+
+def InitializeSBE(q, rr, r0, Emaxxx, lam, Nw, QW):
+    global _default_solver
+    _default_solver = SBESolver(q, rr, r0, Emaxxx, lam, Nw, QW)
+
+
+def QWCalculator(Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz, Rho, DoQWP, DoQWDl):
+    _default_solver.QWCalculator(Exx, Eyy, Ezz, Vrr, rr, q, dt, w, Pxx, Pyy, Pzz, Rho,
+                                 DoQWP, DoQWDl)
+
+
+def chiqw():
+    return _default_solver.chiqw()
+
+
+def getqc():
+    return _default_solver.getqc()
+
+
+def QWArea():
+    return _default_solver.QWArea()
+
+
+def ShutOffOptics():
+    _default_solver.ShutOffOptics()
+
+
+def WriteSBEsData(n):
+    _default_solver.write_sbes_data(n)
+
+
+def ReadSBEsData(Nt):
+    _default_solver.read_sbes_data(Nt)
