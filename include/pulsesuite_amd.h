@@ -98,7 +98,8 @@ typedef struct psa_group {
   double *hdiag;                      /* [Nw][2][N] complex: diagonals of Hee, Hhh */
   double *prep;                       /* [Nw][PSA_PREP_STRIDE(N)] doubles, see PSA_PREP_* */
   double *dedh;                       /* [Nw][2][2N-1][N] off-diagonal dephasing tables */
-  double *wmb;                        /* [Nw][6][N] many-body Win_e, Wout_e, Win_h, Wout_h, |Re ne3|, |Re nh3| */
+  double *wmb;                        /* 9 N Nw doubles: [Nw][4][N] many-body Win_e, Wout_e, Win_h, Wout_h;
+                                         then [2][N][Nw] |Re ne3|, |Re nh3| and [3][N][Nw] VC(d)^2 (wire fastest) */
   double *exq;                        /* [Nw][3][2N-1] complex: FFTG'd Ex, Ey, Ez at q = -(N-1)..N-1 */
   const double *efield;               /* [3][Nw][Nx] complex fields on rr (Exx, Eyy, Ezz) */
   double *src;                        /* [Nw][PSA_SRC_STRIDE] real-space Px, Py, Pz, re, rh + diagonal sums */

@@ -9,6 +9,9 @@
 // Each wave owns a 32x32 sub-tile = 2x2 MFMA 16x16 blocks; a complex
 // product costs four real 16x16x4 MFMAs.  The epilogue hands every output
 // element with its real/imaginary accumulator to the problem's store().
+// A problem supplies nterms(b), krange(t, b, m0, n0, kb, ke) (the K slice
+// that can be non-zero for this tile; loaders return zero outside it) and
+// active(b, m0, n0) (false when every product in the tile is zero).
 //
 // MFMA 16x16x4 lane layout: lane l supplies A[l%16][l/16] and B[l/16][l%16]
 // for both f32 and f64.  Accumulator element v of lane l is D[4*(l/16)+v][l%16]
@@ -98,6 +101,17 @@ __device__ __forceinline__ void load_dense(Frag<T> &s, const double *X, int ldx,
   }
 }
 
+// First term t >= t0 whose K range [kb, ke) is non-empty (nt if none).
+template <class P>
+__device__ __forceinline__ int next_term(const P &p, int t0, int nt, int b, int m0, int n0,
+                                         int &kb, int &ke) {
+  for (int t = t0; t < nt; ++t) {
+    p.krange(t, b, m0, n0, kb, ke);
+    if (kb < ke) return t;
+  }
+  return nt;
+}
+
 template <typename T, class P>
 __global__ __launch_bounds__(256) void gemm_kernel(P p) {
   extern __shared__ char lds_raw[];
@@ -116,56 +130,60 @@ __global__ __launch_bounds__(256) void gemm_kernel(P p) {
     }
   const int kl = lane >> 4, xl = lane & 15;
   const int nt = p.nterms(b);
-  // flattened (term, k-slab) iteration with a one-slab register prefetch
-  int t = 0, k0 = 0;
-  Frag<T> fa, fb;
-  p.fetchA(fa, 0, b, m0, 0);
-  p.fetchB(fb, 0, b, 0, n0);
-  fa.commit(tiles[0]);
-  fb.commit(tiles[1]);
-  __syncthreads();
-  int cur = 0;
-  while (true) {
-    int tn = t, kn = k0 + GBK;
-    if (kn >= p.kext(t, b)) {
-      tn = t + 1;
-      kn = 0;
-    }
-    const bool more = tn < nt;
-    if (more) {
-      p.fetchA(fa, tn, b, m0, kn);
-      p.fetchB(fb, tn, b, kn, n0);
-    }
-    const Tile<T> &sA = tiles[2 * cur], &sB = tiles[2 * cur + 1];
-#pragma unroll
-    for (int kq = 0; kq < GBK; kq += 4) {
-      const int kk = kq + kl;
-      T ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        ar[s] = sA.v[0][kk][wm * 32 + s * 16 + xl];
-        br[s] = sB.v[0][kk][wn * 32 + s * 16 + xl];
-        if (!P::A_REAL) ai[s] = sA.v[1][kk][wm * 32 + s * 16 + xl];
-        if (!P::B_REAL) bi[s] = sB.v[1][kk][wn * 32 + s * 16 + xl];
-      }
-#pragma unroll
-      for (int si = 0; si < 2; ++si)
-#pragma unroll
-        for (int sj = 0; sj < 2; ++sj) {
-          aR[si][sj] = Mfma<T>::mma(ar[si], br[sj], aR[si][sj]);
-          if (!P::A_REAL && !P::B_REAL)
-            aR[si][sj] = Mfma<T>::mma(-ai[si], bi[sj], aR[si][sj]);
-          if (!P::B_REAL) aI[si][sj] = Mfma<T>::mma(ar[si], bi[sj], aI[si][sj]);
-          if (!P::A_REAL) aI[si][sj] = Mfma<T>::mma(ai[si], br[sj], aI[si][sj]);
-        }
-    }
-    if (!more) break;
-    fa.commit(tiles[2 * (cur ^ 1)]);
-    fb.commit(tiles[2 * (cur ^ 1) + 1]);
+  // Terms run over their own K ranges [kb, ke) (block-uniform); a tile the
+  // problem marks inactive, or one whose ranges are all empty, skips the
+  // products and stores zero accumulators.
+  int t = nt, k0 = 0, ke = 0;
+  if (p.active(b, m0, n0)) t = next_term(p, 0, nt, b, m0, n0, k0, ke);
+  if (t < nt) {
+    // flattened (term, k-slab) iteration with a one-slab register prefetch
+    Frag<T> fa, fb;
+    p.fetchA(fa, t, b, m0, k0);
+    p.fetchB(fb, t, b, k0, n0);
+    fa.commit(tiles[0]);
+    fb.commit(tiles[1]);
     __syncthreads();
-    cur ^= 1;
-    t = tn;
-    k0 = kn;
+    int cur = 0;
+    while (true) {
+      int tn = t, kn = k0 + GBK, ken = ke;
+      if (kn >= ke) tn = next_term(p, t + 1, nt, b, m0, n0, kn, ken);
+      const bool more = tn < nt;
+      if (more) {
+        p.fetchA(fa, tn, b, m0, kn);
+        p.fetchB(fb, tn, b, kn, n0);
+      }
+      const Tile<T> &sA = tiles[2 * cur], &sB = tiles[2 * cur + 1];
+#pragma unroll
+      for (int kq = 0; kq < GBK; kq += 4) {
+        const int kk = kq + kl;
+        T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          ar[s] = sA.v[0][kk][wm * 32 + s * 16 + xl];
+          br[s] = sB.v[0][kk][wn * 32 + s * 16 + xl];
+          if (!P::A_REAL) ai[s] = sA.v[1][kk][wm * 32 + s * 16 + xl];
+          if (!P::B_REAL) bi[s] = sB.v[1][kk][wn * 32 + s * 16 + xl];
+        }
+#pragma unroll
+        for (int si = 0; si < 2; ++si)
+#pragma unroll
+          for (int sj = 0; sj < 2; ++sj) {
+            aR[si][sj] = Mfma<T>::mma(ar[si], br[sj], aR[si][sj]);
+            if (!P::A_REAL && !P::B_REAL)
+              aR[si][sj] = Mfma<T>::mma(-ai[si], bi[sj], aR[si][sj]);
+            if (!P::B_REAL) aI[si][sj] = Mfma<T>::mma(ar[si], bi[sj], aI[si][sj]);
+            if (!P::A_REAL) aI[si][sj] = Mfma<T>::mma(ai[si], br[sj], aI[si][sj]);
+          }
+      }
+      if (!more) break;
+      fa.commit(tiles[2 * (cur ^ 1)]);
+      fb.commit(tiles[2 * (cur ^ 1) + 1]);
+      __syncthreads();
+      cur ^= 1;
+      t = tn;
+      k0 = kn;
+      ke = ken;
+    }
   }
 #pragma unroll
   for (int si = 0; si < 2; ++si)
@@ -203,7 +221,11 @@ struct RhsProb {
   __device__ int nterms(int b) const {
     return (b % 3 == 0 && (g.flags & PSA_F_OFFDIAGDPH)) ? 6 : 4;
   }
-  __device__ int kext(int, int) const { return g.N; }
+  __device__ bool active(int, int, int) const { return true; }
+  __device__ void krange(int, int, int, int, int &kb, int &ke) const {
+    kb = 0;
+    ke = g.N;
+  }
 
   // Operand descriptor: which matrix, how it is read (op(X)(x, k) = X[x][k]
   // when xm, else X[k][x]), conjugation, sign, zeroed diagonal, or a band.
@@ -360,7 +382,17 @@ struct CalcHProb {
   int w0;
 
   __device__ int nterms(int) const { return 1; }
-  __device__ int kext(int, int) const { return g.N; }
+  // output (a, dd) is used only when k1 = a + dd - (N-1) lies in [0, N)
+  __device__ bool active(int, int m0, int n0) const {
+    const int N = g.N;
+    return m0 + n0 <= 2 * N - 2 && m0 + n0 + (GBM - 1) + (GBN - 1) >= N - 1;
+  }
+  // B[m][dd] = X[m][m + dd - (N-1)] is non-zero only for m + dd in [N-1, 2N-1)
+  __device__ void krange(int, int, int, int n0, int &kb, int &ke) const {
+    const int N = g.N;
+    kb = max(0, N - 1 - (n0 + GBN - 1));
+    ke = min(N, 2 * N - 1 - n0);
+  }
 
   template <typename T>
   __device__ void fetchA(Frag<T> &s, int, int b, int m0, int k0) const {
@@ -467,11 +499,27 @@ struct OffdiagProb {
   psa_group g;
   int w0, nw;
 
-  __device__ int nterms(int) const { return 1; }
-  __device__ int kext(int, int) const { return 2 * g.N; }
+  // term 0: same-carrier p (K index p), valid for p - q in [1, N);
+  // term 1: other-carrier p (K index N + p), valid for p + q in [0, N).
+  // Rows k are used only when k + q lies in [0, N).
+  __device__ int nterms(int) const { return 2; }
+  __device__ bool active(int b, int m0, int) const {
+    const int N = g.N, q = b / 2 - (N - 1);
+    return m0 < min(N, N - q) && m0 + GBM > max(0, -q);
+  }
+  __device__ void krange(int t, int b, int, int, int &kb, int &ke) const {
+    const int N = g.N, q = b / 2 - (N - 1);
+    if (t == 0) {
+      kb = max(0, q + 1);
+      ke = min(N, N + q);
+    } else {
+      kb = N + max(0, -q);
+      ke = N + min(N, N - q);
+    }
+  }
 
   template <typename T>
-  __device__ void fetchA(Frag<T> &s, int, int b, int m0, int k0) const {
+  __device__ void fetchA(Frag<T> &s, int t, int b, int m0, int k0) const {
     const int N = g.N, q = b / 2 - (N - 1), kind = b % 2;
     const double *Es = kind == 0 ? g.Ee : g.Eh;  // carrier of row k
     const double *Eo = kind == 0 ? g.Eh : g.Ee;  // the other carrier
@@ -485,7 +533,7 @@ struct OffdiagProb {
       const int kr = m0 + x, pp = k0 + k;
       double v = 0.0;
       const int kq = kr + q;
-      if (kr < N && kq >= 0 && kq < N && pp < 2 * N) {
+      if (kr < N && kq >= 0 && kq < N && pp < 2 * N && (t == 0) == (pp < N)) {
         double Ed, hg;
         if (pp < N) {
           const int p = pp, pm = p - q;
@@ -508,7 +556,7 @@ struct OffdiagProb {
   }
 
   template <typename T>
-  __device__ void fetchB(Frag<T> &s, int, int b, int k0, int n0) const {
+  __device__ void fetchB(Frag<T> &s, int t, int b, int k0, int n0) const {
     const int N = g.N, q = b / 2 - (N - 1), kind = b % 2;
     const int aq = q > 0 ? q : -q;
     const int tid = threadIdx.x;
@@ -518,7 +566,7 @@ struct OffdiagProb {
       const int k = idx >> 6, x = idx & 63;
       const int pp = k0 + k, col = n0 + x;
       double v = 0.0;
-      if (pp < 2 * N && col < 2 * nw && aq > 0) {
+      if (pp < 2 * N && col < 2 * nw && aq > 0 && (t == 0) == (pp < N)) {
         const int w = w0 + (col >> 1), sA = col & 1;
         const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
         const double *ne = pr + PSA_PREP_ANE(N), *nh = pr + PSA_PREP_ANH(N);
@@ -573,7 +621,11 @@ struct Prop2qwProb {
   int w0, nw;
 
   __device__ int nterms(int) const { return 1; }
-  __device__ int kext(int, int) const { return g.gnx; }
+  __device__ bool active(int, int, int) const { return true; }
+  __device__ void krange(int, int, int, int, int &kb, int &ke) const {
+    kb = 0;
+    ke = g.gnx;
+  }
 
   template <typename T>
   __device__ void fetchA(Frag<T> &s, int, int, int m0, int k0) const {

@@ -276,79 +276,85 @@ __global__ __launch_bounds__(256) void k_h_simple(psa_group g, int w0) {
 // Many-body Coulomb scattering (MBCE/MBCH, coulomb.py:956-1071) over the
 // sparse lists of non-negligible Gaussian-delta entries.  List l holds
 // entries (i, j, l) of the stored table, grouped (CSR) by the rate index it
-// feeds (see the slot mapping below).  One block per
-// (slot, target k); wires in chunks of 8 accumulated in registers.
+// feeds (see the slot mapping below).  One block per (slot, target k); the
+// 64 lanes of a wave are 64 wires, so each list entry is read once per wave
+// (scalar loads) and its occupation / V^2 gathers are coalesced reads of
+// wire-fastest tables built by k_diag_gather:
+//   wmb + 4 N Nw: occ[2][N][Nw]  |Re ne3|, |Re nh3|
+//   wmb + 6 N Nw: vsq[3][N][Nw]  VC(d)^2 for eh, ee, hh
 // ===========================================================================
-__device__ __forceinline__ double vsq_lower(const double *vc, int a, int b) {
-  // VC[a][b]^2 with the lower-triangular table (zero for a < b)
-  if (a < b) return 0.0;
-  const double v = vc[a - b];
-  return v * v;
-}
+constexpr int MB_WAVES = 8;
 
-__global__ __launch_bounds__(256) void k_mb(psa_group g, int w0, int nw) {
-  __shared__ double red[4][8];
-  const int slot = blockIdx.y, k = blockIdx.x, N = g.N, tid = threadIdx.x;
-  const int lane = tid & 63, wid = tid >> 6;
-  for (int wc = 0; wc < nw; wc += 8) {
-    double acc[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+__global__ __launch_bounds__(64 * MB_WAVES) void k_mb(psa_group g, int w0, int nw) {
+  __shared__ double red[MB_WAVES][64];
+  const int slot = blockIdx.y, k = blockIdx.x, N = g.N, W = g.Nw;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const double *occ = g.wmb + 4L * N * W, *vsq = g.wmb + 6L * N * W;
+  for (int wc = 0; wc < nw; wc += 64) {
+    const int w = w0 + wc + lane;
+    const bool on = wc + lane < nw;
+    const double *ne = occ + w, *nh = occ + (long)N * W + w;
+    const double *v2 = vsq + w;
+    double acc = 0.0;
     for (int half = 0; half < 2; ++half) {
       // slot 0 WinE <- lists 0, 2; 1 WoutE <- 1, 3; 2 WinH <- 4, 6; 3 WoutH <- 5, 7
       const int L = (slot >> 1) * 4 + (slot & 1) + 2 * half;
       if (g.nnz_mb[L] == 0) continue;
       const int beg = g.mb_off[L][k], end = g.mb_off[L][k + 1];
-      for (int e = beg + tid; e < end; e += blockDim.x) {
-        const uint32_t pk = g.mb_idx[L][e];
+      const uint32_t *ix = g.mb_idx[L];
+      const double *vl = g.mb_val[L];
+      // V^2 channel: lists 0, 1, 4, 5 use Veh (5 with the index order swapped),
+      // 2, 3 Vee, 6, 7 Vhh
+      const int ch = (L == 2 || L == 3) ? 1 : ((L >= 6) ? 2 : 0);
+      const double *vv = v2 + (long)ch * N * W;
+      for (int e = beg + wv; e < end; e += MB_WAVES) {
+        const uint32_t pk = ix[e];
+        const double val = vl[e];
         const int i = pk & 1023, j = (pk >> 10) & 1023, l = (pk >> 20) & 1023;
-        const double val = g.mb_val[L][e];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int w = w0 + wc + u;
-          if (wc + u >= nw) break;
-          const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-          const double *vc = pr + PSA_PREP_VC(N);
-          const double *nn = g.wmb + (long)w * 6 * N;
-          const double *ne = nn + 4 * N, *nh = nn + 5 * N;
-          double t = 0.0;
-          switch (L) {
-            case 0: t = vsq_lower(vc, i, l) * (1.0 - nh[j]) * nh[j + l - i] * ne[l]; break;
-            case 1: t = vsq_lower(vc, i, l) * (1.0 - ne[i]) * (1.0 - nh[j]) * nh[j + l - i]; break;
-            case 2: t = vsq_lower(vc + N, i, l) * (1.0 - ne[j]) * ne[i + j - l] * ne[l]; break;
-            case 3: t = vsq_lower(vc + N, i, l) * (1.0 - ne[i]) * (1.0 - ne[j]) * ne[i + j - l]; break;
-            case 4: t = vsq_lower(vc, i, l) * (1.0 - ne[i]) * nh[j + l - i] * ne[l]; break;
-            case 5: t = vsq_lower(vc, l, i) * (1.0 - ne[i]) * (1.0 - nh[j]) * ne[l]; break;
-            case 6: t = vsq_lower(vc + 2 * N, i, l) * (1.0 - nh[j]) * nh[i + j - l] * nh[l]; break;
-            default: t = vsq_lower(vc + 2 * N, i, l) * (1.0 - nh[i]) * (1.0 - nh[j]) * nh[i + j - l]; break;
-          }
-          acc[u] += t * val;
+        const int a = (L == 5) ? l : i, c = (L == 5) ? i : l;
+        if (!on || a < c) continue;  // VC is lower-triangular: V(a, c) = 0 for a < c
+        const double vq = vv[(long)(a - c) * W];
+        double t;
+        switch (L) {
+          case 0: t = (1.0 - nh[(long)j * W]) * nh[(long)(j + l - i) * W] * ne[(long)l * W]; break;
+          case 1: t = (1.0 - ne[(long)i * W]) * (1.0 - nh[(long)j * W]) * nh[(long)(j + l - i) * W]; break;
+          case 2: t = (1.0 - ne[(long)j * W]) * ne[(long)(i + j - l) * W] * ne[(long)l * W]; break;
+          case 3: t = (1.0 - ne[(long)i * W]) * (1.0 - ne[(long)j * W]) * ne[(long)(i + j - l) * W]; break;
+          case 4: t = (1.0 - ne[(long)i * W]) * nh[(long)(j + l - i) * W] * ne[(long)l * W]; break;
+          case 5: t = (1.0 - ne[(long)i * W]) * (1.0 - nh[(long)j * W]) * ne[(long)l * W]; break;
+          case 6: t = (1.0 - nh[(long)j * W]) * nh[(long)(i + j - l) * W] * nh[(long)l * W]; break;
+          default: t = (1.0 - nh[(long)i * W]) * (1.0 - nh[(long)j * W]) * nh[(long)(i + j - l) * W]; break;
         }
+        acc += vq * t * val;
       }
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc[u] = wave_sum(acc[u]);
+    red[wv][lane] = acc;
     __syncthreads();
-    if (lane == 0)
+    if (wv == 0 && on) {
+      double s = 0.0;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) red[wid][u] = acc[u];
-    __syncthreads();
-    if (tid < 8 && wc + tid < nw) {
-      const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-      g.wmb[(long)(w0 + wc + tid) * 6 * N + slot * N + k] = s;
+      for (int u = 0; u < MB_WAVES; ++u) s += red[u][lane];
+      g.wmb[(long)w * 4 * N + slot * N + k] = s;
     }
+    __syncthreads();
   }
 }
 
-// |Re| of the new level's diagonals for the many-body sums (coulomb.py:1307-1310)
+// Wire-fastest occupation and V^2 tables for k_mb from the new level's
+// diagonals (|Re|, coulomb.py:1307-1310) and the screened VC(d).
 __global__ void k_diag_gather(psa_group g, int w0) {
-  const int w = w0 + blockIdx.x, N = g.N;
+  const int w = w0 + blockIdx.x, N = g.N, W = g.Nw;
   const long o = mat_off(g, w);
-  double *nn = g.wmb + (long)w * 6 * N;
+  double *occ = g.wmb + 4L * N * W, *vsq = g.wmb + 6L * N * W;
+  const double *vc = g.prep + (long)w * PSA_PREP_STRIDE(N) + PSA_PREP_VC(N);
   for (int k = threadIdx.x; k < N; k += blockDim.x) {
-    nn[4 * N + k] = fabs(ld(g.CX + o, (long)k * N + k).re);
-    nn[5 * N + k] = fabs(ld(g.DX + o, (long)k * N + k).re);
-    for (int s = 0; s < 4; ++s) nn[s * N + k] = 0.0;
+    occ[(long)k * W + w] = fabs(ld(g.CX + o, (long)k * N + k).re);
+    occ[(long)(N + k) * W + w] = fabs(ld(g.DX + o, (long)k * N + k).re);
+    for (int ch = 0; ch < 3; ++ch) {
+      const double v = vc[ch * N + k];
+      vsq[((long)ch * N + k) * W + w] = v * v;
+    }
   }
 }
 
@@ -361,7 +367,7 @@ __global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
   double *C3 = g.CX + o, *D3 = g.DX + o;
   const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
   const double *E1D = pr + PSA_PREP_E1D(N), *vc = pr + PSA_PREP_VC(N), *Rsp = pr + PSA_PREP_RSP(N);
-  const double *mb = g.wmb + (long)w * 6 * N;
+  const double *mb = g.wmb + (long)w * 4 * N;
   const bool ehs = (g.flags & PSA_F_EHS) != 0, phon = (g.flags & PSA_F_PHONON) != 0;
   const bool recomb = (g.flags & PSA_F_RECOMB) != 0;
   for (int k = tid; k < N; k += blockDim.x) {
@@ -426,52 +432,66 @@ __global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
 // ===========================================================================
 // Transport (dcfield.py:1176-1236): X <- ifft2(fft2(X) exp(-i(Y_a+Y_b) dk)).
 // The phase is separable, so it is a row pass and a column pass of
-// FFT -> phase -> inverse FFT, each in LDS (Stockham radix 2/3,
-// N = 2^a 3^b, twiddles from a per-N table).  One wave per line, four lines
-// per block, cooperative (coalesced) loads and stores of the four lines.
+// FFT -> phase -> inverse FFT, each in LDS (Stockham autosort, radix 4
+// stages while 4 divides the remaining length, then 2 and 3, N = 2^a 3^b,
+// twiddles from a per-N table).  One wave per line, TR_LINES lines per
+// block; the column pass reads and writes TR_LINES adjacent columns per row.
 // ===========================================================================
+constexpr int TR_LINES = 8;
+
 __device__ __forceinline__ cd tw_get(const cd *tw, int idx, int sign) {
   cd w = tw[idx];  // exp(-2 pi i idx / N)
   if (sign > 0) w.im = -w.im;
   return w;
 }
 
+// multiply by i * sign
+__device__ __forceinline__ cd mul_is(cd a, int sign) {
+  return sign > 0 ? cd{-a.im, a.re} : cd{a.im, -a.re};
+}
+
 // Stockham autosort FFT of one line held in LDS (x, ping-pong buffer y),
-// radix 2 and 3 stages, twiddles from the LDS table tw; result left in x.
+// result left in x.  Stage with radix R and sub-length Ns: for j < N/R,
+// k = j mod Ns, inputs x[j + r N/R] w^(r k N/(R Ns)), outputs at
+// (j - k) R + k + s Ns.  sign -1 forward, +1 inverse (unscaled).
 __device__ void fft_line(cd *x, cd *y, const cd *tw, int N, int sign, int lane) {
   int Ns = 1;
   cd *src = x, *dst = y;
   int rem = N;
   while (rem > 1) {
-    if ((rem & 1) == 0) {
-      const int M = N >> 1, step = N / (Ns * 2);
-      for (int j = lane; j < M; j += 64) {
-        const int k = j % Ns;
-        const cd a = src[j];
+    const int R = (rem % 4 == 0) ? 4 : ((rem % 2 == 0) ? 2 : 3);
+    const int M = N / R, step = N / (Ns * R);
+    const uint64_t mg = ((1ull << 32) + Ns - 1) / Ns;  // j / Ns = (j mg) >> 32 for j < 2^20
+    for (int j = lane; j < M; j += 64) {
+      const int k = j - Ns * (int)(((uint64_t)j * mg) >> 32);
+      const int base = (j - k) * R + k;
+      const cd a0 = src[j];
+      if (R == 4) {
+        const cd a1 = cmul(src[j + M], tw_get(tw, k * step, sign));
+        const cd a2 = cmul(src[j + 2 * M], tw_get(tw, 2 * k * step, sign));
+        const cd a3 = cmul(src[j + 3 * M], tw_get(tw, 3 * k * step, sign));
+        const cd s02 = cadd(a0, a2), d02 = csub(a0, a2);
+        const cd s13 = cadd(a1, a3), d13 = mul_is(csub(a1, a3), sign);
+        dst[base] = cadd(s02, s13);
+        dst[base + Ns] = cadd(d02, d13);
+        dst[base + 2 * Ns] = csub(s02, s13);
+        dst[base + 3 * Ns] = csub(d02, d13);
+      } else if (R == 2) {
         const cd b = cmul(src[j + M], tw_get(tw, k * step, sign));
-        const int base = (j - k) * 2 + k;
-        dst[base] = cadd(a, b);
-        dst[base + Ns] = csub(a, b);
-      }
-      Ns *= 2;
-      rem >>= 1;
-    } else {
-      const int M = N / 3, step = N / (Ns * 3);
-      const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
-      const cd w1 = {c3, s3}, w2 = {c3, -s3};
-      for (int j = lane; j < M; j += 64) {
-        const int k = j % Ns;
-        const cd a = src[j];
+        dst[base] = cadd(a0, b);
+        dst[base + Ns] = csub(a0, b);
+      } else {
+        const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
+        const cd w1 = {c3, s3}, w2 = {c3, -s3};
         const cd b = cmul(src[j + M], tw_get(tw, k * step, sign));
         const cd c = cmul(src[j + 2 * M], tw_get(tw, 2 * k * step, sign));
-        const int base = (j - k) * 3 + k;
-        dst[base] = cadd(a, cadd(b, c));
-        dst[base + Ns] = cadd(a, cadd(cmul(b, w1), cmul(c, w2)));
-        dst[base + 2 * Ns] = cadd(a, cadd(cmul(b, w2), cmul(c, w1)));
+        dst[base] = cadd(a0, cadd(b, c));
+        dst[base + Ns] = cadd(a0, cadd(cmul(b, w1), cmul(c, w2)));
+        dst[base + 2 * Ns] = cadd(a0, cadd(cmul(b, w2), cmul(c, w1)));
       }
-      Ns *= 3;
-      rem /= 3;
     }
+    Ns *= R;
+    rem /= R;
     __syncthreads();
     cd *tmp = src;
     src = dst;
@@ -496,54 +516,55 @@ __global__ void k_tphase(psa_group g, int w0, double dt) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_transport(psa_group g, int w0, int pass) {
+// pass 0: rows, pass 1: columns.  With avg set (column pass only), also
+// writes the leapfrog average A = (A + X3) / 2 (SBEs.py:5704-5706) for
+// every element except the diagonals of C and D, which the e/h
+// renormalisation still changes (k_finish_diag averages those).
+__global__ __launch_bounds__(64 * TR_LINES) void k_transport(psa_group g, int w0, int pass, int avg) {
   extern __shared__ double smem[];
   const int N = g.N, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int mat = blockIdx.y;  // 0 C, 1 D, 2 p
   const int w = w0 + blockIdx.z;
-  const int line0 = blockIdx.x * 4;
+  const int line0 = blockIdx.x * TR_LINES;
   cd *base = reinterpret_cast<cd *>(smem);
-  cd *tw = base + 4 * 2 * N;  // [N] twiddles
+  cd *tw = base + TR_LINES * 2 * N;  // [N] twiddles
   for (int j = tid; j < N; j += blockDim.x) tw[j] = ld(g.twid, j);
   const long o = mat_off(g, w);
   double *X = (mat == 0 ? g.CX : (mat == 1 ? g.DX : g.pX)) + o;
-  // cooperative load of 4 lines
-  for (int idx = tid; idx < 4 * N; idx += blockDim.x) {
-    int c, j;
-    if (pass == 0) {
-      c = idx / N;
-      j = idx % N;
-    } else {
-      c = idx & 3;
-      j = idx >> 2;
+  double *A = (mat == 0 ? g.CA : (mat == 1 ? g.DA : g.pA)) + o;
+  const int line = line0 + wv;
+  cd *x = base + wv * 2 * N, *y = x + N;
+  if (pass == 0) {
+    if (line < N)
+      for (int j = lane; j < N; j += 64) x[j] = ld(X, (long)line * N + j);
+  } else {
+    for (int idx = tid; idx < TR_LINES * N; idx += blockDim.x) {
+      const int c = idx % TR_LINES, j = idx / TR_LINES;
+      if (line0 + c < N) base[c * 2 * N + j] = ld(X, (long)j * N + line0 + c);
     }
-    const int line = line0 + c;
-    if (line < N) base[c * 2 * N + j] = pass == 0 ? ld(X, (long)line * N + j) : ld(X, (long)j * N + line);
   }
   __syncthreads();
-  cd *x = base + wv * 2 * N, *y = x + N;
   fft_line(x, y, tw, N, -1, lane);
   const double *ph = g.tphase + (long)w * 2 * N;
   for (int j = lane; j < N; j += 64) x[j] = cmul(x[j], ld(ph, j));
   __syncthreads();
   fft_line(x, y, tw, N, +1, lane);
   const double inv = 1.0 / N;
-  for (int idx = tid; idx < 4 * N; idx += blockDim.x) {
-    int c, j;
-    if (pass == 0) {
-      c = idx / N;
-      j = idx % N;
-    } else {
-      c = idx & 3;
-      j = idx >> 2;
-    }
-    const int line = line0 + c;
-    if (line < N) {
-      const cd v = cscale(base[c * 2 * N + j], inv);
-      if (pass == 0)
-        st(X, (long)line * N + j, v);
-      else
-        st(X, (long)j * N + line, v);
+  if (pass == 0) {
+    if (line < N)
+      for (int j = lane; j < N; j += 64) st(X, (long)line * N + j, cscale(x[j], inv));
+  } else {
+    for (int idx = tid; idx < TR_LINES * N; idx += blockDim.x) {
+      const int c = idx % TR_LINES, j = idx / TR_LINES, col = line0 + c;
+      if (col < N) {
+        const long ij = (long)j * N + col;
+        const cd v = cscale(base[c * 2 * N + j], inv);
+        st(X, ij, v);
+        if (avg && !(mat < 2 && j == col)) {
+          const cd a = ld(A, ij);
+          st(A, ij, cd{(a.re + v.re) / 2.0, (a.im + v.im) / 2.0});
+        }
+      }
     }
   }
 }
@@ -551,7 +572,7 @@ __global__ __launch_bounds__(256) void k_transport(psa_group g, int w0, int pass
 // ===========================================================================
 // finish: diagonal |.| and e/h renormalisation (SBEs.py:5692-5702)
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_finish_diag(psa_group g, int w0) {
+__global__ __launch_bounds__(256) void k_finish_diag(psa_group g, int w0, int avg) {
   __shared__ double sh[8];
   const int w = w0 + blockIdx.x, N = g.N, tid = threadIdx.x;
   const long o = mat_off(g, w);
@@ -565,8 +586,14 @@ __global__ __launch_bounds__(256) void k_finish_diag(psa_group g, int w0) {
   const double total = (Se + Sh) / 2.0;
   for (int k = tid; k < N; k += blockDim.x) {
     const double a = cabs_(ld(C3, (long)k * N + k)), b = cabs_(ld(D3, (long)k * N + k));
-    st(C3, (long)k * N + k, cd{a * total / (Se + 1e-200), 0.0});
-    st(D3, (long)k * N + k, cd{b * total / (Sh + 1e-200), 0.0});
+    const cd c3 = {a * total / (Se + 1e-200), 0.0}, d3 = {b * total / (Sh + 1e-200), 0.0};
+    st(C3, (long)k * N + k, c3);
+    st(D3, (long)k * N + k, d3);
+    if (avg) {  // diagonal part of the leapfrog average (the rest was fused into Transport)
+      const cd ca = ld(g.CA + o, (long)k * N + k), da = ld(g.DA + o, (long)k * N + k);
+      st(g.CA + o, (long)k * N + k, cd{(ca.re + c3.re) / 2.0, (ca.im + c3.im) / 2.0});
+      st(g.DA + o, (long)k * N + k, cd{(da.re + d3.re) / 2.0, (da.im + d3.im) / 2.0});
+    }
   }
 }
 
@@ -888,7 +915,7 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
   hipLaunchKernelGGL(k_diag_gather, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_diag_gather");
   if (g->flags & PSA_F_EHS) {
-    hipLaunchKernelGGL(k_mb, dim3(g->N, 4), dim3(256), 0, s, *g, w0, nw);
+    hipLaunchKernelGGL(k_mb, dim3(g->N, 4), dim3(64 * MB_WAVES), 0, s, *g, w0, nw);
     PSA_LAUNCH_CHECK("k_mb");
   }
   hipLaunchKernelGGL(k_relax, dim3(nw), dim3(256), 0, s, *g, w0, dt);
@@ -896,19 +923,20 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
   return PSA_OK;
 }
 
-int psa_transport(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
-  if (int rc = check_group(g, w0, nw)) return rc;
-  if (!(g->flags & PSA_F_DCTRANS)) return PSA_OK;
+}  // extern "C"
+
+namespace {
+
+int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStream_t s, bool avg) {
   if (!(g->flags & PSA_F_LF)) return fail(PSA_E_UNSUPPORTED, "Transport with LF = 0 (diagonal-only shift) is not supported");
   if (g->N > 512) return fail(PSA_E_UNSUPPORTED, "Transport needs N <= 512 (LDS lines)");
   int n = g->N;
   while (n % 2 == 0) n /= 2;
   while (n % 3 == 0) n /= 3;
   if (n != 1) return fail(PSA_E_UNSUPPORTED, "Transport needs N = 2^a 3^b");
-  const hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_tphase, dim3(nw), dim3(256), 0, s, *g, w0, dt);
   PSA_LAUNCH_CHECK("k_tphase");
-  const size_t shm = 9 * (size_t)g->N * sizeof(cd);  // 4 lines x 2 buffers + twiddles
+  const size_t shm = (2 * TR_LINES + 1) * (size_t)g->N * sizeof(cd);  // lines x 2 buffers + twiddles
   if (shm > 65536) {
     static bool attr_set = false;
     if (!attr_set) {
@@ -919,23 +947,38 @@ int psa_transport(const psa_group *g, int32_t w0, int32_t nw, double dt, void *s
       attr_set = true;
     }
   }
-  for (int pass = 0; pass < 2; ++pass) {
-    hipLaunchKernelGGL(k_transport, dim3((g->N + 3) / 4, 3, nw), dim3(256), shm, s, *g, w0, pass);
-    PSA_LAUNCH_CHECK("k_transport");
-  }
+  const dim3 grid((g->N + TR_LINES - 1) / TR_LINES, 3, nw);
+  hipLaunchKernelGGL(k_transport, grid, dim3(64 * TR_LINES), shm, s, *g, w0, 0, 0);
+  PSA_LAUNCH_CHECK("k_transport");
+  hipLaunchKernelGGL(k_transport, grid, dim3(64 * TR_LINES), shm, s, *g, w0, 1, avg ? 1 : 0);
+  PSA_LAUNCH_CHECK("k_transport");
   return PSA_OK;
 }
 
-int psa_finish(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
-  if (int rc = check_group(g, w0, nw)) return rc;
-  const hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_finish_diag, dim3(nw), dim3(256), 0, s, *g, w0);
+int finish_impl(const psa_group *g, int32_t w0, int32_t nw, hipStream_t s, bool averaged) {
+  hipLaunchKernelGGL(k_finish_diag, dim3(nw), dim3(256), 0, s, *g, w0, averaged ? 1 : 0);
   PSA_LAUNCH_CHECK("k_finish_diag");
+  if (averaged) return PSA_OK;
   const long total = (long)g->N * g->N * nw * 3;
   const int blocks = (int)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
   hipLaunchKernelGGL(k_average, dim3(blocks), dim3(256), 0, s, *g, w0, nw);
   PSA_LAUNCH_CHECK("k_average");
   return PSA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psa_transport(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  if (!(g->flags & PSA_F_DCTRANS)) return PSA_OK;
+  return transport_impl(g, w0, nw, dt, (hipStream_t)stream, false);
+}
+
+int psa_finish(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  return finish_impl(g, w0, nw, (hipStream_t)stream, false);
 }
 
 int psa_sources(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
@@ -982,8 +1025,11 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
     if ((rc = psa_offdiag_tables(g, w0, nw, stream))) return rc;
   if ((rc = psa_rhs(g, w0, nw, 1, dt, stream))) return rc;
   if ((rc = psa_relaxation(g, w0, nw, dt, stream))) return rc;
-  if ((rc = psa_transport(g, w0, nw, dt, stream))) return rc;
-  if ((rc = psa_finish(g, w0, nw, stream))) return rc;
+  // Transport's column pass also writes the leapfrog average (all but the
+  // C/D diagonals, which finish averages after the renormalisation)
+  const bool fused = (g->flags & PSA_F_DCTRANS) != 0;
+  if (fused && (rc = transport_impl(g, w0, nw, dt, (hipStream_t)stream, true))) return rc;
+  if ((rc = finish_impl(g, w0, nw, (hipStream_t)stream, fused))) return rc;
   if ((rc = psa_sources(g, w0, nw, stream))) return rc;
   if ((rc = psa_qw2prop(g, w0, nw, stream))) return rc;
   return PSA_OK;

@@ -86,7 +86,7 @@ class WireGroup:
         self.s["hdiag"] = z(nwz, 2, N)
         self.s["prep"] = z(nwz, _prep_stride(N), dt=F64)
         self.s["dedh"] = z(nwz, 2, N, 2 * N - 1, dt=F64)
-        self.s["wmb"] = z(nwz, 6, N, dt=F64)
+        self.s["wmb"] = z(9 * nwz * N, dt=F64)  # [Nw][4][N] rates, occ [2][N][Nw], V^2 [3][N][Nw]
         self.s["exq"] = z(nwz, 3, 2 * N - 1)
         self.s["efield"] = z(3, nwz, Nx)
         self.s["src"] = z(nwz, _src_stride(N, Nr), dt=F64)

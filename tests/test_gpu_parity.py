@@ -216,6 +216,49 @@ def test_step_vs_oracle_n64():
             assert rel(v, r) < TRAJ_TOL, it
 
 
+@pytest.mark.parametrize("N", [48, 64, 96, 192])
+def test_transport_stage(N):
+    """Transport alone (row and column FFT passes, radix 4/2/3 stages) on
+    dense random p, C, D against fft2 -> phase -> ifft2."""
+    import torch
+    c = Case(N, 2, seed=31 + N)
+    c.wg.prop2qw()
+    rng = np.random.default_rng(N)
+    X = {k: rng.standard_normal((2, N, N)) + 1j * rng.standard_normal((2, N, N))
+         for k in ("CX", "DX", "pX")}
+    for k, v in X.items():
+        c.wg.s[k].copy_(torch.as_tensor(v))
+    dt = 1e-14
+    c.wg.transport(dt)
+    torch.cuda.synchronize()
+    exq = c.wg.s["exq"].cpu().numpy()
+    for w in range(2):
+        eac = exq[w, 0, N - 1].real
+        assert abs(eac) > 0
+        for k, v in X.items():
+            ref = O.transport(c.ot, v[w], c.og.prm.Edc, eac, dt)
+            assert rel(c.wg.s[k][w].cpu().numpy(), ref) < 1e-13, (k, w)
+
+
+def test_step_vs_oracle_n48_radix3():
+    """Three full steps at N = 48 (transport with a radix-3 stage, the
+    leapfrog average fused into the transport column pass)."""
+    import torch
+    c = Case(48, 2, seed=9)
+    dt = 10e-18
+    for it in range(3):
+        c.wg.step(dt)
+        torch.cuda.synchronize()
+        Pxx, Pyy, Pzz, Rho = c.wg.outputs()
+        for w in range(c.nw):
+            ref = c.os.qw_calculator(c.Exx, c.Eyy, c.Ezz, dt, w + 1)
+            assert rel(Pxx[w], ref[0]) < TRAJ_TOL, (it, w)
+            assert rel(Rho[w], ref[3]) < TRAJ_TOL, (it, w)
+        got = c.wg.get_state()
+        for v, r in zip(got, (c.os.CC2, c.os.DD2, c.os.YY2, c.os.CC3, c.os.DD3, c.os.YY3)):
+            assert rel(v, r) < TRAJ_TOL, it
+
+
 def test_mixed_precision_fp32_error_bounded():
     """FP32-MFMA contractions (operands rounded to FP32, FP32 accumulation;
     diagonals, epilogues and state in FP64) against the FP64 path after
