@@ -75,6 +75,7 @@ typedef struct psa_group {
   const double *chi;                  /* [2][N] Chi1De(d), Chi1Dh(d) */
   const double *qeh;                  /* [2][N] qe(d), qh(d) */
   const double *EP, *HP;              /* [N][N] phonon tables */
+  const double *EPT, *HPT;            /* [N][N] their transposes */
   const double *xe, *xh;              /* [N] dephasing weights */
   const int32_t *didx;                /* [6][N][N] k_p_q, k_m_q, k1p_m_q, k1p, k1_m_q, k1 */
   const int32_t *iq;                  /* [N] Vxx2 index rint(|ky/dq|) */
@@ -86,6 +87,9 @@ typedef struct psa_group {
   const double *m2;                   /* [Nr][bnx] real: QW2Prop spline r -> rr (transposed) */
   const double *phi;                  /* [2N-1][Nr] complex exp(i d dkr r) */
   const double *twid;                 /* [N] complex exp(-2 pi i j / N) */
+  double *lor;                        /* [2][N*N][2N] Lorentzian kernels of CalcOffDiagDeph_E2/H2 for
+                                         De, Dh: rows (q, k) packed per q over the N-|q| rows with
+                                         k+q in [0, N); filled once by psa_lorentz_table */
   const int32_t *mb_off[8];           /* CSR offsets [N+1] per many-body list */
   const uint32_t *mb_idx[8];          /* packed (i, j, l) per entry */
   const double *mb_val[8];            /* table value per entry */
@@ -94,7 +98,8 @@ typedef struct psa_group {
   double *pA, *pB, *pX;               /* [Nw][N][N] complex: YY2, YY3, new level */
   double *CA, *CB, *CX;
   double *DA, *DB, *DX;
-  double *Heh, *Heo, *Hho;            /* [Nw][N][N] complex: Heh, Hee/Hhh without diagonal */
+  double *Heh, *Heo, *Hho;            /* [Nw][N][N] complex, TRANSPOSED: element [k2][k1] holds
+                                         Heh[k1][k2] (resp. Hee, Hhh without their diagonal) */
   double *hdiag;                      /* [Nw][2][N] complex: diagonals of Hee, Hhh */
   double *prep;                       /* [Nw][PSA_PREP_STRIDE(N)] doubles, see PSA_PREP_* */
   double *dedh;                       /* [Nw][2][2N-1][N] off-diagonal dephasing tables */
@@ -158,6 +163,12 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream);
 /* Off-diagonal dephasing tables De, Dh: CalcOffDiagDeph_E2/H2
  * (dephasing.py:226-317, 381-414) as used by OffDiagDephasing2 (714-743). */
 int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream);
+
+/* The state-independent Lorentzian factors Lrtz(dE, hbar g) of
+ * CalcOffDiagDeph_E2/H2 (dephasing.py:226-317; Lrtz dephasing.py:220-224),
+ * for every (q, k, p), into g->lor.  Init-time: call once per group after
+ * the energy tables are set (psa_offdiag_tables reads it). */
+int psa_lorentz_table(const psa_group *g, void *stream);
 
 /* dpdt/dCdt/dDdt (SBEs.py:1941-1965, 2308-2327, 2606-2625) incl. the
  * off-diagonal dephasing OffP (_OffDiagDephasing2_jit, dephasing.py:320-356).

@@ -20,8 +20,8 @@ c_ptr = ctypes.c_void_p
 _INTS = ["N", "Nr", "Nw", "Nx", "Nhw", "flags", "prec", "gx0", "gnx", "bx0", "bnx"]
 _DBLS = ["hbar", "e0", "gap", "L", "ehint", "dcv", "area", "dkr", "dr_r", "drr",
          "gam_e", "gam_h", "gam_eh", "Edc", "Vscale", "n1d_max", "dhw"]
-_TABS = ["kr", "Ee", "Eh", "vtab", "chi", "qeh", "EP", "HP", "xe", "xh", "didx",
-         "iq", "homega", "hsq", "window", "rq", "ytr", "gmat", "m2", "phi", "twid"]
+_TABS = ["kr", "Ee", "Eh", "vtab", "chi", "qeh", "EP", "HP", "EPT", "HPT", "xe", "xh", "didx",
+         "iq", "homega", "hsq", "window", "rq", "ytr", "gmat", "m2", "phi", "twid", "lor"]
 _WIRE = ["wire_id", "pA", "pB", "pX", "CA", "CB", "CX", "DA", "DB", "DX", "Heh",
          "Heo", "Hho", "hdiag", "prep", "dedh", "wmb", "exq", "efield", "src",
          "out", "obs", "tphase", "part", "wid_host"]
@@ -49,6 +49,7 @@ SIGNATURES = {
                                       c_dbl, c_dbl, c_ptr, c_ptr]),
     "psa_prep": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_calc_h": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
+    "psa_lorentz_table": (ctypes.c_int, [_GP, c_ptr]),
     "psa_offdiag_tables": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_rhs": (ctypes.c_int, [_GP, c_i32, c_i32, c_i32, c_dbl, c_ptr]),
     "psa_relaxation": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),

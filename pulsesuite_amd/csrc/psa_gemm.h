@@ -118,7 +118,22 @@ __global__ __launch_bounds__(256) void gemm_kernel(P p) {
   Tile<T> *tiles = reinterpret_cast<Tile<T> *>(lds_raw);  // [2 buffers][A, B]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int b = blockIdx.z, m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+  // XCD-aware tile order: consecutive workgroups go to different XCDs
+  // (round robin over 8).  When the batch count is a multiple of 8, batch b
+  // runs entirely on XCD b % 8 (its tiles share that XCD's L2 for their
+  // operands) and neighbouring batches still spread over all XCDs.
+  const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
+  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  int b;
+  if (gridDim.z % 8 == 0) {
+    const unsigned xcd = lin % 8, q = lin / 8;
+    b = xcd + 8 * (q / tpb);
+    lin = q % tpb;
+  } else {
+    b = lin / tpb;
+    lin %= tpb;
+  }
+  const int m0 = (lin / gx) * GBM, n0 = (lin % gx) * GBN;
   using acc_t = typename Mfma<T>::acc;
   acc_t aR[2][2], aI[2][2];
 #pragma unroll
@@ -164,16 +179,30 @@ __global__ __launch_bounds__(256) void gemm_kernel(P p) {
           if (!P::A_REAL) ai[s] = sA.v[1][kk][wm * 32 + s * 16 + xl];
           if (!P::B_REAL) bi[s] = sB.v[1][kk][wn * 32 + s * 16 + xl];
         }
+        // four phases over the 2x2 sub-blocks, so that two MFMAs on the
+        // same accumulator are eight issues apart (no dependency stall)
 #pragma unroll
         for (int si = 0; si < 2; ++si)
 #pragma unroll
-          for (int sj = 0; sj < 2; ++sj) {
-            aR[si][sj] = Mfma<T>::mma(ar[si], br[sj], aR[si][sj]);
-            if (!P::A_REAL && !P::B_REAL)
-              aR[si][sj] = Mfma<T>::mma(-ai[si], bi[sj], aR[si][sj]);
-            if (!P::B_REAL) aI[si][sj] = Mfma<T>::mma(ar[si], bi[sj], aI[si][sj]);
-            if (!P::A_REAL) aI[si][sj] = Mfma<T>::mma(ai[si], br[sj], aI[si][sj]);
-          }
+          for (int sj = 0; sj < 2; ++sj) aR[si][sj] = Mfma<T>::mma(ar[si], br[sj], aR[si][sj]);
+        if (!P::B_REAL) {
+#pragma unroll
+          for (int si = 0; si < 2; ++si)
+#pragma unroll
+            for (int sj = 0; sj < 2; ++sj) aI[si][sj] = Mfma<T>::mma(ar[si], bi[sj], aI[si][sj]);
+        }
+        if (!P::A_REAL && !P::B_REAL) {
+#pragma unroll
+          for (int si = 0; si < 2; ++si)
+#pragma unroll
+            for (int sj = 0; sj < 2; ++sj) aR[si][sj] = Mfma<T>::mma(-ai[si], bi[sj], aR[si][sj]);
+        }
+        if (!P::A_REAL) {
+#pragma unroll
+          for (int si = 0; si < 2; ++si)
+#pragma unroll
+            for (int sj = 0; sj < 2; ++sj) aI[si][sj] = Mfma<T>::mma(ai[si], br[sj], aI[si][sj]);
+        }
       }
       if (!more) break;
       fa.commit(tiles[2 * (cur ^ 1)]);
@@ -229,6 +258,8 @@ struct RhsProb {
 
   // Operand descriptor: which matrix, how it is read (op(X)(x, k) = X[x][k]
   // when xm, else X[k][x]), conjugation, sign, zeroed diagonal, or a band.
+  // The Hamiltonians are stored transposed (buffer [k2][k1] = H[k1][k2]),
+  // so their descriptors read the transpose of what the comment names.
   struct Op {
     const double *X;
     int band;  // -1 dense, else band table kind (0 De, 1 Dh)
@@ -240,20 +271,20 @@ struct RhsProb {
     const double *P = g.pB + o, *C = g.CB + o, *D = g.DB + o;
     const double *He = g.Heh + o, *Ho = g.Heo + o, *Hh = g.Hho + o;
     switch (e * 8 + t) {
-      case 0: return {Hh, -1, true, false, false, 1.0};     // dp: Hhh_o
+      case 0: return {Hh, -1, false, false, false, 1.0};    // dp: Hhh_o
       case 1: return {P, -1, true, false, false, 1.0};      //     p
-      case 2: return {He, -1, false, false, false, -1.0};   //     -Heh^T
+      case 2: return {He, -1, true, false, false, -1.0};    //     -Heh^T
       case 3: return {D, -1, false, false, true, -1.0};     //     -D_o^T
       case 4: return {nullptr, 1, true, false, false, 1.0}; //     i hbar Dh band
       case 5: return {P, -1, true, false, false, 1.0};      //     p
       case 8: return {C, -1, true, false, true, 1.0};       // dC: C_o
-      case 9: return {Ho, -1, false, false, false, -1.0};   //     -Hee_o^T
+      case 9: return {Ho, -1, true, false, false, -1.0};    //     -Hee_o^T
       case 10: return {P, -1, false, true, false, 1.0};     //     p^H
-      case 11: return {He, -1, true, true, false, -1.0};    //     -conj(Heh)
+      case 11: return {He, -1, false, true, false, -1.0};   //     -conj(Heh)
       case 16: return {D, -1, true, false, true, 1.0};      // dD: D_o
-      case 17: return {Hh, -1, false, false, false, -1.0};  //     -Hhh_o^T
+      case 17: return {Hh, -1, true, false, false, -1.0};   //     -Hhh_o^T
       case 18: return {P, -1, true, true, false, 1.0};      //     conj(p)
-      default: return {He, -1, false, true, false, -1.0};   //     -Heh^H
+      default: return {He, -1, true, true, false, -1.0};    //     -Heh^H
     }
   }
 
@@ -263,18 +294,18 @@ struct RhsProb {
     // op(B)[k][x] = X[x][k] when xm
     switch (e * 8 + t) {
       case 0: return {P, -1, false, false, false, 1.0};     // dp: p
-      case 1: return {Ho, -1, true, false, false, 1.0};     //     Hee_o^T
+      case 1: return {Ho, -1, false, false, false, 1.0};    //     Hee_o^T
       case 2: return {C, -1, false, false, true, 1.0};      //     C_o
-      case 3: return {He, -1, true, false, false, 1.0};     //     Heh^T
+      case 3: return {He, -1, false, false, false, 1.0};    //     Heh^T
       case 4: return {P, -1, false, false, false, 1.0};     //     p
       case 5: return {nullptr, 0, true, false, false, 1.0}; //     i hbar De band
-      case 8: return {Ho, -1, true, false, false, 1.0};     // dC: Hee_o^T
+      case 8: return {Ho, -1, false, false, false, 1.0};    // dC: Hee_o^T
       case 9: return {C, -1, false, false, true, 1.0};      //     C_o
-      case 10: return {He, -1, true, false, false, 1.0};    //     Heh^T
+      case 10: return {He, -1, false, false, false, 1.0};   //     Heh^T
       case 11: return {P, -1, false, false, false, 1.0};    //     p
-      case 16: return {Hh, -1, true, false, false, 1.0};    // dD: Hhh_o^T
+      case 16: return {Hh, -1, false, false, false, 1.0};   // dD: Hhh_o^T
       case 17: return {D, -1, false, false, true, 1.0};     //     D_o
-      case 18: return {He, -1, false, false, false, 1.0};   //     Heh
+      case 18: return {He, -1, true, false, false, 1.0};    //     Heh
       default: return {P, -1, true, false, false, 1.0};     //     p^T
     }
   }
@@ -335,7 +366,7 @@ struct RhsProb {
       // Pauli factor Heh[ke,kh] (1 - ne[ke] - nh[kh]) from -Heh^T C - (Heh D)^T.
       const cd p = ld(g.pB + o, ij);
       const cd he = ld(hd, col), hh = ld(hd, N + row);
-      const cd heh = ld(g.Heh + o, (long)col * N + row);
+      const cd heh = ld(g.Heh + o, ij);  // Heh[ke][kh], stored transposed
       const cd ne = ld(pr + PSA_PREP_NE, col), nh = ld(pr + PSA_PREP_NH(N), row);
       const cd pauli = {1.0 - ne.re - nh.re, -ne.im - nh.im};
       cd z = cadd(acc, cmul(cadd(he, hh), p));
@@ -352,7 +383,7 @@ struct RhsProb {
       const cd h2 = ld(hd, sl + col), h1 = ld(hd, sl + row);
       const double *nn = pr + (e == 1 ? PSA_PREP_NE : PSA_PREP_NH(N));
       const cd n1 = ld(nn, row), n2 = ld(nn, col);
-      const cd hot = (row == col) ? cd{0.0, 0.0} : ld(Ho, (long)col * N + row);
+      const cd hot = (row == col) ? cd{0.0, 0.0} : ld(Ho, ij);  // H_o[k2][k1], stored transposed
       cd z = cadd(acc, cmul(csub(h2, h1), x));
       z = cadd(z, cmul(csub(n1, n2), hot));
       d = cdiv_ih(z, hbar);
@@ -436,7 +467,7 @@ struct CalcHProb {
     if (k1 < 0 || k1 >= N) return;
     const int which = b % 3, w = w0 + b / 3;
     const long o = mat_off(g, w);
-    const long ij = (long)k1 * N + k2;
+    const long ij = (long)k2 * N + k1;  // stored transposed: consecutive lanes, consecutive k1
     const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
     const bool lf = (g.flags & PSA_F_LF) != 0;
     double *hd = g.hdiag + (long)w * 4 * N;
@@ -486,14 +517,48 @@ struct CalcHProb {
 // ===========================================================================
 // Problem: off-diagonal dephasing tables (CalcOffDiagDeph_E2/H2,
 // dephasing.py:226-317) batched over wires.  For fixed q the sums over p
-// are a product of a wire-independent Lorentzian kernel L_q[k][p] with
-// occupation vectors of every wire:
+// are a product of a wire-independent Lorentzian kernel L_q[k][p] (built
+// once, psa_lorentz_table) with occupation vectors of every wire:
 //   De[q][k] = pi/hbar ( (1-ne_k) R_A + ne_k R_B ),
 //   R_s = sum_p Lee(q,k,p) Vee2(q) a_s(p) + sum_p Leh(q,k,p) Veh2(q) b_s(p).
 // K runs over [same-carrier p | other-carrier p] (2N); columns are
 // (wire, s) pairs, s in {A, B}.  batch b -> qi = b / 2, kind = b % 2
 // (0 = De for electrons, 1 = Dh for holes).
 // ===========================================================================
+// Packed row of (q index qi = q + N - 1, k) in the Lorentzian table: rows
+// of q hold k in [max(0,-q), min(N,N-q)), N - |q| of them; this is the
+// first row of qi.
+__host__ __device__ inline long lorentz_row0(int N, int qi) {
+  if (qi <= N - 1) return (long)qi * (qi + 1) / 2;
+  const long m = qi - N;
+  return (long)N * (N + 1) / 2 + m * (N - 1) - m * (m - 1) / 2;
+}
+
+// Lrtz(Ed, hbar g) = (hbar g / pi) / (Ed^2 + (hbar g)^2) of
+// CalcOffDiagDeph_E2/H2 (dephasing.py:226-317) for kind 0 (De: rows are
+// electrons, gee = gam_eh, geh = gam_h) or 1 (Dh: rows are holes,
+// ghh = gam_e, geh = gam_h), K index pp: same carrier p = pp (needs p-q in
+// [1, N)), other carrier p = pp - N (needs p+q in [0, N)); 0 elsewhere.
+__device__ inline double lorentz_value(const psa_group &g, int kind, int q, int kr, int pp) {
+  const int N = g.N, kq = kr + q;
+  if (kr < 0 || kr >= N || kq < 0 || kq >= N || pp < 0 || pp >= 2 * N) return 0.0;
+  const double *Es = kind == 0 ? g.Ee : g.Eh;  // carrier of row k
+  const double *Eo = kind == 0 ? g.Eh : g.Ee;  // the other carrier
+  double Ed, hg;
+  if (pp < N) {
+    const int p = pp, pm = p - q;
+    if (pm < 1 || pm >= N) return 0.0;
+    Ed = Es[kq] + Es[pm] - Es[p] - Es[kr];
+    hg = g.hbar * (kind == 0 ? g.gam_eh : g.gam_e);
+  } else {
+    const int p = pp - N, pq = p + q;
+    if (pq < 0 || pq >= N) return 0.0;
+    Ed = Es[kq] + Eo[pq] - Eo[p] - Es[kr];
+    hg = g.hbar * g.gam_h;
+  }
+  return (hg / PI) / (Ed * Ed + hg * hg);
+}
+
 struct OffdiagProb {
   static constexpr bool A_REAL = true, B_REAL = true;
   psa_group g;
@@ -518,39 +583,23 @@ struct OffdiagProb {
     }
   }
 
+  // A[k][pp] from the packed Lorentzian table (lorentz_row), read 16
+  // consecutive pp per row.
   template <typename T>
   __device__ void fetchA(Frag<T> &s, int t, int b, int m0, int k0) const {
-    const int N = g.N, q = b / 2 - (N - 1), kind = b % 2;
-    const double *Es = kind == 0 ? g.Ee : g.Eh;  // carrier of row k
-    const double *Eo = kind == 0 ? g.Eh : g.Ee;  // the other carrier
-    const double hg_same = g.hbar * (kind == 0 ? g.gam_eh : g.gam_e);
-    const double hg_x = g.hbar * g.gam_h;
+    const int N = g.N, qi = b / 2, q = qi - (N - 1), kind = b % 2;
+    const int kb = max(0, -q), ke = min(N, N - q);
+    const double *tab = g.lor + (long)kind * N * N * 2 * N;
+    const long r0 = lorentz_row0(N, qi) - kb;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int idx = tid + 256 * e;
-      const int k = idx >> 6, x = idx & 63;
+      const int x = idx >> 4, k = idx & 15;
       const int kr = m0 + x, pp = k0 + k;
       double v = 0.0;
-      const int kq = kr + q;
-      if (kr < N && kq >= 0 && kq < N && pp < 2 * N && (t == 0) == (pp < N)) {
-        double Ed, hg;
-        if (pp < N) {
-          const int p = pp, pm = p - q;
-          if (pm >= 1 && pm < N) {
-            Ed = Es[kq] + Es[pm] - Es[p] - Es[kr];
-            hg = hg_same;
-            v = (hg / PI) / (Ed * Ed + hg * hg);
-          }
-        } else {
-          const int p = pp - N, pq = p + q;
-          if (pq >= 0 && pq < N) {
-            Ed = Es[kq] + Eo[pq] - Eo[p] - Es[kr];
-            hg = hg_x;
-            v = (hg / PI) / (Ed * Ed + hg * hg);
-          }
-        }
-      }
+      if (kr >= kb && kr < ke && pp < 2 * N && (t == 0) == (pp < N))
+        v = tab[(r0 + kr) * 2 * N + pp];
       s.set(e, k, x, (T)v, (T)0);
     }
   }
@@ -654,6 +703,132 @@ struct Prop2qwProb {
     if (j >= 2 * N - 1 || col >= 3 * nw) return;
     const int w = w0 + col / 3, f = col % 3;
     st(g.exq + (long)w * 3 * 2 * (2 * N - 1), (long)f * (2 * N - 1) + j, cd{re, im});
+  }
+};
+
+// ===========================================================================
+// Problem: the source sums of QWPolarization3 / QWRho5 (qwoptics.py:809-1071)
+// in real space, val[r][(w, f)] = sum_s phi[s][r] S_f(s) over the diagonal
+// sums S_f of the new level (k_diagsums mode 1): f = 0 dcv sum (-1)^kh p,
+// 1 dcv sum p, 2 sum C3, 3 sum conj(D3).  The epilogue applies the window,
+// the polarisation prefactors and the yw(w), (-1)^w signs.
+// ===========================================================================
+struct SrcProb {
+  static constexpr bool A_REAL = false, B_REAL = false;
+  psa_group g;
+  int w0, nw;
+
+  __device__ int nterms(int) const { return 1; }
+  __device__ bool active(int, int, int) const { return true; }
+  __device__ void krange(int, int, int, int, int &kb, int &ke) const {
+    kb = 0;
+    ke = 2 * g.N - 1;
+  }
+
+  template <typename T>
+  __device__ void fetchA(Frag<T> &s, int, int, int m0, int k0) const {
+    load_dense(s, g.phi, g.Nr, g.Nr, 2 * g.N - 1, false, false, 1.0, m0, k0);
+  }
+
+  template <typename T>
+  __device__ void fetchB(Frag<T> &s, int, int, int k0, int n0) const {
+    const int N = g.N, Q = 2 * N - 1, tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int x = idx >> 4, k = idx & 15;
+      const int col = n0 + x, si = k0 + k;
+      cd v = {0.0, 0.0};
+      if (col < 4 * nw && si < Q) {
+        const int w = w0 + (col >> 2), f = col & 3;
+        v = ld(g.src + (long)w * PSA_SRC_STRIDE(N, g.Nr) + PSA_SRC_DS(g.Nr), (long)f * Q + si);
+      }
+      s.set(e, k, x, (T)v.re, (T)v.im);
+    }
+  }
+
+  __device__ void store(int, int r, int col, double re, double im) const {
+    const int N = g.N, Nr = g.Nr;
+    if (r >= Nr || col >= 4 * nw) return;
+    const int w = w0 + (col >> 2), f = col & 3;
+    double *src = g.src + (long)w * PSA_SRC_STRIDE(N, Nr);
+    const double W = g.window[r];
+    const double sP = 2.0 * g.ehint / g.area / g.L;
+    const bool optics = (g.flags & PSA_F_OPTICS) != 0;
+    if (f == 0) {
+      src[r] = optics ? re * W * sP : 0.0;
+    } else if (f == 1) {
+      const int wid = g.wire_id[w];
+      src[Nr + r] = optics ? re * W * yw_of(wid) * sP : 0.0;
+      src[2 * Nr + r] = optics ? -re * W * sgn_pow(wid) * sP : 0.0;
+    } else {
+      const double fl = (g.flags & PSA_F_LF) ? W / (2.0 * g.L) : 0.0;
+      st(src + PSA_SRC_RC(Nr), (f == 2 ? 0 : Nr) + r, cd{re * fl, im * fl});
+    }
+  }
+};
+
+// ===========================================================================
+// Problem: QW2Prop splines r -> rr (qwoptics.py:792-805) for Px, Py, Pz, re,
+// rh of every wire as one real product with the spline matrix m2
+// (stored [Nr][bnx]); rows are the rr points bx0 .. bx0+bnx-1 inside the
+// wire's r range (the others stay zero).  RhoE/RhoH go to the real and
+// imaginary slots of Rho (|.| with LF) for k_qw2prop_b.
+// ===========================================================================
+struct Qw2pProb {
+  static constexpr bool A_REAL = true, B_REAL = true;
+  psa_group g;
+  int w0, nw;
+
+  __device__ int nterms(int) const { return 1; }
+  __device__ bool active(int, int, int) const { return true; }
+  __device__ void krange(int, int, int, int, int &kb, int &ke) const {
+    kb = 0;
+    ke = g.Nr;
+  }
+
+  template <typename T>
+  __device__ void fetchA(Frag<T> &s, int, int, int m0, int k0) const {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int k = idx >> 6, x = idx & 63;
+      const int b = m0 + x, r = k0 + k;
+      double v = 0.0;
+      if (b < g.bnx && r < g.Nr) v = g.m2[(long)r * g.bnx + b];
+      s.set(e, k, x, (T)v, (T)0);
+    }
+  }
+
+  template <typename T>
+  __device__ void fetchB(Frag<T> &s, int, int, int k0, int n0) const {
+    const int N = g.N, Nr = g.Nr, tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int x = idx >> 4, k = idx & 15;
+      const int col = n0 + x, r = k0 + k;
+      double v = 0.0;
+      if (col < 5 * nw && r < Nr) {
+        const int w = w0 + col / 5, f = col % 5;
+        v = g.src[(long)w * PSA_SRC_STRIDE(N, Nr) + f * Nr + r];
+      }
+      s.set(e, k, x, (T)v, (T)0);
+    }
+  }
+
+  __device__ void store(int, int b, int col, double re, double) const {
+    if (b >= g.bnx || col >= 5 * nw) return;
+    const int w = w0 + col / 5, f = col % 5, x = g.bx0 + b;
+    const long NW = g.Nw, Nx = g.Nx;
+    if (f < 3) {
+      st(g.out + ((f * NW + w) * Nx) * 2, x, cd{re, 0.0});
+    } else {
+      const bool lf = (g.flags & PSA_F_LF) != 0;
+      double *Rho = g.out + ((3 * NW + w) * Nx) * 2;
+      Rho[2 * x + (f - 3)] = lf ? fabs(re) : re;  // RhoE (re slot), RhoH (im slot)
+    }
   }
 };
 

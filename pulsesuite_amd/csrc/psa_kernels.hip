@@ -133,38 +133,53 @@ __global__ __launch_bounds__(256) void k_prep_a(psa_group g, int w0) {
 // mode 0: dsC[s] = sum_k C2[k][k+s], dsD likewise (prep);
 // mode 1: sums of the new level for the sources: dcv sum_kh (-1)^kh p[kh][kh+s],
 //         dcv sum p[kh][kh+s], sum C3[k][k+s], sum conj(D3[k][k+s]).
+// Block (wire, 64 diagonals): lane -> diagonal (coalesced along the rows),
+// wave -> every fourth k; the four partial sums are added in wave order.
 __global__ __launch_bounds__(256) void k_diagsums(psa_group g, int w0, int mode) {
+  __shared__ cd red[4][4][64];
   const int w = w0 + blockIdx.x, N = g.N, Q = 2 * N - 1;
-  const int si = blockIdx.y * blockDim.x + threadIdx.x;
-  if (si >= Q) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int si = blockIdx.y * 64 + lane;
   const int s = si - (N - 1);
-  const int k0 = s < 0 ? -s : 0, k1 = s < 0 ? N : N - s;
+  const int k0 = s < 0 ? -s : 0, k1 = si < Q ? (s < 0 ? N : N - s) : 0;
   const long o = mat_off(g, w);
+  cd a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0}, a3 = {0, 0};
   if (mode == 0) {
     const double *C2 = g.CB + o, *D2 = g.DB + o;
-    cd ac = {0, 0}, ad = {0, 0};
-    for (int k = k0; k < k1; ++k) {
-      ac = cadd(ac, ld(C2, (long)k * N + k + s));
-      ad = cadd(ad, ld(D2, (long)k * N + k + s));
+    for (int k = k0 + wv; k < k1; k += 4) {
+      a0 = cadd(a0, ld(C2, (long)k * N + k + s));
+      a1 = cadd(a1, ld(D2, (long)k * N + k + s));
     }
-    double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-    st(pr + PSA_PREP_DSC(N), si, ac);
-    st(pr + PSA_PREP_DSD(N), si, ad);
   } else {
     const double *p3 = g.pX + o, *C3 = g.CX + o, *D3 = g.DX + o;
-    cd ax = {0, 0}, ay = {0, 0}, ac = {0, 0}, ad = {0, 0};
-    for (int k = k0; k < k1; ++k) {
+    for (int k = k0 + wv; k < k1; k += 4) {
       const cd pv = ld(p3, (long)k * N + k + s);
-      ay = cadd(ay, pv);
-      ax = (k & 1) ? csub(ax, pv) : cadd(ax, pv);
-      ac = cadd(ac, ld(C3, (long)k * N + k + s));
-      ad = cadd(ad, cconj(ld(D3, (long)k * N + k + s)));
+      a1 = cadd(a1, pv);
+      a0 = (k & 1) ? csub(a0, pv) : cadd(a0, pv);
+      a2 = cadd(a2, ld(C3, (long)k * N + k + s));
+      a3 = cadd(a3, cconj(ld(D3, (long)k * N + k + s)));
     }
+  }
+  red[0][wv][lane] = a0;
+  red[1][wv][lane] = a1;
+  red[2][wv][lane] = a2;
+  red[3][wv][lane] = a3;
+  __syncthreads();
+  if (wv != 0 || si >= Q) return;
+  cd t[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    t[j] = cadd(cadd(red[j][0][lane], red[j][1][lane]), cadd(red[j][2][lane], red[j][3][lane]));
+  if (mode == 0) {
+    double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+    st(pr + PSA_PREP_DSC(N), si, t[0]);
+    st(pr + PSA_PREP_DSD(N), si, t[1]);
+  } else {
     double *ds = g.src + (long)w * PSA_SRC_STRIDE(N, g.Nr) + PSA_SRC_DS(g.Nr);
-    st(ds, si, cscale(ax, g.dcv));
-    st(ds, Q + si, cscale(ay, g.dcv));
-    st(ds, 2 * Q + si, ac);
-    st(ds, 3 * Q + si, ad);
+    st(ds, si, cscale(t[0], g.dcv));
+    st(ds, Q + si, cscale(t[1], g.dcv));
+    st(ds, 2 * Q + si, t[2]);
+    st(ds, 3 * Q + si, t[3]);
   }
 }
 
@@ -251,7 +266,7 @@ __global__ __launch_bounds__(256) void k_h_simple(psa_group g, int w0) {
   const double yv = yw_of(wid), zs = sgn_pow(wid);
   const double *eq = g.exq + (long)w * 3 * 2 * (2 * N - 1);
   for (long ij = blockIdx.x * (long)blockDim.x + threadIdx.x; ij < (long)N * N; ij += (long)gridDim.x * blockDim.x) {
-    const int k1 = ij / N, k2 = ij % N;
+    const int k2 = ij / N, k1 = ij % N;  // ij indexes the transposed storage [k2][k1]
     cd meh = {0, 0};
     if (g.flags & PSA_F_OPTICS) {
       const int dd = k1 - k2 + N - 1;
@@ -276,26 +291,87 @@ __global__ __launch_bounds__(256) void k_h_simple(psa_group g, int w0) {
 // Many-body Coulomb scattering (MBCE/MBCH, coulomb.py:956-1071) over the
 // sparse lists of non-negligible Gaussian-delta entries.  List l holds
 // entries (i, j, l) of the stored table, grouped (CSR) by the rate index it
-// feeds (see the slot mapping below).  One block per (slot, target k); the
-// 64 lanes of a wave are 64 wires, so each list entry is read once per wave
-// (scalar loads) and its occupation / V^2 gathers are coalesced reads of
-// wire-fastest tables built by k_diag_gather:
+// feeds (see the slot mapping below).  Block = (MB_KG target k, slot,
+// MB_WC wires): the block stages the occupations and the two V^2 channels
+// of its slot for its wires in LDS (from the wire-fastest tables built by
+// k_diag_gather), then a wave takes MB_WC wires x 8 entries per trip
+// (lane = 8 u + c), so every gather is an LDS read.  Partial sums are
+// reduced over u with shuffles and over waves in LDS, in a fixed order.
 //   wmb + 4 N Nw: occ[2][N][Nw]  |Re ne3|, |Re nh3|
 //   wmb + 6 N Nw: vsq[3][N][Nw]  VC(d)^2 for eh, ee, hh
 // ===========================================================================
-constexpr int MB_WAVES = 8;
+constexpr int MB_WC = 8, MB_KG = 4;
 
-__global__ __launch_bounds__(64 * MB_WAVES) void k_mb(psa_group g, int w0, int nw) {
-  __shared__ double red[MB_WAVES][64];
-  const int slot = blockIdx.y, k = blockIdx.x, N = g.N, W = g.Nw;
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+template <int L>
+__device__ __forceinline__ double mb_term(const double *ne, const double *nh, const double *vv,
+                                          uint32_t pk) {
+  // ne, nh, vv: LDS [N][MB_WC] tables already offset to this lane's wire
+  const int i = pk & 1023, j = (pk >> 10) & 1023, l = (pk >> 20) & 1023;
+  const int a = (L == 5) ? l : i, c = (L == 5) ? i : l;
+  // VC is lower-triangular: V(a, c) = 0 for a < c
+  const double vraw = vv[(a >= c ? a - c : 0) * MB_WC];
+  const double vq = (a >= c) ? vraw : 0.0;
+  double t;
+  if (L == 0) t = (1.0 - nh[j * MB_WC]) * nh[(j + l - i) * MB_WC] * ne[l * MB_WC];
+  else if (L == 1) t = (1.0 - ne[i * MB_WC]) * (1.0 - nh[j * MB_WC]) * nh[(j + l - i) * MB_WC];
+  else if (L == 2) t = (1.0 - ne[j * MB_WC]) * ne[(i + j - l) * MB_WC] * ne[l * MB_WC];
+  else if (L == 3) t = (1.0 - ne[i * MB_WC]) * (1.0 - ne[j * MB_WC]) * ne[(i + j - l) * MB_WC];
+  else if (L == 4) t = (1.0 - ne[i * MB_WC]) * nh[(j + l - i) * MB_WC] * ne[l * MB_WC];
+  else if (L == 5) t = (1.0 - ne[i * MB_WC]) * (1.0 - nh[j * MB_WC]) * ne[l * MB_WC];
+  else if (L == 6) t = (1.0 - nh[j * MB_WC]) * nh[(i + j - l) * MB_WC] * nh[l * MB_WC];
+  else t = (1.0 - nh[i * MB_WC]) * (1.0 - nh[j * MB_WC]) * nh[(i + j - l) * MB_WC];
+  return vq * t;
+}
+
+// this lane's share of list L's entries [beg, end): entry beg + 8 wave + u
+// + 32 t for trips t; MB_UNROLL trips per iteration, their list loads issued
+// together (padding entries have value 0 and index 0, which is in range)
+constexpr int MB_UNROLL = 4;
+
+template <int L>
+__device__ __forceinline__ double mb_list_sum(const uint32_t *ix, const double *vl, int beg, int end,
+                                              int wv, int u, const double *ne, const double *nh,
+                                              const double *vv) {
+  double acc = 0.0;
+  for (int e0 = beg + 8 * wv + u; e0 < end; e0 += 32 * MB_UNROLL) {
+    uint32_t pk[MB_UNROLL];
+    double v[MB_UNROLL];
+#pragma unroll
+    for (int t = 0; t < MB_UNROLL; ++t) {
+      const int e = e0 + 32 * t;
+      pk[t] = e < end ? ix[e] : 0u;
+      v[t] = e < end ? vl[e] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < MB_UNROLL; ++t) acc += mb_term<L>(ne, nh, vv, pk[t]) * v[t];
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void k_mb(psa_group g, int w0, int nw) {
+  extern __shared__ double smem[];
+  __shared__ double red[4][MB_WC];
+  const int N = g.N, W = g.Nw, slot = blockIdx.y, kb = blockIdx.x * MB_KG;
+  const int wc0 = blockIdx.z * MB_WC;  // first wire of the chunk (relative to w0)
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int u = lane >> 3, c = lane & 7;
+  const int ch2 = slot < 2 ? 1 : 2;  // second V^2 channel: Vee for electrons, Vhh for holes
+  double *s_ne = smem, *s_nh = smem + N * MB_WC;
+  double *s_v0 = smem + 2 * N * MB_WC, *s_v1 = smem + 3 * N * MB_WC;
   const double *occ = g.wmb + 4L * N * W, *vsq = g.wmb + 6L * N * W;
-  for (int wc = 0; wc < nw; wc += 64) {
-    const int w = w0 + wc + lane;
-    const bool on = wc + lane < nw;
-    const double *ne = occ + w, *nh = occ + (long)N * W + w;
-    const double *v2 = vsq + w;
+  for (int idx = tid; idx < N * MB_WC; idx += blockDim.x) {
+    const int k = idx / MB_WC, cc = idx % MB_WC;
+    const int w = w0 + min(wc0 + cc, nw - 1);  // idle wires of the last chunk repeat one wire
+    s_ne[idx] = occ[(long)k * W + w];
+    s_nh[idx] = occ[(long)(N + k) * W + w];
+    s_v0[idx] = vsq[(long)k * W + w];
+    s_v1[idx] = vsq[((long)ch2 * N + k) * W + w];
+  }
+  __syncthreads();
+  const double *ne = s_ne + c, *nh = s_nh + c;
+  for (int kk = 0; kk < MB_KG; ++kk) {
+    const int k = kb + kk;
+    if (k >= N) break;
     double acc = 0.0;
     for (int half = 0; half < 2; ++half) {
       // slot 0 WinE <- lists 0, 2; 1 WoutE <- 1, 3; 2 WinH <- 4, 6; 3 WoutH <- 5, 7
@@ -304,38 +380,29 @@ __global__ __launch_bounds__(64 * MB_WAVES) void k_mb(psa_group g, int w0, int n
       const int beg = g.mb_off[L][k], end = g.mb_off[L][k + 1];
       const uint32_t *ix = g.mb_idx[L];
       const double *vl = g.mb_val[L];
-      // V^2 channel: lists 0, 1, 4, 5 use Veh (5 with the index order swapped),
-      // 2, 3 Vee, 6, 7 Vhh
-      const int ch = (L == 2 || L == 3) ? 1 : ((L >= 6) ? 2 : 0);
-      const double *vv = v2 + (long)ch * N * W;
-      for (int e = beg + wv; e < end; e += MB_WAVES) {
-        const uint32_t pk = ix[e];
-        const double val = vl[e];
-        const int i = pk & 1023, j = (pk >> 10) & 1023, l = (pk >> 20) & 1023;
-        const int a = (L == 5) ? l : i, c = (L == 5) ? i : l;
-        if (!on || a < c) continue;  // VC is lower-triangular: V(a, c) = 0 for a < c
-        const double vq = vv[(long)(a - c) * W];
-        double t;
-        switch (L) {
-          case 0: t = (1.0 - nh[(long)j * W]) * nh[(long)(j + l - i) * W] * ne[(long)l * W]; break;
-          case 1: t = (1.0 - ne[(long)i * W]) * (1.0 - nh[(long)j * W]) * nh[(long)(j + l - i) * W]; break;
-          case 2: t = (1.0 - ne[(long)j * W]) * ne[(long)(i + j - l) * W] * ne[(long)l * W]; break;
-          case 3: t = (1.0 - ne[(long)i * W]) * (1.0 - ne[(long)j * W]) * ne[(long)(i + j - l) * W]; break;
-          case 4: t = (1.0 - ne[(long)i * W]) * nh[(long)(j + l - i) * W] * ne[(long)l * W]; break;
-          case 5: t = (1.0 - ne[(long)i * W]) * (1.0 - nh[(long)j * W]) * ne[(long)l * W]; break;
-          case 6: t = (1.0 - nh[(long)j * W]) * nh[(long)(i + j - l) * W] * nh[(long)l * W]; break;
-          default: t = (1.0 - nh[(long)i * W]) * (1.0 - nh[(long)j * W]) * nh[(long)(i + j - l) * W]; break;
-        }
-        acc += vq * t * val;
+      // V^2 channel: lists 0, 1, 4, 5 use Veh (5 with the index order
+      // swapped), 2, 3 Vee, 6, 7 Vhh
+      const double *vv = (half == 0 ? s_v0 : s_v1) + c;
+      switch (L) {
+        case 0: acc += mb_list_sum<0>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
+        case 1: acc += mb_list_sum<1>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
+        case 2: acc += mb_list_sum<2>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
+        case 3: acc += mb_list_sum<3>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
+        case 4: acc += mb_list_sum<4>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
+        case 5: acc += mb_list_sum<5>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
+        case 6: acc += mb_list_sum<6>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
+        default: acc += mb_list_sum<7>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
       }
     }
-    red[wv][lane] = acc;
+    // reduce over u (lane bits 3..5), then over the waves
+    acc += __shfl_xor(acc, 8, 64);
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    if (u == 0) red[wv][c] = acc;
     __syncthreads();
-    if (wv == 0 && on) {
-      double s = 0.0;
-#pragma unroll
-      for (int u = 0; u < MB_WAVES; ++u) s += red[u][lane];
-      g.wmb[(long)w * 4 * N + slot * N + k] = s;
+    if (tid < MB_WC && wc0 + tid < nw) {
+      const double s = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+      g.wmb[(long)(w0 + wc0 + tid) * 4 * N + slot * N + k] = s;
     }
     __syncthreads();
   }
@@ -359,10 +426,15 @@ __global__ void k_diag_gather(psa_group g, int w0) {
 }
 
 // Phonon rates (phonons.py:139-268), in/out update and Recomb
-// (SBEs.py:5276-5325) on the diagonal of the new level.  One block per wire.
+// (SBEs.py:5276-5325) on the diagonal of the new level.  Block (wire, 64 k):
+// lane -> k, wave -> every fourth i of the phonon sums (EP[k][i] read as
+// EPT[i][k], so both sums are coalesced along k); partials are added in
+// wave order and wave 0 applies the update.
 __global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
   __shared__ double s_ner[1024], s_nhr[1024];
+  __shared__ double part[4][4][64];
   const int w = w0 + blockIdx.x, N = g.N, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6, k = blockIdx.y * 64 + lane;
   const long o = mat_off(g, w);
   double *C3 = g.CX + o, *D3 = g.DX + o;
   const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
@@ -370,63 +442,69 @@ __global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
   const double *mb = g.wmb + (long)w * 4 * N;
   const bool ehs = (g.flags & PSA_F_EHS) != 0, phon = (g.flags & PSA_F_PHONON) != 0;
   const bool recomb = (g.flags & PSA_F_RECOMB) != 0;
-  for (int k = tid; k < N; k += blockDim.x) {
-    s_ner[k] = ld(C3, (long)k * N + k).re;
-    s_nhr[k] = ld(D3, (long)k * N + k).re;
+  for (int i = tid; i < N; i += blockDim.x) {
+    s_ner[i] = ld(C3, (long)i * N + i).re;
+    s_nhr[i] = ld(D3, (long)i * N + i).re;
   }
   __syncthreads();
-  for (int k = tid; k < N; k += blockDim.x) {
-    double wie = 0, woe = 0, wih = 0, woh = 0;
-    if (ehs) {
-      wie = mb[k];
-      woe = mb[N + k];
-      wih = mb[2 * N + k];
-      woh = mb[3 * N + k];
+  double a = 0, b = 0, c = 0, d = 0;
+  if (phon && k < N) {
+    // Vep[k][i] = VC[k,i,1] / E1D[k,i] * Vscale, VC lower-triangular
+    for (int i = wv; i <= k; i += 4) {
+      const double vep = vc[N + k - i] / E1D[k - i] * g.Vscale;
+      a += vep * s_ner[i] * g.EPT[(long)i * N + k];
+      b += vep * (1.0 - s_ner[i]) * g.EP[(long)i * N + k];
     }
-    if (phon) {
-      // Vep[k][i] = VC[k,i,1] / E1D[k,i] * Vscale, VC lower-triangular
-      double a = 0, b = 0, c = 0, d = 0;
-      for (int i = 0; i <= k; ++i) {
-        const double vep = vc[N + k - i] / E1D[k - i] * g.Vscale;
-        a += vep * s_ner[i] * g.EP[(long)k * N + i];
-        b += vep * (1.0 - s_ner[i]) * g.EP[(long)i * N + k];
-      }
-      for (int i = k; i < N; ++i) {
-        const double vhp = vc[2 * N + i - k] / E1D[i - k] * g.Vscale;
-        c += vhp * s_nhr[i] * g.HP[(long)k * N + i];
-        d += vhp * (1.0 - s_nhr[i]) * g.HP[(long)i * N + k];
-      }
-      wie += a;
-      woe += b;
-      wih += c;
-      woh += d;
+    for (int i = k + wv; i < N; i += 4) {
+      const double vhp = vc[2 * N + i - k] / E1D[i - k] * g.Vscale;
+      c += vhp * s_nhr[i] * g.HPT[(long)i * N + k];
+      d += vhp * (1.0 - s_nhr[i]) * g.HP[(long)i * N + k];
     }
-    cd ne = ld(C3, (long)k * N + k), nh = ld(D3, (long)k * N + k);
-    if (ehs || phon) {
-      const double We = wie + woe, Wh = wih + woh;
-      const double ee = exp(-We * dt), eep = exp(We * dt);
-      const double eh = exp(-Wh * dt), ehp = exp(Wh * dt);
-      cd t1 = {(-1.0 + eep + ne.re) * wie + ne.re * woe, ne.im * wie + ne.im * woe};
-      ne = cd{ee * t1.re / (We + 1e-200), ee * t1.im / (We + 1e-200)};
-      cd t2 = {(-1.0 + ehp + nh.re) * wih + nh.re * woh, nh.im * wih + nh.im * woh};
-      nh = cd{eh * t2.re / (Wh + 1e-200), eh * t2.im / (Wh + 1e-200)};
-      if (recomb) {
-        // ne *= exp(-Rsp nh dt); nh *= exp(-Rsp ne dt) with the updated ne
-        const double r = Rsp[k];
-        cd z = {-r * nh.re * dt, -r * nh.im * dt};
-        double sz, cz;
-        sincos(z.im, &sz, &cz);
-        const double m = exp(z.re);
-        ne = cmul(ne, cd{m * cz, m * sz});
-        z = cd{-r * ne.re * dt, -r * ne.im * dt};
-        sincos(z.im, &sz, &cz);
-        const double m2 = exp(z.re);
-        nh = cmul(nh, cd{m2 * cz, m2 * sz});
-      }
-    }
-    st(C3, (long)k * N + k, ne);
-    st(D3, (long)k * N + k, nh);
   }
+  part[0][wv][lane] = a;
+  part[1][wv][lane] = b;
+  part[2][wv][lane] = c;
+  part[3][wv][lane] = d;
+  __syncthreads();
+  if (wv != 0 || k >= N) return;
+  double wie = 0, woe = 0, wih = 0, woh = 0;
+  if (ehs) {
+    wie = mb[k];
+    woe = mb[N + k];
+    wih = mb[2 * N + k];
+    woh = mb[3 * N + k];
+  }
+  if (phon) {
+    wie += (part[0][0][lane] + part[0][1][lane]) + (part[0][2][lane] + part[0][3][lane]);
+    woe += (part[1][0][lane] + part[1][1][lane]) + (part[1][2][lane] + part[1][3][lane]);
+    wih += (part[2][0][lane] + part[2][1][lane]) + (part[2][2][lane] + part[2][3][lane]);
+    woh += (part[3][0][lane] + part[3][1][lane]) + (part[3][2][lane] + part[3][3][lane]);
+  }
+  cd ne = ld(C3, (long)k * N + k), nh = ld(D3, (long)k * N + k);
+  if (ehs || phon) {
+    const double We = wie + woe, Wh = wih + woh;
+    const double ee = exp(-We * dt), eep = exp(We * dt);
+    const double eh = exp(-Wh * dt), ehp = exp(Wh * dt);
+    cd t1 = {(-1.0 + eep + ne.re) * wie + ne.re * woe, ne.im * wie + ne.im * woe};
+    ne = cd{ee * t1.re / (We + 1e-200), ee * t1.im / (We + 1e-200)};
+    cd t2 = {(-1.0 + ehp + nh.re) * wih + nh.re * woh, nh.im * wih + nh.im * woh};
+    nh = cd{eh * t2.re / (Wh + 1e-200), eh * t2.im / (Wh + 1e-200)};
+    if (recomb) {
+      // ne *= exp(-Rsp nh dt); nh *= exp(-Rsp ne dt) with the updated ne
+      const double r = Rsp[k];
+      cd z = {-r * nh.re * dt, -r * nh.im * dt};
+      double sz, cz;
+      sincos(z.im, &sz, &cz);
+      const double m = exp(z.re);
+      ne = cmul(ne, cd{m * cz, m * sz});
+      z = cd{-r * ne.re * dt, -r * ne.im * dt};
+      sincos(z.im, &sz, &cz);
+      const double m2 = exp(z.re);
+      nh = cmul(nh, cd{m2 * cz, m2 * sz});
+    }
+  }
+  st(C3, (long)k * N + k, ne);
+  st(D3, (long)k * N + k, nh);
 }
 
 // ===========================================================================
@@ -437,7 +515,7 @@ __global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
 // twiddles from a per-N table).  One wave per line, TR_LINES lines per
 // block; the column pass reads and writes TR_LINES adjacent columns per row.
 // ===========================================================================
-constexpr int TR_LINES = 8;
+constexpr int TR_LINES = 4;
 
 __device__ __forceinline__ cd tw_get(const cd *tw, int idx, int sign) {
   cd w = tw[idx];  // exp(-2 pi i idx / N)
@@ -450,56 +528,143 @@ __device__ __forceinline__ cd mul_is(cd a, int sign) {
   return sign > 0 ? cd{-a.im, a.re} : cd{a.im, -a.re};
 }
 
-// Stockham autosort FFT of one line held in LDS (x, ping-pong buffer y),
-// result left in x.  Stage with radix R and sub-length Ns: for j < N/R,
-// k = j mod Ns, inputs x[j + r N/R] w^(r k N/(R Ns)), outputs at
+// LDS slot of element i of a line: one pad slot after every four elements,
+// which spreads the strided Stockham accesses over the banks.
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 2); }
+
+// Orders one wave's LDS writes before its later LDS reads by other lanes of
+// the same wave (the line a wave transforms is private to that wave).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stockham autosort FFT of one line held in LDS (indexed through fpad), in
+// place, by one wave: each stage reads and combines all its inputs into
+// registers, then writes.  Stage with radix R and sub-length Ns: for
+// j < N/R, k = j mod Ns, inputs x[j + r N/R] w^(r k N/(R Ns)), outputs at
 // (j - k) R + k + s Ns.  sign -1 forward, +1 inverse (unscaled).
-__device__ void fft_line(cd *x, cd *y, const cd *tw, int N, int sign, int lane) {
-  int Ns = 1;
-  cd *src = x, *dst = y;
-  int rem = N;
+// N = 2^a 3^b <= 512, so a lane has at most 4 butterflies per stage.
+__device__ void fft_line(cd *x, const cd *tw, int N, int sign, int lane) {
+  int Ns = 1, rem = N;
   while (rem > 1) {
     const int R = (rem % 4 == 0) ? 4 : ((rem % 2 == 0) ? 2 : 3);
     const int M = N / R, step = N / (Ns * R);
     const uint64_t mg = ((1ull << 32) + Ns - 1) / Ns;  // j / Ns = (j mg) >> 32 for j < 2^20
-    for (int j = lane; j < M; j += 64) {
+    cd o[4][4];
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int j = lane + 64 * it;
+      if (j >= M) break;
       const int k = j - Ns * (int)(((uint64_t)j * mg) >> 32);
-      const int base = (j - k) * R + k;
-      const cd a0 = src[j];
+      const cd a0 = x[fpad(j)];
       if (R == 4) {
-        const cd a1 = cmul(src[j + M], tw_get(tw, k * step, sign));
-        const cd a2 = cmul(src[j + 2 * M], tw_get(tw, 2 * k * step, sign));
-        const cd a3 = cmul(src[j + 3 * M], tw_get(tw, 3 * k * step, sign));
+        const cd a1 = cmul(x[fpad(j + M)], tw_get(tw, k * step, sign));
+        const cd a2 = cmul(x[fpad(j + 2 * M)], tw_get(tw, 2 * k * step, sign));
+        const cd a3 = cmul(x[fpad(j + 3 * M)], tw_get(tw, 3 * k * step, sign));
         const cd s02 = cadd(a0, a2), d02 = csub(a0, a2);
         const cd s13 = cadd(a1, a3), d13 = mul_is(csub(a1, a3), sign);
-        dst[base] = cadd(s02, s13);
-        dst[base + Ns] = cadd(d02, d13);
-        dst[base + 2 * Ns] = csub(s02, s13);
-        dst[base + 3 * Ns] = csub(d02, d13);
+        o[it][0] = cadd(s02, s13);
+        o[it][1] = cadd(d02, d13);
+        o[it][2] = csub(s02, s13);
+        o[it][3] = csub(d02, d13);
       } else if (R == 2) {
-        const cd b = cmul(src[j + M], tw_get(tw, k * step, sign));
-        dst[base] = cadd(a0, b);
-        dst[base + Ns] = csub(a0, b);
+        const cd b = cmul(x[fpad(j + M)], tw_get(tw, k * step, sign));
+        o[it][0] = cadd(a0, b);
+        o[it][1] = csub(a0, b);
       } else {
         const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
         const cd w1 = {c3, s3}, w2 = {c3, -s3};
-        const cd b = cmul(src[j + M], tw_get(tw, k * step, sign));
-        const cd c = cmul(src[j + 2 * M], tw_get(tw, 2 * k * step, sign));
-        dst[base] = cadd(a0, cadd(b, c));
-        dst[base + Ns] = cadd(a0, cadd(cmul(b, w1), cmul(c, w2)));
-        dst[base + 2 * Ns] = cadd(a0, cadd(cmul(b, w2), cmul(c, w1)));
+        const cd b = cmul(x[fpad(j + M)], tw_get(tw, k * step, sign));
+        const cd c = cmul(x[fpad(j + 2 * M)], tw_get(tw, 2 * k * step, sign));
+        o[it][0] = cadd(a0, cadd(b, c));
+        o[it][1] = cadd(a0, cadd(cmul(b, w1), cmul(c, w2)));
+        o[it][2] = cadd(a0, cadd(cmul(b, w2), cmul(c, w1)));
       }
     }
+    wave_lds_sync();
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int j = lane + 64 * it;
+      if (j >= M) break;
+      const int k = j - Ns * (int)(((uint64_t)j * mg) >> 32);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r < R) x[fpad(base + r * Ns)] = o[it][r];
+    }
+    wave_lds_sync();
     Ns *= R;
     rem /= R;
-    __syncthreads();
-    cd *tmp = src;
-    src = dst;
-    dst = tmp;
   }
-  if (src != x) {
-    for (int j = lane; j < N; j += 64) x[j] = src[j];
-    __syncthreads();
+}
+
+// The same FFT with N known at compile time: the stage sequence, moduli
+// and strides are constants.
+template <int N, int Ns, int Rem, int SIGN>
+__device__ __forceinline__ void fft_stages(cd *x, const cd *tw, int lane) {
+  if constexpr (Rem > 1) {
+    constexpr int R = (Rem % 4 == 0) ? 4 : ((Rem % 2 == 0) ? 2 : 3);
+    constexpr int M = N / R, step = N / (Ns * R), IT = (M + 63) / 64;
+    cd o[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int j = lane + 64 * it;
+      if (M % 64 == 0 || j < M) {
+        const int k = j % Ns;
+        const cd a0 = x[fpad(j)];
+        if constexpr (R == 4) {
+          // first stage (Ns = 1): k = 0, all twiddles are 1
+          const cd a1 = Ns == 1 ? x[fpad(j + M)] : cmul(x[fpad(j + M)], tw_get(tw, k * step, SIGN));
+          const cd a2 = Ns == 1 ? x[fpad(j + 2 * M)] : cmul(x[fpad(j + 2 * M)], tw_get(tw, 2 * k * step, SIGN));
+          const cd a3 = Ns == 1 ? x[fpad(j + 3 * M)] : cmul(x[fpad(j + 3 * M)], tw_get(tw, 3 * k * step, SIGN));
+          const cd s02 = cadd(a0, a2), d02 = csub(a0, a2);
+          const cd s13 = cadd(a1, a3), d13 = mul_is(csub(a1, a3), SIGN);
+          o[it][0] = cadd(s02, s13);
+          o[it][1] = cadd(d02, d13);
+          o[it][2] = csub(s02, s13);
+          o[it][3] = csub(d02, d13);
+        } else if constexpr (R == 2) {
+          const cd b = cmul(x[fpad(j + M)], tw_get(tw, k * step, SIGN));
+          o[it][0] = cadd(a0, b);
+          o[it][1] = csub(a0, b);
+        } else {
+          constexpr double c3 = -0.5, s3 = SIGN * 0.86602540378443864676;
+          const cd w1 = {c3, s3}, w2 = {c3, -s3};
+          const cd b = cmul(x[fpad(j + M)], tw_get(tw, k * step, SIGN));
+          const cd c = cmul(x[fpad(j + 2 * M)], tw_get(tw, 2 * k * step, SIGN));
+          o[it][0] = cadd(a0, cadd(b, c));
+          o[it][1] = cadd(a0, cadd(cmul(b, w1), cmul(c, w2)));
+          o[it][2] = cadd(a0, cadd(cmul(b, w2), cmul(c, w1)));
+        }
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int j = lane + 64 * it;
+      if (M % 64 == 0 || j < M) {
+        const int k = j % Ns;
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[fpad(base + r * Ns)] = o[it][r];
+      }
+    }
+    wave_lds_sync();
+    fft_stages<N, Ns * R, Rem / R, SIGN>(x, tw, lane);
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void fft_line_n(cd *x, const cd *tw, int N, int sign, int lane) {
+  if constexpr (NT == 0) {
+    fft_line(x, tw, N, sign, lane);
+  } else {
+    if (sign < 0)
+      fft_stages<NT, 1, NT, -1>(x, tw, lane);
+    else
+      fft_stages<NT, 1, NT, +1>(x, tw, lane);
   }
 }
 
@@ -520,45 +685,54 @@ __global__ void k_tphase(psa_group g, int w0, double dt) {
 // writes the leapfrog average A = (A + X3) / 2 (SBEs.py:5704-5706) for
 // every element except the diagonals of C and D, which the e/h
 // renormalisation still changes (k_finish_diag averages those).
+// LDS per line: fpad(N) slots plus one, so the line stride is odd (in
+// 16-byte slots) and the column pass's staging accesses (line index
+// fastest) fall on different banks.
+__host__ __device__ inline int tr_line_stride(int N) { return N + N / 4 + 1; }
+
+// NT: N at compile time (0: taken from the group at run time)
+template <int NT>
 __global__ __launch_bounds__(64 * TR_LINES) void k_transport(psa_group g, int w0, int pass, int avg) {
   extern __shared__ double smem[];
-  const int N = g.N, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int N = NT ? NT : g.N, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int LS = tr_line_stride(N);
   const int mat = blockIdx.y;  // 0 C, 1 D, 2 p
   const int w = w0 + blockIdx.z;
   const int line0 = blockIdx.x * TR_LINES;
   cd *base = reinterpret_cast<cd *>(smem);
-  cd *tw = base + TR_LINES * 2 * N;  // [N] twiddles
+  cd *tw = base + TR_LINES * LS;  // [N] twiddles
   for (int j = tid; j < N; j += blockDim.x) tw[j] = ld(g.twid, j);
   const long o = mat_off(g, w);
   double *X = (mat == 0 ? g.CX : (mat == 1 ? g.DX : g.pX)) + o;
   double *A = (mat == 0 ? g.CA : (mat == 1 ? g.DA : g.pA)) + o;
   const int line = line0 + wv;
-  cd *x = base + wv * 2 * N, *y = x + N;
+  cd *x = base + wv * LS;
   if (pass == 0) {
     if (line < N)
-      for (int j = lane; j < N; j += 64) x[j] = ld(X, (long)line * N + j);
+      for (int j = lane; j < N; j += 64) x[fpad(j)] = ld(X, (long)line * N + j);
   } else {
     for (int idx = tid; idx < TR_LINES * N; idx += blockDim.x) {
       const int c = idx % TR_LINES, j = idx / TR_LINES;
-      if (line0 + c < N) base[c * 2 * N + j] = ld(X, (long)j * N + line0 + c);
+      if (line0 + c < N) base[c * LS + fpad(j)] = ld(X, (long)j * N + line0 + c);
     }
   }
   __syncthreads();
-  fft_line(x, y, tw, N, -1, lane);
+  fft_line_n<NT>(x, tw, N, -1, lane);
   const double *ph = g.tphase + (long)w * 2 * N;
-  for (int j = lane; j < N; j += 64) x[j] = cmul(x[j], ld(ph, j));
-  __syncthreads();
-  fft_line(x, y, tw, N, +1, lane);
+  for (int j = lane; j < N; j += 64) x[fpad(j)] = cmul(x[fpad(j)], ld(ph, j));
+  wave_lds_sync();
+  fft_line_n<NT>(x, tw, N, +1, lane);
+  __syncthreads();  // the column pass stores lines of other waves
   const double inv = 1.0 / N;
   if (pass == 0) {
     if (line < N)
-      for (int j = lane; j < N; j += 64) st(X, (long)line * N + j, cscale(x[j], inv));
+      for (int j = lane; j < N; j += 64) st(X, (long)line * N + j, cscale(x[fpad(j)], inv));
   } else {
     for (int idx = tid; idx < TR_LINES * N; idx += blockDim.x) {
       const int c = idx % TR_LINES, j = idx / TR_LINES, col = line0 + c;
       if (col < N) {
         const long ij = (long)j * N + col;
-        const cd v = cscale(base[c * 2 * N + j], inv);
+        const cd v = cscale(base[c * LS + fpad(j)], inv);
         st(X, ij, v);
         if (avg && !(mat < 2 && j == col)) {
           const cd a = ld(A, ij);
@@ -617,38 +791,12 @@ __global__ __launch_bounds__(256) void k_average(psa_group g, int w0, int nw) {
 // sources: QWPolarization3 (qwoptics.py:809-940) and QWRho5 (942-1071) in
 // real space.  Both are sums over (k, k') of a matrix element times
 // exp(i (k'-k) r); with kr on a uniform grid that is sum_d S_d exp(i d dkr r)
-// over the diagonal sums S_d (k_diagsums mode 1), evaluated per r from the
-// static table phi[d][r].  k_src_norm then applies the boundary-mean
+// over the diagonal sums S_d (k_diagsums mode 1): one product with the
+// static table phi[d][r] on the GEMM engine (SrcProb).  k_src_norm then applies the boundary-mean
 // subtraction, the particle-number normalisation, the 2 e0 ehint / area
 // scaling (SBEs.py:5749-5750) and QW2Prop's |.| / common-total step
 // (qwoptics.py:785-790), and evaluates CalcI0 (dcfield.py:1110-1152).
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_src_r(psa_group g, int w0) {
-  const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, Q = 2 * N - 1;
-  const int r = blockIdx.y * blockDim.x + threadIdx.x;
-  if (r >= Nr) return;
-  double *src = g.src + (long)w * PSA_SRC_STRIDE(N, Nr);
-  const double *ds = src + PSA_SRC_DS(Nr);
-  cd px = {0, 0}, py = {0, 0}, rc = {0, 0}, rd = {0, 0};
-  for (int si = 0; si < Q; ++si) {
-    const cd ph = ld(g.phi, (long)si * Nr + r);
-    px = cadd(px, cmul(ld(ds, si), ph));
-    py = cadd(py, cmul(ld(ds, Q + si), ph));
-    rc = cadd(rc, cmul(ld(ds, 2 * Q + si), ph));
-    rd = cadd(rd, cmul(ld(ds, 3 * Q + si), ph));
-  }
-  const int wid = g.wire_id[w];
-  const double W = g.window[r];
-  const double sP = 2.0 * g.ehint / g.area / g.L;
-  const bool optics = (g.flags & PSA_F_OPTICS) != 0;
-  src[r] = optics ? px.re * W * sP : 0.0;
-  src[Nr + r] = optics ? py.re * W * yw_of(wid) * sP : 0.0;
-  src[2 * Nr + r] = optics ? -py.re * W * sgn_pow(wid) * sP : 0.0;
-  const double f = (g.flags & PSA_F_LF) ? W / (2.0 * g.L) : 0.0;
-  st(src + PSA_SRC_RC(Nr), r, cscale(rc, f));
-  st(src + PSA_SRC_RC(Nr), Nr + r, cscale(rd, f));
-}
-
 __global__ __launch_bounds__(256) void k_src_norm(psa_group g, int w0) {
   __shared__ double sh[8];
   __shared__ double s_ne2[1024], s_nh2[1024], ecE[1024], ecH[1024];
@@ -754,62 +902,42 @@ __global__ __launch_bounds__(256) void k_src_norm(psa_group g, int w0) {
 }
 
 // ===========================================================================
-// QW2Prop splines r -> rr (qwoptics.py:792-805) and Rho = RhoH - RhoE
-// (SBEs.py:5917): k_qw2prop_a applies the spline matrix (transposed,
-// coalesced) and stores per-block partial sums of |RhoE|, |RhoH|;
-// k_qw2prop_b renormalises to the common total.
+// QW2Prop (qwoptics.py:785-805) and Rho = RhoH - RhoE (SBEs.py:5917): the
+// spline products run on the GEMM engine (Qw2pProb), which leaves RhoE and
+// RhoH in the real and imaginary slots of Rho; k_qw2prop_b (one block per
+// wire) renormalises both to the common total and forms Rho.
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_qw2prop_a(psa_group g, int w0) {
-  __shared__ double sh[8];
-  const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, Nx = g.Nx;
-  const int x = blockIdx.y * blockDim.x + threadIdx.x;
-  const double *src = g.src + (long)w * PSA_SRC_STRIDE(N, Nr);
-  const long NW = g.Nw;
-  const bool lf = (g.flags & PSA_F_LF) != 0;
-  double a = 0.0, b2 = 0.0;
-  if (x < Nx) {
-    const int b = x - g.bx0;
-    double v[5] = {0, 0, 0, 0, 0};
-    if (b >= 0 && b < g.bnx) {
-      for (int r = 0; r < Nr; ++r) {
-        const double c = g.m2[(long)r * g.bnx + b];
-#pragma unroll
-        for (int f = 0; f < 5; ++f) v[f] += c * src[f * Nr + r];
-      }
-    }
-    st(g.out + ((0 * NW + w) * (long)Nx) * 2, x, cd{v[0], 0});
-    st(g.out + ((1 * NW + w) * (long)Nx) * 2, x, cd{v[1], 0});
-    st(g.out + ((2 * NW + w) * (long)Nx) * 2, x, cd{v[2], 0});
-    a = lf ? fabs(v[3]) : v[3];
-    b2 = lf ? fabs(v[4]) : v[4];
-    double *Rho = g.out + ((3 * NW + w) * (long)Nx) * 2;
-    Rho[2 * x] = a;       // RhoE, finished by k_qw2prop_b
-    Rho[2 * x + 1] = b2;  // RhoH
-  }
-  const double SE = block_sum(a, sh), SH = block_sum(b2, sh);
-  if (threadIdx.x == 0) {
-    g.part[(long)w * 256 + 2 * blockIdx.y] = SE;
-    g.part[(long)w * 256 + 2 * blockIdx.y + 1] = SH;
-  }
-}
-
 __global__ __launch_bounds__(256) void k_qw2prop_b(psa_group g, int w0) {
-  const int w = w0 + blockIdx.x, Nx = g.Nx;
-  const int nb = gridDim.y;
-  double SE = 0.0, SH = 0.0;
-  for (int i = 0; i < nb; ++i) {
-    SE += g.part[(long)w * 256 + 2 * i];
-    SH += g.part[(long)w * 256 + 2 * i + 1];
+  __shared__ double sh[8];
+  const int w = w0 + blockIdx.x, Nx = g.Nx, tid = threadIdx.x;
+  double *Rho = g.out + ((3 * (long)g.Nw + w) * (long)Nx) * 2;
+  double se = 0.0, sh2 = 0.0;
+  for (int x = g.bx0 + tid; x < g.bx0 + g.bnx; x += blockDim.x) {
+    se += Rho[2 * x];
+    sh2 += Rho[2 * x + 1];
   }
+  const double SE = block_sum(se, sh), SH = block_sum(sh2, sh);
   const bool lf = (g.flags & PSA_F_LF) != 0;
   const double total = g.obs[(long)w * PSA_OBS_N + 3];
   const double fe = lf ? total / (SE * g.drr + 1e-100) : 1.0;
   const double fh = lf ? total / (SH * g.drr + 1e-100) : 1.0;
-  const int x = blockIdx.y * blockDim.x + threadIdx.x;
-  if (x >= Nx) return;
-  double *Rho = g.out + ((3 * (long)g.Nw + w) * (long)Nx) * 2;
-  const double a = Rho[2 * x] * fe, b = Rho[2 * x + 1] * fh;
-  st(Rho, x, cd{b - a, 0.0});
+  for (int x = g.bx0 + tid; x < g.bx0 + g.bnx; x += blockDim.x) {
+    const double a = Rho[2 * x] * fe, b = Rho[2 * x + 1] * fh;
+    st(Rho, x, cd{b - a, 0.0});
+  }
+}
+
+// Lorentzian table for the off-diagonal dephasing products: block (qi, kind)
+// writes its N - |q| packed rows, 2N entries each.
+__global__ __launch_bounds__(256) void k_lorentz_table(psa_group g) {
+  const int N = g.N, qi = blockIdx.x, kind = blockIdx.y, q = qi - (N - 1);
+  const int kb = max(0, -q), ke = min(N, N - q);
+  double *row0 = g.lor + ((long)kind * N * N + lorentz_row0(N, qi)) * 2 * N;
+  const long n = (long)(ke - kb) * 2 * N;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    const int kr = kb + (int)(i / (2 * N)), pp = (int)(i % (2 * N));
+    row0[i] = lorentz_value(g, kind, q, kr, pp);
+  }
 }
 
 // ===========================================================================
@@ -864,7 +992,7 @@ int psa_prep(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   const hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_prep_a, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_prep_a");
-  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 255) / 256), dim3(256), 0, s, *g, w0, 0);
+  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 63) / 64), dim3(256), 0, s, *g, w0, 0);
   PSA_LAUNCH_CHECK("k_diagsums");
   hipLaunchKernelGGL(k_prep_b, dim3(nw, (g->N + 3) / 4), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_prep_b");
@@ -884,6 +1012,14 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   p.w0 = w0;
   // the exchange sums feed Heh, which drives p directly: always FP64
   return launch_gemm(p, PSA_PREC_FP64, g->N, 2 * g->N - 1, 3 * nw, s, "calc_h gemm");
+}
+
+int psa_lorentz_table(const psa_group *g, void *stream) {
+  if (!g || !g->lor || !g->Ee || !g->Eh || g->N < 2 || g->N > 1024)
+    return fail(PSA_E_INVALID, "psa_lorentz_table: bad group");
+  hipLaunchKernelGGL(k_lorentz_table, dim3(2 * g->N - 1, 2), dim3(256), 0, (hipStream_t)stream, *g);
+  PSA_LAUNCH_CHECK("k_lorentz_table");
+  return PSA_OK;
 }
 
 int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
@@ -915,10 +1051,23 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
   hipLaunchKernelGGL(k_diag_gather, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_diag_gather");
   if (g->flags & PSA_F_EHS) {
-    hipLaunchKernelGGL(k_mb, dim3(g->N, 4), dim3(64 * MB_WAVES), 0, s, *g, w0, nw);
+    const size_t shm = 4 * (size_t)g->N * MB_WC * sizeof(double);
+    if (g->N > 512) return fail(PSA_E_UNSUPPORTED, "many-body scattering needs N <= 512 (LDS tables)");
+    if (shm > 65536) {
+      static bool attr_set = false;
+      if (!attr_set) {
+        int rc = hip_check(hipFuncSetAttribute((const void *)k_mb,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
+                           "hipFuncSetAttribute(k_mb)");
+        if (rc) return rc;
+        attr_set = true;
+      }
+    }
+    hipLaunchKernelGGL(k_mb, dim3((g->N + MB_KG - 1) / MB_KG, 4, (nw + MB_WC - 1) / MB_WC), dim3(256), shm, s,
+                       *g, w0, nw);
     PSA_LAUNCH_CHECK("k_mb");
   }
-  hipLaunchKernelGGL(k_relax, dim3(nw), dim3(256), 0, s, *g, w0, dt);
+  hipLaunchKernelGGL(k_relax, dim3(nw, (g->N + 63) / 64), dim3(256), 0, s, *g, w0, dt);
   PSA_LAUNCH_CHECK("k_relax");
   return PSA_OK;
 }
@@ -926,6 +1075,25 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
 }  // extern "C"
 
 namespace {
+
+template <int NT>
+int launch_transport(const psa_group *g, int32_t w0, dim3 grid, size_t shm, hipStream_t s, bool avg) {
+  if (shm > 65536) {
+    static bool attr_set = false;  // per instantiation
+    if (!attr_set) {
+      int rc = hip_check(hipFuncSetAttribute((const void *)k_transport<NT>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
+                         "hipFuncSetAttribute(k_transport)");
+      if (rc) return rc;
+      attr_set = true;
+    }
+  }
+  hipLaunchKernelGGL(k_transport<NT>, grid, dim3(64 * TR_LINES), shm, s, *g, w0, 0, 0);
+  PSA_LAUNCH_CHECK("k_transport");
+  hipLaunchKernelGGL(k_transport<NT>, grid, dim3(64 * TR_LINES), shm, s, *g, w0, 1, avg ? 1 : 0);
+  PSA_LAUNCH_CHECK("k_transport");
+  return PSA_OK;
+}
 
 int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStream_t s, bool avg) {
   if (!(g->flags & PSA_F_LF)) return fail(PSA_E_UNSUPPORTED, "Transport with LF = 0 (diagonal-only shift) is not supported");
@@ -936,23 +1104,18 @@ int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStr
   if (n != 1) return fail(PSA_E_UNSUPPORTED, "Transport needs N = 2^a 3^b");
   hipLaunchKernelGGL(k_tphase, dim3(nw), dim3(256), 0, s, *g, w0, dt);
   PSA_LAUNCH_CHECK("k_tphase");
-  const size_t shm = (2 * TR_LINES + 1) * (size_t)g->N * sizeof(cd);  // lines x 2 buffers + twiddles
-  if (shm > 65536) {
-    static bool attr_set = false;
-    if (!attr_set) {
-      int rc = hip_check(hipFuncSetAttribute((const void *)k_transport,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
-                         "hipFuncSetAttribute(k_transport)");
-      if (rc) return rc;
-      attr_set = true;
-    }
-  }
+  const size_t shm = ((size_t)TR_LINES * tr_line_stride(g->N) + g->N) * sizeof(cd);  // lines + twiddles
   const dim3 grid((g->N + TR_LINES - 1) / TR_LINES, 3, nw);
-  hipLaunchKernelGGL(k_transport, grid, dim3(64 * TR_LINES), shm, s, *g, w0, 0, 0);
-  PSA_LAUNCH_CHECK("k_transport");
-  hipLaunchKernelGGL(k_transport, grid, dim3(64 * TR_LINES), shm, s, *g, w0, 1, avg ? 1 : 0);
-  PSA_LAUNCH_CHECK("k_transport");
-  return PSA_OK;
+  switch (g->N) {
+    case 64: return launch_transport<64>(g, w0, grid, shm, s, avg);
+    case 96: return launch_transport<96>(g, w0, grid, shm, s, avg);
+    case 128: return launch_transport<128>(g, w0, grid, shm, s, avg);
+    case 192: return launch_transport<192>(g, w0, grid, shm, s, avg);
+    case 256: return launch_transport<256>(g, w0, grid, shm, s, avg);
+    case 384: return launch_transport<384>(g, w0, grid, shm, s, avg);
+    case 512: return launch_transport<512>(g, w0, grid, shm, s, avg);
+    default: return launch_transport<0>(g, w0, grid, shm, s, avg);
+  }
 }
 
 int finish_impl(const psa_group *g, int32_t w0, int32_t nw, hipStream_t s, bool averaged) {
@@ -984,10 +1147,15 @@ int psa_finish(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
 int psa_sources(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
   const hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 255) / 256), dim3(256), 0, s, *g, w0, 1);
+  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 63) / 64), dim3(256), 0, s, *g, w0, 1);
   PSA_LAUNCH_CHECK("k_diagsums");
-  hipLaunchKernelGGL(k_src_r, dim3(nw, (g->Nr + 255) / 256), dim3(256), 0, s, *g, w0);
-  PSA_LAUNCH_CHECK("k_src_r");
+  {
+    SrcProb p;
+    p.g = *g;
+    p.w0 = w0;
+    p.nw = nw;
+    if (int rc = launch_gemm(p, PSA_PREC_FP64, g->Nr, 4 * nw, 1, s, "sources gemm")) return rc;
+  }
   hipLaunchKernelGGL(k_src_norm, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_src_norm");
   return PSA_OK;
@@ -1005,12 +1173,15 @@ int psa_prop2qw(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
 
 int psa_qw2prop(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
-  const int nb = (g->Nx + 255) / 256;
-  if (nb > 128) return fail(PSA_E_INVALID, "Nx must be <= 32768");
   const hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_qw2prop_a, dim3(nw, nb), dim3(256), 0, s, *g, w0);
-  PSA_LAUNCH_CHECK("k_qw2prop_a");
-  hipLaunchKernelGGL(k_qw2prop_b, dim3(nw, nb), dim3(256), 0, s, *g, w0);
+  if (g->bnx > 0) {
+    Qw2pProb p;
+    p.g = *g;
+    p.w0 = w0;
+    p.nw = nw;
+    if (int rc = launch_gemm(p, PSA_PREC_FP64, g->bnx, 5 * nw, 1, s, "qw2prop gemm")) return rc;
+  }
+  hipLaunchKernelGGL(k_qw2prop_b, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_qw2prop_b");
   return PSA_OK;
 }

@@ -65,7 +65,8 @@ class WireGroup:
         self.t = {
             "kr": T(g["kr"]), "Ee": T(g["Ee"]), "Eh": T(g["Eh"]),
             "vtab": T(tb["vtab"]), "chi": T(tb["chi"]), "qeh": T(tb["qeh"]),
-            "EP": T(tb["EP"]), "HP": T(tb["HP"]), "xe": T(tb["xe"]), "xh": T(tb["xh"]),
+            "EP": T(tb["EP"]), "HP": T(tb["HP"]),
+            "EPT": T(tb["EP"].T), "HPT": T(tb["HP"].T), "xe": T(tb["xe"]), "xh": T(tb["xh"]),
             "didx": T(tb["didx"], torch.int32), "iq": T(tb["iq"], torch.int32),
             "homega": T(tb["homega"]), "hsq": T(tb["hsq"]), "window": T(tb["window"]),
             "rq": T(g["r"]), "ytr": T(tb["ytr"]),
@@ -73,6 +74,10 @@ class WireGroup:
             "m2": T(tb["m2"].T if tb["bnx"] else np.zeros((Nr, 1))),
             "phi": T(tb["phi"], C128), "twid": T(tb["twid"], C128),
         }
+        # Lorentzian kernels of the off-diagonal dephasing: [2][N*N][2N],
+        # built on the device once per group (psa_lorentz_table)
+        od = bool(prm.flags.get("OffDiagDph"))
+        self.t["lor"] = torch.zeros((2 * N * N * 2 * N) if od else 1, dtype=F64, device=d)
         self.mb = []
         for off, idx, val in tb["mb"]:
             self.mb.append((T(off, torch.int32),
@@ -103,6 +108,8 @@ class WireGroup:
         self._wid_host = self.wire_ids.copy()
         self.init_state()
         self._sync_struct()
+        if od:
+            capi.check(self.lib.psa_lorentz_table(ctypes.byref(self.S), self._stream()))
 
     # ------------------------------------------------------------------
     def _vint(self, Qr, r, ae, ah, Delta0, pref):

@@ -120,10 +120,11 @@ def test_calc_h_stage(case64):
         VC, _ = O.screened_arrays(c.og, c.ot, True, np.diag(C3), np.diag(D3))
         Meh = O.calc_meh(c.og, c.ot, Ex, Ey, Ez, w + 1)
         Heh, Hee, Hhh = O.calc_h(c.og, Meh, C3, D3, p3, VC, True, True)
-        assert rel(Heh_g[w], Heh) < 1e-12
+        # the device stores the Hamiltonians transposed ([k2][k1])
+        assert rel(Heh_g[w].T, Heh) < 1e-12
         off = lambda m: m - np.diag(np.diag(m))  # noqa: E731
-        assert rel(Heo[w], off(Hee)) < 1e-12
-        assert rel(Hho[w], off(Hhh)) < 1e-12
+        assert rel(Heo[w].T, off(Hee)) < 1e-12
+        assert rel(Hho[w].T, off(Hhh)) < 1e-12
         assert rel(hd[w, 0], np.diag(Hee)) < 1e-14
         assert rel(hd[w, 1], np.diag(Hhh)) < 1e-14
 
