@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = "SBE steps/s (Nk=256, 64 wires) mixed vs FP64 @<=1e-6 err; p50/p99 sim latency"
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0  # dense
 
 
 def wire_length(nk, me_rel=0.067, gap_ev=1.5):
@@ -116,12 +117,12 @@ def time_rhs_kernel(wg, reps):
     return s.elapsed_time(e) / reps / 1e3  # seconds
 
 
-def accuracy_vs_fp64(N, nx, nw_err, steps, dt, seed):
-    """Density and polarization error of the mixed path against the FP64
-    path after `steps` steps from the same state."""
+def accuracy_vs_fp64(prec, N, nx, nw_err, steps, dt, seed):
+    """Density and polarization error of a reduced-precision mode against
+    the FP64 path after `steps` steps from the same state."""
     import torch
     a, _, _ = build_group("fp64", nw_err, N, nx, seed)
-    b, _, _ = build_group("fp32", nw_err, N, nx, seed)
+    b, _, _ = build_group(prec, nw_err, N, nx, seed)
     for _ in range(steps):
         a.step(dt)
         b.step(dt)
@@ -183,7 +184,6 @@ def main():
     ap.add_argument("--wires", type=int, default=64)
     ap.add_argument("--nk", type=int, default=256)
     ap.add_argument("--nx", type=int, default=4096)
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
@@ -194,40 +194,50 @@ def main():
     import torch
     dt = 10e-18
     seed = 1234
-    print(f"[bench] building {args.wires} wires Nk={args.nk} Nx={args.nx} fp64", file=sys.stderr, flush=True)
-    wg, prm, rr = build_group("fp64", args.wires, args.nk, args.nx, seed)
-    N = wg.N
-    print("[bench] timing fp64", file=sys.stderr, flush=True)
-    wall64, lat64 = time_steps(wg, args.steps, args.warmup, dt)
-    sps64 = args.steps / wall64
-    t_rhs = time_rhs_kernel(wg, 20)
-    flops_rhs = args.wires * 104.0 * N**3
-    achieved = flops_rhs / t_rhs / 1e12
-    res = dict(fp64_steps_per_s=sps64, fp64_ms_per_step=1e3 * wall64 / args.steps,
-               fp64_p50_ms=float(np.percentile(lat64, 50)),
-               fp64_p99_ms=float(np.percentile(lat64, 99)))
-    value, ms, lat, dtype = sps64, 1e3 * wall64 / args.steps, lat64, "f64"
-    if not args.no_mixed:
+    modes = ["fp64"] + ([] if args.no_mixed else ["fp32", "bf16x3"])
+    res, runs = {}, {}
+    for mode in modes:
         try:
-            print("[bench] timing fp32", file=sys.stderr, flush=True)
+            print(f"[bench] building {args.wires} wires Nk={args.nk} Nx={args.nx} {mode}",
+                  file=sys.stderr, flush=True)
+            wg, prm, rr = build_group(mode, args.wires, args.nk, args.nx, seed)
+            N = wg.N
+            print(f"[bench] timing {mode}", file=sys.stderr, flush=True)
+            wall, lat = time_steps(wg, args.steps, args.warmup, dt)
+            t_rhs = time_rhs_kernel(wg, 20)
+            runs[mode] = dict(sps=args.steps / wall, ms=1e3 * wall / args.steps, lat=lat, t_rhs=t_rhs)
+            res[f"{mode}_steps_per_s"] = args.steps / wall
+            res[f"{mode}_ms_per_step"] = 1e3 * wall / args.steps
+            res[f"{mode}_p50_ms"] = float(np.percentile(lat, 50))
+            res[f"{mode}_p99_ms"] = float(np.percentile(lat, 99))
+            res[f"{mode}_rhs_launch_ms"] = t_rhs * 1e3
+            if mode != "fp64":
+                res[f"{mode}_speedup_vs_fp64"] = runs[mode]["sps"] / runs["fp64"]["sps"]
             del wg
             torch.cuda.empty_cache()
-            wg32, _, _ = build_group("fp32", args.wires, args.nk, args.nx, seed)
-            wall32, lat32 = time_steps(wg32, args.steps, args.warmup, dt)
-            res.update(mixed_steps_per_s=args.steps / wall32,
-                       mixed_p50_ms=float(np.percentile(lat32, 50)),
-                       mixed_p99_ms=float(np.percentile(lat32, 99)),
-                       mixed_speedup_vs_fp64=(args.steps / wall32) / sps64)
-            del wg32
-            torch.cuda.empty_cache()
-            err = accuracy_vs_fp64(args.nk, args.nx, 4, 20, dt, seed)
-            res["mixed_error_vs_fp64_after_20_steps"] = err
-            if max(err["density"], err["polarization"]) <= 1e-6:
-                value, ms, lat, dtype = args.steps / wall32, 1e3 * wall32 / args.steps, lat32, "f32+f64"
-            wg, _, _ = build_group("fp64", args.wires, args.nk, args.nx, seed)
-        except Exception as ex:  # report, keep the FP64 number
-            res["mixed_error"] = repr(ex)
-            wg, _, _ = build_group("fp64", args.wires, args.nk, args.nx, seed)
+            if mode != "fp64":
+                err = accuracy_vs_fp64(mode, args.nk, args.nx, 4, 20, dt, seed)
+                res[f"{mode}_error_vs_fp64_after_20_steps"] = err
+                runs[mode]["ok"] = max(err["density"], err["polarization"]) <= 1e-6
+        except Exception as ex:  # report, keep the other modes
+            if mode == "fp64":
+                raise
+            res[f"{mode}_error"] = repr(ex)
+    # the reported value: the fastest mode whose density and polarization
+    # stay within 1e-6 of FP64 (FP64 itself otherwise)
+    best = "fp64"
+    for mode in modes[1:]:
+        if runs.get(mode, {}).get("ok") and runs[mode]["sps"] > runs[best]["sps"]:
+            best = mode
+    r = runs[best]
+    value, ms, lat = r["sps"], r["ms"], r["lat"]
+    dtype = {"fp64": "f64", "fp32": "f32+f64", "bf16x3": "bf16x3+f64"}[best]
+    t_rhs = r["t_rhs"]
+    flops_rhs = args.wires * 104.0 * N**3
+    achieved = flops_rhs / t_rhs / 1e12
+    peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS}[best]
+    res["reported_mode"] = best
+    wg, prm, rr = build_group("fp64", args.wires, args.nk, args.nx, seed) if not args.no_cpu else (None, prm, rr)
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -250,9 +260,10 @@ def main():
                                "one QWCalculator step of every wire per step",
                    "wires": args.wires, "Nk": N, "Nx": args.nx},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "roofline": {"bound": "mfma", "kernel": "rhs gemm (dp/dC/dD + leapfrog), FP64 MFMA",
-                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+        "roofline": {"bound": "mfma",
+                     "kernel": f"rhs (dp/dC/dD + leapfrog), {best} contractions",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
                      "avg_launch_ms": t_rhs * 1e3,
                      "flops_per_launch": flops_rhs},
         "cpu_baseline": cpu,

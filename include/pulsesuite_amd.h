@@ -111,7 +111,10 @@ typedef struct psa_group {
   double *out;                        /* [4][Nw][Nx] complex: Pxx, Pyy, Pzz, Rho */
   double *obs;                        /* [Nw][PSA_OBS_N] observables (I0, totals, ...) */
   double *tphase;                     /* [Nw][N] complex Transport phase exp(-i Y dk) */
-  double *part;                       /* [Nw][64] partial sums (QW2Prop normalisation) */
+  double *part;                       /* [Nw][256] scratch (reserved) */
+  void *spl;                          /* [Nw][12][4][N][N] bf16 split operands, PSA_PREC_BF16X3 only:
+                                         slot = 2 matrix + transposed (p, C_o, D_o, Heh, Hee_o, Hhh_o),
+                                         planes re_hi, re_lo, im_hi, im_lo */
   const int32_t *wid_host;            /* [Nw] host copy of wire_id */
 } psa_group;
 

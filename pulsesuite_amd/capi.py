@@ -24,7 +24,7 @@ _TABS = ["kr", "Ee", "Eh", "vtab", "chi", "qeh", "EP", "HP", "EPT", "HPT", "xe",
          "iq", "homega", "hsq", "window", "rq", "ytr", "gmat", "m2", "phi", "twid", "lor"]
 _WIRE = ["wire_id", "pA", "pB", "pX", "CA", "CB", "CX", "DA", "DB", "DX", "Heh",
          "Heo", "Hho", "hdiag", "prep", "dedh", "wmb", "exq", "efield", "src",
-         "out", "obs", "tphase", "part", "wid_host"]
+         "out", "obs", "tphase", "part", "spl", "wid_host"]
 
 
 class PsaGroup(ctypes.Structure):

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "psa_gemm.h"
+#include "psa_rhs_bf16.h"
 
 using namespace psa;
 
@@ -968,6 +969,28 @@ int launch_gemm(const P &p, int prec, int M, int Ncols, int batches, hipStream_t
   return launch_gemm_t<float>(p, grid, s, what);
 }
 
+// Split-bf16 RHS: operand planes once per step, then the bf16 MFMA products.
+int rhs_bf16(const RhsProb &p, int nw, hipStream_t s) {
+  const psa_group *g = &p.g;
+  if (!g->spl) return fail(PSA_E_INVALID, "bf16x3 mode needs the spl operand buffer");
+  if (g->N % 8) return fail(PSA_E_UNSUPPORTED, "bf16x3 mode needs N divisible by 8");
+  const int nt = (g->N + 63) / 64;
+  hipLaunchKernelGGL(k_split_bf16, dim3(nt, nt, 6 * nw), dim3(256), 0, s, *g, p.w0);
+  PSA_LAUNCH_CHECK("k_split_bf16");
+  constexpr size_t shm = rhs_bf16_lds_bytes();
+  static bool attr_set = false;
+  if (!attr_set) {
+    int rc = hip_check(hipFuncSetAttribute((const void *)k_rhs_bf16,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
+                       "hipFuncSetAttribute(k_rhs_bf16)");
+    if (rc) return rc;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_rhs_bf16, dim3(nt, nt, 3 * nw), dim3(256), shm, s, p);
+  PSA_LAUNCH_CHECK("k_rhs_bf16");
+  return PSA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1041,6 +1064,7 @@ int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt,
   p.w0 = w0;
   p.mode = mode;
   p.dt = dt;
+  if (g->prec == PSA_PREC_BF16X3) return rhs_bf16(p, nw, (hipStream_t)stream);
   return launch_gemm(p, g->prec, g->N, g->N, 3 * nw, (hipStream_t)stream, "rhs gemm");
 }
 

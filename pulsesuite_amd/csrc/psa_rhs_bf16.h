@@ -1,0 +1,323 @@
+// Split-BF16 right-hand sides (PSA_PREC_BF16X3) for gfx950.
+//
+// Every FP64 operand x of the RHS contractions is written once per step as
+// hi = bf16(x), lo = bf16(x - hi) (relative residual about 2^-17), for the
+// real and the imaginary part: four bf16 planes per matrix and orientation
+// (k_split_bf16).  A real product a b is then a_hi b_hi + a_hi b_lo +
+// a_lo b_hi (the lo*lo term, ~2^-16 relative, is dropped), each a
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation; a complex product is
+// four real ones, 12 MFMAs per 16x16x32 block.  The diagonal parts, the
+// epilogue and the state stay in FP64 (RhsProb::store).
+//
+// Operand slots per wire (plane [N][N] bf16, row-major, k contiguous):
+// slot = 2 * matrix + orientation, matrix 0 p, 1 C_o, 2 D_o (diagonal
+// zeroed), 3 Heh, 4 Hee_o, 5 Hhh_o; orientation 0 the matrix, 1 its
+// transpose.  An A operand A(m, k) reads slot rows m; a B operand B(k, n)
+// reads slot rows n (both k-contiguous, as the MFMA lanes take them:
+// A[l%16][8(l/16)+e], B[8(l/16)+e][l%16]).
+#pragma once
+
+#include "psa_gemm.h"
+
+namespace psa {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t bf16_bits_rn(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ void split_bf16(double x, uint16_t &hi, uint16_t &lo) {
+  hi = bf16_bits_rn((float)x);
+  lo = bf16_bits_rn((float)(x - (double)bf16_to_f(hi)));
+}
+
+__host__ __device__ inline long spl_plane(int N, int w, int slot, int plane) {
+  return (((long)w * 12 + slot) * 4 + plane) * (long)N * N;
+}
+
+// ---------------------------------------------------------------------------
+// k_split_bf16: block (64x64 tile of the stored buffer, matrix, wire).
+// Stored buffers: p, C, D as the matrix; Heh, Heo, Hho as its transpose.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_split_bf16(psa_group g, int w0) {
+  __shared__ uint16_t t[4][64][66];
+  const int N = g.N, mat = blockIdx.z % 6, w = w0 + blockIdx.z / 6;
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64, tid = threadIdx.x;
+  const long o = mat_off(g, w);
+  const double *X = (mat == 0 ? g.pB : mat == 1 ? g.CB : mat == 2 ? g.DB
+                     : mat == 3 ? g.Heh : mat == 4 ? g.Heo : g.Hho) + o;
+  const bool zd = (mat == 1 || mat == 2);
+  const bool stored_t = mat >= 3;  // buffer holds the transpose of the matrix
+  uint16_t *spl = reinterpret_cast<uint16_t *>(g.spl);
+  // slot receiving the buffer as stored, and the one receiving its transpose
+  const int s_same = 2 * mat + (stored_t ? 1 : 0), s_tr = 2 * mat + (stored_t ? 0 : 1);
+#pragma unroll 4
+  for (int e = 0; e < 16; ++e) {
+    const int idx = tid + 256 * e;
+    const int i = idx >> 6, j = idx & 63;  // tile row, column
+    const int gi = r0 + i, gj = c0 + j;
+    uint16_t h[4] = {0, 0, 0, 0};
+    if (gi < N && gj < N) {
+      cd v = ld(X, (long)gi * N + gj);
+      if (zd && gi == gj) v = cd{0.0, 0.0};
+      split_bf16(v.re, h[0], h[1]);
+      split_bf16(v.im, h[2], h[3]);
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) spl[spl_plane(N, w, s_same, pl) + (long)gi * N + gj] = h[pl];
+    }
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl) t[pl][i][j] = h[pl];
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int e = 0; e < 16; ++e) {
+    const int idx = tid + 256 * e;
+    const int j = idx >> 6, i = idx & 63;  // transposed: row j, column i
+    const int gi = r0 + i, gj = c0 + j;
+    if (gi < N && gj < N)
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) spl[spl_plane(N, w, s_tr, pl) + (long)gj * N + gi] = t[pl][i][j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_rhs_bf16: block = 64x64 output tile of one (wire, equation); 4 waves
+// (2x2) of 32x32 = 2x2 MFMA blocks; K slabs of 32 staged in LDS (row stride
+// 40 bf16 = 80 B: the 16 rows a lane group reads land on distinct banks).
+// ---------------------------------------------------------------------------
+constexpr int BK2 = 32, BLD2 = 40;
+
+struct BTile {
+  uint16_t v[4][64][BLD2];  // [plane][row][k]
+};
+
+// One term of an equation: A and B slots (-1: the off-diagonal dephasing
+// band, generated from dedh), conjugation and sign.
+struct BTerm {
+  int sa, sb;
+  bool ca, cb;
+  double sg;
+};
+
+__device__ __forceinline__ BTerm bterm(int e, int t) {
+  switch (e * 8 + t) {
+    case 0: return {10, 1, false, false, 1.0};   // dp:  Hhh_o p
+    case 1: return {0, 8, false, false, 1.0};    //      p Hee_o^T
+    case 2: return {7, 3, false, false, -1.0};   //     -Heh^T C_o
+    case 3: return {5, 6, false, false, -1.0};   //     -D_o^T Heh^T
+    case 4: return {-1, 1, false, false, 1.0};   //      (i hbar Dh band) p
+    case 5: return {0, -1, false, false, 1.0};   //      p (i hbar De band)
+    case 8: return {2, 8, false, false, 1.0};    // dC:  C_o Hee_o^T
+    case 9: return {9, 3, false, false, -1.0};   //     -Hee_o^T C_o
+    case 10: return {1, 6, true, false, 1.0};    //      p^H Heh^T
+    case 11: return {6, 1, true, false, -1.0};   //     -conj(Heh) p
+    case 16: return {4, 10, false, false, 1.0};  // dD:  D_o Hhh_o^T
+    case 17: return {11, 5, false, false, -1.0}; //     -Hhh_o^T D_o
+    case 18: return {0, 7, true, false, 1.0};    //      conj(p) Heh
+    default: return {7, 0, true, false, -1.0};   //     -Heh^H p^T
+  }
+}
+
+// Register fragment of one slab: 4 planes x one 16-byte chunk (8 k).
+struct BFrag {
+  u32x4 c[4];
+  int row, kc;
+};
+
+// Load the slab [k0, k0+32) of rows [x0, x0+64) of a dense slot.
+__device__ __forceinline__ void bload_dense(BFrag &f, const uint16_t *spl, int N, int w, int slot,
+                                            int x0, int k0) {
+  const int tid = threadIdx.x;
+  f.row = tid >> 2;
+  f.kc = (tid & 3) * 8;
+  const int gx = x0 + f.row, gk = k0 + f.kc;
+#pragma unroll
+  for (int pl = 0; pl < 4; ++pl) {
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if (gx < N) {
+      const uint16_t *src = spl + spl_plane(N, w, slot, pl) + (long)gx * N + gk;
+      if (gk + 8 <= N) {
+        v = *reinterpret_cast<const u32x4 *>(src);
+      } else {
+        uint16_t tmp[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) tmp[e] = (gk + e < N) ? src[e] : (uint16_t)0;
+        v = __builtin_bit_cast(u32x4, tmp);
+      }
+    }
+    f.c[pl] = v;
+  }
+}
+
+// The banded dephasing operand i hbar D[m - x][x] for rows x = x0.. and
+// k = m (dedh table kind 1 = Dh as A, 0 = De as B; q = 0 excluded): purely
+// imaginary, so the real planes are zero.
+__device__ __forceinline__ void bload_band(BFrag &f, const psa_group &g, int w, int kind, int x0,
+                                           int k0) {
+  const int N = g.N, Q = 2 * N - 1, tid = threadIdx.x;
+  f.row = tid >> 2;
+  f.kc = (tid & 3) * 8;
+  const int gx = x0 + f.row, gk = k0 + f.kc;
+  const double *tab = g.dedh + ((long)w * 2 + kind) * (long)N * Q;
+  uint16_t hi[8], lo[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int m = gk + e;
+    double v = 0.0;
+    if (gx < N && m < N && m != gx) v = g.hbar * tab[(long)gx * Q + (m - gx + N - 1)];
+    split_bf16(v, hi[e], lo[e]);
+  }
+  f.c[0] = u32x4{0u, 0u, 0u, 0u};
+  f.c[1] = u32x4{0u, 0u, 0u, 0u};
+  f.c[2] = __builtin_bit_cast(u32x4, hi);
+  f.c[3] = __builtin_bit_cast(u32x4, lo);
+}
+
+__device__ __forceinline__ void bcommit(const BFrag &f, BTile &t) {
+#pragma unroll
+  for (int pl = 0; pl < 4; ++pl) *reinterpret_cast<u32x4 *>(&t.v[pl][f.row][f.kc]) = f.c[pl];
+}
+
+__device__ __forceinline__ bf16x8 bfrag_read(const BTile &t, int pl, int row, int kq) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(&t.v[pl][row][kq]));
+}
+
+__device__ __forceinline__ bf16x8 bneg(bf16x8 x) {
+  u32x4 u = __builtin_bit_cast(u32x4, x);
+  u ^= u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ fv4 mma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, fv4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+__global__ __launch_bounds__(256) void k_rhs_bf16(RhsProb p) {
+  extern __shared__ char lds_raw[];
+  BTile *tiles = reinterpret_cast<BTile *>(lds_raw);  // [2 buffers][A, B]
+  const psa_group &g = p.g;
+  const int N = g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  // XCD-aware order as in gemm_kernel: batch b on XCD b % 8 when possible
+  const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
+  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  int b;
+  if (gridDim.z % 8 == 0) {
+    const unsigned xcd = lin % 8, q = lin / 8;
+    b = xcd + 8 * (q / tpb);
+    lin = q % tpb;
+  } else {
+    b = lin / tpb;
+    lin %= tpb;
+  }
+  const int m0 = (lin / gx) * GBM, n0 = (lin % gx) * GBN;
+  const int e = b % 3, w = p.w0 + b / 3;
+  const int nt = p.nterms(b);
+  const uint16_t *spl = reinterpret_cast<const uint16_t *>(g.spl);
+  fv4 aR[2][2], aI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      aR[i][j] = fv4{0, 0, 0, 0};
+      aI[i][j] = fv4{0, 0, 0, 0};
+    }
+  auto fetch = [&](BFrag &fa, BFrag &fb, int t, int k0) {
+    const BTerm bt = bterm(e, t);
+    if (bt.sa >= 0) bload_dense(fa, spl, N, w, bt.sa, m0, k0);
+    else bload_band(fa, g, w, 1, m0, k0);
+    if (bt.sb >= 0) bload_dense(fb, spl, N, w, bt.sb, n0, k0);
+    else bload_band(fb, g, w, 0, n0, k0);
+  };
+  BFrag fa, fb;
+  int t = 0, k0 = 0;
+  fetch(fa, fb, 0, 0);
+  bcommit(fa, tiles[0]);
+  bcommit(fb, tiles[1]);
+  __syncthreads();
+  int cur = 0;
+  const int kl = lane >> 4, xl = lane & 15;
+  while (true) {
+    int tn = t, kn = k0 + BK2;
+    if (kn >= N) {
+      tn = t + 1;
+      kn = 0;
+    }
+    const bool more = tn < nt;
+    if (more) fetch(fa, fb, tn, kn);
+    const BTile &sA = tiles[2 * cur], &sB = tiles[2 * cur + 1];
+    const BTerm bt = bterm(e, t);
+    // signs of the four real products (conj flips the imaginary part)
+    const double cA = bt.ca ? -1.0 : 1.0, cB = bt.cb ? -1.0 : 1.0;
+    const bool nRR = bt.sg < 0, nII = -bt.sg * cA * cB < 0, nRI = bt.sg * cB < 0, nIR = bt.sg * cA < 0;
+    const bool aRe0 = bt.sa < 0, bRe0 = bt.sb < 0;  // band operands are purely imaginary
+    bf16x8 A[2][4], B[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) {
+        A[s][pl] = bfrag_read(sA, pl, wm * 32 + s * 16 + xl, kl * 8);
+        B[s][pl] = bfrag_read(sB, pl, wn * 32 + s * 16 + xl, kl * 8);
+      }
+    if (!aRe0 && !bRe0) {  // Re A Re B
+#pragma unroll
+      for (int si = 0; si < 2; ++si) {
+        const bf16x8 h = nRR ? bneg(A[si][0]) : A[si][0], l = nRR ? bneg(A[si][1]) : A[si][1];
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) aR[si][sj] = mma3(h, l, B[sj][0], B[sj][1], aR[si][sj]);
+      }
+    }
+    if (!aRe0) {  // Re A Im B
+#pragma unroll
+      for (int si = 0; si < 2; ++si) {
+        const bf16x8 h = nRI ? bneg(A[si][0]) : A[si][0], l = nRI ? bneg(A[si][1]) : A[si][1];
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) aI[si][sj] = mma3(h, l, B[sj][2], B[sj][3], aI[si][sj]);
+      }
+    }
+    {  // Im A Im B (subtracted)
+#pragma unroll
+      for (int si = 0; si < 2; ++si) {
+        const bf16x8 h = nII ? bneg(A[si][2]) : A[si][2], l = nII ? bneg(A[si][3]) : A[si][3];
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) aR[si][sj] = mma3(h, l, B[sj][2], B[sj][3], aR[si][sj]);
+      }
+    }
+    if (!bRe0) {  // Im A Re B
+#pragma unroll
+      for (int si = 0; si < 2; ++si) {
+        const bf16x8 h = nIR ? bneg(A[si][2]) : A[si][2], l = nIR ? bneg(A[si][3]) : A[si][3];
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) aI[si][sj] = mma3(h, l, B[sj][0], B[sj][1], aI[si][sj]);
+      }
+    }
+    if (!more) break;
+    bcommit(fa, tiles[2 * (cur ^ 1)]);
+    bcommit(fb, tiles[2 * (cur ^ 1) + 1]);
+    __syncthreads();
+    cur ^= 1;
+    t = tn;
+    k0 = kn;
+  }
+#pragma unroll
+  for (int si = 0; si < 2; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 2; ++sj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = m0 + wm * 32 + si * 16 + 4 * kl + v;
+        const int col = n0 + wn * 32 + sj * 16 + xl;
+        p.store(b, row, col, (double)aR[si][sj][v], (double)aI[si][sj][v]);
+      }
+}
+
+constexpr size_t rhs_bf16_lds_bytes() { return 4 * sizeof(BTile); }
+
+}  // namespace psa

@@ -99,6 +99,9 @@ class WireGroup:
         self.s["obs"] = z(nwz, 8, dt=F64)
         self.s["tphase"] = z(nwz, N)
         self.s["part"] = z(nwz, 256, dt=F64)
+        # split-bf16 operand planes (bf16x3 only): [Nw][12][4][N][N] bf16
+        nspl = nwz * 12 * 4 * N * N if prec == "bf16x3" else 1
+        self.s["spl"] = torch.zeros(nspl, dtype=torch.int16, device=d)
         if wire_ids is None:
             wire_ids = np.arange(1, nwz + 1)
         self.wire_ids = np.asarray(wire_ids, dtype=np.int32)

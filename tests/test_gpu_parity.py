@@ -277,3 +277,32 @@ def test_mixed_precision_fp32_error_bounded():
     print("fp32 state errors", errs, "Pxx", rel(pb, pa))
     assert max(errs) < 1e-6
     assert rel(pb, pa) < 1e-6
+
+
+def test_bf16x3_rhs_and_state_error_bounded():
+    """Split-bf16 contractions (hi/lo planes, three products, FP32
+    accumulation; diagonals, epilogue and state in FP64): one RHS evaluation
+    against the FP64 path (normwise), then five steps of the state.  The
+    emulated per-product error of the split is ~2^-17 relative, ten times
+    FP32's, so the bounds here are ten times those of the FP32 test."""
+    import torch
+    a = Case(64, 2, seed=21)
+    b = Case(64, 2, seed=21, prec="bf16x3")
+    a.wg.rhs_eval()
+    b.wg.rhs_eval()
+    torch.cuda.synchronize()
+    rerr = [rel(b.wg.s[k].cpu().numpy(), a.wg.s[k].cpu().numpy()) for k in ("pX", "CX", "DX")]
+    print("bf16x3 rhs errors", rerr)
+    assert max(rerr) < 1e-4
+    a = Case(64, 2, seed=21)
+    b = Case(64, 2, seed=21, prec="bf16x3")
+    for _ in range(5):
+        a.wg.step(1e-17)
+        b.wg.step(1e-17)
+    torch.cuda.synchronize()
+    Sa, Sb = a.wg.get_state(), b.wg.get_state()
+    errs = [rel(x, y) for x, y in zip(Sb, Sa)]
+    pa, pb = a.wg.outputs()[0], b.wg.outputs()[0]
+    print("bf16x3 state errors", errs, "Pxx", rel(pb, pa))
+    assert max(errs) < 1e-5
+    assert rel(pb, pa) < 1e-5
