@@ -77,7 +77,7 @@ typedef struct psa_group {
   const double *EP, *HP;              /* [N][N] phonon tables */
   const double *EPT, *HPT;            /* [N][N] their transposes */
   const double *xe, *xh;              /* [N] dephasing weights */
-  const int32_t *didx;                /* [6][N][N] k_p_q, k_m_q, k1p_m_q, k1p, k1_m_q, k1 */
+  const int32_t *didx;                /* [6][N k][N q] k_p_q, k_m_q, k1p_m_q, k1p, k1_m_q, k1 */
   const int32_t *iq;                  /* [N] Vxx2 index rint(|ky/dq|) */
   const double *homega, *hsq;         /* [Nhw] emission grid and weights */
   const double *window;               /* [Nr] QW window */
@@ -103,8 +103,9 @@ typedef struct psa_group {
   double *hdiag;                      /* [Nw][2][N] complex: diagonals of Hee, Hhh */
   double *prep;                       /* [Nw][PSA_PREP_STRIDE(N)] doubles, see PSA_PREP_* */
   double *dedh;                       /* [Nw][2][2N-1][N] off-diagonal dephasing tables */
-  double *wmb;                        /* 9 N Nw doubles: [Nw][4][N] many-body Win_e, Wout_e, Win_h, Wout_h;
-                                         then [2][N][Nw] |Re ne3|, |Re nh3| and [3][N][Nw] VC(d)^2 (wire fastest) */
+  double *wmb;                        /* 11 N Nw doubles: [Nw][4][N] many-body Win_e, Wout_e, Win_h, Wout_h;
+                                         then, wire fastest: [2][N][Nw] |Re ne3|, |Re nh3| of the new level,
+                                         [3][N][Nw] VC(d)^2 (eh, ee, hh), [2][N][Nw] |ne|, |nh| of the checkout level */
   double *exq;                        /* [Nw][3][2N-1] complex: FFTG'd Ex, Ey, Ez at q = -(N-1)..N-1 */
   const double *efield;               /* [3][Nw][Nx] complex fields on rr (Exx, Eyy, Ezz) */
   double *src;                        /* [Nw][PSA_SRC_STRIDE] real-space Px, Py, Pz, re, rh + diagonal sums */
@@ -112,9 +113,11 @@ typedef struct psa_group {
   double *obs;                        /* [Nw][PSA_OBS_N] observables (I0, totals, ...) */
   double *tphase;                     /* [Nw][N] complex Transport phase exp(-i Y dk) */
   double *part;                       /* [Nw][256] scratch (reserved) */
-  void *spl;                          /* [Nw][12][4][N][N] bf16 split operands, PSA_PREC_BF16X3 only:
-                                         slot = 2 matrix + transposed (p, C_o, D_o, Heh, Hee_o, Hhh_o),
-                                         planes re_hi, re_lo, im_hi, im_lo */
+  void *spl;                          /* bf16 split operands, PSA_PREC_BF16X3 only, zero-initialised:
+                                         [Nw][12 slots][NR/64][NK/32][4 planes][64][32] with
+                                         NR = 64 ceil(N/64), NK = 32 ceil(N/32); slot = 2 matrix +
+                                         transposed (p, C_o, D_o, Heh, Hee_o, Hhh_o), planes
+                                         re_hi, re_lo, im_hi, im_lo */
   const int32_t *wid_host;            /* [Nw] host copy of wire_id */
 } psa_group;
 

@@ -604,10 +604,14 @@ struct OffdiagProb {
     }
   }
 
+  // B[p][(w, s)]: V^2(q) times the occupation factor, from the wire-fastest
+  // tables (|n| of the checkout level, VC^2), so lanes (columns) read
+  // consecutive wires.
   template <typename T>
   __device__ void fetchB(Frag<T> &s, int t, int b, int k0, int n0) const {
-    const int N = g.N, q = b / 2 - (N - 1), kind = b % 2;
+    const int N = g.N, W = g.Nw, q = b / 2 - (N - 1), kind = b % 2;
     const int aq = q > 0 ? q : -q;
+    const double *vsq = g.wmb + 6L * N * W, *aocc = g.wmb + 9L * N * W;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -617,28 +621,23 @@ struct OffdiagProb {
       double v = 0.0;
       if (pp < 2 * N && col < 2 * nw && aq > 0 && (t == 0) == (pp < N)) {
         const int w = w0 + (col >> 1), sA = col & 1;
-        const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-        const double *ne = pr + PSA_PREP_ANE(N), *nh = pr + PSA_PREP_ANH(N);
-        const double *vc = pr + PSA_PREP_VC(N);
         // same-carrier occupations: n[p-q] (1-n[p]) | (1-n[p-q]) n[p]
         // other-carrier:            o[p+q] (1-o[p]) | (1-o[p+q]) o[p]
-        const double *ns = kind == 0 ? ne : nh;
-        const double *no = kind == 0 ? nh : ne;
+        const double *ns = aocc + (kind == 0 ? 0L : (long)N * W) + w;
+        const double *no = aocc + (kind == 0 ? (long)N * W : 0L) + w;
         if (pp < N) {
           const int p = pp, pm = p - q;
           if (pm >= 1 && pm < N) {
-            const double a = ns[pm];
-            const double c = ns[p];
-            const double V = vc[(kind == 0 ? 1 : 2) * N + aq];
-            v = V * V * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
+            const double a = ns[(long)pm * W], c = ns[(long)p * W];
+            const double V2 = vsq[((long)(kind == 0 ? 1 : 2) * N + aq) * W + w];
+            v = V2 * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
           }
         } else {
           const int p = pp - N, pq = p + q;
           if (pq >= 0 && pq < N) {
-            const double a = no[pq];
-            const double c = no[p];
-            const double V = vc[aq];
-            v = V * V * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
+            const double a = no[(long)pq * W], c = no[(long)p * W];
+            const double V2 = vsq[(long)aq * W + w];
+            v = V2 * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
           }
         }
       }

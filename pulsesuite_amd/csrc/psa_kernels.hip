@@ -200,11 +200,11 @@ __global__ __launch_bounds__(256) void k_prep_b(psa_group g, int w0) {
   } else {
     const long NN = (long)N * N;
     const int *kpq = g.didx, *kmq = g.didx + NN, *k1pmq = g.didx + 2 * NN, *k1p = g.didx + 3 * NN,
-              *k1mq = g.didx + 4 * NN, *k1 = g.didx + 5 * NN;  // stored [q][k]
+              *k1mq = g.didx + 4 * NN, *k1 = g.didx + 5 * NN;  // stored [k][q]
     const double ph = PI / g.hbar;
     double ge = 0.0, gh = 0.0;
     for (int q = lane; q < N; q += 64) {
-      const long qk = (long)q * N + k;
+      const long qk = (long)k * N + q;
       const int iq = g.iq[q];
       const double Vee = vc[N + iq], Veh = vc[iq], Vhh = vc[2 * N + iq];
       const int i = kpq[qk];
@@ -409,20 +409,31 @@ __global__ __launch_bounds__(256) void k_mb(psa_group g, int w0, int nw) {
   }
 }
 
-// Wire-fastest occupation and V^2 tables for k_mb from the new level's
-// diagonals (|Re|, coulomb.py:1307-1310) and the screened VC(d).
+// Wire-fastest occupations of the new level's diagonals for k_mb (|Re|,
+// coulomb.py:1307-1310).
 __global__ void k_diag_gather(psa_group g, int w0) {
   const int w = w0 + blockIdx.x, N = g.N, W = g.Nw;
   const long o = mat_off(g, w);
-  double *occ = g.wmb + 4L * N * W, *vsq = g.wmb + 6L * N * W;
-  const double *vc = g.prep + (long)w * PSA_PREP_STRIDE(N) + PSA_PREP_VC(N);
+  double *occ = g.wmb + 4L * N * W;
   for (int k = threadIdx.x; k < N; k += blockDim.x) {
     occ[(long)k * W + w] = fabs(ld(g.CX + o, (long)k * N + k).re);
     occ[(long)(N + k) * W + w] = fabs(ld(g.DX + o, (long)k * N + k).re);
+  }
+}
+
+// Wire-fastest tables from prep for k_mb and the off-diagonal dephasing
+// products: VC(d)^2 per channel and |ne|, |nh| of the checkout level.
+__global__ void k_wire_tables(psa_group g, int w0) {
+  const int w = w0 + blockIdx.x, N = g.N, W = g.Nw;
+  const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+  double *vsq = g.wmb + 6L * N * W, *aocc = g.wmb + 9L * N * W;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
     for (int ch = 0; ch < 3; ++ch) {
-      const double v = vc[ch * N + k];
+      const double v = pr[PSA_PREP_VC(N) + ch * N + k];
       vsq[((long)ch * N + k) * W + w] = v * v;
     }
+    aocc[(long)k * W + w] = pr[PSA_PREP_ANE(N) + k];
+    aocc[(long)(N + k) * W + w] = pr[PSA_PREP_ANH(N) + k];
   }
 }
 
@@ -973,7 +984,6 @@ int launch_gemm(const P &p, int prec, int M, int Ncols, int batches, hipStream_t
 int rhs_bf16(const RhsProb &p, int nw, hipStream_t s) {
   const psa_group *g = &p.g;
   if (!g->spl) return fail(PSA_E_INVALID, "bf16x3 mode needs the spl operand buffer");
-  if (g->N % 8) return fail(PSA_E_UNSUPPORTED, "bf16x3 mode needs N divisible by 8");
   const int nt = (g->N + 63) / 64;
   hipLaunchKernelGGL(k_split_bf16, dim3(nt, nt, 6 * nw), dim3(256), 0, s, *g, p.w0);
   PSA_LAUNCH_CHECK("k_split_bf16");
@@ -1019,6 +1029,8 @@ int psa_prep(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   PSA_LAUNCH_CHECK("k_diagsums");
   hipLaunchKernelGGL(k_prep_b, dim3(nw, (g->N + 3) / 4), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_prep_b");
+  hipLaunchKernelGGL(k_wire_tables, dim3(nw), dim3(256), 0, s, *g, w0);
+  PSA_LAUNCH_CHECK("k_wire_tables");
   return PSA_OK;
 }
 

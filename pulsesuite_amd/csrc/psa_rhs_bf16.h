@@ -9,7 +9,8 @@
 // four real ones, 12 MFMAs per 16x16x32 block.  The diagonal parts, the
 // epilogue and the state stay in FP64 (RhsProb::store).
 //
-// Operand slots per wire (plane [N][N] bf16, row-major, k contiguous):
+// Operand slots per wire (four bf16 planes, rows k-contiguous, blocked as
+// described at spl_chunk):
 // slot = 2 * matrix + orientation, matrix 0 p, 1 C_o, 2 D_o (diagonal
 // zeroed), 3 Heh, 4 Hee_o, 5 Hhh_o; orientation 0 the matrix, 1 its
 // transpose.  An A operand A(m, k) reads slot rows m; a B operand B(k, n)
@@ -35,8 +36,18 @@ __device__ __forceinline__ void split_bf16(double x, uint16_t &hi, uint16_t &lo)
   lo = bf16_bits_rn((float)(x - (double)bf16_to_f(hi)));
 }
 
-__host__ __device__ inline long spl_plane(int N, int w, int slot, int plane) {
-  return (((long)w * 12 + slot) * 4 + plane) * (long)N * N;
+// Blocked operand layout: slot matrices padded to NR = 64 ceil(N/64) rows
+// and NK = 32 ceil(N/32) columns (k), stored as [row block][k block]
+// chunks of 4 planes x 64 rows x 32 k (16 KB, contiguous), so one K slab
+// of one 64-row tile is a single contiguous chunk.  The padding stays zero.
+__host__ __device__ inline int spl_nr(int N) { return (N + 63) / 64 * 64; }
+__host__ __device__ inline int spl_nk(int N) { return (N + 31) / 32 * 32; }
+__host__ __device__ inline long spl_slot_elems(int N) { return 4L * spl_nr(N) * spl_nk(N); }
+__host__ __device__ inline long spl_chunk(int N, int w, int slot, int rb, int kb) {
+  return ((long)w * 12 + slot) * spl_slot_elems(N) + ((long)rb * (spl_nk(N) / 32) + kb) * (4 * 64 * 32);
+}
+__device__ __forceinline__ long spl_index(int N, int w, int slot, int pl, int row, int k) {
+  return spl_chunk(N, w, slot, row >> 6, k >> 5) + pl * (64 * 32) + (row & 63) * 32 + (k & 31);
 }
 
 // ---------------------------------------------------------------------------
@@ -67,7 +78,7 @@ __global__ __launch_bounds__(256) void k_split_bf16(psa_group g, int w0) {
       split_bf16(v.re, h[0], h[1]);
       split_bf16(v.im, h[2], h[3]);
 #pragma unroll
-      for (int pl = 0; pl < 4; ++pl) spl[spl_plane(N, w, s_same, pl) + (long)gi * N + gj] = h[pl];
+      for (int pl = 0; pl < 4; ++pl) spl[spl_index(N, w, s_same, pl, gi, gj)] = h[pl];
     }
 #pragma unroll
     for (int pl = 0; pl < 4; ++pl) t[pl][i][j] = h[pl];
@@ -80,7 +91,7 @@ __global__ __launch_bounds__(256) void k_split_bf16(psa_group g, int w0) {
     const int gi = r0 + i, gj = c0 + j;
     if (gi < N && gj < N)
 #pragma unroll
-      for (int pl = 0; pl < 4; ++pl) spl[spl_plane(N, w, s_tr, pl) + (long)gj * N + gi] = t[pl][i][j];
+      for (int pl = 0; pl < 4; ++pl) spl[spl_index(N, w, s_tr, pl, gj, gi)] = t[pl][i][j];
   }
 }
 
@@ -128,29 +139,16 @@ struct BFrag {
   int row, kc;
 };
 
-// Load the slab [k0, k0+32) of rows [x0, x0+64) of a dense slot.
+// Load the slab [k0, k0+32) of rows [x0, x0+64) of a dense slot: one
+// contiguous 16 KB chunk (x0, k0 are multiples of 64 and 32).
 __device__ __forceinline__ void bload_dense(BFrag &f, const uint16_t *spl, int N, int w, int slot,
                                             int x0, int k0) {
   const int tid = threadIdx.x;
   f.row = tid >> 2;
   f.kc = (tid & 3) * 8;
-  const int gx = x0 + f.row, gk = k0 + f.kc;
+  const uint16_t *src = spl + spl_chunk(N, w, slot, x0 >> 6, k0 >> 5) + f.row * 32 + f.kc;
 #pragma unroll
-  for (int pl = 0; pl < 4; ++pl) {
-    u32x4 v = {0u, 0u, 0u, 0u};
-    if (gx < N) {
-      const uint16_t *src = spl + spl_plane(N, w, slot, pl) + (long)gx * N + gk;
-      if (gk + 8 <= N) {
-        v = *reinterpret_cast<const u32x4 *>(src);
-      } else {
-        uint16_t tmp[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) tmp[e] = (gk + e < N) ? src[e] : (uint16_t)0;
-        v = __builtin_bit_cast(u32x4, tmp);
-      }
-    }
-    f.c[pl] = v;
-  }
+  for (int pl = 0; pl < 4; ++pl) f.c[pl] = *reinterpret_cast<const u32x4 *>(src + pl * (64 * 32));
 }
 
 // The banded dephasing operand i hbar D[m - x][x] for rows x = x0.. and
@@ -199,19 +197,72 @@ __device__ __forceinline__ fv4 mma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, 
   return c;
 }
 
+// The 12 (or 6, for a band operand) split products of one K slab of one
+// term into this wave's 2x2 blocks of accumulators.
+__device__ __forceinline__ void bslab(const BTile &sA, const BTile &sB, const BTerm bt,
+                                      fv4 (&aR)[2][2], fv4 (&aI)[2][2], int wm, int wn, int kl,
+                                      int xl) {
+  // signs of the four real products (conj flips the imaginary part)
+  const double cA = bt.ca ? -1.0 : 1.0, cB = bt.cb ? -1.0 : 1.0;
+  const bool nRR = bt.sg < 0, nII = -bt.sg * cA * cB < 0, nRI = bt.sg * cB < 0, nIR = bt.sg * cA < 0;
+  const bool aRe0 = bt.sa < 0, bRe0 = bt.sb < 0;  // band operands are purely imaginary
+  bf16x8 A[2][4], B[2][4];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl) {
+      A[s][pl] = bfrag_read(sA, pl, wm * 32 + s * 16 + xl, kl * 8);
+      B[s][pl] = bfrag_read(sB, pl, wn * 32 + s * 16 + xl, kl * 8);
+    }
+  if (!aRe0 && !bRe0) {  // Re A Re B
+#pragma unroll
+    for (int si = 0; si < 2; ++si) {
+      const bf16x8 h = nRR ? bneg(A[si][0]) : A[si][0], l = nRR ? bneg(A[si][1]) : A[si][1];
+#pragma unroll
+      for (int sj = 0; sj < 2; ++sj) aR[si][sj] = mma3(h, l, B[sj][0], B[sj][1], aR[si][sj]);
+    }
+  }
+  if (!aRe0) {  // Re A Im B
+#pragma unroll
+    for (int si = 0; si < 2; ++si) {
+      const bf16x8 h = nRI ? bneg(A[si][0]) : A[si][0], l = nRI ? bneg(A[si][1]) : A[si][1];
+#pragma unroll
+      for (int sj = 0; sj < 2; ++sj) aI[si][sj] = mma3(h, l, B[sj][2], B[sj][3], aI[si][sj]);
+    }
+  }
+  {  // Im A Im B (subtracted)
+#pragma unroll
+    for (int si = 0; si < 2; ++si) {
+      const bf16x8 h = nII ? bneg(A[si][2]) : A[si][2], l = nII ? bneg(A[si][3]) : A[si][3];
+#pragma unroll
+      for (int sj = 0; sj < 2; ++sj) aR[si][sj] = mma3(h, l, B[sj][2], B[sj][3], aR[si][sj]);
+    }
+  }
+  if (!bRe0) {  // Im A Re B
+#pragma unroll
+    for (int si = 0; si < 2; ++si) {
+      const bf16x8 h = nIR ? bneg(A[si][2]) : A[si][2], l = nIR ? bneg(A[si][3]) : A[si][3];
+#pragma unroll
+      for (int sj = 0; sj < 2; ++sj) aI[si][sj] = mma3(h, l, B[sj][0], B[sj][1], aI[si][sj]);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_rhs_bf16(RhsProb p) {
   extern __shared__ char lds_raw[];
   BTile *tiles = reinterpret_cast<BTile *>(lds_raw);  // [2 buffers][A, B]
   const psa_group &g = p.g;
   const int N = g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  // XCD-aware order as in gemm_kernel: batch b on XCD b % 8 when possible
+  // XCD-aware order: the three equations of wire j run on XCD j % 8 (they
+  // share the wire's operand planes in that XCD's L2) when the wire count
+  // is a multiple of 8; consecutive workgroups go round robin over XCDs.
   const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
   unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
   int b;
-  if (gridDim.z % 8 == 0) {
-    const unsigned xcd = lin % 8, q = lin / 8;
-    b = xcd + 8 * (q / tpb);
+  if (gridDim.z % 24 == 0) {
+    const unsigned xcd = lin % 8, q = lin / 8, per = 3 * tpb;
+    b = 3 * (xcd + 8 * (q / per)) + (q % per) / tpb;
     lin = q % tpb;
   } else {
     b = lin / tpb;
@@ -236,75 +287,31 @@ __global__ __launch_bounds__(256) void k_rhs_bf16(RhsProb p) {
     if (bt.sb >= 0) bload_dense(fb, spl, N, w, bt.sb, n0, k0);
     else bload_band(fb, g, w, 0, n0, k0);
   };
-  BFrag fa, fb;
-  int t = 0, k0 = 0;
-  fetch(fa, fb, 0, 0);
-  bcommit(fa, tiles[0]);
-  bcommit(fb, tiles[1]);
-  __syncthreads();
-  int cur = 0;
+  // Flattened slabs s = t * KS + k0 / 32.  Two-slab register prefetch:
+  // while slab s is multiplied from one LDS buffer, slab s+1 sits in the
+  // other buffer or in registers and slab s+2 is in flight.
+  const int KS = (N + BK2 - 1) / BK2, S = nt * KS;
   const int kl = lane >> 4, xl = lane & 15;
-  while (true) {
-    int tn = t, kn = k0 + BK2;
-    if (kn >= N) {
-      tn = t + 1;
-      kn = 0;
-    }
-    const bool more = tn < nt;
-    if (more) fetch(fa, fb, tn, kn);
-    const BTile &sA = tiles[2 * cur], &sB = tiles[2 * cur + 1];
-    const BTerm bt = bterm(e, t);
-    // signs of the four real products (conj flips the imaginary part)
-    const double cA = bt.ca ? -1.0 : 1.0, cB = bt.cb ? -1.0 : 1.0;
-    const bool nRR = bt.sg < 0, nII = -bt.sg * cA * cB < 0, nRI = bt.sg * cB < 0, nIR = bt.sg * cA < 0;
-    const bool aRe0 = bt.sa < 0, bRe0 = bt.sb < 0;  // band operands are purely imaginary
-    bf16x8 A[2][4], B[2][4];
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-      for (int pl = 0; pl < 4; ++pl) {
-        A[s][pl] = bfrag_read(sA, pl, wm * 32 + s * 16 + xl, kl * 8);
-        B[s][pl] = bfrag_read(sB, pl, wn * 32 + s * 16 + xl, kl * 8);
-      }
-    if (!aRe0 && !bRe0) {  // Re A Re B
-#pragma unroll
-      for (int si = 0; si < 2; ++si) {
-        const bf16x8 h = nRR ? bneg(A[si][0]) : A[si][0], l = nRR ? bneg(A[si][1]) : A[si][1];
-#pragma unroll
-        for (int sj = 0; sj < 2; ++sj) aR[si][sj] = mma3(h, l, B[sj][0], B[sj][1], aR[si][sj]);
-      }
-    }
-    if (!aRe0) {  // Re A Im B
-#pragma unroll
-      for (int si = 0; si < 2; ++si) {
-        const bf16x8 h = nRI ? bneg(A[si][0]) : A[si][0], l = nRI ? bneg(A[si][1]) : A[si][1];
-#pragma unroll
-        for (int sj = 0; sj < 2; ++sj) aI[si][sj] = mma3(h, l, B[sj][2], B[sj][3], aI[si][sj]);
-      }
-    }
-    {  // Im A Im B (subtracted)
-#pragma unroll
-      for (int si = 0; si < 2; ++si) {
-        const bf16x8 h = nII ? bneg(A[si][2]) : A[si][2], l = nII ? bneg(A[si][3]) : A[si][3];
-#pragma unroll
-        for (int sj = 0; sj < 2; ++sj) aR[si][sj] = mma3(h, l, B[sj][2], B[sj][3], aR[si][sj]);
-      }
-    }
-    if (!bRe0) {  // Im A Re B
-#pragma unroll
-      for (int si = 0; si < 2; ++si) {
-        const bf16x8 h = nIR ? bneg(A[si][2]) : A[si][2], l = nIR ? bneg(A[si][3]) : A[si][3];
-#pragma unroll
-        for (int sj = 0; sj < 2; ++sj) aI[si][sj] = mma3(h, l, B[sj][0], B[sj][1], aI[si][sj]);
-      }
-    }
-    if (!more) break;
-    bcommit(fa, tiles[2 * (cur ^ 1)]);
-    bcommit(fb, tiles[2 * (cur ^ 1) + 1]);
+  BFrag f0a, f0b, f1a, f1b;
+  fetch(f0a, f0b, 0, 0);
+  if (S > 1) fetch(f1a, f1b, 1 / KS, (1 % KS) * BK2);
+  bcommit(f0a, tiles[0]);
+  bcommit(f0b, tiles[1]);
+  if (S > 2) fetch(f0a, f0b, 2 / KS, (2 % KS) * BK2);
+  __syncthreads();
+  for (int s = 0; s < S; s += 2) {
+    bslab(tiles[0], tiles[1], bterm(e, s / KS), aR, aI, wm, wn, kl, xl);
+    if (s + 1 >= S) break;
+    bcommit(f1a, tiles[2]);
+    bcommit(f1b, tiles[3]);
     __syncthreads();
-    cur ^= 1;
-    t = tn;
-    k0 = kn;
+    if (s + 3 < S) fetch(f1a, f1b, (s + 3) / KS, ((s + 3) % KS) * BK2);
+    bslab(tiles[2], tiles[3], bterm(e, (s + 1) / KS), aR, aI, wm, wn, kl, xl);
+    if (s + 2 >= S) break;
+    bcommit(f0a, tiles[0]);
+    bcommit(f0b, tiles[1]);
+    __syncthreads();
+    if (s + 4 < S) fetch(f0a, f0b, (s + 4) / KS, ((s + 4) % KS) * BK2);
   }
 #pragma unroll
   for (int si = 0; si < 2; ++si)

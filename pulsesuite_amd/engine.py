@@ -91,7 +91,8 @@ class WireGroup:
         self.s["hdiag"] = z(nwz, 2, N)
         self.s["prep"] = z(nwz, _prep_stride(N), dt=F64)
         self.s["dedh"] = z(nwz, 2, N, 2 * N - 1, dt=F64)
-        self.s["wmb"] = z(9 * nwz * N, dt=F64)  # [Nw][4][N] rates, occ [2][N][Nw], V^2 [3][N][Nw]
+        # [Nw][4][N] rates; wire fastest: occ [2][N][Nw], V^2 [3][N][Nw], |n| [2][N][Nw]
+        self.s["wmb"] = z(11 * nwz * N, dt=F64)
         self.s["exq"] = z(nwz, 3, 2 * N - 1)
         self.s["efield"] = z(3, nwz, Nx)
         self.s["src"] = z(nwz, _src_stride(N, Nr), dt=F64)
@@ -99,8 +100,9 @@ class WireGroup:
         self.s["obs"] = z(nwz, 8, dt=F64)
         self.s["tphase"] = z(nwz, N)
         self.s["part"] = z(nwz, 256, dt=F64)
-        # split-bf16 operand planes (bf16x3 only): [Nw][12][4][N][N] bf16
-        nspl = nwz * 12 * 4 * N * N if prec == "bf16x3" else 1
+        # split-bf16 operand planes (bf16x3 only): [Nw][12 slots] of 4 planes
+        # on a zero-padded (64 ceil(N/64)) x (32 ceil(N/32)) grid, blocked
+        nspl = nwz * 12 * 4 * (-(-N // 64) * 64) * (-(-N // 32) * 32) if prec == "bf16x3" else 1
         self.s["spl"] = torch.zeros(nspl, dtype=torch.int16, device=d)
         if wire_ids is None:
             wire_ids = np.arange(1, nwz + 1)

@@ -385,7 +385,7 @@ def build_tables(prm, rr, vint_fn):
     t["EP"] = (NO / ((dEe - hO) ** 2 + hG**2) + (NO + 1.0) / ((dEe + hO) ** 2 + hG**2)) * 2.0 * prm.Gph * idel
     t["HP"] = (NO / ((dEh - hO) ** 2 + hG**2) + (NO + 1.0) / ((dEh + hO) ** 2 + hG**2)) * 2.0 * prm.Gph * idel
     t["Vscale"] = hbar * prm.Oph * prm.epsr * (1.0 / 8.2 - 1.0 / 10.0)
-    # dephasing.py:472-539, stored transposed [q][k]
+    # dephasing.py:472-539, stored [k][q] (a wave reads one k, lanes along q)
     dk = kr[1] - kr[0]
     kmaxd, kmind = kr[N - 1] + dk, kr[0] - dk
     nk0 = (N - 1) // 2 + 1
@@ -398,7 +398,7 @@ def build_tables(prm, rr, vint_fn):
     k1p0 = ((prm.me + prm.mh) * Qv - 2 * prm.mh * K) / 2.0 / prm.me
     k10 = ((prm.me + prm.mh) * Qv - 2 * prm.me * K) / 2.0 / prm.mh
     tabs = [idx(K + Qv), idx(K - Qv), idx(k1p0 - Qv), idx(k1p0), idx(k10 - Qv), idx(k10)]
-    t["didx"] = np.ascontiguousarray(np.stack([a.T for a in tabs]).astype(np.int32))
+    t["didx"] = np.ascontiguousarray(np.stack(tabs).astype(np.int32))
     t["xe"] = prm.me / hbar**2 * np.abs(kr) / (np.abs(kr) + dk * 1e-5) ** 2 / dk
     t["xh"] = prm.mh / hbar**2 * np.abs(kr) / (np.abs(kr) + dk * 1e-5) ** 2 / dk
     t["iq"] = np.round(np.abs(kr / dk)).astype(np.int32)
