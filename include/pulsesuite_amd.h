@@ -107,7 +107,7 @@ typedef struct psa_group {
                                          then, wire fastest: [2][N][Nw] |Re ne3|, |Re nh3| of the new level,
                                          [3][N][Nw] VC(d)^2 (eh, ee, hh), [2][N][Nw] |ne|, |nh| of the checkout level */
   double *exq;                        /* [Nw][3][2N-1] complex: FFTG'd Ex, Ey, Ez at q = -(N-1)..N-1 */
-  const double *efield;               /* [3][Nw][Nx] complex fields on rr (Exx, Eyy, Ezz) */
+  double *efield;                     /* [3][Nw][Nx] complex fields on rr (Exx, Eyy, Ezz); set by the caller or psa_drive */
   double *src;                        /* [Nw][PSA_SRC_STRIDE] real-space Px, Py, Pz, re, rh + diagonal sums */
   double *out;                        /* [4][Nw][Nx] complex: Pxx, Pyy, Pzz, Rho */
   double *obs;                        /* [Nw][PSA_OBS_N] observables (I0, totals, ...) */
@@ -208,6 +208,19 @@ int psa_prop2qw(const psa_group *g, int32_t w0, int32_t nw, void *stream);
 
 /* QW2Prop splines (qwoptics.py:792-805) and Rho = RhoH - RhoE (SBEs.py:5917). */
 int psa_qw2prop(const psa_group *g, int32_t w0, int32_t nw, void *stream);
+
+/* Driver side of a served simulation, on the device so a whole step can be
+ * graph-captured.  psa_drive: the sbetestprop pulse (sbetestprop.py:183-198),
+ * spatially uniform, E_f = E0_f exp(-(w0 (t-tp))^2 / (w0 tw)^2) cos(w0 (t-tp))
+ * exp(-((w0 (t-tp)) / (2 tw w0))^20), written to efield for wires
+ * [w0, w0+nw) at each wire's own time; drive[w][8] = E0x, E0y, E0z, omega0,
+ * tw, tp, t, unused; t advances by dt.  psa_record: the per-step record of
+ * sbetestprop.py:210-224 (Pxx at the middle point) plus the carrier totals:
+ * trace[w][count[w]][4] = Re Pxx[xmid], Im Pxx[xmid], sum ne2, sum nh2, then
+ * count[w]++ (nothing once count[w] == maxn). */
+int psa_drive(const psa_group *g, int32_t w0, int32_t nw, double *drive, double dt, void *stream);
+int psa_record(const psa_group *g, int32_t w0, int32_t nw, int32_t xmid, double *trace,
+               int32_t *count, int32_t maxn, void *stream);
 
 /* One full SBESolver.QWCalculator step (SBEs.py:5826-5985 minus file I/O)
  * for wires [w0, w0+nw): prop2qw, prep, calc_h, offdiag, rhs (leapfrog),

@@ -59,6 +59,8 @@ SIGNATURES = {
     "psa_prop2qw": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_qw2prop": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_step": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
+    "psa_drive": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr, c_dbl, c_ptr]),
+    "psa_record": (ctypes.c_int, [_GP, c_i32, c_i32, c_i32, c_ptr, c_ptr, c_i32, c_ptr]),
 }
 
 _lib = None

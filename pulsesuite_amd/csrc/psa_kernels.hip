@@ -953,6 +953,46 @@ __global__ __launch_bounds__(256) void k_lorentz_table(psa_group g) {
 }
 
 // ===========================================================================
+// Driver side of a served simulation (sbetestprop.py:183-198, 210-224):
+// the per-wire pulse, spatially uniform, evaluated on the device at the
+// wire's own clock, and a per-step trace of the recorded observables.
+// drive[w][8] = E0x, E0y, E0z, omega0, tw, tp, t, (unused); t += dt per call.
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_drive(psa_group g, int w0, double *drive, double dt) {
+  const int w = w0 + blockIdx.y, Nx = g.Nx;
+  const double *d = drive + (long)w * 8;
+  const double t = d[6], w0f = d[3], tw = d[4], tp = d[5];
+  const double x = w0f * (t - tp), a = w0f * tw, b = 2.0 * tw * w0f;
+  const double xa = x / b, x20 = pow(xa, 20.0);
+  const double env = exp(-(x * x) / (a * a)) * cos(x) * exp(-x20);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < Nx; i += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+      st(g.efield + (((long)f * g.Nw + w) * Nx) * 2, i, cd{d[f] * env, 0.0});
+  }
+}
+
+__global__ void k_drive_clock(int w0, int nw, double *drive, double dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nw) drive[(long)(w0 + i) * 8 + 6] += dt;
+}
+
+// trace[w][n][4] = Re Pxx[xmid], Im Pxx[xmid], sum ne2, sum nh2 at step n = count[w]
+__global__ void k_record(psa_group g, int w0, int nw, int xmid, double *trace, int *count, int maxn) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nw) return;
+  const int w = w0 + i, n = count[w];
+  if (n >= maxn) return;
+  const cd px = ld(g.out + (((long)0 * g.Nw + w) * g.Nx) * 2, xmid);
+  double *tr = trace + ((long)w * maxn + n) * 4;
+  tr[0] = px.re;
+  tr[1] = px.im;
+  tr[2] = g.obs[(long)w * PSA_OBS_N + 1];
+  tr[3] = g.obs[(long)w * PSA_OBS_N + 2];
+  count[w] = n + 1;
+}
+
+// ===========================================================================
 // host launchers
 // ===========================================================================
 namespace {
@@ -1219,6 +1259,29 @@ int psa_qw2prop(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   }
   hipLaunchKernelGGL(k_qw2prop_b, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_qw2prop_b");
+  return PSA_OK;
+}
+
+int psa_drive(const psa_group *g, int32_t w0, int32_t nw, double *drive, double dt, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  if (!drive) return fail(PSA_E_INVALID, "psa_drive: null drive table");
+  const hipStream_t s = (hipStream_t)stream;
+  const int bx = (g->Nx + 255) / 256 < 16 ? (g->Nx + 255) / 256 : 16;
+  hipLaunchKernelGGL(k_drive, dim3(bx, nw), dim3(256), 0, s, *g, w0, drive, dt);
+  PSA_LAUNCH_CHECK("k_drive");
+  hipLaunchKernelGGL(k_drive_clock, dim3((nw + 63) / 64), dim3(64), 0, s, w0, nw, drive, dt);
+  PSA_LAUNCH_CHECK("k_drive_clock");
+  return PSA_OK;
+}
+
+int psa_record(const psa_group *g, int32_t w0, int32_t nw, int32_t xmid, double *trace,
+               int32_t *count, int32_t maxn, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  if (!trace || !count || maxn < 1 || xmid < 0 || xmid >= g->Nx)
+    return fail(PSA_E_INVALID, "psa_record: bad arguments");
+  hipLaunchKernelGGL(k_record, dim3((nw + 63) / 64), dim3(64), 0, (hipStream_t)stream, *g, w0, nw, xmid,
+                     trace, count, maxn);
+  PSA_LAUNCH_CHECK("k_record");
   return PSA_OK;
 }
 

@@ -174,6 +174,31 @@ class WireGroup:
         self.s["CX"].zero_()
         self.s["DX"].zero_()
 
+    def reset_wire(self, w):
+        """Initial state (init_state) for wire w only, levels 2 and 3."""
+        N = self.N
+        f = tables.fermi(self.grid["Ee"] + self.prm.gap / 2.0)
+        diag = torch.as_tensor(np.diag(f).astype(np.complex128), device=self.dev)
+        for k in ("pA", "pB"):
+            self.s[k][w].zero_()
+        for k in ("CA", "CB", "DA", "DB"):
+            self.s[k][w].copy_(diag)
+
+    def swap_levels(self):
+        """Level 3 <- the new level for every wire of the group (pointer
+        swap of the B and X buffers); wires that were not stepped exchange
+        stale buffers, which callers reset before use."""
+        for a, b in (("pB", "pX"), ("CB", "CX"), ("DB", "DX")):
+            self.s[a], self.s[b] = self.s[b], self.s[a]
+        self._sync_struct()
+
+    def footprint_bytes(self):
+        """Device bytes held by this group (state, scratch and tables)."""
+        tot = sum(t.numel() * t.element_size() for t in self.s.values())
+        tot += sum(t.numel() * t.element_size() for t in self.t.values())
+        tot += sum(sum(x.numel() * x.element_size() for x in m) for m in self.mb)
+        return int(tot)
+
     def set_state(self, C2, D2, p2, C3, D3, p3):
         """Load levels 2 and 3 from arrays in the reference layout [N, N, Nw]."""
         for k, a in (("CA", C2), ("DA", D2), ("pA", p2), ("CB", C3), ("DB", D3), ("pB", p3)):
@@ -263,9 +288,7 @@ class WireGroup:
         group, device copy for a sub-range)."""
         w0, nw = self._range(w0, nw)
         if w0 == 0 and nw == self.nw:
-            for a, b in (("pB", "pX"), ("CB", "CX"), ("DB", "DX")):
-                self.s[a], self.s[b] = self.s[b], self.s[a]
-            self._sync_struct()
+            self.swap_levels()
         else:
             for a, b in (("pB", "pX"), ("CB", "CX"), ("DB", "DX")):
                 self.s[a][w0:w0 + nw].copy_(self.s[b][w0:w0 + nw])

@@ -1,0 +1,293 @@
+"""Continuous-batching server for many independent SBE simulations on one GPU.
+
+A simulation request is one quantum wire driven by the sbetestprop pulse
+(``sbetestprop.py:183-198``: Ey = 2 Ex as there, Ez = 0, spatially
+uniform) for a given number of active steps, recording Pxx at the middle
+point and the carrier totals every step (``sbetestprop.py:210-224``).  Every
+simulation is a single-wire solver of its own, so all run with the
+reference wire index w = 1.
+
+Design (one process, one GPU):
+
+* One ``SlotPool`` per k-grid size Nk: a ``WireGroup`` with a fixed number
+  of wire slots, allocated once.  Its capacity comes from the HBM budget
+  (``footprint`` per slot + the pool's shared tables).  The density
+  matrices of the requests in flight stay resident in their slots.
+* Continuous batching: requests are admitted into free slots between ticks
+  (lowest free slot first, so the occupied slots form a short prefix);
+  a tick advances every occupied slot of every pool by one step.  The
+  RHS and all other stages of all in-flight requests of one Nk run in the
+  same kernel launches; pools of different Nk run on their own HIP streams,
+  so their kernels overlap on the device.
+* A tick of a pool is drive -> QWCalculator step -> record on the device
+  (``psa_drive``, ``psa_step``, ``psa_record``) for the occupied prefix,
+  optionally replayed from a captured HIP graph (one per pointer parity of
+  the level swap and prefix length).
+* Requests whose step count is reached are harvested (trace and final
+  populations copied out), their slot is freed and reused.
+
+Before its first active step a request waits for the reference's wire-off
+gate (``SBEs.py:5849-5858``): steps with |E| < 1e-3 Emax0 before the field
+first exceeds that do nothing, so the clock starts at the first open step.
+"""
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+from scipy.constants import e as e0
+from scipy.constants import hbar
+
+from . import capi, tables
+from .engine import WireGroup
+
+
+@dataclass
+class SimRequest:
+    """One simulation: wire k-grid size, pulse and step count."""
+
+    rid: int
+    nk: int
+    e0: float                 # peak Ex (V/m); Ey = 2 e0, Ez = 0
+    fwhm: float               # field FWHM (s); tw = fwhm / (2 sqrt(ln 2))
+    detuning_ev: float        # hbar omega0 = gap + detuning
+    nsteps: int               # active steps
+    arrival: float = 0.0      # seconds after the server clock starts
+    # filled by the server
+    t_admit: float = field(default=None, repr=False)
+    t_done: float = field(default=None, repr=False)
+    trace: np.ndarray = field(default=None, repr=False)
+    final: dict = field(default=None, repr=False)
+
+    def pulse(self, gap_ev):
+        """(E0x, E0y, E0z, omega0, tw, tp)."""
+        tw = self.fwhm / (2.0 * math.sqrt(math.log(2.0)))
+        omega0 = (gap_ev + self.detuning_ev) * e0 / hbar
+        return self.e0, 2.0 * self.e0, 0.0, omega0, tw, 3.0 * self.fwhm
+
+    def gate_open_time(self, gap_ev, dt):
+        """First step time at which |E| >= 1e-3 Emax0 (the wire-off gate)."""
+        E0x, E0y, E0z, w0, tw, tp = self.pulse(gap_ev)
+        emax = math.sqrt(E0x**2 + E0y**2 + E0z**2)
+        n = np.arange(int(math.ceil(2 * tp / dt)) + 2)
+        t = n * dt
+        x = w0 * (t - tp)
+        env = np.exp(-(x**2) / (w0 * tw) ** 2) * np.cos(x) * np.exp(-((x / (2 * tw * w0)) ** 20))
+        mag = np.abs(env) * emax
+        open_ = np.nonzero(mag >= 1e-3 * emax)[0]
+        return float(t[open_[0]]) if open_.size else 0.0
+
+
+def wire_length(nk, me_rel=0.067, gap_ev=1.5):
+    """Wire length giving exactly nk k-points (SBEs.py:4754-4758)."""
+    from scipy.constants import m_e
+    kmax = math.sqrt(1.2 * gap_ev * e0 * 2.0 * me_rel * m_e / hbar**2)
+    return (nk / 2 + 0.5) * math.pi / kmax
+
+
+class SlotPool:
+    """Fixed-capacity slots of one Nk on one stream."""
+
+    def __init__(self, nk, capacity, rr, dt, prec, maxsteps, graphs=True, prm_kw=None):
+        prm_kw = dict(prm_kw or {})
+        prm_kw.setdefault("me_rel", 0.067)
+        self.prm = tables.QWParams(L=wire_length(nk, prm_kw["me_rel"]), **prm_kw)
+        self.group = WireGroup(self.prm, capacity, rr, wire_ids=np.ones(capacity, np.int32), prec=prec)
+        if self.group.N != nk:
+            raise ValueError(f"k-grid gave N={self.group.N}, wanted {nk}")
+        self.nk, self.cap, self.dt = nk, int(capacity), float(dt)
+        self.maxsteps = int(maxsteps)
+        d = self.group.dev
+        self.drive = torch.zeros(self.cap, 8, dtype=torch.float64, device=d)
+        self.trace = torch.zeros(self.cap, self.maxsteps, 4, dtype=torch.float64, device=d)
+        self.count = torch.zeros(self.cap, dtype=torch.int32, device=d)
+        self.slots = [None] * self.cap
+        self.done_steps = [0] * self.cap
+        self.stream = torch.cuda.Stream(device=d)
+        self.use_graphs = graphs
+        self.graphs = {}
+        self.parity = 0
+        self.xmid = len(rr) // 2
+        self.lib = self.group.lib
+        self.ticks = 0
+
+    # -- slots -----------------------------------------------------------
+    def free(self):
+        return sum(1 for s in self.slots if s is None)
+
+    def active(self):
+        return sum(1 for s in self.slots if s is not None)
+
+    def prefix(self):
+        hi = 0
+        for i, s in enumerate(self.slots):
+            if s is not None:
+                hi = i + 1
+        return hi
+
+    def admit(self, req, now):
+        if req.nsteps > self.maxsteps:
+            raise ValueError("request longer than the pool's trace buffer")
+        i = self.slots.index(None)
+        with torch.cuda.stream(self.stream):
+            self.group.reset_wire(i)
+            t0 = req.gate_open_time(self.prm.gap, self.dt)
+            row = list(req.pulse(self.prm.gap)) + [t0, 0.0]
+            self.drive[i].copy_(torch.tensor(row, dtype=torch.float64))
+            self.count[i] = 0
+        self.slots[i] = req
+        self.done_steps[i] = 0
+        req.t_admit = now
+        return i
+
+    # -- one tick ----------------------------------------------------------
+    def _launch(self, hi, S, stream_handle):
+        c = capi.check
+        c(self.lib.psa_drive(S, 0, hi, self.drive.data_ptr(), self.dt, stream_handle))
+        c(self.lib.psa_step(S, 0, hi, self.dt, stream_handle))
+        c(self.lib.psa_record(S, 0, hi, self.xmid, self.trace.data_ptr(), self.count.data_ptr(),
+                              self.maxsteps, stream_handle))
+
+    def tick(self):
+        """Advance every occupied slot by one step (asynchronous)."""
+        hi = self.prefix()
+        if hi == 0:
+            return 0
+        hi = min(self.cap, -(-hi // 8) * 8)  # prefix rounded up to 8 (fewer graph shapes)
+        import ctypes
+        S = ctypes.byref(self.group.S)
+        with torch.cuda.stream(self.stream):
+            key = (self.parity, hi)
+            if self.use_graphs and self.ticks >= 2:
+                g = self.graphs.get(key)
+                if g is None:
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g, stream=self.stream):
+                        self._launch(hi, S, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+                    self.graphs[key] = g
+                g.replay()
+            else:
+                self._launch(hi, S, ctypes.c_void_p(self.stream.cuda_stream))
+        self.group.swap_levels()
+        self.parity ^= 1
+        self.ticks += 1
+        for i, s in enumerate(self.slots):
+            if s is not None:
+                self.done_steps[i] += 1
+        return hi
+
+    def harvest(self, now_fn):
+        """Collect finished requests (waits for this pool's stream)."""
+        fin = [i for i, s in enumerate(self.slots) if s is not None and self.done_steps[i] >= s.nsteps]
+        if not fin:
+            return []
+        self.stream.synchronize()
+        out = []
+        g = self.group
+        for i in fin:
+            req = self.slots[i]
+            req.trace = self.trace[i, :req.nsteps].cpu().numpy()
+            # final level-3 populations and polarization matrix of this slot
+            # (after the swap, level 3 is in the B buffers)
+            C = g.s["CB"][i].cpu().numpy()
+            D = g.s["DB"][i].cpu().numpy()
+            p = g.s["pB"][i].cpu().numpy()
+            req.final = dict(ne=np.real(np.diag(C)).copy(), nh=np.real(np.diag(D)).copy(), p=p)
+            req.t_done = now_fn()
+            self.slots[i] = None
+            out.append(req)
+        return out
+
+
+def slot_bytes(nk, nx, prec, maxsteps):
+    """Per-slot device bytes of a pool (state, scratch, split planes, trace)."""
+    N = nk
+    b = 12 * N * N * 16                      # pA..Hho
+    b += 2 * N * 16 + (8 * N + 4 * (2 * N - 1) + 5 * N) * 8
+    b += 2 * N * (2 * N - 1) * 8 + 11 * N * 8 + 3 * (2 * N - 1) * 16
+    b += 3 * nx * 16 + 4 * nx * 16 + (9 * (2 * N + 2) + 8 * (2 * N - 1) + 8) * 8
+    if prec == "bf16x3":
+        b += 12 * 4 * (-(-N // 64) * 64) * (-(-N // 32) * 32) * 2
+    b += maxsteps * 4 * 8 + 8 * 8
+    return b
+
+
+def pool_bytes(nk):
+    """Shared per-pool tables (dominated by the Lorentzian table)."""
+    N = nk
+    return 2 * N * N * 2 * N * 8 + 64 * N * N * N // 8
+
+
+class Server:
+    """Admission by HBM footprint, continuous batching over Nk pools."""
+
+    def __init__(self, rr, dt=10e-18, prec="bf16x3", hbm_budget=64 << 30, max_slots=64,
+                 maxsteps=20000, graphs=True, prm_kw=None):
+        self.rr = np.asarray(rr, dtype=float)
+        self.dt, self.prec = float(dt), prec
+        self.budget, self.max_slots = int(hbm_budget), int(max_slots)
+        self.maxsteps, self.graphs = int(maxsteps), graphs
+        self.prm_kw = prm_kw
+        self.pools = {}
+        self.used = 0
+
+    def pool(self, nk):
+        p = self.pools.get(nk)
+        if p is None:
+            per = slot_bytes(nk, len(self.rr), self.prec, self.maxsteps)
+            room = self.budget - self.used - pool_bytes(nk)
+            cap = min(self.max_slots, room // per)
+            if cap < 1:
+                return None
+            p = SlotPool(nk, cap, self.rr, self.dt, self.prec, self.maxsteps, self.graphs, self.prm_kw)
+            self.used += p.group.footprint_bytes() + p.trace.numel() * 8
+            self.pools[nk] = p
+        return p
+
+    def run(self, requests, progress=None, time_limit=None):
+        """Serve `requests` (sorted by arrival, seconds from start).  Returns
+        the finished requests; latencies are t_done - arrival."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        for nk in sorted({r.nk for r in pending}):
+            self.pool(nk)  # build the tables up front, not on the clock
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        waiting, done = [], []
+        qi = 0
+        steps = 0
+        while qi < len(pending) or waiting or any(p.active() for p in self.pools.values()):
+            t = now()
+            while qi < len(pending) and pending[qi].arrival <= t:
+                waiting.append(pending[qi])
+                qi += 1
+            # admit in arrival order where a slot of the right Nk is free
+            rest = []
+            for r in waiting:
+                p = self.pools.get(r.nk)
+                if p is not None and p.free() > 0:
+                    p.admit(r, t)
+                else:
+                    rest.append(r)
+            waiting = rest
+            busy = False
+            for p in self.pools.values():
+                if p.active():
+                    steps += p.active()
+                    p.tick()
+                    busy = True
+            for p in self.pools.values():
+                done.extend(p.harvest(now))
+            if not busy and qi < len(pending):
+                # idle until the next arrival
+                time.sleep(max(0.0, min(pending[qi].arrival - now(), 0.01)))
+            if progress and len(done) and len(done) % progress == 0:
+                print(f"[serve] done {len(done)}/{len(pending)} t={now():.1f}s", flush=True)
+            if time_limit is not None and now() > time_limit:
+                break
+        torch.cuda.synchronize()
+        self.wall = now()
+        self.steps = steps
+        return done

@@ -304,3 +304,4 @@ def test_mbce_brute_force_loops_tiny_n(golden_dir):
                                     * ne[v] * S_ee[k4 - 1, k2 - 1, k - 1])
     a, b = O.mbce(g, S_eh, S_ee, ne0, nh0, VC)
     assert rel(a, Win) < 1e-14 and rel(b, Wout) < 1e-14
+
