@@ -176,6 +176,42 @@ def cpu_baseline(wg, prm, rr, dt, budget_s=20.0):
                        f"NumPy oracle ({per_wire_step:.2f} s/wire-step), scaled to 64 wires")
 
 
+def serve_sample(prec, seed, nreq=200, rate=40.0, nks=(64, 128, 256), steps=(200, 1000),
+                 dt=10e-18, nx=1024, time_limit=90.0):
+    """A bounded sample of the serving stream: Poisson arrivals at `rate`,
+    Nk uniform in `nks`, E0 log-uniform in [1e5, 1e8] V/m, FWHM uniform in
+    [50, 500] fs, detuning uniform in +-50 meV, active steps uniform in
+    `steps`; continuous batching over per-Nk slot pools."""
+    import torch
+    from pulsesuite_amd.serving import Server, SimRequest
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    reqs = [SimRequest(rid=i, nk=int(rng.choice(nks)), e0=float(10 ** rng.uniform(5, 8)),
+                       fwhm=float(rng.uniform(50e-15, 500e-15)),
+                       detuning_ev=float(rng.uniform(-0.05, 0.05)),
+                       nsteps=int(rng.integers(steps[0], steps[1] + 1)), arrival=float(t[i]))
+            for i in range(nreq)]
+    rr = (np.arange(nx) - nx // 2) * 1e-9
+    srv = Server(rr, dt=dt, prec=prec, hbm_budget=64 << 30, max_slots=64,
+                 maxsteps=steps[1], graphs=True)
+    print(f"[bench] serving {nreq} requests at {rate}/s ({prec})", file=sys.stderr, flush=True)
+    done = srv.run(reqs, time_limit=time_limit)
+    lat = np.array([r.t_done - r.arrival for r in done])
+    wait = np.array([r.t_admit - r.arrival for r in done])
+    nsteps = sum(r.nsteps for r in done)
+    out = dict(requests=nreq, completed=len(done), wall_s=srv.wall,
+               sims_per_s=len(done) / srv.wall, steps_per_s=nsteps / srv.wall,
+               latency_s={"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+               queue_wait_s={"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99))},
+               pools={int(k): p.cap for k, p in srv.pools.items()}, prec=prec,
+               workload=f"Poisson {rate}/s, Nk in {list(nks)}, E0 1e5-1e8 V/m, FWHM 50-500 fs, "
+                        f"detuning +-50 meV, {steps[0]}-{steps[1]} active steps, dt = 10 as, "
+                        f"Nx = {nx}; sample of the 1000-request stream")
+    del srv
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +222,10 @@ def main():
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--modes", default="", help="reduced-precision modes to time, e.g. bf16x3 "
+                    "(FP64 is always timed; default fp32,bf16x3)")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only (the SBE wire batch is stepped on a single "
@@ -195,6 +235,8 @@ def main():
     dt = 10e-18
     seed = 1234
     modes = ["fp64"] + ([] if args.no_mixed else ["fp32", "bf16x3"])
+    if args.modes:
+        modes = ["fp64"] + [m for m in args.modes.split(",") if m in ("fp32", "bf16x3")]
     res, runs = {}, {}
     for mode in modes:
         try:
@@ -238,6 +280,12 @@ def main():
     peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS}[best]
     res["reported_mode"] = best
     wg, prm, rr = build_group("fp64", args.wires, args.nk, args.nx, seed) if not args.no_cpu else (None, prm, rr)
+    serving = None
+    if not args.no_serve:
+        try:
+            serving = serve_sample(best, seed, nreq=args.serve_requests)
+        except Exception as ex:  # report, keep the batch number
+            serving = {"error": repr(ex)}
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -267,6 +315,7 @@ def main():
                      "avg_launch_ms": t_rhs * 1e3,
                      "flops_per_launch": flops_rhs},
         "cpu_baseline": cpu,
+        "serving": serving,
         "extra": res,
     }
     print(json.dumps(line), flush=True)
