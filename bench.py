@@ -27,6 +27,10 @@ METRIC = "SBE steps/s (Nk=256, 64 wires) mixed vs FP64 @<=1e-6 err; p50/p99 sim 
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0  # dense
+# HBM bytes per RHS launch at 64 wires x Nk = 256 from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs, scaled by a 1 GiB copy's counts (x2.0, x1.0); the runs and
+# the calibration are in profiles/sbe_step_64w_nk256.md.
+RHS_TRAFFIC_BYTES = {"fp64": 2.626e9, "bf16x3": 2.339e9}
 
 
 def wire_length(nk, me_rel=0.067, gap_ev=1.5):
@@ -311,7 +315,9 @@ def main():
         "roofline": {"bound": "mfma",
                      "kernel": f"rhs (dp/dC/dD + leapfrog), {best} contractions",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": (RHS_TRAFFIC_BYTES.get(best)
+                                 if (args.wires, args.nk) == (64, 256) else None),
                      "avg_launch_ms": t_rhs * 1e3,
                      "flops_per_launch": flops_rhs},
         "cpu_baseline": cpu,
