@@ -69,7 +69,7 @@ def make_fields(rng, rr, nw, L):
     return Exx, 2.0 * Exx, np.zeros_like(Exx)
 
 
-def build_group(prec, nw, N, nx, seed):
+def build_group(prec, nw, N, nx, seed, tune=False):
     from pulsesuite_amd.engine import WireGroup
     from pulsesuite_amd.tables import QWParams
     L = wire_length(N)
@@ -80,6 +80,8 @@ def build_group(prec, nw, N, nx, seed):
     wg.set_state(*seeded_state(rng, wg.N, nw))
     Exx, Eyy, Ezz = make_fields(rng, rr, nw, L)
     wg.set_fields(Exx, Eyy, Ezz)
+    if tune:
+        wg.tuned = wg.autotune()
     return wg, prm, rr
 
 
@@ -246,8 +248,9 @@ def main():
         try:
             print(f"[bench] building {args.wires} wires Nk={args.nk} Nx={args.nx} {mode}",
                   file=sys.stderr, flush=True)
-            wg, prm, rr = build_group(mode, args.wires, args.nk, args.nx, seed)
+            wg, prm, rr = build_group(mode, args.wires, args.nk, args.nx, seed, tune=True)
             N = wg.N
+            res[f"{mode}_tune"] = {str(k): v[0] for k, v in wg.tuned.items()}
             print(f"[bench] timing {mode}", file=sys.stderr, flush=True)
             wall, lat = time_steps(wg, args.steps, args.warmup, dt)
             t_rhs = time_rhs_kernel(wg, 20)

@@ -65,6 +65,9 @@ typedef struct psa_group {
   int32_t gx0, gnx; /* support of the Prop2QW transform in rr: [gx0, gx0+gnx) */
   int32_t bx0, bnx; /* rows of rr inside the wire for QW2Prop: [bx0, bx0+bnx) */
   int32_t nnz_mb[8];/* entries in each many-body scattering list */
+  int32_t tune[4];  /* launch choices, 0 = default: [0] split-BF16 RHS tile (0: 64, 1: 128),
+                       [1] transport lines per block (0: 4, else 2, 4 or 8),
+                       [2] GEMM block order (0: batches grouped per XCD, 1: linear), [3] reserved */
   int32_t pad0;
   /* ---- scalars (SI units) ---- */
   double hbar, e0, gap, L, ehint, dcv, area, dkr, dr_r, drr;
@@ -115,7 +118,7 @@ typedef struct psa_group {
   double *part;                       /* [Nw][256] scratch (reserved) */
   void *spl;                          /* bf16 split operands, PSA_PREC_BF16X3 only, zero-initialised:
                                          [Nw][12 slots][NR/64][NK/32][4 planes][64][32] with
-                                         NR = 64 ceil(N/64), NK = 32 ceil(N/32); slot = 2 matrix +
+                                         NR = 128 ceil(N/128), NK = 32 ceil(N/32); slot = 2 matrix +
                                          transposed (p, C_o, D_o, Heh, Hee_o, Hhh_o), planes
                                          re_hi, re_lo, im_hi, im_lo */
   const int32_t *wid_host;            /* [Nw] host copy of wire_id */

@@ -31,7 +31,7 @@ class PsaGroup(ctypes.Structure):
     """Mirror of ``psa_group``; the size is checked against the library."""
 
     _fields_ = ([(n, c_i32) for n in _INTS]
-                + [("nnz_mb", c_i32 * 8), ("pad0", c_i32)]
+                + [("nnz_mb", c_i32 * 8), ("tune", c_i32 * 4), ("pad0", c_i32)]
                 + [(n, c_dbl) for n in _DBLS]
                 + [(n, c_ptr) for n in _TABS]
                 + [("mb_off", c_ptr * 8), ("mb_idx", c_ptr * 8), ("mb_val", c_ptr * 8)]

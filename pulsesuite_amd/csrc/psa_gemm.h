@@ -94,9 +94,11 @@ __device__ __forceinline__ void load_dense(Frag<T> &s, const double *X, int ldx,
       x = idx & 63;
     }
     const int gx = x0 + x, gk = k0 + k;
-    cd v = {0.0, 0.0};
-    if (gx < xext && gk < kext && !(zero_diag && gx == gk))
-      v = ld(X, xmajor ? (long)gx * ldx + gk : (long)gk * ldx + gx);
+    // unconditional load from a clamped in-range address, masked after, so
+    // the loads of a slab are in flight together
+    const int cx = min(gx, xext - 1), ck = min(gk, kext - 1);
+    cd v = ld(X, xmajor ? (long)cx * ldx + ck : (long)ck * ldx + cx);
+    if (!(gx < xext && gk < kext && !(zero_diag && gx == gk))) v = cd{0.0, 0.0};
     s.set(e, k, x, (T)(sgn * v.re), (T)(conj ? -sgn * v.im : sgn * v.im));
   }
 }
@@ -121,11 +123,12 @@ __global__ __launch_bounds__(256) void gemm_kernel(P p) {
   // XCD-aware tile order: consecutive workgroups go to different XCDs
   // (round robin over 8).  When the batch count is a multiple of 8, batch b
   // runs entirely on XCD b % 8 (its tiles share that XCD's L2 for their
-  // operands) and neighbouring batches still spread over all XCDs.
+  // operands) and neighbouring batches still spread over all XCDs
+  // (tune[2] = 1 selects the plain linear order).
   const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
   unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
   int b;
-  if (gridDim.z % 8 == 0) {
+  if (gridDim.z % 8 == 0 && p.g.tune[2] == 0) {
     const unsigned xcd = lin % 8, q = lin / 8;
     b = xcd + 8 * (q / tpb);
     lin = q % tpb;
@@ -344,8 +347,8 @@ struct RhsProb {
       const int idx = tid + 256 * e;
       const int x = idx >> 4, k = idx & 15;
       const int gx = x0 + x, gk = k0 + k;
-      double v = 0.0;
-      if (gx < N && gk < N && gk != gx) v = tab[(long)gx * Q + (gk - gx + N - 1)];
+      const double raw = tab[(long)min(gx, N - 1) * Q + min(max(gk - gx + N - 1, 0), Q - 1)];
+      const double v = (gx < N && gk < N && gk != gx) ? raw : 0.0;
       s.set(e, k, x, (T)0, (T)(g.hbar * v));
     }
   }
@@ -454,6 +457,7 @@ struct CalcHProb {
       const int m = k0 + k, dd = n0 + x;
       const int c = m + dd - (N - 1);  // column m + d
       cd v = {0.0, 0.0};
+      // about half of this operand is structurally zero: load only the used part
       if (m < N && dd < 2 * N - 1 && c >= 0 && c < N) v = ld(X, (long)m * N + c);
       s.set(e, k, x, (T)v.re, (T)v.im);
     }
@@ -597,9 +601,9 @@ struct OffdiagProb {
       const int idx = tid + 256 * e;
       const int x = idx >> 4, k = idx & 15;
       const int kr = m0 + x, pp = k0 + k;
-      double v = 0.0;
-      if (kr >= kb && kr < ke && pp < 2 * N && (t == 0) == (pp < N))
-        v = tab[(r0 + kr) * 2 * N + pp];
+      // clamped in-range address, masked after: the slab's loads are in flight together
+      const double raw = tab[(r0 + min(max(kr, kb), ke - 1)) * 2 * N + min(pp, 2 * N - 1)];
+      const double v = (kr >= kb && kr < ke && pp < 2 * N && (t == 0) == (pp < N)) ? raw : 0.0;
       s.set(e, k, x, (T)v, (T)0);
     }
   }
@@ -618,29 +622,21 @@ struct OffdiagProb {
       const int idx = tid + 256 * e;
       const int k = idx >> 6, x = idx & 63;
       const int pp = k0 + k, col = n0 + x;
-      double v = 0.0;
-      if (pp < 2 * N && col < 2 * nw && aq > 0 && (t == 0) == (pp < N)) {
-        const int w = w0 + (col >> 1), sA = col & 1;
-        // same-carrier occupations: n[p-q] (1-n[p]) | (1-n[p-q]) n[p]
-        // other-carrier:            o[p+q] (1-o[p]) | (1-o[p+q]) o[p]
-        const double *ns = aocc + (kind == 0 ? 0L : (long)N * W) + w;
-        const double *no = aocc + (kind == 0 ? (long)N * W : 0L) + w;
-        if (pp < N) {
-          const int p = pp, pm = p - q;
-          if (pm >= 1 && pm < N) {
-            const double a = ns[(long)pm * W], c = ns[(long)p * W];
-            const double V2 = vsq[((long)(kind == 0 ? 1 : 2) * N + aq) * W + w];
-            v = V2 * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
-          }
-        } else {
-          const int p = pp - N, pq = p + q;
-          if (pq >= 0 && pq < N) {
-            const double a = no[(long)pq * W], c = no[(long)p * W];
-            const double V2 = vsq[(long)aq * W + w];
-            v = V2 * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c);
-          }
-        }
-      }
+      // same-carrier occupations: n[p-q] (1-n[p]) | (1-n[p-q]) n[p]
+      // other-carrier:            o[p+q] (1-o[p]) | (1-o[p+q]) o[p]
+      // loads from clamped in-range indices, masked after
+      const int w = w0 + min(col >> 1, nw - 1), sA = col & 1;
+      const bool same = pp < N;
+      const int p = same ? pp : pp - N;
+      const int j = same ? p - q : p + q;  // p-q (same) or p+q (other)
+      const double *nn = aocc + ((kind == 0) == same ? 0L : (long)N * W) + w;
+      const int jc = min(max(j, 0), N - 1), pc = min(max(p, 0), N - 1);
+      const double a = nn[(long)jc * W], c = nn[(long)pc * W];
+      const long vrow = same ? (long)(kind == 0 ? 1 : 2) * N : 0L;
+      const double V2 = vsq[(vrow + min(aq, N - 1)) * W + w];
+      const bool ok = pp < 2 * N && col < 2 * nw && aq > 0 && (t == 0) == same &&
+                      (same ? (j >= 1 && j < N) : (j >= 0 && j < N));
+      const double v = ok ? V2 * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c) : 0.0;
       s.set(e, k, x, (T)v, (T)0);
     }
   }
@@ -688,11 +684,10 @@ struct Prop2qwProb {
       const int idx = tid + 256 * e;
       const int x = idx >> 4, k = idx & 15;
       const int col = n0 + x, xi = k0 + k;
-      double v = 0.0;
-      if (col < 3 * nw && xi < g.gnx) {
-        const int w = w0 + col / 3, f = col % 3;
-        v = g.efield[(((long)f * g.Nw + w) * g.Nx + g.gx0 + xi) * 2];
-      }
+      const int cc = min(col, 3 * nw - 1);  // clamped, masked below
+      const int w = w0 + cc / 3, f = cc % 3;
+      const double raw = g.efield[(((long)f * g.Nw + w) * g.Nx + g.gx0 + min(xi, g.gnx - 1)) * 2];
+      const double v = (col < 3 * nw && xi < g.gnx) ? raw : 0.0;
       s.set(e, k, x, (T)v, (T)0);
     }
   }
@@ -737,11 +732,10 @@ struct SrcProb {
       const int idx = tid + 256 * e;
       const int x = idx >> 4, k = idx & 15;
       const int col = n0 + x, si = k0 + k;
-      cd v = {0.0, 0.0};
-      if (col < 4 * nw && si < Q) {
-        const int w = w0 + (col >> 2), f = col & 3;
-        v = ld(g.src + (long)w * PSA_SRC_STRIDE(N, g.Nr) + PSA_SRC_DS(g.Nr), (long)f * Q + si);
-      }
+      const int cc = min(col, 4 * nw - 1);  // clamped, masked below
+      const int w = w0 + (cc >> 2), f = cc & 3;
+      cd v = ld(g.src + (long)w * PSA_SRC_STRIDE(N, g.Nr) + PSA_SRC_DS(g.Nr), (long)f * Q + min(si, Q - 1));
+      if (!(col < 4 * nw && si < Q)) v = cd{0.0, 0.0};
       s.set(e, k, x, (T)v.re, (T)v.im);
     }
   }
@@ -794,8 +788,8 @@ struct Qw2pProb {
       const int idx = tid + 256 * e;
       const int k = idx >> 6, x = idx & 63;
       const int b = m0 + x, r = k0 + k;
-      double v = 0.0;
-      if (b < g.bnx && r < g.Nr) v = g.m2[(long)r * g.bnx + b];
+      const double raw = g.m2[(long)min(r, g.Nr - 1) * g.bnx + min(b, g.bnx - 1)];  // clamped
+      const double v = (b < g.bnx && r < g.Nr) ? raw : 0.0;
       s.set(e, k, x, (T)v, (T)0);
     }
   }
@@ -808,11 +802,10 @@ struct Qw2pProb {
       const int idx = tid + 256 * e;
       const int x = idx >> 4, k = idx & 15;
       const int col = n0 + x, r = k0 + k;
-      double v = 0.0;
-      if (col < 5 * nw && r < Nr) {
-        const int w = w0 + col / 5, f = col % 5;
-        v = g.src[(long)w * PSA_SRC_STRIDE(N, Nr) + f * Nr + r];
-      }
+      const int cc = min(col, 5 * nw - 1);  // clamped, masked below
+      const int w = w0 + cc / 5, f = cc % 5;
+      const double raw = g.src[(long)w * PSA_SRC_STRIDE(N, Nr) + f * Nr + min(r, Nr - 1)];
+      const double v = (col < 5 * nw && r < Nr) ? raw : 0.0;
       s.set(e, k, x, (T)v, (T)0);
     }
   }

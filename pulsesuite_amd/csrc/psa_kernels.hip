@@ -524,10 +524,10 @@ __global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
 // The phase is separable, so it is a row pass and a column pass of
 // FFT -> phase -> inverse FFT, each in LDS (Stockham autosort, radix 4
 // stages while 4 divides the remaining length, then 2 and 3, N = 2^a 3^b,
-// twiddles from a per-N table).  One wave per line, TR_LINES lines per
-// block; the column pass reads and writes TR_LINES adjacent columns per row.
+// twiddles from a per-N table).  One wave per line, 2, 4 or 8 lines per
+// block (tune[1]); the column pass reads and writes that many adjacent
+// columns per row.
 // ===========================================================================
-constexpr int TR_LINES = 4;
 
 __device__ __forceinline__ cd tw_get(const cd *tw, int idx, int sign) {
   cd w = tw[idx];  // exp(-2 pi i idx / N)
@@ -702,17 +702,18 @@ __global__ void k_tphase(psa_group g, int w0, double dt) {
 // fastest) fall on different banks.
 __host__ __device__ inline int tr_line_stride(int N) { return N + N / 4 + 1; }
 
-// NT: N at compile time (0: taken from the group at run time)
-template <int NT>
-__global__ __launch_bounds__(64 * TR_LINES) void k_transport(psa_group g, int w0, int pass, int avg) {
+// NT: N at compile time (0: taken from the group at run time); LINES lines
+// (waves) per block.
+template <int NT, int LINES>
+__global__ __launch_bounds__(64 * LINES) void k_transport(psa_group g, int w0, int pass, int avg) {
   extern __shared__ double smem[];
   const int N = NT ? NT : g.N, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int LS = tr_line_stride(N);
   const int mat = blockIdx.y;  // 0 C, 1 D, 2 p
   const int w = w0 + blockIdx.z;
-  const int line0 = blockIdx.x * TR_LINES;
+  const int line0 = blockIdx.x * LINES;
   cd *base = reinterpret_cast<cd *>(smem);
-  cd *tw = base + TR_LINES * LS;  // [N] twiddles
+  cd *tw = base + LINES * LS;  // [N] twiddles
   for (int j = tid; j < N; j += blockDim.x) tw[j] = ld(g.twid, j);
   const long o = mat_off(g, w);
   double *X = (mat == 0 ? g.CX : (mat == 1 ? g.DX : g.pX)) + o;
@@ -723,8 +724,8 @@ __global__ __launch_bounds__(64 * TR_LINES) void k_transport(psa_group g, int w0
     if (line < N)
       for (int j = lane; j < N; j += 64) x[fpad(j)] = ld(X, (long)line * N + j);
   } else {
-    for (int idx = tid; idx < TR_LINES * N; idx += blockDim.x) {
-      const int c = idx % TR_LINES, j = idx / TR_LINES;
+    for (int idx = tid; idx < LINES * N; idx += blockDim.x) {
+      const int c = idx % LINES, j = idx / LINES;
       if (line0 + c < N) base[c * LS + fpad(j)] = ld(X, (long)j * N + line0 + c);
     }
   }
@@ -740,8 +741,8 @@ __global__ __launch_bounds__(64 * TR_LINES) void k_transport(psa_group g, int w0
     if (line < N)
       for (int j = lane; j < N; j += 64) st(X, (long)line * N + j, cscale(x[fpad(j)], inv));
   } else {
-    for (int idx = tid; idx < TR_LINES * N; idx += blockDim.x) {
-      const int c = idx % TR_LINES, j = idx / TR_LINES, col = line0 + c;
+    for (int idx = tid; idx < LINES * N; idx += blockDim.x) {
+      const int c = idx % LINES, j = idx / LINES, col = line0 + c;
       if (col < N) {
         const long ij = (long)j * N + col;
         const cd v = cscale(base[c * LS + fpad(j)], inv);
@@ -1020,25 +1021,31 @@ int launch_gemm(const P &p, int prec, int M, int Ncols, int batches, hipStream_t
   return launch_gemm_t<float>(p, grid, s, what);
 }
 
-// Split-bf16 RHS: operand planes once per step, then the bf16 MFMA products.
+template <class K, size_t SHM, int TILE>
+int launch_rhs_bf16(const RhsProb &p, int nw, hipStream_t s, K kern) {
+  static bool attr_set = false;  // per instantiation
+  if (!attr_set) {
+    int rc = hip_check(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)SHM),
+                       "hipFuncSetAttribute(k_rhs bf16)");
+    if (rc) return rc;
+    attr_set = true;
+  }
+  const int nb = (p.g.N + TILE - 1) / TILE;
+  hipLaunchKernelGGL(kern, dim3(nb, nb, 3 * nw), dim3(256), SHM, s, p);
+  PSA_LAUNCH_CHECK("k_rhs bf16");
+  return PSA_OK;
+}
+
+// Split-bf16 RHS: operand planes once per step, then the bf16 MFMA products
+// (64x64 or 128x128 tiles, tune[0]).
 int rhs_bf16(const RhsProb &p, int nw, hipStream_t s) {
   const psa_group *g = &p.g;
   if (!g->spl) return fail(PSA_E_INVALID, "bf16x3 mode needs the spl operand buffer");
   const int nt = (g->N + 63) / 64;
   hipLaunchKernelGGL(k_split_bf16, dim3(nt, nt, 6 * nw), dim3(256), 0, s, *g, p.w0);
   PSA_LAUNCH_CHECK("k_split_bf16");
-  constexpr size_t shm = rhs_bf16_lds_bytes();
-  static bool attr_set = false;
-  if (!attr_set) {
-    int rc = hip_check(hipFuncSetAttribute((const void *)k_rhs_bf16,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
-                       "hipFuncSetAttribute(k_rhs_bf16)");
-    if (rc) return rc;
-    attr_set = true;
-  }
-  hipLaunchKernelGGL(k_rhs_bf16, dim3(nt, nt, 3 * nw), dim3(256), shm, s, p);
-  PSA_LAUNCH_CHECK("k_rhs_bf16");
-  return PSA_OK;
+  if (g->tune[0] == 1) return launch_rhs_bf16<decltype(&t128::k_rhs), t128::lds_bytes(), t128::TILE>(p, nw, s, &t128::k_rhs);
+  return launch_rhs_bf16<decltype(&t64::k_rhs), t64::lds_bytes(), t64::TILE>(p, nw, s, &t64::k_rhs);
 }
 
 }  // namespace
@@ -1152,23 +1159,34 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
 
 namespace {
 
-template <int NT>
-int launch_transport(const psa_group *g, int32_t w0, dim3 grid, size_t shm, hipStream_t s, bool avg) {
+template <int NT, int LINES>
+int launch_transport_l(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
+  const size_t shm = ((size_t)LINES * tr_line_stride(g->N) + g->N) * sizeof(cd);  // lines + twiddles
   if (shm > 65536) {
     static bool attr_set = false;  // per instantiation
     if (!attr_set) {
-      int rc = hip_check(hipFuncSetAttribute((const void *)k_transport<NT>,
+      int rc = hip_check(hipFuncSetAttribute((const void *)k_transport<NT, LINES>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
                          "hipFuncSetAttribute(k_transport)");
       if (rc) return rc;
       attr_set = true;
     }
   }
-  hipLaunchKernelGGL(k_transport<NT>, grid, dim3(64 * TR_LINES), shm, s, *g, w0, 0, 0);
+  const dim3 grid((g->N + LINES - 1) / LINES, 3, nw);
+  hipLaunchKernelGGL((k_transport<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 0, 0);
   PSA_LAUNCH_CHECK("k_transport");
-  hipLaunchKernelGGL(k_transport<NT>, grid, dim3(64 * TR_LINES), shm, s, *g, w0, 1, avg ? 1 : 0);
+  hipLaunchKernelGGL((k_transport<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 1, avg ? 1 : 0);
   PSA_LAUNCH_CHECK("k_transport");
   return PSA_OK;
+}
+
+template <int NT>
+int launch_transport(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
+  switch (g->tune[1]) {
+    case 2: return launch_transport_l<NT, 2>(g, w0, nw, s, avg);
+    case 8: return launch_transport_l<NT, 8>(g, w0, nw, s, avg);
+    default: return launch_transport_l<NT, 4>(g, w0, nw, s, avg);
+  }
 }
 
 int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStream_t s, bool avg) {
@@ -1180,17 +1198,15 @@ int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStr
   if (n != 1) return fail(PSA_E_UNSUPPORTED, "Transport needs N = 2^a 3^b");
   hipLaunchKernelGGL(k_tphase, dim3(nw), dim3(256), 0, s, *g, w0, dt);
   PSA_LAUNCH_CHECK("k_tphase");
-  const size_t shm = ((size_t)TR_LINES * tr_line_stride(g->N) + g->N) * sizeof(cd);  // lines + twiddles
-  const dim3 grid((g->N + TR_LINES - 1) / TR_LINES, 3, nw);
   switch (g->N) {
-    case 64: return launch_transport<64>(g, w0, grid, shm, s, avg);
-    case 96: return launch_transport<96>(g, w0, grid, shm, s, avg);
-    case 128: return launch_transport<128>(g, w0, grid, shm, s, avg);
-    case 192: return launch_transport<192>(g, w0, grid, shm, s, avg);
-    case 256: return launch_transport<256>(g, w0, grid, shm, s, avg);
-    case 384: return launch_transport<384>(g, w0, grid, shm, s, avg);
-    case 512: return launch_transport<512>(g, w0, grid, shm, s, avg);
-    default: return launch_transport<0>(g, w0, grid, shm, s, avg);
+    case 64: return launch_transport<64>(g, w0, nw, s, avg);
+    case 96: return launch_transport<96>(g, w0, nw, s, avg);
+    case 128: return launch_transport<128>(g, w0, nw, s, avg);
+    case 192: return launch_transport<192>(g, w0, nw, s, avg);
+    case 256: return launch_transport<256>(g, w0, nw, s, avg);
+    case 384: return launch_transport<384>(g, w0, nw, s, avg);
+    case 512: return launch_transport<512>(g, w0, nw, s, avg);
+    default: return launch_transport<0>(g, w0, nw, s, avg);
   }
 }
 

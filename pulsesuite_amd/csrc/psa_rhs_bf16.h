@@ -36,11 +36,11 @@ __device__ __forceinline__ void split_bf16(double x, uint16_t &hi, uint16_t &lo)
   lo = bf16_bits_rn((float)(x - (double)bf16_to_f(hi)));
 }
 
-// Blocked operand layout: slot matrices padded to NR = 64 ceil(N/64) rows
+// Blocked operand layout: slot matrices padded to NR = 128 ceil(N/128) rows
 // and NK = 32 ceil(N/32) columns (k), stored as [row block][k block]
 // chunks of 4 planes x 64 rows x 32 k (16 KB, contiguous), so one K slab
 // of one 64-row tile is a single contiguous chunk.  The padding stays zero.
-__host__ __device__ inline int spl_nr(int N) { return (N + 63) / 64 * 64; }
+__host__ __device__ inline int spl_nr(int N) { return (N + 127) / 128 * 128; }
 __host__ __device__ inline int spl_nk(int N) { return (N + 31) / 32 * 32; }
 __host__ __device__ inline long spl_slot_elems(int N) { return 4L * spl_nr(N) * spl_nk(N); }
 __host__ __device__ inline long spl_chunk(int N, int w, int slot, int rb, int kb) {
@@ -95,17 +95,6 @@ __global__ __launch_bounds__(256) void k_split_bf16(psa_group g, int w0) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// k_rhs_bf16: block = 64x64 output tile of one (wire, equation); 4 waves
-// (2x2) of 32x32 = 2x2 MFMA blocks; K slabs of 32 staged in LDS (row stride
-// 40 bf16 = 80 B: the 16 rows a lane group reads land on distinct banks).
-// ---------------------------------------------------------------------------
-constexpr int BK2 = 32, BLD2 = 40;
-
-struct BTile {
-  uint16_t v[4][64][BLD2];  // [plane][row][k]
-};
-
 // One term of an equation: A and B slots (-1: the off-diagonal dephasing
 // band, generated from dedh), conjugation and sign.
 struct BTerm {
@@ -132,6 +121,26 @@ __device__ __forceinline__ BTerm bterm(int e, int t) {
     default: return {7, 0, true, false, -1.0};   //     -Heh^H p^T
   }
 }
+
+__device__ __forceinline__ bf16x8 bneg(bf16x8 x) {
+  u32x4 u = __builtin_bit_cast(u32x4, x);
+  u ^= u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+namespace t64 {
+
+// ---------------------------------------------------------------------------
+// k_rhs (64-tile variant): block = 64x64 output tile of one (wire, equation); 4 waves
+// (2x2) of 32x32 = 2x2 MFMA blocks; K slabs of 32 staged in LDS (row stride
+// 40 bf16 = 80 B: the 16 rows a lane group reads land on distinct banks).
+// ---------------------------------------------------------------------------
+constexpr int BK2 = 32, BLD2 = 40, TILE = 64;
+
+struct BTile {
+  uint16_t v[4][64][BLD2];  // [plane][row][k]
+};
+
 
 // Register fragment of one slab: 4 planes x one 16-byte chunk (8 k).
 struct BFrag {
@@ -161,12 +170,15 @@ __device__ __forceinline__ void bload_band(BFrag &f, const psa_group &g, int w, 
   f.kc = (tid & 3) * 8;
   const int gx = x0 + f.row, gk = k0 + f.kc;
   const double *tab = g.dedh + ((long)w * 2 + kind) * (long)N * Q;
+  const double *row = tab + (long)min(gx, N - 1) * Q;
+  double raw[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) raw[e] = row[min(max(gk + e - gx + N - 1, 0), Q - 1)];
   uint16_t hi[8], lo[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int m = gk + e;
-    double v = 0.0;
-    if (gx < N && m < N && m != gx) v = g.hbar * tab[(long)gx * Q + (m - gx + N - 1)];
+    const double v = (gx < N && m < N && m != gx) ? g.hbar * raw[e] : 0.0;
     split_bf16(v, hi[e], lo[e]);
   }
   f.c[0] = u32x4{0u, 0u, 0u, 0u};
@@ -184,11 +196,6 @@ __device__ __forceinline__ bf16x8 bfrag_read(const BTile &t, int pl, int row, in
   return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(&t.v[pl][row][kq]));
 }
 
-__device__ __forceinline__ bf16x8 bneg(bf16x8 x) {
-  u32x4 u = __builtin_bit_cast(u32x4, x);
-  u ^= u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-  return __builtin_bit_cast(bf16x8, u);
-}
 
 __device__ __forceinline__ fv4 mma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, fv4 c) {
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
@@ -248,7 +255,7 @@ __device__ __forceinline__ void bslab(const BTile &sA, const BTile &sB, const BT
   }
 }
 
-__global__ __launch_bounds__(256) void k_rhs_bf16(RhsProb p) {
+__global__ __launch_bounds__(256) void k_rhs(RhsProb p) {
   extern __shared__ char lds_raw[];
   BTile *tiles = reinterpret_cast<BTile *>(lds_raw);  // [2 buffers][A, B]
   const psa_group &g = p.g;
@@ -260,7 +267,7 @@ __global__ __launch_bounds__(256) void k_rhs_bf16(RhsProb p) {
   const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
   unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
   int b;
-  if (gridDim.z % 24 == 0) {
+  if (gridDim.z % 24 == 0 && g.tune[2] == 0) {
     const unsigned xcd = lin % 8, q = lin / 8, per = 3 * tpb;
     b = 3 * (xcd + 8 * (q / per)) + (q % per) / tpb;
     lin = q % tpb;
@@ -325,6 +332,223 @@ __global__ __launch_bounds__(256) void k_rhs_bf16(RhsProb p) {
       }
 }
 
-constexpr size_t rhs_bf16_lds_bytes() { return 4 * sizeof(BTile); }
+constexpr size_t lds_bytes() { return 4 * sizeof(BTile); }
+
+}  // namespace t64
+
+namespace t128 {
+
+// ---------------------------------------------------------------------------
+// k_rhs (128-tile variant): block = 128x128 output tile of one (wire, equation); 4 waves
+// (2x2) of 64x64 = 4x4 MFMA blocks each; K slabs of 32 staged in LDS (two
+// stages, 64 KB each).  LDS rows are 64 B with the 16-byte chunk index
+// XOR-ed with (row >> 2) & 3, which keeps both the staging writes (4 rows x
+// 4 chunks per 16 lanes) and the MFMA operand reads (16 rows x 1 chunk) on
+// distinct banks.  The epilogue stages the accumulators through LDS and
+// stores in row-major order.
+// ---------------------------------------------------------------------------
+constexpr int BT = 128, BK2 = 32, TILE = BT;
+
+struct BStage {
+  uint16_t a[4][BT][BK2];  // [plane][row][k]
+  uint16_t b[4][BT][BK2];
+};
+
+__device__ __forceinline__ int bswz(int row, int chunk) { return chunk ^ ((row >> 2) & 3); }
+
+
+// Register fragment of one slab of one operand: this thread's 8 16-byte
+// chunks, piece j = plane (j >> 1), row (tid >> 2) + 64 (j & 1), chunk tid & 3.
+struct BFrag {
+  u32x4 c[8];
+};
+
+// Rows [x0, x0+128) of a dense slot, slab [k0, k0+32): two contiguous 16 KB
+// chunks of the blocked layout (x0, k0 multiples of 128 and 32).
+__device__ __forceinline__ void bload_dense(BFrag &f, const uint16_t *spl, int N, int w, int slot,
+                                            int x0, int k0) {
+  const int tid = threadIdx.x;
+  const int off = (tid >> 2) * 32 + (tid & 3) * 8;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint16_t *src = spl + spl_chunk(N, w, slot, (x0 >> 6) + h, k0 >> 5) + off;
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl) f.c[pl * 2 + h] = *reinterpret_cast<const u32x4 *>(src + pl * (64 * 32));
+  }
+}
+
+// The banded dephasing operand i hbar D[m - x][x] for rows x and k = m
+// (dedh table kind 1 = Dh as A, 0 = De as B; q = 0 excluded): purely
+// imaginary, so the real planes are zero.
+__device__ __forceinline__ void bload_band(BFrag &f, const psa_group &g, int w, int kind, int x0,
+                                           int k0) {
+  const int N = g.N, Q = 2 * N - 1, tid = threadIdx.x;
+  const double *tab = g.dedh + ((long)w * 2 + kind) * (long)N * Q;
+  const int gk = k0 + (tid & 3) * 8;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int gx = x0 + (tid >> 2) + 64 * h;
+    // unconditional loads from clamped in-range addresses, masked after, so
+    // the eight loads are in flight together (no wait per element)
+    const double *row = tab + (long)min(gx, N - 1) * Q;
+    double raw[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) raw[e] = row[min(max(gk + e - gx + N - 1, 0), Q - 1)];
+    uint16_t hi[8], lo[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int m = gk + e;
+      const double v = (gx < N && m < N && m != gx) ? g.hbar * raw[e] : 0.0;
+      split_bf16(v, hi[e], lo[e]);
+    }
+    f.c[0 * 2 + h] = u32x4{0u, 0u, 0u, 0u};
+    f.c[1 * 2 + h] = u32x4{0u, 0u, 0u, 0u};
+    f.c[2 * 2 + h] = __builtin_bit_cast(u32x4, hi);
+    f.c[3 * 2 + h] = __builtin_bit_cast(u32x4, lo);
+  }
+}
+
+__device__ __forceinline__ void bcommit(const BFrag &f, uint16_t (*t)[BT][BK2]) {
+  const int tid = threadIdx.x, chunk = tid & 3;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int pl = j >> 1, row = (tid >> 2) + 64 * (j & 1);
+    *reinterpret_cast<u32x4 *>(&t[pl][row][bswz(row, chunk) * 8]) = f.c[j];
+  }
+}
+
+__device__ __forceinline__ bf16x8 bread(const uint16_t (*t)[BT][BK2], int pl, int row, int kc) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(&t[pl][row][bswz(row, kc) * 8]));
+}
+
+
+// acc[si][sj] += (sign) A_x B_y over this wave's 4x4 blocks with the three
+// split products (hi hi, hi lo, lo hi), issued so that products on the same
+// accumulator are 16 issues apart.
+__device__ __forceinline__ void bpart(const BStage &st, int pa, int pb, bool neg, fv4 (&acc)[4][4],
+                                      int wm, int wn, int kl, int xl) {
+  bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int ra = wm * 64 + s * 16 + xl, rb = wn * 64 + s * 16 + xl;
+    ah[s] = bread(st.a, pa, ra, kl);
+    al[s] = bread(st.a, pa + 1, ra, kl);
+    bh[s] = bread(st.b, pb, rb, kl);
+    bl[s] = bread(st.b, pb + 1, rb, kl);
+    if (neg) {
+      ah[s] = bneg(ah[s]);
+      al[s] = bneg(al[s]);
+    }
+  }
+#pragma unroll
+  for (int si = 0; si < 4; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 4; ++sj)
+      acc[si][sj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[si], bh[sj], acc[si][sj], 0, 0, 0);
+#pragma unroll
+  for (int si = 0; si < 4; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 4; ++sj)
+      acc[si][sj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[si], bl[sj], acc[si][sj], 0, 0, 0);
+#pragma unroll
+  for (int si = 0; si < 4; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 4; ++sj)
+      acc[si][sj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[si], bh[sj], acc[si][sj], 0, 0, 0);
+}
+
+// The split products of one K slab of one term.
+__device__ __forceinline__ void bslab(const BStage &st, const BTerm bt, fv4 (&aR)[4][4], fv4 (&aI)[4][4],
+                                      int wm, int wn, int kl, int xl) {
+  // signs of the four real products (conj flips the imaginary part)
+  const double cA = bt.ca ? -1.0 : 1.0, cB = bt.cb ? -1.0 : 1.0;
+  const bool nRR = bt.sg < 0, nII = -bt.sg * cA * cB < 0, nRI = bt.sg * cB < 0, nIR = bt.sg * cA < 0;
+  const bool aRe0 = bt.sa < 0, bRe0 = bt.sb < 0;  // band operands are purely imaginary
+  if (!aRe0 && !bRe0) bpart(st, 0, 0, nRR, aR, wm, wn, kl, xl);  // Re A Re B
+  if (!aRe0) bpart(st, 0, 2, nRI, aI, wm, wn, kl, xl);           // Re A Im B
+  bpart(st, 2, 2, nII, aR, wm, wn, kl, xl);                      // -Im A Im B
+  if (!bRe0) bpart(st, 2, 0, nIR, aI, wm, wn, kl, xl);           // Im A Re B
+}
+
+__global__ __launch_bounds__(256) void k_rhs(RhsProb p) {
+  extern __shared__ char lds_raw[];
+  BStage *stage = reinterpret_cast<BStage *>(lds_raw);  // [2]
+  const psa_group &g = p.g;
+  const int N = g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  // XCD-aware order: the three equations of wire j run on XCD j % 8 (they
+  // share the wire's operand planes in that XCD's L2) when the wire count
+  // is a multiple of 8; consecutive workgroups go round robin over XCDs.
+  const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
+  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  int b;
+  if (gridDim.z % 24 == 0 && g.tune[2] == 0) {
+    const unsigned xcd = lin % 8, q = lin / 8, per = 3 * tpb;
+    b = 3 * (xcd + 8 * (q / per)) + (q % per) / tpb;
+    lin = q % tpb;
+  } else {
+    b = lin / tpb;
+    lin %= tpb;
+  }
+  const int m0 = (lin / gx) * BT, n0 = (lin % gx) * BT;
+  const int e = b % 3, w = p.w0 + b / 3;
+  const int nt = p.nterms(b);
+  const uint16_t *spl = reinterpret_cast<const uint16_t *>(g.spl);
+  fv4 aR[4][4], aI[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      aR[i][j] = fv4{0, 0, 0, 0};
+      aI[i][j] = fv4{0, 0, 0, 0};
+    }
+  auto fetch = [&](BFrag &fa, BFrag &fb, int s, int KS) {
+    const BTerm bt = bterm(e, s / KS);
+    const int k0 = (s % KS) * BK2;
+    if (bt.sa >= 0) bload_dense(fa, spl, N, w, bt.sa, m0, k0);
+    else bload_band(fa, g, w, 1, m0, k0);
+    if (bt.sb >= 0) bload_dense(fb, spl, N, w, bt.sb, n0, k0);
+    else bload_band(fb, g, w, 0, n0, k0);
+  };
+  // flattened slabs s = term * KS + k0 / 32, one-slab register prefetch
+  const int KS = (N + BK2 - 1) / BK2, S = nt * KS;
+  const int kl = lane >> 4, xl = lane & 15;
+  BFrag fa, fb;
+  fetch(fa, fb, 0, KS);
+  bcommit(fa, stage[0].a);
+  bcommit(fb, stage[0].b);
+  __syncthreads();
+  for (int s = 0; s < S; ++s) {
+    if (s + 1 < S) fetch(fa, fb, s + 1, KS);
+    bslab(stage[s & 1], bterm(e, s / KS), aR, aI, wm, wn, kl, xl);
+    if (s + 1 >= S) break;
+    bcommit(fa, stage[(s + 1) & 1].a);
+    bcommit(fb, stage[(s + 1) & 1].b);
+    __syncthreads();
+  }
+  // epilogue: accumulators -> LDS [re/im][128][128] f32 -> row-major stores
+  __syncthreads();
+  float *acc = reinterpret_cast<float *>(lds_raw);
+#pragma unroll
+  for (int si = 0; si < 4; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 4; ++sj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = wm * 64 + si * 16 + 4 * kl + v, c = wn * 64 + sj * 16 + xl;
+        acc[r * BT + c] = aR[si][sj][v];
+        acc[BT * BT + r * BT + c] = aI[si][sj][v];
+      }
+  __syncthreads();
+#pragma unroll 1
+  for (int i = tid; i < BT * BT; i += 256) {
+    const int r = i / BT, c = i % BT;
+    p.store(b, m0 + r, n0 + c, (double)acc[i], (double)acc[BT * BT + i]);
+  }
+}
+
+constexpr size_t lds_bytes() { return 2 * sizeof(BStage); }
+
+}  // namespace t128
 
 }  // namespace psa

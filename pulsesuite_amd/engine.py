@@ -101,8 +101,8 @@ class WireGroup:
         self.s["tphase"] = z(nwz, N)
         self.s["part"] = z(nwz, 256, dt=F64)
         # split-bf16 operand planes (bf16x3 only): [Nw][12 slots] of 4 planes
-        # on a zero-padded (64 ceil(N/64)) x (32 ceil(N/32)) grid, blocked
-        nspl = nwz * 12 * 4 * (-(-N // 64) * 64) * (-(-N // 32) * 32) if prec == "bf16x3" else 1
+        # on a zero-padded (128 ceil(N/128)) x (32 ceil(N/32)) grid, blocked
+        nspl = nwz * 12 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) if prec == "bf16x3" else 1
         self.s["spl"] = torch.zeros(nspl, dtype=torch.int16, device=d)
         if wire_ids is None:
             wire_ids = np.arange(1, nwz + 1)
@@ -111,6 +111,7 @@ class WireGroup:
             raise ValueError("wire_ids must have one entry per wire")
         self.s["wire_id"] = T(self.wire_ids, torch.int32)
         self._wid_host = self.wire_ids.copy()
+        self.tune = [0, 0, 0, 0]  # launch choices (psa_group.tune), 0 = defaults
         self.init_state()
         self._sync_struct()
         if od:
@@ -147,6 +148,8 @@ class WireGroup:
         S.drr = self.rr[1] - self.rr[0] if len(self.rr) > 1 else 1.0
         S.gam_e, S.gam_h, S.gam_eh, S.Edc = prm.gam_e, prm.gam_h, g["gam_eh"], prm.Edc
         S.Vscale, S.n1d_max, S.dhw = tb["Vscale"], tb["n1d_max"], tb["dhw"]
+        for i, v in enumerate(self.tune):
+            S.tune[i] = int(v)
         for k, v in self.t.items():
             setattr(S, k, v.data_ptr())
         for i, (off, idx, val) in enumerate(self.mb):
@@ -191,6 +194,49 @@ class WireGroup:
         for a, b in (("pB", "pX"), ("CB", "CX"), ("DB", "DX")):
             self.s[a], self.s[b] = self.s[b], self.s[a]
         self._sync_struct()
+
+    def autotune(self, reps=3):
+        """Pick the launch choices of this group by timing them on its own
+        buffers: tune[2] GEMM block order (XCD-grouped or linear), tune[0]
+        split-BF16 tile (64 or 128), tune[1] transport lines per block (4, 2,
+        8).  Only per-step scratch is written (the new level, Hamiltonians,
+        dephasing tables), so the state is unchanged.  Returns {knob: (choice,
+        ms per candidate)}."""
+        od = bool(self.prm.flags.get("OffDiagDph"))
+        self.prop2qw()
+        self.prep()
+        torch.cuda.synchronize()
+
+        def contractions():
+            self.calc_h()
+            if od:
+                self.offdiag()
+            self.rhs(1, 1e-18)
+
+        knobs = [(2, [0, 1], contractions)]
+        if self.prec == "bf16x3":
+            knobs.append((0, [0, 1], lambda: self.rhs(1, 1e-18)))
+        if self.prm.flags.get("DCTrans") and self.prm.flags.get("LF"):
+            knobs.append((1, [4, 2, 8], lambda: self.transport(1e-18)))
+        out = {}
+        for idx, cands, fn in knobs:
+            times = []
+            for c in cands:
+                self.tune[idx] = c
+                self._sync_struct()
+                fn()  # first launch of this choice (attribute set-up)
+                torch.cuda.synchronize()
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                for _ in range(reps):
+                    fn()
+                b.record()
+                torch.cuda.synchronize()
+                times.append(a.elapsed_time(b) / reps)
+            self.tune[idx] = cands[int(np.argmin(times))]
+            self._sync_struct()
+            out[idx] = (self.tune[idx], times)
+        return out
 
     def footprint_bytes(self):
         """Device bytes held by this group (state, scratch and tables)."""

@@ -90,7 +90,7 @@ def wire_length(nk, me_rel=0.067, gap_ev=1.5):
 class SlotPool:
     """Fixed-capacity slots of one Nk on one stream."""
 
-    def __init__(self, nk, capacity, rr, dt, prec, maxsteps, graphs=True, prm_kw=None):
+    def __init__(self, nk, capacity, rr, dt, prec, maxsteps, graphs=True, prm_kw=None, autotune=True):
         prm_kw = dict(prm_kw or {})
         prm_kw.setdefault("me_rel", 0.067)
         self.prm = tables.QWParams(L=wire_length(nk, prm_kw["me_rel"]), **prm_kw)
@@ -112,6 +112,8 @@ class SlotPool:
         self.xmid = len(rr) // 2
         self.lib = self.group.lib
         self.ticks = 0
+        if autotune:
+            self.tuned = self.group.autotune()
 
     # -- slots -----------------------------------------------------------
     def free(self):
@@ -209,7 +211,7 @@ def slot_bytes(nk, nx, prec, maxsteps):
     b += 2 * N * (2 * N - 1) * 8 + 11 * N * 8 + 3 * (2 * N - 1) * 16
     b += 3 * nx * 16 + 4 * nx * 16 + (9 * (2 * N + 2) + 8 * (2 * N - 1) + 8) * 8
     if prec == "bf16x3":
-        b += 12 * 4 * (-(-N // 64) * 64) * (-(-N // 32) * 32) * 2
+        b += 12 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) * 2
     b += maxsteps * 4 * 8 + 8 * 8
     return b
 
