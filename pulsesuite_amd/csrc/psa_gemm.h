@@ -636,12 +636,18 @@ struct OffdiagProb {
       const double V2 = vsq[(vrow + min(aq, N - 1)) * W + w];
       const bool ok = pp < 2 * N && col < 2 * nw && aq > 0 && (t == 0) == same &&
                       (same ? (j >= 1 && j < N) : (j >= 0 && j < N));
-      const double v = ok ? V2 * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c) : 0.0;
+      // V^2 ~ 1e-42 J^2 is below the FP32 range: the reduced-precision
+      // product runs on B scaled by 2^100 (exact), undone in store()
+      const double sc = sizeof(T) == 4 ? 0x1p100 : 1.0;
+      const double v = ok ? sc * V2 * (sA == 0 ? a * (1.0 - c) : (1.0 - a) * c) : 0.0;
       s.set(e, k, x, (T)v, (T)0);
     }
   }
 
+  int scaled;  // 1 when the products ran on the 2^100-scaled B operand
+
   __device__ void store(int b, int kr, int col, double re, double) const {
+    if (scaled) re *= 0x1p-100;
     const double other = __shfl_xor(re, 1, 64);
     const int N = g.N, kind = b % 2, qi = b / 2;
     if ((col & 1) || kr >= N || col >= 2 * nw) return;

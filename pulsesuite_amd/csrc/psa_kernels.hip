@@ -1092,8 +1092,11 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   CalcHProb p;
   p.g = *g;
   p.w0 = w0;
-  // the exchange sums feed Heh, which drives p directly: always FP64
-  return launch_gemm(p, PSA_PREC_FP64, g->N, 2 * g->N - 1, 3 * nw, s, "calc_h gemm");
+  // FP64, or FP32 MFMA in the reduced-precision modes (V ~ 1e-21 J and the
+  // products stay inside the FP32 range); the Meh, Hartree and band-energy
+  // parts are added in FP64 in the epilogue
+  const int prec = g->prec == PSA_PREC_FP64 ? PSA_PREC_FP64 : PSA_PREC_FP32;
+  return launch_gemm(p, prec, g->N, 2 * g->N - 1, 3 * nw, s, "calc_h gemm");
 }
 
 int psa_lorentz_table(const psa_group *g, void *stream) {
@@ -1110,9 +1113,11 @@ int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream)
   p.g = *g;
   p.w0 = w0;
   p.nw = nw;
-  // OffP enters dp directly: always FP64
-  return launch_gemm(p, PSA_PREC_FP64, g->N, 2 * nw, 2 * (2 * g->N - 1), (hipStream_t)stream,
-                     "offdiag gemm");
+  // FP64, or FP32 MFMA in the reduced-precision modes on 2^100-scaled
+  // occupation columns (positive terms only: no cancellation)
+  const int prec = g->prec == PSA_PREC_FP64 ? PSA_PREC_FP64 : PSA_PREC_FP32;
+  p.scaled = prec == PSA_PREC_FP32 ? 1 : 0;
+  return launch_gemm(p, prec, g->N, 2 * nw, 2 * (2 * g->N - 1), (hipStream_t)stream, "offdiag gemm");
 }
 
 int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt, void *stream) {
