@@ -294,73 +294,101 @@ __global__ __launch_bounds__(256) void k_h_simple(psa_group g, int w0) {
 // entries (i, j, l) of the stored table, grouped (CSR) by the rate index it
 // feeds (see the slot mapping below).  Block = (MB_KG target k, slot,
 // MB_WC wires): the block stages the occupations and the two V^2 channels
-// of its slot for its wires in LDS (from the wire-fastest tables built by
-// k_diag_gather), then a wave takes MB_WC wires x 8 entries per trip
-// (lane = 8 u + c), so every gather is an LDS read.  Partial sums are
-// reduced over u with shuffles and over waves in LDS, in a fixed order.
+// of its slot for its wires in LDS (rows of MB_WC wires, from the
+// wire-fastest tables built by k_diag_gather / k_wire_tables); a lane is
+// (entry group, wire) and the eight lanes of a group read one 64-byte LDS
+// row.  The sums are reduced over the block in a fixed order.
 //   wmb + 4 N Nw: occ[2][N][Nw]  |Re ne3|, |Re nh3|
 //   wmb + 6 N Nw: vsq[3][N][Nw]  VC(d)^2 for eh, ee, hh
 // ===========================================================================
-constexpr int MB_WC = 8, MB_KG = 4;
+constexpr int MB_WC = 8, MB_KG = 4, MB_THREADS = 512;
+
+// The three occupation factors of list L for entry (i, j, l): rows of the
+// electron (E) or hole (H) table, complemented (1 - n) or not.
+template <int L>
+struct MbList;
+// case 0: (1 - nh[j]) nh[j+l-i] ne[l]
+template <> struct MbList<0> { static constexpr int c1 = 1, c2 = 1, c3 = 0; static constexpr bool x1 = 1, x2 = 0, x3 = 0; };
+// case 1: (1 - ne[i]) (1 - nh[j]) nh[j+l-i]
+template <> struct MbList<1> { static constexpr int c1 = 0, c2 = 1, c3 = 1; static constexpr bool x1 = 1, x2 = 1, x3 = 0; };
+// case 2: (1 - ne[j]) ne[i+j-l] ne[l]
+template <> struct MbList<2> { static constexpr int c1 = 0, c2 = 0, c3 = 0; static constexpr bool x1 = 1, x2 = 0, x3 = 0; };
+// case 3: (1 - ne[i]) (1 - ne[j]) ne[i+j-l]
+template <> struct MbList<3> { static constexpr int c1 = 0, c2 = 0, c3 = 0; static constexpr bool x1 = 1, x2 = 1, x3 = 0; };
+// case 4: (1 - ne[i]) nh[j+l-i] ne[l]
+template <> struct MbList<4> { static constexpr int c1 = 0, c2 = 1, c3 = 0; static constexpr bool x1 = 1, x2 = 0, x3 = 0; };
+// case 5: (1 - ne[i]) (1 - nh[j]) ne[l]
+template <> struct MbList<5> { static constexpr int c1 = 0, c2 = 1, c3 = 0; static constexpr bool x1 = 1, x2 = 1, x3 = 0; };
+// case 6: (1 - nh[j]) nh[i+j-l] nh[l]
+template <> struct MbList<6> { static constexpr int c1 = 1, c2 = 1, c3 = 1; static constexpr bool x1 = 1, x2 = 0, x3 = 0; };
+// case 7: (1 - nh[i]) (1 - nh[j]) nh[i+j-l]
+template <> struct MbList<7> { static constexpr int c1 = 1, c2 = 1, c3 = 1; static constexpr bool x1 = 1, x2 = 1, x3 = 0; };
 
 template <int L>
-__device__ __forceinline__ double mb_term(const double *ne, const double *nh, const double *vv,
-                                          uint32_t pk) {
-  // ne, nh, vv: LDS [N][MB_WC] tables already offset to this lane's wire
+__device__ __forceinline__ void mb_indices(uint32_t pk, int &a, int &c, int &r1, int &r2, int &r3) {
   const int i = pk & 1023, j = (pk >> 10) & 1023, l = (pk >> 20) & 1023;
-  const int a = (L == 5) ? l : i, c = (L == 5) ? i : l;
-  // VC is lower-triangular: V(a, c) = 0 for a < c
-  const double vraw = vv[(a >= c ? a - c : 0) * MB_WC];
-  const double vq = (a >= c) ? vraw : 0.0;
-  double t;
-  if (L == 0) t = (1.0 - nh[j * MB_WC]) * nh[(j + l - i) * MB_WC] * ne[l * MB_WC];
-  else if (L == 1) t = (1.0 - ne[i * MB_WC]) * (1.0 - nh[j * MB_WC]) * nh[(j + l - i) * MB_WC];
-  else if (L == 2) t = (1.0 - ne[j * MB_WC]) * ne[(i + j - l) * MB_WC] * ne[l * MB_WC];
-  else if (L == 3) t = (1.0 - ne[i * MB_WC]) * (1.0 - ne[j * MB_WC]) * ne[(i + j - l) * MB_WC];
-  else if (L == 4) t = (1.0 - ne[i * MB_WC]) * nh[(j + l - i) * MB_WC] * ne[l * MB_WC];
-  else if (L == 5) t = (1.0 - ne[i * MB_WC]) * (1.0 - nh[j * MB_WC]) * ne[l * MB_WC];
-  else if (L == 6) t = (1.0 - nh[j * MB_WC]) * nh[(i + j - l) * MB_WC] * nh[l * MB_WC];
-  else t = (1.0 - nh[i * MB_WC]) * (1.0 - nh[j * MB_WC]) * nh[(i + j - l) * MB_WC];
-  return vq * t;
+  a = (L == 5) ? l : i;  // V(a, c): Veh with the index order swapped for list 5
+  c = (L == 5) ? i : l;
+  switch (L) {
+    case 0: r1 = j; r2 = j + l - i; r3 = l; break;
+    case 1: r1 = i; r2 = j; r3 = j + l - i; break;
+    case 2: r1 = j; r2 = i + j - l; r3 = l; break;
+    case 3: r1 = i; r2 = j; r3 = i + j - l; break;
+    case 4: r1 = i; r2 = j + l - i; r3 = l; break;
+    case 5: r1 = i; r2 = j; r3 = l; break;
+    case 6: r1 = j; r2 = i + j - l; r3 = l; break;
+    default: r1 = i; r2 = j; r3 = i + j - l; break;
+  }
 }
 
-// this lane's share of list L's entries [beg, end): entry beg + 8 wave + u
-// + 32 t for trips t; MB_UNROLL trips per iteration, their list loads issued
-// together (padding entries have value 0 and index 0, which is in range)
-constexpr int MB_UNROLL = 4;
-
+// This lane's share of list L, for wire c = lane & 7: the wave loads 64
+// consecutive entries (one per lane, coalesced) and hands them round by
+// round to its eight 8-lane groups (lane >> 3 picks the entry), so the eight
+// lanes of a group read one 64-byte row of a staged table (consecutive
+// wires).  Padding entries (past the end) have value 0 and index 0, which
+// is in range.
 template <int L>
 __device__ __forceinline__ double mb_list_sum(const uint32_t *ix, const double *vl, int beg, int end,
-                                              int wv, int u, const double *ne, const double *nh,
-                                              const double *vv) {
+                                              const double *s_e, const double *s_h, const double *vv) {
+  using T = MbList<L>;
+  const double *t1 = T::c1 ? s_h : s_e, *t2 = T::c2 ? s_h : s_e, *t3 = T::c3 ? s_h : s_e;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, u = lane >> 3, c = lane & 7;
   double acc = 0.0;
-  for (int e0 = beg + 8 * wv + u; e0 < end; e0 += 32 * MB_UNROLL) {
-    uint32_t pk[MB_UNROLL];
-    double v[MB_UNROLL];
+  for (int e0 = beg + wv * 64; e0 < end; e0 += MB_THREADS) {
+    const int e = e0 + lane;
+    const uint32_t pk_l = e < end ? ix[e] : 0u;
+    const double val_l = e < end ? vl[e] : 0.0;
 #pragma unroll
-    for (int t = 0; t < MB_UNROLL; ++t) {
-      const int e = e0 + 32 * t;
-      pk[t] = e < end ? ix[e] : 0u;
-      v[t] = e < end ? vl[e] : 0.0;
+    for (int r = 0; r < 8; ++r) {
+      const int src = 8 * r + u;
+      const uint32_t pk = (uint32_t)__shfl((int)pk_l, src, 64);
+      const double val = __shfl(val_l, src, 64);
+      int a, cc, r1, r2, r3;
+      mb_indices<L>(pk, a, cc, r1, r2, r3);
+      // VC is lower-triangular: V(a, c) = 0 for a < c
+      const double v2 = vv[(a >= cc ? a - cc : 0) * MB_WC + c];
+      const double w = (a >= cc) ? val : 0.0;
+      const double p1 = t1[r1 * MB_WC + c], p2 = t2[r2 * MB_WC + c], p3 = t3[r3 * MB_WC + c];
+      const double f1 = T::x1 ? 1.0 - p1 : p1;
+      const double f2 = T::x2 ? 1.0 - p2 : p2;
+      const double f3 = T::x3 ? 1.0 - p3 : p3;
+      acc += v2 * (f1 * f2 * f3) * w;
     }
-#pragma unroll
-    for (int t = 0; t < MB_UNROLL; ++t) acc += mb_term<L>(ne, nh, vv, pk[t]) * v[t];
   }
   return acc;
 }
 
-__global__ __launch_bounds__(256) void k_mb(psa_group g, int w0, int nw) {
+__global__ __launch_bounds__(MB_THREADS) void k_mb(psa_group g, int w0, int nw) {
   extern __shared__ double smem[];
-  __shared__ double red[4][MB_WC];
+  __shared__ double red[MB_THREADS / 64][MB_WC];
   const int N = g.N, W = g.Nw, slot = blockIdx.y, kb = blockIdx.x * MB_KG;
   const int wc0 = blockIdx.z * MB_WC;  // first wire of the chunk (relative to w0)
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int u = lane >> 3, c = lane & 7;
   const int ch2 = slot < 2 ? 1 : 2;  // second V^2 channel: Vee for electrons, Vhh for holes
   double *s_ne = smem, *s_nh = smem + N * MB_WC;
   double *s_v0 = smem + 2 * N * MB_WC, *s_v1 = smem + 3 * N * MB_WC;
   const double *occ = g.wmb + 4L * N * W, *vsq = g.wmb + 6L * N * W;
-  for (int idx = tid; idx < N * MB_WC; idx += blockDim.x) {
+  for (int idx = tid; idx < N * MB_WC; idx += MB_THREADS) {
     const int k = idx / MB_WC, cc = idx % MB_WC;
     const int w = w0 + min(wc0 + cc, nw - 1);  // idle wires of the last chunk repeat one wire
     s_ne[idx] = occ[(long)k * W + w];
@@ -369,7 +397,6 @@ __global__ __launch_bounds__(256) void k_mb(psa_group g, int w0, int nw) {
     s_v1[idx] = vsq[((long)ch2 * N + k) * W + w];
   }
   __syncthreads();
-  const double *ne = s_ne + c, *nh = s_nh + c;
   for (int kk = 0; kk < MB_KG; ++kk) {
     const int k = kb + kk;
     if (k >= N) break;
@@ -381,29 +408,31 @@ __global__ __launch_bounds__(256) void k_mb(psa_group g, int w0, int nw) {
       const int beg = g.mb_off[L][k], end = g.mb_off[L][k + 1];
       const uint32_t *ix = g.mb_idx[L];
       const double *vl = g.mb_val[L];
-      // V^2 channel: lists 0, 1, 4, 5 use Veh (5 with the index order
-      // swapped), 2, 3 Vee, 6, 7 Vhh
-      const double *vv = (half == 0 ? s_v0 : s_v1) + c;
+      // V^2 channel: lists 0, 1, 4, 5 use Veh, 2, 3 Vee, 6, 7 Vhh
+      const double *vv = half == 0 ? s_v0 : s_v1;
       switch (L) {
-        case 0: acc += mb_list_sum<0>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
-        case 1: acc += mb_list_sum<1>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
-        case 2: acc += mb_list_sum<2>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
-        case 3: acc += mb_list_sum<3>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
-        case 4: acc += mb_list_sum<4>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
-        case 5: acc += mb_list_sum<5>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
-        case 6: acc += mb_list_sum<6>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
-        default: acc += mb_list_sum<7>(ix, vl, beg, end, wv, u, ne, nh, vv); break;
+        case 0: acc += mb_list_sum<0>(ix, vl, beg, end, s_ne, s_nh, vv); break;
+        case 1: acc += mb_list_sum<1>(ix, vl, beg, end, s_ne, s_nh, vv); break;
+        case 2: acc += mb_list_sum<2>(ix, vl, beg, end, s_ne, s_nh, vv); break;
+        case 3: acc += mb_list_sum<3>(ix, vl, beg, end, s_ne, s_nh, vv); break;
+        case 4: acc += mb_list_sum<4>(ix, vl, beg, end, s_ne, s_nh, vv); break;
+        case 5: acc += mb_list_sum<5>(ix, vl, beg, end, s_ne, s_nh, vv); break;
+        case 6: acc += mb_list_sum<6>(ix, vl, beg, end, s_ne, s_nh, vv); break;
+        default: acc += mb_list_sum<7>(ix, vl, beg, end, s_ne, s_nh, vv); break;
       }
     }
-    // reduce over u (lane bits 3..5), then over the waves
+    // reduce over the eight entry groups of a wave (lane bits 3..5), then
+    // over the waves, in a fixed order
     acc += __shfl_xor(acc, 8, 64);
     acc += __shfl_xor(acc, 16, 64);
     acc += __shfl_xor(acc, 32, 64);
-    if (u == 0) red[wv][c] = acc;
+    if (lane < MB_WC) red[wv][lane] = acc;
     __syncthreads();
     if (tid < MB_WC && wc0 + tid < nw) {
-      const double s = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
-      g.wmb[(long)(w0 + wc0 + tid) * 4 * N + slot * N + k] = s;
+      double sum = 0.0;
+#pragma unroll
+      for (int u = 0; u < MB_THREADS / 64; ++u) sum += red[u][tid];
+      g.wmb[(long)(w0 + wc0 + tid) * 4 * N + slot * N + k] = sum;
     }
     __syncthreads();
   }
@@ -613,12 +642,28 @@ __device__ void fft_line(cd *x, const cd *tw, int N, int sign, int lane) {
 }
 
 // The same FFT with N known at compile time: the stage sequence, moduli
-// and strides are constants.
-template <int N, int Ns, int Rem, int SIGN>
+// and strides are constants, and each stage reads its own twiddle table
+// tw[TOFF + (r-1) Ns + k] = w^(r k N / (R Ns)), contiguous in k, so the
+// lanes of a wave read consecutive (or the same) slots.
+template <int N, int Ns, int Rem, int TOFF>
+__device__ __forceinline__ void build_stage_twiddles(cd *tw, const double *twid, int tid, int nthr) {
+  if constexpr (Rem > 1) {
+    constexpr int R = (Rem % 4 == 0) ? 4 : ((Rem % 2 == 0) ? 2 : 3);
+    constexpr int step = N / (Ns * R);
+    for (int i = tid; i < (R - 1) * Ns; i += nthr) {
+      const int r = i / Ns + 1, k = i % Ns;
+      tw[TOFF + i] = ld(twid, r * k * step);
+    }
+    build_stage_twiddles<N, Ns * R, Rem / R, TOFF + (R - 1) * Ns>(tw, twid, tid, nthr);
+  }
+}
+
+template <int N, int Ns, int Rem, int SIGN, int TOFF>
 __device__ __forceinline__ void fft_stages(cd *x, const cd *tw, int lane) {
   if constexpr (Rem > 1) {
     constexpr int R = (Rem % 4 == 0) ? 4 : ((Rem % 2 == 0) ? 2 : 3);
-    constexpr int M = N / R, step = N / (Ns * R), IT = (M + 63) / 64;
+    constexpr int M = N / R, IT = (M + 63) / 64;
+    const cd *tws = tw + TOFF;
     cd o[IT][R];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -628,9 +673,9 @@ __device__ __forceinline__ void fft_stages(cd *x, const cd *tw, int lane) {
         const cd a0 = x[fpad(j)];
         if constexpr (R == 4) {
           // first stage (Ns = 1): k = 0, all twiddles are 1
-          const cd a1 = Ns == 1 ? x[fpad(j + M)] : cmul(x[fpad(j + M)], tw_get(tw, k * step, SIGN));
-          const cd a2 = Ns == 1 ? x[fpad(j + 2 * M)] : cmul(x[fpad(j + 2 * M)], tw_get(tw, 2 * k * step, SIGN));
-          const cd a3 = Ns == 1 ? x[fpad(j + 3 * M)] : cmul(x[fpad(j + 3 * M)], tw_get(tw, 3 * k * step, SIGN));
+          const cd a1 = Ns == 1 ? x[fpad(j + M)] : cmul(x[fpad(j + M)], tw_get(tws, k, SIGN));
+          const cd a2 = Ns == 1 ? x[fpad(j + 2 * M)] : cmul(x[fpad(j + 2 * M)], tw_get(tws, Ns + k, SIGN));
+          const cd a3 = Ns == 1 ? x[fpad(j + 3 * M)] : cmul(x[fpad(j + 3 * M)], tw_get(tws, 2 * Ns + k, SIGN));
           const cd s02 = cadd(a0, a2), d02 = csub(a0, a2);
           const cd s13 = cadd(a1, a3), d13 = mul_is(csub(a1, a3), SIGN);
           o[it][0] = cadd(s02, s13);
@@ -638,14 +683,14 @@ __device__ __forceinline__ void fft_stages(cd *x, const cd *tw, int lane) {
           o[it][2] = csub(s02, s13);
           o[it][3] = csub(d02, d13);
         } else if constexpr (R == 2) {
-          const cd b = cmul(x[fpad(j + M)], tw_get(tw, k * step, SIGN));
+          const cd b = cmul(x[fpad(j + M)], tw_get(tws, k, SIGN));
           o[it][0] = cadd(a0, b);
           o[it][1] = csub(a0, b);
         } else {
           constexpr double c3 = -0.5, s3 = SIGN * 0.86602540378443864676;
           const cd w1 = {c3, s3}, w2 = {c3, -s3};
-          const cd b = cmul(x[fpad(j + M)], tw_get(tw, k * step, SIGN));
-          const cd c = cmul(x[fpad(j + 2 * M)], tw_get(tw, 2 * k * step, SIGN));
+          const cd b = cmul(x[fpad(j + M)], tw_get(tws, k, SIGN));
+          const cd c = cmul(x[fpad(j + 2 * M)], tw_get(tws, Ns + k, SIGN));
           o[it][0] = cadd(a0, cadd(b, c));
           o[it][1] = cadd(a0, cadd(cmul(b, w1), cmul(c, w2)));
           o[it][2] = cadd(a0, cadd(cmul(b, w2), cmul(c, w1)));
@@ -664,7 +709,7 @@ __device__ __forceinline__ void fft_stages(cd *x, const cd *tw, int lane) {
       }
     }
     wave_lds_sync();
-    fft_stages<N, Ns * R, Rem / R, SIGN>(x, tw, lane);
+    fft_stages<N, Ns * R, Rem / R, SIGN, TOFF + (R - 1) * Ns>(x, tw, lane);
   }
 }
 
@@ -674,9 +719,9 @@ __device__ __forceinline__ void fft_line_n(cd *x, const cd *tw, int N, int sign,
     fft_line(x, tw, N, sign, lane);
   } else {
     if (sign < 0)
-      fft_stages<NT, 1, NT, -1>(x, tw, lane);
+      fft_stages<NT, 1, NT, -1, 0>(x, tw, lane);
     else
-      fft_stages<NT, 1, NT, +1>(x, tw, lane);
+      fft_stages<NT, 1, NT, +1, 0>(x, tw, lane);
   }
 }
 
@@ -697,10 +742,14 @@ __global__ void k_tphase(psa_group g, int w0, double dt) {
 // writes the leapfrog average A = (A + X3) / 2 (SBEs.py:5704-5706) for
 // every element except the diagonals of C and D, which the e/h
 // renormalisation still changes (k_finish_diag averages those).
-// LDS per line: fpad(N) slots plus one, so the line stride is odd (in
-// 16-byte slots) and the column pass's staging accesses (line index
-// fastest) fall on different banks.
-__host__ __device__ inline int tr_line_stride(int N) { return N + N / 4 + 1; }
+// LDS per line: at least fpad(N) slots, rounded so that the line stride is
+// 16 / LINES modulo 16 slots (of 16 bytes): the column pass's staging
+// accesses (line index fastest, LINES lines x 16 / LINES rows per 16 lanes)
+// then fall on 16 different bank groups.
+__host__ __device__ inline int tr_line_stride(int N, int lines) {
+  const int need = N + N / 4, want = 16 / lines;
+  return need + ((want - need % 16) + 16) % 16;
+}
 
 // NT: N at compile time (0: taken from the group at run time); LINES lines
 // (waves) per block.
@@ -708,13 +757,17 @@ template <int NT, int LINES>
 __global__ __launch_bounds__(64 * LINES) void k_transport(psa_group g, int w0, int pass, int avg) {
   extern __shared__ double smem[];
   const int N = NT ? NT : g.N, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int LS = tr_line_stride(N);
+  const int LS = tr_line_stride(N, LINES);
   const int mat = blockIdx.y;  // 0 C, 1 D, 2 p
   const int w = w0 + blockIdx.z;
   const int line0 = blockIdx.x * LINES;
   cd *base = reinterpret_cast<cd *>(smem);
   cd *tw = base + LINES * LS;  // [N] twiddles
-  for (int j = tid; j < N; j += blockDim.x) tw[j] = ld(g.twid, j);
+  if constexpr (NT == 0) {
+    for (int j = tid; j < N; j += blockDim.x) tw[j] = ld(g.twid, j);
+  } else {
+    build_stage_twiddles<NT, 1, NT, 0>(tw, g.twid, tid, blockDim.x);
+  }
   const long o = mat_off(g, w);
   double *X = (mat == 0 ? g.CX : (mat == 1 ? g.DX : g.pX)) + o;
   double *A = (mat == 0 ? g.CA : (mat == 1 ? g.DA : g.pA)) + o;
@@ -1151,8 +1204,8 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
         attr_set = true;
       }
     }
-    hipLaunchKernelGGL(k_mb, dim3((g->N + MB_KG - 1) / MB_KG, 4, (nw + MB_WC - 1) / MB_WC), dim3(256), shm, s,
-                       *g, w0, nw);
+    hipLaunchKernelGGL(k_mb, dim3((g->N + MB_KG - 1) / MB_KG, 4, (nw + MB_WC - 1) / MB_WC), dim3(MB_THREADS),
+                       shm, s, *g, w0, nw);
     PSA_LAUNCH_CHECK("k_mb");
   }
   hipLaunchKernelGGL(k_relax, dim3(nw, (g->N + 63) / 64), dim3(256), 0, s, *g, w0, dt);
@@ -1166,7 +1219,7 @@ namespace {
 
 template <int NT, int LINES>
 int launch_transport_l(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
-  const size_t shm = ((size_t)LINES * tr_line_stride(g->N) + g->N) * sizeof(cd);  // lines + twiddles
+  const size_t shm = ((size_t)LINES * tr_line_stride(g->N, LINES) + g->N) * sizeof(cd);  // lines + twiddles
   if (shm > 65536) {
     static bool attr_set = false;  // per instantiation
     if (!attr_set) {
