@@ -1047,6 +1047,90 @@ __global__ void k_record(psa_group g, int w0, int nw, int xmid, double *trace, i
 }
 
 // ===========================================================================
+// Many-body scattering lists on the device (the tables of CalcMBArrays,
+// coulomb.py:367-499, as _MBCE_core/_MBCH_core read them, coulomb.py:956-1071):
+// entries (i, j, l) of the stored (N+1)^3 Gaussian table (stored index =
+// physical index + 1; index 0 planes are zero) whose weight is not zero,
+// CSR-grouped by the rate index they feed and ordered (i, j, l) within a
+// target.  Block = one target index; the threads walk the two free indices
+// in that order, 256 at a time, and a block-wide ballot scan keeps the order.
+//   list  kind  target     condition
+//   0     eh    i          j+l-i in [0,N), i >= l
+//   1     eh    l          j+l-i in [0,N), i >= l
+//   2     ee    i          i+j-l in [0,N), i >= l
+//   3     ee    l          i+j-l in [0,N), i >= l
+//   4     eh    j          j+l-i in [0,N), i >= l
+//   5     eh    j+l-i      j+l-i in [0,N), l >= i
+//   6     hh    i          i+j-l in [0,N), i >= l
+//   7     hh    l          i+j-l in [0,N), i >= l
+// ===========================================================================
+__device__ __forceinline__ double mb_weight(int kind, int i, int j, int l, int N, const double *Ee,
+                                            const double *Eh, double ge, double gh, double hbar) {
+  if (i < 1 || j < 1 || l < 1) return 0.0;
+  const int a = i - 1, b = j - 1, c = l - 1;
+  double Ed, hb;
+  int k30;
+  if (kind == 0) {  // eh
+    k30 = c + b - a;
+    if (k30 < 0 || k30 >= N) return 0.0;
+    hb = hbar * ((ge + gh) / 2.0);
+    Ed = Ee[a] + Eh[b] - Eh[k30] - Ee[c];
+  } else {
+    const double *E = kind == 1 ? Ee : Eh;
+    k30 = a + b - c;
+    if (k30 < 0 || k30 >= N) return 0.0;
+    hb = hbar * (kind == 1 ? ge : gh);
+    Ed = E[a] + E[b] - E[k30] - E[c];
+  }
+  if (a == c || b == k30) return 0.0;
+  const double x = Ed / hb;
+  return 2.0 * PI / hbar * (1.0 / (sqrt(PI) * hb) * exp(-(x * x)));
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_mb_list(int N, const double *Ee, const double *Eh, double ge,
+                                                 double gh, double hbar, const int *off, int *counts,
+                                                 uint32_t *idx, double *val) {
+  __shared__ int wtot[4];
+  const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int kind = (L == 2 || L == 3) ? 1 : ((L == 6 || L == 7) ? 2 : 0);
+  constexpr bool jli = (kind == 0);  // third index j+l-i (eh) or i+j-l (ee, hh)
+  const long NN = (long)N * N;
+  int run = off ? off[t] : 0;
+  for (long f0 = 0; f0 < NN; f0 += 256) {
+    const long f = f0 + tid;
+    const int x = (int)(f / N), y = (int)(f % N);
+    int i, j, l;
+    if (L == 0 || L == 2 || L == 6) { i = t; j = x; l = y; }
+    else if (L == 1 || L == 3 || L == 7) { l = t; i = x; j = y; }
+    else if (L == 4) { j = t; i = x; l = y; }
+    else { i = x; j = y; l = t + i - j; }
+    bool ok = f < NN && l >= 0 && l < N;
+    const int i3 = jli ? j + l - i : i + j - l;
+    ok = ok && i3 >= 0 && i3 < N && (L == 5 ? l >= i : i >= l);
+    const double v = ok ? mb_weight(kind, i, j, l, N, Ee, Eh, ge, gh, hbar) : 0.0;
+    ok = ok && v > 0.0;
+    const unsigned long long m = __ballot(ok);
+    const int pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wtot[wv] = __popcll(m);
+    __syncthreads();
+    int wbase = 0, tot = 0;
+    for (int u = 0; u < 4; ++u) {
+      if (u < wv) wbase += wtot[u];
+      tot += wtot[u];
+    }
+    if (off && ok) {
+      const int pos = run + wbase + pre;
+      idx[pos] = (uint32_t)i | ((uint32_t)j << 10) | ((uint32_t)l << 20);
+      val[pos] = v;
+    }
+    run += tot;
+    __syncthreads();
+  }
+  if (!off && tid == 0) counts[t] = run;
+}
+
+// ===========================================================================
 // host launchers
 // ===========================================================================
 namespace {
@@ -1150,6 +1234,26 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   // parts are added in FP64 in the epilogue
   const int prec = g->prec == PSA_PREC_FP64 ? PSA_PREC_FP64 : PSA_PREC_FP32;
   return launch_gemm(p, prec, g->N, 2 * g->N - 1, 3 * nw, s, "calc_h gemm");
+}
+
+int psa_mb_list(int32_t list, int32_t N, const double *Ee, const double *Eh, double ge, double gh,
+                double hbar, const int32_t *off, int32_t *counts, uint32_t *idx, double *val,
+                void *stream) {
+  if (list < 0 || list > 7 || N < 2 || N > 1023 || !Ee || !Eh)
+    return fail(PSA_E_INVALID, "psa_mb_list: bad arguments");
+  if (off ? (!idx || !val) : !counts) return fail(PSA_E_INVALID, "psa_mb_list: missing output");
+  const hipStream_t s = (hipStream_t)stream;
+  switch (list) {
+#define PSA_MBL(L)                                                                                \
+  case L:                                                                                         \
+    hipLaunchKernelGGL(k_mb_list<L>, dim3(N), dim3(256), 0, s, N, Ee, Eh, ge, gh, hbar, off, counts, \
+                       idx, val);                                                                 \
+    break;
+    PSA_MBL(0) PSA_MBL(1) PSA_MBL(2) PSA_MBL(3) PSA_MBL(4) PSA_MBL(5) PSA_MBL(6) PSA_MBL(7)
+#undef PSA_MBL
+  }
+  PSA_LAUNCH_CHECK("k_mb_list");
+  return PSA_OK;
 }
 
 int psa_lorentz_table(const psa_group *g, void *stream) {

@@ -40,7 +40,7 @@ class WireGroup:
     prec   -- "fp64" (reference-equivalent), "fp32" or "bf16x3"
     """
 
-    def __init__(self, prm, nw, rr, wire_ids=None, prec="fp64", device="cuda"):
+    def __init__(self, prm, nw, rr, wire_ids=None, prec="fp64", device="cuda", host_mb=False):
         self.lib = capi.load()
         if not torch.cuda.is_available():
             raise capi.PsaError("no GPU visible: the device path has no CPU fallback")
@@ -55,7 +55,7 @@ class WireGroup:
             # _preparation never fills Wee/Whh (SBEs.py:5173-5226): FreePot
             # adds zero in the reference; nothing to do here either.
             pass
-        self.grid, tb = tables.build_tables(prm, self.rr, self._vint)
+        self.grid, tb = tables.build_tables(prm, self.rr, self._vint, host_mb=host_mb)
         self.tab = tb
         g = self.grid
         N, Nr, Nx = g["N"], g["Nr"], len(self.rr)
@@ -78,11 +78,16 @@ class WireGroup:
         # built on the device once per group (psa_lorentz_table)
         od = bool(prm.flags.get("OffDiagDph"))
         self.t["lor"] = torch.zeros((2 * N * N * 2 * N) if od else 1, dtype=F64, device=d)
-        self.mb = []
-        for off, idx, val in tb["mb"]:
-            self.mb.append((T(off, torch.int32),
-                            T(idx.astype(np.int64).astype(np.int32) if idx.size else np.zeros(1, np.int32), torch.int32),
-                            T(val if val.size else np.zeros(1))))
+        if tb["mb"] is None:
+            self.mb = self._device_mb_lists()
+            self.mb_nnz = list(self.mb_sizes)
+        else:
+            self.mb = []
+            for off, idx, val in tb["mb"]:
+                self.mb.append((T(off, torch.int32),
+                                T(idx.astype(np.int64).astype(np.int32) if idx.size else np.zeros(1, np.int32), torch.int32),
+                                T(val if val.size else np.zeros(1))))
+            self.mb_nnz = [int(v.size) for _, _, v in tb["mb"]]
         nwz = self.nw
         z = lambda *s, dt=C128: torch.zeros(s, dtype=dt, device=d)  # noqa: E731
         self.s = {}
@@ -118,6 +123,38 @@ class WireGroup:
             capi.check(self.lib.psa_lorentz_table(ctypes.byref(self.S), self._stream()))
 
     # ------------------------------------------------------------------
+    def _device_mb_lists(self):
+        """The eight many-body lists built on the device (psa_mb_list):
+        count per target, prefix sum on the host, fill."""
+        g, prm, d = self.grid, self.prm, self.dev
+        N = len(g["Ee"])
+        Ee = torch.as_tensor(g["Ee"], dtype=F64, device=d)
+        Eh = torch.as_tensor(g["Eh"], dtype=F64, device=d)
+        st = self._stream()
+        out, sizes = [], []
+        for L in range(8):
+            cnt = torch.zeros(N, dtype=torch.int32, device=d)
+            capi.check(self.lib.psa_mb_list(L, N, Ee.data_ptr(), Eh.data_ptr(), prm.gam_e, prm.gam_h,
+                                            hbar, None, cnt.data_ptr(), None, None, st))
+            c = cnt.cpu().numpy().astype(np.int64)
+            off = np.zeros(N + 1, dtype=np.int64)
+            off[1:] = np.cumsum(c)
+            nnz = int(off[-1])
+            if nnz >= 2**31:
+                raise capi.PsaError("many-body list too long for 32-bit offsets")
+            offt = torch.as_tensor(off.astype(np.int32), device=d)
+            idx = torch.zeros(max(nnz, 1), dtype=torch.int32, device=d)
+            val = torch.zeros(max(nnz, 1), dtype=F64, device=d)
+            if nnz:
+                capi.check(self.lib.psa_mb_list(L, N, Ee.data_ptr(), Eh.data_ptr(), prm.gam_e, prm.gam_h,
+                                                hbar, offt.data_ptr(), None, idx.data_ptr(),
+                                                val.data_ptr(), st))
+            out.append((offt, idx, val))
+            sizes.append(nnz)
+        torch.cuda.synchronize()
+        self.mb_sizes = sizes
+        return out
+
     def _vint(self, Qr, r, ae, ah, Delta0, pref):
         d = self.dev
         Q = torch.as_tensor(Qr, dtype=F64, device=d)
@@ -140,8 +177,8 @@ class WireGroup:
         S.flags = prm.flag_bits()
         S.prec = capi.PREC[self.prec]
         S.gx0, S.gnx, S.bx0, S.bnx = tb["gx0"], tb["gnx"], tb["bx0"], tb["bnx"]
-        for i, (off, idx, val) in enumerate(tb["mb"]):
-            S.nnz_mb[i] = int(val.size)
+        for i, n in enumerate(self.mb_nnz):
+            S.nnz_mb[i] = int(n)
         S.hbar, S.e0, S.gap, S.L = hbar, e0, prm.gap, prm.L
         S.ehint, S.dcv, S.area, S.dkr = g["ehint"], g["dcv"], g["area"], g["dkr"]
         S.dr_r = g["r"][1] - g["r"][0]

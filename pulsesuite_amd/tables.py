@@ -349,7 +349,7 @@ def many_body_lists(N, Ee, Eh, ge, gh, rel_cut=0.0):
 # all per-N tables
 # ---------------------------------------------------------------------------
 
-def build_tables(prm, rr, vint_fn):
+def build_tables(prm, rr, vint_fn, host_mb=True):
     """Per-N tables for the device path.  ``rr`` is the propagation grid;
     ``vint_fn(Qr, r, ae, ah, Delta0, pref) -> (3, Nr)`` evaluates the
     unscreened Coulomb integrals (on the GPU)."""
@@ -449,7 +449,8 @@ def build_tables(prm, rr, vint_fn):
             raise NotImplementedError(
                 "LorentzDelta = 1 (Lorentzian many-body tables) is not supported "
                 "by the device path")
-        t["mb"] = many_body_lists(N, Ee, Eh, prm.gam_e, prm.gam_h)
+        # host_mb False: the caller builds the lists on the device
+        t["mb"] = many_body_lists(N, Ee, Eh, prm.gam_e, prm.gam_h) if host_mb else None
     else:
         z = np.zeros(N + 1, dtype=np.int32)
         t["mb"] = [(z, np.zeros(0, np.uint32), np.zeros(0))] * 8

@@ -306,3 +306,35 @@ def test_bf16x3_rhs_and_state_error_bounded():
     print("bf16x3 state errors", errs, "Pxx", rel(pb, pa))
     assert max(errs) < 1e-5
     assert rel(pb, pa) < 1e-5
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_many_body_lists_device_vs_host(N):
+    """psa_mb_list (device) against the host builder: same entries in the
+    same order, weights within 1e-13 (exp evaluated on the device); an entry
+    present in only one list may only be an underflow-edge weight < 1e-300."""
+    from pulsesuite_amd import tables
+    from pulsesuite_amd.engine import WireGroup
+    L = wire_length(N)
+    prm, _ = params_pair(L)
+    rr = np.linspace(-1.2 * L, 1.2 * L, 64)
+    dev = WireGroup(prm, 1, rr, host_mb=False)
+    g = dev.grid
+    host = tables.many_body_lists(N, g["Ee"], g["Eh"], prm.gam_e, prm.gam_h)
+    for li in range(8):
+        off_d = dev.mb[li][0].cpu().numpy()
+        n = dev.mb_nnz[li]
+        idx_d = dev.mb[li][1].cpu().numpy()[:n].astype(np.uint32)
+        val_d = dev.mb[li][2].cpu().numpy()[:n]
+        off_h, idx_h, val_h = host[li]
+        if np.array_equal(off_d, off_h) and np.array_equal(idx_d, idx_h):
+            assert n == 0 or rel(val_d, val_h) < 1e-13, li
+            continue
+        dd = dict(zip(idx_d.tolist(), val_d.tolist()))
+        hh = dict(zip(idx_h.tolist(), val_h.tolist()))
+        for k in set(dd) ^ set(hh):
+            assert max(dd.get(k, 0.0), hh.get(k, 0.0)) < 1e-300, (li, k)
+        common = sorted(set(dd) & set(hh))
+        a = np.array([dd[k] for k in common])
+        b = np.array([hh[k] for k in common])
+        assert rel(a, b) < 1e-13, li
