@@ -182,7 +182,7 @@ def cpu_baseline(wg, prm, rr, dt, budget_s=20.0):
                        f"NumPy oracle ({per_wire_step:.2f} s/wire-step), scaled to 64 wires")
 
 
-def serve_sample(prec, seed, nreq=200, rate=40.0, nks=(64, 128, 256), steps=(200, 1000),
+def serve_sample(prec, seed, nreq=150, rate=30.0, nks=(64, 128, 256, 384, 512), steps=(100, 500),
                  dt=10e-18, nx=1024, time_limit=90.0):
     """A bounded sample of the serving stream: Poisson arrivals at `rate`,
     Nk uniform in `nks`, E0 log-uniform in [1e5, 1e8] V/m, FWHM uniform in
@@ -231,7 +231,7 @@ def main():
     ap.add_argument("--modes", default="", help="reduced-precision modes to time, e.g. bf16x3 "
                     "(FP64 is always timed; default fp32,bf16x3)")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-requests", type=int, default=150)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only (the SBE wire batch is stepped on a single "

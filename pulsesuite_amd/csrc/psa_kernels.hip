@@ -1301,8 +1301,9 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
     if (shm > 65536) {
       static bool attr_set = false;
       if (!attr_set) {
-        int rc = hip_check(hipFuncSetAttribute((const void *)k_mb,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
+        // the largest table set (N = 512); k_mb also has a small static array
+        int rc = hip_check(hipFuncSetAttribute((const void *)k_mb, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)(4 * 512 * MB_WC * sizeof(double))),
                            "hipFuncSetAttribute(k_mb)");
         if (rc) return rc;
         attr_set = true;

@@ -338,3 +338,35 @@ def test_many_body_lists_device_vs_host(N):
         a = np.array([dd[k] for k in common])
         b = np.array([hh[k] for k in common])
         assert rel(a, b) < 1e-13, li
+
+
+@pytest.mark.parametrize("N", [384, 512])
+def test_transport_large_n(N):
+    """Transport at the serving sizes (radix 4/2/3 stages up to 512) against
+    fft2 -> phase -> ifft2; the ramp Y is the group's table (pinned against
+    the oracle at N = 48..192 above)."""
+    import torch
+    from types import SimpleNamespace
+    from pulsesuite_amd.engine import WireGroup
+    L = wire_length(N)
+    prm, _ = params_pair(L)
+    rr = np.linspace(-1.2 * L, 1.2 * L, 128)
+    print(f"[transport] N={N}", flush=True)
+    wg = WireGroup(prm, 2, rr)
+    E = 3e7 * np.cos(2 * np.pi * rr / (3 * L)) + 0j
+    wg.set_fields(E, 2 * E, 0 * E)
+    wg.prop2qw()
+    rng = np.random.default_rng(N)
+    X = {k: rng.standard_normal((2, N, N)) + 1j * rng.standard_normal((2, N, N)) for k in ("CX", "DX", "pX")}
+    for k, v in X.items():
+        wg.s[k].copy_(torch.as_tensor(v))
+    dt = 1e-14
+    wg.transport(dt)
+    torch.cuda.synchronize()
+    exq = wg.s["exq"].cpu().numpy()
+    t = SimpleNamespace(Ytr=wg.tab["ytr"])
+    for w in range(2):
+        eac = exq[w, 0, N - 1].real
+        for k, v in X.items():
+            ref = O.transport(t, v[w], prm.Edc, eac, dt)
+            assert rel(wg.s[k][w].cpu().numpy(), ref) < 1e-13, (k, w)
