@@ -690,10 +690,11 @@ struct Prop2qwProb {
       const int idx = tid + 256 * e;
       const int x = idx >> 4, k = idx & 15;
       const int col = n0 + x, xi = k0 + k;
-      const int cc = min(col, 3 * nw - 1);  // clamped, masked below
-      const int w = w0 + cc / 3, f = cc % 3;
-      const double raw = g.efield[(((long)f * g.Nw + w) * g.Nx + g.gx0 + min(xi, g.gnx - 1)) * 2];
-      const double v = (col < 3 * nw && xi < g.gnx) ? raw : 0.0;
+      double v = 0.0;
+      if (col < 3 * nw && xi < g.gnx) {
+        const int w = w0 + col / 3, f = col % 3;
+        v = g.efield[(((long)f * g.Nw + w) * g.Nx + g.gx0 + xi) * 2];
+      }
       s.set(e, k, x, (T)v, (T)0);
     }
   }
@@ -738,10 +739,11 @@ struct SrcProb {
       const int idx = tid + 256 * e;
       const int x = idx >> 4, k = idx & 15;
       const int col = n0 + x, si = k0 + k;
-      const int cc = min(col, 4 * nw - 1);  // clamped, masked below
-      const int w = w0 + (cc >> 2), f = cc & 3;
-      cd v = ld(g.src + (long)w * PSA_SRC_STRIDE(N, g.Nr) + PSA_SRC_DS(g.Nr), (long)f * Q + min(si, Q - 1));
-      if (!(col < 4 * nw && si < Q)) v = cd{0.0, 0.0};
+      cd v = {0.0, 0.0};
+      if (col < 4 * nw && si < Q) {
+        const int w = w0 + (col >> 2), f = col & 3;
+        v = ld(g.src + (long)w * PSA_SRC_STRIDE(N, g.Nr) + PSA_SRC_DS(g.Nr), (long)f * Q + si);
+      }
       s.set(e, k, x, (T)v.re, (T)v.im);
     }
   }
@@ -794,8 +796,8 @@ struct Qw2pProb {
       const int idx = tid + 256 * e;
       const int k = idx >> 6, x = idx & 63;
       const int b = m0 + x, r = k0 + k;
-      const double raw = g.m2[(long)min(r, g.Nr - 1) * g.bnx + min(b, g.bnx - 1)];  // clamped
-      const double v = (b < g.bnx && r < g.Nr) ? raw : 0.0;
+      double v = 0.0;
+      if (b < g.bnx && r < g.Nr) v = g.m2[(long)r * g.bnx + b];
       s.set(e, k, x, (T)v, (T)0);
     }
   }
@@ -808,10 +810,11 @@ struct Qw2pProb {
       const int idx = tid + 256 * e;
       const int x = idx >> 4, k = idx & 15;
       const int col = n0 + x, r = k0 + k;
-      const int cc = min(col, 5 * nw - 1);  // clamped, masked below
-      const int w = w0 + cc / 5, f = cc % 5;
-      const double raw = g.src[(long)w * PSA_SRC_STRIDE(N, Nr) + f * Nr + min(r, Nr - 1)];
-      const double v = (col < 5 * nw && r < Nr) ? raw : 0.0;
+      double v = 0.0;
+      if (col < 5 * nw && r < Nr) {
+        const int w = w0 + col / 5, f = col % 5;
+        v = g.src[(long)w * PSA_SRC_STRIDE(N, Nr) + f * Nr + r];
+      }
       s.set(e, k, x, (T)v, (T)0);
     }
   }
