@@ -8,6 +8,7 @@
 
 #include "psa_gemm.h"
 #include "psa_rhs_bf16.h"
+#include "psa_offdiag.h"
 
 using namespace psa;
 
@@ -1266,15 +1267,27 @@ int psa_lorentz_table(const psa_group *g, void *stream) {
 
 int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
+  // FP64, or FP32 MFMA in the reduced-precision modes on 2^100-scaled
+  // occupation columns (positive terms only: no cancellation)
+  const int prec = g->prec == PSA_PREC_FP64 ? PSA_PREC_FP64 : PSA_PREC_FP32;
+  const hipStream_t s = (hipStream_t)stream;
+  if (!g->lor) {
+    // Lorentzian generated in registers
+    if (g->N > odg::NMAX) return fail(PSA_E_UNSUPPORTED, "psa_offdiag_tables: N > 512 needs the lor table");
+    const dim3 grid((g->N + odg::ROWS - 1) / odg::ROWS, (2 * nw + odg::COLS - 1) / odg::COLS, 2 * (2 * g->N - 1));
+    if (prec == PSA_PREC_FP64)
+      hipLaunchKernelGGL(odg::k_offdiag<double>, grid, dim3(256), 0, s, *g, w0, nw, 0);
+    else
+      hipLaunchKernelGGL(odg::k_offdiag<float>, grid, dim3(256), 0, s, *g, w0, nw, 1);
+    PSA_LAUNCH_CHECK("k_offdiag");
+    return PSA_OK;
+  }
   OffdiagProb p;
   p.g = *g;
   p.w0 = w0;
   p.nw = nw;
-  // FP64, or FP32 MFMA in the reduced-precision modes on 2^100-scaled
-  // occupation columns (positive terms only: no cancellation)
-  const int prec = g->prec == PSA_PREC_FP64 ? PSA_PREC_FP64 : PSA_PREC_FP32;
   p.scaled = prec == PSA_PREC_FP32 ? 1 : 0;
-  return launch_gemm(p, prec, g->N, 2 * nw, 2 * (2 * g->N - 1), (hipStream_t)stream, "offdiag gemm");
+  return launch_gemm(p, prec, g->N, 2 * nw, 2 * (2 * g->N - 1), s, "offdiag gemm");
 }
 
 int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt, void *stream) {

@@ -1,0 +1,245 @@
+// Off-diagonal dephasing tables (CalcOffDiagDeph_E2/H2, dephasing.py:226-317)
+// as one MFMA kernel that generates its Lorentzian operand in registers.
+//
+// For fixed q and kind (0 = De, rows are electrons; 1 = Dh, rows are holes)
+//   D[q][k] = pi/hbar ( (1-n_k) R_A(k) + n_k R_B(k) ),
+//   R_s(k)  = sum_p L(q,k,p) V2(q) f_s(p),
+// with K running over the same-carrier p (p - q in [1, N)) and then the
+// other-carrier p (p + q in [0, N)), L the Lorentzian of the energy mismatch
+// and f_A = n[j](1-n[p]), f_B = (1-n[j]) n[p] (j = p -/+ q) the occupation
+// columns of every wire.  Columns are (wire, s) pairs.
+//
+// Block = 4 waves over 256 rows k x 64 columns of one (q, kind).  The
+// occupation operand B [K][64] is built cooperatively, 32 K indices at a
+// time, into LDS (16-byte units [k / E][col][E], E = 16 / sizeof(T), so a
+// lane's four K values of one MFMA group are one or two ds_read_b128), with
+// the next chunk's global loads in flight during the current chunk's MFMAs.
+// The Lorentzian operand A is computed per lane from the band energies: the
+// row energies stay in registers, the K-side energies sit in LDS.  Lane l
+// supplies A[k = l%16][K = 4(l/16) + j] to MFMA j of a 16-deep slab, and B
+// with the same K order, so the four MFMAs of a slab cover its 16 K values.
+//
+// FP64 computes the Lorentzian exactly as the reference expression; FP32
+// (the reduced-precision modes) forms the mismatch in FP64 and the rest in
+// FP32 on x = Ed / (hbar g), and runs on the 2^100-scaled occupation columns
+// (V^2 ~ 1e-42 J^2 is below the FP32 range), unscaled in the epilogue.
+#pragma once
+
+#include "psa_gemm.h"
+
+namespace psa {
+namespace odg {
+
+constexpr int ROWS = 256;   // rows per block: 4 waves x 64
+constexpr int COLS = 64;    // (wire, s) columns per block
+constexpr int KC = 32;      // K indices per LDS chunk
+constexpr int KMAX = 1024;  // K <= 2N - 1 for N <= 512
+constexpr int NMAX = 512;
+
+template <typename T>
+struct Unit;
+template <>
+struct Unit<float> {
+  typedef fv4 t;
+};
+template <>
+struct Unit<double> {
+  typedef double t __attribute__((ext_vector_type(2)));
+};
+
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw, int scaled) {
+  constexpr int E = 16 / sizeof(T);       // elements per 16-byte unit
+  constexpr int PER = KC * COLS / 256;    // B elements per thread per chunk (8)
+  typedef typename Unit<T>::t unit_t;
+  __shared__ double e1[KMAX], e2[KMAX];  // K-side energies: E[p -/+ q], E[p]
+  __shared__ unit_t bs[2][KC / E * COLS];
+
+  const int N = g.N, W = g.Nw, Q = 2 * N - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // heavy q first: z = 0, 1 -> q = 0; 2, 3 -> q = 1; 4, 5 -> q = -1; ...
+  const int pr = blockIdx.z >> 1, kind = blockIdx.z & 1;
+  const int q = (pr & 1) ? (pr + 1) >> 1 : -(pr >> 1), qi = q + N - 1;
+  const int aq = q < 0 ? -q : q;
+  const int col0 = blockIdx.y * COLS, row0 = blockIdx.x * ROWS;
+  const int rb = max(0, -q), re = min(N, N - q);  // rows with k + q in [0, N)
+  const int kb0 = max(0, q + 1), K0 = max(0, min(N, N + q) - kb0);
+  const int kb1 = rb;
+  // q = 0 is excluded (the occupation columns are zero there)
+  const int K = (q == 0 || row0 >= re || row0 + ROWS <= rb) ? 0 : K0 + (re - rb);
+  const int nch = (K + KC - 1) / KC;
+  const double *Es = kind == 0 ? g.Ee : g.Eh;  // carrier of the rows
+  const double *Eo = kind == 0 ? g.Eh : g.Ee;  // the other carrier
+
+  for (int kk = tid; kk < nch * KC; kk += 256) {
+    double a = 0.0, b = 0.0;
+    if (kk < K0) {
+      const int p = kb0 + kk;
+      a = Es[p - q];
+      b = Es[p];
+    } else if (kk < K) {
+      const int p = kb1 + kk - K0;
+      a = Eo[p + q];
+      b = Eo[p];
+    }
+    e1[kk] = a;
+    e2[kk] = b;
+  }
+
+  // column owned by this thread when building B
+  const int cB = tid & (COLS - 1), colg = col0 + cB;
+  const bool colok = colg < 2 * nw;
+  const int wB = w0 + min(colg >> 1, nw - 1), sB = colg & 1;
+  const double *occ = g.wmb + 9L * N * W;  // |n| of the checkout level [2][N][W]
+  const double *vsq = g.wmb + 6L * N * W;  // VC(d)^2 [3][N][W]
+  const double *nnS = occ + (kind == 0 ? 0L : (long)N * W) + wB;
+  const double *nnO = occ + (kind == 0 ? (long)N * W : 0L) + wB;
+  const double sc = scaled ? 0x1p100 : 1.0;
+  const int aqc = min(aq, N - 1);
+  const double v2S = colok ? sc * vsq[((long)(kind == 0 ? 1 : 2) * N + aqc) * W + wB] : 0.0;
+  const double v2O = colok ? sc * vsq[(long)aqc * W + wB] : 0.0;
+
+  double ra[PER], rc[PER];
+  // element e of this thread: unit (tid >> 6) + 4 (e / E), position e % E
+  auto issue = [&](int ch) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int kk = ch * KC + ((tid >> 6) + 4 * (e / E)) * E + e % E;
+      int p = 0, j = 0;
+      const double *nn = nnS;
+      if (kk < K0) {
+        p = kb0 + kk;
+        j = p - q;
+      } else if (kk < K) {
+        p = kb1 + kk - K0;
+        j = p + q;
+        nn = nnO;
+      }
+      ra[e] = nn[(long)j * W];
+      rc[e] = nn[(long)p * W];
+    }
+  };
+  auto commit = [&](int ch, unit_t *dst) {
+#pragma unroll
+    for (int u = 0; u < PER / E; ++u) {
+      unit_t val;
+#pragma unroll
+      for (int s = 0; s < E; ++s) {
+        const int e = u * E + s;
+        const int kk = ch * KC + ((tid >> 6) + 4 * u) * E + s;
+        const double a = ra[e], c = rc[e];
+        const double f = sB ? (1.0 - a) * c : a * (1.0 - c);
+        val[s] = (T)(kk < K ? (kk < K0 ? v2S : v2O) * f : 0.0);
+      }
+      dst[((tid >> 6) + 4 * u) * COLS + cB] = val;
+    }
+  };
+
+  using acc_t = typename Mfma<T>::acc;
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[i][c] = acc_t{0, 0, 0, 0};
+
+  const int g4 = lane >> 4, r16 = lane & 15;
+  const int wrow0 = row0 + 64 * wv;
+  const bool wact = K > 0 && wrow0 < re && wrow0 + 64 > rb;  // wave-uniform
+  double ekq[4], ek[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = wrow0 + 16 * i + r16;
+    ek[i] = Es[min(k, N - 1)];
+    ekq[i] = Es[min(max(k + q, 0), N - 1)];
+  }
+  const double hgS = g.hbar * (kind == 0 ? g.gam_eh : g.gam_e), hgO = g.hbar * g.gam_h;
+  // Lrtz(Ed, hg) = (hg / pi) / (Ed^2 + hg^2); FP32: (1 / (pi hg)) / (x^2 + 1), x = Ed / hg
+  const double c1S = hgS / PI, c2S = hgS * hgS, c1O = hgO / PI, c2O = hgO * hgO;
+  const float ihS = (float)(1.0 / hgS), ihO = (float)(1.0 / hgO);
+  const float f1S = (float)(1.0 / (PI * hgS)), f1O = (float)(1.0 / (PI * hgO));
+  double ud[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) ud[i] = ekq[i] - ek[i];
+
+  if (K > 0) {
+    issue(0);
+    commit(0, bs[0]);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+      const bool more = ch + 1 < nch;
+      if (more) issue(ch + 1);
+      if (wact) {
+        const unit_t *B = bs[ch & 1];
+#pragma unroll
+        for (int sl = 0; sl < KC / 16; ++sl) {
+          const int kl = 16 * sl + 4 * g4;  // this lane's first K index in the chunk
+          const int kk = ch * KC + kl;
+          // one 16-byte unit per column block holds E of the lane's K values;
+          // the Lorentzian column for each K value is made just before its MFMAs
+#pragma unroll
+          for (int u = 0; u < 4 / E; ++u) {
+            unit_t bu[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bu[c] = B[(kl / E + u) * COLS + 16 * c + r16];
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+              const int j = u * E + s;
+              const double d1 = e1[kk + j], d2 = e2[kk + j];
+              const bool same = kk + j < K0;
+              T a[4];
+              if constexpr (sizeof(T) == 8) {
+                const double c1 = same ? c1S : c1O, c2 = same ? c2S : c2O;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                  const double Ed = ekq[i] + d1 - d2 - ek[i];
+                  a[i] = c1 / (Ed * Ed + c2);
+                }
+              } else {
+                const float ih = same ? ihS : ihO, f1 = same ? f1S : f1O;
+                const double dv = d1 - d2;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                  const float x = (float)(ud[i] + dv) * ih;
+                  a[i] = f1 * __builtin_amdgcn_rcpf(fmaf(x, x, 1.0f));
+                }
+              }
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[i][c] = Mfma<T>::mma(a[i], (T)bu[c][s], acc[i][c]);
+            }
+          }
+        }
+      }
+      if (more) commit(ch + 1, bs[(ch + 1) & 1]);
+      __syncthreads();
+    }
+  }
+
+  // epilogue: lanes l, l^1 hold the A and B columns of one wire
+  const double *pbase = g.prep;
+  const double unscale = scaled ? 0x1p-100 : 1.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double rA = (double)acc[i][c][v] * unscale;
+        const double rBv = __shfl_xor(rA, 1, 64);
+        const int kr = wrow0 + 16 * i + Mfma<T>::row(g4, v);
+        const int cg = col0 + 16 * c + r16;
+        if ((cg & 1) || cg >= 2 * nw || kr >= N) continue;
+        const int w = w0 + (cg >> 1);
+        double D = 0.0;
+        if (K > 0 && kr >= rb && kr < re) {
+          const double *pw = pbase + (long)w * PSA_PREP_STRIDE(N);
+          const double nk = pw[(kind == 0 ? PSA_PREP_ANE(N) : PSA_PREP_ANH(N)) + kr];
+          D = ((1.0 - nk) * rA + nk * rBv) * PI / g.hbar;
+        }
+        g.dedh[(((long)w * 2 + kind) * N + kr) * Q + qi] = D;
+      }
+}
+
+}  // namespace odg
+}  // namespace psa

@@ -40,7 +40,8 @@ class WireGroup:
     prec   -- "fp64" (reference-equivalent), "fp32" or "bf16x3"
     """
 
-    def __init__(self, prm, nw, rr, wire_ids=None, prec="fp64", device="cuda", host_mb=False):
+    def __init__(self, prm, nw, rr, wire_ids=None, prec="fp64", device="cuda", host_mb=False,
+                 lor_table=False):
         self.lib = capi.load()
         if not torch.cuda.is_available():
             raise capi.PsaError("no GPU visible: the device path has no CPU fallback")
@@ -74,10 +75,13 @@ class WireGroup:
             "m2": T(tb["m2"].T if tb["bnx"] else np.zeros((Nr, 1))),
             "phi": T(tb["phi"], C128), "twid": T(tb["twid"], C128),
         }
-        # Lorentzian kernels of the off-diagonal dephasing: [2][N*N][2N],
-        # built on the device once per group (psa_lorentz_table)
+        # The off-diagonal dephasing kernel makes its Lorentzian operand in
+        # registers (N <= 512); the stored table [2][N*N][2N] (built once by
+        # psa_lorentz_table, 0.54 GB at N = 256) serves larger N, or on request.
         od = bool(prm.flags.get("OffDiagDph"))
-        self.t["lor"] = torch.zeros((2 * N * N * 2 * N) if od else 1, dtype=F64, device=d)
+        self.lor_table = od and (bool(lor_table) or N > 512)
+        if self.lor_table:
+            self.t["lor"] = torch.zeros(2 * N * N * 2 * N, dtype=F64, device=d)
         if tb["mb"] is None:
             self.mb = self._device_mb_lists()
             self.mb_nnz = list(self.mb_sizes)
@@ -119,7 +123,7 @@ class WireGroup:
         self.tune = [0, 0, 0, 0]  # launch choices (psa_group.tune), 0 = defaults
         self.init_state()
         self._sync_struct()
-        if od:
+        if self.lor_table:
             capi.check(self.lib.psa_lorentz_table(ctypes.byref(self.S), self._stream()))
 
     # ------------------------------------------------------------------

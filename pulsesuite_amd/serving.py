@@ -217,9 +217,11 @@ def slot_bytes(nk, nx, prec, maxsteps):
 
 
 def pool_bytes(nk):
-    """Shared per-pool tables (dominated by the Lorentzian table)."""
+    """Shared per-pool tables: the many-body lists, and the stored Lorentzian
+    table of the off-diagonal dephasing where N > 512 needs it."""
     N = nk
-    return 2 * N * N * 2 * N * 8 + 64 * N * N * N // 8
+    lor = 2 * N * N * 2 * N * 8 if N > 512 else 0
+    return lor + 64 * N * N * N // 8
 
 
 class Server:

@@ -148,6 +148,40 @@ def test_offdiag_tables(case64):
         assert rel(dd[w, 1], Dh[1:2 * N, :].T) < 1e-12
 
 
+@pytest.mark.parametrize("N,nw", [(256, 40), (384, 5)])
+def test_offdiag_register_lorentzian_vs_table(N, nw):
+    """The off-diagonal dephasing kernel that makes its Lorentzian operand in
+    registers against the GEMM over the stored Lorentzian table (the form
+    pinned against the oracle at N = 64 above, there on the same kernel
+    family), on one random state: FP64 within 1e-12; FP32 (the reduced
+    modes) within 1e-5 of FP64.  N = 384 covers two row groups, 40 wires a
+    partial column tile."""
+    import torch
+    from pulsesuite_amd.engine import WireGroup
+    L = wire_length(N)
+    prm, _ = params_pair(L)
+    rr = np.linspace(-1.2 * L, 1.2 * L, 64)
+    st = random_state(np.random.default_rng(N + nw), N, nw)
+    print(f"[offdiag] N={N} nw={nw}", flush=True)
+    out = {}
+    for tag, kw in (("table", dict(lor_table=True)), ("reg", {}), ("reg32", dict(prec="fp32"))):
+        wg = WireGroup(prm, nw, rr, **kw)
+        assert ("lor" in wg.t) == (tag == "table")
+        wg.set_state(*st)
+        wg.prep()
+        wg.offdiag()
+        torch.cuda.synchronize()
+        out[tag] = wg.s["dedh"].cpu().numpy()
+        del wg
+    ref = out["table"]
+    assert np.abs(ref).max() > 0
+    assert rel(out["reg"], ref) < 1e-12
+    for w in range(nw):
+        for kind in range(2):
+            assert rel(out["reg"][w, kind], ref[w, kind]) < 1e-12, (w, kind)
+            assert rel(out["reg32"][w, kind], ref[w, kind]) < 1e-5, (w, kind)
+
+
 def test_rhs_fp64(case64):
     import torch
     c = case64
