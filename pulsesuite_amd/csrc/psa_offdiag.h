@@ -54,6 +54,10 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
   typedef typename Unit<T>::t unit_t;
   __shared__ double e1[KMAX], e2[KMAX];  // K-side energies: E[p -/+ q], E[p]
   __shared__ unit_t bs[2][KC / E * COLS];
+  // per-term Lorentzian constants, read by term index in the inner loop (a
+  // register select of two constants would let the compiler move their
+  // divisions into the loop)
+  __shared__ double tcst[2][2];
 
   const int N = g.N, W = g.Nw, Q = 2 * N - 1;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -154,9 +158,16 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
   }
   const double hgS = g.hbar * (kind == 0 ? g.gam_eh : g.gam_e), hgO = g.hbar * g.gam_h;
   // Lrtz(Ed, hg) = (hg / pi) / (Ed^2 + hg^2); FP32: (1 / (pi hg)) / (x^2 + 1), x = Ed / hg
-  const double c1S = hgS / PI, c2S = hgS * hgS, c1O = hgO / PI, c2O = hgO * hgO;
-  const float ihS = (float)(1.0 / hgS), ihO = (float)(1.0 / hgO);
-  const float f1S = (float)(1.0 / (PI * hgS)), f1O = (float)(1.0 / (PI * hgO));
+  if (tid < 2) {
+    const double hg = tid == 0 ? hgS : hgO;
+    if constexpr (sizeof(T) == 8) {
+      tcst[tid][0] = hg / PI;
+      tcst[tid][1] = hg * hg;
+    } else {
+      tcst[tid][0] = 1.0 / hg;
+      tcst[tid][1] = 1.0 / (PI * hg);
+    }
+  }
   double ud[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) ud[i] = ekq[i] - ek[i];
@@ -185,17 +196,17 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
             for (int s = 0; s < E; ++s) {
               const int j = u * E + s;
               const double d1 = e1[kk + j], d2 = e2[kk + j];
-              const bool same = kk + j < K0;
+              const double *tc = tcst[kk + j < K0 ? 0 : 1];
               T a[4];
               if constexpr (sizeof(T) == 8) {
-                const double c1 = same ? c1S : c1O, c2 = same ? c2S : c2O;
+                const double c1 = tc[0], c2 = tc[1];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                   const double Ed = ekq[i] + d1 - d2 - ek[i];
                   a[i] = c1 / (Ed * Ed + c2);
                 }
               } else {
-                const float ih = same ? ihS : ihO, f1 = same ? f1S : f1O;
+                const float ih = (float)tc[0], f1 = (float)tc[1];
                 const double dv = d1 - d2;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -219,6 +230,7 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
   // epilogue: lanes l, l^1 hold the A and B columns of one wire
   const double *pbase = g.prep;
   const double unscale = scaled ? 0x1p-100 : 1.0;
+  const double pih = PI / g.hbar;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -235,7 +247,7 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
         if (K > 0 && kr >= rb && kr < re) {
           const double *pw = pbase + (long)w * PSA_PREP_STRIDE(N);
           const double nk = pw[(kind == 0 ? PSA_PREP_ANE(N) : PSA_PREP_ANH(N)) + kr];
-          D = ((1.0 - nk) * rA + nk * rBv) * PI / g.hbar;
+          D = ((1.0 - nk) * rA + nk * rBv) * pih;
         }
         g.dedh[(((long)w * 2 + kind) * N + kr) * Q + qi] = D;
       }

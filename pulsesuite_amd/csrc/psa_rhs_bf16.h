@@ -197,60 +197,95 @@ __device__ __forceinline__ bf16x8 bfrag_read(const BTile &t, int pl, int row, in
 }
 
 
-__device__ __forceinline__ fv4 mma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, fv4 c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
-  return c;
+__device__ __forceinline__ fv4 mfma_bf16(bf16x8 a, bf16x8 b, fv4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 // The 12 (or 6, for a band operand) split products of one K slab of one
-// term into this wave's 2x2 blocks of accumulators.
+// term into this wave's 2x2 blocks of accumulators.  The term's sign and
+// conjugations are put on the operands (exact bf16 sign flips):
+// A' = sg conj?(A), B' = conj?(B), then
+//   Re += Re A' Re B' - Im A' Im B',  Im += Re A' Im B' + Im A' Re B'.
+// Each split product (hi hi, hi lo, lo hi) is issued over all eight
+// accumulators before the next one, so dependent MFMAs are eight issues
+// apart; every accumulator still receives its products in the order
+// RR (or RI) hi hi, hi lo, lo hi, then II (or IR) hi hi, hi lo, lo hi.
 __device__ __forceinline__ void bslab(const BTile &sA, const BTile &sB, const BTerm bt,
                                       fv4 (&aR)[2][2], fv4 (&aI)[2][2], int wm, int wn, int kl,
                                       int xl) {
-  // signs of the four real products (conj flips the imaginary part)
-  const double cA = bt.ca ? -1.0 : 1.0, cB = bt.cb ? -1.0 : 1.0;
-  const bool nRR = bt.sg < 0, nII = -bt.sg * cA * cB < 0, nRI = bt.sg * cB < 0, nIR = bt.sg * cA < 0;
+  const bool nA = bt.sg < 0;
   const bool aRe0 = bt.sa < 0, bRe0 = bt.sb < 0;  // band operands are purely imaginary
-  bf16x8 A[2][4], B[2][4];
+  bf16x8 A[2][4], B[2][4], nAi[2][2];
 #pragma unroll
   for (int s = 0; s < 2; ++s)
 #pragma unroll
     for (int pl = 0; pl < 4; ++pl) {
-      A[s][pl] = bfrag_read(sA, pl, wm * 32 + s * 16 + xl, kl * 8);
-      B[s][pl] = bfrag_read(sB, pl, wn * 32 + s * 16 + xl, kl * 8);
+      const bf16x8 x = bfrag_read(sA, pl, wm * 32 + s * 16 + xl, kl * 8);
+      const bf16x8 y = bfrag_read(sB, pl, wn * 32 + s * 16 + xl, kl * 8);
+      A[s][pl] = (nA != (pl >= 2 && bt.ca)) ? bneg(x) : x;
+      B[s][pl] = (pl >= 2 && bt.cb) ? bneg(y) : y;
     }
-  if (!aRe0 && !bRe0) {  // Re A Re B
 #pragma unroll
-    for (int si = 0; si < 2; ++si) {
-      const bf16x8 h = nRR ? bneg(A[si][0]) : A[si][0], l = nRR ? bneg(A[si][1]) : A[si][1];
-#pragma unroll
-      for (int sj = 0; sj < 2; ++sj) aR[si][sj] = mma3(h, l, B[sj][0], B[sj][1], aR[si][sj]);
-    }
+  for (int s = 0; s < 2; ++s) {
+    nAi[s][0] = bneg(A[s][2]);
+    nAi[s][1] = bneg(A[s][3]);
   }
-  if (!aRe0) {  // Re A Im B
+  // split product p: A plane offset (p == 2: lo), B plane offset (p == 1: lo)
+  if (!aRe0 && !bRe0) {
 #pragma unroll
-    for (int si = 0; si < 2; ++si) {
-      const bf16x8 h = nRI ? bneg(A[si][0]) : A[si][0], l = nRI ? bneg(A[si][1]) : A[si][1];
+    for (int p = 0; p < 3; ++p) {
+      const int ha = p == 2, hb = p == 1;
 #pragma unroll
-      for (int sj = 0; sj < 2; ++sj) aI[si][sj] = mma3(h, l, B[sj][2], B[sj][3], aI[si][sj]);
+      for (int si = 0; si < 2; ++si)
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) {
+          aR[si][sj] = mfma_bf16(A[si][ha], B[sj][hb], aR[si][sj]);      // Re Re
+          aI[si][sj] = mfma_bf16(A[si][ha], B[sj][2 + hb], aI[si][sj]);  // Re Im
+        }
     }
-  }
-  {  // Im A Im B (subtracted)
 #pragma unroll
-    for (int si = 0; si < 2; ++si) {
-      const bf16x8 h = nII ? bneg(A[si][2]) : A[si][2], l = nII ? bneg(A[si][3]) : A[si][3];
+    for (int p = 0; p < 3; ++p) {
+      const int ha = p == 2, hb = p == 1;
 #pragma unroll
-      for (int sj = 0; sj < 2; ++sj) aR[si][sj] = mma3(h, l, B[sj][2], B[sj][3], aR[si][sj]);
+      for (int si = 0; si < 2; ++si)
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) {
+          aR[si][sj] = mfma_bf16(nAi[si][ha], B[sj][2 + hb], aR[si][sj]);  // -Im Im
+          aI[si][sj] = mfma_bf16(A[si][2 + ha], B[sj][hb], aI[si][sj]);    // Im Re
+        }
     }
-  }
-  if (!bRe0) {  // Im A Re B
+  } else if (!aRe0) {  // B is a band: Re A Im B, -Im A Im B
 #pragma unroll
-    for (int si = 0; si < 2; ++si) {
-      const bf16x8 h = nIR ? bneg(A[si][2]) : A[si][2], l = nIR ? bneg(A[si][3]) : A[si][3];
+    for (int p = 0; p < 3; ++p) {
+      const int ha = p == 2, hb = p == 1;
 #pragma unroll
-      for (int sj = 0; sj < 2; ++sj) aI[si][sj] = mma3(h, l, B[sj][0], B[sj][1], aI[si][sj]);
+      for (int si = 0; si < 2; ++si)
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) {
+          aI[si][sj] = mfma_bf16(A[si][ha], B[sj][2 + hb], aI[si][sj]);
+          aR[si][sj] = mfma_bf16(nAi[si][ha], B[sj][2 + hb], aR[si][sj]);
+        }
+    }
+  } else if (!bRe0) {  // A is a band: -Im A Im B, Im A Re B
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const int ha = p == 2, hb = p == 1;
+#pragma unroll
+      for (int si = 0; si < 2; ++si)
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) {
+          aR[si][sj] = mfma_bf16(nAi[si][ha], B[sj][2 + hb], aR[si][sj]);
+          aI[si][sj] = mfma_bf16(A[si][2 + ha], B[sj][hb], aI[si][sj]);
+        }
+    }
+  } else {  // both bands: -Im A Im B
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const int ha = p == 2, hb = p == 1;
+#pragma unroll
+      for (int si = 0; si < 2; ++si)
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) aR[si][sj] = mfma_bf16(nAi[si][ha], B[sj][2 + hb], aR[si][sj]);
     }
   }
 }
