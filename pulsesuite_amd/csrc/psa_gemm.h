@@ -402,6 +402,65 @@ struct RhsProb {
   }
 };
 
+// Epilogue of the exchange sums for output (a, dd) of Hamiltonian `which`
+// (0 = Heh, 1 = Hee, 2 = Hhh) of wire w: adds Meh, the Hartree (direct)
+// terms and the band energies and scatters to (k1, k2) = (a + d, a),
+// d = dd - (N - 1), in the transposed Hamiltonian buffers
+// (_CalcH_jit, SBEs.py:808-924).
+__device__ inline void calch_store(const psa_group &g, int w, int which, int a, int dd, double re,
+                                   double im) {
+  const int N = g.N;
+  if (a >= N || dd >= 2 * N - 1) return;
+  const int d = dd - (N - 1);
+  const int k1 = a + d, k2 = a;
+  if (k1 < 0 || k1 >= N) return;
+  const long o = mat_off(g, w);
+  const long ij = (long)k2 * N + k1;  // stored transposed: consecutive lanes, consecutive k1
+  const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+  const bool lf = (g.flags & PSA_F_LF) != 0;
+  double *hd = g.hdiag + (long)w * 4 * N;
+  if (which == 0) {
+    cd meh = {0.0, 0.0};
+    if (g.flags & PSA_F_OPTICS) {
+      const int wid = g.wire_id[w];
+      const double *eq = g.exq + (long)w * 3 * 2 * (2 * N - 1);
+      const cd ex = ld(eq, dd), ey = ld(eq, (2 * N - 1) + dd), ez = ld(eq, 2 * (2 * N - 1) + dd);
+      const double xcv = (k2 & 1) ? -g.dcv : g.dcv;  // Xcv0[ke,kh] = dcv (-1)^kh
+      const double yv = yw_of(wid), zs = sgn_pow(wid);
+      // -ehint*Xcv*Ex - ehint*Ycv*Ey*yw - ehint*Zcv*Ez*(-1)^w, Ycv = dcv, Zcv = -dcv
+      const double a1 = -g.ehint * xcv, a2 = -g.ehint * g.dcv, a3 = -g.ehint * (-g.dcv);
+      meh.re = a1 * ex.re + (a2 * ey.re) * yv + (a3 * ez.re) * zs;
+      meh.im = a1 * ex.im + (a2 * ey.im) * yv + (a3 * ez.im) * zs;
+    }
+    st(g.Heh + o, ij, cd{meh.re + re, meh.im + im});
+  } else {
+    const bool ee = (which == 1);
+    if (d == 0) {
+      const double e0 = ee ? g.Ee[a] + g.gap : g.Eh[a];
+      st(hd, (ee ? 0 : N) + a, cd{e0 - re, -im});
+      return;
+    }
+    double *Hout = (ee ? g.Heo : g.Hho) + o;
+    if (!lf) {
+      st(Hout, ij, cd{0.0, 0.0});
+      return;
+    }
+    const int ad = d > 0 ? d : -d;
+    const double *vc = pr + PSA_PREP_VC(N);
+    const double vself = vc[(ee ? 1 : 2) * N + ad], veh = vc[ad];
+    const cd dsC = ld(pr + PSA_PREP_DSC(N), d + N - 1);
+    const cd dsD = ld(pr + PSA_PREP_DSD(N), d + N - 1);
+    const cd dsCm = ld(pr + PSA_PREP_DSC(N), -d + N - 1);
+    const cd dsDm = ld(pr + PSA_PREP_DSD(N), -d + N - 1);
+    cd h;
+    if (ee)  // - exchange + Vee(qd) sum_k C[k,k+qd] - Veh(qd) sum_k D[k+qd,k]
+      h = cd{-re + vself * dsC.re - veh * dsDm.re, -im + vself * dsC.im - veh * dsDm.im};
+    else     // - exchange + Vhh(qd) sum_k D[k,k+qd] - Veh(qd) sum_k C[k+qd,k]
+      h = cd{-re + vself * dsD.re - veh * dsCm.re, -im + vself * dsD.im - veh * dsCm.im};
+    st(Hout, ij, h);
+  }
+}
+
 // ===========================================================================
 // Problem: Hartree-Fock exchange as a skewed Toeplitz product
 // (_CalcH_jit, SBEs.py:808-924).  For diagonal offset d = k1 - k2 and
@@ -464,57 +523,7 @@ struct CalcHProb {
   }
 
   __device__ void store(int b, int a, int dd, double re, double im) const {
-    const int N = g.N;
-    if (a >= N || dd >= 2 * N - 1) return;
-    const int d = dd - (N - 1);
-    const int k1 = a + d, k2 = a;
-    if (k1 < 0 || k1 >= N) return;
-    const int which = b % 3, w = w0 + b / 3;
-    const long o = mat_off(g, w);
-    const long ij = (long)k2 * N + k1;  // stored transposed: consecutive lanes, consecutive k1
-    const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-    const bool lf = (g.flags & PSA_F_LF) != 0;
-    double *hd = g.hdiag + (long)w * 4 * N;
-    if (which == 0) {
-      cd meh = {0.0, 0.0};
-      if (g.flags & PSA_F_OPTICS) {
-        const int wid = g.wire_id[w];
-        const double *eq = g.exq + (long)w * 3 * 2 * (2 * N - 1);
-        const cd ex = ld(eq, dd), ey = ld(eq, (2 * N - 1) + dd), ez = ld(eq, 2 * (2 * N - 1) + dd);
-        const double xcv = (k2 & 1) ? -g.dcv : g.dcv;  // Xcv0[ke,kh] = dcv (-1)^kh
-        const double yv = yw_of(wid), zs = sgn_pow(wid);
-        // -ehint*Xcv*Ex - ehint*Ycv*Ey*yw - ehint*Zcv*Ez*(-1)^w, Ycv = dcv, Zcv = -dcv
-        const double a1 = -g.ehint * xcv, a2 = -g.ehint * g.dcv, a3 = -g.ehint * (-g.dcv);
-        meh.re = a1 * ex.re + (a2 * ey.re) * yv + (a3 * ez.re) * zs;
-        meh.im = a1 * ex.im + (a2 * ey.im) * yv + (a3 * ez.im) * zs;
-      }
-      st(g.Heh + o, ij, cd{meh.re + re, meh.im + im});
-    } else {
-      const bool ee = (which == 1);
-      if (d == 0) {
-        const double e0 = ee ? g.Ee[a] + g.gap : g.Eh[a];
-        st(hd, (ee ? 0 : N) + a, cd{e0 - re, -im});
-        return;
-      }
-      double *Hout = (ee ? g.Heo : g.Hho) + o;
-      if (!lf) {
-        st(Hout, ij, cd{0.0, 0.0});
-        return;
-      }
-      const int ad = d > 0 ? d : -d;
-      const double *vc = pr + PSA_PREP_VC(N);
-      const double vself = vc[(ee ? 1 : 2) * N + ad], veh = vc[ad];
-      const cd dsC = ld(pr + PSA_PREP_DSC(N), d + N - 1);
-      const cd dsD = ld(pr + PSA_PREP_DSD(N), d + N - 1);
-      const cd dsCm = ld(pr + PSA_PREP_DSC(N), -d + N - 1);
-      const cd dsDm = ld(pr + PSA_PREP_DSD(N), -d + N - 1);
-      cd h;
-      if (ee)  // - exchange + Vee(qd) sum_k C[k,k+qd] - Veh(qd) sum_k D[k+qd,k]
-        h = cd{-re + vself * dsC.re - veh * dsDm.re, -im + vself * dsC.im - veh * dsDm.im};
-      else     // - exchange + Vhh(qd) sum_k D[k,k+qd] - Veh(qd) sum_k C[k+qd,k]
-        h = cd{-re + vself * dsD.re - veh * dsCm.re, -im + vself * dsD.im - veh * dsCm.im};
-      st(Hout, ij, h);
-    }
+    calch_store(g, w0 + b / 3, b % 3, a, dd, re, im);
   }
 };
 

@@ -9,6 +9,7 @@
 #include "psa_gemm.h"
 #include "psa_rhs_bf16.h"
 #include "psa_offdiag.h"
+#include "psa_calch.h"
 
 using namespace psa;
 
@@ -1227,13 +1228,22 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
     PSA_LAUNCH_CHECK("k_h_simple");
     return PSA_OK;
   }
-  CalcHProb p;
-  p.g = *g;
-  p.w0 = w0;
   // FP64, or FP32 MFMA in the reduced-precision modes (V ~ 1e-21 J and the
   // products stay inside the FP32 range); the Meh, Hartree and band-energy
   // parts are added in FP64 in the epilogue
   const int prec = g->prec == PSA_PREC_FP64 ? PSA_PREC_FP64 : PSA_PREC_FP32;
+  // FP32: the shifted-Toeplitz kernel; FP64 runs faster on the GEMM engine
+  // (the FP64 MFMA rate leaves the shifted kernel's larger row count behind)
+  if (prec == PSA_PREC_FP32 && g->N <= chx::NMAX) {
+    const int cols = (g->N + 31) / 32;  // column blocks: 32 positive and 32 non-positive d each
+    constexpr int R = chx::rows_per_block<float>();
+    hipLaunchKernelGGL(chx::k_calch<float>, dim3(cols, (g->N + R - 1) / R, 3 * nw), dim3(256), 0, s, *g, w0);
+    PSA_LAUNCH_CHECK("k_calch");
+    return PSA_OK;
+  }
+  CalcHProb p;
+  p.g = *g;
+  p.w0 = w0;
   return launch_gemm(p, prec, g->N, 2 * g->N - 1, 3 * nw, s, "calc_h gemm");
 }
 

@@ -129,6 +129,54 @@ def test_calc_h_stage(case64):
         assert rel(hd[w, 1], np.diag(Hhh)) < 1e-14
 
 
+@pytest.mark.parametrize("N", [256, 384])
+def test_calc_h_large_n(N):
+    """Exchange sums at the serving sizes against the oracle: FP64 (GEMM
+    engine) within 1e-12; FP32 (the reduced modes: shifted-Toeplitz kernel,
+    N = 384 with two row groups and a partial column block) within 1e-5."""
+    import torch
+    from pulsesuite_amd.engine import WireGroup
+    L = wire_length(N)
+    prm, oprm = params_pair(L)
+    nx = 96
+    rr = np.linspace(-1.2 * L, 1.2 * L, nx)
+    og = O.Grid(oprm)
+    ot = O.Tables(og)
+    nw = 2
+    st = random_state(np.random.default_rng(N), N, nw)
+    E = 3e7 * np.cos(2 * np.pi * rr / (3 * L)) + 0j
+    fields = (E, 2.0 * E, 0.3 * E)
+    oq = tuple(O.prop2qw(og, ot, rr, F) for F in fields)
+    ref = []
+    for w in range(nw):
+        C3, D3, p3 = st[3][:, :, w], st[4][:, :, w], st[5][:, :, w]
+        VC, _ = O.screened_arrays(og, ot, True, np.diag(C3), np.diag(D3))
+        Meh = O.calc_meh(og, ot, *oq, w + 1)
+        ref.append(O.calc_h(og, Meh, C3, D3, p3, VC, True, True))
+    off = lambda m: m - np.diag(np.diag(m))  # noqa: E731
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+        print(f"[calc_h] N={N} prec={prec}", flush=True)
+        wg = WireGroup(prm, nw, rr, prec=prec)
+        wg.set_state(*st)
+        wg.set_fields(*fields)
+        wg.prop2qw()
+        wg.prep()
+        wg.calc_h()
+        torch.cuda.synchronize()
+        Heh_g = wg.s["Heh"].cpu().numpy()
+        Heo = wg.s["Heo"].cpu().numpy()
+        Hho = wg.s["Hho"].cpu().numpy()
+        hd = wg.s["hdiag"].cpu().numpy()
+        for w in range(nw):
+            Heh, Hee, Hhh = ref[w]
+            assert rel(Heh_g[w].T, Heh) < tol, (prec, w)
+            assert rel(Heo[w].T, off(Hee)) < tol, (prec, w)
+            assert rel(Hho[w].T, off(Hhh)) < tol, (prec, w)
+            assert rel(hd[w, 0], np.diag(Hee)) < tol, (prec, w)
+            assert rel(hd[w, 1], np.diag(Hhh)) < tol, (prec, w)
+        del wg
+
+
 def test_offdiag_tables(case64):
     import torch
     c = case64
