@@ -138,30 +138,76 @@ __global__ __launch_bounds__(256) void k_prep_a(psa_group g, int w0) {
 //         dcv sum p[kh][kh+s], sum C3[k][k+s], sum conj(D3[k][k+s]).
 // Block (wire, 64 diagonals): lane -> diagonal (coalesced along the rows),
 // wave -> every fourth k; the four partial sums are added in wave order.
-__global__ __launch_bounds__(256) void k_diagsums(psa_group g, int w0, int mode) {
-  __shared__ cd red[4][4][64];
+// Block = 64 diagonals (one per lane) x DS_WAVES waves over the rows k (k
+// congruent to the wave index); each wave keeps four rows' loads in flight
+// (partial sums per row slot, combined in a fixed order), then the waves'
+// sums are added in a fixed order.
+constexpr int DS_WAVES = 8;
+__global__ __launch_bounds__(64 * DS_WAVES) void k_diagsums(psa_group g, int w0, int mode) {
+  __shared__ cd red[4][DS_WAVES][64];
   const int w = w0 + blockIdx.x, N = g.N, Q = 2 * N - 1;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int si = blockIdx.y * 64 + lane;
   const int s = si - (N - 1);
   const int k0 = s < 0 ? -s : 0, k1 = si < Q ? (s < 0 ? N : N - s) : 0;
+  // rows run uniformly over the block's lanes (the union of their ranges),
+  // so the lanes of one load read consecutive elements of one row
+  const int s0 = blockIdx.y * 64 - (N - 1);
+  const int kmin = max(0, -(s0 + 63)), kmax = N - max(0, s0);
   const long o = mat_off(g, w);
+  constexpr int ST = DS_WAVES;
   cd a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0}, a3 = {0, 0};
   if (mode == 0) {
     const double *C2 = g.CB + o, *D2 = g.DB + o;
-    for (int k = k0 + wv; k < k1; k += 4) {
-      a0 = cadd(a0, ld(C2, (long)k * N + k + s));
-      a1 = cadd(a1, ld(D2, (long)k * N + k + s));
+    cd pc[4] = {}, pd[4] = {};
+    for (int kb = kmin + wv; kb < kmax; kb += 4 * ST) {
+      cd vc_[4], vd_[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = min(max(kb + ST * u, k0), max(k1 - 1, k0));
+        const long ix = k1 > k0 ? (long)k * N + k + s : 0;
+        vc_[u] = ld(C2, ix);
+        vd_[u] = ld(D2, ix);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = kb + ST * u;
+        if (k >= k0 && k < k1) {
+          pc[u] = cadd(pc[u], vc_[u]);
+          pd[u] = cadd(pd[u], vd_[u]);
+        }
+      }
     }
+    a0 = cadd(cadd(pc[0], pc[1]), cadd(pc[2], pc[3]));
+    a1 = cadd(cadd(pd[0], pd[1]), cadd(pd[2], pd[3]));
   } else {
     const double *p3 = g.pX + o, *C3 = g.CX + o, *D3 = g.DX + o;
-    for (int k = k0 + wv; k < k1; k += 4) {
-      const cd pv = ld(p3, (long)k * N + k + s);
-      a1 = cadd(a1, pv);
-      a0 = (k & 1) ? csub(a0, pv) : cadd(a0, pv);
-      a2 = cadd(a2, ld(C3, (long)k * N + k + s));
-      a3 = cadd(a3, cconj(ld(D3, (long)k * N + k + s)));
+    cd q0[4] = {}, q1[4] = {}, q2[4] = {}, q3[4] = {};
+    for (int kb = kmin + wv; kb < kmax; kb += 4 * ST) {
+      cd vp[4], vc_[4], vd_[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = min(max(kb + ST * u, k0), max(k1 - 1, k0));
+        const long ix = k1 > k0 ? (long)k * N + k + s : 0;
+        vp[u] = ld(p3, ix);
+        vc_[u] = ld(C3, ix);
+        vd_[u] = ld(D3, ix);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = kb + ST * u;
+        if (k >= k0 && k < k1) {
+          q1[u] = cadd(q1[u], vp[u]);
+          q0[u] = (k & 1) ? csub(q0[u], vp[u]) : cadd(q0[u], vp[u]);
+          q2[u] = cadd(q2[u], vc_[u]);
+          q3[u] = cadd(q3[u], cconj(vd_[u]));
+        }
+      }
     }
+    a0 = cadd(cadd(q0[0], q0[1]), cadd(q0[2], q0[3]));
+    a1 = cadd(cadd(q1[0], q1[1]), cadd(q1[2], q1[3]));
+    a2 = cadd(cadd(q2[0], q2[1]), cadd(q2[2], q2[3]));
+    a3 = cadd(cadd(q3[0], q3[1]), cadd(q3[2], q3[3]));
   }
   red[0][wv][lane] = a0;
   red[1][wv][lane] = a1;
@@ -171,8 +217,11 @@ __global__ __launch_bounds__(256) void k_diagsums(psa_group g, int w0, int mode)
   if (wv != 0 || si >= Q) return;
   cd t[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-    t[j] = cadd(cadd(red[j][0][lane], red[j][1][lane]), cadd(red[j][2][lane], red[j][3][lane]));
+  for (int j = 0; j < 4; ++j) {
+    t[j] = red[j][0][lane];
+#pragma unroll
+    for (int u = 1; u < DS_WAVES; ++u) t[j] = cadd(t[j], red[j][u][lane]);
+  }
   if (mode == 0) {
     double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
     st(pr + PSA_PREP_DSC(N), si, t[0]);
@@ -185,7 +234,6 @@ __global__ __launch_bounds__(256) void k_diagsums(psa_group g, int w0, int mode)
     st(ds, 3 * Q + si, t[3]);
   }
 }
-
 __global__ __launch_bounds__(256) void k_prep_b(psa_group g, int w0) {
   const int w = w0 + blockIdx.x, N = g.N, lane = threadIdx.x & 63;
   const int k = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1211,7 +1259,7 @@ int psa_prep(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   const hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_prep_a, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_prep_a");
-  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 63) / 64), dim3(256), 0, s, *g, w0, 0);
+  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 63) / 64), dim3(64 * DS_WAVES), 0, s, *g, w0, 0);
   PSA_LAUNCH_CHECK("k_diagsums");
   hipLaunchKernelGGL(k_prep_b, dim3(nw, (g->N + 3) / 4), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_prep_b");
@@ -1425,7 +1473,7 @@ int psa_finish(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
 int psa_sources(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
   const hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 63) / 64), dim3(256), 0, s, *g, w0, 1);
+  hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 63) / 64), dim3(64 * DS_WAVES), 0, s, *g, w0, 1);
   PSA_LAUNCH_CHECK("k_diagsums");
   {
     SrcProb p;

@@ -53,10 +53,16 @@ __device__ __forceinline__ long spl_index(int N, int w, int slot, int pl, int ro
 // ---------------------------------------------------------------------------
 // k_split_bf16: block (64x64 tile of the stored buffer, matrix, wire).
 // Stored buffers: p, C, D as the matrix; Heh, Heo, Hho as its transpose.
+// Loads are lane-contiguous along the buffer's rows (two rows per thread
+// and pass); the bf16 planes go to an LDS tile of row pairs per column
+// (dword = rows 2r, 2r+1; column stride 33 dwords), from which both slots
+// are written with 16-byte stores of 8 consecutive k.  Groups of 8 k at or
+// past the padded width NK are not written; values past N are written as
+// zero (the padding).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_split_bf16(psa_group g, int w0) {
-  __shared__ uint16_t t[4][64][66];
-  const int N = g.N, mat = blockIdx.z % 6, w = w0 + blockIdx.z / 6;
+  __shared__ uint32_t t[4][64][33];  // [plane][column j][row pair]
+  const int N = g.N, NK = spl_nk(N), mat = blockIdx.z % 6, w = w0 + blockIdx.z / 6;
   const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64, tid = threadIdx.x;
   const long o = mat_off(g, w);
   const double *X = (mat == 0 ? g.pB : mat == 1 ? g.CB : mat == 2 ? g.DB
@@ -66,32 +72,68 @@ __global__ __launch_bounds__(256) void k_split_bf16(psa_group g, int w0) {
   uint16_t *spl = reinterpret_cast<uint16_t *>(g.spl);
   // slot receiving the buffer as stored, and the one receiving its transpose
   const int s_same = 2 * mat + (stored_t ? 1 : 0), s_tr = 2 * mat + (stored_t ? 0 : 1);
-#pragma unroll 4
-  for (int e = 0; e < 16; ++e) {
-    const int idx = tid + 256 * e;
-    const int i = idx >> 6, j = idx & 63;  // tile row, column
-    const int gi = r0 + i, gj = c0 + j;
-    uint16_t h[4] = {0, 0, 0, 0};
-    if (gi < N && gj < N) {
-      cd v = ld(X, (long)gi * N + gj);
-      if (zd && gi == gj) v = cd{0.0, 0.0};
-      split_bf16(v.re, h[0], h[1]);
-      split_bf16(v.im, h[2], h[3]);
+  {
+    const int j = tid & 63, gj = c0 + j;
+    cd v[8][2];
 #pragma unroll
-      for (int pl = 0; pl < 4; ++pl) spl[spl_index(N, w, s_same, pl, gi, gj)] = h[pl];
+    for (int e = 0; e < 8; ++e) {
+      const int rp = (tid >> 6) + 4 * e;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int gi = r0 + 2 * rp + r;
+        v[e][r] = ld(X, (long)min(gi, N - 1) * N + min(gj, N - 1));
+        if (gi >= N || gj >= N || (zd && gi == gj)) v[e][r] = cd{0.0, 0.0};
+      }
     }
 #pragma unroll
-    for (int pl = 0; pl < 4; ++pl) t[pl][i][j] = h[pl];
+    for (int e = 0; e < 8; ++e) {
+      const int rp = (tid >> 6) + 4 * e;
+      uint16_t h0[4], h1[4];
+      split_bf16(v[e][0].re, h0[0], h0[1]);
+      split_bf16(v[e][0].im, h0[2], h0[3]);
+      split_bf16(v[e][1].re, h1[0], h1[1]);
+      split_bf16(v[e][1].im, h1[2], h1[3]);
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) t[pl][j][rp] = (uint32_t)h0[pl] | ((uint32_t)h1[pl] << 16);
+    }
   }
   __syncthreads();
-#pragma unroll 4
-  for (int e = 0; e < 16; ++e) {
-    const int idx = tid + 256 * e;
-    const int j = idx >> 6, i = idx & 63;  // transposed: row j, column i
-    const int gi = r0 + i, gj = c0 + j;
-    if (gi < N && gj < N)
+  const int x = tid >> 2, kq = tid & 3;
+  // the buffer as stored: row i = x, k = columns 16 kq .. 16 kq + 15
+  {
+    const int gi = r0 + x;
+    if (gi < N) {
 #pragma unroll
-      for (int pl = 0; pl < 4; ++pl) spl[spl_index(N, w, s_tr, pl, gj, gi)] = t[pl][i][j];
+      for (int half = 0; half < 2; ++half) {
+        const int k0 = c0 + 16 * kq + 8 * half;
+        if (k0 >= NK) continue;
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+          uint16_t hv[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) hv[q] = (uint16_t)(t[pl][16 * kq + 8 * half + q][x >> 1] >> (16 * (x & 1)));
+          *reinterpret_cast<u32x4 *>(&spl[spl_index(N, w, s_same, pl, gi, k0)]) = __builtin_bit_cast(u32x4, hv);
+        }
+      }
+    }
+  }
+  // transposed: row j = x (a column of the buffer), k = rows 16 kq .. 16 kq + 15
+  {
+    const int gj = c0 + x;
+    if (gj < N) {
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int k0 = r0 + 16 * kq + 8 * half;
+        if (k0 >= NK) continue;
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+          u32x4 v;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v[q] = t[pl][x][8 * kq + 4 * half + q];
+          *reinterpret_cast<u32x4 *>(&spl[spl_index(N, w, s_tr, pl, gj, k0)]) = v;
+        }
+      }
+    }
   }
 }
 
