@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PSA_ABI_VERSION 1
+#define PSA_ABI_VERSION 2
 
 #define PSA_OK 0
 #define PSA_E_INVALID (-1)   /* bad argument / shape */
@@ -105,7 +105,7 @@ typedef struct psa_group {
                                          Heh[k1][k2] (resp. Hee, Hhh without their diagonal) */
   double *hdiag;                      /* [Nw][2][N] complex: diagonals of Hee, Hhh */
   double *prep;                       /* [Nw][PSA_PREP_STRIDE(N)] doubles, see PSA_PREP_* */
-  double *dedh;                       /* [Nw][2][2N-1][N] off-diagonal dephasing tables */
+  double *dedh;                       /* [Nw][2][N][2N-1] off-diagonal dephasing tables D[q][k] stored [k][q + N - 1] */
   double *wmb;                        /* 11 N Nw doubles: [Nw][4][N] many-body Win_e, Wout_e, Win_h, Wout_h;
                                          then, wire fastest: [2][N][Nw] |Re ne3|, |Re nh3| of the new level,
                                          [3][N][Nw] VC(d)^2 (eh, ee, hh), [2][N][Nw] |ne|, |nh| of the checkout level */
@@ -121,6 +121,10 @@ typedef struct psa_group {
                                          NR = 128 ceil(N/128), NK = 32 ceil(N/32); slot = 2 matrix +
                                          transposed (p, C_o, D_o, Heh, Hee_o, Hhh_o), planes
                                          re_hi, re_lo, im_hi, im_lo */
+  double *kscr;                       /* optional scratch for split-K partial sums of the source and
+                                         propagation-grid products (NULL: no split); kscr_len doubles,
+                                         16 x 2 x max(Nr x 4 Nw, (2N-1) x 3 Nw, bnx x 5 Nw) uses all splits */
+  int64_t kscr_len;
   const int32_t *wid_host;            /* [Nw] host copy of wire_id */
 } psa_group;
 

@@ -10,7 +10,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libpulsesuite_amd.so")
 
-PSA_ABI_VERSION = 1
+PSA_ABI_VERSION = 2
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 c_i32 = ctypes.c_int32
@@ -24,7 +24,7 @@ _TABS = ["kr", "Ee", "Eh", "vtab", "chi", "qeh", "EP", "HP", "EPT", "HPT", "xe",
          "iq", "homega", "hsq", "window", "rq", "ytr", "gmat", "m2", "phi", "twid", "lor"]
 _WIRE = ["wire_id", "pA", "pB", "pX", "CA", "CB", "CX", "DA", "DB", "DX", "Heh",
          "Heo", "Hho", "hdiag", "prep", "dedh", "wmb", "exq", "efield", "src",
-         "out", "obs", "tphase", "part", "spl", "wid_host"]
+         "out", "obs", "tphase", "part", "spl", "kscr"]
 
 
 class PsaGroup(ctypes.Structure):
@@ -35,7 +35,8 @@ class PsaGroup(ctypes.Structure):
                 + [(n, c_dbl) for n in _DBLS]
                 + [(n, c_ptr) for n in _TABS]
                 + [("mb_off", c_ptr * 8), ("mb_idx", c_ptr * 8), ("mb_val", c_ptr * 8)]
-                + [(n, c_ptr) for n in _WIRE])
+                + [(n, c_ptr) for n in _WIRE]
+                + [("kscr_len", ctypes.c_int64), ("wid_host", c_ptr)])
 
 
 _GP = ctypes.POINTER(PsaGroup)

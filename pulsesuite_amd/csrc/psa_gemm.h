@@ -229,6 +229,56 @@ __global__ __launch_bounds__(256) void gemm_kernel(P p) {
       }
 }
 
+// Split-K view of a single-batch problem: the batch index is the K split
+// (K values [b ks, (b+1) ks)); store() writes the split's raw accumulators
+// to scr[b][M][ncol][2], summed in a fixed order by splitk_reduce, which
+// applies the problem's own epilogue.  Every (split, row, column) is
+// written, zero where the split has no products.
+template <class P>
+struct SplitK {
+  static constexpr bool A_REAL = P::A_REAL, B_REAL = P::B_REAL;
+  P p;
+  psa_group g;
+  int ks, M, ncol;
+  double *scr;
+
+  __device__ int nterms(int) const { return p.nterms(0); }
+  __device__ bool active(int, int m0, int n0) const { return p.active(0, m0, n0); }
+  __device__ void krange(int t, int b, int m0, int n0, int &kb, int &ke) const {
+    p.krange(t, 0, m0, n0, kb, ke);
+    kb = max(kb, b * ks);
+    ke = min(ke, (b + 1) * ks);
+  }
+  template <typename T>
+  __device__ void fetchA(Frag<T> &s, int t, int, int m0, int k0) const {
+    p.fetchA(s, t, 0, m0, k0);
+  }
+  template <typename T>
+  __device__ void fetchB(Frag<T> &s, int t, int, int k0, int n0) const {
+    p.fetchB(s, t, 0, k0, n0);
+  }
+  __device__ void store(int b, int r, int c, double re, double im) const {
+    if (r >= M || c >= ncol) return;
+    double *o = scr + (((long)b * M + r) * ncol + c) * 2;
+    o[0] = re;
+    o[1] = im;
+  }
+};
+
+template <class P>
+__global__ __launch_bounds__(256) void splitk_reduce(P p, const double *scr, int S, int M, int ncol) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)M * ncol) return;
+  const int r = (int)(i / ncol), c = (int)(i % ncol);
+  double re = 0.0, im = 0.0;
+  for (int s = 0; s < S; ++s) {
+    const double *o = scr + (((long)s * M + r) * ncol + c) * 2;
+    re += o[0];
+    im += o[1];
+  }
+  p.store(0, r, c, re, im);
+}
+
 template <typename T>
 constexpr size_t gemm_lds_bytes() {
   return 4 * sizeof(Tile<T>);

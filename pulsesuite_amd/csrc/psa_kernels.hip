@@ -1208,6 +1208,40 @@ int launch_gemm(const P &p, int prec, int M, int Ncols, int batches, hipStream_t
   return launch_gemm_t<float>(p, grid, s, what);
 }
 
+// A single-batch FP64 product of M rows x ncol columns over K, split over
+// K when the tile grid alone would leave most of the GPU idle and the
+// group provides the scratch (g.kscr): about two workgroups per CU, at most
+// 16 splits, each at least one 16-deep slab.
+template <class P>
+int launch_gemm_splitk(const P &p, int M, int ncol, int K, hipStream_t s, const char *what) {
+  const psa_group &g = p.g;
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  const int tiles = ((M + GBM - 1) / GBM) * ((ncol + GBN - 1) / GBN);
+  int S = min(16, max(1, (2 * cus + tiles - 1) / tiles));
+  S = min(S, (K + GBK - 1) / GBK);
+  while (S > 1 && (long)S * M * ncol * 2 > g.kscr_len) --S;
+  if (S <= 1 || !g.kscr) return launch_gemm(p, PSA_PREC_FP64, M, ncol, 1, s, what);
+  SplitK<P> q;
+  q.p = p;
+  q.g = g;
+  q.ks = ((K + S - 1) / S + GBK - 1) / GBK * GBK;
+  q.M = M;
+  q.ncol = ncol;
+  q.scr = g.kscr;
+  S = (K + q.ks - 1) / q.ks;  // splits with work
+  if (int rc = launch_gemm(q, PSA_PREC_FP64, M, ncol, S, s, what)) return rc;
+  const long n = (long)M * ncol;
+  hipLaunchKernelGGL(splitk_reduce<P>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p,
+                     (const double *)g.kscr, S, M, ncol);
+  PSA_LAUNCH_CHECK("splitk_reduce");
+  return PSA_OK;
+}
+
 template <class K, size_t SHM, int TILE>
 int launch_rhs_bf16(const RhsProb &p, int nw, hipStream_t s, K kern) {
   static bool attr_set = false;  // per instantiation
@@ -1480,7 +1514,7 @@ int psa_sources(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
     p.g = *g;
     p.w0 = w0;
     p.nw = nw;
-    if (int rc = launch_gemm(p, PSA_PREC_FP64, g->Nr, 4 * nw, 1, s, "sources gemm")) return rc;
+    if (int rc = launch_gemm_splitk(p, g->Nr, 4 * nw, 2 * g->N - 1, s, "sources gemm")) return rc;
   }
   hipLaunchKernelGGL(k_src_norm, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_src_norm");
@@ -1494,7 +1528,7 @@ int psa_prop2qw(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   p.w0 = w0;
   p.nw = nw;
   // the field transform always runs in FP64 (it sets the drive amplitude)
-  return launch_gemm(p, PSA_PREC_FP64, 2 * g->N - 1, 3 * nw, 1, (hipStream_t)stream, "prop2qw gemm");
+  return launch_gemm_splitk(p, 2 * g->N - 1, 3 * nw, g->gnx, (hipStream_t)stream, "prop2qw gemm");
 }
 
 int psa_qw2prop(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
@@ -1505,7 +1539,7 @@ int psa_qw2prop(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
     p.g = *g;
     p.w0 = w0;
     p.nw = nw;
-    if (int rc = launch_gemm(p, PSA_PREC_FP64, g->bnx, 5 * nw, 1, s, "qw2prop gemm")) return rc;
+    if (int rc = launch_gemm_splitk(p, g->bnx, 5 * nw, g->Nr, s, "qw2prop gemm")) return rc;
   }
   hipLaunchKernelGGL(k_qw2prop_b, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_qw2prop_b");

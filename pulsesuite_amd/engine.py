@@ -113,6 +113,9 @@ class WireGroup:
         # on a zero-padded (128 ceil(N/128)) x (32 ceil(N/32)) grid, blocked
         nspl = nwz * 12 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) if prec == "bf16x3" else 1
         self.s["spl"] = torch.zeros(nspl, dtype=torch.int16, device=d)
+        # split-K partial sums of the source / propagation-grid products
+        self._kscr_len = 16 * 2 * max(Nr * 4 * nwz, (2 * N - 1) * 3 * nwz, max(tb["bnx"], 1) * 5 * nwz)
+        self.s["kscr"] = torch.empty(self._kscr_len, dtype=F64, device=d)
         if wire_ids is None:
             wire_ids = np.arange(1, nwz + 1)
         self.wire_ids = np.asarray(wire_ids, dtype=np.int32)
@@ -199,6 +202,7 @@ class WireGroup:
             S.mb_val[i] = val.data_ptr()
         for k, v in self.s.items():
             setattr(S, k, v.data_ptr())
+        S.kscr_len = self._kscr_len
         S.wid_host = self._wid_host.ctypes.data
         self.S = S
 
