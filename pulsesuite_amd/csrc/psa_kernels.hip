@@ -253,6 +253,8 @@ __global__ __launch_bounds__(256) void k_prep_b(psa_group g, int w0) {
               *k1mq = g.didx + 4 * NN, *k1 = g.didx + 5 * NN;  // stored [k][q]
     const double ph = PI / g.hbar;
     double ge = 0.0, gh = 0.0;
+    // unrolled so that the index loads of several q are in flight together
+#pragma unroll 4
     for (int q = lane; q < N; q += 64) {
       const long qk = (long)k * N + q;
       const int iq = g.iq[q];
@@ -521,9 +523,12 @@ __global__ void k_wire_tables(psa_group g, int w0) {
 // lane -> k, wave -> every fourth i of the phonon sums (EP[k][i] read as
 // EPT[i][k], so both sums are coalesced along k); partials are added in
 // wave order and wave 0 applies the update.
-__global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
+// Block = 64 rates k (one per lane) x RL_WAVES waves over the phonon sums
+// (i congruent to the wave index), added in a fixed order.
+constexpr int RL_WAVES = 16;
+__global__ __launch_bounds__(64 * RL_WAVES) void k_relax(psa_group g, int w0, double dt) {
   __shared__ double s_ner[1024], s_nhr[1024];
-  __shared__ double part[4][4][64];
+  __shared__ double part[4][RL_WAVES][64];
   const int w = w0 + blockIdx.x, N = g.N, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6, k = blockIdx.y * 64 + lane;
   const long o = mat_off(g, w);
@@ -541,12 +546,12 @@ __global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
   double a = 0, b = 0, c = 0, d = 0;
   if (phon && k < N) {
     // Vep[k][i] = VC[k,i,1] / E1D[k,i] * Vscale, VC lower-triangular
-    for (int i = wv; i <= k; i += 4) {
+    for (int i = wv; i <= k; i += RL_WAVES) {
       const double vep = vc[N + k - i] / E1D[k - i] * g.Vscale;
       a += vep * s_ner[i] * g.EPT[(long)i * N + k];
       b += vep * (1.0 - s_ner[i]) * g.EP[(long)i * N + k];
     }
-    for (int i = k + wv; i < N; i += 4) {
+    for (int i = k + wv; i < N; i += RL_WAVES) {
       const double vhp = vc[2 * N + i - k] / E1D[i - k] * g.Vscale;
       c += vhp * s_nhr[i] * g.HPT[(long)i * N + k];
       d += vhp * (1.0 - s_nhr[i]) * g.HP[(long)i * N + k];
@@ -566,10 +571,17 @@ __global__ __launch_bounds__(256) void k_relax(psa_group g, int w0, double dt) {
     woh = mb[3 * N + k];
   }
   if (phon) {
-    wie += (part[0][0][lane] + part[0][1][lane]) + (part[0][2][lane] + part[0][3][lane]);
-    woe += (part[1][0][lane] + part[1][1][lane]) + (part[1][2][lane] + part[1][3][lane]);
-    wih += (part[2][0][lane] + part[2][1][lane]) + (part[2][2][lane] + part[2][3][lane]);
-    woh += (part[3][0][lane] + part[3][1][lane]) + (part[3][2][lane] + part[3][3][lane]);
+    double t[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      t[j] = part[j][0][lane];
+#pragma unroll
+      for (int u = 1; u < RL_WAVES; ++u) t[j] += part[j][u][lane];
+    }
+    wie += t[0];
+    woe += t[1];
+    wih += t[2];
+    woh += t[3];
   }
   cd ne = ld(C3, (long)k * N + k), nh = ld(D3, (long)k * N + k);
   if (ehs || phon) {
@@ -915,7 +927,7 @@ __global__ __launch_bounds__(256) void k_average(psa_group g, int w0, int nw) {
 // ===========================================================================
 __global__ __launch_bounds__(256) void k_src_norm(psa_group g, int w0) {
   __shared__ double sh[8];
-  __shared__ double s_ne2[1024], s_nh2[1024], ecE[1024], ecH[1024];
+  __shared__ double s_ne2[1024], s_nh2[1024], ecE[1024], ecH[1024], s_vee[1024], s_vhh[1024];
   const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, tid = threadIdx.x;
   const long o = mat_off(g, w);
   double *src = g.src + (long)w * PSA_SRC_STRIDE(N, Nr);
@@ -970,17 +982,20 @@ __global__ __launch_bounds__(256) void k_src_norm(psa_group g, int w0) {
   for (int k = tid; k < N; k += blockDim.x) {
     s_ne2[k] = cabs_(ld(g.CA + o, (long)k * N + k));
     s_nh2[k] = cabs_(ld(g.DA + o, (long)k * N + k));
+    s_vee[k] = vc[N + k];
+    s_vhh[k] = vc[2 * N + k];
   }
   __syncthreads();
+  const double vee0 = s_vee[0], vhh0 = s_vhh[0];
   for (int k = tid; k < N; k += blockDim.x) {
     double se = 0, shh = 0;
     for (int i = 0; i <= k; ++i) {  // V[k][i] nonzero for i <= k (EkReNorm, dcfield.py:338-345)
-      se += s_ne2[i] * (vc[N] - vc[N + k - i]);
-      shh += s_nh2[i] * (vc[2 * N] - vc[2 * N + k - i]);
+      se += s_ne2[i] * (vee0 - s_vee[k - i]);
+      shh += s_nh2[i] * (vhh0 - s_vhh[k - i]);
     }
     for (int i = k + 1; i < N; ++i) {
-      se += s_ne2[i] * vc[N];
-      shh += s_nh2[i] * vc[2 * N];
+      se += s_ne2[i] * vee0;
+      shh += s_nh2[i] * vhh0;
     }
     ecE[k] = g.Ee[k] + se / 2.0;
     ecH[k] = g.Eh[k] + shh / 2.0;
@@ -1418,7 +1433,7 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
                        shm, s, *g, w0, nw);
     PSA_LAUNCH_CHECK("k_mb");
   }
-  hipLaunchKernelGGL(k_relax, dim3(nw, (g->N + 63) / 64), dim3(256), 0, s, *g, w0, dt);
+  hipLaunchKernelGGL(k_relax, dim3(nw, (g->N + 63) / 64), dim3(64 * RL_WAVES), 0, s, *g, w0, dt);
   PSA_LAUNCH_CHECK("k_relax");
   return PSA_OK;
 }
