@@ -871,6 +871,252 @@ __global__ __launch_bounds__(64 * LINES) void k_transport(psa_group g, int w0, i
   }
 }
 
+// ---------------------------------------------------------------------------
+// Transport with N known at compile time, stages fused across the FFT pair.
+// The forward FFT takes the radices 4 (while 4 divides the rest), then 2,
+// then 3; the inverse takes the same radices in reverse order.  The forward
+// FFT's last stage leaves lane j holding the elements j + r N/R, which are
+// exactly the inputs of the inverse FFT's first stage (radix R, no
+// twiddles) in the same lane, so the phase multiply and that stage run in
+// registers between the two.  In the row pass the first forward stage reads
+// its inputs straight from HBM and the last inverse stage writes straight
+// back (its outputs j + r N/R are contiguous across lanes); the column pass
+// stages its lines through LDS as before.  Each stage has its own twiddle
+// table (w^(r k N/(R Ns)), contiguous in k), forward and inverse apart.
+// ---------------------------------------------------------------------------
+struct RadixPlan {
+  int S;
+  int R[12];
+};
+__host__ __device__ constexpr RadixPlan radix_plan(int N) {
+  RadixPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int rem = N;
+  while (rem > 1) {
+    const int r = rem % 4 == 0 ? 4 : (rem % 2 == 0 ? 2 : 3);
+    p.R[p.S++] = r;
+    rem /= r;
+  }
+  return p;
+}
+// radix of stage s of direction dir (0 forward, 1 inverse)
+__host__ __device__ constexpr int plan_r(int N, int dir, int s) {
+  return dir == 0 ? radix_plan(N).R[s] : radix_plan(N).R[radix_plan(N).S - 1 - s];
+}
+__host__ __device__ constexpr int plan_ns(int N, int dir, int s) {
+  int ns = 1;
+  for (int t = 0; t < s; ++t) ns *= plan_r(N, dir, t);
+  return ns;
+}
+// twiddle slots before stage s of direction dir (forward tables first)
+__host__ __device__ constexpr int plan_toff(int N, int dir, int s) {
+  int off = 0;
+  if (dir == 1)
+    for (int t = 0; t < radix_plan(N).S; ++t) off += (plan_r(N, 0, t) - 1) * plan_ns(N, 0, t);
+  for (int t = 0; t < s; ++t) off += (plan_r(N, dir, t) - 1) * plan_ns(N, dir, t);
+  return off;
+}
+__host__ __device__ constexpr int plan_twiddles(int N) { return plan_toff(N, 1, radix_plan(N).S); }
+
+template <int N, int DIR, int s>
+__device__ __forceinline__ void plan_build_twiddles(cd *tw, const double *twid, int tid, int nthr) {
+  if constexpr (s < radix_plan(N).S) {
+    constexpr int R = plan_r(N, DIR, s), Ns = plan_ns(N, DIR, s), TOFF = plan_toff(N, DIR, s);
+    constexpr int step = N / (R * Ns);
+    for (int i = tid; i < (R - 1) * Ns; i += nthr) {
+      const int r = i / Ns + 1, k = i % Ns;
+      tw[TOFF + i] = ld(twid, r * k * step);
+    }
+    plan_build_twiddles<N, DIR, s + 1>(tw, twid, tid, nthr);
+  }
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_small(const cd (&a)[R], cd (&o)[R]) {
+  if constexpr (R == 4) {
+    const cd s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+    const cd s13 = cadd(a[1], a[3]), d13 = mul_is(csub(a[1], a[3]), SIGN);
+    o[0] = cadd(s02, s13);
+    o[1] = cadd(d02, d13);
+    o[2] = csub(s02, s13);
+    o[3] = csub(d02, d13);
+  } else if constexpr (R == 2) {
+    o[0] = cadd(a[0], a[1]);
+    o[1] = csub(a[0], a[1]);
+  } else {
+    constexpr double c3 = -0.5, s3 = SIGN * 0.86602540378443864676;
+    const cd w1 = {c3, s3}, w2 = {c3, -s3};
+    o[0] = cadd(a[0], cadd(a[1], a[2]));
+    o[1] = cadd(a[0], cadd(cmul(a[1], w1), cmul(a[2], w2)));
+    o[2] = cadd(a[0], cadd(cmul(a[1], w2), cmul(a[2], w1)));
+  }
+}
+
+// Inputs of stage s for this lane's butterflies, from the LDS line x.
+template <int N, int DIR, int s, int IT, int R>
+__device__ __forceinline__ void stage_read(const cd *x, const cd *tw, int lane, cd (&a)[IT][R]) {
+  constexpr int Ns = plan_ns(N, DIR, s), M = N / R, TOFF = plan_toff(N, DIR, s);
+  constexpr int SIGN = DIR == 0 ? -1 : +1;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int j = lane + 64 * it;
+    if (M % 64 == 0 || j < M) {
+      const int k = j % Ns;
+      a[it][0] = x[fpad(j)];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const cd v = x[fpad(j + r * M)];
+        a[it][r] = Ns == 1 ? v : cmul(v, tw_get(tw + TOFF, (r - 1) * Ns + k, SIGN));
+      }
+    }
+  }
+}
+
+// Outputs of stage s to the LDS line x (positions (j - k) R + k + r Ns).
+template <int N, int DIR, int s, int IT, int R>
+__device__ __forceinline__ void stage_write(cd *x, int lane, const cd (&o)[IT][R]) {
+  constexpr int Ns = plan_ns(N, DIR, s), M = N / R;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int j = lane + 64 * it;
+    if (M % 64 == 0 || j < M) {
+      const int k = j % Ns, base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[fpad(base + r * Ns)] = o[it][r];
+    }
+  }
+}
+
+// Stages s .. e-1 of direction DIR, LDS to LDS.
+template <int N, int DIR, int s, int e>
+__device__ __forceinline__ void stages_lds(cd *x, const cd *tw, int lane) {
+  if constexpr (s < e) {
+    constexpr int R = plan_r(N, DIR, s), IT = (N / R + 63) / 64;
+    constexpr int SIGN = DIR == 0 ? -1 : +1;
+    cd a[IT][R], o[IT][R];
+    stage_read<N, DIR, s, IT, R>(x, tw, lane, a);
+#pragma unroll
+    for (int it = 0; it < IT; ++it) dft_small<R, SIGN>(a[it], o[it]);
+    wave_lds_sync();
+    stage_write<N, DIR, s, IT, R>(x, lane, o);
+    wave_lds_sync();
+    stages_lds<N, DIR, s + 1, e>(x, tw, lane);
+  }
+}
+
+// Forward last stage, phase multiply, inverse first stage, in registers;
+// ph holds this lane's phases at the positions j + r N/R.
+template <int N, int IT, int R>
+__device__ __forceinline__ void fused_middle(cd *x, const cd *tw, int lane, const cd (&ph)[IT][R]) {
+  constexpr int S = radix_plan(N).S;
+  cd a[IT][R], o[IT][R];
+  stage_read<N, 0, S - 1, IT, R>(x, tw, lane, a);
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    dft_small<R, -1>(a[it], o[it]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[it][r] = cmul(o[it][r], ph[it][r]);
+    dft_small<R, +1>(a[it], o[it]);
+  }
+  wave_lds_sync();
+  stage_write<N, 1, 0, IT, R>(x, lane, o);
+  wave_lds_sync();
+}
+
+template <int NT, int LINES>
+__global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0, int pass, int avg) {
+  extern __shared__ double smem[];
+  constexpr int N = NT, S = radix_plan(N).S;
+  constexpr int RL = plan_r(N, 0, S - 1), ITL = (N / RL + 63) / 64;  // the fused radix
+  constexpr int R0 = plan_r(N, 0, 0), IT0 = (N / R0 + 63) / 64;      // first forward stage
+  constexpr int RE = plan_r(N, 1, S - 1), ITE = (N / RE + 63) / 64;  // last inverse stage
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int LS = tr_line_stride(N, LINES);
+  const int mat = blockIdx.y;  // 0 C, 1 D, 2 p
+  const int w = w0 + blockIdx.z;
+  const int line0 = blockIdx.x * LINES, line = line0 + wv;
+  cd *base = reinterpret_cast<cd *>(smem);
+  cd *tw = base + LINES * LS;
+  cd *x = base + wv * LS;
+  const long o = mat_off(g, w);
+  double *X = (mat == 0 ? g.CX : (mat == 1 ? g.DX : g.pX)) + o;
+  double *A = (mat == 0 ? g.CA : (mat == 1 ? g.DA : g.pA)) + o;
+  const double *phg = g.tphase + (long)w * 2 * N;
+  // this lane's phases for the fused middle: positions j + r N/RL
+  cd ph[ITL][RL];
+#pragma unroll
+  for (int it = 0; it < ITL; ++it)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int j = min(lane + 64 * it, N / RL - 1);
+      ph[it][r] = ld(phg, j + r * (N / RL));
+    }
+  const double inv = 1.0 / N;
+  if (pass == 0) {
+    // first forward stage straight from the row (radix R0, no twiddles)
+    const bool on = line < N;
+    const int row = on ? line : 0;
+    cd a[IT0][R0];
+#pragma unroll
+    for (int it = 0; it < IT0; ++it)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) {
+        const int j = min(lane + 64 * it, N / R0 - 1);
+        a[it][r] = ld(X, (long)row * N + j + r * (N / R0));
+      }
+    plan_build_twiddles<N, 0, 0>(tw, g.twid, tid, blockDim.x);
+    plan_build_twiddles<N, 1, 0>(tw, g.twid, tid, blockDim.x);
+    __syncthreads();
+    if (!on) return;  // whole wave; no block barrier follows
+    {
+      cd ob[IT0][R0];
+#pragma unroll
+      for (int it = 0; it < IT0; ++it) dft_small<R0, -1>(a[it], ob[it]);
+      stage_write<N, 0, 0, IT0, R0>(x, lane, ob);
+      wave_lds_sync();
+    }
+    stages_lds<N, 0, 1, S - 1>(x, tw, lane);
+    fused_middle<N, ITL, RL>(x, tw, lane, ph);
+    stages_lds<N, 1, 1, S - 1>(x, tw, lane);
+    // last inverse stage straight to the row: outputs j + r N/RE
+    cd b[ITE][RE], ob[ITE][RE];
+    stage_read<N, 1, S - 1, ITE, RE>(x, tw, lane, b);
+#pragma unroll
+    for (int it = 0; it < ITE; ++it) {
+      dft_small<RE, +1>(b[it], ob[it]);
+      const int j = lane + 64 * it;
+      if ((N / RE) % 64 == 0 || j < N / RE)
+#pragma unroll
+        for (int r = 0; r < RE; ++r) st(X, (long)line * N + j + r * (N / RE), cscale(ob[it][r], inv));
+    }
+  } else {
+    plan_build_twiddles<N, 0, 0>(tw, g.twid, tid, blockDim.x);
+    plan_build_twiddles<N, 1, 0>(tw, g.twid, tid, blockDim.x);
+    for (int idx = tid; idx < LINES * N; idx += blockDim.x) {
+      const int c = idx % LINES, j = idx / LINES;
+      if (line0 + c < N) base[c * LS + fpad(j)] = ld(X, (long)j * N + line0 + c);
+    }
+    __syncthreads();
+    if (line < N) {
+      stages_lds<N, 0, 0, S - 1>(x, tw, lane);
+      fused_middle<N, ITL, RL>(x, tw, lane, ph);
+      stages_lds<N, 1, 1, S>(x, tw, lane);
+    }
+    __syncthreads();  // the stores below take lines of other waves
+    for (int idx = tid; idx < LINES * N; idx += blockDim.x) {
+      const int c = idx % LINES, j = idx / LINES, col = line0 + c;
+      if (col < N) {
+        const long ij = (long)j * N + col;
+        const cd v = cscale(base[c * LS + fpad(j)], inv);
+        st(X, ij, v);
+        if (avg && !(mat < 2 && j == col)) {
+          const cd a = ld(A, ij);
+          st(A, ij, cd{(a.re + v.re) / 2.0, (a.im + v.im) / 2.0});
+        }
+      }
+    }
+  }
+}
+
 // ===========================================================================
 // finish: diagonal |.| and e/h renormalisation (SBEs.py:5692-5702)
 // ===========================================================================
@@ -1443,24 +1689,47 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
 namespace {
 
 template <int NT, int LINES>
-int launch_transport_l(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
-  const size_t shm = ((size_t)LINES * tr_line_stride(g->N, LINES) + g->N) * sizeof(cd);  // lines + twiddles
-  if (shm > 65536) {
-    static bool attr_set = false;  // per instantiation
-    if (!attr_set) {
-      int rc = hip_check(hipFuncSetAttribute((const void *)k_transport<NT, LINES>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
-                         "hipFuncSetAttribute(k_transport)");
-      if (rc) return rc;
-      attr_set = true;
-    }
+int launch_transport_fl(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
+  const size_t shm = ((size_t)LINES * tr_line_stride(NT, LINES) + plan_twiddles(NT)) * sizeof(cd);
+  static bool attr_set = false;  // per instantiation
+  if (shm > 65536 && !attr_set) {
+    int rc = hip_check(hipFuncSetAttribute((const void *)k_transport_f<NT, LINES>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
+                       "hipFuncSetAttribute(k_transport_f)");
+    if (rc) return rc;
+    attr_set = true;
   }
-  const dim3 grid((g->N + LINES - 1) / LINES, 3, nw);
-  hipLaunchKernelGGL((k_transport<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 0, 0);
-  PSA_LAUNCH_CHECK("k_transport");
-  hipLaunchKernelGGL((k_transport<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 1, avg ? 1 : 0);
-  PSA_LAUNCH_CHECK("k_transport");
+  const dim3 grid((NT + LINES - 1) / LINES, 3, nw);
+  hipLaunchKernelGGL((k_transport_f<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 0, 0);
+  PSA_LAUNCH_CHECK("k_transport_f");
+  hipLaunchKernelGGL((k_transport_f<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 1, avg ? 1 : 0);
+  PSA_LAUNCH_CHECK("k_transport_f");
   return PSA_OK;
+}
+
+template <int NT, int LINES>
+int launch_transport_l(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
+  if constexpr (NT > 0) {
+    return launch_transport_fl<NT, LINES>(g, w0, nw, s, avg);
+  } else {
+    const size_t shm = ((size_t)LINES * tr_line_stride(g->N, LINES) + g->N) * sizeof(cd);  // lines + twiddles
+    if (shm > 65536) {
+      static bool attr_set = false;  // per instantiation
+      if (!attr_set) {
+        int rc = hip_check(hipFuncSetAttribute((const void *)k_transport<NT, LINES>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024),
+                           "hipFuncSetAttribute(k_transport)");
+        if (rc) return rc;
+        attr_set = true;
+      }
+    }
+    const dim3 grid((g->N + LINES - 1) / LINES, 3, nw);
+    hipLaunchKernelGGL((k_transport<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 0, 0);
+    PSA_LAUNCH_CHECK("k_transport");
+    hipLaunchKernelGGL((k_transport<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 1, avg ? 1 : 0);
+    PSA_LAUNCH_CHECK("k_transport");
+    return PSA_OK;
+  }
 }
 
 template <int NT>
