@@ -66,8 +66,10 @@ typedef struct psa_group {
   int32_t bx0, bnx; /* rows of rr inside the wire for QW2Prop: [bx0, bx0+bnx) */
   int32_t nnz_mb[8];/* entries in each many-body scattering list */
   int32_t tune[4];  /* launch choices, 0 = default: [0] split-BF16 RHS tile (0: 64, 1: 128),
-                       [1] transport lines per block (0: 4, else 2, 4 or 8),
-                       [2] GEMM block order (0: batches grouped per XCD, 1: linear), [3] reserved */
+                       [1] transport lines per block, row pass (0: 4, else 2, 4 or 8),
+                       [2] GEMM block order (0: batches grouped per XCD, 1: linear),
+                       [3] transport lines per block, column pass (0: 4, else 2, 4 or 8;
+                       N known at compile time: 64, 96, 128, 192, 256, 384, 512) */
   int32_t pad0;
   /* ---- scalars (SI units) ---- */
   double hbar, e0, gap, L, ehint, dcv, area, dkr, dr_r, drr;
@@ -123,7 +125,7 @@ typedef struct psa_group {
                                          re_hi, re_lo, im_hi, im_lo */
   double *kscr;                       /* optional scratch for split-K partial sums of the source and
                                          propagation-grid products (NULL: no split); kscr_len doubles,
-                                         16 x 2 x max(Nr x 4 Nw, (2N-1) x 3 Nw, bnx x 5 Nw) uses all splits */
+                                         8 x 2 x max(Nr x 4 Nw, (2N-1) x 3 Nw, bnx x 5 Nw) uses all splits */
   int64_t kscr_len;
   const int32_t *wid_host;            /* [Nw] host copy of wire_id */
 } psa_group;

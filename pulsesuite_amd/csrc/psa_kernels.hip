@@ -616,8 +616,9 @@ __global__ __launch_bounds__(64 * RL_WAVES) void k_relax(psa_group g, int w0, do
 // FFT -> phase -> inverse FFT, each in LDS (Stockham autosort, radix 4
 // stages while 4 divides the remaining length, then 2 and 3, N = 2^a 3^b,
 // twiddles from a per-N table).  One wave per line, 2, 4 or 8 lines per
-// block (tune[1]); the column pass reads and writes that many adjacent
-// columns per row.
+// block (tune[1] for the row pass, tune[3] for the column pass of the
+// compile-time-N kernel); the column pass reads and writes that many
+// adjacent columns per row.
 // ===========================================================================
 
 __device__ __forceinline__ cd tw_get(const cd *tw, int idx, int sign) {
@@ -884,6 +885,17 @@ __global__ __launch_bounds__(64 * LINES) void k_transport(psa_group g, int w0, i
 // stages its lines through LDS as before.  Each stage has its own twiddle
 // table (w^(r k N/(R Ns)), contiguous in k), forward and inverse apart.
 // ---------------------------------------------------------------------------
+// LDS slot of element i of a line: i with bits 0-3 XOR-ed by 5 ((i >> 4) & 3),
+// a permutation within each 16-element block that keeps the 16-lane groups
+// of contiguous accesses and of the radix-4 write patterns (stride 4, and
+// runs of four at stride 16) on 16 different 16-byte bank groups.
+__device__ __forceinline__ int fswz(int i) { return i ^ (5 * ((i >> 4) & 3)); }
+// line stride (slots) for fswz lines: at least N, = 16 / lines modulo 16
+__host__ __device__ inline int tr_line_stride_s(int N, int lines) {
+  const int want = 16 / lines;
+  return N + ((want - N % 16) + 16) % 16;
+}
+
 struct RadixPlan {
   int S;
   int R[12];
@@ -961,10 +973,10 @@ __device__ __forceinline__ void stage_read(const cd *x, const cd *tw, int lane, 
     const int j = lane + 64 * it;
     if (M % 64 == 0 || j < M) {
       const int k = j % Ns;
-      a[it][0] = x[fpad(j)];
+      a[it][0] = x[fswz(j)];
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const cd v = x[fpad(j + r * M)];
+        const cd v = x[fswz(j + r * M)];
         a[it][r] = Ns == 1 ? v : cmul(v, tw_get(tw + TOFF, (r - 1) * Ns + k, SIGN));
       }
     }
@@ -981,7 +993,7 @@ __device__ __forceinline__ void stage_write(cd *x, int lane, const cd (&o)[IT][R
     if (M % 64 == 0 || j < M) {
       const int k = j % Ns, base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[fpad(base + r * Ns)] = o[it][r];
+      for (int r = 0; r < R; ++r) x[fswz(base + r * Ns)] = o[it][r];
     }
   }
 }
@@ -1030,7 +1042,7 @@ __global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0,
   constexpr int R0 = plan_r(N, 0, 0), IT0 = (N / R0 + 63) / 64;      // first forward stage
   constexpr int RE = plan_r(N, 1, S - 1), ITE = (N / RE + 63) / 64;  // last inverse stage
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int LS = tr_line_stride(N, LINES);
+  const int LS = tr_line_stride_s(N, LINES);
   const int mat = blockIdx.y;  // 0 C, 1 D, 2 p
   const int w = w0 + blockIdx.z;
   const int line0 = blockIdx.x * LINES, line = line0 + wv;
@@ -1093,7 +1105,7 @@ __global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0,
     plan_build_twiddles<N, 1, 0>(tw, g.twid, tid, blockDim.x);
     for (int idx = tid; idx < LINES * N; idx += blockDim.x) {
       const int c = idx % LINES, j = idx / LINES;
-      if (line0 + c < N) base[c * LS + fpad(j)] = ld(X, (long)j * N + line0 + c);
+      if (line0 + c < N) base[c * LS + fswz(j)] = ld(X, (long)j * N + line0 + c);
     }
     __syncthreads();
     if (line < N) {
@@ -1106,7 +1118,7 @@ __global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0,
       const int c = idx % LINES, j = idx / LINES, col = line0 + c;
       if (col < N) {
         const long ij = (long)j * N + col;
-        const cd v = cscale(base[c * LS + fpad(j)], inv);
+        const cd v = cscale(base[c * LS + fswz(j)], inv);
         st(X, ij, v);
         if (avg && !(mat < 2 && j == col)) {
           const cd a = ld(A, ij);
@@ -1472,7 +1484,7 @@ int launch_gemm(const P &p, int prec, int M, int Ncols, int batches, hipStream_t
 // A single-batch FP64 product of M rows x ncol columns over K, split over
 // K when the tile grid alone would leave most of the GPU idle and the
 // group provides the scratch (g.kscr): about two workgroups per CU, at most
-// 16 splits, each at least one 16-deep slab.
+// 8 splits, each at least one 16-deep slab.
 template <class P>
 int launch_gemm_splitk(const P &p, int M, int ncol, int K, hipStream_t s, const char *what) {
   const psa_group &g = p.g;
@@ -1483,7 +1495,7 @@ int launch_gemm_splitk(const P &p, int M, int ncol, int K, hipStream_t s, const 
       cus = 256;
   }
   const int tiles = ((M + GBM - 1) / GBM) * ((ncol + GBN - 1) / GBN);
-  int S = min(16, max(1, (2 * cus + tiles - 1) / tiles));
+  int S = min(8, max(1, (2 * cus + tiles - 1) / tiles));
   S = min(S, (K + GBK - 1) / GBK);
   while (S > 1 && (long)S * M * ncol * 2 > g.kscr_len) --S;
   if (S <= 1 || !g.kscr) return launch_gemm(p, PSA_PREC_FP64, M, ncol, 1, s, what);
@@ -1688,9 +1700,10 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
 
 namespace {
 
+// one pass of the compile-time-N transport (0 rows, 1 columns) with LINES lines per block
 template <int NT, int LINES>
-int launch_transport_fl(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
-  const size_t shm = ((size_t)LINES * tr_line_stride(NT, LINES) + plan_twiddles(NT)) * sizeof(cd);
+int launch_transport_fp(const psa_group *g, int32_t w0, int nw, hipStream_t s, int pass, bool avg) {
+  const size_t shm = ((size_t)LINES * tr_line_stride_s(NT, LINES) + plan_twiddles(NT)) * sizeof(cd);
   static bool attr_set = false;  // per instantiation
   if (shm > 65536 && !attr_set) {
     int rc = hip_check(hipFuncSetAttribute((const void *)k_transport_f<NT, LINES>,
@@ -1700,18 +1713,25 @@ int launch_transport_fl(const psa_group *g, int32_t w0, int nw, hipStream_t s, b
     attr_set = true;
   }
   const dim3 grid((NT + LINES - 1) / LINES, 3, nw);
-  hipLaunchKernelGGL((k_transport_f<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 0, 0);
-  PSA_LAUNCH_CHECK("k_transport_f");
-  hipLaunchKernelGGL((k_transport_f<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, 1, avg ? 1 : 0);
+  hipLaunchKernelGGL((k_transport_f<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, pass,
+                     (pass == 1 && avg) ? 1 : 0);
   PSA_LAUNCH_CHECK("k_transport_f");
   return PSA_OK;
 }
 
+template <int NT>
+int launch_transport_fpass(const psa_group *g, int32_t w0, int nw, hipStream_t s, int pass, bool avg) {
+  switch (pass == 0 ? g->tune[1] : g->tune[3]) {
+    case 2: return launch_transport_fp<NT, 2>(g, w0, nw, s, pass, avg);
+    case 8: return launch_transport_fp<NT, 8>(g, w0, nw, s, pass, avg);
+    default: return launch_transport_fp<NT, 4>(g, w0, nw, s, pass, avg);
+  }
+}
+
 template <int NT, int LINES>
 int launch_transport_l(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
-  if constexpr (NT > 0) {
-    return launch_transport_fl<NT, LINES>(g, w0, nw, s, avg);
-  } else {
+  static_assert(NT == 0, "compile-time N uses k_transport_f");
+  {
     const size_t shm = ((size_t)LINES * tr_line_stride(g->N, LINES) + g->N) * sizeof(cd);  // lines + twiddles
     if (shm > 65536) {
       static bool attr_set = false;  // per instantiation
@@ -1734,10 +1754,16 @@ int launch_transport_l(const psa_group *g, int32_t w0, int nw, hipStream_t s, bo
 
 template <int NT>
 int launch_transport(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
-  switch (g->tune[1]) {
-    case 2: return launch_transport_l<NT, 2>(g, w0, nw, s, avg);
-    case 8: return launch_transport_l<NT, 8>(g, w0, nw, s, avg);
-    default: return launch_transport_l<NT, 4>(g, w0, nw, s, avg);
+  if constexpr (NT > 0) {
+    // row pass with tune[1] lines per block, column pass with tune[3]
+    if (int rc = launch_transport_fpass<NT>(g, w0, nw, s, 0, avg)) return rc;
+    return launch_transport_fpass<NT>(g, w0, nw, s, 1, avg);
+  } else {
+    switch (g->tune[1]) {
+      case 2: return launch_transport_l<NT, 2>(g, w0, nw, s, avg);
+      case 8: return launch_transport_l<NT, 8>(g, w0, nw, s, avg);
+      default: return launch_transport_l<NT, 4>(g, w0, nw, s, avg);
+    }
   }
 }
 

@@ -114,7 +114,7 @@ class WireGroup:
         nspl = nwz * 12 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) if prec == "bf16x3" else 1
         self.s["spl"] = torch.zeros(nspl, dtype=torch.int16, device=d)
         # split-K partial sums of the source / propagation-grid products
-        self._kscr_len = 16 * 2 * max(Nr * 4 * nwz, (2 * N - 1) * 3 * nwz, max(tb["bnx"], 1) * 5 * nwz)
+        self._kscr_len = 8 * 2 * max(Nr * 4 * nwz, (2 * N - 1) * 3 * nwz, max(tb["bnx"], 1) * 5 * nwz)
         self.s["kscr"] = torch.empty(self._kscr_len, dtype=F64, device=d)
         if wire_ids is None:
             wire_ids = np.arange(1, nwz + 1)
@@ -243,10 +243,12 @@ class WireGroup:
     def autotune(self, reps=3):
         """Pick the launch choices of this group by timing them on its own
         buffers: tune[2] GEMM block order (XCD-grouped or linear), tune[0]
-        split-BF16 tile (64 or 128), tune[1] transport lines per block (4, 2,
-        8).  Only per-step scratch is written (the new level, Hamiltonians,
-        dephasing tables), so the state is unchanged.  Returns {knob: (choice,
-        ms per candidate)}."""
+        split-BF16 tile (64 or 128), tune[1] / tune[3] transport lines per
+        block for the row / column pass (4, 2, 8), timed inside full steps
+        (the column pass carries the leapfrog average there).  The state is
+        unchanged: only per-step scratch is written, and the averaged level
+        that the steps overwrite is saved and restored.  Returns {knob:
+        (choice, ms per candidate)}."""
         od = bool(self.prm.flags.get("OffDiagDph"))
         self.prop2qw()
         self.prep()
@@ -258,11 +260,17 @@ class WireGroup:
                 self.offdiag()
             self.rhs(1, 1e-18)
 
+        def full_step():  # psa_step without the level swap
+            capi.check(self.lib.psa_step(ctypes.byref(self.S), 0, self.nw, 1e-18, self._stream()))
+
         knobs = [(2, [0, 1], contractions)]
         if self.prec == "bf16x3":
             knobs.append((0, [0, 1], lambda: self.rhs(1, 1e-18)))
+        saved = None
         if self.prm.flags.get("DCTrans") and self.prm.flags.get("LF"):
-            knobs.append((1, [4, 2, 8], lambda: self.transport(1e-18)))
+            saved = {k: self.s[k].clone() for k in ("pA", "CA", "DA")}
+            knobs.append((1, [4, 2, 8], full_step))
+            knobs.append((3, [4, 2, 8], full_step))
         out = {}
         for idx, cands, fn in knobs:
             times = []
@@ -281,6 +289,10 @@ class WireGroup:
             self.tune[idx] = cands[int(np.argmin(times))]
             self._sync_struct()
             out[idx] = (self.tune[idx], times)
+        if saved is not None:
+            for k, v in saved.items():
+                self.s[k].copy_(v)
+            torch.cuda.synchronize()
         return out
 
     def footprint_bytes(self):

@@ -32,6 +32,10 @@ def test_every_launch_choice_gives_identical_results(prec):
                 got = _step_outputs(c, [t0, t1, t2, 0])
                 for a, b in zip(got, ref):
                     assert np.array_equal(a, b), (t0, t1, t2)
+    for t3 in (2, 4, 8):
+        got = _step_outputs(c, [0, 0, 0, t3])
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), t3
 
 
 def test_autotune_keeps_state_and_picks_valid_choices():
@@ -42,4 +46,5 @@ def test_autotune_keeps_state_and_picks_valid_choices():
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
     assert c.wg.tune[0] in (0, 1) and c.wg.tune[1] in (2, 4, 8) and c.wg.tune[2] in (0, 1)
-    assert set(tuned) == {0, 1, 2}
+    assert c.wg.tune[3] in (2, 4, 8)
+    assert set(tuned) == {0, 1, 2, 3}
