@@ -174,10 +174,11 @@ namespace t64 {
 
 // ---------------------------------------------------------------------------
 // k_rhs (64-tile variant): block = 64x64 output tile of one (wire, equation); 4 waves
-// (2x2) of 32x32 = 2x2 MFMA blocks; K slabs of 32 staged in LDS (row stride
-// 40 bf16 = 80 B: the 16 rows a lane group reads land on distinct banks).
+// (2x2) of 32x32 = 2x2 MFMA blocks; K slabs of 32 staged in LDS (rows of
+// 64 B, the 16-byte chunk index XOR-ed with (row >> 2) & 3 so the 16 rows
+// a lane group reads, and the staging writes, fall on distinct banks).
 // ---------------------------------------------------------------------------
-constexpr int BK2 = 32, BLD2 = 40, TILE = 64;
+constexpr int BK2 = 32, BLD2 = 32, TILE = 64;
 
 struct BTile {
   uint16_t v[4][64][BLD2];  // [plane][row][k]
@@ -231,11 +232,13 @@ __device__ __forceinline__ void bload_band(BFrag &f, const psa_group &g, int w, 
 
 __device__ __forceinline__ void bcommit(const BFrag &f, BTile &t) {
 #pragma unroll
-  for (int pl = 0; pl < 4; ++pl) *reinterpret_cast<u32x4 *>(&t.v[pl][f.row][f.kc]) = f.c[pl];
+  for (int pl = 0; pl < 4; ++pl)
+    *reinterpret_cast<u32x4 *>(&t.v[pl][f.row][((f.kc >> 3) ^ ((f.row >> 2) & 3)) << 3]) = f.c[pl];
 }
 
 __device__ __forceinline__ bf16x8 bfrag_read(const BTile &t, int pl, int row, int kq) {
-  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(&t.v[pl][row][kq]));
+  return __builtin_bit_cast(bf16x8,
+                            *reinterpret_cast<const u32x4 *>(&t.v[pl][row][((kq >> 3) ^ ((row >> 2) & 3)) << 3]));
 }
 
 
