@@ -14,7 +14,7 @@
 // H[k2 = a][k1 = a + d] are contiguous.
 //
 // FP32 (the reduced-precision modes); FP64 stays on the GEMM engine.
-// Block = 4 waves of 64 rows.  Y[m'][col] =
+// Block = 8 waves of 32 rows.  Y[m'][col] =
 // X[m' + s][m' + s + d] is staged in LDS 32 K values at a time (16-byte units
 // [k / E][col][E], separate real and imaginary planes) with the next chunk's
 // loads in flight; the Toeplitz operand comes from a per-block LDS table of
@@ -35,13 +35,17 @@ template <typename T>
 struct Cfg;
 template <>
 struct Cfg<float> {
-  static constexpr int RB = 4;  // 16-row blocks per wave
+  static constexpr int WAVES = 8, RB = 2;  // waves per block, 16-row blocks per wave
   typedef fv4 unit;
 };
 
 template <typename T>
 constexpr int rows_per_block() {
-  return 4 * 16 * Cfg<T>::RB;
+  return Cfg<T>::WAVES * 16 * Cfg<T>::RB;
+}
+template <typename T>
+constexpr int threads_per_block() {
+  return 64 * Cfg<T>::WAVES;
 }
 
 // diagonal offset d and row/K shift s of column x (0..63) of column block cb
@@ -56,9 +60,10 @@ __device__ __forceinline__ void col_info(int cb, int x, int &d, int &s) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_calch(psa_group g, int w0) {
+__global__ __launch_bounds__(threads_per_block<T>(), 2) void k_calch(psa_group g, int w0) {
   constexpr int RB = Cfg<T>::RB, ROWS = rows_per_block<T>(), E = 16 / sizeof(T);
-  constexpr int PER = KC * COLS / 256;  // Y elements per thread per chunk (8)
+  constexpr int THREADS = threads_per_block<T>();
+  constexpr int PER = KC * COLS / THREADS;  // Y elements per thread per chunk (consecutive K)
   typedef typename Cfg<T>::unit unit_t;
   __shared__ T vt[NMAX];                              // V(|m' - a'|), V(0) -> 0
   __shared__ unit_t bs[2][2][KC / E * COLS];          // [buffer][re, im]
@@ -71,9 +76,9 @@ __global__ __launch_bounds__(256, 2) void k_calch(psa_group g, int w0) {
   if (row0 >= Lmax) return;      // block-uniform, before any barrier
 
   const double *vc = g.prep + (long)w * PSA_PREP_STRIDE(N) + PSA_PREP_VC(N) + (long)which * N;
-  for (int i = tid; i < N; i += 256) vt[i] = (T)(i == 0 ? 0.0 : vc[i]);
+  for (int i = tid; i < N; i += THREADS) vt[i] = (T)(i == 0 ? 0.0 : vc[i]);
 
-  // this thread's Y column and K rows 8 (tid >> 6) + e of each chunk
+  // this thread's Y column and K rows PER (tid >> 6) + e of each chunk
   const long o = mat_off(g, w);
   const double *X = (which == 0 ? g.pB : (which == 1 ? g.CB : g.DB)) + o;
   const int xB = tid & (COLS - 1);
@@ -81,7 +86,7 @@ __global__ __launch_bounds__(256, 2) void k_calch(psa_group g, int w0) {
   col_info(cb, xB, dB, sB);
   // K values m' in [loB, hiB) have m = m' + s and m + d in [0, N)
   const int loB = max(0, -(sB + dB)), hiB = min(N - sB, N - sB - dB);
-  const int kq = 8 * (tid >> 6);
+  const int kq = PER * (tid >> 6);
   const int nch = (Lmax + KC - 1) / KC;
 
   cd rv[PER];

@@ -1592,7 +1592,8 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (prec == PSA_PREC_FP32 && g->N <= chx::NMAX) {
     const int cols = (g->N + 31) / 32;  // column blocks: 32 positive and 32 non-positive d each
     constexpr int R = chx::rows_per_block<float>();
-    hipLaunchKernelGGL(chx::k_calch<float>, dim3(cols, (g->N + R - 1) / R, 3 * nw), dim3(256), 0, s, *g, w0);
+    hipLaunchKernelGGL(chx::k_calch<float>, dim3(cols, (g->N + R - 1) / R, 3 * nw),
+                       dim3(chx::threads_per_block<float>()), 0, s, *g, w0);
     PSA_LAUNCH_CHECK("k_calch");
     return PSA_OK;
   }
@@ -1641,9 +1642,9 @@ int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream)
     if (g->N > odg::NMAX) return fail(PSA_E_UNSUPPORTED, "psa_offdiag_tables: N > 512 needs the lor table");
     const dim3 grid((g->N + odg::ROWS - 1) / odg::ROWS, (2 * nw + odg::COLS - 1) / odg::COLS, 2 * (2 * g->N - 1));
     if (prec == PSA_PREC_FP64)
-      hipLaunchKernelGGL(odg::k_offdiag<double>, grid, dim3(256), 0, s, *g, w0, nw, 0);
+      hipLaunchKernelGGL(odg::k_offdiag<double>, grid, dim3(odg::Cfg<double>::THREADS), 0, s, *g, w0, nw, 0);
     else
-      hipLaunchKernelGGL(odg::k_offdiag<float>, grid, dim3(256), 0, s, *g, w0, nw, 1);
+      hipLaunchKernelGGL(odg::k_offdiag<float>, grid, dim3(odg::Cfg<float>::THREADS), 0, s, *g, w0, nw, 1);
     PSA_LAUNCH_CHECK("k_offdiag");
     return PSA_OK;
   }

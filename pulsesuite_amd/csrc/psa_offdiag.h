@@ -9,7 +9,8 @@
 // and f_A = n[j](1-n[p]), f_B = (1-n[j]) n[p] (j = p -/+ q) the occupation
 // columns of every wire.  Columns are (wire, s) pairs.
 //
-// Block = 4 waves over 256 rows k x 64 columns of one (q, kind).  The
+// Block = 8 waves of 32 rows (FP32) or 4 of 64 (FP64) over 256 rows k x 64
+// columns of one (q, kind).  The
 // occupation operand B [K][64] is built cooperatively, 32 K indices at a
 // time, into LDS (16-byte units [k / E][col][E], E = 16 / sizeof(T), so a
 // lane's four K values of one MFMA group are one or two ds_read_b128), with
@@ -30,7 +31,13 @@
 namespace psa {
 namespace odg {
 
-constexpr int ROWS = 256;   // rows per block: 4 waves x 64
+constexpr int ROWS = 256;  // rows per block
+// waves per block and 16-row blocks per wave: FP32 8 x 32 rows (4 waves per
+// SIMD fit), FP64 4 x 64 rows (its accumulators take twice the registers)
+template <typename T>
+struct Cfg {
+  static constexpr int WAVES = sizeof(T) == 4 ? 8 : 4, RB = ROWS / (16 * WAVES), THREADS = 64 * WAVES;
+};
 constexpr int COLS = 64;    // (wire, s) columns per block
 constexpr int KC = 32;      // K indices per LDS chunk
 constexpr int KMAX = 1024;  // K <= 2N - 1 for N <= 512
@@ -48,9 +55,10 @@ struct Unit<double> {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw, int scaled) {
+__global__ __launch_bounds__(Cfg<T>::THREADS, 2) void k_offdiag(psa_group g, int w0, int nw, int scaled) {
+  constexpr int WAVES = Cfg<T>::WAVES, RB = Cfg<T>::RB, THREADS = Cfg<T>::THREADS;
   constexpr int E = 16 / sizeof(T);       // elements per 16-byte unit
-  constexpr int PER = KC * COLS / 256;    // B elements per thread per chunk (8)
+  constexpr int PER = KC * COLS / THREADS;  // B elements per thread per chunk
   typedef typename Unit<T>::t unit_t;
   __shared__ double e1[KMAX], e2[KMAX];  // K-side energies: E[p -/+ q], E[p]
   __shared__ unit_t bs[2][KC / E * COLS];
@@ -75,7 +83,7 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
   const double *Es = kind == 0 ? g.Ee : g.Eh;  // carrier of the rows
   const double *Eo = kind == 0 ? g.Eh : g.Ee;  // the other carrier
 
-  for (int kk = tid; kk < nch * KC; kk += 256) {
+  for (int kk = tid; kk < nch * KC; kk += THREADS) {
     double a = 0.0, b = 0.0;
     if (kk < K0) {
       const int p = kb0 + kk;
@@ -104,11 +112,11 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
   const double v2O = colok ? sc * vsq[(long)aqc * W + wB] : 0.0;
 
   double ra[PER], rc[PER];
-  // element e of this thread: unit (tid >> 6) + 4 (e / E), position e % E
+  // element e of this thread: unit (tid >> 6) + WAVES (e / E), position e % E
   auto issue = [&](int ch) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const int kk = ch * KC + ((tid >> 6) + 4 * (e / E)) * E + e % E;
+      const int kk = ch * KC + ((tid >> 6) + WAVES * (e / E)) * E + e % E;
       int p = 0, j = 0;
       const double *nn = nnS;
       if (kk < K0) {
@@ -130,28 +138,28 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
 #pragma unroll
       for (int s = 0; s < E; ++s) {
         const int e = u * E + s;
-        const int kk = ch * KC + ((tid >> 6) + 4 * u) * E + s;
+        const int kk = ch * KC + ((tid >> 6) + WAVES * u) * E + s;
         const double a = ra[e], c = rc[e];
         const double f = sB ? (1.0 - a) * c : a * (1.0 - c);
         val[s] = (T)(kk < K ? (kk < K0 ? v2S : v2O) * f : 0.0);
       }
-      dst[((tid >> 6) + 4 * u) * COLS + cB] = val;
+      dst[((tid >> 6) + WAVES * u) * COLS + cB] = val;
     }
   };
 
   using acc_t = typename Mfma<T>::acc;
-  acc_t acc[4][4];
+  acc_t acc[RB][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[i][c] = acc_t{0, 0, 0, 0};
 
   const int g4 = lane >> 4, r16 = lane & 15;
-  const int wrow0 = row0 + 64 * wv;
-  const bool wact = K > 0 && wrow0 < re && wrow0 + 64 > rb;  // wave-uniform
-  double ekq[4], ek[4];
+  const int wrow0 = row0 + 16 * RB * wv;
+  const bool wact = K > 0 && wrow0 < re && wrow0 + 16 * RB > rb;  // wave-uniform
+  double ekq[RB], ek[RB];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < RB; ++i) {
     const int k = wrow0 + 16 * i + r16;
     ek[i] = Es[min(k, N - 1)];
     ekq[i] = Es[min(max(k + q, 0), N - 1)];
@@ -168,9 +176,9 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
       tcst[tid][1] = 1.0 / (PI * hg);
     }
   }
-  double ud[4];
+  double ud[RB];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) ud[i] = ekq[i] - ek[i];
+  for (int i = 0; i < RB; ++i) ud[i] = ekq[i] - ek[i];
 
   if (K > 0) {
     issue(0);
@@ -197,11 +205,11 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
               const int j = u * E + s;
               const double d1 = e1[kk + j], d2 = e2[kk + j];
               const double *tc = tcst[kk + j < K0 ? 0 : 1];
-              T a[4];
+              T a[RB];
               if constexpr (sizeof(T) == 8) {
                 const double c1 = tc[0], c2 = tc[1];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                for (int i = 0; i < RB; ++i) {
                   const double Ed = ekq[i] + d1 - d2 - ek[i];
                   a[i] = c1 / (Ed * Ed + c2);
                 }
@@ -209,13 +217,13 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
                 const float ih = (float)tc[0], f1 = (float)tc[1];
                 const double dv = d1 - d2;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                for (int i = 0; i < RB; ++i) {
                   const float x = (float)(ud[i] + dv) * ih;
                   a[i] = f1 * __builtin_amdgcn_rcpf(fmaf(x, x, 1.0f));
                 }
               }
 #pragma unroll
-              for (int i = 0; i < 4; ++i)
+              for (int i = 0; i < RB; ++i)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) acc[i][c] = Mfma<T>::mma(a[i], (T)bu[c][s], acc[i][c]);
             }
@@ -232,7 +240,7 @@ __global__ __launch_bounds__(256, 2) void k_offdiag(psa_group g, int w0, int nw,
   const double unscale = scaled ? 0x1p-100 : 1.0;
   const double pih = PI / g.hbar;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
