@@ -490,6 +490,113 @@ __global__ __launch_bounds__(MB_THREADS) void k_mb(psa_group g, int w0, int nw) 
   }
 }
 
+// The same sums for the reduced-precision modes: occupations and V^2 staged
+// in FP32 (V^2 scaled by 2^100, exact, to stay in the FP32 normal range;
+// unscaled with the FP64 result) for 16 wires per block, so a lane (entry
+// group lane >> 3, wires 2c and 2c + 1 for c = lane & 7) decodes each entry
+// once for two wires and reads each table with one 8-byte LDS load.  The
+// occupation and V^2 products run in FP32; the table weight (~1e-270 ..
+// 1e57) multiplies them in FP64 into FP64 sums.
+constexpr int MB2_WC = 16;
+
+template <int L>
+__device__ __forceinline__ void mb_list_sum2(const uint32_t *ix, const double *vl, int beg, int end,
+                                             const float *s_e, const float *s_h, const float *vv,
+                                             double &acc0, double &acc1) {
+  using T = MbList<L>;
+  const float *t1 = T::c1 ? s_h : s_e, *t2 = T::c2 ? s_h : s_e, *t3 = T::c3 ? s_h : s_e;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, u = lane >> 3, c2 = 2 * (lane & 7);
+  for (int e0 = beg + wv * 64; e0 < end; e0 += MB_THREADS) {
+    const int e = e0 + lane;
+    const uint32_t pk_l = e < end ? ix[e] : 0u;
+    const double val_l = e < end ? vl[e] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int src = 8 * r + u;
+      const uint32_t pk = (uint32_t)__shfl((int)pk_l, src, 64);
+      const double val = __shfl(val_l, src, 64);
+      int a, cc, r1, r2, r3;
+      mb_indices<L>(pk, a, cc, r1, r2, r3);
+      // VC is lower-triangular: V(a, c) = 0 for a < c
+      const float2 v2 = *reinterpret_cast<const float2 *>(&vv[(a >= cc ? a - cc : 0) * MB2_WC + c2]);
+      // the table weights span ~1e-270 .. 1e57: they stay FP64
+      const double w = (a >= cc) ? val : 0.0;
+      const float2 p1 = *reinterpret_cast<const float2 *>(&t1[r1 * MB2_WC + c2]);
+      const float2 p2 = *reinterpret_cast<const float2 *>(&t2[r2 * MB2_WC + c2]);
+      const float2 p3 = *reinterpret_cast<const float2 *>(&t3[r3 * MB2_WC + c2]);
+      const float f1x = T::x1 ? 1.0f - p1.x : p1.x, f1y = T::x1 ? 1.0f - p1.y : p1.y;
+      const float f2x = T::x2 ? 1.0f - p2.x : p2.x, f2y = T::x2 ? 1.0f - p2.y : p2.y;
+      const float f3x = T::x3 ? 1.0f - p3.x : p3.x, f3y = T::x3 ? 1.0f - p3.y : p3.y;
+      acc0 = fma((double)(v2.x * (f1x * f2x * f3x)), w, acc0);
+      acc1 = fma((double)(v2.y * (f1y * f2y * f3y)), w, acc1);
+    }
+  }
+}
+
+__global__ __launch_bounds__(MB_THREADS) void k_mb2(psa_group g, int w0, int nw) {
+  extern __shared__ float smf[];
+  __shared__ double red2[MB_THREADS / 64][MB2_WC];
+  const int N = g.N, W = g.Nw, slot = blockIdx.y, kb = blockIdx.x * MB_KG;
+  const int wc0 = blockIdx.z * MB2_WC;  // first wire of the chunk (relative to w0)
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int ch2 = slot < 2 ? 1 : 2;  // second V^2 channel: Vee for electrons, Vhh for holes
+  float *s_ne = smf, *s_nh = smf + N * MB2_WC;
+  float *s_v0 = smf + 2 * N * MB2_WC, *s_v1 = smf + 3 * N * MB2_WC;
+  const double *occ = g.wmb + 4L * N * W, *vsq = g.wmb + 6L * N * W;
+  for (int idx = tid; idx < N * MB2_WC; idx += MB_THREADS) {
+    const int k = idx / MB2_WC, cc = idx % MB2_WC;
+    const int w = w0 + min(wc0 + cc, nw - 1);  // idle wires of the last chunk repeat one wire
+    s_ne[idx] = (float)occ[(long)k * W + w];
+    s_nh[idx] = (float)occ[(long)(N + k) * W + w];
+    s_v0[idx] = (float)(0x1p100 * vsq[(long)k * W + w]);
+    s_v1[idx] = (float)(0x1p100 * vsq[((long)ch2 * N + k) * W + w]);
+  }
+  __syncthreads();
+  for (int kk = 0; kk < MB_KG; ++kk) {
+    const int k = kb + kk;
+    if (k >= N) break;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int half = 0; half < 2; ++half) {
+      // slot 0 WinE <- lists 0, 2; 1 WoutE <- 1, 3; 2 WinH <- 4, 6; 3 WoutH <- 5, 7
+      const int L = (slot >> 1) * 4 + (slot & 1) + 2 * half;
+      if (g.nnz_mb[L] == 0) continue;
+      const int beg = g.mb_off[L][k], end = g.mb_off[L][k + 1];
+      const uint32_t *ix = g.mb_idx[L];
+      const double *vl = g.mb_val[L];
+      const float *vv = half == 0 ? s_v0 : s_v1;
+      switch (L) {
+        case 0: mb_list_sum2<0>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
+        case 1: mb_list_sum2<1>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
+        case 2: mb_list_sum2<2>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
+        case 3: mb_list_sum2<3>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
+        case 4: mb_list_sum2<4>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
+        case 5: mb_list_sum2<5>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
+        case 6: mb_list_sum2<6>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
+        default: mb_list_sum2<7>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
+      }
+    }
+    // reduce over the eight entry groups of a wave (lane bits 3..5), then
+    // over the waves, in a fixed order
+#pragma unroll
+    for (int o = 8; o < 64; o *= 2) {
+      acc0 += __shfl_xor(acc0, o, 64);
+      acc1 += __shfl_xor(acc1, o, 64);
+    }
+    if (lane < 8) {
+      red2[wv][2 * lane] = acc0;
+      red2[wv][2 * lane + 1] = acc1;
+    }
+    __syncthreads();
+    if (tid < MB2_WC && wc0 + tid < nw) {
+      double sum = 0.0;
+#pragma unroll
+      for (int u = 0; u < MB_THREADS / 64; ++u) sum += red2[u][tid];
+      g.wmb[(long)(w0 + wc0 + tid) * 4 * N + slot * N + k] = sum * 0x1p-100;
+    }
+    __syncthreads();
+  }
+}
+
 // Wire-fastest occupations of the new level's diagonals for k_mb (|Re|,
 // coulomb.py:1307-1310).
 __global__ void k_diag_gather(psa_group g, int w0) {
@@ -1688,9 +1795,25 @@ int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *
         attr_set = true;
       }
     }
-    hipLaunchKernelGGL(k_mb, dim3((g->N + MB_KG - 1) / MB_KG, 4, (nw + MB_WC - 1) / MB_WC), dim3(MB_THREADS),
-                       shm, s, *g, w0, nw);
-    PSA_LAUNCH_CHECK("k_mb");
+    if (g->prec != PSA_PREC_FP64) {
+      // reduced modes: FP32 tables for 16 wires per block
+      const size_t shm2 = 4 * (size_t)g->N * MB2_WC * sizeof(float);
+      static bool attr2_set = false;
+      if (shm2 > 65536 && !attr2_set) {
+        int rc = hip_check(hipFuncSetAttribute((const void *)k_mb2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)(4 * 512 * MB2_WC * sizeof(float))),
+                           "hipFuncSetAttribute(k_mb2)");
+        if (rc) return rc;
+        attr2_set = true;
+      }
+      hipLaunchKernelGGL(k_mb2, dim3((g->N + MB_KG - 1) / MB_KG, 4, (nw + MB2_WC - 1) / MB2_WC),
+                         dim3(MB_THREADS), shm2, s, *g, w0, nw);
+      PSA_LAUNCH_CHECK("k_mb2");
+    } else {
+      hipLaunchKernelGGL(k_mb, dim3((g->N + MB_KG - 1) / MB_KG, 4, (nw + MB_WC - 1) / MB_WC), dim3(MB_THREADS),
+                         shm, s, *g, w0, nw);
+      PSA_LAUNCH_CHECK("k_mb");
+    }
   }
   hipLaunchKernelGGL(k_relax, dim3(nw, (g->N + 63) / 64), dim3(64 * RL_WAVES), 0, s, *g, w0, dt);
   PSA_LAUNCH_CHECK("k_relax");

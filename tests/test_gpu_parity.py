@@ -422,6 +422,37 @@ def test_many_body_lists_device_vs_host(N):
         assert rel(a, b) < 1e-13, li
 
 
+@pytest.mark.parametrize("N,nw", [(64, 20), (256, 18)])
+def test_many_body_rates_reduced_vs_fp64(N, nw):
+    """The many-body in/out rates of the reduced modes (FP32 tables for 16
+    wires per block, two wires per lane) against the FP64 kernel (pinned
+    against the oracle through the step tests) on one random state: within
+    1e-5 per wire and rate; 18 and 20 wires cover a partial wire chunk."""
+    import torch
+    from pulsesuite_amd.engine import WireGroup
+    L = wire_length(N)
+    prm, _ = params_pair(L)
+    rr = np.linspace(-1.2 * L, 1.2 * L, 64)
+    st = random_state(np.random.default_rng(7 * N + nw), N, nw)
+    print(f"[mb rates] N={N} nw={nw}", flush=True)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        wg = WireGroup(prm, nw, rr, prec=prec)
+        wg.set_state(*st)
+        for a, b in (("pX", "pB"), ("CX", "CB"), ("DX", "DB")):
+            wg.s[a].copy_(wg.s[b])
+        wg.prep()
+        wg.relaxation(1e-17)
+        torch.cuda.synchronize()
+        out[prec] = wg.s["wmb"][:4 * N * nw].cpu().numpy().reshape(nw, 4, N)
+        del wg
+    ref = out["fp64"]
+    assert np.abs(ref).max() > 0
+    for w in range(nw):
+        for r in range(4):
+            assert rel(out["fp32"][w, r], ref[w, r]) < 1e-5, (w, r)
+
+
 @pytest.mark.parametrize("N", [384, 512])
 def test_transport_large_n(N):
     """Transport at the serving sizes (radix 4/2/3 stages up to 512) against
