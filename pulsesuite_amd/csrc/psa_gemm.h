@@ -409,7 +409,7 @@ struct RhsProb {
     const int e = b % 3, w = w0 + b / 3;
     const long o = mat_off(g, w);
     const long ij = (long)row * N + col;
-    const double hbar = g.hbar;
+    const double ihb = 1.0 / g.hbar;  // z / (i hbar) = -i z / hbar
     cd acc = {re, im};
     cd d;
     const double *hd = g.hdiag + (long)w * 4 * N;  // [2][N] complex
@@ -424,7 +424,7 @@ struct RhsProb {
       const cd pauli = {1.0 - ne.re - nh.re, -ne.im - nh.im};
       cd z = cadd(acc, cmul(cadd(he, hh), p));
       z = cadd(z, cmul(heh, pauli));
-      d = cdiv_ih(z, hbar);
+      d = cd{z.im * ihb, -z.re * ihb};
       const double gam = pr[PSA_PREP_GAME(N) + col] + pr[PSA_PREP_GAMH(N) + row];
       d = csub(d, cscale(p, gam));
     } else {
@@ -439,7 +439,7 @@ struct RhsProb {
       const cd hot = (row == col) ? cd{0.0, 0.0} : ld(Ho, ij);  // H_o[k2][k1], stored transposed
       cd z = cadd(acc, cmul(csub(h2, h1), x));
       z = cadd(z, cmul(csub(n1, n2), hot));
-      d = cdiv_ih(z, hbar);
+      d = cd{z.im * ihb, -z.re * ihb};
     }
     double *out = (e == 0 ? g.pX : (e == 1 ? g.CX : g.DX)) + o;
     if (mode == 0) {
