@@ -247,6 +247,76 @@ int psa_record(const psa_group *g, int32_t w0, int32_t nw, int32_t xmid, double 
  * relaxation, transport, finish, sources, qw2prop. */
 int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream);
 
+/* ---- 1-D pseudo-spectral field propagator coupled to the wires ----
+ *
+ * The reference drives the wires with spatially uniform analytic pulses
+ * (sbetestprop.py:183-198) and has no propagator on this path; it ships the
+ * pieces of one: the staggered differences dEdx / dHdx (usefulsubs.py:612-655,
+ * 704-747) and the real-space absorber ApplyABC (usefulsubs.py:584-609).
+ * This propagator carries a pulse along y, normal to the wires (which lie
+ * along x), through an array of wires, so the transmitted field can be
+ * recorded:
+ *  - fields (Ex, Hz) and (Ez, Hx) on N points y_j = j dy (E) and
+ *    y_j + dy/2 (H), kept in k-space (numpy.fft convention);
+ *  - dEx/dt = (dHz/dy - Jx)/eps, dHz/dt = (dEx/dy)/mu,
+ *    dEz/dt = (-dHx/dy - Jz)/eps, dHx/dt = -(dEz/dy)/mu;
+ *  - d/dy as per-k multipliers (spectral i k exp(+-i k dy/2), or the
+ *    reference's differences); each mode advanced over dt by the exact
+ *    solution of its 2x2 system with J constant over the step;
+ *  - after the step, ApplyABC's real-space multiplication (optional);
+ *  - wire slot s sits on grid point jw[s]: the field there drives the wire
+ *    (broadcast along its propagation grid rr), and its polarization,
+ *    averaged over rr and multiplied by `sheet`, is the grid polarization
+ *    at jw[s]; J = (P_new - P_old) / dt.
+ * Pointers are device pointers; complex values interleaved (re, im). */
+#define PSA_PSTD_MAXW 256
+#define PSA_PSTD_ABC 1 /* flags: apply the absorbing profile after each step */
+
+typedef struct psa_pstd {
+  int32_t N;          /* grid points, a power of two in [64, 8192] */
+  int32_t nw;         /* wire slots, [0, PSA_PSTD_MAXW] */
+  int32_t flags;      /* PSA_PSTD_* */
+  int32_t nobs;       /* observation points recorded per step */
+  double dy;          /* grid spacing (m) */
+  double eps, mu;     /* background permittivity eps0 n0^2 and mu0 (SI) */
+  double dt;          /* step the mode table was built for (s) */
+  double sheet;       /* wire polarization -> grid polarization factor */
+  double *state;      /* [4][N] complex k-space lines: Ex, Hz, Ez, Hx */
+  const double *mode; /* [N][8] per k: ike (re, im), ikh (re, im), cos(w dt),
+                         sin(w dt)/w, (1 - cos(w dt))/w^2, 0; w^2 = -ike ikh/(eps mu) */
+  const double *abc;  /* [N] real absorbing profile (PSA_PSTD_ABC) */
+  const double *twid; /* [N] complex exp(-2 pi i m / N) */
+  const int32_t *jw;  /* [nw] grid point of each wire slot */
+  const int32_t *obs; /* [nobs] recorded grid points */
+  double *pold;       /* [2][nw] complex sheet polarization (x, z) at the step start */
+  double *jsrc;       /* [2][nw] complex current density of the step (x, z) */
+  double *ew;         /* [2][nw] complex Ex, Ez at each wire slot */
+  double *rec;        /* [nrec][2][nobs] complex Ex, Ez at the observation points */
+  int64_t nrec;
+  int64_t *count;     /* [1] steps taken by psa_pstd_advance (device) */
+} psa_pstd;
+
+/* sizeof(psa_pstd), so bindings can check their struct mirror */
+int psa_pstd_size(void);
+/* In place: the four lines of `state`, holding real-space values, become
+ * their forward FFTs. */
+int psa_pstd_to_k(const psa_pstd *p, void *stream);
+/* out[4][N] complex = inverse FFT of each line of `state` (state unchanged). */
+int psa_pstd_to_real(const psa_pstd *p, double *out, void *stream);
+/* ew = Ex, Ez at the wire slots from the current state. */
+int psa_pstd_sample(const psa_pstd *p, void *stream);
+/* One step of dt: sources from jsrc, mode update, absorber, ew, and the
+ * observation record rec[count] (if count < nrec); count++. */
+int psa_pstd_advance(const psa_pstd *p, void *stream);
+/* Wires [w0, w0+nw) of g -> slots [slot0, slot0+nw): sheet polarization
+ * from Pxx, Pzz of g->out averaged over rr; jsrc = (P - pold)/dt; pold = P. */
+int psa_pstd_gather(const psa_pstd *p, const psa_group *g, int32_t w0, int32_t nw,
+                    int32_t slot0, void *stream);
+/* Slots [slot0, slot0+nw) -> g->efield of wires [w0, w0+nw): Exx = Ex and
+ * Ezz = Ez of the slot on every rr point (Eyy untouched). */
+int psa_pstd_scatter(const psa_pstd *p, const psa_group *g, int32_t w0, int32_t nw,
+                     int32_t slot0, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,21 @@ class PsaGroup(ctypes.Structure):
 
 _GP = ctypes.POINTER(PsaGroup)
 
+
+class PsaPstd(ctypes.Structure):
+    """Mirror of ``psa_pstd``; the size is checked against the library."""
+
+    _fields_ = ([(n, c_i32) for n in ("N", "nw", "flags", "nobs")]
+                + [(n, c_dbl) for n in ("dy", "eps", "mu", "dt", "sheet")]
+                + [(n, c_ptr) for n in ("state", "mode", "abc", "twid", "jw", "obs", "pold",
+                                        "jsrc", "ew", "rec")]
+                + [("nrec", ctypes.c_int64), ("count", c_ptr)])
+
+
+_PP = ctypes.POINTER(PsaPstd)
+PSA_PSTD_ABC = 1
+PSA_PSTD_MAXW = 256
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "psa_abi_version": (ctypes.c_int, []),
@@ -64,6 +79,13 @@ SIGNATURES = {
     "psa_step": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
     "psa_drive": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr, c_dbl, c_ptr]),
     "psa_record": (ctypes.c_int, [_GP, c_i32, c_i32, c_i32, c_ptr, c_ptr, c_i32, c_ptr]),
+    "psa_pstd_size": (ctypes.c_int, []),
+    "psa_pstd_to_k": (ctypes.c_int, [_PP, c_ptr]),
+    "psa_pstd_to_real": (ctypes.c_int, [_PP, c_ptr, c_ptr]),
+    "psa_pstd_sample": (ctypes.c_int, [_PP, c_ptr]),
+    "psa_pstd_advance": (ctypes.c_int, [_PP, c_ptr]),
+    "psa_pstd_gather": (ctypes.c_int, [_PP, _GP, c_i32, c_i32, c_i32, c_ptr]),
+    "psa_pstd_scatter": (ctypes.c_int, [_PP, _GP, c_i32, c_i32, c_i32, c_ptr]),
 }
 
 _lib = None
@@ -98,6 +120,9 @@ def load(path=None):
     if lib.psa_group_size() != ctypes.sizeof(PsaGroup):
         raise PsaError(f"psa_group size mismatch: C {lib.psa_group_size()} vs "
                        f"ctypes {ctypes.sizeof(PsaGroup)}")
+    if lib.psa_pstd_size() != ctypes.sizeof(PsaPstd):
+        raise PsaError(f"psa_pstd size mismatch: C {lib.psa_pstd_size()} vs "
+                       f"ctypes {ctypes.sizeof(PsaPstd)}")
     _lib = lib
     return lib
 

@@ -10,6 +10,7 @@
 #include "psa_rhs_bf16.h"
 #include "psa_offdiag.h"
 #include "psa_calch.h"
+#include "psa_pstd.h"
 
 using namespace psa;
 
@@ -2020,6 +2021,112 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
   if ((rc = finish_impl(g, w0, nw, (hipStream_t)stream, fused))) return rc;
   if ((rc = psa_sources(g, w0, nw, stream))) return rc;
   if ((rc = psa_qw2prop(g, w0, nw, stream))) return rc;
+  return PSA_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// 1-D pseudo-spectral propagator (psa_pstd.h)
+// ===========================================================================
+namespace {
+
+int check_pstd(const psa_pstd *p) {
+  if (!p) return fail(PSA_E_INVALID, "null psa_pstd");
+  if (p->N < 64 || p->N > 8192 || (p->N & (p->N - 1)))
+    return fail(PSA_E_INVALID, "psa_pstd: N must be a power of two in [64, 8192]");
+  if (p->nw < 0 || p->nw > PSA_PSTD_MAXW) return fail(PSA_E_INVALID, "psa_pstd: nw outside [0, 256]");
+  if (p->nobs < 0 || p->nrec < 0) return fail(PSA_E_INVALID, "psa_pstd: negative nobs / nrec");
+  if (!p->state || !p->mode || !p->twid || !p->count)
+    return fail(PSA_E_INVALID, "psa_pstd: state, mode, twid and count are required");
+  if (p->nw > 0 && (!p->jw || !p->pold || !p->jsrc || !p->ew))
+    return fail(PSA_E_INVALID, "psa_pstd: wire slots need jw, pold, jsrc and ew");
+  if (p->nobs > 0 && p->nrec > 0 && (!p->obs || !p->rec))
+    return fail(PSA_E_INVALID, "psa_pstd: observation points need obs and rec");
+  if ((p->flags & PSA_PSTD_ABC) && !p->abc) return fail(PSA_E_INVALID, "psa_pstd: PSA_PSTD_ABC needs abc");
+  if (!(p->dy > 0.0) || !(p->eps > 0.0) || !(p->mu > 0.0) || !(p->dt > 0.0))
+    return fail(PSA_E_INVALID, "psa_pstd: dy, eps, mu and dt must be positive");
+  return PSA_OK;
+}
+
+template <int LOGN>
+int launch_pstd_line_n(const psa_pstd &p, int mode, double *out, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(pstd::k_pstd_line<LOGN>, dim3(blocks), dim3(pstd::Fft<LOGN>::T), 0, s, p, mode, out);
+  PSA_LAUNCH_CHECK("k_pstd_line");
+  return PSA_OK;
+}
+
+int launch_pstd_line(const psa_pstd &p, int mode, double *out, hipStream_t s) {
+  const bool e_only = mode == pstd::SAMPLE || (mode == pstd::STEP && !(p.flags & PSA_PSTD_ABC));
+  const int blocks = e_only ? 2 : 4;
+  switch (p.N) {
+    case 64: return launch_pstd_line_n<6>(p, mode, out, blocks, s);
+    case 128: return launch_pstd_line_n<7>(p, mode, out, blocks, s);
+    case 256: return launch_pstd_line_n<8>(p, mode, out, blocks, s);
+    case 512: return launch_pstd_line_n<9>(p, mode, out, blocks, s);
+    case 1024: return launch_pstd_line_n<10>(p, mode, out, blocks, s);
+    case 2048: return launch_pstd_line_n<11>(p, mode, out, blocks, s);
+    case 4096: return launch_pstd_line_n<12>(p, mode, out, blocks, s);
+    case 8192: return launch_pstd_line_n<13>(p, mode, out, blocks, s);
+  }
+  return fail(PSA_E_INVALID, "psa_pstd: unsupported N");
+}
+
+int check_pstd_group(const psa_pstd *p, const psa_group *g, int w0, int nw, int slot0) {
+  if (int rc = check_pstd(p)) return rc;
+  if (int rc = check_group(g, w0, nw)) return rc;
+  if (slot0 < 0 || slot0 + nw > p->nw) return fail(PSA_E_INVALID, "psa_pstd: wire slots outside [0, nw)");
+  if (g->Nx < 1 || !g->out || !g->efield) return fail(PSA_E_INVALID, "psa_pstd: group has no rr grid");
+  return PSA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psa_pstd_size(void) { return (int)sizeof(psa_pstd); }
+
+int psa_pstd_to_k(const psa_pstd *p, void *stream) {
+  if (int rc = check_pstd(p)) return rc;
+  return launch_pstd_line(*p, pstd::TO_K, nullptr, (hipStream_t)stream);
+}
+
+int psa_pstd_to_real(const psa_pstd *p, double *out, void *stream) {
+  if (int rc = check_pstd(p)) return rc;
+  if (!out) return fail(PSA_E_INVALID, "psa_pstd_to_real: null out");
+  return launch_pstd_line(*p, pstd::TO_REAL, out, (hipStream_t)stream);
+}
+
+int psa_pstd_sample(const psa_pstd *p, void *stream) {
+  if (int rc = check_pstd(p)) return rc;
+  if (p->nw == 0) return PSA_OK;
+  return launch_pstd_line(*p, pstd::SAMPLE, nullptr, (hipStream_t)stream);
+}
+
+int psa_pstd_advance(const psa_pstd *p, void *stream) {
+  if (int rc = check_pstd(p)) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(pstd::k_pstd_update, dim3((p->N + pstd::UPD_T - 1) / pstd::UPD_T), dim3(pstd::UPD_T), 0,
+                     s, *p);
+  PSA_LAUNCH_CHECK("k_pstd_update");
+  if (!(p->flags & PSA_PSTD_ABC) && p->nw == 0 && (p->nobs == 0 || p->nrec == 0)) return PSA_OK;
+  return launch_pstd_line(*p, pstd::STEP, nullptr, s);
+}
+
+int psa_pstd_gather(const psa_pstd *p, const psa_group *g, int32_t w0, int32_t nw, int32_t slot0,
+                    void *stream) {
+  if (int rc = check_pstd_group(p, g, w0, nw, slot0)) return rc;
+  hipLaunchKernelGGL(pstd::k_pstd_gather, dim3(nw, 2), dim3(256), 0, (hipStream_t)stream, *p, *g, w0, slot0);
+  PSA_LAUNCH_CHECK("k_pstd_gather");
+  return PSA_OK;
+}
+
+int psa_pstd_scatter(const psa_pstd *p, const psa_group *g, int32_t w0, int32_t nw, int32_t slot0,
+                     void *stream) {
+  if (int rc = check_pstd_group(p, g, w0, nw, slot0)) return rc;
+  hipLaunchKernelGGL(pstd::k_pstd_scatter, dim3((g->Nx + 255) / 256, nw, 2), dim3(256), 0, (hipStream_t)stream,
+                     *p, *g, w0, slot0);
+  PSA_LAUNCH_CHECK("k_pstd_scatter");
   return PSA_OK;
 }
 

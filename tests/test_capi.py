@@ -32,3 +32,4 @@ def test_struct_mirror_matches():
     import ctypes
     assert lib.psa_group_size() == ctypes.sizeof(capi.PsaGroup)
     assert lib.psa_abi_version() == capi.PSA_ABI_VERSION
+    assert lib.psa_pstd_size() == ctypes.sizeof(capi.PsaPstd)
