@@ -218,6 +218,50 @@ def serve_sample(prec, seed, nreq=150, rate=30.0, nks=(64, 128, 256, 384, 512), 
     return out
 
 
+def transmission_sample(seed, nw=16, nk=256, nx=4096, npts=4096, steps=20, warmup=3, dt=10e-18):
+    """Wires in a propagating pulse's path: `nw` FP64 wires of Nk = `nk` on a
+    1-D pseudo-spectral grid of `npts` points of 2 nm (the 8.192 um window),
+    n0 = 3.01662, absorbing layers of 0.5 um, wires every 100 nm from 3 um,
+    sheet factor area / (lz dy) with area 1e-16 m^2, lz 100 nm.  Times the
+    coupled step and, apart, the propagator's share (gather, advance,
+    scatter) with HIP events on the stream they run on."""
+    import torch
+    from pulsesuite_amd.pstd import Pstd1D, sheet_factor
+    wg, _, _ = build_group("fp64", nw, nk, nx, seed)
+    dy = 2e-9
+    p = Pstd1D(npts, dy, 3.01662, dt, wires_y=3.0e-6 + 100e-9 * np.arange(nw),
+               sheet=sheet_factor(1e-16, 100e-9, dy), abc=(0.5e-6, 0.5e-6),
+               obs_y=[0.6e-6, 7.5e-6], nrec=steps + warmup)
+    E, H = p.pulse(1e7, 2.0e-6, 0.3e-6, 800e-9, "x")
+    p.set_fields(E, H)
+    p.attach([(wg, 0)])
+    for _ in range(warmup):
+        p.couple_step()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b, c in ev:
+        a.record()
+        wg.step(dt)
+        b.record()
+        p.gather(wg, 0)
+        p.advance()
+        p.scatter(wg, 0)
+        c.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    wire = np.array([a.elapsed_time(b) for a, b, _ in ev])
+    prop = np.array([b.elapsed_time(c) for _, b, c in ev])
+    out = dict(workload=f"{nw} FP64 wires x Nk={nk} on a {npts}-point pseudo-spectral grid (2 nm), "
+                        f"dt = 10 as, absorbing layers 0.5 um",
+               ms_per_coupled_step=1e3 * wall / steps,
+               wire_step_ms=float(np.median(wire)), propagator_us=float(np.median(prop) * 1e3),
+               propagator_share=float(np.median(prop) / (np.median(prop) + np.median(wire))))
+    del wg, p
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +276,7 @@ def main():
                     "(FP64 is always timed; default fp32,bf16x3)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=150)
+    ap.add_argument("--no-transmission", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only (the SBE wire batch is stepped on a single "
@@ -293,6 +338,12 @@ def main():
             serving = serve_sample(best, seed, nreq=args.serve_requests)
         except Exception as ex:  # report, keep the batch number
             serving = {"error": repr(ex)}
+    if not args.no_transmission:
+        try:
+            print("[bench] wires + propagator", file=sys.stderr, flush=True)
+            res["transmission"] = transmission_sample(seed)
+        except Exception as ex:  # report, keep the batch number
+            res["transmission"] = {"error": repr(ex)}
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

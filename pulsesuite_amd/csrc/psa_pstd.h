@@ -34,13 +34,34 @@ __device__ __forceinline__ int slot(int i) { return i ^ (5 * ((i >> 4) & 3)); }
 template <int LOGN>
 struct Fft {
   static constexpr int N = 1 << LOGN;
-  static constexpr int T = N >= 2048 ? 512 : (N / 4 < 64 ? 64 : N / 4);  // threads
+  static constexpr int T = N >= 4096 ? 1024 : (N / 4 < 64 ? 64 : N / 4);  // threads
+};
+
+// Twiddles w^m = exp(-2 pi i m / N): from the global table, or from an LDS
+// quarter table q[r] = w^r (r < N/4) rotated by (-i)^(m / (N/4)) -- exact
+// sign flips and swaps, so both give the same values.
+struct TwGlobal {
+  const double *t;
+  __device__ __forceinline__ cd operator()(int m) const { return ld(t, m); }
+};
+template <int LOGN>
+struct TwQuarter {
+  const cd *q;
+  __device__ __forceinline__ cd operator()(int m) const {
+    const cd a = q[m & ((1 << (LOGN - 2)) - 1)];
+    switch (m >> (LOGN - 2)) {
+      case 0: return a;
+      case 1: return cd{a.im, -a.re};
+      case 2: return cd{-a.re, -a.im};
+      default: return cd{-a.im, a.re};
+    }
+  }
 };
 
 // x[j + r M] w^(r k N/(4 Ns)) -> DFT4 -> x[(j - k) 4 + k + r Ns], k = j mod Ns;
 // SIGN -1 forward (numpy.fft.fft), +1 inverse without the 1/N.
-template <int LOGN, int SIGN>
-__device__ __forceinline__ void block_fft(cd *x, const double *twid, int tid) {
+template <int LOGN, int SIGN, class TW>
+__device__ __forceinline__ void block_fft(cd *x, const TW &twid, int tid) {
   constexpr int N = Fft<LOGN>::N, T = Fft<LOGN>::T;
   constexpr int M4 = N / 4, IT4 = (M4 + T - 1) / T;
   // stages not unrolled: unrolling lets the compiler hoist later stages'
@@ -59,7 +80,7 @@ __device__ __forceinline__ void block_fft(cd *x, const double *twid, int tid) {
         a[0] = x[slot(j)];
 #pragma unroll
         for (int r = 1; r < 4; ++r) {
-          cd w = ld(twid, r * k * step);
+          cd w = twid(r * k * step);
           if (SIGN > 0) w.im = -w.im;
           a[r] = s == 0 ? x[slot(j + r * M4)] : cmul(x[slot(j + r * M4)], w);
         }
@@ -92,7 +113,7 @@ __device__ __forceinline__ void block_fft(cd *x, const double *twid, int tid) {
     for (int it = 0; it < IT2; ++it) {
       const int j = tid + T * it;
       if (M2 % T == 0 || j < M2) {
-        cd w = ld(twid, j);
+        cd w = twid(j);
         if (SIGN > 0) w.im = -w.im;
         const cd a0 = x[slot(j)], a1 = cmul(x[slot(j + M2)], w);
         x[slot(j)] = cadd(a0, a1);
@@ -118,19 +139,29 @@ __device__ __forceinline__ cd at(const cd *x, int j, double sc) {
 template <int LOGN>
 __global__ __launch_bounds__(Fft<LOGN>::T) void k_pstd_line(psa_pstd p, int mode, double *out) {
   constexpr int N = Fft<LOGN>::N, T = Fft<LOGN>::T;
+  // the quarter twiddle table in LDS up to N = 4096 (80 KB with the line);
+  // N = 8192 reads the global table (the line alone takes 128 KB)
+  constexpr bool QTW = LOGN <= 12;
   __shared__ cd x[N];
+  __shared__ cd tq[QTW ? N / 4 : 1];
   const int tid = threadIdx.x;
+  if constexpr (QTW)
+    for (int i = tid; i < N / 4; i += T) tq[i] = ld(p.twid, i);
+  const auto tw = [&]() {
+    if constexpr (QTW) return TwQuarter<LOGN>{tq};
+    else return TwGlobal{p.twid};
+  }();
   const bool abc = mode == STEP && (p.flags & PSA_PSTD_ABC);
   const int line = (mode == SAMPLE || (mode == STEP && !abc)) ? 2 * blockIdx.x : blockIdx.x;
   double *st_line = p.state + 2L * N * line;
   for (int i = tid; i < N; i += T) x[slot(i)] = ld(st_line, i);
   __syncthreads();
   if (mode == TO_K) {
-    block_fft<LOGN, -1>(x, p.twid, tid);
+    block_fft<LOGN, -1>(x, tw, tid);
     for (int i = tid; i < N; i += T) st(st_line, i, x[slot(i)]);
     return;
   }
-  block_fft<LOGN, +1>(x, p.twid, tid);
+  block_fft<LOGN, +1>(x, tw, tid);
   const double inv = 1.0 / (double)N;
   if (mode == TO_REAL) {
     for (int i = tid; i < N; i += T) st(out + 2L * N * line, i, cscale(x[slot(i)], inv));
@@ -152,7 +183,7 @@ __global__ __launch_bounds__(Fft<LOGN>::T) void k_pstd_line(psa_pstd p, int mode
   }
   if (abc) {
     __syncthreads();
-    block_fft<LOGN, -1>(x, p.twid, tid);
+    block_fft<LOGN, -1>(x, tw, tid);
     for (int i = tid; i < N; i += T) st(st_line, i, x[slot(i)]);
   }
 }
@@ -161,7 +192,7 @@ __global__ __launch_bounds__(Fft<LOGN>::T) void k_pstd_line(psa_pstd p, int mode
 // then for each pair (E, H, s) = (Ex, Hz, +1), (Ez, Hx, -1), with
 // a = s ikh / eps, b = s ike / mu, d = a H - J^/eps:
 //   E' = C E + S d,   H' = H + b (S E + Q d).
-constexpr int UPD_T = 256;
+constexpr int UPD_T = 64;
 __global__ __launch_bounds__(UPD_T) void k_pstd_update(psa_pstd p) {
   __shared__ cd js[2][PSA_PSTD_MAXW];
   __shared__ int jws[PSA_PSTD_MAXW];
@@ -176,10 +207,21 @@ __global__ __launch_bounds__(UPD_T) void k_pstd_update(psa_pstd p) {
   const int k = blockIdx.x * UPD_T + tid;
   if (k >= N) return;
   cd jx{0.0, 0.0}, jz{0.0, 0.0};
-  for (int s = 0; s < nw; ++s) {
-    const cd w = ld(p.twid, (int)(((long)k * jws[s]) & (N - 1)));
-    jx = cadd(jx, cmul(js[0][s], w));
-    jz = cadd(jz, cmul(js[1][s], w));
+  // eight wires at a time, their twiddle loads in flight together
+  for (int s0 = 0; s0 < nw; s0 += 8) {
+    cd w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int s = min(s0 + u, nw - 1);
+      w[u] = ld(p.twid, (int)(((long)k * jws[s]) & (N - 1)));
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (s0 + u < nw) {
+        jx = cadd(jx, cmul(js[0][s0 + u], w[u]));
+        jz = cadd(jz, cmul(js[1][s0 + u], w[u]));
+      }
+    }
   }
   const double *m = p.mode + 8L * k;
   const cd ike{m[0], m[1]}, ikh{m[2], m[3]};
@@ -206,17 +248,24 @@ __global__ __launch_bounds__(256) void k_pstd_gather(psa_pstd p, psa_group g, in
   __shared__ double sh[8];
   const int f = blockIdx.y, w = w0 + blockIdx.x, s = slot0 + blockIdx.x;
   const double *o = g.out + 2L * ((long)(f == 0 ? 0 : 2) * g.Nw + w) * g.Nx;
-  double re = 0.0, im = 0.0;
-  for (int i = threadIdx.x; i < g.Nx; i += 256) {
-    const cd v = ld(o, i);
-    re += v.re;
-    im += v.im;
+  // four independent partial sums per thread (loads in flight together),
+  // combined in a fixed order
+  double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i0 = threadIdx.x; i0 < g.Nx; i0 += 1024) {
+    cd v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = i0 + 256 * u < g.Nx ? ld(o, i0 + 256 * u) : cd{0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      re[u] += v[u].re;
+      im[u] += v[u].im;
+    }
   }
-  re = block_sum(re, sh);
-  im = block_sum(im, sh);
+  const double rs = block_sum((re[0] + re[1]) + (re[2] + re[3]), sh);
+  const double is = block_sum((im[0] + im[1]) + (im[2] + im[3]), sh);
   if (threadIdx.x == 0) {
     const double sc = p.sheet / (double)g.Nx, idt = 1.0 / p.dt;
-    const cd P{re * sc, im * sc};
+    const cd P{rs * sc, is * sc};
     const long at = (long)f * p.nw + s;
     const cd P0 = ld(p.pold, at);
     st(p.jsrc, at, cscale(csub(P, P0), idt));
