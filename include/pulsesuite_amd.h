@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PSA_ABI_VERSION 2
+#define PSA_ABI_VERSION 3
 
 #define PSA_OK 0
 #define PSA_E_INVALID (-1)   /* bad argument / shape */
@@ -128,6 +128,8 @@ typedef struct psa_group {
                                          8 x 2 x max(Nr x 4 Nw, (2N-1) x 3 Nw, bnx x 5 Nw) uses all splits */
   int64_t kscr_len;
   const int32_t *wid_host;            /* [Nw] host copy of wire_id */
+  double *pR, *CR, *DR;               /* [Nw][N][N] complex RK4 stage accumulators (psa_step_rk4,
+                                         psa_rhs modes 2-5 only; NULL otherwise) */
 } psa_group;
 
 /* layout of the per-wire prep block (doubles) */
@@ -199,7 +201,12 @@ int psa_lorentz_table(const psa_group *g, void *stream);
 /* dpdt/dCdt/dDdt (SBEs.py:1941-1965, 2308-2327, 2606-2625) incl. the
  * off-diagonal dephasing OffP (_OffDiagDephasing2_jit, dephasing.py:320-356).
  * mode 0: write dp/dt, dC/dt, dD/dt into pX, CX, DX.
- * mode 1: leapfrog X3 = X1 + 2 dt dX/dt(X2) into pX, CX, DX (SBEs.py:5673-5675). */
+ * mode 1: leapfrog X3 = X1 + 2 dt dX/dt(X2) into pX, CX, DX (SBEs.py:5673-5675).
+ * modes 2-5: classic RK4 stage s = mode - 1 (the stage combination of _rk4,
+ * integrator.py:143-157) with k_s = dX/dt of the stage input in B and the
+ * step start in A: s = 1: R = k1, X = A + dt/2 k1; s = 2: R += 2 k2,
+ * X = A + dt/2 k2; s = 3: R += 2 k3, X = A + dt k3; s = 4:
+ * X = A + dt/6 (R + k4) (X may alias A).  R = pR, CR, DR. */
 int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode,
             double dt, void *stream);
 
@@ -246,6 +253,18 @@ int psa_record(const psa_group *g, int32_t w0, int32_t nw, int32_t xmid, double 
  * for wires [w0, w0+nw): prop2qw, prep, calc_h, offdiag, rhs (leapfrog),
  * relaxation, transport, finish, sources, qw2prop. */
 int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream);
+
+/* The same step with classic RK4 (the stage combination of _rk4,
+ * integrator.py:143-157) in place of the leapfrog + average.  The reference
+ * has no RK4 on the wire path; it integrates with the leapfrog of psa_step.
+ * The state at the step start is level B (pB, CB, DB) and the new state is
+ * written back there; the four RHS evaluations (prep, calc_h, offdiag, rhs on
+ * the stage input) use the field of the step start; then relaxation (dt),
+ * transport (dt), the diagonal renormalisation (no average), sources and
+ * qw2prop on the new state, with the screening of the last stage.  Levels A
+ * and X are stage scratch afterwards (a later leapfrog step needs A = B);
+ * pR, CR, DR are required. */
+int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream);
 
 /* ---- 1-D pseudo-spectral field propagator coupled to the wires ----
  *

@@ -983,12 +983,27 @@ def prop2qw(g, t, rr, Exx):
 # Solver (SBEs.py:4619-5985): state, one step, QWCalculator
 # ---------------------------------------------------------------------------
 
+def rk4(y, dydt, h, D):
+    """Classic RK4 step (_rk4, libpulsesuite/integrator.py:143-157):
+    returns y + h/6 (k1 + k4 + 2 (k2 + k3)) with dydt = k1 and D(y) the
+    derivative (autonomous here: the wire step holds the field)."""
+    ytmp = y + (h / 2.0) * dydt
+    dyt = D(ytmp)
+    ytmp = y + (h / 2.0) * dyt
+    dym = D(ytmp)
+    ytmp = y + h * dym
+    dym = dym + dyt
+    dyt = D(ytmp)
+    return y + (h / 6.0) * (dydt + dyt + 2.0 * dym)
+
+
 class Solver:
     """Mirror of SBESolver for parity: one N, Nw wires, state in the
     reference's [N, N, Nw] layout."""
 
-    def __init__(self, prm, Nw, rr, Emax0, tables=None, grid=None):
+    def __init__(self, prm, Nw, rr, Emax0, tables=None, grid=None, integrator="leapfrog"):
         self.prm = prm
+        self.integrator = integrator
         self.g = grid if grid is not None else Grid(prm)
         self.t = tables if tables is not None else Tables(self.g)
         g = self.g
@@ -1157,6 +1172,66 @@ class Solver:
                          Px=Px, Py=Py, Pz=Pz)
         return Px, Py, Pz, re, rh
 
+    def rk4_step(self, Ex, Ey, Ez, dt, w):
+        """The wire step with classic RK4 (rk4, the restatement of _rk4 at
+        integrator.py:143-157) for the coherent part in place of the leapfrog
+        + average of SBEs.py:5673-5706: four RHS evaluations of the state
+        (level 3) with the field of the step start; relaxation with the
+        screening of the last stage, transport and the diagonal
+        renormalisation on the new state, which is also what CalcI0 and the
+        sources read.  The new state goes to levels 2 and 3."""
+        g, t, prm = self.g, self.t, self.prm
+        N = g.Nk
+        i = w - 1
+        y = np.array([self.YY3[:, :, i], self.CC3[:, :, i], self.DD3[:, :, i]])
+        last = {}
+
+        def D(ys):
+            dp, dC, dD, pr = self.rhs(ys[0], ys[1], ys[2], Ex, Ey, Ez, w)
+            last["pr"] = pr
+            return np.array([dp, dC, dD])
+
+        y = rk4(y, D(y), dt, D)
+        pr = last["pr"]
+        p3, C3, D3 = y[0].copy(), y[1].copy(), y[2].copy()
+        if prm.EHs or prm.Phonon:
+            ne3, nh3 = self.relaxation(np.diag(C3).copy(), np.diag(D3).copy(),
+                                       pr["VC"], pr["E1D"], pr["Rsp"], dt)
+            C3[np.diag_indices(N)] = ne3
+            D3[np.diag_indices(N)] = nh3
+        if prm.DCTrans:
+            Eac = np.real(Ex[g.NQ0])
+            C3 = transport(t, C3, prm.Edc, Eac, dt)
+            D3 = transport(t, D3, prm.Edc, Eac, dt)
+            p3 = transport(t, p3, prm.Edc, Eac, dt)
+        ne3 = np.abs(np.diag(C3)).astype(complex)
+        nh3 = np.abs(np.diag(D3)).astype(complex)
+        total = (np.sum(np.abs(ne3)) + np.sum(np.abs(nh3))) / 2.0
+        ne3 = ne3 * total / (np.sum(np.abs(ne3)) + SMALL)
+        nh3 = nh3 * total / (np.sum(np.abs(nh3)) + SMALL)
+        C3[np.diag_indices(N)] = ne3
+        D3[np.diag_indices(N)] = nh3
+        Nr = g.Nr
+        z = np.zeros(Nr, dtype=complex)
+        Px, Py, Pz = z, z, z
+        if prm.Optics:
+            Px, Py, Pz = qw_polarization3(g, t, p3, w)
+        if prm.LF:
+            re, rh = qw_rho5(g, t, C3, D3, ne3, nh3)
+            re = 2.0 * e0 * re / g.area * g.ehint
+            rh = 2.0 * e0 * rh / g.area * g.ehint
+        else:
+            re, rh = z, z
+        self.I0[i] = np.real(calc_i0(g, ne3, nh3, pr["VC"]))
+        for lv in (self.YY2, self.YY3):
+            lv[:, :, i] = p3
+        for lv in (self.CC2, self.CC3):
+            lv[:, :, i] = C3
+        for lv in (self.DD2, self.DD3):
+            lv[:, :, i] = D3
+        self.last = dict(pr, ne3=ne3, nh3=nh3, re=re, rh=rh, Px=Px, Py=Py, Pz=Pz)
+        return Px, Py, Pz, re, rh
+
     def qw2prop(self, Px, Py, Pz, re, rh):
         """qwoptics.py:717-805 (LF branch)."""
         g = self.g
@@ -1201,7 +1276,8 @@ class Solver:
         Ex = prop2qw(self.g, self.t, self.rr, Exx)
         Ey = prop2qw(self.g, self.t, self.rr, Eyy)
         Ez = prop2qw(self.g, self.t, self.rr, Ezz)
-        Px, Py, Pz, re, rh = self.sbe_step(Ex, Ey, Ez, dt, w)
+        step = self.rk4_step if self.integrator == "rk4" else self.sbe_step
+        Px, Py, Pz, re, rh = step(Ex, Ey, Ez, dt, w)
         Pxx, Pyy, Pzz, RhoE, RhoH = self.qw2prop(Px, Py, Pz, re, rh)
         Rho = RhoH - RhoE
         if prm.ignorewire:

@@ -10,7 +10,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libpulsesuite_amd.so")
 
-PSA_ABI_VERSION = 2
+PSA_ABI_VERSION = 3
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 c_i32 = ctypes.c_int32
@@ -36,7 +36,8 @@ class PsaGroup(ctypes.Structure):
                 + [(n, c_ptr) for n in _TABS]
                 + [("mb_off", c_ptr * 8), ("mb_idx", c_ptr * 8), ("mb_val", c_ptr * 8)]
                 + [(n, c_ptr) for n in _WIRE]
-                + [("kscr_len", ctypes.c_int64), ("wid_host", c_ptr)])
+                + [("kscr_len", ctypes.c_int64), ("wid_host", c_ptr)]
+                + [(n, c_ptr) for n in ("pR", "CR", "DR")])
 
 
 _GP = ctypes.POINTER(PsaGroup)
@@ -77,6 +78,7 @@ SIGNATURES = {
     "psa_prop2qw": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_qw2prop": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_step": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
+    "psa_step_rk4": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
     "psa_drive": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr, c_dbl, c_ptr]),
     "psa_record": (ctypes.c_int, [_GP, c_i32, c_i32, c_i32, c_ptr, c_ptr, c_i32, c_ptr]),
     "psa_pstd_size": (ctypes.c_int, []),

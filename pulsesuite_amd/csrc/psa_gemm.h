@@ -444,10 +444,29 @@ struct RhsProb {
     double *out = (e == 0 ? g.pX : (e == 1 ? g.CX : g.DX)) + o;
     if (mode == 0) {
       st(out, ij, d);
-    } else {
+    } else if (mode == 1) {
       const double *X1 = (e == 0 ? g.pA : (e == 1 ? g.CA : g.DA)) + o;
       const cd x1 = ld(X1, ij);
       st(out, ij, cd{x1.re + d.re * dt * 2.0, x1.im + d.im * dt * 2.0});
+    } else {
+      // RK4 stage mode - 1 (_rk4, integrator.py:143-157): A = step start,
+      // R = k1 + 2 k2 + 2 k3 so far; out may alias A (same element only)
+      const cd x0 = ld((e == 0 ? g.pA : (e == 1 ? g.CA : g.DA)) + o, ij);
+      double *R = (e == 0 ? g.pR : (e == 1 ? g.CR : g.DR)) + o;
+      if (mode == 5) {
+        const cd r = ld(R, ij);
+        const double h6 = dt / 6.0;
+        st(out, ij, cd{x0.re + h6 * (r.re + d.re), x0.im + h6 * (r.im + d.im)});
+      } else {
+        const double c = mode == 4 ? dt : 0.5 * dt;
+        if (mode == 2) {
+          st(R, ij, d);
+        } else {
+          const cd r = ld(R, ij);
+          st(R, ij, cd{r.re + 2.0 * d.re, r.im + 2.0 * d.im});
+        }
+        st(out, ij, cd{x0.re + c * d.re, x0.im + c * d.im});
+      }
     }
   }
 };

@@ -1766,7 +1766,9 @@ int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream)
 
 int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
-  if (mode != 0 && mode != 1) return fail(PSA_E_INVALID, "psa_rhs: mode must be 0 or 1");
+  if (mode < 0 || mode > 5) return fail(PSA_E_INVALID, "psa_rhs: mode must be in [0, 5]");
+  if (mode >= 2 && (!g->pR || !g->CR || !g->DR))
+    return fail(PSA_E_INVALID, "psa_rhs: RK4 stages need pR, CR, DR");
   RhsProb p;
   p.g = *g;
   p.w0 = w0;
@@ -2021,6 +2023,39 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
   if ((rc = finish_impl(g, w0, nw, (hipStream_t)stream, fused))) return rc;
   if ((rc = psa_sources(g, w0, nw, stream))) return rc;
   if ((rc = psa_qw2prop(g, w0, nw, stream))) return rc;
+  return PSA_OK;
+}
+
+int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  if (!g->pR || !g->CR || !g->DR) return fail(PSA_E_INVALID, "psa_step_rk4: needs pR, CR, DR");
+  const hipStream_t s = (hipStream_t)stream;
+  int rc;
+  if ((rc = psa_prop2qw(g, w0, nw, stream))) return rc;
+  // stage buffers: the state S = B stays the base; stage inputs alternate
+  // between X and A; the last stage writes the new state over S in place
+  double *const S[3] = {g->pB, g->CB, g->DB}, *const X[3] = {g->pX, g->CX, g->DX},
+                *const A[3] = {g->pA, g->CA, g->DA};
+  double *const *in[4] = {S, X, A, X}, *const *out[4] = {X, A, X, S};
+  psa_group q = *g;
+  for (int st = 0; st < 4; ++st) {
+    q.pA = S[0], q.CA = S[1], q.DA = S[2];
+    q.pB = in[st][0], q.CB = in[st][1], q.DB = in[st][2];
+    q.pX = out[st][0], q.CX = out[st][1], q.DX = out[st][2];
+    if ((rc = psa_prep(&q, w0, nw, stream))) return rc;
+    if ((rc = psa_calc_h(&q, w0, nw, stream))) return rc;
+    if (q.flags & PSA_F_OFFDIAGDPH)
+      if ((rc = psa_offdiag_tables(&q, w0, nw, stream))) return rc;
+    if ((rc = psa_rhs(&q, w0, nw, 2 + st, dt, stream))) return rc;
+  }
+  // q: A = X = the new state.  No leapfrog average: the renormalised
+  // diagonal is the state, and CalcI0 reads it through A.
+  if ((rc = psa_relaxation(&q, w0, nw, dt, stream))) return rc;
+  if ((q.flags & PSA_F_DCTRANS) && (rc = transport_impl(&q, w0, nw, dt, s, false))) return rc;
+  hipLaunchKernelGGL(k_finish_diag, dim3(nw), dim3(256), 0, s, q, w0, 0);
+  PSA_LAUNCH_CHECK("k_finish_diag");
+  if ((rc = psa_sources(&q, w0, nw, stream))) return rc;
+  if ((rc = psa_qw2prop(&q, w0, nw, stream))) return rc;
   return PSA_OK;
 }
 

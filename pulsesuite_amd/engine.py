@@ -38,10 +38,13 @@ class WireGroup:
     rr     -- propagation grid the fields live on (QWCalculator's rr)
     wire_ids -- 1-based reference wire index per wire (sets yw(w), (-1)^w)
     prec   -- "fp64" (reference-equivalent), "fp32" or "bf16x3"
+    integrator -- "leapfrog" (the reference's leapfrog + average) or "rk4"
+               (classic RK4, psa_step_rk4: the state is level 3 and stays
+               there; level 2 is stage scratch)
     """
 
     def __init__(self, prm, nw, rr, wire_ids=None, prec="fp64", device="cuda", host_mb=False,
-                 lor_table=False):
+                 lor_table=False, integrator="leapfrog"):
         self.lib = capi.load()
         if not torch.cuda.is_available():
             raise capi.PsaError("no GPU visible: the device path has no CPU fallback")
@@ -50,6 +53,9 @@ class WireGroup:
         self.nw = int(nw)
         self.rr = np.asarray(rr, dtype=float)
         self.prec = prec
+        if integrator not in ("leapfrog", "rk4"):
+            raise ValueError(f"unknown integrator {integrator!r}")
+        self.integrator = integrator
         if prec not in capi.PREC:
             raise ValueError(f"unknown precision {prec!r}")
         if prm.flags.get("FreePot"):
@@ -111,6 +117,9 @@ class WireGroup:
         self.s["part"] = z(nwz, 256, dt=F64)
         # split-bf16 operand planes (bf16x3 only): [Nw][12 slots] of 4 planes
         # on a zero-padded (128 ceil(N/128)) x (32 ceil(N/32)) grid, blocked
+        if integrator == "rk4":  # stage accumulators R = k1 + 2 k2 + 2 k3
+            for k in ("pR", "CR", "DR"):
+                self.s[k] = z(nwz, N, N)
         nspl = nwz * 12 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) if prec == "bf16x3" else 1
         self.s["spl"] = torch.zeros(nspl, dtype=torch.int16, device=d)
         # split-K partial sums of the source / propagation-grid products
@@ -398,8 +407,12 @@ class WireGroup:
 
     def step(self, dt, w0=0, nw=None):
         """One full QWCalculator step (without the host bookkeeping) for
-        wires [w0, w0+nw) with the fields already in ``efield``."""
+        wires [w0, w0+nw) with the fields already in ``efield``: the
+        reference's leapfrog + average, or RK4 for an "rk4" group."""
         w0, nw = self._range(w0, nw)
+        if self.integrator == "rk4":
+            capi.check(self.lib.psa_step_rk4(ctypes.byref(self.S), w0, nw, dt, self._stream()))
+            return
         capi.check(self.lib.psa_step(ctypes.byref(self.S), w0, nw, dt, self._stream()))
         self.advance_levels(w0, nw)
 

@@ -305,3 +305,68 @@ def test_mbce_brute_force_loops_tiny_n(golden_dir):
     a, b = O.mbce(g, S_eh, S_ee, ne0, nh0, VC)
     assert rel(a, Win) < 1e-14 and rel(b, Wout) < 1e-14
 
+
+
+# ---------------------------------------------------------------------------
+# RK4 (the restatement of _rk4, libpulsesuite/integrator.py:143-157), checked
+# as the reference's own tests check it (tests/libpulsesuite/
+# test_integrator.py:153-201) and against the exact amplification factor
+# ---------------------------------------------------------------------------
+
+def _t4(z):
+    return 1.0 + z + z**2 / 2.0 + z**3 / 6.0 + z**4 / 24.0
+
+
+def test_rk4_restatement():
+    decay = lambda y: -y  # noqa: E731
+    y = O.rk4(np.array([1.0]), np.array([-1.0]), 0.01, decay)
+    assert np.isclose(y[0], np.exp(-0.01), rtol=1e-10)
+    rot = lambda y: 1j * y  # noqa: E731
+    y = O.rk4(np.array([1.0 + 0j]), np.array([1j]), 0.001, rot)
+    assert np.isclose(abs(y[0]), 1.0, rtol=1e-8)
+    e1 = abs(O.rk4(np.array([1.0]), np.array([-1.0]), 0.1, decay)[0] - np.exp(-0.1))
+    e2 = abs(O.rk4(np.array([1.0]), np.array([-1.0]), 0.05, decay)[0] - np.exp(-0.05))
+    assert e1 / e2 > 10
+    ho = lambda y: np.array([y[1], -y[0]])  # noqa: E731
+    y = np.array([1.0, 0.0])
+    for _ in range(1000):
+        y = O.rk4(y, ho(y), 0.01, ho)
+    assert np.isclose(y[0] ** 2 + y[1] ** 2, 1.0, rtol=1e-6)
+    # linear y' = lam y: one step multiplies by the degree-4 Taylor polynomial
+    lam = np.array([-0.3 + 2.1j, -1.7 - 0.4j, 2.9j])
+    y0 = np.array([1.0 + 0.5j, -0.2j, 3.0])
+    y = O.rk4(y0, lam * y0, 0.7, lambda v: lam * v)
+    assert np.max(np.abs(y - _t4(0.7 * lam) * y0)) < 1e-14 * np.max(np.abs(y0))
+
+
+def test_rk4_wire_step_free_rotation():
+    """All mb.params terms off: dp = (-i (Ee[ke] + gap + Eh[kh]) / hbar -
+    gam_e - gam_h) p, dC = -i (Ee[k2] - Ee[k1]) / hbar C, dD likewise with
+    Eh (the free-carrier Hamiltonian and the constant dephasing), so three
+    RK4 wire steps multiply every element by T4(dt lam)^3."""
+    from scipy.constants import hbar
+    from tests.helpers import params_pair, random_state, wire_length
+    N = 16
+    L = wire_length(N)
+    _, oprm = params_pair(L, (0,) * 15)
+    s = O.Solver(oprm, 2, np.linspace(-L, L, 32), 0.0, integrator="rk4")
+    st = random_state(np.random.default_rng(4), N, 2)
+    # equal electron and hole totals: the diagonal renormalisation
+    # (SBEs.py:5692-5702) is then the identity
+    for c, d in ((st[0], st[1]), (st[3], st[4])):
+        for w in range(2):
+            np.fill_diagonal(d[:, :, w], np.diag(c[:, :, w]))
+    for dst, src in zip((s.CC2, s.DD2, s.YY2, s.CC3, s.DD3, s.YY3), st):
+        dst[:] = src
+    g = s.g
+    lp = -1j * (g.Ee[None, :] + oprm.gap + g.Eh[:, None]) / hbar - (oprm.gam_e + oprm.gam_h)
+    lc = -1j * (g.Ee[None, :] - g.Ee[:, None]) / hbar
+    ld = -1j * (g.Eh[None, :] - g.Eh[:, None]) / hbar
+    dt = 1e-15
+    z = np.zeros(32, dtype=complex)
+    for _ in range(3):
+        for w in (1, 2):
+            s.qw_calculator(z, z, z, dt, w)
+    for lam, x0, x in ((lp, st[5], s.YY3), (lc, st[3], s.CC3), (ld, st[4], s.DD3)):
+        ref = _t4(dt * lam)[:, :, None] ** 3 * x0
+        assert np.max(np.abs(x - ref)) < 1e-13 * np.max(np.abs(ref))
