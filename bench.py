@@ -69,13 +69,13 @@ def make_fields(rng, rr, nw, L):
     return Exx, 2.0 * Exx, np.zeros_like(Exx)
 
 
-def build_group(prec, nw, N, nx, seed, tune=False):
+def build_group(prec, nw, N, nx, seed, tune=False, integrator="leapfrog"):
     from pulsesuite_amd.engine import WireGroup
     from pulsesuite_amd.tables import QWParams
     L = wire_length(N)
     prm = QWParams(L=L, me_rel=0.067)
     rr = np.linspace(-(nx - 1) * 1e-9 / 2, (nx - 1) * 1e-9 / 2, nx)
-    wg = WireGroup(prm, nw, rr, prec=prec)
+    wg = WireGroup(prm, nw, rr, prec=prec, integrator=integrator)
     rng = np.random.default_rng(seed)
     wg.set_state(*seeded_state(rng, wg.N, nw))
     Exx, Eyy, Ezz = make_fields(rng, rr, nw, L)
@@ -277,6 +277,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--no-transmission", action="store_true")
+    ap.add_argument("--no-rk4", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only (the SBE wire batch is stepped on a single "
@@ -331,6 +332,19 @@ def main():
     achieved = flops_rhs / t_rhs / 1e12
     peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS}[best]
     res["reported_mode"] = best
+    if not args.no_rk4:
+        try:  # the RK4 integrator on the same workload (four RHS evaluations per step)
+            print(f"[bench] rk4 {best}", file=sys.stderr, flush=True)
+            wr, _, _ = build_group(best, args.wires, args.nk, args.nx, seed, tune=True, integrator="rk4")
+            wall_r, lat_r = time_steps(wr, max(args.steps // 4, 5), 2, 0.1e-15)
+            nr = max(args.steps // 4, 5)
+            res["rk4"] = {"mode": best, "steps_per_s": nr / wall_r, "ms_per_step": 1e3 * wall_r / nr,
+                          "p50_ms": float(np.percentile(lat_r, 50)),
+                          "cost_vs_leapfrog_step": (1e3 * wall_r / nr) / ms}
+            del wr
+            torch.cuda.empty_cache()
+        except Exception as ex:  # report, keep the batch number
+            res["rk4"] = {"error": repr(ex)}
     wg, prm, rr = build_group("fp64", args.wires, args.nk, args.nx, seed) if not args.no_cpu else (None, prm, rr)
     serving = None
     if not args.no_serve:
