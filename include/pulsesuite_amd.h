@@ -266,6 +266,16 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
  * pR, CR, DR are required. */
 int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream);
 
+/* chi(q, w) table that SBESolver.__init__ records (_record_Xqw,
+ * SBEs.py:5395-5432) through SBESolver.calc_Xqw (SBEs.py:5331-5357 ->
+ * _CalcXqw_jit 3165-3243), on the device: out[q][iw] complex for q in
+ * [0, N], w = iw dw, iw < nw; x = 2 dkr / (2 pi area) dcv^2 / eps0,
+ * hfloor = the 0.1 meV dephasing floor (J).  Ee, Eh, fe, fh, game, gamh:
+ * [N] device arrays; N <= 1023. */
+int psa_xqw_table(int32_t N, const double *Ee, const double *Eh, const double *fe, const double *fh,
+                  const double *game, const double *gamh, double gap, double x, double hbar,
+                  double hfloor, double dw, int32_t nw, double *out, void *stream);
+
 /* ---- 1-D pseudo-spectral field propagator coupled to the wires ----
  *
  * The reference drives the wires with spatially uniform analytic pulses

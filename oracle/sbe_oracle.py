@@ -983,6 +983,31 @@ def prop2qw(g, t, rr, Exx):
 # Solver (SBEs.py:4619-5985): state, one step, QWCalculator
 # ---------------------------------------------------------------------------
 
+def calc_xqw(iq, w, dkr, fe, fh, Ee, Eh, gap, area, game, gamh, dcv):
+    """Linear susceptibility chi(q, w) of the wire (_CalcXqw_jit,
+    SBEs.py:3165-3243, called through SBESolver.calc_Xqw 5331-5357):
+    dephasing floored at 0.1 meV, Ec = gap + Ee, Ev = -Eh, fc = fe,
+    fv = 1 - fh, x = 2 dkr / (2 pi area) dcv^2 / eps0, and over
+    k in [max(0, -iq), min(N, N - iq)) the absorption sum
+    x (fc[k] - fv[k+q]) / (Ev[k+q] - Ec[k] - hbar w - i (hgh[k+q] + hge[k]))
+    followed by the emission sum
+    x (fv[k] - fc[k+q]) / (Ec[k+q] - Ev[k] - hbar w - i (hge[k+q] + hgh[k]))."""
+    N = len(Ee)
+    floor = 1e-4 * e0
+    hge = np.maximum(floor, hbar * np.asarray(game))
+    hgh = np.maximum(floor, hbar * np.asarray(gamh))
+    Ec, Ev = gap + Ee, -Eh
+    fc, fv = fe, 1.0 - fh
+    x = 2.0 * dkr / twopi / area * dcv * dcv / eps0
+    k0, k1 = (0, N - iq) if iq >= 0 else (-iq, N)
+    X = 0j
+    for k in range(k0, k1):
+        X += x * (fc[k] - fv[k + iq]) / (Ev[k + iq] - Ec[k] - hbar * w - 1j * (hgh[k + iq] + hge[k]))
+    for k in range(k0, k1):
+        X += x * (fv[k] - fc[k + iq]) / (Ec[k + iq] - Ev[k] - hbar * w - 1j * (hge[k + iq] + hgh[k]))
+    return X
+
+
 def rk4(y, dydt, h, D):
     """Classic RK4 step (_rk4, libpulsesuite/integrator.py:143-157):
     returns y + h/6 (k1 + k4 + 2 (k2 + k3)) with dydt = k1 and D(y) the

@@ -110,7 +110,58 @@ class SBESolver:
                     open(f"dataQW/{nm}.{w:02d}.t.dat", "w").close()
             os.makedirs("dataQW/Wire/info", exist_ok=True)
             open("dataQW/Wire/info/ETHz.t.dat", "w").close()
+            self.record_xqw(np.zeros(self.Nk), np.zeros(self.Nk), np.full(self.Nk, self.gam_e),
+                            np.full(self.Nk, self.gam_h), 0)
         self.start = True
+
+    def xqw_table(self, fe, fh, game, gamh):
+        """chi(q, w) for q = 0..Nk (units of dkr) and w = iw dw, dw = 2 gap /
+        hbar / 2000 (SBEs.py:5395-5407), on the device (psa_xqw_table).
+        Returns (w [nw], X [Nk+1][nw] complex)."""
+        import ctypes
+        from . import capi
+        lib = self.group.lib
+        wmax = self.gap / hbar * 2.0
+        dw = wmax / 2000.0
+        nw = int(wmax / dw) + 1
+        N = self.Nk
+        dev = self.group.dev
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=float), device=dev)  # noqa: E731
+        args = [T(self.Ee), T(self.Eh), T(fe), T(fh), T(game), T(gamh)]
+        out = torch.empty((N + 1, nw), dtype=torch.complex128, device=dev)
+        x = 2.0 * self.dkr / twopi / self.area * self.dcv * self.dcv / eps0
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        capi.check(lib.psa_xqw_table(N, *[a.data_ptr() for a in args], self.gap, x, hbar,
+                                     1e-4 * e0, dw, nw, out.data_ptr(), stream))
+        return np.arange(nw) * dw, out.cpu().numpy()
+
+    def record_xqw(self, fe, fh, game, gamh, ind):
+        """_record_Xqw (SBEs.py:5395-5432): dataQW/Wire/Xqw/Xqw.<ind>.dat,
+        lines "w Re Im" (16.6e) with q fastest, and Xqw.params once when the
+        Xqwparams flag is set."""
+        w, X = self.xqw_table(fe, fh, game, gamh)
+        os.makedirs("dataQW/Wire/Xqw", exist_ok=True)
+        nq = X.shape[0]
+        rows = np.empty((len(w) * nq, 3))
+        rows[:, 0] = np.repeat(w, nq)
+        rows[:, 1] = X.T.real.reshape(-1)
+        rows[:, 2] = X.T.imag.reshape(-1)
+        np.savetxt(f"dataQW/Wire/Xqw/Xqw.{ind:06d}.dat", rows, fmt="%16.6e %16.6e %16.6e")
+        if getattr(self, "Xqwparams", False):
+            wmax = self.gap / hbar * 2.0
+            dw = wmax / 2000.0
+            with open("dataQW/Wire/Xqw/Xqw.params", "w", encoding="utf-8") as f:
+                f.write(f"Nq = {self.Nk + 1}\n")
+                f.write(f"Nw = {len(w)}\n")
+                f.write(" \n")
+                f.write(f"hbar dw (eV) = {dw * hbar / e0}\n")
+                f.write(f"dq (rad/m) = {self.dkr}\n")
+                f.write(" \n")
+                f.write("wmin = 0.0\n")
+                f.write(f"wmax = {wmax * hbar / e0}\n")
+                f.write("qmin = 0.0\n")
+                f.write(f"qmax = {self.Nk * self.dkr}\n")
+            self.Xqwparams = False
 
     # ------------------------------------------------------------------
     # reference-style state access ([N, N, Nw], host copies)

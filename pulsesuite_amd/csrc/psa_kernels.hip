@@ -2062,6 +2062,68 @@ int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *st
 }  // extern "C"
 
 // ===========================================================================
+// chi(q, w) table recorded at start-up (_record_Xqw, SBEs.py:5395-5432) by
+// _CalcXqw_jit (SBEs.py:3165-3243): one thread per (q, w), q = blockIdx.y in
+// [0, N], w = iw dw.  The two sums run in the reference's order; each term
+// is a real numerator over a complex denominator, divided as CPython
+// divides complex numbers (Smith's scaling).
+// ===========================================================================
+namespace {
+
+__device__ __forceinline__ cd real_over(double num, double dr, double di) {
+  if (fabs(dr) >= fabs(di)) {
+    const double r = di / dr, den = dr + di * r;
+    return cd{num / den, (-num * r) / den};
+  }
+  const double r = dr / di, den = dr * r + di;
+  return cd{(num * r) / den, -num / den};
+}
+
+__global__ __launch_bounds__(256) void k_xqw(int N, const double *Ee, const double *Eh, const double *fe,
+                                             const double *fh, const double *game, const double *gamh,
+                                             double gap, double x, double hbar, double hfloor, double dw,
+                                             int nw, double *out) {
+  __shared__ double sEc[1024], sEv[1024], sfc[1024], sfv[1024], shge[1024], shgh[1024];
+  for (int k = threadIdx.x; k < N; k += 256) {
+    sEc[k] = gap + Ee[k];
+    sEv[k] = -Eh[k];
+    sfc[k] = fe[k];
+    sfv[k] = 1.0 - fh[k];
+    shge[k] = fmax(hfloor, hbar * game[k]);
+    shgh[k] = fmax(hfloor, hbar * gamh[k]);
+  }
+  __syncthreads();
+  const int iq = blockIdx.y, iw = blockIdx.x * 256 + threadIdx.x;
+  if (iw >= nw) return;
+  const double hw = hbar * ((double)iw * dw);
+  double re = 0.0, im = 0.0;
+  for (int k = 0; k < N - iq; ++k) {
+    const cd t = real_over(x * (sfc[k] - sfv[k + iq]), sEv[k + iq] - sEc[k] - hw, -(shgh[k + iq] + shge[k]));
+    re += t.re;
+    im += t.im;
+  }
+  for (int k = 0; k < N - iq; ++k) {
+    const cd t = real_over(x * (sfv[k] - sfc[k + iq]), sEc[k + iq] - sEv[k] - hw, -(shge[k + iq] + shgh[k]));
+    re += t.re;
+    im += t.im;
+  }
+  st(out, (long)iq * nw + iw, cd{re, im});
+}
+
+}  // namespace
+
+extern "C" int psa_xqw_table(int32_t N, const double *Ee, const double *Eh, const double *fe, const double *fh,
+                             const double *game, const double *gamh, double gap, double x, double hbar,
+                             double hfloor, double dw, int32_t nw, double *out, void *stream) {
+  if (N < 2 || N > 1023 || nw < 1 || !Ee || !Eh || !fe || !fh || !game || !gamh || !out)
+    return fail(PSA_E_INVALID, "psa_xqw_table: bad arguments");
+  hipLaunchKernelGGL(k_xqw, dim3((nw + 255) / 256, N + 1), dim3(256), 0, (hipStream_t)stream, N, Ee, Eh, fe, fh,
+                     game, gamh, gap, x, hbar, hfloor, dw, nw, out);
+  PSA_LAUNCH_CHECK("k_xqw");
+  return PSA_OK;
+}
+
+// ===========================================================================
 // 1-D pseudo-spectral propagator (psa_pstd.h)
 // ===========================================================================
 namespace {

@@ -26,9 +26,13 @@ recorded (composed and index-heavy behaviour):
 * ``QWOptics.QWRho5`` (``qwoptics.py:942-1071``);
 * ``SBESolver.QWCalculator`` trajectories (``SBEs.py:5826-5985``), with the
   shipped ``mb.params`` and with the many-body relaxation/transport switched
-  off so the leapfrog RHS can be read back from the state.
+  off so the leapfrog RHS can be read back from the state;
+* ``SBESolver.calc_Xqw`` -> ``_CalcXqw_jit`` (``SBEs.py:5331-5357``,
+  ``3165-3243``) and the chi(q, w) files of
+  ``_record_Xqw`` (``SBEs.py:5395-5432``).
 
-Usage: ``python tests/golden/make_golden.py`` (writes ``tests/golden/*.npz``).
+Usage: ``python tests/golden/make_golden.py`` (writes ``tests/golden/*.npz``;
+``python tests/golden/make_golden.py xqw`` writes only ``ref_xqw.npz``).
 """
 
 import contextlib
@@ -279,14 +283,70 @@ def run_case(nk, seed, mb, nsteps, tag, nx=64, drr=10e-9):
     print("wrote", path, "N =", nk)
 
 
+def xqw_case(nk=16, seed=5, nx=64, drr=10e-9):
+    """SBESolver.calc_Xqw (SBEs.py:5331-5357 -> 3165-3243) at random occupations and dephasing for
+    a spread of (iq, w), and the head and tail of the chi(q, w) file and the
+    Xqw.params file that SBESolver.__init__ writes (_record_Xqw,
+    SBEs.py:5395-5432)."""
+    rng = np.random.default_rng(seed)
+    L = wire_length(nk)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            write_params(d, L, MB_SHIPPED)
+            buf = io.StringIO()
+            with contextlib.redirect_stdout(buf):
+                from pulsesuite.PSTD3D import SBEs
+                from pulsesuite.PSTD3D.typespace import GetKArray, GetSpaceArray
+                rr = GetSpaceArray(nx, (nx - 1) * drr)
+                qrr = GetKArray(nx, nx * drr)
+                s = SBEs.SBESolver(qrr, rr, 0.0, 1e7, 800e-9, 1, True)
+            n = s.Nk
+            assert n == nk
+            fe = rng.uniform(0.0, 0.6, n)
+            fh = rng.uniform(0.0, 0.6, n)
+            game = rng.uniform(1e11, 1e13, n)
+            gamh = rng.uniform(1e11, 1e13, n)
+            game[0] = 1.0  # below the 0.1 meV floor
+            wmax = s.gap / SBEs.hbar * 2.0
+            iqs = np.array([0, 1, 2, n // 2, n - 1, n, -1, -(n // 2)])
+            ws = np.array([0.0, 0.37, 0.5, 0.93, 1.0]) * wmax
+            # the method (SBEs.py:5331-5357) passes the solver's Nk; the module
+            # function reads a module global that only the legacy API sets
+            X = np.array([[s.calc_Xqw(int(iq), float(w), s.kr, fe, fh, s.Ee, s.Eh, s.gap,
+                                      s.area, game, gamh, s.dcv) for w in ws] for iq in iqs])
+            out.update(nk=n, L=L, kr=s.kr, Ee=s.Ee, Eh=s.Eh, gap=s.gap, area=s.area,
+                       dcv=s.dcv, dkr=s.dkr, gam_e=s.gam_e, gam_h=s.gam_h, fe=fe, fh=fh,
+                       game=game, gamh=gamh, iqs=iqs, ws=ws, X=X)
+            with open(os.path.join("dataQW", "Wire", "Xqw", "Xqw.000000.dat")) as f:
+                lines = f.readlines()
+            out["xqw_nlines"] = len(lines)
+            out["xqw_head"] = np.array(lines[:300])
+            out["xqw_tail"] = np.array(lines[-300:])
+            with open(os.path.join("dataQW", "Wire", "Xqw", "Xqw.params")) as f:
+                out["xqw_params"] = np.array(f.readlines())
+            s.close()
+        finally:
+            os.chdir(cwd)
+    path = os.path.join(OUT_DIR, "ref_xqw.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "N =", nk)
+
+
 def main():
     if not os.path.isdir(REF_SRC):
         raise SystemExit("reference tree not found; run in the build container")
     _install_shims()
     sys.path.insert(0, REF_SRC)
+    if sys.argv[1:] == ["xqw"]:
+        xqw_case()
+        return
     run_case(16, seed=1, mb=MB_SHIPPED, nsteps=4, tag="n16_mb")
     run_case(16, seed=2, mb=MB_RHS_ONLY, nsteps=3, tag="n16_rhs")
     run_case(32, seed=3, mb=MB_SHIPPED, nsteps=3, tag="n32_mb")
+    xqw_case()
 
 
 if __name__ == "__main__":

@@ -73,3 +73,69 @@ def test_backup_round_trip(tmp_path, monkeypatch):
     b = t._levels()
     for x, y in zip(a, b):
         assert rel(x, y) < 1e-15
+
+
+def test_xqw_record_matches_reference(tmp_path, monkeypatch):
+    """chi(q, w) on the device against the reference's calc_Xqw at random
+    occupations and dephasing, and the files the reference's __init__
+    writes (_record_Xqw: Xqw.000000.dat, Xqw.params)."""
+    z = load_golden("xqw")
+    write_params(str(tmp_path), {"L": z["L"], "mb": [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 0, 1, 0]})
+    monkeypatch.chdir(tmp_path)
+    from pulsesuite_amd import SBEs
+    nx, drr = 64, 10e-9
+    rr = (np.arange(nx) - nx // 2) * drr
+    qrr = 2 * np.pi * np.fft.fftfreq(nx, drr)
+    print("[xqw] init", flush=True)
+    s = SBEs.SBESolver(qrr, rr, 0.0, 1e7, 800e-9, 1, True)
+    assert s.Nk == int(z["nk"])
+    w, X = s.xqw_table(z["fe"], z["fh"], z["game"], z["gamh"])
+    wmax = w[-1]
+    for a, iq in enumerate(z["iqs"]):
+        if iq < 0:
+            continue
+        for b, wf in enumerate(z["ws"]):
+            iw = int(round(wf / wmax * (len(w) - 1)))
+            ref = z["X"][a, b]
+            assert abs(X[iq, iw] - ref) <= 1e-11 * max(abs(ref), 1e-30), (iq, iw)
+    lines = open("dataQW/Wire/Xqw/Xqw.000000.dat").readlines()
+    assert len(lines) == int(z["xqw_nlines"])
+    for ours, theirs in ((lines[:300], z["xqw_head"]), (lines[-300:], z["xqw_tail"])):
+        a = np.array([[float(t) for t in ln.split()] for ln in ours])
+        b = np.array([[float(t) for t in ln.split()] for ln in theirs])
+        assert np.all(np.abs(a - b) <= 1e-6 * np.max(np.abs(b), axis=0) + 5e-7 * np.abs(b))
+        assert all(len(x) == len(y) for x, y in zip(ours, theirs))
+    assert open("dataQW/Wire/Xqw/Xqw.params").readlines() == list(z["xqw_params"])
+    s.close()
+
+
+def test_xqw_table_vs_oracle_n64():
+    """The device table against the oracle's calc_Xqw at N = 64."""
+    from tests.helpers import params_pair, wire_length
+    from oracle import sbe_oracle as O
+    import ctypes
+    import torch
+    from scipy.constants import e, epsilon_0, hbar
+    from pulsesuite_amd import capi
+    N = 64
+    _, oprm = params_pair(wire_length(N))
+    g = O.Grid(oprm)
+    rng = np.random.default_rng(9)
+    fe, fh = rng.uniform(0, 0.5, N), rng.uniform(0, 0.5, N)
+    game, gamh = rng.uniform(1e11, 1e13, N), rng.uniform(1e11, 1e13, N)
+    lib = capi.load()
+    dev = torch.device("cuda")
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=float), device=dev)  # noqa: E731
+    args = [T(g.Ee), T(g.Eh), T(fe), T(fh), T(game), T(gamh)]
+    nw = 97
+    dw = 2 * oprm.gap / hbar / 96
+    out = torch.empty((N + 1, nw), dtype=torch.complex128, device=dev)
+    x = 2.0 * g.dkr / (2 * np.pi) / g.area * g.dcv * g.dcv / epsilon_0
+    capi.check(lib.psa_xqw_table(N, *[a.data_ptr() for a in args], oprm.gap, x, hbar, 1e-4 * e,
+                                 dw, nw, out.data_ptr(), ctypes.c_void_p(0)))
+    torch.cuda.synchronize()
+    X = out.cpu().numpy()
+    for iq in (0, 1, 17, 63, 64):
+        for iw in (0, 13, 48, 96):
+            ref = O.calc_xqw(iq, iw * dw, g.dkr, fe, fh, g.Ee, g.Eh, oprm.gap, g.area, game, gamh, g.dcv)
+            assert abs(X[iq, iw] - ref) <= 1e-12 * max(abs(ref), 1e-30), (iq, iw)

@@ -370,3 +370,16 @@ def test_rk4_wire_step_free_rotation():
     for lam, x0, x in ((lp, st[5], s.YY3), (lc, st[3], s.CC3), (ld, st[4], s.DD3)):
         ref = _t4(dt * lam)[:, :, None] ** 3 * x0
         assert np.max(np.abs(x - ref)) < 1e-13 * np.max(np.abs(ref))
+
+
+def test_calc_xqw_matches_reference(golden_dir):
+    """chi(q, w) against SBESolver.calc_Xqw of the reference at random
+    occupations and dephasing (one rate below the 0.1 meV floor), q of both
+    signs and the empty q = N, w from 0 to twice the gap."""
+    z = np.load(os.path.join(golden_dir, "ref_xqw.npz"))
+    for a, iq in enumerate(z["iqs"]):
+        for b, w in enumerate(z["ws"]):
+            got = O.calc_xqw(int(iq), float(w), float(z["dkr"]), z["fe"], z["fh"], z["Ee"], z["Eh"],
+                             float(z["gap"]), float(z["area"]), z["game"], z["gamh"], float(z["dcv"]))
+            ref = z["X"][a, b]
+            assert abs(got - ref) <= 1e-13 * max(abs(ref), 1e-300) + 1e-300, (iq, w)
