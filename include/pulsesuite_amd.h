@@ -108,7 +108,9 @@ typedef struct psa_group {
   double *hdiag;                      /* [Nw][2][N] complex: diagonals of Hee, Hhh */
   double *prep;                       /* [Nw][PSA_PREP_STRIDE(N)] doubles, see PSA_PREP_* */
   double *dedh;                       /* [Nw][2][N][2N-1] off-diagonal dephasing tables D[q][k] stored [k][q + N - 1] */
-  double *wmb;                        /* 11 N Nw doubles: [Nw][4][N] many-body Win_e, Wout_e, Win_h, Wout_h;
+  double *wmb;                        /* 11 N Nw doubles: [Nw][4][N] many-body Win_e, Wout_e, Win_h, Wout_h
+                                         (after psa_relaxation: the total rates times the occupation
+                                         factors Win_e (1 - ne), Wout_e ne, Win_h (1 - nh), Wout_h nh);
                                          then, wire fastest: [2][N][Nw] |Re ne3|, |Re nh3| of the new level,
                                          [3][N][Nw] VC(d)^2 (eh, ee, hh), [2][N][Nw] |ne|, |nh| of the checkout level */
   double *exq;                        /* [Nw][3][2N-1] complex: FFTG'd Ex, Ey, Ez at q = -(N-1)..N-1 */
@@ -146,7 +148,7 @@ typedef struct psa_group {
 #define PSA_PREP_ANH(N) (PSA_PREP_GAME(N) + 4 * (N))  /* [N] |nh| */
 #define PSA_PREP_STRIDE(N) (PSA_PREP_GAME(N) + 5 * (N))
 
-#define PSA_OBS_N 8 /* I0, sum ne2, sum nh2, total(rho), ... */
+#define PSA_OBS_N 8 /* I0, sum ne2, sum nh2, total(rho); max/min De, max/min Dh (psa_offdiag_extrema) */
 /* per-wire src block: [5][Nr] Px, Py, Pz, re, rh (real space, final);
  * [2][Nr] complex re, rh before the LF normalisation; [4][2N-1] complex
  * diagonal sums; 8 scalars */
@@ -265,6 +267,12 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
  * and X are stage scratch afterwards (a later leapfrog step needs A = B);
  * pR, CR, DR are required. */
 int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream);
+
+/* max and min of the off-diagonal dephasing tables De, Dh of the last
+ * psa_offdiag_tables call (the MaxOffDiag.e/h.dat lines written by
+ * OffDiagDephasing2, dephasing.py:733-743) into obs[w][4..7]:
+ * max De, min De, max Dh, min Dh. */
+int psa_offdiag_extrema(const psa_group *g, int32_t w0, int32_t nw, void *stream);
 
 /* chi(q, w) table that SBESolver.__init__ records (_record_Xqw,
  * SBEs.py:5395-5432) through SBESolver.calc_Xqw (SBEs.py:5331-5357 ->

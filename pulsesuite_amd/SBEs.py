@@ -34,6 +34,60 @@ from .engine import WireGroup
 
 twopi = 2.0 * np.pi
 ii = 1j
+eV = e0
+# printITReal2 (qwoptics.py:1189-1215) writes the coordinate column only in
+# the first file of the process
+_prop_first = [True]
+
+
+def _f32(x):
+    return np.float32(x)
+
+
+def _write_it(V, name):
+    """WriteIt / WriteIT1D (usefulsubs.py:2339, 1915-1919)."""
+    with open(f"dataQW/{name}.dat", "w", encoding="utf-8") as f:
+        f.write("".join(f"{v}\n" for v in V))
+
+
+def _print_it(Dx, z, n, name, width=6):
+    """printIT (usefulsubs.py:2068-2100): z, Re, Im as float32."""
+    fn = f"dataQW/{name}{n:0{width}d}.dat"
+    os.makedirs(os.path.dirname(fn), exist_ok=True)
+    Dx = np.asarray(Dx, dtype=complex)
+    with open(fn, "w", encoding="utf-8") as f:
+        f.write("".join(f"{_f32(a)} {_f32(b.real)} {_f32(b.imag)}\n" for a, b in zip(z, Dx)))
+
+
+def _print_itr(Dx, z, n, name, width=6):
+    """printITR (usefulsubs.py:2103-2133) and printGam (dephasing.py:417-427,
+    width 5): z, value as float32."""
+    fn = f"dataQW/{name}{n:0{width}d}.dat"
+    os.makedirs(os.path.dirname(fn), exist_ok=True)
+    with open(fn, "w", encoding="utf-8") as f:
+        f.write("".join(f"{_f32(a)} {_f32(b)}\n" for a, b in zip(z, Dx)))
+
+
+def _print_it2d(Dx, n, name):
+    """printIT2D (usefulsubs.py:2136-2168): |Dx| as float32, column by column."""
+    fn = f"dataQW/{name}{n:07d}.dat"
+    os.makedirs(os.path.dirname(fn), exist_ok=True)
+    with open(fn, "w", encoding="utf-8") as f:
+        f.write("".join(f"{_f32(v)}\n" for v in np.abs(np.asarray(Dx)).T.reshape(-1)))
+
+
+def _print_it_real2(Dx, z, n, name):
+    """printITReal2 (qwoptics.py:1189-1215): Re Dx (and z in the first file
+    of the process) as float32."""
+    fn = f"dataQW/{name}{n:06d}.dat"
+    os.makedirs(os.path.dirname(fn), exist_ok=True)
+    re = np.real(np.asarray(Dx))
+    with open(fn, "w", encoding="utf-8") as f:
+        if _prop_first[0]:
+            f.write("".join(f"{_f32(a)} {_f32(b)}\n" for a, b in zip(re, z)))
+            _prop_first[0] = False
+        else:
+            f.write("".join(f"{_f32(a)}\n" for a in re))
 
 
 def yw(w):
@@ -103,6 +157,7 @@ class SBESolver:
         self._lvl1 = None
         # real-space field at r[Nr//2] for the EP file: window * spline row
         self._m1 = tables.spline_matrix(self.rr, self.r) if self.io else None
+        self._m2 = None
         if self.io:
             os.makedirs("dataQW", exist_ok=True)
             for w in range(1, self.Nw + 1):
@@ -110,9 +165,110 @@ class SBESolver:
                     open(f"dataQW/{nm}.{w:02d}.t.dat", "w").close()
             os.makedirs("dataQW/Wire/info", exist_ok=True)
             open("dataQW/Wire/info/ETHz.t.dat", "w").close()
+            self._write_init_files()
             self.record_xqw(np.zeros(self.Nk), np.zeros(self.Nk), np.full(self.Nk, self.gam_e),
                             np.full(self.Nk, self.gam_h), 0)
         self.start = True
+
+    def _write_init_files(self):
+        """Start-up files of SBESolver.__init__ (SBEs.py:4857-4870), the
+        window of QWOptics.__init__ (qwoptics.py:650), the empty Fe/Fh files
+        of the DC-field module (dcfield.py:792-793) and the MaxOffDiag files
+        of InitializeDephasing (dephasing.py:538-539)."""
+        _write_it(self.kr, "kr")
+        _write_it(self.Qr, "Qr")
+        _write_it(self.r, "R")
+        _write_it(self.Ee / eV, "Ee.k")
+        _write_it(self.Eh / eV, "Eh.k")
+        _write_it((self.Ee + self.Eh + self.gap - hbar * c0 * twopi / self.lam) / eV, "Echw.k")
+        _write_it((self.Ee + self.Eh + self.gap) / eV, "Etrn.k")
+        with open("dataQW/Etr.dat", "w", encoding="utf-8") as f:
+            f.write("".join(f"{k} {(a + b + self.gap) / eV}\n"
+                            for k, a, b in zip(self.kr, self.Ee, self.Eh)))
+        _print_it(self.group.tab["window"] * (ii) ** 0.0, self.r, 0, "Envl.y.")
+        if self.DCTrans:
+            open("dataQW/Fe.dat", "w").close()
+            open("dataQW/Fh.dat", "w").close()
+        open("dataQW/Wire/info/MaxOffDiag.e.dat", "w").close()
+        open("dataQW/Wire/info/MaxOffDiag.h.dat", "w").close()
+
+    def _write_max_offdiag(self, i):
+        """One line per wire step of OffDiagDephasing2 (dephasing.py:733-743):
+        t, max and min of De (resp. Dh), from psa_offdiag_extrema."""
+        import ctypes
+        from . import capi
+        g = self.group
+        capi.check(g.lib.psa_offdiag_extrema(ctypes.byref(g.S), i, 1, g._stream()))
+        ob = g.s["obs"][i].cpu().numpy()
+        t = np.real(self.t)
+        for fn, a, b in (("e", ob[4], ob[5]), ("h", ob[6], ob[7])):
+            with open(f"dataQW/Wire/info/MaxOffDiag.{fn}.dat", "a", encoding="utf-8") as f:
+                f.write(f"{t} {a} {b}\n")
+
+    def _write_snapshots(self, Exx, Eyy, Ezz, out, w):
+        """The WriteFields files of one wire step (every jmax steps): the
+        relaxation rates (SBEs.py:5282-5307), WriteSBESolns and
+        WriteDephasing (5712-5715; qwoptics.py:1263-1291, dephasing.py:430-437),
+        WriteQWFields in QW2Prop (qwoptics.py:792-793, 1316-1349) and
+        WritePropFields (SBEs.py:5919-5922, qwoptics.py:1352-1386)."""
+        i = w - 1
+        g = self.group
+        N, Nr, xxx = self.Nk, self.Nr, self.xxx
+        kr, r, rr = self.kr, self.r, self.rr
+        ws = f"{w:02d}"
+        pr = g.s["prep"][i].cpu().numpy()
+        if self.EHs or self.Phonon:
+            mb = g.s["wmb"][i * 4 * N:(i + 1) * 4 * N].cpu().numpy().reshape(4, N)
+            _print_itr(mb[0], kr, xxx, f"Wire/Win/Win.e.{ws}.k.")
+            _print_itr(mb[2], kr, xxx, f"Wire/Win/Win.h.{ws}.k.")
+            _print_itr(mb[1], kr, xxx, f"Wire/Wout/Wout.e.{ws}.k.")
+            _print_itr(mb[3], kr, xxx, f"Wire/Wout/Wout.h.{ws}.k.")
+        # the new level is level 3 (B) once the step has advanced the levels
+        C3 = g.s["CB"][i].cpu().numpy()
+        D3 = g.s["DB"][i].cpu().numpy()
+        p3 = g.s["pB"][i].cpu().numpy()
+        hd = g.s["hdiag"][i].cpu().numpy()
+        _print_it2d(C3, xxx, f"Wire/C/C.{ws}.k.kp.")
+        _print_it2d(D3, xxx, f"Wire/D/D.{ws}.k.kp.")
+        _print_it2d(p3, xxx, f"Wire/P/P.{ws}.k.kp.")
+        _print_it(np.diag(C3), kr, xxx, f"Wire/ne/ne.{ws}.k.")
+        _print_it(np.diag(D3), kr, xxx, f"Wire/nh/nh.{ws}.k.")
+        _print_it(hd[0] / eV, kr, xxx, f"Wire/Ee/Ee.{ws}.k.")
+        _print_it(hd[1] / eV, kr, xxx, f"Wire/Eh/Eh.{ws}.k.")
+        if self.DiagDph:
+            gs = 8 * N + 4 * (2 * N - 1)  # PSA_PREP_GAME
+            _print_itr(pr[gs:gs + N], kr, xxx, f"Wire/Ge/Ge.{ws}.k.", width=5)
+            _print_itr(pr[gs + N:gs + 2 * N], kr, xxx, f"Wire/Gh/Gh.{ws}.k.", width=5)
+        # QW2Prop: the QW-grid fields (window x spline of the rr fields),
+        # polarizations and charge densities on r
+        src = g.s["src"][i].cpu().numpy()
+        win = g.tab["window"]
+        Eq = [win * np.real(self._m1 @ np.asarray(E)) + 0j for E in (Exx, Eyy, Ezz)]
+        P = [src[c * Nr:(c + 1) * Nr] + 0j for c in range(3)]
+        re, rh = src[3 * Nr:4 * Nr] + 0j, src[4 * Nr:5 * Nr] + 0j
+        zq = np.zeros(Nr, dtype=complex)
+        for nm, v in (("Ex", Eq[0]), ("Ey", Eq[1]), ("Ez", Eq[2]), ("Vr", zq), ("Px", P[0]),
+                      ("Py", P[1]), ("Pz", P[2]), ("Re", re), ("Rh", rh), ("Rho", rh - re)):
+            _print_it(v, r, xxx, f"Wire/{nm}/{nm}.{ws}.r.")
+        # propagation grid: RhoE, RhoH as QW2Prop forms them (splines to rr,
+        # then |.| renormalised to the QW-grid total, qwoptics.py:795-805)
+        if self._m2 is None:
+            self._m2 = tables.spline_matrix(self.r, self.rr)
+        dr = r[1] - r[0]
+        dRR = rr[1] - rr[0] if len(rr) > 1 else 1.0
+        RhoE = self._m2 @ re
+        RhoH = self._m2 @ rh
+        if self.LF:
+            total = (np.sum(re) * dr + np.sum(rh) * dr + 1e-100) / 2.0
+            RhoE = np.abs(RhoE) + 0j
+            RhoH = np.abs(RhoH) + 0j
+            RhoE = RhoE * total / (np.sum(RhoE) * dRR + 1e-100)
+            RhoH = RhoH * total / (np.sum(RhoH) * dRR + 1e-100)
+        zr = np.zeros(len(rr), dtype=complex)
+        for nm, v in (("Ex", Exx), ("Ey", Eyy), ("Ez", Ezz), ("Vr", zr), ("Px", out[0]),
+                      ("Py", out[1]), ("Pz", out[2]), ("Re", RhoE), ("Rh", RhoH)):
+            _print_it_real2(v, rr, xxx, f"Prop/{nm}/{nm}.{ws}.r.")
+        _print_it(RhoH - RhoE, rr, xxx, f"Prop/Rho/Rho.{ws}.r.")
 
     def xqw_table(self, fe, fh, game, gamh):
         """chi(q, w) for q = 0..Nk (units of dkr) and w = iw dw, dw = 2 gap /
@@ -247,7 +403,7 @@ class SBESolver:
         Pyy[:] = 0.0
         Pzz[:] = 0.0
         Rho[:] = 0.0
-        self._bookkeeping(w, dt)
+        write_fields = self._bookkeeping(w, dt)
         if self._gate(Exx, Eyy, Ezz):
             return
         DoQWP[0] = self.Optics
@@ -269,6 +425,10 @@ class SBESolver:
         self._energy(np.asarray(Exx), np.asarray(Eyy), np.asarray(Ezz), out, i)
         if self.io:
             self._write_files(np.asarray(Exx), np.asarray(Eyy), np.asarray(Ezz), dt, w)
+            if self.OffDiagDph:
+                self._write_max_offdiag(i)
+            if write_fields:
+                self._write_snapshots(np.asarray(Exx), np.asarray(Eyy), np.asarray(Ezz), out, w)
 
     def QWCalculatorBatch(self, Exx, Eyy, Ezz, dt):
         """Step every wire at once; fields [Nw, Nx] (or [Nx] shared) as numpy
@@ -349,7 +509,7 @@ class SBESolver:
 
         src = g.s["src"][i].cpu().numpy()
         rc = src[5 * Nr:9 * Nr].view(complex).reshape(2, Nr)
-        X3d = (g.s["CX"][i].diagonal().cpu().numpy(), g.s["DX"][i].diagonal().cpu().numpy())
+        X3d = (g.s["CB"][i].diagonal().cpu().numpy(), g.s["DB"][i].diagonal().cpu().numpy())
         dr = self.r[1] - self.r[0]
         rmax = []
         for c in range(2):

@@ -81,6 +81,7 @@ SIGNATURES = {
     "psa_step_rk4": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
     "psa_drive": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr, c_dbl, c_ptr]),
     "psa_record": (ctypes.c_int, [_GP, c_i32, c_i32, c_i32, c_ptr, c_ptr, c_i32, c_ptr]),
+    "psa_offdiag_extrema": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_xqw_table": (ctypes.c_int, [c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_dbl,
                                      c_dbl, c_dbl, c_dbl, c_i32, c_ptr, c_ptr]),
     "psa_pstd_size": (ctypes.c_int, []),

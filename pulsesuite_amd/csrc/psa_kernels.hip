@@ -643,7 +643,7 @@ __global__ __launch_bounds__(64 * RL_WAVES) void k_relax(psa_group g, int w0, do
   double *C3 = g.CX + o, *D3 = g.DX + o;
   const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
   const double *E1D = pr + PSA_PREP_E1D(N), *vc = pr + PSA_PREP_VC(N), *Rsp = pr + PSA_PREP_RSP(N);
-  const double *mb = g.wmb + (long)w * 4 * N;
+  double *mb = g.wmb + (long)w * 4 * N;
   const bool ehs = (g.flags & PSA_F_EHS) != 0, phon = (g.flags & PSA_F_PHONON) != 0;
   const bool recomb = (g.flags & PSA_F_RECOMB) != 0;
   for (int i = tid; i < N; i += blockDim.x) {
@@ -693,6 +693,13 @@ __global__ __launch_bounds__(64 * RL_WAVES) void k_relax(psa_group g, int w0, do
   }
   cd ne = ld(C3, (long)k * N + k), nh = ld(D3, (long)k * N + k);
   if (ehs || phon) {
+    // the rates as the reference's WriteFields snapshots print them
+    // (SBEs.py:5282-5307): Win_e (1 - Re ne), Wout_e Re ne, Win_h (1 - Re nh),
+    // Wout_h Re nh with the occupations before the update
+    mb[k] = wie * (1.0 - ne.re);
+    mb[N + k] = woe * ne.re;
+    mb[2 * N + k] = wih * (1.0 - nh.re);
+    mb[3 * N + k] = woh * nh.re;
     const double We = wie + woe, Wh = wih + woh;
     const double ee = exp(-We * dt), eep = exp(We * dt);
     const double eh = exp(-Wh * dt), ehp = exp(Wh * dt);
@@ -2060,6 +2067,44 @@ int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *st
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Largest and smallest entry of the off-diagonal dephasing tables De, Dh of
+// each wire (the MaxOffDiag.e/h.dat lines of OffDiagDephasing2,
+// dephasing.py:733-743) into obs[w][4..7].  Unused (k, q) entries are 0, as
+// in the reference's tables.
+// ===========================================================================
+namespace {
+
+__global__ __launch_bounds__(256) void k_dedh_extrema(psa_group g, int w0) {
+  __shared__ double sh[8];
+  const int w = w0 + blockIdx.x, N = g.N;
+  const long per = (long)N * (2 * N - 1);
+  for (int kind = 0; kind < 2; ++kind) {
+    const double *t = g.dedh + ((long)w * 2 + kind) * per;
+    double mx = -INFINITY, mn = INFINITY;
+    for (long i = threadIdx.x; i < per; i += 256) {
+      mx = fmax(mx, t[i]);
+      mn = fmin(mn, t[i]);
+    }
+    mx = block_max(mx, sh);
+    mn = -block_max(-mn, sh);
+    if (threadIdx.x == 0) {
+      g.obs[(long)w * PSA_OBS_N + 4 + 2 * kind] = mx;
+      g.obs[(long)w * PSA_OBS_N + 5 + 2 * kind] = mn;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int psa_offdiag_extrema(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  if (!(g->flags & PSA_F_OFFDIAGDPH)) return fail(PSA_E_INVALID, "psa_offdiag_extrema: OffDiagDph is off");
+  hipLaunchKernelGGL(k_dedh_extrema, dim3(nw), dim3(256), 0, (hipStream_t)stream, *g, w0);
+  PSA_LAUNCH_CHECK("k_dedh_extrema");
+  return PSA_OK;
+}
 
 // ===========================================================================
 // chi(q, w) table recorded at start-up (_record_Xqw, SBEs.py:5395-5432) by

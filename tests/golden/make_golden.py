@@ -27,12 +27,16 @@ recorded (composed and index-heavy behaviour):
 * ``SBESolver.QWCalculator`` trajectories (``SBEs.py:5826-5985``), with the
   shipped ``mb.params`` and with the many-body relaxation/transport switched
   off so the leapfrog RHS can be read back from the state;
+* the snapshot files of the ``WriteFields`` steps (``jmax`` = 2: ``Wire/*``
+  via ``WriteSBESolns``, ``WriteDephasing``, the relaxation rates,
+  ``WriteQWFields``; ``Prop/*`` via ``WritePropFields``) as text;
 * ``SBESolver.calc_Xqw`` -> ``_CalcXqw_jit`` (``SBEs.py:5331-5357``,
   ``3165-3243``) and the chi(q, w) files of
   ``_record_Xqw`` (``SBEs.py:5395-5432``).
 
 Usage: ``python tests/golden/make_golden.py`` (writes ``tests/golden/*.npz``;
-``python tests/golden/make_golden.py xqw`` writes only ``ref_xqw.npz``).
+``python tests/golden/make_golden.py xqw`` writes only ``ref_xqw.npz``,
+``... snap`` only ``ref_n16_snap.npz``).
 """
 
 import contextlib
@@ -91,10 +95,10 @@ def wire_length(nk, me_rel=0.067, gap_ev=1.5):
     return (nk / 2 + 0.5) * np.pi / kmax
 
 
-def write_params(d, L, mb, me_rel=0.067):
+def write_params(d, L, mb, me_rel=0.067, jmax=500):
     os.makedirs(os.path.join(d, "params"), exist_ok=True)
     qw = [L, 5e-9, 1.5, me_rel, 0.45, 100e-3, 1e12, 1e12, 1e12, 9.1,
-          36e-3, 3e-3, 0.0, 500, 20000]
+          36e-3, 3e-3, 0.0, jmax, 20000]
     with open(os.path.join(d, "params", "qw.params"), "w") as f:
         for v in qw:
             f.write(f"{v if isinstance(v, int) else float(v)!r}\n")
@@ -125,7 +129,7 @@ def pulse(t, E0, tw, tp, w0):
             * np.exp(-(x**20) / (2 * tw * w0) ** 20))
 
 
-def run_case(nk, seed, mb, nsteps, tag, nx=64, drr=10e-9):
+def run_case(nk, seed, mb, nsteps, tag, nx=64, drr=10e-9, jmax=500, files=False):
     rng = np.random.default_rng(seed)
     L = wire_length(nk)
     out = {}
@@ -133,7 +137,7 @@ def run_case(nk, seed, mb, nsteps, tag, nx=64, drr=10e-9):
         cwd = os.getcwd()
         os.chdir(d)
         try:
-            write_params(d, L, mb)
+            write_params(d, L, mb, jmax=jmax)
             buf = io.StringIO()
             with contextlib.redirect_stdout(buf):
                 from pulsesuite.PSTD3D import (SBEs, coulomb, dephasing,
@@ -275,6 +279,22 @@ def run_case(nk, seed, mb, nsteps, tag, nx=64, drr=10e-9):
                                         for line in f if line.strip()])
             out["info1"] = info[1]
             out["info2"] = info[2]
+            out["jmax"] = jmax
+            if files:
+                # every file the run wrote (the chi(q, w) record excepted):
+                # the Wire/* and Prop/* snapshots of the WriteFields steps,
+                # the window file, info/EP files
+                names, texts = [], []
+                for root, _, fs in os.walk("dataQW"):
+                    for fn in sorted(fs):
+                        path = os.path.join(root, fn)
+                        if "Xqw" in path:
+                            continue
+                        names.append(path)
+                        with open(path) as f:
+                            texts.append(f.read())
+                out["file_names"] = np.array(names)
+                out["file_texts"] = np.array(texts)
             s.close()
         finally:
             os.chdir(cwd)
@@ -343,9 +363,13 @@ def main():
     if sys.argv[1:] == ["xqw"]:
         xqw_case()
         return
+    if sys.argv[1:] == ["snap"]:
+        run_case(16, seed=6, mb=MB_SHIPPED, nsteps=3, tag="n16_snap", jmax=2, files=True)
+        return
     run_case(16, seed=1, mb=MB_SHIPPED, nsteps=4, tag="n16_mb")
     run_case(16, seed=2, mb=MB_RHS_ONLY, nsteps=3, tag="n16_rhs")
     run_case(32, seed=3, mb=MB_SHIPPED, nsteps=3, tag="n32_mb")
+    run_case(16, seed=6, mb=MB_SHIPPED, nsteps=3, tag="n16_snap", jmax=2, files=True)
     xqw_case()
 
 

@@ -424,10 +424,12 @@ def test_many_body_lists_device_vs_host(N):
 
 @pytest.mark.parametrize("N,nw", [(64, 20), (256, 18)])
 def test_many_body_rates_reduced_vs_fp64(N, nw):
-    """The many-body in/out rates of the reduced modes (FP32 tables for 16
+    """The in/out rates of the reduced modes (FP32 many-body tables for 16
     wires per block, two wires per lane) against the FP64 kernel (pinned
-    against the oracle through the step tests) on one random state: within
-    1e-5 per wire and rate; 18 and 20 wires cover a partial wire chunk."""
+    against the oracle through the step tests) on one random state, as
+    psa_relaxation leaves them (many-body + phonon rates times the
+    occupation factors): within 1e-5 per wire and rate; 18 and 20 wires
+    cover a partial wire chunk."""
     import torch
     from pulsesuite_amd.engine import WireGroup
     L = wire_length(N)
