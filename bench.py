@@ -262,6 +262,45 @@ def transmission_sample(seed, nw=16, nk=256, nx=4096, npts=4096, steps=20, warmu
     return out
 
 
+def qw3d_sample(seed, nw=4, nk=64, shape=(256, 64, 64), reps=20, dt=10e-18):
+    """The 3-D coupling of rhoPJ.QuantumWire on the space.params grid
+    (256 x 64 x 64 points of 2 nm, epsr 9.1) with the qwarray.params wires
+    (4 wires, a0 = 5 nm): HIP-event times of the drive and of the deposit
+    (lines, placement and ElongfromRho's 14 3-D FFTs), the wires' step apart."""
+    import torch
+    from pulsesuite_amd.engine import WireGroup
+    from pulsesuite_amd.qw3d import QuantumWire3D
+    from pulsesuite_amd.tables import QWParams
+    nx, ny, nz = shape
+    d = 2e-9
+    r = [(np.arange(n) - n // 2) * d for n in shape]
+    q = [2 * np.pi * np.fft.fftfreq(n, d) for n in shape]
+    wg = WireGroup(QWParams(L=wire_length(nk), me_rel=0.067), nw, r[0], prec="fp64")
+    wg.set_state(*seeded_state(np.random.default_rng(seed), wg.N, nw))
+    q3 = QuantumWire3D(wg, *r, *q, 0.0, 0.0, 5e-9, 5e-9, 9.1, dt)
+    dev = torch.device("cuda")
+    G = [torch.zeros(shape, dtype=torch.complex128, device=dev) for _ in range(7)]
+    G[0].fill_(1e6)
+    q3.step(*G[:3], *G[3:])
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record()
+    for _ in range(reps):
+        q3.drive(*G[:3])
+    ev[1].record()
+    for _ in range(reps):
+        q3.deposit(*G[3:])
+    ev[2].record()
+    torch.cuda.synchronize()
+    out = dict(workload=f"{nw} FP64 wires x Nk={nk} along x of a {nx}x{ny}x{nz} grid",
+               drive_us=ev[0].elapsed_time(ev[1]) / reps * 1e3,
+               deposit_us=ev[1].elapsed_time(ev[2]) / reps * 1e3,
+               fft_pass_bytes=14 * 3 * 2 * 16 * nx * ny * nz)
+    del wg, q3, G
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,6 +397,11 @@ def main():
             res["transmission"] = transmission_sample(seed)
         except Exception as ex:  # report, keep the batch number
             res["transmission"] = {"error": repr(ex)}
+        try:
+            print("[bench] 3-D coupling", file=sys.stderr, flush=True)
+            res["qw3d"] = qw3d_sample(seed)
+        except Exception as ex:  # report, keep the batch number
+            res["qw3d"] = {"error": repr(ex)}
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

@@ -284,6 +284,50 @@ int psa_xqw_table(int32_t N, const double *Ee, const double *Eh, const double *f
                   const double *game, const double *gamh, double gap, double x, double hbar,
                   double hfloor, double dw, int32_t nw, double *out, void *stream);
 
+/* ---- 3-D coupling of the wires to a Maxwell grid (rhoPJ.QuantumWire) ----
+ *
+ * The per-step part of QuantumWire (rhoPJ.py:281-449) around the wires'
+ * QWCalculator calls, for the Nw wires of one group lying along x of a
+ * [nx][ny][nz] grid (nx = the group's Nx, the grid's x line being the
+ * wires' propagation grid):
+ *  - psa_qw3d_drive: EAtXYZ (usefulsubs.py:1996-2065) of Ex, Ey, Ez at the
+ *    wire point (trilinear cell i0, j0, k0 and weights from the caller's
+ *    locator), broadcast along x into every wire's efield;
+ *  - psa_qw3d_deposit (after psa_step): J = sum_w (P - P_old) / dt / Nw plus
+ *    CalcJfx (rhoPJ.py:491-523) of the summed charge / Nw for x; placement
+ *    (QWPlacement, rhoPJ.py:622-700) of J into Jx, Jy, Jz, of sum Rho / Nw
+ *    into Rho and of sum P / Nw into fresh Px, Py, Pz, with profiles gy, gz;
+ *    then ElongfromRho (rhoPJ.py:703-880): 3-D FFTs of Rho and P, the
+ *    k-space loop (rhoPJ.py:526-619), inverse FFTs of the nine longitudinal
+ *    fields and the bound charge; P_old <- P.
+ * Grid arrays are complex [nx][ny][nz], z fastest, device pointers. */
+#define PSA_QW3D_CURR 1 /* place J (DoQWCurr) */
+#define PSA_QW3D_RHO 2  /* place Rho (DoQWDl) */
+#define PSA_QW3D_POL 4  /* place P and compute the longitudinal fields (DoQWP) */
+
+typedef struct psa_qw3d {
+  int32_t nx, ny, nz;           /* each 2^a 3^b in [2, 512] */
+  int32_t flags;                /* PSA_QW3D_* */
+  int32_t i0, j0, k0;           /* EAtXYZ cell: lower corner */
+  int32_t pad;
+  double ax, ay, az;            /* EAtXYZ weights of the upper corner (alpha, beta, gamma) */
+  double dt, dx, eps;           /* step, x spacing, background permittivity eps0 epsr (F/m) */
+  const double *qx, *qy, *qz;   /* [nx], [ny], [nz] wavenumbers (FFT order) */
+  const double *gy, *gz;        /* [ny], [nz] real placement profiles */
+  const double *tw;             /* complex exp(-2 pi i m / n): [nx], then [ny], then [nz] */
+  const double *Ex, *Ey, *Ez;   /* grid fields (read) */
+  double *Jx, *Jy, *Jz, *Rho;   /* grid sources, accumulated into */
+  double *Fk;                   /* [4] grids scratch: Rho, Px, Py, Pz, then their FFTs */
+  double *El;                   /* [10] grids: E_l from P+Rho (x, y, z), from P (x, y, z),
+                                   from Rho (x, y, z), bound charge */
+  double *lines;                /* [8][nx] complex scratch */
+  double *old;                  /* [4][Nw][nx] complex: the wires' previous Pxx, Pyy, Pzz, Rho */
+} psa_qw3d;
+
+int psa_qw3d_size(void);
+int psa_qw3d_drive(const psa_qw3d *q, const psa_group *g, void *stream);
+int psa_qw3d_deposit(const psa_qw3d *q, const psa_group *g, void *stream);
+
 /* ---- 1-D pseudo-spectral field propagator coupled to the wires ----
  *
  * The reference drives the wires with spatially uniform analytic pulses

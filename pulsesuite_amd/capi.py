@@ -54,6 +54,19 @@ class PsaPstd(ctypes.Structure):
 
 
 _PP = ctypes.POINTER(PsaPstd)
+
+
+class PsaQw3d(ctypes.Structure):
+    """Mirror of ``psa_qw3d``; the size is checked against the library."""
+
+    _fields_ = ([(n, c_i32) for n in ("nx", "ny", "nz", "flags", "i0", "j0", "k0", "pad")]
+                + [(n, c_dbl) for n in ("ax", "ay", "az", "dt", "dx", "eps")]
+                + [(n, c_ptr) for n in ("qx", "qy", "qz", "gy", "gz", "tw", "Ex", "Ey", "Ez", "Jx",
+                                        "Jy", "Jz", "Rho", "Fk", "El", "lines", "old")])
+
+
+_QP = ctypes.POINTER(PsaQw3d)
+PSA_QW3D_CURR, PSA_QW3D_RHO, PSA_QW3D_POL = 1, 2, 4
 PSA_PSTD_ABC = 1
 PSA_PSTD_MAXW = 256
 
@@ -84,6 +97,9 @@ SIGNATURES = {
     "psa_offdiag_extrema": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_xqw_table": (ctypes.c_int, [c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_dbl,
                                      c_dbl, c_dbl, c_dbl, c_i32, c_ptr, c_ptr]),
+    "psa_qw3d_size": (ctypes.c_int, []),
+    "psa_qw3d_drive": (ctypes.c_int, [_QP, _GP, c_ptr]),
+    "psa_qw3d_deposit": (ctypes.c_int, [_QP, _GP, c_ptr]),
     "psa_pstd_size": (ctypes.c_int, []),
     "psa_pstd_to_k": (ctypes.c_int, [_PP, c_ptr]),
     "psa_pstd_to_real": (ctypes.c_int, [_PP, c_ptr, c_ptr]),
@@ -125,6 +141,8 @@ def load(path=None):
     if lib.psa_group_size() != ctypes.sizeof(PsaGroup):
         raise PsaError(f"psa_group size mismatch: C {lib.psa_group_size()} vs "
                        f"ctypes {ctypes.sizeof(PsaGroup)}")
+    if lib.psa_qw3d_size() != ctypes.sizeof(PsaQw3d):
+        raise PsaError("psa_qw3d size mismatch")
     if lib.psa_pstd_size() != ctypes.sizeof(PsaPstd):
         raise PsaError(f"psa_pstd size mismatch: C {lib.psa_pstd_size()} vs "
                        f"ctypes {ctypes.sizeof(PsaPstd)}")

@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
 
 #include "psa_gemm.h"
@@ -2167,6 +2168,262 @@ extern "C" int psa_xqw_table(int32_t N, const double *Ee, const double *Eh, cons
   PSA_LAUNCH_CHECK("k_xqw");
   return PSA_OK;
 }
+
+// ===========================================================================
+// 3-D coupling (rhoPJ.QuantumWire): drive, current/charge/polarization
+// lines, placement, and ElongfromRho's 3-D FFTs.  The 3-D FFT is three
+// passes of line FFTs (fft_line, one wave per line, N = 2^a 3^b <= 512),
+// eight lines per block; for the strided axes the eight lines of a block are
+// neighbours in memory, so a row of eight elements is one 128-byte read.
+// ===========================================================================
+namespace {
+
+constexpr int Q3_LINES = 8;
+
+// lines of `narr` grids [n0][n1][n2] along `axis`; sign -1 forward, +1
+// inverse (scaled by `scale` on the way out)
+__global__ __launch_bounds__(64 * Q3_LINES) void k_q3_fft_axis(double *data, int n0, int n1, int n2, int axis,
+                                                              int sign, const double *twid, double scale) {
+  __shared__ cd xs[Q3_LINES][640];
+  __shared__ cd tw[512];
+  const int N = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
+  const long grid = (long)n0 * n1 * n2;
+  const long es = axis == 0 ? (long)n1 * n2 : (axis == 1 ? n2 : 1);
+  // lines: axis 2 -> (i0, i1); axis 1 -> (i0, i2); axis 0 -> (i1, i2)
+  const long ninner = axis == 2 ? 1 : (axis == 1 ? n2 : (long)n1 * n2);
+  const long hstride = axis == 2 ? n2 : (axis == 1 ? (long)n1 * n2 : 0);
+  const long nlines = grid / N;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = tid; i < N; i += 64 * Q3_LINES) tw[i] = ld(twid, i);
+  const long L0 = (long)blockIdx.x * Q3_LINES;  // first line of the block (within one grid)
+  double *base_arr = data + 2 * grid * blockIdx.y;
+  auto line_base = [&](long L) { return (L / ninner) * hstride + (L % ninner) * (axis == 2 ? n2 : 1); };
+  // load: element e of line l by thread (e * 8 + l) for strided axes (neighbour
+  // lines adjacent), by lane-along-e per wave for the contiguous axis
+  if (axis == 2) {
+    const long L = L0 + wv;
+    if (L < nlines) {
+      const long b = line_base(L);
+      for (int e = lane; e < N; e += 64) xs[wv][fpad(e)] = ld(base_arr, b + e);
+    }
+  } else {
+    for (int t = tid; t < N * Q3_LINES; t += 64 * Q3_LINES) {
+      const int l = t % Q3_LINES, e = t / Q3_LINES;
+      const long L = L0 + l;
+      if (L < nlines) xs[l][fpad(e)] = ld(base_arr, line_base(L) + e * es);
+    }
+  }
+  __syncthreads();
+  if (L0 + wv < nlines) fft_line(xs[wv], tw, N, sign, lane);
+  __syncthreads();
+  if (axis == 2) {
+    const long L = L0 + wv;
+    if (L < nlines) {
+      const long b = line_base(L);
+      for (int e = lane; e < N; e += 64) st(base_arr, b + e, cscale(xs[wv][fpad(e)], scale));
+    }
+  } else {
+    for (int t = tid; t < N * Q3_LINES; t += 64 * Q3_LINES) {
+      const int l = t % Q3_LINES, e = t / Q3_LINES;
+      const long L = L0 + l;
+      if (L < nlines) st(base_arr, line_base(L) + e * es, cscale(xs[l][fpad(e)], scale));
+    }
+  }
+}
+
+int q3_fft(const psa_qw3d &q, double *data, int narr, int sign, hipStream_t s) {
+  const long grid = (long)q.nx * q.ny * q.nz;
+  const int ns[3] = {q.nx, q.ny, q.nz};
+  const double *tws[3] = {q.tw, q.tw + 2L * q.nx, q.tw + 2L * (q.nx + q.ny)};
+  // inverse: z, y, x with the 1/(nx ny nz) of ifftn on the last pass
+  for (int a = 0; a < 3; ++a) {
+    const int axis = sign < 0 ? a : 2 - a;
+    const double scale = (sign > 0 && a == 2) ? 1.0 / (double)grid : 1.0;
+    const long nlines = grid / ns[axis];
+    hipLaunchKernelGGL(k_q3_fft_axis, dim3((unsigned)((nlines + Q3_LINES - 1) / Q3_LINES), narr),
+                       dim3(64 * Q3_LINES), 0, s, data, q.nx, q.ny, q.nz, axis, sign, tws[axis], scale);
+    PSA_LAUNCH_CHECK("k_q3_fft_axis");
+  }
+  return PSA_OK;
+}
+
+// EAtXYZ of Ex, Ey, Ez (usefulsubs.py:2057-2065 blend order) broadcast into
+// efield[c][w][x] of every wire
+__global__ __launch_bounds__(256) void k_q3_drive(psa_qw3d q, psa_group g) {
+  const int x = blockIdx.x * 256 + threadIdx.x, w = blockIdx.y;
+  if (x >= g.Nx) return;
+  const double wx[2] = {1.0 - q.ax, q.ax}, wy[2] = {1.0 - q.ay, q.ay}, wz[2] = {1.0 - q.az, q.az};
+  const double *F[3] = {q.Ex, q.Ey, q.Ez};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    cd f0{0.0, 0.0};
+    for (int r = 0; r < 2; ++r)
+      for (int qq = 0; qq < 2; ++qq)
+        for (int p = 0; p < 2; ++p) {
+          const int i = min(q.i0 + p, q.nx - 1), j = min(q.j0 + qq, q.ny - 1), k = min(q.k0 + r, q.nz - 1);
+          const double wgt = wx[p] * wy[qq] * wz[r];
+          const cd v = ld(F[c], ((long)i * q.ny + j) * q.nz + k);
+          f0 = cd{f0.re + wgt * v.re, f0.im + wgt * v.im};
+        }
+    st(g.efield, ((long)c * g.Nw + w) * g.Nx + x, f0);
+  }
+}
+
+// per x: lines[0..2] = sum_w (P - P_old) / dt, [3] = sum_w Rho, [4..6] =
+// sum_w P, [7] = (sum_w Rho - sum_w Rho_old) / dt; P_old <- P
+__global__ __launch_bounds__(256) void k_q3_lines(psa_qw3d q, psa_group g) {
+  const int i = blockIdx.x * 256 + threadIdx.x, nx = q.nx, W = g.Nw;
+  if (i >= nx) return;
+  cd d[3] = {}, ps[3] = {}, rs{0.0, 0.0}, ro{0.0, 0.0};
+  for (int w = 0; w < W; ++w) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const long at = ((long)c * W + w) * nx + i;
+      const cd v = ld(g.out, at), o = ld(q.old, at);
+      if (c < 3) {
+        d[c] = cadd(d[c], csub(v, o));
+        ps[c] = cadd(ps[c], v);
+      } else {
+        rs = cadd(rs, v);
+        ro = cadd(ro, o);
+      }
+      st(q.old, at, v);
+    }
+  }
+  const double idt = 1.0 / q.dt;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    st(q.lines, (long)c * nx + i, cscale(d[c], idt));
+    st(q.lines, (long)(4 + c) * nx + i, ps[c]);
+  }
+  st(q.lines, 3L * nx + i, rs);
+  st(q.lines, 7L * nx + i, cscale(csub(rs, ro), idt));
+}
+
+// CalcJfx (sequential, the reference's order) and the 1/Nw factors
+__global__ __launch_bounds__(256) void k_q3_jfx(psa_qw3d q, int W) {
+  __shared__ cd jf[512];
+  const int nx = q.nx;
+  if (threadIdx.x == 0) {
+    cd a{0.0, 0.0};
+    jf[0] = a;
+    for (int i = 1; i < nx; ++i) {
+      const cd r = ld(q.lines, 7L * nx + i);
+      a = csub(a, cscale(r, q.dx));
+      jf[i] = a;
+    }
+  }
+  __syncthreads();
+  const double iw = 1.0 / (double)W;
+  for (int i = threadIdx.x; i < nx; i += 256) {
+    const cd jx = ld(q.lines, i);
+    st(q.lines, i, cadd(cscale(jx, iw), cscale(jf[i], iw)));
+    for (int c = 1; c < 7; ++c) st(q.lines, (long)c * nx + i, cscale(ld(q.lines, (long)c * nx + i), iw));
+  }
+}
+
+// QWPlacement: grid += (gx gy) gz f[i] (gx = 1) for J, Rho; fresh P into Fk[1..3]
+__global__ __launch_bounds__(256) void k_q3_place(psa_qw3d q) {
+  const long n = (long)q.nx * q.ny * q.nz, plane = (long)q.ny * q.nz;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int i = (int)(t / plane), j = (int)((t / q.nz) % q.ny), k = (int)(t % q.nz);
+    const double gg = (1.0 * q.gy[j]) * q.gz[k];
+    if (q.flags & PSA_QW3D_CURR) {
+      double *J[3] = {q.Jx, q.Jy, q.Jz};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) st(J[c], t, cadd(ld(J[c], t), cscale(ld(q.lines, (long)c * q.nx + i), gg)));
+    }
+    if (q.flags & PSA_QW3D_RHO) st(q.Rho, t, cadd(ld(q.Rho, t), cscale(ld(q.lines, 3L * q.nx + i), gg)));
+    if (q.flags & PSA_QW3D_POL) {
+      st(q.Fk, t, ld(q.Rho, t));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) st(q.Fk, (1 + c) * n + t, cscale(ld(q.lines, (long)(4 + c) * q.nx + i), gg));
+    }
+  }
+}
+
+// the k-space loop of ElongfromRho (rhoPJ.py:526-619)
+__global__ __launch_bounds__(256) void k_q3_elong(psa_qw3d q) {
+  const long n = (long)q.nx * q.ny * q.nz, plane = (long)q.ny * q.nz;
+  const double small = (q.qx[1] - q.qx[0]) / 1e4;
+  const double ie = 1.0 / q.eps;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int i = (int)(t / plane), j = (int)((t / q.nz) % q.ny), k = (int)(t % q.nz);
+    const double qx = q.qx[i], qy = q.qy[j], qz = q.qz[k];
+    const double qsq = qx * qx + qy * qy + qz * qz + small;
+    const cd rk = ld(q.Fk, t), px = ld(q.Fk, n + t), py = ld(q.Fk, 2 * n + t), pz = ld(q.Fk, 3 * n + t);
+    const cd qdp = cadd(cadd(cscale(px, qx), cscale(py, qy)), cscale(pz, qz));
+    const cd rb = {qdp.im, -qdp.re};                              // -i q.P
+    const cd rt0 = {rk.im, -rk.re};                               // -i Rho
+    const cd rt = {rt0.re * ie / qsq, rt0.im * ie / qsq};
+    const cd pt = {-qdp.re * ie / qsq, -qdp.im * ie / qsq};
+    const cd sp = cadd(rt, pt);
+    const double qs[3] = {qx, qy, qz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      st(q.El, (long)c * n + t, cscale(sp, qs[c]));
+      st(q.El, (long)(3 + c) * n + t, cscale(pt, qs[c]));
+      st(q.El, (long)(6 + c) * n + t, cscale(rt, qs[c]));
+    }
+    st(q.El, 9L * n + t, rb);
+  }
+}
+
+bool q3_len_ok(int n) {
+  if (n < 2 || n > 512) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  return n == 1;
+}
+
+int check_qw3d(const psa_qw3d *q, const psa_group *g, bool deposit) {
+  if (!q || !g) return fail(PSA_E_INVALID, "psa_qw3d: null argument");
+  if (int rc = check_group(g, 0, g->Nw)) return rc;
+  if (!q3_len_ok(q->nx) || !q3_len_ok(q->ny) || !q3_len_ok(q->nz))
+    return fail(PSA_E_INVALID, "psa_qw3d: nx, ny, nz must be 2^a 3^b in [2, 512]");
+  if (g->Nx != q->nx) return fail(PSA_E_INVALID, "psa_qw3d: the group's Nx must equal nx");
+  if (q->i0 < 0 || q->i0 >= q->nx || q->j0 < 0 || q->j0 >= q->ny || q->k0 < 0 || q->k0 >= q->nz)
+    return fail(PSA_E_INVALID, "psa_qw3d: EAtXYZ cell outside the grid");
+  if (!deposit && (!q->Ex || !q->Ey || !q->Ez)) return fail(PSA_E_INVALID, "psa_qw3d_drive: missing field");
+  if (deposit && (!q->qx || !q->qy || !q->qz || !q->gy || !q->gz || !q->tw || !q->lines || !q->old ||
+                  !q->Jx || !q->Jy || !q->Jz || !q->Rho || !q->Fk || !q->El))
+    return fail(PSA_E_INVALID, "psa_qw3d_deposit: missing buffer");
+  if (!(q->dt > 0.0) || !(q->eps > 0.0)) return fail(PSA_E_INVALID, "psa_qw3d: dt and eps must be positive");
+  return PSA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psa_qw3d_size(void) { return (int)sizeof(psa_qw3d); }
+
+int psa_qw3d_drive(const psa_qw3d *q, const psa_group *g, void *stream) {
+  if (int rc = check_qw3d(q, g, false)) return rc;
+  hipLaunchKernelGGL(k_q3_drive, dim3((g->Nx + 255) / 256, g->Nw), dim3(256), 0, (hipStream_t)stream, *q, *g);
+  PSA_LAUNCH_CHECK("k_q3_drive");
+  return PSA_OK;
+}
+
+int psa_qw3d_deposit(const psa_qw3d *q, const psa_group *g, void *stream) {
+  if (int rc = check_qw3d(q, g, true)) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_q3_lines, dim3((q->nx + 255) / 256), dim3(256), 0, s, *q, *g);
+  PSA_LAUNCH_CHECK("k_q3_lines");
+  hipLaunchKernelGGL(k_q3_jfx, dim3(1), dim3(256), 0, s, *q, g->Nw);
+  PSA_LAUNCH_CHECK("k_q3_jfx");
+  const long n = (long)q->nx * q->ny * q->nz;
+  const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_q3_place, dim3(blocks), dim3(256), 0, s, *q);
+  PSA_LAUNCH_CHECK("k_q3_place");
+  if (!(q->flags & PSA_QW3D_POL)) return PSA_OK;
+  if (int rc = q3_fft(*q, q->Fk, 4, -1, s)) return rc;
+  hipLaunchKernelGGL(k_q3_elong, dim3(blocks), dim3(256), 0, s, *q);
+  PSA_LAUNCH_CHECK("k_q3_elong");
+  return q3_fft(*q, q->El, 10, +1, s);
+}
+
+}  // extern "C"
 
 // ===========================================================================
 // 1-D pseudo-spectral propagator (psa_pstd.h)
