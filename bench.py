@@ -317,6 +317,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=150)
     ap.add_argument("--no-transmission", action="store_true")
     ap.add_argument("--no-rk4", action="store_true")
+    ap.add_argument("--no-qw3d", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only (the SBE wire batch is stepped on a single "
@@ -397,6 +398,7 @@ def main():
             res["transmission"] = transmission_sample(seed)
         except Exception as ex:  # report, keep the batch number
             res["transmission"] = {"error": repr(ex)}
+    if not args.no_qw3d:
         try:
             print("[bench] 3-D coupling", file=sys.stderr, flush=True)
             res["qw3d"] = qw3d_sample(seed)

@@ -820,6 +820,67 @@ __device__ void fft_line(cd *x, const cd *tw, int N, int sign, int lane) {
   }
 }
 
+// fft_line for nl lines of one wave at once (line l at x + l LS): the
+// butterflies of all lines share the lanes, so short lines keep the wave
+// busy.  nl (N / R) <= 256 for every radix R of the plan.
+__device__ void fft_lines(cd *x, int LS, int nl, const cd *tw, int N, int sign, int lane) {
+  int Ns = 1, rem = N;
+  while (rem > 1) {
+    const int R = (rem % 4 == 0) ? 4 : ((rem % 2 == 0) ? 2 : 3);
+    const int M = N / R, step = N / (Ns * R), tot = nl * M;
+    const uint64_t mg = ((1ull << 32) + Ns - 1) / Ns;
+    cd o[4][4];
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int jg = lane + 64 * it;
+      if (jg >= tot) break;
+      const int l = jg / M, j = jg - l * M;
+      const cd *xl = x + l * LS;
+      const int k = j - Ns * (int)(((uint64_t)j * mg) >> 32);
+      const cd a0 = xl[fpad(j)];
+      if (R == 4) {
+        const cd a1 = cmul(xl[fpad(j + M)], tw_get(tw, k * step, sign));
+        const cd a2 = cmul(xl[fpad(j + 2 * M)], tw_get(tw, 2 * k * step, sign));
+        const cd a3 = cmul(xl[fpad(j + 3 * M)], tw_get(tw, 3 * k * step, sign));
+        const cd s02 = cadd(a0, a2), d02 = csub(a0, a2);
+        const cd s13 = cadd(a1, a3), d13 = mul_is(csub(a1, a3), sign);
+        o[it][0] = cadd(s02, s13);
+        o[it][1] = cadd(d02, d13);
+        o[it][2] = csub(s02, s13);
+        o[it][3] = csub(d02, d13);
+      } else if (R == 2) {
+        const cd b = cmul(xl[fpad(j + M)], tw_get(tw, k * step, sign));
+        o[it][0] = cadd(a0, b);
+        o[it][1] = csub(a0, b);
+      } else {
+        const double c3 = -0.5, s3 = sign * 0.86602540378443864676;
+        const cd w1 = {c3, s3}, w2 = {c3, -s3};
+        const cd b = cmul(xl[fpad(j + M)], tw_get(tw, k * step, sign));
+        const cd c = cmul(xl[fpad(j + 2 * M)], tw_get(tw, 2 * k * step, sign));
+        o[it][0] = cadd(a0, cadd(b, c));
+        o[it][1] = cadd(a0, cadd(cmul(b, w1), cmul(c, w2)));
+        o[it][2] = cadd(a0, cadd(cmul(b, w2), cmul(c, w1)));
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int jg = lane + 64 * it;
+      if (jg >= tot) break;
+      const int l = jg / M, j = jg - l * M;
+      cd *xl = x + l * LS;
+      const int k = j - Ns * (int)(((uint64_t)j * mg) >> 32);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r < R) xl[fpad(base + r * Ns)] = o[it][r];
+    }
+    wave_lds_sync();
+    Ns *= R;
+    rem /= R;
+  }
+}
+
 // The same FFT with N known at compile time: the stage sequence, moduli
 // and strides are constants, and each stage reads its own twiddle table
 // tw[TOFF + (r-1) Ns + k] = w^(r k N / (R Ns)), contiguous in k, so the
@@ -2182,11 +2243,17 @@ constexpr int Q3_LINES = 8;
 
 // lines of `narr` grids [n0][n1][n2] along `axis`; sign -1 forward, +1
 // inverse (scaled by `scale` on the way out)
+// LDS (dynamic, sized for the axis length): the block's lines of fpad(N)
+// slots, then N twiddles.  Each wave transforms q3_lpw(N) lines together
+// (short lines share a wave), so a block holds Q3_LINES q3_lpw(N) lines.
+__host__ __device__ inline int q3_line_slots(int N) { return N + N / 4; }
+__host__ __device__ inline int q3_lpw(int N) { return N >= 256 ? 1 : 256 / N; }
 __global__ __launch_bounds__(64 * Q3_LINES) void k_q3_fft_axis(double *data, int n0, int n1, int n2, int axis,
                                                               int sign, const double *twid, double scale) {
-  __shared__ cd xs[Q3_LINES][640];
-  __shared__ cd tw[512];
+  extern __shared__ cd q3_lds[];
   const int N = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
+  const int LS = q3_line_slots(N), LPW = q3_lpw(N), BL = Q3_LINES * LPW;
+  cd *const tw = q3_lds + BL * LS;
   const long grid = (long)n0 * n1 * n2;
   const long es = axis == 0 ? (long)n1 * n2 : (axis == 1 ? n2 : 1);
   // lines: axis 2 -> (i0, i1); axis 1 -> (i0, i2); axis 0 -> (i1, i2)
@@ -2195,38 +2262,42 @@ __global__ __launch_bounds__(64 * Q3_LINES) void k_q3_fft_axis(double *data, int
   const long nlines = grid / N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int i = tid; i < N; i += 64 * Q3_LINES) tw[i] = ld(twid, i);
-  const long L0 = (long)blockIdx.x * Q3_LINES;  // first line of the block (within one grid)
+  const long L0 = (long)blockIdx.x * BL;  // first line of the block (within one grid)
   double *base_arr = data + 2 * grid * blockIdx.y;
   auto line_base = [&](long L) { return (L / ninner) * hstride + (L % ninner) * (axis == 2 ? n2 : 1); };
-  // load: element e of line l by thread (e * 8 + l) for strided axes (neighbour
-  // lines adjacent), by lane-along-e per wave for the contiguous axis
+  // element e of block line l: thread t = e BL + l for the strided axes (the
+  // block's lines are neighbours in memory), t = l N + e for the contiguous axis
   if (axis == 2) {
-    const long L = L0 + wv;
-    if (L < nlines) {
-      const long b = line_base(L);
-      for (int e = lane; e < N; e += 64) xs[wv][fpad(e)] = ld(base_arr, b + e);
+    for (int t = tid; t < N * BL; t += 64 * Q3_LINES) {
+      const int l = t / N, e = t - l * N;
+      const long L = L0 + l;
+      if (L < nlines) q3_lds[l * LS + fpad(e)] = ld(base_arr, line_base(L) + e);
     }
   } else {
-    for (int t = tid; t < N * Q3_LINES; t += 64 * Q3_LINES) {
-      const int l = t % Q3_LINES, e = t / Q3_LINES;
+    for (int t = tid; t < N * BL; t += 64 * Q3_LINES) {
+      const int l = t % BL, e = t / BL;
       const long L = L0 + l;
-      if (L < nlines) xs[l][fpad(e)] = ld(base_arr, line_base(L) + e * es);
+      if (L < nlines) q3_lds[l * LS + fpad(e)] = ld(base_arr, line_base(L) + e * es);
     }
   }
   __syncthreads();
-  if (L0 + wv < nlines) fft_line(xs[wv], tw, N, sign, lane);
+  {
+    const long Lw = L0 + (long)wv * LPW;
+    const int nl = (int)(nlines - Lw < LPW ? nlines - Lw : LPW);
+    if (nl > 0) fft_lines(q3_lds + wv * LPW * LS, LS, nl, tw, N, sign, lane);
+  }
   __syncthreads();
   if (axis == 2) {
-    const long L = L0 + wv;
-    if (L < nlines) {
-      const long b = line_base(L);
-      for (int e = lane; e < N; e += 64) st(base_arr, b + e, cscale(xs[wv][fpad(e)], scale));
+    for (int t = tid; t < N * BL; t += 64 * Q3_LINES) {
+      const int l = t / N, e = t - l * N;
+      const long L = L0 + l;
+      if (L < nlines) st(base_arr, line_base(L) + e, cscale(q3_lds[l * LS + fpad(e)], scale));
     }
   } else {
-    for (int t = tid; t < N * Q3_LINES; t += 64 * Q3_LINES) {
-      const int l = t % Q3_LINES, e = t / Q3_LINES;
+    for (int t = tid; t < N * BL; t += 64 * Q3_LINES) {
+      const int l = t % BL, e = t / BL;
       const long L = L0 + l;
-      if (L < nlines) st(base_arr, line_base(L) + e * es, cscale(xs[l][fpad(e)], scale));
+      if (L < nlines) st(base_arr, line_base(L) + e * es, cscale(q3_lds[l * LS + fpad(e)], scale));
     }
   }
 }
@@ -2240,8 +2311,19 @@ int q3_fft(const psa_qw3d &q, double *data, int narr, int sign, hipStream_t s) {
     const int axis = sign < 0 ? a : 2 - a;
     const double scale = (sign > 0 && a == 2) ? 1.0 / (double)grid : 1.0;
     const long nlines = grid / ns[axis];
-    hipLaunchKernelGGL(k_q3_fft_axis, dim3((unsigned)((nlines + Q3_LINES - 1) / Q3_LINES), narr),
-                       dim3(64 * Q3_LINES), 0, s, data, q.nx, q.ny, q.nz, axis, sign, tws[axis], scale);
+    const int bl = Q3_LINES * q3_lpw(ns[axis]);
+    const size_t shm = sizeof(cd) * (size_t)(bl * q3_line_slots(ns[axis]) + ns[axis]);
+    static bool attr_set = false;
+    if (!attr_set) {  // up to 88 KB for 512-point lines
+      if (int rc = hip_check(hipFuncSetAttribute((const void *)k_q3_fft_axis,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)(sizeof(cd) * (Q3_LINES * q3_line_slots(512) + 512))),
+                             "hipFuncSetAttribute(k_q3_fft_axis)"))
+        return rc;
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(k_q3_fft_axis, dim3((unsigned)((nlines + bl - 1) / bl), narr),
+                       dim3(64 * Q3_LINES), shm, s, data, q.nx, q.ny, q.nz, axis, sign, tws[axis], scale);
     PSA_LAUNCH_CHECK("k_q3_fft_axis");
   }
   return PSA_OK;
@@ -2302,14 +2384,15 @@ __global__ __launch_bounds__(256) void k_q3_lines(psa_qw3d q, psa_group g) {
 
 // CalcJfx (sequential, the reference's order) and the 1/Nw factors
 __global__ __launch_bounds__(256) void k_q3_jfx(psa_qw3d q, int W) {
-  __shared__ cd jf[512];
+  __shared__ cd jf[512], dr[512];
   const int nx = q.nx;
+  for (int i = threadIdx.x; i < nx; i += 256) dr[i] = ld(q.lines, 7L * nx + i);
+  __syncthreads();
   if (threadIdx.x == 0) {
     cd a{0.0, 0.0};
     jf[0] = a;
     for (int i = 1; i < nx; ++i) {
-      const cd r = ld(q.lines, 7L * nx + i);
-      a = csub(a, cscale(r, q.dx));
+      a = csub(a, cscale(dr[i], q.dx));
       jf[i] = a;
     }
   }
