@@ -235,6 +235,14 @@ class SBESolver:
         _print_it(np.diag(D3), kr, xxx, f"Wire/nh/nh.{ws}.k.")
         _print_it(hd[0] / eV, kr, xxx, f"Wire/Ee/Ee.{ws}.k.")
         _print_it(hd[1] / eV, kr, xxx, f"Wire/Eh/Eh.{ws}.k.")
+        if w == int(np.ceil(self.Nw / 2.0)):
+            # the middle wire's statistics on the console (SBEs.py:5755-5767),
+            # with CalcVD / CalcPD (dcfield.py:128-178)
+            for lab, n, m in (("Elec", np.diag(C3), self.me), ("Hole", np.diag(D3), self.mh)):
+                vd = np.sum(np.real(n) * hbar * kr / m) / np.sum(np.real(n) + 1e-200)
+                pd = np.sum(np.abs(n) * hbar * kr) / np.sum(np.abs(n) + 1e-200)
+                print(f"{xxx:8d} {lab} {np.sum(np.real(n)):18.6e} {np.max(np.real(n)):18.6e} "
+                      f"{np.min(np.real(n)):18.6e} {vd:18.6e} {np.real(pd):18.6e}")
         if self.DiagDph:
             gs = 8 * N + 4 * (2 * N - 1)  # PSA_PREP_GAME
             _print_itr(pr[gs:gs + N], kr, xxx, f"Wire/Ge/Ge.{ws}.k.", width=5)
