@@ -20,9 +20,11 @@ Design (one process, one GPU):
   same kernel launches; pools of different Nk run on their own HIP streams,
   so their kernels overlap on the device.
 * A tick of a pool is drive -> QWCalculator step -> record on the device
-  (``psa_drive``, ``psa_step``, ``psa_record``) for the occupied prefix,
+  (``psa_drive``, ``psa_step`` -- or ``psa_step_rk4`` for a server built
+  with ``integrator="rk4"`` -- and ``psa_record``) for the occupied prefix,
   optionally replayed from a captured HIP graph (one per pointer parity of
-  the level swap and prefix length).
+  the leapfrog's level swap and prefix length; RK4 updates the state in
+  place, so one per prefix length).
 * Requests whose step count is reached are harvested (trace and final
   populations copied out), their slot is freed and reused.
 
@@ -90,11 +92,14 @@ def wire_length(nk, me_rel=0.067, gap_ev=1.5):
 class SlotPool:
     """Fixed-capacity slots of one Nk on one stream."""
 
-    def __init__(self, nk, capacity, rr, dt, prec, maxsteps, graphs=True, prm_kw=None, autotune=True):
+    def __init__(self, nk, capacity, rr, dt, prec, maxsteps, graphs=True, prm_kw=None, autotune=True,
+                 integrator="leapfrog"):
         prm_kw = dict(prm_kw or {})
         prm_kw.setdefault("me_rel", 0.067)
         self.prm = tables.QWParams(L=wire_length(nk, prm_kw["me_rel"]), **prm_kw)
-        self.group = WireGroup(self.prm, capacity, rr, wire_ids=np.ones(capacity, np.int32), prec=prec)
+        self.group = WireGroup(self.prm, capacity, rr, wire_ids=np.ones(capacity, np.int32), prec=prec,
+                               integrator=integrator)
+        self.rk4 = integrator == "rk4"
         if self.group.N != nk:
             raise ValueError(f"k-grid gave N={self.group.N}, wanted {nk}")
         self.nk, self.cap, self.dt = nk, int(capacity), float(dt)
@@ -148,7 +153,8 @@ class SlotPool:
     def _launch(self, hi, S, stream_handle):
         c = capi.check
         c(self.lib.psa_drive(S, 0, hi, self.drive.data_ptr(), self.dt, stream_handle))
-        c(self.lib.psa_step(S, 0, hi, self.dt, stream_handle))
+        step = self.lib.psa_step_rk4 if self.rk4 else self.lib.psa_step
+        c(step(S, 0, hi, self.dt, stream_handle))
         c(self.lib.psa_record(S, 0, hi, self.xmid, self.trace.data_ptr(), self.count.data_ptr(),
                               self.maxsteps, stream_handle))
 
@@ -172,8 +178,9 @@ class SlotPool:
                 g.replay()
             else:
                 self._launch(hi, S, ctypes.c_void_p(self.stream.cuda_stream))
-        self.group.swap_levels()
-        self.parity ^= 1
+        if not self.rk4:  # the leapfrog's new level becomes level 3; RK4 updates it in place
+            self.group.swap_levels()
+            self.parity ^= 1
         self.ticks += 1
         for i, s in enumerate(self.slots):
             if s is not None:
@@ -203,10 +210,10 @@ class SlotPool:
         return out
 
 
-def slot_bytes(nk, nx, prec, maxsteps):
+def slot_bytes(nk, nx, prec, maxsteps, integrator="leapfrog"):
     """Per-slot device bytes of a pool (state, scratch, split planes, trace)."""
     N = nk
-    b = 12 * N * N * 16                      # pA..Hho
+    b = (15 if integrator == "rk4" else 12) * N * N * 16  # pA..Hho (+ the RK4 accumulators)
     b += 2 * N * 16 + (8 * N + 4 * (2 * N - 1) + 5 * N) * 8
     b += 2 * N * (2 * N - 1) * 8 + 11 * N * 8 + 3 * (2 * N - 1) * 16
     b += 3 * nx * 16 + 4 * nx * 16 + (9 * (2 * N + 2) + 8 * (2 * N - 1) + 8) * 8
@@ -229,9 +236,10 @@ class Server:
     """Admission by HBM footprint, continuous batching over Nk pools."""
 
     def __init__(self, rr, dt=10e-18, prec="bf16x3", hbm_budget=64 << 30, max_slots=64,
-                 maxsteps=20000, graphs=True, prm_kw=None):
+                 maxsteps=20000, graphs=True, prm_kw=None, integrator="leapfrog"):
         self.rr = np.asarray(rr, dtype=float)
         self.dt, self.prec = float(dt), prec
+        self.integrator = integrator
         self.budget, self.max_slots = int(hbm_budget), int(max_slots)
         self.maxsteps, self.graphs = int(maxsteps), graphs
         self.prm_kw = prm_kw
@@ -241,12 +249,13 @@ class Server:
     def pool(self, nk):
         p = self.pools.get(nk)
         if p is None:
-            per = slot_bytes(nk, len(self.rr), self.prec, self.maxsteps)
+            per = slot_bytes(nk, len(self.rr), self.prec, self.maxsteps, self.integrator)
             room = self.budget - self.used - pool_bytes(nk)
             cap = min(self.max_slots, room // per)
             if cap < 1:
                 return None
-            p = SlotPool(nk, cap, self.rr, self.dt, self.prec, self.maxsteps, self.graphs, self.prm_kw)
+            p = SlotPool(nk, cap, self.rr, self.dt, self.prec, self.maxsteps, self.graphs, self.prm_kw,
+                         integrator=self.integrator)
             self.used += p.group.footprint_bytes() + p.trace.numel() * 8
             self.pools[nk] = p
         return p

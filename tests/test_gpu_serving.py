@@ -23,21 +23,22 @@ def _req(rid, nk, e0=3e7, fwhm=2e-15, det=0.02, nsteps=40, arrival=0.0):
                       arrival=arrival)
 
 
-def _serve(reqs, graphs=True, prec="fp64"):
+def _serve(reqs, graphs=True, prec="fp64", integrator="leapfrog"):
     from pulsesuite_amd.serving import Server
     srv = Server(RR, dt=DT, prec=prec, hbm_budget=8 << 30, max_slots=16, maxsteps=200,
-                 graphs=graphs)
+                 graphs=graphs, integrator=integrator)
     done = srv.run(reqs)
     return {r.rid: r for r in done}, srv
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("integrator", ["leapfrog", "rk4"])
+def test_graph_replay_matches_eager(integrator):
     mk = lambda: [_req(0, 64, nsteps=30), _req(1, 64, e0=1e7, nsteps=45),  # noqa: E731
                   _req(2, 64, fwhm=3e-15, nsteps=20)]
-    print("[serve] eager", flush=True)
-    a, _ = _serve(mk(), graphs=False)
-    print("[serve] graphs", flush=True)
-    b, srv = _serve(mk(), graphs=True)
+    print(f"[serve] eager {integrator}", flush=True)
+    a, _ = _serve(mk(), graphs=False, integrator=integrator)
+    print(f"[serve] graphs {integrator}", flush=True)
+    b, srv = _serve(mk(), graphs=True, integrator=integrator)
     assert sum(len(p.graphs) for p in srv.pools.values()) > 0
     for rid in a:
         assert a[rid].trace.shape == b[rid].trace.shape
@@ -45,14 +46,14 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(a[rid].final["p"], b[rid].final["p"]), rid
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
-def test_request_same_alone_and_in_mix(prec):
-    print(f"[serve] alone {prec}", flush=True)
-    alone, _ = _serve([_req(7, 64, nsteps=50)], prec=prec)
+@pytest.mark.parametrize("prec,integrator", [("fp64", "leapfrog"), ("bf16x3", "leapfrog"), ("bf16x3", "rk4")])
+def test_request_same_alone_and_in_mix(prec, integrator):
+    print(f"[serve] alone {prec} {integrator}", flush=True)
+    alone, _ = _serve([_req(7, 64, nsteps=50)], prec=prec, integrator=integrator)
     mix = [_req(1, 64, e0=5e7, nsteps=80), _req(2, 128, nsteps=40), _req(3, 64, fwhm=3e-15, nsteps=30),
            _req(7, 64, nsteps=50, arrival=0.002), _req(4, 128, e0=1e7, nsteps=60, arrival=0.001)]
-    print(f"[serve] mix {prec}", flush=True)
-    got, srv = _serve(mix, prec=prec)
+    print(f"[serve] mix {prec} {integrator}", flush=True)
+    got, srv = _serve(mix, prec=prec, integrator=integrator)
     assert set(got) == {1, 2, 3, 4, 7}
     a, b = alone[7], got[7]
     assert np.isfinite(a.trace).all()
