@@ -398,6 +398,46 @@ int psa_pstd_gather(const psa_pstd *p, const psa_group *g, int32_t w0, int32_t n
 int psa_pstd_scatter(const psa_pstd *p, const psa_group *g, int32_t w0, int32_t nw,
                      int32_t slot0, void *stream);
 
+/* ---- Module API on the caller's dense arrays (coulomb.py, phonons.py) ----
+ *
+ * The per-call functions of the reference's coulomb and phonons modules for
+ * any k-grid: arrays in the reference's own layouts (row-major): VC
+ * [N][N][3], E1D [N][N], k3 [N][N][N] (1-based entries, 0 = invalid),
+ * Ceh/Cee/Chh [(N+1)^3] filled at 1-based positions (the zero planes are
+ * left as allocated), complex occupations interleaved (re, im).  All device
+ * pointers; N <= 1024.  Win/Wout receive += like the reference. */
+
+/* MakeK3 (coulomb.py:271-299) into k3 and CalcMBArrays (coulomb.py:367-499,
+ * GaussDelta 245-268; lorentz != 0: the LorentzDelta branch) into Ceh, Cee,
+ * Chh.  Replaces CoulombModule.__init__'s table loops (coulomb.py:1124-1164). */
+int psa_mod_mb_arrays(int32_t N, const double *Ee, const double *Eh, double ge, double gh,
+                      double hbar, int32_t lorentz, int32_t *k3, double *Ceh, double *Cee,
+                      double *Chh, void *stream);
+
+/* CoulombModule.CalcScreenedArrays (coulomb.py:1195-1230) with ne, nh given:
+ * n1D = min(sum(Re ne + Re nh) / 2 / L, nmax); E1D = Eps1D(n1D) (coulomb.py:
+ * 1172-1193) when screened, else 1; VC[.][.][c] = V0[c] / E1D (or V0).
+ * ne = nh = NULL: Eps1D(n1d) alone into E1D (VC may be NULL).
+ * V0 [3][N][N] = Veh0, Vee0, Vhh0; chi [2][N][N] = Chi1De, Chi1Dh;
+ * q [2][N][N] = qe, qh. */
+int psa_mod_screen(int32_t N, int32_t screened, double L, double nmax, const double *ne,
+                   const double *nh, double n1d, const double *V0, const double *chi,
+                   const double *q, double *VC, double *E1D, void *stream);
+
+/* MBCE / MBCE2 (hole = 0; coulomb.py:897-1012, 1283-1311; Cx = Cee) and
+ * MBCH (hole = 1; coulomb.py:1015-1071, 1313-1326; Cx = Chh) on the
+ * occupations |ne0|, |nh0| (complex [N]). */
+int psa_mod_mbc(int32_t N, int32_t hole, const double *ne0, const double *nh0, const double *VC,
+                const int32_t *k3, const double *Ceh, const double *Cx, double *Win, double *Wout,
+                void *stream);
+
+/* MBPE (hole = 0; phonons.py:139-202; P = EP, PT = EPT) and MBPH (hole = 1;
+ * phonons.py:205-268; P = HP, PT = HPT) on the complex occupations n [N]
+ * (their real part enters the real rates). */
+int psa_mod_mbp(int32_t N, int32_t hole, const double *n, const double *VC, const double *E1D,
+                const double *P, const double *PT, double Vscale, double *Win, double *Wout,
+                void *stream);
+
 #ifdef __cplusplus
 }
 #endif

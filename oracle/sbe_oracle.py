@@ -693,9 +693,10 @@ def spont_emission(g, t, ne, nh, VC):
     return np.sum(H * rho0(H) * t.square[None, :], axis=1) * dhw
 
 
-def mb_tables(g, kind, ge, gh):
-    """Stored (N+1)^3 tables of CalcMBArrays (coulomb.py:367-499, Gaussian
-    branch), index 0 planes zero, 1-based storage."""
+def mb_tables(g, kind, ge, gh, lorentz=False):
+    """Stored (N+1)^3 tables of CalcMBArrays (coulomb.py:367-499; Gaussian
+    branch 448-497, ``lorentz``: the LorentzDelta branch 405-446), index 0
+    planes zero, 1-based storage."""
     N = g.Nk
     Ee, Eh = g.Ee, g.Eh
     geh = (ge + gh) / 2.0
@@ -708,7 +709,10 @@ def mb_tables(g, kind, ge, gh):
         ok = (k30 >= 0) & (k30 < N)
         kk = np.clip(k30, 0, N - 1)
         Ed = Ee[a] + Eh[b] - Eh[kk] - Ee[c]
-        val = twopi / hbar * (a != c) * (b != k30) * gauss_delta(Ed, hbar * geh)
+        if lorentz:
+            val = 2.0 * geh * (a != c) * (b != k30) / (Ed**2 + (hbar * geh) ** 2)
+        else:
+            val = twopi / hbar * (a != c) * (b != k30) * gauss_delta(Ed, hbar * geh)
     else:
         E = Ee if kind == "ee" else Eh
         gam = ge if kind == "ee" else gh
@@ -716,7 +720,10 @@ def mb_tables(g, kind, ge, gh):
         ok = (k30 >= 0) & (k30 < N)
         kk = np.clip(k30, 0, N - 1)
         Ed = E[a] + E[b] - E[kk] - E[c]
-        val = twopi / hbar * (a != c) * (b != k30) * gauss_delta(Ed, hbar * gam)
+        if lorentz:
+            val = 2.0 * gam * (a != c) * (b != k30) / (Ed**2 + (hbar * gam) ** 2)
+        else:
+            val = twopi / hbar * (a != c) * (b != k30) * gauss_delta(Ed, hbar * gam)
     S[1:, 1:, 1:] = np.where(ok, val, 0.0)
     return S
 

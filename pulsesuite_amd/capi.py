@@ -107,6 +107,14 @@ SIGNATURES = {
     "psa_pstd_advance": (ctypes.c_int, [_PP, c_ptr]),
     "psa_pstd_gather": (ctypes.c_int, [_PP, _GP, c_i32, c_i32, c_i32, c_ptr]),
     "psa_pstd_scatter": (ctypes.c_int, [_PP, _GP, c_i32, c_i32, c_i32, c_ptr]),
+    "psa_mod_mb_arrays": (ctypes.c_int, [c_i32, c_ptr, c_ptr, c_dbl, c_dbl, c_dbl, c_i32, c_ptr, c_ptr,
+                                         c_ptr, c_ptr, c_ptr]),
+    "psa_mod_screen": (ctypes.c_int, [c_i32, c_i32, c_dbl, c_dbl, c_ptr, c_ptr, c_dbl, c_ptr, c_ptr,
+                                      c_ptr, c_ptr, c_ptr, c_ptr]),
+    "psa_mod_mbc": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr,
+                                   c_ptr]),
+    "psa_mod_mbp": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_ptr, c_ptr,
+                                   c_ptr]),
 }
 
 _lib = None

@@ -12,6 +12,7 @@
 #include "psa_offdiag.h"
 #include "psa_calch.h"
 #include "psa_pstd.h"
+#include "psa_modules.h"
 
 using namespace psa;
 
@@ -2609,6 +2610,58 @@ int psa_pstd_scatter(const psa_pstd *p, const psa_group *g, int32_t w0, int32_t 
   hipLaunchKernelGGL(pstd::k_pstd_scatter, dim3((g->Nx + 255) / 256, nw, 2), dim3(256), 0, (hipStream_t)stream,
                      *p, *g, w0, slot0);
   PSA_LAUNCH_CHECK("k_pstd_scatter");
+  return PSA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Module API on the caller's dense arrays (pulsesuite_amd.coulomb / .phonons)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int psa_mod_mb_arrays(int32_t N, const double *Ee, const double *Eh, double ge, double gh, double hbar,
+                      int32_t lorentz, int32_t *k3, double *Ceh, double *Cee, double *Chh, void *stream) {
+  if (N < 1 || N > 1024 || !Ee || !Eh || !k3 || !Ceh || !Cee || !Chh)
+    return fail(PSA_E_INVALID, "psa_mod_mb_arrays: bad arguments");
+  hipLaunchKernelGGL(mod::k_mb_arrays, dim3(N, N), dim3(N >= 256 ? 256 : 64), 0, (hipStream_t)stream, N, Ee, Eh,
+                     ge, gh, hbar, (int)lorentz, k3, Ceh, Cee, Chh);
+  PSA_LAUNCH_CHECK("k_mb_arrays");
+  return PSA_OK;
+}
+
+int psa_mod_screen(int32_t N, int32_t screened, double L, double nmax, const double *ne, const double *nh,
+                   double n1d, const double *V0, const double *chi, const double *q, double *VC, double *E1D,
+                   void *stream) {
+  if (N < 1 || N > 1024 || !E1D || (!ne) != (!nh) || (VC && !V0) || (screened && (!chi || !q)) ||
+      (screened && ne && !(L > 0.0)) || (!ne && !screened))
+    return fail(PSA_E_INVALID, "psa_mod_screen: bad arguments");
+  const long NN = (long)N * N;
+  hipLaunchKernelGGL(mod::k_screen, dim3((unsigned)((NN + mod::RED_T - 1) / mod::RED_T)), dim3(mod::RED_T), 0,
+                     (hipStream_t)stream, N, (int)(screened != 0), L, nmax, reinterpret_cast<const cd *>(ne),
+                     reinterpret_cast<const cd *>(nh), n1d, V0, chi, q, VC, E1D);
+  PSA_LAUNCH_CHECK("k_screen");
+  return PSA_OK;
+}
+
+int psa_mod_mbc(int32_t N, int32_t hole, const double *ne0, const double *nh0, const double *VC, const int32_t *k3,
+                const double *Ceh, const double *Cx, double *Win, double *Wout, void *stream) {
+  if (N < 1 || N > 1024 || (hole != 0 && hole != 1) || !ne0 || !nh0 || !VC || !k3 || !Ceh || !Cx || !Win || !Wout)
+    return fail(PSA_E_INVALID, "psa_mod_mbc: bad arguments");
+  hipLaunchKernelGGL(mod::k_mbc, dim3(N), dim3(mod::RED_T), 0, (hipStream_t)stream, N, (int)hole,
+                     reinterpret_cast<const cd *>(ne0), reinterpret_cast<const cd *>(nh0), VC, k3, Ceh, Cx, Win,
+                     Wout);
+  PSA_LAUNCH_CHECK("k_mbc");
+  return PSA_OK;
+}
+
+int psa_mod_mbp(int32_t N, int32_t hole, const double *n, const double *VC, const double *E1D, const double *P,
+                const double *PT, double Vscale, double *Win, double *Wout, void *stream) {
+  if (N < 1 || N > 1024 || (hole != 0 && hole != 1) || !n || !VC || !E1D || !P || !PT || !Win || !Wout)
+    return fail(PSA_E_INVALID, "psa_mod_mbp: bad arguments");
+  hipLaunchKernelGGL(mod::k_mbp, dim3(N), dim3(mod::RED_T), 0, (hipStream_t)stream, N, (int)hole,
+                     reinterpret_cast<const cd *>(n), VC, E1D, P, PT, Vscale, Win, Wout);
+  PSA_LAUNCH_CHECK("k_mbp");
   return PSA_OK;
 }
 
