@@ -438,6 +438,47 @@ int psa_mod_mbp(int32_t N, int32_t hole, const double *n, const double *VC, cons
                 const double *P, const double *PT, double Vscale, double *Win, double *Wout,
                 void *stream);
 
+/* ---- Module API: QWOptics methods (qwoptics.py:565-1090) ----
+ * Complex arrays interleaved (re, im), row-major; all device pointers. */
+
+/* FFTG (sign -1, scale -1/n; usefulsubs.py:2245-2268) and iFFTG (sign +1,
+ * scale -1; usefulsubs.py:2271-2297) of nvec vectors of length n as direct
+ * DFTs: out[v][k] = scale sum_j in[v][j] exp(sign 2 pi i j k / n); in != out. */
+int psa_mod_dft(int32_t n, int32_t nvec, int32_t sign, double scale, const double *in, double *out,
+                void *stream);
+
+/* rescale_1D (spliner.py:153-188, the cubic spline being linear in the
+ * data) as the real matrix M [nout][nin]: out[v][r] = w[r] sum_j M[r][j]
+ * f(in[v][j]), f = Re if real_in (Prop2QW's window-then-real-part,
+ * qwoptics.py:692-713), w = NULL for 1.  in, out complex; in != out. */
+int psa_mod_resample(int32_t nout, int32_t nin, int32_t nvec, const double *M, const double *w,
+                     int32_t real_in, const double *in, double *out, void *stream);
+
+/* QWOptics.QWPolarization3 before its FFTG (qwoptics.py:49-76, 809-940):
+ * out[c][r] (complex [3][Nr], imaginary parts 0) = scale s_c W[r]
+ * sum_{ke,kh} Re(p[kh][ke] V_c[kh][ke] E[ke][r] conj(E[kh][r])) with
+ * V = Xvc0, Yvc0, Zvc0 (complex [N][N]), E = Expikr (complex [N][Nr]),
+ * s = (1, syw, sz), scale = 2 ehint / area / L. */
+int psa_mod_pol3(int32_t N, int32_t Nr, const double *p, const double *Xvc, const double *Yvc,
+                 const double *Zvc, const double *E, const double *W, double syw, double sz,
+                 double scale, double *out, void *stream);
+
+/* QWOptics.QWRho5 before its FFTG (qwoptics.py:360-374, 942-1071): out
+ * (complex [2][Nr]) = re, rh summed from C, conj(D) (complex [N][N]) with
+ * E = Expikr and W, minus the mean of the real parts of the first and last
+ * 10 points, scaled to sum|ne| + 1e-50 (sum|nh| + 1e-50) over
+ * (sum |.| dr + 1e-100).  Nr >= 20. */
+int psa_mod_rho5(int32_t N, int32_t Nr, const double *C, const double *D, const double *E,
+                 const double *W, double L, double dr, const double *ne, const double *nh,
+                 double *out, void *stream);
+
+/* QWOptics.QW2Prop's Plasmonics step (qwoptics.py:785-790, 801-805) on
+ * complex re, rh [n] in place: absolute values, then each scaled to
+ * total / (sum dx + 1e-100); mode 0 first sets *total (device double) =
+ * (sum re dx + sum rh dx + 1e-100) / 2, mode 1 reuses it. */
+int psa_mod_plasmonics(int32_t n, double dx, int32_t mode, double *re, double *rh, double *total,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -115,6 +115,13 @@ SIGNATURES = {
                                    c_ptr]),
     "psa_mod_mbp": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_ptr, c_ptr,
                                    c_ptr]),
+    "psa_mod_dft": (ctypes.c_int, [c_i32, c_i32, c_i32, c_dbl, c_ptr, c_ptr, c_ptr]),
+    "psa_mod_resample": (ctypes.c_int, [c_i32, c_i32, c_i32, c_ptr, c_ptr, c_i32, c_ptr, c_ptr, c_ptr]),
+    "psa_mod_pol3": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_dbl,
+                                    c_dbl, c_ptr, c_ptr]),
+    "psa_mod_rho5": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_dbl, c_ptr, c_ptr,
+                                    c_ptr, c_ptr]),
+    "psa_mod_plasmonics": (ctypes.c_int, [c_i32, c_dbl, c_i32, c_ptr, c_ptr, c_ptr, c_ptr]),
 }
 
 _lib = None

@@ -2665,4 +2665,58 @@ int psa_mod_mbp(int32_t N, int32_t hole, const double *n, const double *VC, cons
   return PSA_OK;
 }
 
+int psa_mod_dft(int32_t n, int32_t nvec, int32_t sign, double scale, const double *in, double *out, void *stream) {
+  if (n < 1 || nvec < 1 || nvec > 65535 || (sign != 1 && sign != -1) || !in || !out || in == out)
+    return fail(PSA_E_INVALID, "psa_mod_dft: bad arguments");
+  hipLaunchKernelGGL(mod::k_dft, dim3(n, nvec), dim3(mod::RED_T), 0, (hipStream_t)stream, n, (int)sign, scale,
+                     reinterpret_cast<const cd *>(in), reinterpret_cast<cd *>(out));
+  PSA_LAUNCH_CHECK("k_dft");
+  return PSA_OK;
+}
+
+int psa_mod_resample(int32_t nout, int32_t nin, int32_t nvec, const double *M, const double *w, int32_t real_in,
+                     const double *in, double *out, void *stream) {
+  if (nout < 1 || nin < 1 || nvec < 1 || nvec > 65535 || !M || !in || !out || in == out)
+    return fail(PSA_E_INVALID, "psa_mod_resample: bad arguments");
+  hipLaunchKernelGGL(mod::k_resample, dim3(nout, nvec), dim3(mod::RED_T), 0, (hipStream_t)stream, nout, nin, M, w,
+                     (int)(real_in != 0), reinterpret_cast<const cd *>(in), reinterpret_cast<cd *>(out));
+  PSA_LAUNCH_CHECK("k_resample");
+  return PSA_OK;
+}
+
+int psa_mod_pol3(int32_t N, int32_t Nr, const double *p, const double *Xvc, const double *Yvc, const double *Zvc,
+                 const double *E, const double *W, double syw, double sz, double scale, double *out, void *stream) {
+  if (N < 1 || N > 1024 || Nr < 1 || !p || !Xvc || !Yvc || !Zvc || !E || !W || !out)
+    return fail(PSA_E_INVALID, "psa_mod_pol3: bad arguments");
+  hipLaunchKernelGGL(mod::k_pol3, dim3(Nr), dim3(mod::RED_T), 0, (hipStream_t)stream, N, Nr,
+                     reinterpret_cast<const cd *>(p), reinterpret_cast<const cd *>(Xvc),
+                     reinterpret_cast<const cd *>(Yvc), reinterpret_cast<const cd *>(Zvc),
+                     reinterpret_cast<const cd *>(E), W, syw, sz, scale, out);
+  PSA_LAUNCH_CHECK("k_pol3");
+  return PSA_OK;
+}
+
+int psa_mod_rho5(int32_t N, int32_t Nr, const double *C, const double *D, const double *E, const double *W, double L,
+                 double dr, const double *ne, const double *nh, double *out, void *stream) {
+  if (N < 1 || N > 1024 || Nr < 20 || !C || !D || !E || !W || !ne || !nh || !out || !(L > 0.0))
+    return fail(PSA_E_INVALID, "psa_mod_rho5: bad arguments (Nr >= 20)");
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(mod::k_rho5, dim3(Nr), dim3(mod::RED_T), 0, s, N, Nr, reinterpret_cast<const cd *>(C),
+                     reinterpret_cast<const cd *>(D), reinterpret_cast<const cd *>(E), W, L, reinterpret_cast<cd *>(out));
+  PSA_LAUNCH_CHECK("k_rho5");
+  hipLaunchKernelGGL(mod::k_rho5_finish, dim3(1), dim3(mod::RED_T), 0, s, N, Nr, dr, reinterpret_cast<const cd *>(ne),
+                     reinterpret_cast<const cd *>(nh), reinterpret_cast<cd *>(out));
+  PSA_LAUNCH_CHECK("k_rho5_finish");
+  return PSA_OK;
+}
+
+int psa_mod_plasmonics(int32_t n, double dx, int32_t mode, double *re, double *rh, double *total, void *stream) {
+  if (n < 1 || (mode != 0 && mode != 1) || !re || !rh || !total)
+    return fail(PSA_E_INVALID, "psa_mod_plasmonics: bad arguments");
+  hipLaunchKernelGGL(mod::k_plasmonics, dim3(1), dim3(mod::RED_T), 0, (hipStream_t)stream, n, dx, (int)mode,
+                     reinterpret_cast<cd *>(re), reinterpret_cast<cd *>(rh), total);
+  PSA_LAUNCH_CHECK("k_plasmonics");
+  return PSA_OK;
+}
+
 }  // extern "C"

@@ -15,7 +15,8 @@ namespace mod {
 
 constexpr int RED_T = 256;
 
-// Fixed-order block sum of two values (tree over the 256 threads).
+// Fixed-order block sum of two values (tree over the 256 threads); every
+// thread gets the sums.  The closing barrier lets the next call reuse sh.
 __device__ __forceinline__ void block_sum2(double &a, double &b, double (*sh)[RED_T]) {
   const int t = threadIdx.x;
   sh[0][t] = a;
@@ -30,6 +31,7 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double (*sh)[RE
   }
   a = sh[0][0];
   b = sh[1][0];
+  __syncthreads();
 }
 
 // GaussDelta (coulomb.py:245-268)
@@ -192,6 +194,178 @@ __global__ __launch_bounds__(RED_T) void k_mbp(int N, int hole, const cd *n, con
   if (threadIdx.x == 0) {
     Win[k] += a;
     Wout[k] += b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// qwoptics.py module API (QWOptics methods) on the caller's arrays
+// ---------------------------------------------------------------------------
+
+// Fixed-order block sum of a complex value.
+__device__ __forceinline__ cd block_sum_c(cd a, double (*sh)[RED_T]) {
+  double re = a.re, im = a.im;
+  block_sum2(re, im, sh);
+  return cd{re, im};
+}
+
+// Direct DFT of nvec complex vectors of length n: block (k, v), threads
+// over j.  out[v][k] = scale sum_j in[v][j] exp(sign 2 pi i (j k mod n) / n).
+__global__ __launch_bounds__(RED_T) void k_dft(int n, int sign, double scale, const cd *in, cd *out) {
+  __shared__ double sh[2][RED_T];
+  const int k = blockIdx.x, v = blockIdx.y;
+  const cd *x = in + (long)v * n;
+  cd a{0.0, 0.0};
+  for (int j = threadIdx.x; j < n; j += RED_T) {
+    const long m = ((long)j * k) % n;
+    double sn, cs;
+    sincos(2.0 * PI * (double)m / (double)n, &sn, &cs);
+    sn *= (double)sign;
+    a.re += x[j].re * cs - x[j].im * sn;
+    a.im += x[j].re * sn + x[j].im * cs;
+  }
+  a = block_sum_c(a, sh);
+  if (threadIdx.x == 0) out[(long)v * n + k] = cd{scale * a.re, scale * a.im};
+}
+
+// rescale_1D as a linear map: out[v][r] = w[r] sum_j M[r][j] f(in[v][j]),
+// f = Re (real_in) or the identity; block (r, v), threads over j.
+__global__ __launch_bounds__(RED_T) void k_resample(int nout, int nin, const double *M, const double *w, int real_in,
+                                                    const cd *in, cd *out) {
+  __shared__ double sh[2][RED_T];
+  const int r = blockIdx.x, v = blockIdx.y;
+  const cd *x = in + (long)v * nin;
+  const double *row = M + (long)r * nin;
+  cd a{0.0, 0.0};
+  for (int j = threadIdx.x; j < nin; j += RED_T) {
+    a.re += row[j] * x[j].re;
+    if (!real_in) a.im += row[j] * x[j].im;
+  }
+  a = block_sum_c(a, sh);
+  if (threadIdx.x == 0) {
+    const double s = w ? w[r] : 1.0;
+    out[(long)v * nout + r] = cd{a.re * s, a.im * s};
+  }
+}
+
+// QWPolarization3 sums (qwoptics.py:49-76): block = one r, threads over the
+// (ke, kh) pairs; prod = p[kh][ke] V[kh][ke] E[ke][r] conj(E[kh][r]) in
+// the reference's order, its real part summed for V = Xvc0, Yvc0, Zvc0.
+__global__ __launch_bounds__(RED_T) void k_pol3(int N, int Nr, const cd *p, const cd *X, const cd *Y, const cd *Z,
+                                                const cd *E, const double *W, double syw, double sz, double scale,
+                                                double *out) {
+  __shared__ double sh[2][RED_T];
+  const int r = blockIdx.x;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  const long NN = (long)N * N;
+  for (long i = threadIdx.x; i < NN; i += RED_T) {
+    const int ke = (int)(i / N), kh = (int)(i % N);
+    const long hk = (long)kh * N + ke;
+    const cd e1 = E[(long)ke * Nr + r], e2 = E[(long)kh * Nr + r];
+    const cd e2c{e2.re, -e2.im};
+    const cd pv = p[hk];
+    ax += cmul(cmul(cmul(pv, X[hk]), e1), e2c).re;
+    ay += cmul(cmul(cmul(pv, Y[hk]), e1), e2c).re;
+    az += cmul(cmul(cmul(pv, Z[hk]), e1), e2c).re;
+  }
+  double dummy = 0.0;
+  block_sum2(ax, ay, sh);
+  block_sum2(az, dummy, sh);
+  if (threadIdx.x == 0) {
+    const double wr = W[r];
+    // P = sum * W (* yw, * (-1)^w) * 2 ehint / area / L, as complex values
+    out[2 * r] = ax * wr * scale;
+    out[2 * r + 1] = 0.0;
+    out[2 * ((long)Nr + r)] = ay * wr * syw * scale;
+    out[2 * ((long)Nr + r) + 1] = 0.0;
+    out[2 * (2L * Nr + r)] = az * wr * sz * scale;
+    out[2 * (2L * Nr + r) + 1] = 0.0;
+  }
+}
+
+// QWRho5 sums (qwoptics.py:360-374): block = one r, threads over (k1, k2):
+// re[r] = sum C[k1][k2] conj(E[k1][r]) E[k2][r] W[r] / (2 L), rh with conj(D).
+__global__ __launch_bounds__(RED_T) void k_rho5(int N, int Nr, const cd *C, const cd *D, const cd *E, const double *W,
+                                                double L, cd *out) {
+  __shared__ double sh[2][RED_T];
+  const int r = blockIdx.x;
+  cd ae{0.0, 0.0}, ah{0.0, 0.0};
+  const double f = W[r] / (2.0 * L);
+  const long NN = (long)N * N;
+  for (long i = threadIdx.x; i < NN; i += RED_T) {
+    const int k1 = (int)(i / N), k2 = (int)(i % N);
+    const cd e1 = E[(long)k1 * Nr + r], e2 = E[(long)k2 * Nr + r];
+    const cd e1c{e1.re, -e1.im};
+    const cd c = C[i], d = D[i];
+    const cd t = cmul(e1c, e2);
+    ae = cadd(ae, cmul(c, t));
+    ah = cadd(ah, cmul(cd{d.re, -d.im}, t));
+  }
+  ae = block_sum_c(ae, sh);
+  ah = block_sum_c(ah, sh);
+  if (threadIdx.x == 0) {
+    out[r] = cd{ae.re * f, ae.im * f};
+    out[Nr + r] = cd{ah.re * f, ah.im * f};
+  }
+}
+
+// QWRho5 tail (qwoptics.py:1062-1068), one block: subtract the mean of the
+// real parts of the first and last 10 points, then scale each density to
+// sum |n| + 1e-50 over (sum |x| dr + 1e-100).
+__global__ __launch_bounds__(RED_T) void k_rho5_finish(int N, int Nr, double dr, const cd *ne, const cd *nh, cd *x) {
+  __shared__ double sh[2][RED_T];
+  const int t = threadIdx.x;
+  double tn = 0.0, th = 0.0;
+  for (int k = t; k < N; k += RED_T) {
+    tn += hypot(ne[k].re, ne[k].im);
+    th += hypot(nh[k].re, nh[k].im);
+  }
+  block_sum2(tn, th, sh);
+  const double NeT = tn + 1e-50, NhT = th + 1e-50;
+  double me = 0.0, mh = 0.0;
+  for (int k = t; k < Nr; k += RED_T)
+    if (k < 10 || k >= Nr - 10) {
+      me += x[k].re;
+      mh += x[Nr + k].re;
+    }
+  block_sum2(me, mh, sh);
+  me /= 20.0;
+  mh /= 20.0;
+  double se = 0.0, shh = 0.0;
+  for (int k = t; k < Nr; k += RED_T) {
+    x[k].re -= me;
+    x[Nr + k].re -= mh;
+    se += hypot(x[k].re, x[k].im);
+    shh += hypot(x[Nr + k].re, x[Nr + k].im);
+  }
+  block_sum2(se, shh, sh);
+  const double fe = NeT / (se * dr + 1e-100), fh = NhT / (shh * dr + 1e-100);
+  for (int k = t; k < Nr; k += RED_T) {
+    x[k] = cd{x[k].re * fe, x[k].im * fe};
+    x[Nr + k] = cd{x[Nr + k].re * fh, x[Nr + k].im * fh};
+  }
+}
+
+// QW2Prop's Plasmonics renormalisation (qwoptics.py:785-790, 801-805), one
+// block: re = |re|, rh = |rh|; mode 0 sets total = (sum re dx + sum rh dx +
+// 1e-100) / 2; then re *= total / (sum re dx + 1e-100), rh likewise.
+__global__ __launch_bounds__(RED_T) void k_plasmonics(int n, double dx, int mode, cd *re, cd *rh, double *total) {
+  __shared__ double sh[2][RED_T];
+  const int t = threadIdx.x;
+  double se = 0.0, sr = 0.0;
+  for (int k = t; k < n; k += RED_T) {
+    const double a = hypot(re[k].re, re[k].im), b = hypot(rh[k].re, rh[k].im);
+    re[k] = cd{a, 0.0};
+    rh[k] = cd{b, 0.0};
+    se += a;
+    sr += b;
+  }
+  block_sum2(se, sr, sh);
+  const double tot = mode == 0 ? (se * dx + sr * dx + 1e-100) / 2.0 : *total;
+  if (mode == 0 && t == 0) *total = tot;
+  const double fe = tot / (se * dx + 1e-100), fh = tot / (sr * dx + 1e-100);
+  for (int k = t; k < n; k += RED_T) {
+    re[k].re *= fe;
+    rh[k].re *= fh;
   }
 }
 

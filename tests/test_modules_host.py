@@ -87,3 +87,14 @@ def test_oracle_lorentz_tables_support_and_zero_mismatch():
         at_zero = np.isclose(G, peak_g, rtol=1e-12, atol=0)
         assert at_zero.sum() > 0, kind
         assert np.allclose(L[at_zero] / G[at_zero], 1 / np.sqrt(np.pi), rtol=1e-12)
+
+
+def test_qwoptics_host_helpers():
+    from pulsesuite_amd import qwoptics
+    assert [qwoptics.yw(w) for w in range(1, 9)] == [O.yw(w) for w in range(1, 9)] == [1, 1, -1, -1, 1, 1, -1, -1]
+    import torch
+    if not torch.cuda.is_available():
+        from pulsesuite_amd import capi
+        z, prm, g = _setup("n16_mb")
+        with pytest.raises(capi.PsaError):
+            qwoptics.QWOptics(g.r, prm.L, g.dcv, g.kr, g.Qr, g.Ee, g.Eh, g.ehint, g.area, prm.gap)
