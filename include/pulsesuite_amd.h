@@ -479,6 +479,35 @@ int psa_mod_rho5(int32_t N, int32_t Nr, const double *C, const double *D, const 
 int psa_mod_plasmonics(int32_t n, double dx, int32_t mode, double *re, double *rh, double *total,
                        void *stream);
 
+/* ---- Module API: dephasing.py, emission.py ---- */
+
+/* CalcGammaE (hole = 0; dephasing.py:28-66, 541-580) / CalcGammaH (hole = 1;
+ * dephasing.py:68-103, 582-625) into out [N]: occupations Re ne0, Re nh0
+ * (complex [N]); V^2(q) = VC[iq[q]][0][c]^2 (Vxx2, dephasing.py:362-378);
+ * tabs [4][N][N] int = k_p_q, k_m_q, k1p_m_q, k1p (electrons) or k_p_q,
+ * k_m_q, k1_m_q, k1 (holes) of InitializeDephasing (dephasing.py:472-539);
+ * xe, xh [N]. */
+int psa_mod_gamma(int32_t N, int32_t hole, const double *ne0, const double *nh0, const double *VC,
+                  const int32_t *iq, const int32_t *tabs, const double *xe, const double *xh,
+                  double hbar, double *out, void *stream);
+
+/* CalcOffDiagDeph_E2 / _H2 (dephasing.py:226-317, 381-414) into D [2][2N+1][N]
+ * (De, Dh; row q + N) on |ne0|, |nh0|, widths hbar gee, hbar ghh, hbar geh;
+ * with x != NULL also OffDiagDephasing2's contraction (dephasing.py:320-356,
+ * 714-743) of p (complex [N][N]) into x (complex [N][N]). */
+int psa_mod_offdiag2(int32_t N, const double *ne0, const double *nh0, const double *p,
+                     const double *VC, const double *Ee, const double *Eh, double gee, double ghh,
+                     double geh, double hbar, double *D, double *x, void *stream);
+
+/* SpontEmission (emission.py:94-130) with Ec (133-171) and SpontIntegral
+ * (174-198): Rsp[k] = dhw sum_n (H_n + Ek) (H_n + Ek)^2 / rden square_n,
+ * Ek = gap + Ee[k] + Eh[k] + Ec[k] on Re ne0, Re nh0; rden = c0^3 pi^2
+ * hbar^3 (rho0, emission.py:201-217); ec != NULL receives Ec. */
+int psa_mod_spont(int32_t N, int32_t Nhw, const double *ne0, const double *nh0, const double *VC,
+                  const double *Ee, const double *Eh, double gap, const double *homega,
+                  const double *square, double dhw, double rden, double *Rsp, double *ec,
+                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,6 +122,12 @@ SIGNATURES = {
     "psa_mod_rho5": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_dbl, c_ptr, c_ptr,
                                     c_ptr, c_ptr]),
     "psa_mod_plasmonics": (ctypes.c_int, [c_i32, c_dbl, c_i32, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "psa_mod_gamma": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl,
+                                     c_ptr, c_ptr]),
+    "psa_mod_offdiag2": (ctypes.c_int, [c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_dbl, c_dbl,
+                                        c_dbl, c_ptr, c_ptr, c_ptr]),
+    "psa_mod_spont": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_ptr, c_ptr,
+                                     c_dbl, c_dbl, c_ptr, c_ptr, c_ptr]),
 }
 
 _lib = None

@@ -2719,4 +2719,47 @@ int psa_mod_plasmonics(int32_t n, double dx, int32_t mode, double *re, double *r
   return PSA_OK;
 }
 
+int psa_mod_gamma(int32_t N, int32_t hole, const double *ne0, const double *nh0, const double *VC, const int32_t *iq,
+                  const int32_t *tabs, const double *xe, const double *xh, double hbar, double *out, void *stream) {
+  if (N < 1 || N > 1024 || (hole != 0 && hole != 1) || !ne0 || !nh0 || !VC || !iq || !tabs || !xe || !xh || !out)
+    return fail(PSA_E_INVALID, "psa_mod_gamma: bad arguments");
+  const long NN = (long)N * N;
+  hipLaunchKernelGGL(mod::k_gamma, dim3(N), dim3(mod::RED_T), 0, (hipStream_t)stream, N, (int)hole,
+                     reinterpret_cast<const cd *>(ne0), reinterpret_cast<const cd *>(nh0), VC, iq, tabs, tabs + NN,
+                     tabs + 2 * NN, tabs + 3 * NN, xe, xh, PI / hbar, out);
+  PSA_LAUNCH_CHECK("k_gamma");
+  return PSA_OK;
+}
+
+int psa_mod_offdiag2(int32_t N, const double *ne0, const double *nh0, const double *p, const double *VC,
+                     const double *Ee, const double *Eh, double gee, double ghh, double geh, double hbar, double *D,
+                     double *x, void *stream) {
+  if (N < 1 || N > 1024 || !ne0 || !nh0 || !VC || !Ee || !Eh || !D || (x && !p))
+    return fail(PSA_E_INVALID, "psa_mod_offdiag2: bad arguments");
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(mod::k_od2_tables, dim3(2 * N + 1, N, 2), dim3(mod::RED_T), 0, s, N,
+                     reinterpret_cast<const cd *>(ne0), reinterpret_cast<const cd *>(nh0), VC, Ee, Eh, hbar * gee,
+                     hbar * ghh, hbar * geh, hbar, D);
+  PSA_LAUNCH_CHECK("k_od2_tables");
+  if (x) {
+    const long NN = (long)N * N;
+    hipLaunchKernelGGL(mod::k_od2_apply, dim3((unsigned)((NN + mod::RED_T - 1) / mod::RED_T)), dim3(mod::RED_T), 0, s,
+                       N, D, reinterpret_cast<const cd *>(p), hbar, reinterpret_cast<cd *>(x));
+    PSA_LAUNCH_CHECK("k_od2_apply");
+  }
+  return PSA_OK;
+}
+
+int psa_mod_spont(int32_t N, int32_t Nhw, const double *ne0, const double *nh0, const double *VC, const double *Ee,
+                  const double *Eh, double gap, const double *homega, const double *square, double dhw, double rden,
+                  double *Rsp, double *ec, void *stream) {
+  if (N < 1 || Nhw < 2 || !ne0 || !nh0 || !VC || !Ee || !Eh || !homega || !square || !Rsp)
+    return fail(PSA_E_INVALID, "psa_mod_spont: bad arguments");
+  hipLaunchKernelGGL(mod::k_spont, dim3(N), dim3(mod::RED_T), 0, (hipStream_t)stream, N, Nhw,
+                     reinterpret_cast<const cd *>(ne0), reinterpret_cast<const cd *>(nh0), VC, Ee, Eh, gap, homega,
+                     square, dhw, rden, Rsp, ec);
+  PSA_LAUNCH_CHECK("k_spont");
+  return PSA_OK;
+}
+
 }  // extern "C"

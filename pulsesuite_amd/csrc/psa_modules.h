@@ -369,5 +369,147 @@ __global__ __launch_bounds__(RED_T) void k_plasmonics(int n, double dx, int mode
   }
 }
 
+// ---------------------------------------------------------------------------
+// dephasing.py / emission.py module API
+// ---------------------------------------------------------------------------
+
+// CalcGammaE / CalcGammaH (dephasing.py:28-103, 541-625): block = one k,
+// threads over q.  t0..t3 = k_p_q, k_m_q, k1p_m_q, k1p (electrons) or
+// k_p_q, k_m_q, k1_m_q, k1 (holes), [N][N] row k; an index outside [0, N)
+// drops the term as the reference's bounds test does.  V^2(q) =
+// VC[iq[q]][0][c]^2 (Vxx2, dephasing.py:362-378).
+__global__ __launch_bounds__(RED_T) void k_gamma(int N, int hole, const cd *ne0, const cd *nh0, const double *VC,
+                                                 const int32_t *iq, const int32_t *t0, const int32_t *t1,
+                                                 const int32_t *t2, const int32_t *t3, const double *xe,
+                                                 const double *xh, double pih, double *out) {
+  __shared__ double sh[2][RED_T];
+  const int k = blockIdx.x;
+  auto ok = [&](int i) { return i >= 0 && i < N; };
+  auto n_e = [&](int i) { return ne0[i].re; };
+  auto n_h = [&](int i) { return nh0[i].re; };
+  double acc = 0.0, dummy = 0.0;
+  for (int q = threadIdx.x; q < N; q += RED_T) {
+    const long r = (long)k * N + q;
+    const long vq = (long)iq[q] * N * 3;
+    const double veh = VC[vq] * VC[vq];
+    const int a = t0[r], d = t1[r], b = t2[r], c = t3[r];
+    if (!hole) {
+      const double vee = VC[vq + 1] * VC[vq + 1];
+      if (ok(a)) acc += pih * vee * n_e(a) * (1.0 - n_e(a)) * fabs(xe[q]);
+      if (ok(b) && ok(c) && ok(d))
+        acc += pih * veh * (n_h(b) * (1.0 - n_h(c)) * n_e(d) + (1.0 - n_h(b)) * n_h(c) * (1.0 - n_e(d))) * fabs(xh[q]);
+    } else {
+      const double vhh = VC[vq + 2] * VC[vq + 2];
+      if (ok(a)) acc += pih * vhh * n_h(a) * (1.0 - n_h(a)) * fabs(xh[q]);
+      if (ok(b) && ok(c) && ok(d))
+        acc += pih * veh * (n_e(b) * (1.0 - n_e(c)) * n_h(d) + (1.0 - n_e(b)) * n_e(c) * (1.0 - n_h(d))) * fabs(xe[q]);
+    }
+  }
+  block_sum2(acc, dummy, sh);
+  if (threadIdx.x == 0) out[k] = acc;
+}
+
+// CalcOffDiagDeph_E2 / _H2 (dephasing.py:226-317, 381-414): block (q + N,
+// k, kind), threads over p; D[kind][q + N][k] for q in [-N, N].  Kind 0
+// (De): same-carrier electrons (V_ee, width hgS = hbar gee) and electron-
+// hole (V_eh, hbar geh); kind 1 (Dh) the mirror with hbar ghh.  Lrtz(a, b)
+// = (b / pi) / (a^2 + b^2) (dephasing.py:220-224); V^2(q) = VC[|q|][0][c]^2
+// for 0 < |q| < N, else 0.
+__global__ __launch_bounds__(RED_T) void k_od2_tables(int N, const cd *ne0, const cd *nh0, const double *VC,
+                                                      const double *Ee, const double *Eh, double hgee, double hghh,
+                                                      double hgeh, double hbar, double *D) {
+  __shared__ double sh[2][RED_T];
+  const int q = (int)blockIdx.x - N, k = blockIdx.y, kind = blockIdx.z;
+  const int aq = q < 0 ? -q : q;
+  auto vsq = [&](int c) {
+    if (aq < 1 || aq > N - 1) return 0.0;
+    const double v = VC[(long)aq * N * 3 + c];
+    return v * v;
+  };
+  const double vS = vsq(kind == 0 ? 1 : 2), vO = vsq(0);
+  const double *ES = kind == 0 ? Ee : Eh, *EO = kind == 0 ? Eh : Ee;
+  const cd *nS = kind == 0 ? ne0 : nh0, *nO = kind == 0 ? nh0 : ne0;
+  const double hgS = kind == 0 ? hgee : hghh;
+  auto lrtz = [](double a, double b) { return (b / PI) / (a * a + b * b); };
+  auto mag = [](cd z) { return hypot(z.re, z.im); };
+  double acc = 0.0, dummy = 0.0;
+  const bool kq = k + q >= 0 && k + q < N;
+  if (kq) {
+    const double nk = mag(nS[k]);
+    for (int p = threadIdx.x; p < N; p += RED_T) {
+      const int pm = p - q, pp = p + q;
+      if (pm >= 1 && pm < N) {  // q <= p - 1 and p - q < N
+        const double Ed = ES[k + q] + ES[pm] - ES[p] - ES[k];
+        const double a = mag(nS[pm]), b = mag(nS[p]);
+        acc += vS * lrtz(Ed, hgS) * (a * (1.0 - b) * (1.0 - nk) + (1.0 - a) * b * nk);
+      }
+      if (pp >= 0 && pp < N) {
+        const double Ed = ES[k + q] + EO[pp] - EO[p] - ES[k];
+        const double a = mag(nO[pp]), b = mag(nO[p]);
+        acc += vO * lrtz(Ed, hgeh) * (a * (1.0 - b) * (1.0 - nk) + (1.0 - a) * b * nk);
+      }
+    }
+  }
+  block_sum2(acc, dummy, sh);
+  if (threadIdx.x == 0) D[((long)kind * (2 * N + 1) + (q + N)) * N + k] = acc * PI / hbar;
+}
+
+// OffDiagDephasing2's contraction (dephasing.py:320-356): x[kh][ke] =
+// i hbar (sum_{q != 0} Dh[q + N][kh] p[kh + q][ke] + sum_{q != 0} De[q + N][ke]
+// p[kh][ke + q]); one thread per (kh, ke), lanes along ke.
+__global__ __launch_bounds__(RED_T) void k_od2_apply(int N, const double *D, const cd *p, double hbar, cd *x) {
+  const long i = (long)blockIdx.x * RED_T + threadIdx.x;
+  if (i >= (long)N * N) return;
+  const int kh = (int)(i / N), ke = (int)(i % N);
+  const double *De = D, *Dh = D + (long)(2 * N + 1) * N;
+  cd s{0.0, 0.0};
+  for (int q = -kh; q < N - kh; ++q) {
+    if (q == 0) continue;
+    const double d = Dh[(long)(q + N) * N + kh];
+    const cd v = p[(long)(kh + q) * N + ke];
+    s = cd{s.re + d * v.re, s.im + d * v.im};
+  }
+  for (int q = -ke; q < N - ke; ++q) {
+    if (q == 0) continue;
+    const double d = De[(long)(q + N) * N + ke];
+    const cd v = p[(long)kh * N + ke + q];
+    s = cd{s.re + d * v.re, s.im + d * v.im};
+  }
+  x[i] = cd{-s.im * hbar, s.re * hbar};
+}
+
+// SpontEmission (emission.py:94-130) with Ec (133-171) and SpontIntegral
+// (174-198): block = one k.  Ek = gap + Ee[k] + Eh[k] + Ec[k] on the real
+// occupations; Rsp[k] = dhw sum_n (H_n + Ek) rho0(H_n + Ek) sq_n with
+// rho0(h) = h^2 / rden (rden = c0^3 pi^2 hbar^3).  ec (optional) gets Ec.
+__global__ __launch_bounds__(RED_T) void k_spont(int N, int Nhw, const cd *ne0, const cd *nh0, const double *VC,
+                                                 const double *Ee, const double *Eh, double gap, const double *hom,
+                                                 const double *sq, double dhw, double rden, double *Rsp,
+                                                 double *ec) {
+  __shared__ double sh[2][RED_T];
+  const int k = blockIdx.x;
+  auto V = [&](int a, int b, int c) { return VC[((long)a * N + b) * 3 + c]; };
+  const double vkk_eh = V(k, k, 0), vkk_ee = V(k, k, 1), vkk_hh = V(k, k, 2);
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = threadIdx.x; i < N; i += RED_T) {
+    const double a = ne0[i].re, b = nh0[i].re, id = (i == k) ? 0.0 : 1.0;
+    s1 += a * (vkk_ee - V(i, k, 1)) + b * (vkk_hh - V(i, k, 2));
+    s2 += a * vkk_eh * id - b * vkk_eh * id;
+  }
+  block_sum2(s1, s2, sh);
+  const double Ec = s1 + s2 - vkk_eh;
+  const double Ek = gap + Ee[k] + Eh[k] + Ec;
+  double acc = 0.0, dummy = 0.0;
+  for (int n = threadIdx.x; n < Nhw; n += RED_T) {
+    const double h = hom[n] + Ek;
+    acc += h * (h * h / rden) * sq[n];
+  }
+  block_sum2(acc, dummy, sh);
+  if (threadIdx.x == 0) {
+    Rsp[k] = acc * dhw;
+    if (ec) ec[k] = Ec;
+  }
+}
+
 }  // namespace mod
 }  // namespace psa

@@ -98,3 +98,21 @@ def test_qwoptics_host_helpers():
         z, prm, g = _setup("n16_mb")
         with pytest.raises(capi.PsaError):
             qwoptics.QWOptics(g.r, prm.L, g.dcv, g.kr, g.Qr, g.Ee, g.Eh, g.ehint, g.area, prm.gap)
+
+
+@pytest.mark.parametrize("tag", ["n16_mb", "n32_mb"])
+def test_dephasing_and_emission_tables_match_reference(tag, tmp_path, monkeypatch):
+    from pulsesuite_amd import dephasing, emission
+    z, prm, g = _setup(tag)
+    monkeypatch.chdir(tmp_path)  # InitializeDephasing opens dataQW/Wire/info/MaxOffDiag.*
+    dephasing.InitializeDephasing(g.kr, prm.me, prm.mh)
+    for k in ("k_p_q", "k_m_q", "k1_m_q", "k1p_m_q", "k1", "k1p"):
+        assert np.array_equal(getattr(dephasing, "_" + k), z[k]), k
+    assert rel(dephasing._xe, z["xe"]) < 1e-15 and rel(dephasing._xh, z["xh"]) < 1e-15
+    assert (tmp_path / "dataQW/Wire/info/MaxOffDiag.e.dat").exists()
+    dephasing._fe_file.close()
+    dephasing._fh_file.close()
+    dephasing._fe_file = dephasing._fh_file = None
+    emission.InitializeEmission(g.kr, g.Ee, g.Eh, abs(g.dcv), prm.epsr, g.gam_eh, g.ehint)
+    assert rel(emission._HOmega, z["HOmega"]) == 0 and rel(emission._square, z["square"]) < 1e-15
+    assert emission._RScale == z["RScale"]
