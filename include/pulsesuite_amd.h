@@ -398,6 +398,40 @@ int psa_pstd_gather(const psa_pstd *p, const psa_group *g, int32_t w0, int32_t n
 int psa_pstd_scatter(const psa_pstd *p, const psa_group *g, int32_t w0, int32_t nw,
                      int32_t slot0, void *stream);
 
+/* ---- The RHS of several wire groups in one persistent launch ----
+ *
+ * psa_rhs (dpdt/dCdt/dDdt, SBEs.py:1941-2625) for up to PSA_RHS_MULTI_MAX
+ * groups of any N and one precision mode, as one launch whose workgroups
+ * take 64 x 64 output tiles from a work queue (an atomic counter pair in
+ * device memory; the launch leaves it at zero).  Every tile is computed as
+ * psa_rhs computes it, so the results are identical.  The wires are dealt
+ * to one list per XCD (a wire's tiles share that XCD's L2); a workgroup
+ * drains its own list, then takes from the others.  The reference has no
+ * batched form: it evaluates the RHS wire by wire (SBEs.py:5612-5675). */
+#define PSA_RHS_MULTI_MAX 16
+#define PSA_RHS_MULTI_QUEUE 16 /* uint32 words of the queue buffer */
+/* bytes of the packed description of n groups of nw[i] wires each (-1 if
+ * n or an nw[i] is out of range) */
+int64_t psa_rhs_multi_bytes(const int32_t *nw, int32_t n);
+/* Validate and pack groups[i] (wires [w0[i], w0[i]+nw[i]), psa_rhs mode and
+ * dt) into host memory `packed` (psa_rhs_multi_bytes(n) bytes).  The caller
+ * keeps a device copy of the same bytes for psa_rhs_multi; repack when a
+ * group's buffers, wire range or mode change. */
+int psa_rhs_multi_pack(const psa_group *const *groups, const int32_t *w0, const int32_t *nw,
+                       int32_t n, int32_t mode, double dt, void *packed);
+/* One launch (plus the per-group bf16 operand split in PSA_PREC_BF16X3):
+ * packed_host and packed_dev hold the same bytes; queue is a device
+ * uint32[PSA_RHS_MULTI_QUEUE] that is zero before the first call (each
+ * launch leaves it zero).  Calls sharing a queue must not overlap. */
+int psa_rhs_multi(const void *packed_host, const void *packed_dev, uint32_t *queue, void *stream);
+/* psa_step (one leapfrog step, SBEs.py:5612-5820) of every packed group,
+ * with the RHS of all of them as the one psa_rhs_multi launch; the
+ * description must be packed with mode 1 and this dt (and repacked after
+ * the caller exchanges the levels: one description per level parity).
+ * Results equal psa_step's per group. */
+int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *queue, double dt,
+                   void *stream);
+
 /* ---- Module API on the caller's dense arrays (coulomb.py, phonons.py) ----
  *
  * The per-call functions of the reference's coulomb and phonons modules for

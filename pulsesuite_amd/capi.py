@@ -69,6 +69,8 @@ _QP = ctypes.POINTER(PsaQw3d)
 PSA_QW3D_CURR, PSA_QW3D_RHO, PSA_QW3D_POL = 1, 2, 4
 PSA_PSTD_ABC = 1
 PSA_PSTD_MAXW = 256
+PSA_RHS_MULTI_MAX = 16
+PSA_RHS_MULTI_QUEUE = 16
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -107,6 +109,10 @@ SIGNATURES = {
     "psa_pstd_advance": (ctypes.c_int, [_PP, c_ptr]),
     "psa_pstd_gather": (ctypes.c_int, [_PP, _GP, c_i32, c_i32, c_i32, c_ptr]),
     "psa_pstd_scatter": (ctypes.c_int, [_PP, _GP, c_i32, c_i32, c_i32, c_ptr]),
+    "psa_rhs_multi_bytes": (ctypes.c_int64, [c_ptr, c_i32]),
+    "psa_rhs_multi_pack": (ctypes.c_int, [c_ptr, c_ptr, c_ptr, c_i32, c_i32, c_dbl, c_ptr]),
+    "psa_rhs_multi": (ctypes.c_int, [c_ptr, c_ptr, c_ptr, c_ptr]),
+    "psa_step_multi": (ctypes.c_int, [c_ptr, c_ptr, c_ptr, c_dbl, c_ptr]),
     "psa_mod_mb_arrays": (ctypes.c_int, [c_i32, c_ptr, c_ptr, c_dbl, c_dbl, c_dbl, c_i32, c_ptr, c_ptr,
                                          c_ptr, c_ptr, c_ptr]),
     "psa_mod_screen": (ctypes.c_int, [c_i32, c_i32, c_dbl, c_dbl, c_ptr, c_ptr, c_dbl, c_ptr, c_ptr,

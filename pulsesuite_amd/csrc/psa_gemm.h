@@ -114,29 +114,13 @@ __device__ __forceinline__ int next_term(const P &p, int t0, int nt, int b, int 
   return nt;
 }
 
+// One 64 x 64 output tile (batch b, rows m0.., columns n0..) of problem p;
+// lds_raw holds the double-buffered operand tiles (gemm_lds_bytes<T>()).
 template <typename T, class P>
-__global__ __launch_bounds__(256) void gemm_kernel(P p) {
-  extern __shared__ char lds_raw[];
+__device__ __forceinline__ void gemm_tile(const P &p, int b, int m0, int n0, char *lds_raw) {
   Tile<T> *tiles = reinterpret_cast<Tile<T> *>(lds_raw);  // [2 buffers][A, B]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  // XCD-aware tile order: consecutive workgroups go to different XCDs
-  // (round robin over 8).  When the batch count is a multiple of 8, batch b
-  // runs entirely on XCD b % 8 (its tiles share that XCD's L2 for their
-  // operands) and neighbouring batches still spread over all XCDs
-  // (tune[2] = 1 selects the plain linear order).
-  const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
-  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  int b;
-  if (gridDim.z % 8 == 0 && p.g.tune[2] == 0) {
-    const unsigned xcd = lin % 8, q = lin / 8;
-    b = xcd + 8 * (q / tpb);
-    lin = q % tpb;
-  } else {
-    b = lin / tpb;
-    lin %= tpb;
-  }
-  const int m0 = (lin / gx) * GBM, n0 = (lin % gx) * GBN;
   using acc_t = typename Mfma<T>::acc;
   acc_t aR[2][2], aI[2][2];
 #pragma unroll
@@ -227,6 +211,28 @@ __global__ __launch_bounds__(256) void gemm_kernel(P p) {
         const int col = n0 + wn * 32 + sj * 16 + xl;
         p.store(b, row, col, (double)aR[si][sj][v], (double)aI[si][sj][v]);
       }
+}
+
+template <typename T, class P>
+__global__ __launch_bounds__(256) void gemm_kernel(P p) {
+  extern __shared__ char lds_raw[];
+  // XCD-aware tile order: consecutive workgroups go to different XCDs
+  // (round robin over 8).  When the batch count is a multiple of 8, batch b
+  // runs entirely on XCD b % 8 (its tiles share that XCD's L2 for their
+  // operands) and neighbouring batches still spread over all XCDs
+  // (tune[2] = 1 selects the plain linear order).
+  const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
+  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  int b;
+  if (gridDim.z % 8 == 0 && p.g.tune[2] == 0) {
+    const unsigned xcd = lin % 8, q = lin / 8;
+    b = xcd + 8 * (q / tpb);
+    lin = q % tpb;
+  } else {
+    b = lin / tpb;
+    lin %= tpb;
+  }
+  gemm_tile<T, P>(p, b, (lin / gx) * GBM, (lin % gx) * GBN, lds_raw);
 }
 
 // Split-K view of a single-batch problem: the batch index is the K split

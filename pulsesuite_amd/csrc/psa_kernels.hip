@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <type_traits>
+#include <vector>
 #include <string>
 
 #include "psa_gemm.h"
@@ -1720,6 +1722,110 @@ int rhs_bf16(const RhsProb &p, int nw, hipStream_t s) {
   return launch_rhs_bf16<decltype(&t64::k_rhs), t64::lds_bytes(), t64::TILE>(p, nw, s, &t64::k_rhs);
 }
 
+// ---------------------------------------------------------------------------
+// RHS of several wire groups (any mix of N, one precision mode) in one
+// persistent launch over work queues.  Work items are the 64 x 64 output
+// tiles of every (group, wire, equation).  The wires, in decreasing N, are
+// dealt round robin to eight lists, one per XCD (workgroup i runs on XCD
+// i % 8), each list wire-major so a wire's tiles share that XCD's L2.  A
+// workgroup takes items from its own list's atomic counter and, when that
+// list is exhausted, from the other lists in turn, until all are; the last
+// workgroup to leave resets the counters, so the next launch (or graph
+// replay) finds them at zero without a memset.  The tile bodies are the
+// single-group ones, so every tile is computed exactly as psa_rhs does.
+// ---------------------------------------------------------------------------
+constexpr int NXCD = 8;
+
+struct MultiHdr {
+  int32_t n, prec, kind, nwires;
+  int32_t xstart[NXCD + 1];  // wire entries of list x: [xstart[x], xstart[x+1])
+  int32_t xcnt[NXCD];        // items in list x
+  int32_t gx[PSA_RHS_MULTI_MAX];
+  int32_t nw[PSA_RHS_MULTI_MAX];
+};
+struct WireEnt {
+  int32_t gi, j, off, pad;  // group (packed order), wire offset in its range, first item in the list
+};
+constexpr size_t multi_hdr_bytes() { return (sizeof(MultiHdr) + 255) / 256 * 256; }
+constexpr size_t multi_probs_bytes(int n) { return ((size_t)n * sizeof(RhsProb) + 255) / 256 * 256; }
+
+struct RhsQueue {
+  int32_t xstart[NXCD + 1];
+  int32_t xcnt[NXCD];
+  int32_t gx[PSA_RHS_MULTI_MAX];
+};
+
+template <int KIND>
+constexpr size_t queue_tile_bytes() {
+  return KIND == 2 ? t64::lds_bytes() : (KIND == 0 ? gemm_lds_bytes<double>() : gemm_lds_bytes<float>());
+}
+
+template <int KIND>  // 0: FP64 products, 1: FP32, 2: split-BF16
+__global__ __launch_bounds__(256) void k_rhs_queue(const RhsProb *__restrict__ probs, const WireEnt *__restrict__ ents,
+                                                   RhsQueue q, unsigned *queue) {
+  // Dynamic LDS only: the operand tiles at offset 0 (aligned as the tile
+  // bodies assume), the broadcast item index after them.
+  extern __shared__ char lds_raw[];
+  int *s_item = reinterpret_cast<int *>(lds_raw + queue_tile_bytes<KIND>());
+  const int home = blockIdx.x % NXCD;
+  for (int xo = 0; xo < NXCD; ++xo) {
+    const int x = (home + xo) % NXCD;
+    for (;;) {
+      if (threadIdx.x == 0) *s_item = (int)atomicAdd(&queue[x], 1u);
+      __syncthreads();
+      const int it = __builtin_amdgcn_readfirstlane(*s_item);
+      __syncthreads();
+      if (it >= q.xcnt[x]) break;
+      // the list's last wire entry starting at or before item it
+      int lo = q.xstart[x], hi = q.xstart[x + 1] - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ents[mid].off <= it) lo = mid;
+        else hi = mid - 1;
+      }
+      const WireEnt e = ents[lo];
+      const int gx = q.gx[e.gi], tpb = gx * gx, local = it - e.off;
+      const int b = 3 * e.j + local / tpb, lin = local % tpb;
+      const int m0 = (lin / gx) * GBM, n0 = (lin % gx) * GBN;
+      if constexpr (KIND == 2)
+        t64::rhs_tile(probs[e.gi], b, m0, n0, lds_raw);
+      else
+        gemm_tile<typename std::conditional<KIND == 0, double, float>::type, RhsProb>(probs[e.gi], b, m0, n0,
+                                                                                      lds_raw);
+    }
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[NXCD], 1u) == gridDim.x - 1) {
+      for (int x = 0; x <= NXCD; ++x) atomicExch(&queue[x], 0u);
+    }
+  }
+}
+
+template <int KIND>
+int launch_rhs_queue(const RhsProb *probs_dev, const WireEnt *ents_dev, const RhsQueue &q, int total, unsigned *queue,
+                     hipStream_t s) {
+  constexpr size_t shm = queue_tile_bytes<KIND>() + 16;
+  static int blocks = 0;  // resident workgroups on the whole device, per instantiation
+  if (!blocks) {
+    int rc = hip_check(hipFuncSetAttribute((const void *)k_rhs_queue<KIND>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)shm),
+                       "hipFuncSetAttribute(k_rhs_queue)");
+    if (rc) return rc;
+    int per = 0, dev = 0, cus = 0;
+    rc = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_queue<KIND>, 256, shm),
+                   "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (rc) return rc;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return fail(PSA_E_HIP, "psa_rhs_multi: device query failed");
+    blocks = std::max(1, per) * std::max(1, cus);
+  }
+  const int grid = std::min(blocks, std::max(total, NXCD));
+  hipLaunchKernelGGL(k_rhs_queue<KIND>, dim3(grid), dim3(256), shm, s, probs_dev, ents_dev, q, queue);
+  PSA_LAUNCH_CHECK("k_rhs_queue");
+  return PSA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1846,6 +1952,108 @@ int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt,
   p.dt = dt;
   if (g->prec == PSA_PREC_BF16X3) return rhs_bf16(p, nw, (hipStream_t)stream);
   return launch_gemm(p, g->prec, g->N, g->N, 3 * nw, (hipStream_t)stream, "rhs gemm");
+}
+
+int64_t psa_rhs_multi_bytes(const int32_t *nw, int32_t n) {
+  if (!nw || n < 1 || n > PSA_RHS_MULTI_MAX) return -1;
+  long W = 0;
+  for (int i = 0; i < n; ++i) {
+    if (nw[i] < 1) return -1;
+    W += nw[i];
+  }
+  return (int64_t)(multi_hdr_bytes() + multi_probs_bytes(n) + (size_t)W * sizeof(WireEnt));
+}
+
+int psa_rhs_multi_pack(const psa_group *const *groups, const int32_t *w0, const int32_t *nw, int32_t n, int32_t mode,
+                       double dt, void *packed) {
+  if (!groups || !w0 || !nw || !packed || n < 1 || n > PSA_RHS_MULTI_MAX)
+    return fail(PSA_E_INVALID, "psa_rhs_multi_pack: bad arguments (1 <= n <= PSA_RHS_MULTI_MAX)");
+  if (mode < 0 || mode > 5) return fail(PSA_E_INVALID, "psa_rhs_multi_pack: mode must be in [0, 5]");
+  int order[PSA_RHS_MULTI_MAX];
+  for (int i = 0; i < n; ++i) {
+    if (int rc = check_group(groups[i], w0[i], nw[i])) return rc;
+    if (groups[i]->prec != groups[0]->prec) return fail(PSA_E_INVALID, "psa_rhs_multi_pack: groups differ in precision");
+    if (mode >= 2 && (!groups[i]->pR || !groups[i]->CR || !groups[i]->DR))
+      return fail(PSA_E_INVALID, "psa_rhs_multi_pack: RK4 stages need pR, CR, DR");
+    if (groups[i]->prec == PSA_PREC_BF16X3 && !groups[i]->spl)
+      return fail(PSA_E_INVALID, "psa_rhs_multi_pack: bf16x3 mode needs the spl operand buffer");
+    order[i] = i;
+  }
+  std::stable_sort(order, order + n, [&](int a, int b) { return groups[a]->N > groups[b]->N; });
+  MultiHdr h;
+  std::memset(&h, 0, sizeof(h));
+  h.n = n;
+  h.prec = groups[0]->prec;
+  h.kind = h.prec == PSA_PREC_FP64 ? 0 : (h.prec == PSA_PREC_FP32 ? 1 : 2);
+  char *base = static_cast<char *>(packed);
+  RhsProb *probs = reinterpret_cast<RhsProb *>(base + multi_hdr_bytes());
+  WireEnt *ents = reinterpret_cast<WireEnt *>(base + multi_hdr_bytes() + multi_probs_bytes(n));
+  std::vector<WireEnt> lists[NXCD];
+  long cnt[NXCD] = {0};
+  int ord = 0;
+  for (int j = 0; j < n; ++j) {
+    const int i = order[j];
+    RhsProb p;
+    std::memset(&p, 0, sizeof(p));
+    p.g = *groups[i];
+    p.w0 = w0[i];
+    p.mode = mode;
+    p.dt = dt;
+    std::memcpy(&probs[j], &p, sizeof(p));
+    h.gx[j] = (groups[i]->N + GBM - 1) / GBM;
+    h.nw[j] = nw[i];
+    for (int k = 0; k < nw[i]; ++k, ++ord) {
+      const int x = ord % NXCD;
+      lists[x].push_back(WireEnt{j, k, (int32_t)cnt[x], 0});
+      cnt[x] += 3L * h.gx[j] * h.gx[j];
+      if (cnt[x] > 0x7fffffffL) return fail(PSA_E_INVALID, "psa_rhs_multi_pack: too many tiles");
+    }
+  }
+  h.nwires = ord;
+  int e = 0;
+  for (int x = 0; x < NXCD; ++x) {
+    h.xstart[x] = e;
+    h.xcnt[x] = (int32_t)cnt[x];
+    for (const WireEnt &w : lists[x]) ents[e++] = w;
+  }
+  h.xstart[NXCD] = e;
+  std::memcpy(packed, &h, sizeof(h));
+  return PSA_OK;
+}
+
+int psa_rhs_multi(const void *packed_host, const void *packed_dev, uint32_t *queue, void *stream) {
+  if (!packed_host || !packed_dev || !queue) return fail(PSA_E_INVALID, "psa_rhs_multi: null argument");
+  MultiHdr h;
+  std::memcpy(&h, packed_host, sizeof(h));
+  if (h.n < 1 || h.n > PSA_RHS_MULTI_MAX || h.nwires < 1 || h.kind < 0 || h.kind > 2 || h.xstart[NXCD] != h.nwires)
+    return fail(PSA_E_INVALID, "psa_rhs_multi: not a packed description");
+  const hipStream_t s = (hipStream_t)stream;
+  const char *bh = static_cast<const char *>(packed_host), *bd = static_cast<const char *>(packed_dev);
+  const RhsProb *probs_h = reinterpret_cast<const RhsProb *>(bh + multi_hdr_bytes());
+  if (h.kind == 2) {
+    for (int j = 0; j < h.n; ++j) {
+      RhsProb p;
+      std::memcpy(&p, &probs_h[j], sizeof(p));
+      const int nt = (p.g.N + 63) / 64;
+      hipLaunchKernelGGL(k_split_bf16, dim3(nt, nt, 6 * h.nw[j]), dim3(256), 0, s, p.g, p.w0);
+      PSA_LAUNCH_CHECK("k_split_bf16");
+    }
+  }
+  RhsQueue q;
+  long total = 0;
+  for (int x = 0; x <= NXCD; ++x) q.xstart[x] = h.xstart[x];
+  for (int x = 0; x < NXCD; ++x) {
+    q.xcnt[x] = h.xcnt[x];
+    total += h.xcnt[x];
+  }
+  for (int j = 0; j < PSA_RHS_MULTI_MAX; ++j) q.gx[j] = h.gx[j];
+  const RhsProb *probs_d = reinterpret_cast<const RhsProb *>(bd + multi_hdr_bytes());
+  const WireEnt *ents_d = reinterpret_cast<const WireEnt *>(bd + multi_hdr_bytes() + multi_probs_bytes(h.n));
+  unsigned *qu = reinterpret_cast<unsigned *>(queue);
+  const int tot = (int)std::min(total, 0x7fffffffL);
+  if (h.kind == 0) return launch_rhs_queue<0>(probs_d, ents_d, q, tot, qu, s);
+  if (h.kind == 1) return launch_rhs_queue<1>(probs_d, ents_d, q, tot, qu, s);
+  return launch_rhs_queue<2>(probs_d, ents_d, q, tot, qu, s);
 }
 
 int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
@@ -2093,6 +2301,47 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
   if ((rc = finish_impl(g, w0, nw, (hipStream_t)stream, fused))) return rc;
   if ((rc = psa_sources(g, w0, nw, stream))) return rc;
   if ((rc = psa_qw2prop(g, w0, nw, stream))) return rc;
+  return PSA_OK;
+}
+
+int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *queue, double dt, void *stream) {
+  if (!packed_host || !packed_dev || !queue) return fail(PSA_E_INVALID, "psa_step_multi: null argument");
+  MultiHdr h;
+  std::memcpy(&h, packed_host, sizeof(h));
+  if (h.n < 1 || h.n > PSA_RHS_MULTI_MAX || h.nwires < 1 || h.xstart[NXCD] != h.nwires)
+    return fail(PSA_E_INVALID, "psa_step_multi: not a packed description");
+  const RhsProb *probs_h =
+      reinterpret_cast<const RhsProb *>(static_cast<const char *>(packed_host) + multi_hdr_bytes());
+  std::vector<psa_group> gs(h.n);
+  std::vector<int> w0(h.n);
+  for (int j = 0; j < h.n; ++j) {
+    RhsProb p;
+    std::memcpy(&p, &probs_h[j], sizeof(p));
+    if (p.mode != 1 || p.dt != dt)
+      return fail(PSA_E_INVALID, "psa_step_multi: the description must be packed with mode 1 and this dt");
+    gs[j] = p.g;
+    w0[j] = p.w0;
+  }
+  int rc;
+  // the single-group step (psa_step) up to its RHS, for every group
+  for (int j = 0; j < h.n; ++j) {
+    const psa_group *g = &gs[j];
+    if ((rc = psa_prop2qw(g, w0[j], h.nw[j], stream))) return rc;
+    if ((rc = psa_prep(g, w0[j], h.nw[j], stream))) return rc;
+    if ((rc = psa_calc_h(g, w0[j], h.nw[j], stream))) return rc;
+    if (g->flags & PSA_F_OFFDIAGDPH)
+      if ((rc = psa_offdiag_tables(g, w0[j], h.nw[j], stream))) return rc;
+  }
+  if ((rc = psa_rhs_multi(packed_host, packed_dev, queue, stream))) return rc;
+  for (int j = 0; j < h.n; ++j) {
+    const psa_group *g = &gs[j];
+    if ((rc = psa_relaxation(g, w0[j], h.nw[j], dt, stream))) return rc;
+    const bool fused = (g->flags & PSA_F_DCTRANS) != 0;
+    if (fused && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)stream, true))) return rc;
+    if ((rc = finish_impl(g, w0[j], h.nw[j], (hipStream_t)stream, fused))) return rc;
+    if ((rc = psa_sources(g, w0[j], h.nw[j], stream))) return rc;
+    if ((rc = psa_qw2prop(g, w0[j], h.nw[j], stream))) return rc;
+  }
   return PSA_OK;
 }
 

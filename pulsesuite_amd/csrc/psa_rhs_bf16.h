@@ -335,27 +335,13 @@ __device__ __forceinline__ void bslab(const BTile &sA, const BTile &sB, const BT
   }
 }
 
-__global__ __launch_bounds__(256) void k_rhs(RhsProb p) {
-  extern __shared__ char lds_raw[];
+// One 64 x 64 output tile (batch b = 3 wire + equation, rows m0..,
+// columns n0..); lds_raw holds lds_bytes() of operand tiles.
+__device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0, char *lds_raw) {
   BTile *tiles = reinterpret_cast<BTile *>(lds_raw);  // [2 buffers][A, B]
   const psa_group &g = p.g;
   const int N = g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  // XCD-aware order: the three equations of wire j run on XCD j % 8 (they
-  // share the wire's operand planes in that XCD's L2) when the wire count
-  // is a multiple of 8; consecutive workgroups go round robin over XCDs.
-  const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
-  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  int b;
-  if (gridDim.z % 24 == 0 && g.tune[2] == 0) {
-    const unsigned xcd = lin % 8, q = lin / 8, per = 3 * tpb;
-    b = 3 * (xcd + 8 * (q / per)) + (q % per) / tpb;
-    lin = q % tpb;
-  } else {
-    b = lin / tpb;
-    lin %= tpb;
-  }
-  const int m0 = (lin / gx) * GBM, n0 = (lin % gx) * GBN;
   const int e = b % 3, w = p.w0 + b / 3;
   const int nt = p.nterms(b);
   const uint16_t *spl = reinterpret_cast<const uint16_t *>(g.spl);
@@ -410,6 +396,25 @@ __global__ __launch_bounds__(256) void k_rhs(RhsProb p) {
         const int col = n0 + wn * 32 + sj * 16 + xl;
         p.store(b, row, col, (double)aR[si][sj][v], (double)aI[si][sj][v]);
       }
+}
+
+__global__ __launch_bounds__(256) void k_rhs(RhsProb p) {
+  extern __shared__ char lds_raw[];
+  // XCD-aware order: the three equations of wire j run on XCD j % 8 (they
+  // share the wire's operand planes in that XCD's L2) when the wire count
+  // is a multiple of 8; consecutive workgroups go round robin over XCDs.
+  const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
+  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  int b;
+  if (gridDim.z % 24 == 0 && p.g.tune[2] == 0) {
+    const unsigned xcd = lin % 8, q = lin / 8, per = 3 * tpb;
+    b = 3 * (xcd + 8 * (q / per)) + (q % per) / tpb;
+    lin = q % tpb;
+  } else {
+    b = lin / tpb;
+    lin %= tpb;
+  }
+  rhs_tile(p, b, (lin / gx) * GBM, (lin % gx) * GBN, lds_raw);
 }
 
 constexpr size_t lds_bytes() { return 4 * sizeof(BTile); }

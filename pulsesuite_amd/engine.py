@@ -426,3 +426,65 @@ class WireGroup:
         """Per-wire I0 (CalcI0), sum ne2, sum nh2 as numpy."""
         ob = self.s["obs"].cpu().numpy()
         return dict(I0=ob[:, 0], sum_ne2=ob[:, 1], sum_nh2=ob[:, 2])
+
+
+class MultiRhs:
+    """The RHS (``psa_rhs`` with ``mode`` and ``dt``) of several wire groups
+    of any N and one precision mode, as one persistent launch over work
+    queues, one per XCD (``psa_rhs_multi``).  The groups' structs are packed when this is
+    built; build one per level parity (``swap_levels`` exchanges the level
+    pointers) and again when a group's wire range changes.  Results equal
+    the per-group ``WireGroup.rhs`` exactly."""
+
+    def __init__(self, groups, mode=0, dt=0.0, ranges=None):
+        self.lib = capi.load()
+        self.groups = list(groups)
+        n = len(self.groups)
+        if not 1 <= n <= capi.PSA_RHS_MULTI_MAX:
+            raise ValueError(f"1 to {capi.PSA_RHS_MULTI_MAX} groups")
+        ranges = ranges or [(0, g.nw) for g in self.groups]
+        self.dev = self.groups[0].dev
+        ptrs = (ctypes.c_void_p * n)(*[ctypes.addressof(g.S) for g in self.groups])
+        w0 = (ctypes.c_int32 * n)(*[int(r[0]) for r in ranges])
+        nw = (ctypes.c_int32 * n)(*[int(r[1]) for r in ranges])
+        nb = int(self.lib.psa_rhs_multi_bytes(ctypes.cast(nw, ctypes.c_void_p), n))
+        self.host = ctypes.create_string_buffer(nb)
+        capi.check(self.lib.psa_rhs_multi_pack(ctypes.cast(ptrs, ctypes.c_void_p), ctypes.cast(w0, ctypes.c_void_p),
+                                               ctypes.cast(nw, ctypes.c_void_p), n, int(mode), float(dt),
+                                               ctypes.cast(self.host, ctypes.c_void_p)))
+        self.packed = torch.frombuffer(bytearray(self.host.raw), dtype=torch.uint8).to(self.dev)
+        self.queue = torch.zeros(capi.PSA_RHS_MULTI_QUEUE, dtype=torch.int32, device=self.dev)
+
+    def __call__(self):
+        capi.check(self.lib.psa_rhs_multi(ctypes.cast(self.host, ctypes.c_void_p), self.packed.data_ptr(),
+                                          self.queue.data_ptr(),
+                                          ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)))
+
+
+class MultiStep:
+    """One leapfrog step (``WireGroup.step``) of several wire groups of any
+    N and one precision mode, with their RHS as one persistent launch over
+    work queues (``psa_step_multi``).  A packed description is kept per
+    level parity.  Results equal ``WireGroup.step`` per group exactly."""
+
+    def __init__(self, groups, dt):
+        self.groups = list(groups)
+        if any(g.integrator != "leapfrog" for g in self.groups):
+            raise ValueError("MultiStep steps leapfrog groups")
+        self.dt = float(dt)
+        self.lib = capi.load()
+        self._packs = {}
+
+    def _pack(self):
+        key = tuple(g.S.pB for g in self.groups)
+        if key not in self._packs:
+            self._packs[key] = MultiRhs(self.groups, mode=1, dt=self.dt)
+        return self._packs[key]
+
+    def __call__(self):
+        m = self._pack()
+        capi.check(self.lib.psa_step_multi(ctypes.cast(m.host, ctypes.c_void_p), m.packed.data_ptr(),
+                                           m.queue.data_ptr(), self.dt,
+                                           ctypes.c_void_p(torch.cuda.current_stream(m.dev).cuda_stream)))
+        for g in self.groups:
+            g.advance_levels()
