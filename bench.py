@@ -301,6 +301,48 @@ def qw3d_sample(seed, nw=4, nk=64, shape=(256, 64, 64), reps=20, dt=10e-18):
     return out
 
 
+def uneven_sample(seed, sizes=(64, 128, 192, 256), per=16, nx=4096, steps=20, warmup=4, dt=10e-18):
+    """64 wires of uneven Nk (16 each of 64, 128, 192, 256), as one wire group
+    per Nk: HIP-event ms per step with every group stepped on its own
+    (psa_step) and with the groups' RHS as one persistent launch over the
+    per-XCD work queues (psa_step_multi), in FP64 and split-BF16."""
+    import torch
+    from pulsesuite_amd.engine import MultiStep, WireGroup
+    from pulsesuite_amd.tables import QWParams
+    out = dict(workload=f"{per * len(sizes)} wires, {per} each of Nk = "
+               + ", ".join(str(n) for n in sizes) + f", Nx={nx}")
+    for prec in ("fp64", "bf16x3"):
+        rng = np.random.default_rng(seed)
+        gs = []
+        for nk in sizes:
+            rr = np.linspace(-2e-6, 2e-6, nx)
+            g = WireGroup(QWParams(L=wire_length(nk), me_rel=0.067), per, rr, prec=prec)
+            g.set_state(*seeded_state(rng, g.N, per))
+            amp = 10 ** rng.uniform(5, 7, per)
+            E = np.outer(amp, np.cos(rr / 3e-7)).astype(complex)
+            g.set_fields(E, 0.5 * E, 0 * E)
+            gs.append(g)
+        ms = MultiStep(gs, dt)
+
+        def each():
+            for g in gs:
+                g.step(dt)
+        for name, fn in (("per_group", each), ("multi", ms)):
+            for _ in range(warmup):
+                fn()
+            torch.cuda.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            for _ in range(steps):
+                fn()
+            ev[1].record()
+            torch.cuda.synchronize()
+            out[f"{prec}_{name}_ms"] = ev[0].elapsed_time(ev[1]) / steps
+        del gs, ms
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,6 +360,7 @@ def main():
     ap.add_argument("--no-transmission", action="store_true")
     ap.add_argument("--no-rk4", action="store_true")
     ap.add_argument("--no-qw3d", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only (the SBE wire batch is stepped on a single "
@@ -404,6 +447,12 @@ def main():
             res["qw3d"] = qw3d_sample(seed)
         except Exception as ex:  # report, keep the batch number
             res["qw3d"] = {"error": repr(ex)}
+    if not args.no_uneven:
+        try:
+            print("[bench] uneven Nk (persistent multi-group RHS)", file=sys.stderr, flush=True)
+            res["uneven_nk"] = uneven_sample(seed)
+        except Exception as ex:  # report, keep the batch number
+            res["uneven_nk"] = {"error": repr(ex)}
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
