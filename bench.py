@@ -183,11 +183,12 @@ def cpu_baseline(wg, prm, rr, dt, budget_s=20.0):
 
 
 def serve_sample(prec, seed, nreq=150, rate=30.0, nks=(64, 128, 256, 384, 512), steps=(100, 500),
-                 dt=10e-18, nx=1024, time_limit=90.0):
+                 dt=10e-18, nx=1024, time_limit=90.0, merged=False):
     """A bounded sample of the serving stream: Poisson arrivals at `rate`,
     Nk uniform in `nks`, E0 log-uniform in [1e5, 1e8] V/m, FWHM uniform in
     [50, 500] fs, detuning uniform in +-50 meV, active steps uniform in
-    `steps`; continuous batching over per-Nk slot pools."""
+    `steps`; continuous batching over per-Nk slot pools (merged: all pools
+    stepped together, their RHS as one persistent launch)."""
     import torch
     from pulsesuite_amd.serving import Server, SimRequest
     rng = np.random.default_rng(seed)
@@ -199,8 +200,9 @@ def serve_sample(prec, seed, nreq=150, rate=30.0, nks=(64, 128, 256, 384, 512), 
             for i in range(nreq)]
     rr = (np.arange(nx) - nx // 2) * 1e-9
     srv = Server(rr, dt=dt, prec=prec, hbm_budget=64 << 30, max_slots=64,
-                 maxsteps=steps[1], graphs=True)
-    print(f"[bench] serving {nreq} requests at {rate}/s ({prec})", file=sys.stderr, flush=True)
+                 maxsteps=steps[1], graphs=True, merged=merged)
+    print(f"[bench] serving {nreq} requests at {rate}/s ({prec}{', merged' if merged else ''})",
+          file=sys.stderr, flush=True)
     done = srv.run(reqs, time_limit=time_limit)
     lat = np.array([r.t_done - r.arrival for r in done])
     wait = np.array([r.t_admit - r.arrival for r in done])
@@ -209,7 +211,7 @@ def serve_sample(prec, seed, nreq=150, rate=30.0, nks=(64, 128, 256, 384, 512), 
                sims_per_s=len(done) / srv.wall, steps_per_s=nsteps / srv.wall,
                latency_s={"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
                queue_wait_s={"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99))},
-               pools={int(k): p.cap for k, p in srv.pools.items()}, prec=prec,
+               pools={int(k): p.cap for k, p in srv.pools.items()}, prec=prec, merged=merged,
                workload=f"Poisson {rate}/s, Nk in {list(nks)}, E0 1e5-1e8 V/m, FWHM 50-500 fs, "
                         f"detuning +-50 meV, {steps[0]}-{steps[1]} active steps, dt = 10 as, "
                         f"Nx = {nx}; sample of the 1000-request stream")
@@ -435,6 +437,10 @@ def main():
             serving = serve_sample(best, seed, nreq=args.serve_requests)
         except Exception as ex:  # report, keep the batch number
             serving = {"error": repr(ex)}
+        try:
+            res["serving_merged"] = serve_sample(best, seed, nreq=args.serve_requests, merged=True)
+        except Exception as ex:  # report, keep the batch number
+            res["serving_merged"] = {"error": repr(ex)}
     if not args.no_transmission:
         try:
             print("[bench] wires + propagator", file=sys.stderr, flush=True)

@@ -27,12 +27,17 @@ Design (one process, one GPU):
   place, so one per prefix length).
 * Requests whose step count is reached are harvested (trace and final
   populations copied out), their slot is freed and reused.
+* ``Server(..., merged=True)`` steps all pools together instead: the drives,
+  one ``psa_step_multi`` whose RHS covers the occupied slots of every pool
+  in one persistent launch over per-XCD work queues, and the records, on one
+  stream; the results equal the per-pool ticks exactly.
 
 Before its first active step a request waits for the reference's wire-off
 gate (``SBEs.py:5849-5858``): steps with |E| < 1e-3 Emax0 before the field
 first exceeds that do nothing, so the clock starts at the first open step.
 """
 
+import ctypes
 import math
 import time
 from dataclasses import dataclass, field
@@ -43,7 +48,7 @@ from scipy.constants import e as e0
 from scipy.constants import hbar
 
 from . import capi, tables
-from .engine import WireGroup
+from .engine import MultiRhs, WireGroup
 
 
 @dataclass
@@ -236,10 +241,18 @@ class Server:
     """Admission by HBM footprint, continuous batching over Nk pools."""
 
     def __init__(self, rr, dt=10e-18, prec="bf16x3", hbm_budget=64 << 30, max_slots=64,
-                 maxsteps=20000, graphs=True, prm_kw=None, integrator="leapfrog"):
+                 maxsteps=20000, graphs=True, prm_kw=None, integrator="leapfrog", merged=False):
         self.rr = np.asarray(rr, dtype=float)
         self.dt, self.prec = float(dt), prec
         self.integrator = integrator
+        # merged: every tick steps all pools together on one stream, their RHS
+        # as one persistent launch over the per-XCD work queues
+        # (psa_step_multi); results equal the per-pool ticks exactly
+        self.merged = bool(merged)
+        if self.merged and integrator != "leapfrog":
+            raise ValueError("merged ticks step leapfrog pools")
+        self._stream = None
+        self._multi = {}
         self.budget, self.max_slots = int(hbm_budget), int(max_slots)
         self.maxsteps, self.graphs = int(maxsteps), graphs
         self.prm_kw = prm_kw
@@ -256,9 +269,40 @@ class Server:
                 return None
             p = SlotPool(nk, cap, self.rr, self.dt, self.prec, self.maxsteps, self.graphs, self.prm_kw,
                          integrator=self.integrator)
+            if self.merged:
+                if self._stream is None:
+                    self._stream = torch.cuda.Stream(device=p.group.dev)
+                p.stream = self._stream
             self.used += p.group.footprint_bytes() + p.trace.numel() * 8
             self.pools[nk] = p
         return p
+
+    def _tick_merged(self, pools):
+        """One step of every occupied prefix of every pool: the drives, one
+        psa_step_multi over all pools, the records (eager, one stream)."""
+        his = [min(p.cap, -(-p.prefix() // 8) * 8) for p in pools]
+        key = tuple((p.nk, p.parity, hi) for p, hi in zip(pools, his))
+        m = self._multi.get(key)
+        if m is None:
+            m = MultiRhs([p.group for p in pools], mode=1, dt=self.dt, ranges=[(0, hi) for hi in his])
+            self._multi[key] = m
+        h = ctypes.c_void_p(self._stream.cuda_stream)
+        lib = pools[0].lib
+        with torch.cuda.stream(self._stream):
+            for p, hi in zip(pools, his):
+                capi.check(lib.psa_drive(ctypes.byref(p.group.S), 0, hi, p.drive.data_ptr(), p.dt, h))
+            capi.check(lib.psa_step_multi(ctypes.cast(m.host, ctypes.c_void_p), m.packed.data_ptr(),
+                                          m.queue.data_ptr(), self.dt, h))
+            for p, hi in zip(pools, his):
+                capi.check(lib.psa_record(ctypes.byref(p.group.S), 0, hi, p.xmid, p.trace.data_ptr(),
+                                          p.count.data_ptr(), p.maxsteps, h))
+        for p in pools:
+            p.group.swap_levels()
+            p.parity ^= 1
+            p.ticks += 1
+            for i, r in enumerate(p.slots):
+                if r is not None:
+                    p.done_steps[i] += 1
 
     def run(self, requests, progress=None, time_limit=None):
         """Serve `requests` (sorted by arrival, seconds from start).  Returns
@@ -287,8 +331,13 @@ class Server:
                     rest.append(r)
             waiting = rest
             busy = False
-            for p in self.pools.values():
-                if p.active():
+            act = [p for p in self.pools.values() if p.active()]
+            if act and self.merged:
+                steps += sum(p.active() for p in act)
+                self._tick_merged(act)
+                busy = True
+            else:
+                for p in act:
                     steps += p.active()
                     p.tick()
                     busy = True

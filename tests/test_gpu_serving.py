@@ -64,3 +64,23 @@ def test_request_same_alone_and_in_mix(prec, integrator):
         assert rel(b.final[k], a.final[k]) < 1e-12, k
     lat = [r.t_done - r.arrival for r in got.values()]
     assert min(lat) > 0
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_merged_ticks_equal_per_pool_ticks(prec):
+    """Stepping every pool with one persistent multi-group RHS launch
+    (Server(merged=True)) gives the per-pool results bit for bit."""
+    from pulsesuite_amd.serving import Server
+    mk = lambda: [_req(1, 64, e0=5e7, nsteps=40), _req(2, 128, nsteps=30),  # noqa: E731
+                  _req(3, 64, fwhm=3e-15, nsteps=25), _req(4, 128, e0=1e7, nsteps=35)]
+    print(f"[serve] per-pool {prec}", flush=True)
+    a, _ = _serve(mk(), prec=prec)
+    print(f"[serve] merged {prec}", flush=True)
+    srv = Server(RR, dt=DT, prec=prec, hbm_budget=8 << 30, max_slots=16, maxsteps=200, merged=True)
+    b = {r.rid: r for r in srv.run(mk())}
+    assert set(a) == set(b) == {1, 2, 3, 4}
+    assert len(srv._multi) > 0
+    for rid in a:
+        assert np.array_equal(a[rid].trace, b[rid].trace), rid
+        for k in ("ne", "nh", "p"):
+            assert np.array_equal(a[rid].final[k], b[rid].final[k]), (rid, k)
