@@ -70,8 +70,11 @@ __global__ __launch_bounds__(threads_per_block<T>(), 2) void k_calch(psa_group g
 
   const int N = g.N;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int which = blockIdx.z % 3, w = w0 + blockIdx.z / 3;
-  const int cb = blockIdx.x, row0 = ROWS * blockIdx.y;
+  // grid (3 Nw, row groups, column blocks): the column block varies slowest,
+  // so the long column blocks (cb = 0 spans all N rows and K values, the
+  // last ones 32) are dispatched first and the short ones fill the tail
+  const int which = blockIdx.x % 3, w = w0 + blockIdx.x / 3;
+  const int cb = blockIdx.z, row0 = ROWS * blockIdx.y;
   const int Lmax = N - 32 * cb;  // rows a' and K m' below this
   if (row0 >= Lmax) return;      // block-uniform, before any barrier
 

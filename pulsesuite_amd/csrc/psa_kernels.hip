@@ -1876,7 +1876,7 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (prec == PSA_PREC_FP32 && g->N <= chx::NMAX) {
     const int cols = (g->N + 31) / 32;  // column blocks: 32 positive and 32 non-positive d each
     constexpr int R = chx::rows_per_block<float>();
-    hipLaunchKernelGGL(chx::k_calch<float>, dim3(cols, (g->N + R - 1) / R, 3 * nw),
+    hipLaunchKernelGGL(chx::k_calch<float>, dim3(3 * nw, (g->N + R - 1) / R, cols),
                        dim3(chx::threads_per_block<float>()), 0, s, *g, w0);
     PSA_LAUNCH_CHECK("k_calch");
     return PSA_OK;
