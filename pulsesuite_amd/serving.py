@@ -30,7 +30,8 @@ Design (one process, one GPU):
 * ``Server(..., merged=True)`` steps all pools together instead: the drives,
   one ``psa_step_multi`` whose RHS covers the occupied slots of every pool
   in one persistent launch over per-XCD work queues, and the records, on one
-  stream; the results equal the per-pool ticks exactly.
+  stream, replayed from a graph per (Nk, parity, prefix) combination; the
+  results equal the per-pool ticks exactly.
 
 Before its first active step a request waits for the reference's wire-off
 gate (``SBEs.py:5849-5858``): steps with |E| < 1e-3 Emax0 before the field
@@ -253,6 +254,8 @@ class Server:
             raise ValueError("merged ticks step leapfrog pools")
         self._stream = None
         self._multi = {}
+        self._mgraphs = {}
+        self._mseen = {}
         self.budget, self.max_slots = int(hbm_budget), int(max_slots)
         self.maxsteps, self.graphs = int(maxsteps), graphs
         self.prm_kw = prm_kw
@@ -279,16 +282,18 @@ class Server:
 
     def _tick_merged(self, pools):
         """One step of every occupied prefix of every pool: the drives, one
-        psa_step_multi over all pools, the records (eager, one stream)."""
+        psa_step_multi over all pools, the records, on one stream; replayed
+        from a captured graph once the same (Nk, parity, prefix) combination
+        has run twice."""
         his = [min(p.cap, -(-p.prefix() // 8) * 8) for p in pools]
         key = tuple((p.nk, p.parity, hi) for p, hi in zip(pools, his))
         m = self._multi.get(key)
         if m is None:
             m = MultiRhs([p.group for p in pools], mode=1, dt=self.dt, ranges=[(0, hi) for hi in his])
             self._multi[key] = m
-        h = ctypes.c_void_p(self._stream.cuda_stream)
         lib = pools[0].lib
-        with torch.cuda.stream(self._stream):
+
+        def launch(h):
             for p, hi in zip(pools, his):
                 capi.check(lib.psa_drive(ctypes.byref(p.group.S), 0, hi, p.drive.data_ptr(), p.dt, h))
             capi.check(lib.psa_step_multi(ctypes.cast(m.host, ctypes.c_void_p), m.packed.data_ptr(),
@@ -296,6 +301,20 @@ class Server:
             for p, hi in zip(pools, his):
                 capi.check(lib.psa_record(ctypes.byref(p.group.S), 0, hi, p.xmid, p.trace.data_ptr(),
                                           p.count.data_ptr(), p.maxsteps, h))
+
+        seen = self._mseen.get(key, 0)
+        self._mseen[key] = seen + 1
+        with torch.cuda.stream(self._stream):
+            if self.graphs and seen >= 2:
+                gph = self._mgraphs.get(key)
+                if gph is None:
+                    gph = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(gph, stream=self._stream):
+                        launch(ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+                    self._mgraphs[key] = gph
+                gph.replay()
+            else:
+                launch(ctypes.c_void_p(self._stream.cuda_stream))
         for p in pools:
             p.group.swap_levels()
             p.parity ^= 1

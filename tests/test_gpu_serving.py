@@ -79,7 +79,7 @@ def test_merged_ticks_equal_per_pool_ticks(prec):
     srv = Server(RR, dt=DT, prec=prec, hbm_budget=8 << 30, max_slots=16, maxsteps=200, merged=True)
     b = {r.rid: r for r in srv.run(mk())}
     assert set(a) == set(b) == {1, 2, 3, 4}
-    assert len(srv._multi) > 0
+    assert len(srv._multi) > 0 and len(srv._mgraphs) > 0
     for rid in a:
         assert np.array_equal(a[rid].trace, b[rid].trace), rid
         for k in ("ne", "nh", "p"):
