@@ -294,16 +294,23 @@ int psa_xqw_table(int32_t N, const double *Ee, const double *Eh, const double *f
  *    wire point (trilinear cell i0, j0, k0 and weights from the caller's
  *    locator), broadcast along x into every wire's efield;
  *  - psa_qw3d_deposit (after psa_step): J = sum_w (P - P_old) / dt / Nw plus
- *    CalcJfx (rhoPJ.py:491-523) of the summed charge / Nw for x; placement
- *    (QWPlacement, rhoPJ.py:622-700) of J into Jx, Jy, Jz, of sum Rho / Nw
- *    into Rho and of sum P / Nw into fresh Px, Py, Pz, with profiles gy, gz;
- *    then ElongfromRho (rhoPJ.py:703-880): 3-D FFTs of Rho and P, the
- *    k-space loop (rhoPJ.py:526-619), inverse FFTs of the nine longitudinal
- *    fields and the bound charge; P_old <- P.
+ *    CalcJfx (rhoPJ.py:491-523) of (sum Rho - sum Rho_old) / Nw for x;
+ *    placement (QWPlacement, rhoPJ.py:655-700, which zeroes the grid first)
+ *    of J into Jx, Jy, Jz, of sum Rho / Nw into Rho and of sum P / Nw into
+ *    fresh Px, Py, Pz, with profiles gy, gz: placed grids are OVERWRITTEN,
+ *    grids whose flag is off are left as the caller passed them; then
+ *    ElongfromRho (rhoPJ.py:703-880): 3-D FFTs of Rho and P, the k-space
+ *    loop (rhoPJ.py:526-619), inverse FFTs of the nine longitudinal fields
+ *    and the bound charge.
+ *    P_old and Rho_old are zero, as in the reference, which discards its
+ *    previous values at the top of every call (rhoPJ.py:256-270): J is then
+ *    sum P / (dt Nw).  With PSA_QW3D_KEEP_OLD they are the previous call's
+ *    values instead (kept in `old`), so J is the polarization current.
  * Grid arrays are complex [nx][ny][nz], z fastest, device pointers. */
-#define PSA_QW3D_CURR 1 /* place J (DoQWCurr) */
-#define PSA_QW3D_RHO 2  /* place Rho (DoQWDl) */
-#define PSA_QW3D_POL 4  /* place P and compute the longitudinal fields (DoQWP) */
+#define PSA_QW3D_CURR 1     /* place J (DoQWCurr) */
+#define PSA_QW3D_RHO 2      /* place Rho (DoQWDl) */
+#define PSA_QW3D_POL 4      /* place P and compute the longitudinal fields (DoQWP) */
+#define PSA_QW3D_KEEP_OLD 8 /* J from P - P_old, CalcJfx from Rho - Rho_old (not the reference) */
 
 typedef struct psa_qw3d {
   int32_t nx, ny, nz;           /* each 2^a 3^b in [2, 512] */
@@ -316,12 +323,13 @@ typedef struct psa_qw3d {
   const double *gy, *gz;        /* [ny], [nz] real placement profiles */
   const double *tw;             /* complex exp(-2 pi i m / n): [nx], then [ny], then [nz] */
   const double *Ex, *Ey, *Ez;   /* grid fields (read) */
-  double *Jx, *Jy, *Jz, *Rho;   /* grid sources, accumulated into */
+  double *Jx, *Jy, *Jz, *Rho;   /* grid sources, overwritten when placed */
   double *Fk;                   /* [4] grids scratch: Rho, Px, Py, Pz, then their FFTs */
   double *El;                   /* [10] grids: E_l from P+Rho (x, y, z), from P (x, y, z),
                                    from Rho (x, y, z), bound charge */
   double *lines;                /* [8][nx] complex scratch */
-  double *old;                  /* [4][Nw][nx] complex: the wires' previous Pxx, Pyy, Pzz, Rho */
+  double *old;                  /* [4][Nw][nx] complex: the wires' previous Pxx, Pyy, Pzz, Rho
+                                   (read and written only with PSA_QW3D_KEEP_OLD) */
 } psa_qw3d;
 
 int psa_qw3d_size(void);

@@ -11,19 +11,26 @@ Restated from the reference (paths relative to ``src/pulsesuite/PSTD3D/``):
 * ``calc_jfx``: ``CalcJfx`` (``rhoPJ.py:491-523``), J[0] = 0,
   J[i] = J[i-1] - (rho_new - rho_old)[i] / dt * dx;
 * ``qw_placement``: ``QWPlacement`` / ``_QWPlacement_jit``
-  (``rhoPJ.py:622-700``), F[i,j,k] += gx[i] gy[j] gz[k] f[i];
+  (``rhoPJ.py:622-700``): F[:] = 0 (``rhoPJ.py:682``), then
+  F[i,j,k] = F[i,j,k] + gx[i] gy[j] gz[k] f[i];
 * ``elong_from_rho``: ``ElongfromRho`` (``rhoPJ.py:703-880``) with the k-space
   loop ``_ElongfromRho_loop_jit`` (``rhoPJ.py:526-619``);
 * ``QuantumWire3D.step``: the per-step part of ``QuantumWire``
   (``rhoPJ.py:281-449``) around the wires' QWCalculator calls: every wire
-  is driven by the grid field at (xqw = 0, y0, z0) broadcast along x, the
-  currents are the summed polarization changes over dt plus CalcJfx of the
-  summed charge, divided by Nw; J, Rho and P are placed with Gaussian
-  profiles in y and z; the longitudinal fields follow from Rho and P.
+  is driven by the grid field at (xqw = 0, y0, z0) broadcast along x; the
+  previous P and Rho are discarded at the top of the call
+  (``rhoPJ.py:256-270``), so the currents are the summed polarizations over
+  dt plus CalcJfx of the summed charge against zero, divided by Nw
+  (``keep_old=True`` keeps the previous call's values instead, which the
+  reference does not do); J, Rho and P are placed with Gaussian profiles in
+  y and z; the longitudinal fields follow from Rho and P.
 
-Pinning (tests/test_qw3d_oracle.py): trilinear interpolation is exact on
-trilinear functions and at grid points; CalcJfx satisfies the discrete
-continuity equation; the placement conserves sum_jk F = f sum gy sum gz;
+Pinning (tests/test_qw3d_oracle.py): the reference's own QuantumWire
+(vectors written by tests/golden/make_golden.py: two wires, three calls,
+non-zero incoming grids); trilinear interpolation is exact on trilinear
+functions and at grid points; CalcJfx satisfies the discrete continuity
+equation; the placement of f on any prior grid gives f gy gz, with
+sum_jk F = f sum gy sum gz;
 the longitudinal fields are curl-free, satisfy Gauss's law i q.E = rho/eps
 (up to the reference's small shift of q^2) and cancel the longitudinal
 part of P (i q.(eps E_P) = -i q.P = the bound charge); numpy.fft.fftn is
@@ -76,6 +83,7 @@ def calc_jfx(rho_new, rho_old, dt, dx):
 
 
 def qw_placement(fwire, gx, gy, gz, fgrid):
+    fgrid[:] = 0.0
     fgrid += ((gx[:, None, None] * gy[None, :, None]) * gz[None, None, :]) * fwire[:, None, None]
 
 
@@ -104,7 +112,7 @@ class QuantumWire3D:
     [nx][ny][nz] with coordinates rx, ry, rz and wavenumbers qx, qy, qz."""
 
     def __init__(self, solver, rx, ry, rz, qx, qy, qz, y0, z0, ay, az, epsr,
-                 curr=True, dl=True, pol=True):
+                 curr=True, dl=True, pol=True, keep_old=False):
         self.sv = solver
         self.rx, self.ry, self.rz = rx, ry, rz
         self.qx, self.qy, self.qz = qx, qy, qz
@@ -112,14 +120,16 @@ class QuantumWire3D:
         self.gx = np.ones(len(rx))
         self.gy = np.exp(-((ry - y0) ** 2) / (2.0 * ay**2))
         self.gz = np.exp(-((rz - z0) ** 2) / (2.0 * az**2))
-        self.curr, self.dl, self.pol = curr, dl, pol
+        self.curr, self.dl, self.pol, self.keep_old = curr, dl, pol, keep_old
         nx, nw = len(rx), solver.Nw
         self.old = np.zeros((4, nx, nw), dtype=complex)
 
     def step(self, Ex, Ey, Ez, dt, Jx, Jy, Jz, Rho):
-        """One QuantumWire call; Jx..Rho accumulated in place.  Returns the
-        longitudinal fields (elong_from_rho) or None."""
+        """One QuantumWire call; Jx..Rho overwritten in place when placed.
+        Returns the longitudinal fields (elong_from_rho) or None."""
         nx, nw = len(self.rx), self.sv.Nw
+        if not self.keep_old:
+            self.old = np.zeros((4, nx, nw), dtype=complex)
         out = np.zeros((4, nx, nw), dtype=complex)
         E = [eat_xyz(f, self.rx, self.ry, self.rz, 0.0, self.y0, self.z0) for f in (Ex, Ey, Ez)]
         for w in range(nw):

@@ -590,63 +590,48 @@ def _vsq(VC, N):
     return Vsq
 
 
-def offdiag_deph_e2(g, ne, nh, VC, gee, geh):
-    """dephasing.py:226-270, 381-397 (vectorised over p, same index sets)."""
+def _offdiag_deph2(g, Ea, Eb, na, nb, Vsq, ch, gaa, gab):
+    """The two sums of CalcOffDiagDeph_E2/H2 for carrier a (na, Ea) against
+    itself (channel ch, broadening gaa) and against the other carrier b
+    (channel 0, broadening gab): for every q, all k with k + q in range at
+    once (a row per k), the p-sums over the reference's index sets; each row
+    is reduced on its own, in the order the loop over k would."""
     N = g.Nk
-    Ee, Eh = g.Ee, g.Eh
-    Vsq = _vsq(VC, N)
-    ne = np.abs(ne)
-    nh = np.abs(nh)
     D = np.zeros((2 * N + 1, N))
     P = np.arange(N)
-    for k in range(N):
-        for q in range(-N, N + 1):
-            # loop 1: p - q in [1, N-1] (q <= p-1, q >= p-N), k+q in range
-            if 0 <= k + q < N:
-                pm = P[(P - q >= 1) & (P - q < N)]
-                if pm.size:
-                    Ed = Ee[k + q] + Ee[pm - q] - Ee[pm] - Ee[k]
-                    D[q + N, k] += np.sum(
-                        Vsq[q + N, 1] * lrtz(Ed, hbar * gee)
-                        * (ne[pm - q] * (1.0 - ne[pm]) * (1.0 - ne[k])
-                           + (1.0 - ne[pm - q]) * ne[pm] * ne[k]))
-                pp = P[(P + q >= 0) & (P + q < N)]
-                if pp.size:
-                    Ed = Ee[k + q] + Eh[pp + q] - Eh[pp] - Ee[k]
-                    D[q + N, k] += np.sum(
-                        Vsq[q + N, 0] * lrtz(Ed, hbar * geh)
-                        * (nh[pp + q] * (1.0 - nh[pp]) * (1.0 - ne[k])
-                           + (1.0 - nh[pp + q]) * nh[pp] * ne[k]))
+    for q in range(-N, N + 1):
+        ks = P[(P + q >= 0) & (P + q < N)]
+        if not ks.size:
+            continue
+        kc = ks[:, None]
+        # loop 1: p - q in [1, N-1]
+        pm = P[(P - q >= 1) & (P - q < N)]
+        if pm.size:
+            Ed = Ea[kc + q] + Ea[pm - q][None, :] - Ea[pm][None, :] - Ea[kc]
+            D[q + N, ks] += np.sum(
+                Vsq[q + N, ch] * lrtz(Ed, hbar * gaa)
+                * (na[pm - q][None, :] * (1.0 - na[pm])[None, :] * (1.0 - na[kc])
+                   + (1.0 - na[pm - q])[None, :] * na[pm][None, :] * na[kc]), axis=1)
+        # loop 2: p + q in [0, N-1]
+        pp = P[(P + q >= 0) & (P + q < N)]
+        if pp.size:
+            Ed = Ea[kc + q] + Eb[pp + q][None, :] - Eb[pp][None, :] - Ea[kc]
+            D[q + N, ks] += np.sum(
+                Vsq[q + N, 0] * lrtz(Ed, hbar * gab)
+                * (nb[pp + q][None, :] * (1.0 - nb[pp])[None, :] * (1.0 - na[kc])
+                   + (1.0 - nb[pp + q])[None, :] * nb[pp][None, :] * na[kc]), axis=1)
     return D * pi / hbar
+
+
+def offdiag_deph_e2(g, ne, nh, VC, gee, geh):
+    """dephasing.py:226-270, 381-397 (vectorised over k and p, same index
+    sets and per-element operation order)."""
+    return _offdiag_deph2(g, g.Ee, g.Eh, np.abs(ne), np.abs(nh), _vsq(VC, g.Nk), 1, gee, geh)
 
 
 def offdiag_deph_h2(g, ne, nh, VC, ghh, geh):
     """dephasing.py:273-317, 400-414."""
-    N = g.Nk
-    Ee, Eh = g.Ee, g.Eh
-    Vsq = _vsq(VC, N)
-    ne = np.abs(ne)
-    nh = np.abs(nh)
-    D = np.zeros((2 * N + 1, N))
-    P = np.arange(N)
-    for k in range(N):
-        for q in range(-N, N + 1):
-            if 0 <= k + q < N:
-                pm = P[(P - q >= 1) & (P - q < N)]
-                if pm.size:
-                    Ed = Eh[k + q] + Eh[pm - q] - Eh[pm] - Eh[k]
-                    D[q + N, k] += np.sum(
-                        Vsq[q + N, 2] * lrtz(Ed, hbar * ghh)
-                        * (nh[pm - q] * (1.0 - nh[pm]) * (1.0 - nh[k])
-                           + (1.0 - nh[pm - q]) * nh[pm] * nh[k]))
-                pp = P[(P + q >= 0) & (P + q < N)]
-                if pp.size:
-                    Ed = Eh[k + q] + Ee[pp + q] - Ee[pp] - Eh[k]
-                    D[q + N, k] += np.sum(
-                        Vsq[q + N, 0] * lrtz(Ed, hbar * geh)
-                        * (ne[pp + q] * (1.0 - ne[pp]) * (1.0 - nh[k])
-                           + (1.0 - ne[pp + q]) * ne[pp] * nh[k]))
-    return D * pi / hbar
+    return _offdiag_deph2(g, g.Eh, g.Ee, np.abs(nh), np.abs(ne), _vsq(VC, g.Nk), 2, ghh, geh)
 
 
 def offdiag_dephasing2(g, ne, nh, p, VC):

@@ -423,13 +423,9 @@ class SBESolver:
         g.step(dt, w0=i, nw=1)
         P = g.outputs(i, 1)
         out = (P[0][0], P[1][0], P[2][0], P[3][0])
-        if self.ignorewire:
-            out = tuple(np.zeros_like(o) for o in out)
-        Pxx[:] = out[0]
-        Pyy[:] = out[1]
-        Pzz[:] = out[2]
-        Rho[:] = out[3]
         self.I0[i] = g.observables()["I0"][i]
+        # The P.E energy and the Prop/* snapshots see the wire's real sources;
+        # ignorewire only zeroes what the caller gets back (SBEs.py:5919-5985).
         self._energy(np.asarray(Exx), np.asarray(Eyy), np.asarray(Ezz), out, i)
         if self.io:
             self._write_files(np.asarray(Exx), np.asarray(Eyy), np.asarray(Ezz), dt, w)
@@ -437,6 +433,12 @@ class SBESolver:
                 self._write_max_offdiag(i)
             if write_fields:
                 self._write_snapshots(np.asarray(Exx), np.asarray(Eyy), np.asarray(Ezz), out, w)
+        if self.ignorewire:
+            out = tuple(np.zeros_like(o) for o in out)
+        Pxx[:] = out[0]
+        Pyy[:] = out[1]
+        Pzz[:] = out[2]
+        Rho[:] = out[3]
 
     def QWCalculatorBatch(self, Exx, Eyy, Ezz, dt):
         """Step every wire at once; fields [Nw, Nx] (or [Nx] shared) as numpy

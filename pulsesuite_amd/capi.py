@@ -66,7 +66,7 @@ class PsaQw3d(ctypes.Structure):
 
 
 _QP = ctypes.POINTER(PsaQw3d)
-PSA_QW3D_CURR, PSA_QW3D_RHO, PSA_QW3D_POL = 1, 2, 4
+PSA_QW3D_CURR, PSA_QW3D_RHO, PSA_QW3D_POL, PSA_QW3D_KEEP_OLD = 1, 2, 4, 8
 PSA_PSTD_ABC = 1
 PSA_PSTD_MAXW = 256
 PSA_RHS_MULTI_MAX = 16

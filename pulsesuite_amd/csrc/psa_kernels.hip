@@ -2602,16 +2602,20 @@ __global__ __launch_bounds__(256) void k_q3_drive(psa_qw3d q, psa_group g) {
 }
 
 // per x: lines[0..2] = sum_w (P - P_old) / dt, [3] = sum_w Rho, [4..6] =
-// sum_w P, [7] = (sum_w Rho - sum_w Rho_old) / dt; P_old <- P
+// sum_w P, [7] = (sum_w Rho - sum_w Rho_old) / dt.  The reference discards
+// its previous P and Rho at the top of every call (rhoPJ.py:256-270), so
+// P_old = Rho_old = 0 there; with PSA_QW3D_KEEP_OLD they are the values
+// this kernel stored on the previous call (P_old <- P, Rho_old <- Rho).
 __global__ __launch_bounds__(256) void k_q3_lines(psa_qw3d q, psa_group g) {
   const int i = blockIdx.x * 256 + threadIdx.x, nx = q.nx, W = g.Nw;
   if (i >= nx) return;
+  const bool keep = (q.flags & PSA_QW3D_KEEP_OLD) != 0;
   cd d[3] = {}, ps[3] = {}, rs{0.0, 0.0}, ro{0.0, 0.0};
   for (int w = 0; w < W; ++w) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const long at = ((long)c * W + w) * nx + i;
-      const cd v = ld(g.out, at), o = ld(q.old, at);
+      const cd v = ld(g.out, at), o = keep ? ld(q.old, at) : cd{0.0, 0.0};
       if (c < 3) {
         d[c] = cadd(d[c], csub(v, o));
         ps[c] = cadd(ps[c], v);
@@ -2619,7 +2623,7 @@ __global__ __launch_bounds__(256) void k_q3_lines(psa_qw3d q, psa_group g) {
         rs = cadd(rs, v);
         ro = cadd(ro, o);
       }
-      st(q.old, at, v);
+      if (keep) st(q.old, at, v);
     }
   }
   const double idt = 1.0 / q.dt;
@@ -2655,7 +2659,11 @@ __global__ __launch_bounds__(256) void k_q3_jfx(psa_qw3d q, int W) {
   }
 }
 
-// QWPlacement: grid += (gx gy) gz f[i] (gx = 1) for J, Rho; fresh P into Fk[1..3]
+// QWPlacement (rhoPJ.py:655-700): the grid is zeroed, then
+// F[i][j][k] = 0 + (gx gy) gz f[i] with gx = 1, so a placed grid is
+// overwritten: J into Jx, Jy, Jz (CURR), the charge into Rho (RHO), the
+// polarization into Fk[1..3] (POL, with Fk[0] = Rho as ElongfromRho reads
+// it).  Grids whose flag is off keep the caller's values.
 __global__ __launch_bounds__(256) void k_q3_place(psa_qw3d q) {
   const long n = (long)q.nx * q.ny * q.nz, plane = (long)q.ny * q.nz;
   for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
@@ -2664,11 +2672,17 @@ __global__ __launch_bounds__(256) void k_q3_place(psa_qw3d q) {
     if (q.flags & PSA_QW3D_CURR) {
       double *J[3] = {q.Jx, q.Jy, q.Jz};
 #pragma unroll
-      for (int c = 0; c < 3; ++c) st(J[c], t, cadd(ld(J[c], t), cscale(ld(q.lines, (long)c * q.nx + i), gg)));
+      for (int c = 0; c < 3; ++c) st(J[c], t, cscale(ld(q.lines, (long)c * q.nx + i), gg));
     }
-    if (q.flags & PSA_QW3D_RHO) st(q.Rho, t, cadd(ld(q.Rho, t), cscale(ld(q.lines, 3L * q.nx + i), gg)));
+    cd rho;
+    if (q.flags & PSA_QW3D_RHO) {
+      rho = cscale(ld(q.lines, 3L * q.nx + i), gg);
+      st(q.Rho, t, rho);
+    } else {
+      rho = ld(q.Rho, t);
+    }
     if (q.flags & PSA_QW3D_POL) {
-      st(q.Fk, t, ld(q.Rho, t));
+      st(q.Fk, t, rho);
 #pragma unroll
       for (int c = 0; c < 3; ++c) st(q.Fk, (1 + c) * n + t, cscale(ld(q.lines, (long)(4 + c) * q.nx + i), gg));
     }

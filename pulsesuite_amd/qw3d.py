@@ -10,12 +10,21 @@ Each step (:meth:`QuantumWire3D.step`):
   interpolated (``EAtXYZ``, ``usefulsubs.py:1996-2065``), broadcast along
   every wire;
 * the wires step (``psa_step``);
-* deposit: J = sum over wires of the polarization change / dt / Nw, plus
-  ``CalcJfx`` of the summed charge / Nw along x (``rhoPJ.py:491-523``); J
-  added into Jx, Jy, Jz, the mean charge into Rho and the mean polarization
-  into fresh Px, Py, Pz with the profiles gy = exp(-(y - y0)^2 / 2 ay^2),
-  gz likewise (``QWPlacement``, ``rhoPJ.py:622-700``); then the longitudinal
-  fields of ``ElongfromRho`` (``rhoPJ.py:703-880``) by 3-D FFTs.
+* deposit: J = sum over wires of P / dt / Nw, plus ``CalcJfx`` of the
+  summed charge / Nw along x (``rhoPJ.py:491-523``); J placed into Jx, Jy,
+  Jz, the mean charge into Rho and the mean polarization into fresh Px, Py,
+  Pz with the profiles gy = exp(-(y - y0)^2 / 2 ay^2), gz likewise
+  (``QWPlacement``, ``rhoPJ.py:655-700``); then the longitudinal fields of
+  ``ElongfromRho`` (``rhoPJ.py:703-880``) by 3-D FFTs.
+
+As in the reference, placement zeroes a grid before it deposits
+(``rhoPJ.py:682``), so Jx, Jy, Jz and Rho are overwritten every step (a grid
+whose flag is off keeps the caller's values), and the previous polarization
+and charge are discarded at the top of every call (``rhoPJ.py:256-270``), so
+J = sum P / (dt Nw) and CalcJfx sees a previous charge of zero.
+``keep_old=True`` departs from the reference on purpose: it keeps the
+previous call's P and Rho, so J is the polarization current
+sum (P - P_old) / (dt Nw) and CalcJfx sees the change of the charge.
 
 The 3-D Maxwell propagator that would consume J, Rho and the longitudinal
 fields is not part of this package; the caller owns the grid arrays.
@@ -58,10 +67,11 @@ class QuantumWire3D:
     rx, ry, rz -- grid coordinates; qx, qy, qz -- grid wavenumbers (FFT
     order); y0, z0, ay, az -- wire position and profile widths
     (``params/qwarray.params``); epsr -- background dielectric constant;
-    dt -- step; curr, rho, pol -- DoQWCurr, DoQWDl, DoQWP."""
+    dt -- step; curr, rho, pol -- DoQWCurr, DoQWDl, DoQWP; keep_old -- use
+    the previous call's P and Rho in J (not the reference's behaviour)."""
 
     def __init__(self, group, rx, ry, rz, qx, qy, qz, y0, z0, ay, az, epsr, dt,
-                 curr=True, rho=True, pol=True):
+                 curr=True, rho=True, pol=True, keep_old=False):
         self.g = group
         self.lib = capi.load()
         rx, ry, rz = (np.asarray(a, dtype=float) for a in (rx, ry, rz))
@@ -93,7 +103,7 @@ class QuantumWire3D:
         S = capi.PsaQw3d()
         S.nx, S.ny, S.nz = nx, ny, nz
         S.flags = ((capi.PSA_QW3D_CURR if curr else 0) | (capi.PSA_QW3D_RHO if rho else 0)
-                   | (capi.PSA_QW3D_POL if pol else 0))
+                   | (capi.PSA_QW3D_POL if pol else 0) | (capi.PSA_QW3D_KEEP_OLD if keep_old else 0))
         S.i0, S.j0, S.k0 = cell
         S.ax, S.ay, S.az = w
         S.dt, S.dx, S.eps = float(dt), float(rx[1] - rx[0]), eps0 * float(epsr)
@@ -122,7 +132,7 @@ class QuantumWire3D:
 
     def step(self, Ex, Ey, Ez, Jx, Jy, Jz, Rho):
         """One QuantumWire call: drive, the wires' step, deposit.  Jx, Jy,
-        Jz, Rho are accumulated in place; the longitudinal fields and the
+        Jz, Rho are overwritten in place when placed; the longitudinal fields and the
         bound charge are in :attr:`El` (device, [10][nx][ny][nz], order
         ``EL_NAMES``)."""
         self.drive(Ex, Ey, Ez)

@@ -36,6 +36,17 @@ Design (one process, one GPU):
 Before its first active step a request waits for the reference's wire-off
 gate (``SBEs.py:5849-5858``): steps with |E| < 1e-3 Emax0 before the field
 first exceeds that do nothing, so the clock starts at the first open step.
+
+Failure handling.  Every request ends with a ``status``: ``"done"``,
+``"rejected"`` (before the clock starts, with the reason in ``error``: a
+step count beyond the trace buffer or below one, a k-grid the device path
+does not support, or no room for a slot of its Nk in the HBM budget --
+nothing waits forever), ``"cancelled"`` (``Server.cancel(rid)``, while
+waiting or in flight: its slot is freed at once and reset when reused) or
+``"failed"`` (its recorded observables became non-finite: checked every
+``nan_check`` ticks and at harvest; the slot is freed).  Wires never share
+data inside a step, so a request that fails leaves the others' results
+unchanged.
 """
 
 import ctypes
@@ -68,6 +79,8 @@ class SimRequest:
     t_done: float = field(default=None, repr=False)
     trace: np.ndarray = field(default=None, repr=False)
     final: dict = field(default=None, repr=False)
+    status: str = field(default="pending", repr=False)  # done, rejected, cancelled, failed
+    error: str = field(default=None, repr=False)
 
     def pulse(self, gap_ev):
         """(E0x, E0y, E0z, omega0, tw, tp)."""
@@ -126,6 +139,10 @@ class SlotPool:
         if autotune:
             self.tuned = self.group.autotune()
 
+    def footprint(self):
+        """Device bytes held by this pool."""
+        return self.group.footprint_bytes() + self.trace.numel() * 8
+
     # -- slots -----------------------------------------------------------
     def free(self):
         return sum(1 for s in self.slots if s is None)
@@ -141,8 +158,10 @@ class SlotPool:
         return hi
 
     def admit(self, req, now):
-        if req.nsteps > self.maxsteps:
-            raise ValueError("request longer than the pool's trace buffer")
+        """Put `req` into the lowest free slot (the server has checked that
+        its step count fits the trace buffer)."""
+        if not 1 <= req.nsteps <= self.maxsteps:
+            raise ValueError("request step count outside the pool's trace buffer")
         i = self.slots.index(None)
         with torch.cuda.stream(self.stream):
             self.group.reset_wire(i)
@@ -153,7 +172,28 @@ class SlotPool:
         self.slots[i] = req
         self.done_steps[i] = 0
         req.t_admit = now
+        req.status = "running"
         return i
+
+    def release(self, i):
+        """Free slot i without harvesting it (cancellation, failure); the
+        slot's state is reset when it is next admitted."""
+        req = self.slots[i]
+        self.slots[i] = None
+        return req
+
+    def nonfinite(self):
+        """Slots whose last recorded step holds a non-finite observable
+        (one small gather and copy, on this pool's stream)."""
+        hi = self.prefix()
+        if hi == 0:
+            return []
+        with torch.cuda.stream(self.stream):
+            cnt = self.count[:hi].long()
+            rows = self.trace[torch.arange(hi, device=cnt.device), (cnt - 1).clamp(min=0)]
+            # a slot with nothing recorded yet holds a previous request's rows
+            ok = (torch.isfinite(rows).all(dim=1) | (cnt == 0)).cpu().numpy()
+        return [i for i in range(hi) if self.slots[i] is not None and not ok[i]]
 
     # -- one tick ----------------------------------------------------------
     def _launch(self, hi, S, stream_handle):
@@ -204,6 +244,9 @@ class SlotPool:
         for i in fin:
             req = self.slots[i]
             req.trace = self.trace[i, :req.nsteps].cpu().numpy()
+            req.status = "done"
+            if not np.isfinite(req.trace).all():
+                req.status, req.error = "failed", "non-finite observables in the trace"
             # final level-3 populations and polarization matrix of this slot
             # (after the swap, level 3 is in the B buffers)
             C = g.s["CB"][i].cpu().numpy()
@@ -242,8 +285,15 @@ class Server:
     """Admission by HBM footprint, continuous batching over Nk pools."""
 
     def __init__(self, rr, dt=10e-18, prec="bf16x3", hbm_budget=64 << 30, max_slots=64,
-                 maxsteps=20000, graphs=True, prm_kw=None, integrator="leapfrog", merged=False):
+                 maxsteps=20000, graphs=True, prm_kw=None, integrator="leapfrog", merged=False,
+                 nan_check=64, pool_factory=None):
         self.rr = np.asarray(rr, dtype=float)
+        # the pool class (SlotPool on the device; tests of the host logic
+        # substitute one with the same methods)
+        self.pool_factory = pool_factory or SlotPool
+        self.nan_check = int(nan_check)
+        self.rejected, self.cancelled, self.failed = [], [], []
+        self._cancel = set()
         self.dt, self.prec = float(dt), prec
         self.integrator = integrator
         # merged: every tick steps all pools together on one stream, their RHS
@@ -263,6 +313,8 @@ class Server:
         self.used = 0
 
     def pool(self, nk):
+        """The slot pool of `nk`, built on first use; None when the HBM
+        budget left cannot hold its tables and one slot."""
         p = self.pools.get(nk)
         if p is None:
             per = slot_bytes(nk, len(self.rr), self.prec, self.maxsteps, self.integrator)
@@ -270,15 +322,46 @@ class Server:
             cap = min(self.max_slots, room // per)
             if cap < 1:
                 return None
-            p = SlotPool(nk, cap, self.rr, self.dt, self.prec, self.maxsteps, self.graphs, self.prm_kw,
-                         integrator=self.integrator)
+            p = self.pool_factory(nk, cap, self.rr, self.dt, self.prec, self.maxsteps, self.graphs, self.prm_kw,
+                                  integrator=self.integrator)
             if self.merged:
                 if self._stream is None:
                     self._stream = torch.cuda.Stream(device=p.group.dev)
                 p.stream = self._stream
-            self.used += p.group.footprint_bytes() + p.trace.numel() * 8
+            self.used += p.footprint()
             self.pools[nk] = p
         return p
+
+    def cancel(self, rid):
+        """Cancel request `rid` (waiting or in flight) at the next loop
+        iteration of run(); it ends in ``cancelled``."""
+        self._cancel.add(rid)
+
+    def _reject(self, r, why):
+        r.status, r.error = "rejected", why
+        self.rejected.append(r)
+
+    def _admission_check(self, pending):
+        """Reject, before the clock starts, every request that could never
+        run: a step count outside [1, maxsteps], an Nk whose pool cannot be
+        built (unsupported k-grid) or does not fit the HBM budget."""
+        ok = []
+        bad_nk = {}
+        for nk in sorted({r.nk for r in pending}):
+            try:
+                if self.pool(nk) is None:
+                    bad_nk[nk] = (f"no HBM room for a slot of Nk={nk} "
+                                  f"(budget {self.budget} B, {self.used} B in use)")
+            except (ValueError, capi.PsaError) as exc:
+                bad_nk[nk] = f"Nk={nk} not supported: {exc}"
+        for r in pending:
+            if not 1 <= r.nsteps <= self.maxsteps:
+                self._reject(r, f"nsteps={r.nsteps} outside [1, {self.maxsteps}]")
+            elif r.nk in bad_nk:
+                self._reject(r, bad_nk[r.nk])
+            else:
+                ok.append(r)
+        return ok
 
     def _tick_merged(self, pools):
         """One step of every occupied prefix of every pool: the drives, one
@@ -323,13 +406,38 @@ class Server:
                 if r is not None:
                     p.done_steps[i] += 1
 
-    def run(self, requests, progress=None, time_limit=None):
+    def _cancel_and_fail(self, waiting, now):
+        """Apply cancellations and evict requests with non-finite observables."""
+        if self._cancel:
+            keep = []
+            for r in waiting:
+                if r.rid in self._cancel:
+                    r.status, r.t_done = "cancelled", now
+                    self.cancelled.append(r)
+                else:
+                    keep.append(r)
+            waiting[:] = keep
+            for p in self.pools.values():
+                for i, r in enumerate(p.slots):
+                    if r is not None and r.rid in self._cancel:
+                        p.release(i)
+                        r.status, r.t_done = "cancelled", now
+                        self.cancelled.append(r)
+        for p in self.pools.values():
+            if self.nan_check > 0 and p.active() and p.ticks % self.nan_check == 0 and p.ticks > 0:
+                for i in p.nonfinite():
+                    r = p.release(i)
+                    r.status, r.error, r.t_done = "failed", f"non-finite observables after tick {p.ticks}", now
+                    self.failed.append(r)
+
+    def run(self, requests, progress=None, time_limit=None, on_tick=None):
         """Serve `requests` (sorted by arrival, seconds from start).  Returns
-        the finished requests; latencies are t_done - arrival."""
-        pending = sorted(requests, key=lambda r: r.arrival)
-        for nk in sorted({r.nk for r in pending}):
-            self.pool(nk)  # build the tables up front, not on the clock
-        torch.cuda.synchronize()
+        the finished requests; latencies are t_done - arrival.  Requests that
+        end otherwise are in ``rejected``, ``cancelled`` and ``failed``.
+        ``on_tick(server, now)`` is called once per loop iteration."""
+        pending = self._admission_check(sorted(requests, key=lambda r: r.arrival))
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         waiting, done = [], []
@@ -338,8 +446,12 @@ class Server:
         while qi < len(pending) or waiting or any(p.active() for p in self.pools.values()):
             t = now()
             while qi < len(pending) and pending[qi].arrival <= t:
+                pending[qi].status = "waiting"
                 waiting.append(pending[qi])
                 qi += 1
+            if on_tick is not None:
+                on_tick(self, t)
+            self._cancel_and_fail(waiting, t)
             # admit in arrival order where a slot of the right Nk is free
             rest = []
             for r in waiting:
@@ -361,7 +473,8 @@ class Server:
                     p.tick()
                     busy = True
             for p in self.pools.values():
-                done.extend(p.harvest(now))
+                for r in p.harvest(now):
+                    (done if r.status == "done" else self.failed).append(r)
             if not busy and qi < len(pending):
                 # idle until the next arrival
                 time.sleep(max(0.0, min(pending[qi].arrival - now(), 0.01)))
@@ -369,7 +482,8 @@ class Server:
                 print(f"[serve] done {len(done)}/{len(pending)} t={now():.1f}s", flush=True)
             if time_limit is not None and now() > time_limit:
                 break
-        torch.cuda.synchronize()
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
         self.wall = now()
         self.steps = steps
         return done

@@ -32,11 +32,21 @@ recorded (composed and index-heavy behaviour):
   ``WriteQWFields``; ``Prop/*`` via ``WritePropFields``) as text;
 * ``SBESolver.calc_Xqw`` -> ``_CalcXqw_jit`` (``SBEs.py:5331-5357``,
   ``3165-3243``) and the chi(q, w) files of
-  ``_record_Xqw`` (``SBEs.py:5395-5432``).
+  ``_record_Xqw`` (``SBEs.py:5395-5432``);
+* in the snapshot run, also ``output/PdotE.p.dat`` / ``PdotE.w.dat``
+  (``SBEs.py:5925-5979``) and the text checkpoint of ``write_sbes_data``
+  (``SBEs.py:5524-5547``); a run with ``ignorewire`` = 1 for the PdotE lines
+  of that mode (``SBEs.py:5981-5985``);
+* ``CalcMBArrays`` with ``LorentzDelta`` = 1 (``coulomb.py:367-499``) and
+  ``MBCE``/``MBCH`` on those tables;
+* ``rhoPJ.QuantumWire`` (``rhoPJ.py:117-449``): two wires on a small 3-D
+  grid, three calls with non-zero incoming J and Rho grids.
 
 Usage: ``python tests/golden/make_golden.py`` (writes ``tests/golden/*.npz``;
 ``python tests/golden/make_golden.py xqw`` writes only ``ref_xqw.npz``,
-``... snap`` only ``ref_n16_snap.npz``).
+``... snap`` only ``ref_n16_snap.npz`` and ``ref_n16_ignore.npz``,
+``... lorentz`` only ``ref_n16_lorentz.npz``, ``... qw3d`` only
+``ref_qw3d.npz``).
 """
 
 import contextlib
@@ -55,6 +65,8 @@ MB_SHIPPED = [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 0, 1, 0]
 # Optics, Excitons, EHs, Screened, Phonon, DCTrans, LF, FreePot, DiagDph,
 # OffDiagDph, Recomb, PLSpec, ignorewire, Xqwparams, LorentzDelta
 MB_RHS_ONLY = [1, 1, 0, 1, 0, 0, 1, 0, 1, 1, 0, 0, 0, 0, 0]
+MB_IGNORE = [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 1, 1, 0]
+MB_LORENTZ = [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 0, 1, 1]
 
 
 def _install_shims():
@@ -281,11 +293,15 @@ def run_case(nk, seed, mb, nsteps, tag, nx=64, drr=10e-9, jmax=500, files=False)
             out["info2"] = info[2]
             out["jmax"] = jmax
             if files:
+                # the text checkpoint of the final levels (SBEs.py:5524-5547)
+                s.write_sbes_data(7)
+                s.close()
                 # every file the run wrote (the chi(q, w) record excepted):
                 # the Wire/* and Prop/* snapshots of the WriteFields steps,
-                # the window file, info/EP files
+                # the window file, info/EP files, the backups, PdotE
                 names, texts = [], []
-                for root, _, fs in os.walk("dataQW"):
+                walk = list(os.walk("dataQW")) + list(os.walk("output"))
+                for root, _, fs in walk:
                     for fn in sorted(fs):
                         path = os.path.join(root, fn)
                         if "Xqw" in path:
@@ -299,6 +315,132 @@ def run_case(nk, seed, mb, nsteps, tag, nx=64, drr=10e-9, jmax=500, files=False)
         finally:
             os.chdir(cwd)
     path = os.path.join(OUT_DIR, f"ref_{tag}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "N =", nk)
+
+
+def lorentz_case(nk=16, seed=11, nx=64, drr=10e-9):
+    """CalcMBArrays with LorentzDelta = 1 (coulomb.py:405-446) as
+    SBESolver.__init__ builds it from mb.params line 15, and MBCE/MBCH
+    (coulomb.py:1298-1326) on those tables at a random state."""
+    rng = np.random.default_rng(seed)
+    L = wire_length(nk)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            write_params(d, L, MB_LORENTZ)
+            buf = io.StringIO()
+            with contextlib.redirect_stdout(buf):
+                from pulsesuite.PSTD3D import SBEs, coulomb
+                from pulsesuite.PSTD3D.typespace import GetKArray, GetSpaceArray
+                rr = GetSpaceArray(nx, (nx - 1) * drr)
+                qrr = GetKArray(nx, nx * drr)
+                s = SBEs.SBESolver(qrr, rr, 0.0, 1e7, 800e-9, 1, True)
+            n = s.Nk
+            assert n == nk
+            ci = coulomb._instance
+            assert ci.LorentzDelta
+            C, D, p = random_state(rng, n)
+            ne = np.diag(C).copy()
+            nh = np.diag(D).copy()
+            VC = np.zeros((n, n, 3))
+            E1D = np.zeros((n, n))
+            with contextlib.redirect_stdout(buf):
+                coulomb.CalcScreenedArrays(True, s.L, ne, nh, VC, E1D)
+            W = [np.zeros(n) for _ in range(4)]
+            coulomb.MBCE(np.real(ne), np.real(nh), s.kr, s.Ee, s.Eh, VC,
+                         s.gam_eh, s.gam_e, W[0], W[1])
+            coulomb.MBCH(np.real(ne), np.real(nh), s.kr, s.Ee, s.Eh, VC,
+                         s.gam_eh, s.gam_h, W[2], W[3])
+            out.update(nk=n, L=L, kr=s.kr, Ee=s.Ee, Eh=s.Eh, gam_e=s.gam_e,
+                       gam_h=s.gam_h, gam_eh=s.gam_eh, Ceh=ci.Ceh, Cee=ci.Cee,
+                       Chh=ci.Chh, k3=ci.k3, fC=C, fD=D, fVC=VC,
+                       fWinE=W[0], fWoutE=W[1], fWinH=W[2], fWoutH=W[3])
+            s.close()
+        finally:
+            os.chdir(cwd)
+    path = os.path.join(OUT_DIR, "ref_n16_lorentz.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "N =", nk)
+
+
+def qw3d_case(nk=16, seed=13, nx=32, ny=8, nz=6, d=2e-9, nsteps=3, nw=4):
+    """rhoPJ.QuantumWire, i.e. the shim over QWArray.QuantumWire
+    (rhoPJ.py:1180-1240, 941-1160; same steps as rhoPJ.py:117-449), for the
+    wires of params/qwarray.params on a [nx][ny][nz] grid built from
+    typespace.ss, after InitializeSBE on the grid's x axis: the wires start
+    from seeded random levels, the incoming Jx, Jy, Jz and Rho grids are
+    random and non-zero, the fields are smooth and strong enough to open the
+    wire-off gate.  Records every grid after each call.  QWArray allocates
+    its per-wire arrays with its default Nw = 4 before it reads the file, so
+    only a file with Nw = 4 runs."""
+    rng = np.random.default_rng(seed)
+    L = wire_length(nk)
+    out = {}
+    with tempfile.TemporaryDirectory() as dd:
+        cwd = os.getcwd()
+        os.chdir(dd)
+        try:
+            write_params(dd, L, MB_SHIPPED)
+            with open(os.path.join(dd, "params", "qwarray.params"), "w") as f:
+                f.write(f"1\n{nw}\n0.0e-9\n0.0e-9\n5.0e-9\n4.0e-9\n")
+            buf = io.StringIO()
+            with contextlib.redirect_stdout(buf):
+                from pulsesuite.PSTD3D import SBEs, rhoPJ
+                from pulsesuite.PSTD3D import typespace as TS
+                space = TS.ss(Dims=3, Nx=nx, Ny=ny, Nz=nz, dx=d, dy=d, dz=d, epsr=9.1)
+                rx, ry, rz = TS.GetXArray(space), TS.GetYArray(space), TS.GetZArray(space)
+                qx, qy, qz = TS.GetKxArray(space), TS.GetKyArray(space), TS.GetKzArray(space)
+                lam = 800e-9
+                SBEs.InitializeSBE(qx, rx, 0.0, 3e7, lam, nw, True)
+            s = SBEs._default_solver
+            n = s.Nk
+            assert n == nk
+            for w in range(nw):
+                C0, D0, p0 = random_state(rng, n, occ=0.2, coh=0.02, pol=0.01)
+                C1, D1, p1 = random_state(rng, n, occ=0.2, coh=0.02, pol=0.01)
+                s.CC2[:, :, w] = C0
+                s.DD2[:, :, w] = D0
+                s.YY2[:, :, w] = p0
+                s.CC3[:, :, w] = C1
+                s.DD3[:, :, w] = D1
+                s.YY3[:, :, w] = p1
+            out.update(s0_C2=s.CC2.copy(), s0_D2=s.DD2.copy(), s0_p2=s.YY2.copy(),
+                       s0_C3=s.CC3.copy(), s0_D3=s.DD3.copy(), s0_p3=s.YY3.copy())
+            shp = (nx, ny, nz)
+            X, Y, Z = np.meshgrid(rx, ry, rz, indexing="ij")
+            env = np.exp(-(Y**2 + Z**2) / (8e-9) ** 2)
+            grids = [(rng.standard_normal(shp) + 1j * rng.standard_normal(shp)) * sc
+                     for sc in (1e10, 1e10, 1e10, 1e3)]
+            out.update(J0x=grids[0].copy(), J0y=grids[1].copy(), J0z=grids[2].copy(),
+                       Rho0=grids[3].copy())
+            El = [np.zeros(shp, dtype=complex) for _ in range(10)]
+            dt = 10e-18
+            w0 = 2 * np.pi * 299792458.0 / lam
+            rec = {k: [] for k in ("Ex", "Ey", "Ez", "Jx", "Jy", "Jz", "Rho", "El")}
+            for it in range(nsteps):
+                t = 40e-15 + it * dt
+                E = [(a * np.cos(w0 * t + ph) * env * (1 + 0.3 * np.cos(2 * np.pi * X / (nx * d))))
+                     .astype(complex) for a, ph in zip((3e7, 1e7, 2e6), (0.0, 0.4, 1.1))]
+                with contextlib.redirect_stdout(buf):
+                    rhoPJ.QuantumWire(space, dt, it + 1, *E, *grids, *El)
+                for k, v in zip(("Ex", "Ey", "Ez"), E):
+                    rec[k].append(v)
+                for k, v in zip(("Jx", "Jy", "Jz", "Rho"), grids):
+                    rec[k].append(v.copy())
+                rec["El"].append(np.array([a.copy() for a in El]))
+            for k, v in rec.items():
+                out["q_" + k] = np.array(v)
+            qa = rhoPJ._default_qw
+            out.update(nk=n, L=L, rx=rx, ry=ry, rz=rz, qx=qx, qy=qy, qz=qz, dt=dt,
+                       y0=qa.y0, z0=qa.z0, ay=qa.ay, az=qa.az, epsr=9.1,
+                       nw=qa.Nw, lam=lam, Emax0=3e7)
+            s.close()
+        finally:
+            os.chdir(cwd)
+    path = os.path.join(OUT_DIR, "ref_qw3d.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, "N =", nk)
 
@@ -365,12 +507,22 @@ def main():
         return
     if sys.argv[1:] == ["snap"]:
         run_case(16, seed=6, mb=MB_SHIPPED, nsteps=3, tag="n16_snap", jmax=2, files=True)
+        run_case(16, seed=9, mb=MB_IGNORE, nsteps=3, tag="n16_ignore", files=True)
+        return
+    if sys.argv[1:] == ["lorentz"]:
+        lorentz_case()
+        return
+    if sys.argv[1:] == ["qw3d"]:
+        qw3d_case()
         return
     run_case(16, seed=1, mb=MB_SHIPPED, nsteps=4, tag="n16_mb")
     run_case(16, seed=2, mb=MB_RHS_ONLY, nsteps=3, tag="n16_rhs")
     run_case(32, seed=3, mb=MB_SHIPPED, nsteps=3, tag="n32_mb")
     run_case(16, seed=6, mb=MB_SHIPPED, nsteps=3, tag="n16_snap", jmax=2, files=True)
+    run_case(16, seed=9, mb=MB_IGNORE, nsteps=3, tag="n16_ignore", files=True)
     xqw_case()
+    lorentz_case()
+    qw3d_case()
 
 
 if __name__ == "__main__":

@@ -1,6 +1,8 @@
 """The SBEs drop-in (reference call signatures, params files from the CWD,
-per-wire QWCalculator calls) against the reference's golden trajectories,
-and the backup text format round trip."""
+per-wire QWCalculator calls) against the reference's golden trajectories
+and the files the reference wrote (snapshots, PdotE, the text checkpoint of
+write_sbes_data; with ignorewire = 1, zeroed outputs but the PdotE energy of
+the real sources), and the backup text format round trip."""
 
 import os
 
@@ -51,13 +53,18 @@ def compare_files(z):
             continue
         assert [len(x) for x in got] == [len(x) for x in ref], name
         a, b = np.array(got), np.array(ref)
+        if "/backup/" in name:
+            # the checkpoint prints full precision: the levels themselves,
+            # which the trajectory tolerance bounds
+            assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b)), name
+            continue
         scale = np.max(np.abs(b), axis=0)
         if b.shape[1] == 3:  # z, Re, Im: the value columns share one scale
             scale[1:] = max(scale[1], scale[2])
         assert np.all(np.abs(a - b) <= 1e-5 * scale + 1e-6 * np.abs(b)), name
 
 
-@pytest.mark.parametrize("tag", ["n16_mb", "n32_mb", "n16_snap"])
+@pytest.mark.parametrize("tag", ["n16_mb", "n32_mb", "n16_snap", "n16_ignore"])
 def test_dropin_qwcalculator_matches_reference(tag, tmp_path, monkeypatch):
     z = load_golden(tag)
     write_params(str(tmp_path), z)
@@ -91,6 +98,11 @@ def test_dropin_qwcalculator_matches_reference(tag, tmp_path, monkeypatch):
     for col in (0, 1, 2, 3, 4, 5, 6, 12, 13, 14, 15):
         assert np.allclose(info[:, col], ref[:, col], rtol=1e-5, atol=1e-30), col
     if "file_names" in z:
+        names = [str(n) for n in z["file_names"]]
+        assert any("PdotE" in n for n in names)
+        if any("/backup/" in n for n in names):
+            s.write_sbes_data(7)
+        s.close()
         compare_files(z)
     s.close()
 

@@ -299,7 +299,7 @@ def test_step_vs_oracle_n64():
             assert rel(v, r) < TRAJ_TOL, it
 
 
-@pytest.mark.parametrize("N", [48, 64, 96, 192])
+@pytest.mark.parametrize("N", [48, 64, 96, 128, 192, 256])
 def test_transport_stage(N):
     """Transport alone (row and column FFT passes, radix 4/2/3 stages) on
     dense random p, C, D against fft2 -> phase -> ifft2."""
@@ -485,3 +485,107 @@ def test_transport_large_n(N):
         for k, v in X.items():
             ref = O.transport(t, v[w], prm.Edc, eac, dt)
             assert rel(wg.s[k][w].cpu().numpy(), ref) < 1e-13, (k, w)
+
+
+# ---------------------------------------------------------------------------
+# The benchmarked sizes: multi-tile RHS products (tile offsets m0, n0 > 0),
+# the compile-time N = 256 transport, the dephasing tables and the
+# many-body sums at N = 256, and the RHS at N = 512, against the oracle.
+# ---------------------------------------------------------------------------
+
+# The oracle's tables are the slow part at N = 512 (Vint with the crude K0,
+# about a minute), so one case per N serves every test below.
+@pytest.fixture(scope="module")
+def case256():
+    return Case(256, 2, seed=256)
+
+
+@pytest.fixture(scope="module")
+def case512():
+    return Case(512, 1, seed=512)
+
+
+def _rhs_check(c, precs):
+    import torch
+    from pulsesuite_amd.engine import WireGroup
+    Ex, Ey, Ez = c.oracle_fields()
+    ref = []
+    for w in range(c.nw):
+        C3, D3, p3 = c.state[3][:, :, w], c.state[4][:, :, w], c.state[5][:, :, w]
+        ref.append(c.os.rhs(p3, C3, D3, Ex, Ey, Ez, w + 1)[:3])
+    for prec, tol in precs:
+        print(f"[rhs] N={c.wg.N} nw={c.nw} prec={prec}", flush=True)
+        wg = c.wg if prec == "fp64" else WireGroup(c.wg.prm, c.nw, c.rr, prec=prec)
+        wg.set_state(*c.state)
+        wg.set_fields(c.Exx, c.Eyy, c.Ezz)
+        wg.rhs_eval()
+        torch.cuda.synchronize()
+        got = [wg.s[k].cpu().numpy() for k in ("pX", "CX", "DX")]
+        for w in range(c.nw):
+            for k in range(3):
+                e = rel(got[k][w], ref[w][k])
+                print(f"  wire {w} eq {k} rel err {e:.2e}", flush=True)
+                assert e < tol, (prec, w, k)
+        if wg is not c.wg:
+            del wg
+
+
+def test_rhs_vs_oracle_n256(case256):
+    """One RHS evaluation (dp, dC, dD with every mb.params term) of two
+    wires at N = 256 (4 x 4 output tiles per equation): FP64 within 1e-12
+    normwise of the oracle; FP32 and split-BF16 within 1e-5 (the reduced
+    modes' contractions, diagonals exact in FP64)."""
+    _rhs_check(case256, (("fp64", RHS_TOL), ("fp32", 1e-5), ("bf16x3", 1e-5)))
+
+
+def test_rhs_vs_oracle_n512(case512):
+    """The same at N = 512 (8 x 8 tiles), one wire."""
+    _rhs_check(case512, (("fp64", RHS_TOL), ("bf16x3", 1e-5)))
+
+
+def test_offdiag_tables_n256(case256):
+    """De, Dh (the FP64 register-Lorentzian kernel) against the oracle at
+    N = 256."""
+    import torch
+    c = case256
+    c.wg.set_state(*c.state)
+    c.wg.prep()
+    c.wg.offdiag()
+    torch.cuda.synchronize()
+    N = c.wg.N
+    dd = c.wg.s["dedh"].cpu().numpy()
+    g = c.og
+    for w in range(c.nw):
+        C3, D3 = c.state[3][:, :, w], c.state[4][:, :, w]
+        ne, nh = np.diag(C3).copy(), np.diag(D3).copy()
+        VC, _ = O.screened_arrays(c.og, c.ot, True, ne, nh)
+        De = O.offdiag_deph_e2(g, ne, nh, VC, g.gam_eh, g.prm.gam_h)
+        Dh = O.offdiag_deph_h2(g, ne, nh, VC, g.prm.gam_e, g.prm.gam_h)
+        assert rel(dd[w, 0], De[1:2 * N, :].T) < 1e-12
+        assert rel(dd[w, 1], Dh[1:2 * N, :].T) < 1e-12
+
+
+def test_step_vs_oracle_n256(case256):
+    """One full leapfrog step of two wires at N = 256 with every
+    mb.params term (multi-tile RHS, N = 256 transport with the fused
+    average, many-body and phonon relaxation, sources): outputs and every
+    level within 1e-9 of the oracle."""
+    import torch
+    c = case256
+    c.wg.set_state(*c.state)
+    c.wg.set_fields(c.Exx, c.Eyy, c.Ezz)
+    for dst, src in zip((c.os.CC2, c.os.DD2, c.os.YY2, c.os.CC3, c.os.DD3, c.os.YY3), c.state):
+        dst[:] = src
+    dt = 10e-18
+    print("[step] N=256 nw=2", flush=True)
+    c.wg.step(dt)
+    torch.cuda.synchronize()
+    Pxx, Pyy, Pzz, Rho = c.wg.outputs()
+    for w in range(c.nw):
+        ref = c.os.qw_calculator(c.Exx, c.Eyy, c.Ezz, dt, w + 1)
+        assert ref is not None
+        for got, r in zip((Pxx[w], Pyy[w], Pzz[w], Rho[w]), ref):
+            assert rel(got, r) < TRAJ_TOL, w
+    got = c.wg.get_state()
+    for v, r in zip(got, (c.os.CC2, c.os.DD2, c.os.YY2, c.os.CC3, c.os.DD3, c.os.YY3)):
+        assert rel(v, r) < TRAJ_TOL

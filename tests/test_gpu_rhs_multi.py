@@ -1,8 +1,10 @@
 """The persistent multi-group RHS (psa_rhs_multi through the C ABI): three
 wire groups of different N in one launch over a work queue give exactly the
-per-group psa_rhs results, in FP64 and split-BF16, for the raw RHS (mode 0)
-and the leapfrog write (mode 1); repeated launches (the queue resets
-itself) and a graph replay give the same bits."""
+per-group psa_rhs results, in FP64, FP32 and split-BF16, for the raw RHS
+(mode 0) and the leapfrog write (mode 1); repeated launches (the queue
+resets itself) and a graph replay give the same bits.  Uneven Nk
+{64, 128, 192, 256} stepped together in the reduced modes stay within
+1e-6 of FP64 in density and polarization."""
 
 import numpy as np
 import pytest
@@ -14,11 +16,11 @@ pytestmark = pytest.mark.gpu
 SIZES = ((64, 3), (96, 2), (128, 4))
 
 
-def _groups(prec, seed=3):
+def _groups(prec, seed=3, sizes=SIZES):
     from pulsesuite_amd.engine import WireGroup
     out = []
     rng = np.random.default_rng(seed)
-    for N, nw in SIZES:
+    for N, nw in sizes:
         prm, _ = params_pair(wire_length(N))
         rr = np.linspace(-1.2 * wire_length(N), 1.2 * wire_length(N), 96)
         wg = WireGroup(prm, nw, rr, prec=prec)
@@ -37,7 +39,7 @@ def _levels(gs):
     return [[g.s[k].cpu().numpy().copy() for k in ("pX", "CX", "DX")] for g in gs]
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_multi_rhs_equals_per_group(prec, mode):
     import torch
@@ -79,7 +81,7 @@ def test_multi_rhs_equals_per_group(prec, mode):
                 assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 def test_multi_step_equals_per_group_steps(prec):
     import torch
     from pulsesuite_amd.engine import MultiStep
@@ -100,3 +102,37 @@ def test_multi_step_equals_per_group_steps(prec):
             assert np.array_equal(x, y)
         for x, y in zip(ga.outputs(), gb.outputs()):
             assert np.array_equal(x, y)
+
+
+UNEVEN = ((64, 2), (128, 2), (192, 2), (256, 2))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_uneven_nk_reduced_precision_vs_fp64(prec):
+    """Wires of Nk 64, 128, 192, 256 stepped together (psa_step_multi, the
+    persistent RHS) for 20 leapfrog steps in a reduced mode against the
+    same groups in FP64: the densities (diagonals of C, D), the coherences
+    p and the polarization Pxx stay within 1e-6 relative per group."""
+    import torch
+    from pulsesuite_amd.engine import MultiStep
+    print(f"[uneven] prec={prec}", flush=True)
+    dt = 10e-18
+    ref = _groups("fp64", seed=9, sizes=UNEVEN)
+    red = _groups(prec, seed=9, sizes=UNEVEN)
+    a, b = MultiStep(ref, dt), MultiStep(red, dt)
+    for _ in range(20):
+        a()
+        b()
+    torch.cuda.synchronize()
+    for ga, gb in zip(ref, red):
+        Sa, Sb = ga.get_state(), gb.get_state()
+        for k, nm in ((3, "C3"), (4, "D3")):
+            na = np.einsum("iiw->iw", Sa[k])
+            nb = np.einsum("iiw->iw", Sb[k])
+            e = float(np.max(np.abs(nb - na)) / np.max(np.abs(na)))
+            print(f"  N={ga.N} {nm} density err {e:.2e}", flush=True)
+            assert e < 1e-6
+        e = float(np.max(np.abs(Sb[5] - Sa[5])) / np.max(np.abs(Sa[5])))
+        ep = float(np.max(np.abs(gb.outputs()[0] - ga.outputs()[0])) / np.max(np.abs(ga.outputs()[0])))
+        print(f"  N={ga.N} p err {e:.2e} Pxx err {ep:.2e}", flush=True)
+        assert e < 1e-6 and ep < 1e-6

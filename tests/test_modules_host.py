@@ -89,6 +89,28 @@ def test_oracle_lorentz_tables_support_and_zero_mismatch():
         assert np.allclose(L[at_zero] / G[at_zero], 1 / np.sqrt(np.pi), rtol=1e-12)
 
 
+def test_oracle_lorentz_tables_match_reference():
+    """CalcMBArrays with LorentzDelta = 1 (coulomb.py:405-446), as the
+    reference's SBESolver builds it from mb.params line 15, and MBCE/MBCH on
+    those tables (tests/golden/ref_n16_lorentz.npz)."""
+    from tests.helpers import load_golden
+    z = load_golden("n16_lorentz")
+    prm = O.Params(L=float(z["L"]), me_rel=0.067, flags=O.SHIPPED_FLAGS)
+    g = O.Grid(prm)
+    assert rel(g.Ee, z["Ee"]) == 0 and rel(g.Eh, z["Eh"]) == 0
+    S = {k: O.mb_tables(g, k, prm.gam_e, prm.gam_h, lorentz=True) for k in ("eh", "ee", "hh")}
+    for k, nm in (("eh", "Ceh"), ("ee", "Cee"), ("hh", "Chh")):
+        assert rel(S[k], z[nm]) < 1e-15, nm
+        # a wrong dependence on the energy mismatch would not survive this:
+        # the Lorentzian is nowhere zero where the index conditions hold
+        assert np.count_nonzero(z[nm]) > 0.5 * np.count_nonzero(O.mb_tables(g, k, 1e15, 1e15))
+    ne, nh = np.real(np.diag(z["fC"])), np.real(np.diag(z["fD"]))
+    a, b = O.mbce(g, S["eh"], S["ee"], ne, nh, z["fVC"])
+    assert rel(a, z["fWinE"]) < 1e-14 and rel(b, z["fWoutE"]) < 1e-14
+    a, b = O.mbch(g, S["eh"], S["hh"], ne, nh, z["fVC"])
+    assert rel(a, z["fWinH"]) < 1e-14 and rel(b, z["fWoutH"]) < 1e-14
+
+
 def test_qwoptics_host_helpers():
     from pulsesuite_amd import qwoptics
     assert [qwoptics.yw(w) for w in range(1, 9)] == [O.yw(w) for w in range(1, 9)] == [1, 1, -1, -1, 1, 1, -1, -1]

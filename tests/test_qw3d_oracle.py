@@ -1,12 +1,16 @@
-"""The 3-D coupling oracle (oracle/qw3d_oracle.py) pinned on the CPU by
-closed forms: trilinear interpolation, the discrete continuity equation of
-CalcJfx, the placement's separability, and Gauss's law / curl-free
-longitudinal fields of ElongfromRho."""
+"""The 3-D coupling oracle (oracle/qw3d_oracle.py) pinned on the CPU by the
+reference's own QuantumWire (tests/golden/ref_qw3d.npz: four wires, three
+calls, non-zero incoming grids) and by closed forms: trilinear
+interpolation, the discrete continuity equation of CalcJfx, the placement
+(which overwrites the grid), and Gauss's law / curl-free longitudinal fields
+of ElongfromRho."""
 
 import numpy as np
 from scipy.constants import epsilon_0
 
 from oracle import qw3d_oracle as Q
+from oracle import sbe_oracle as O
+from tests.helpers import load_golden, rel
 
 
 def _grid(nx=16, ny=12, nz=10, d=2e-9):
@@ -41,15 +45,67 @@ def test_calc_jfx_continuity():
     assert np.max(np.abs(lhs - rhs)) < 1e-9 * np.max(np.abs(rhs))
 
 
-def test_placement_is_separable():
+def test_placement_overwrites_and_is_separable():
+    """QWPlacement zeroes the grid (rhoPJ.py:682): whatever the grid held,
+    it ends as f gy gz, and placing twice gives the same grid."""
     rng = np.random.default_rng(2)
     nx, ny, nz = 8, 6, 5
     f = rng.standard_normal(nx) + 1j * rng.standard_normal(nx)
     gy, gz = rng.uniform(0, 1, ny), rng.uniform(0, 1, nz)
-    F = np.ones((nx, ny, nz), dtype=complex)
+    F = 1e3 * (rng.standard_normal((nx, ny, nz)) + 1j)
     Q.qw_placement(f, np.ones(nx), gy, gz, F)
-    assert np.allclose(F.sum(axis=(1, 2)) - ny * nz, f * gy.sum() * gz.sum(), rtol=1e-13)
-    assert np.allclose(F[:, 2, 3] - 1, f * gy[2] * gz[3], rtol=1e-14)
+    assert np.allclose(F.sum(axis=(1, 2)), f * gy.sum() * gz.sum(), rtol=1e-13)
+    assert np.allclose(F[:, 2, 3], f * gy[2] * gz[3], rtol=1e-14)
+    F2 = F.copy()
+    Q.qw_placement(f, np.ones(nx), gy, gz, F2)
+    assert np.array_equal(F, F2)
+
+
+def qw3d_from_golden(z, keep_old=False):
+    """The oracle solver and coupling set up as the reference run was:
+    InitializeSBE on the grid's x axis, seeded levels for every wire."""
+    prm = O.Params(L=float(z["L"]), me_rel=0.067, flags=O.SHIPPED_FLAGS)
+    g = O.Grid(prm)
+    nw = int(z["nw"])
+    sv = O.Solver(prm, nw, z["rx"], float(z["Emax0"]), tables=O.Tables(g), grid=g)
+    for nm, dst in (("C2", sv.CC2), ("D2", sv.DD2), ("p2", sv.YY2),
+                    ("C3", sv.CC3), ("D3", sv.DD3), ("p3", sv.YY3)):
+        dst[:] = z["s0_" + nm]
+    oq = Q.QuantumWire3D(sv, z["rx"], z["ry"], z["rz"], z["qx"], z["qy"], z["qz"],
+                         float(z["y0"]), float(z["z0"]), float(z["ay"]), float(z["az"]),
+                         float(z["epsr"]), keep_old=keep_old)
+    return sv, oq
+
+
+def test_quantum_wire_matches_reference():
+    """Three QuantumWire calls (rhoPJ.py:1180-1240 -> QWArray.QuantumWire)
+    from random non-zero Jx, Jy, Jz, Rho grids: the placed grids are
+    overwritten and J = sum P / (dt Nw) (the previous P is discarded), so
+    the grids and the ten longitudinal outputs match the reference."""
+    z = load_golden("qw3d")
+    sv, oq = qw3d_from_golden(z)
+    grids = [z[k].copy() for k in ("J0x", "J0y", "J0z", "Rho0")]
+    dt = float(z["dt"])
+    for it in range(z["q_Jx"].shape[0]):
+        res = oq.step(z["q_Ex"][it], z["q_Ey"][it], z["q_Ez"][it], dt, *grids)
+        for a, k in zip(grids, ("Jx", "Jy", "Jz", "Rho")):
+            assert rel(a, z["q_" + k][it]) < 1e-13, (it, k)
+        els = list(res[0]) + list(res[1]) + list(res[2]) + [res[3]]
+        for c in range(10):
+            assert rel(els[c], z["q_El"][it][c]) < 1e-13, (it, c)
+
+
+def test_keep_old_differs_from_the_reference():
+    """keep_old (not the reference) makes J the polarization current: from
+    the second call on, J differs from the reference's sum P / dt."""
+    z = load_golden("qw3d")
+    sv, oq = qw3d_from_golden(z, keep_old=True)
+    grids = [z[k].copy() for k in ("J0x", "J0y", "J0z", "Rho0")]
+    dt = float(z["dt"])
+    oq.step(z["q_Ex"][0], z["q_Ey"][0], z["q_Ez"][0], dt, *grids)
+    assert rel(grids[0], z["q_Jx"][0]) < 1e-12
+    oq.step(z["q_Ex"][1], z["q_Ey"][1], z["q_Ez"][1], dt, *grids)
+    assert rel(grids[1], z["q_Jy"][1]) > 1e-3
 
 
 def test_elong_gauss_law_and_curl_free():
