@@ -256,6 +256,21 @@ int psa_record(const psa_group *g, int32_t w0, int32_t nw, int32_t xmid, double 
  * relaxation, transport, finish, sources, qw2prop. */
 int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream);
 
+/* psa_step_noalloc: NO COUNTERPART IN THE REFERENCE.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): psa_step with the
+ * split-K partial sums of its Prop2QW, source and QW2Prop products in a
+ * device array owned by the library (PSA_NOALLOC_SCRATCH_DOUBLES doubles,
+ * 32 MiB, held on every GPU the library runs on until the process ends)
+ * instead of g->kscr, which it ignores.  Calls no hipMalloc / hipFree.
+ * Results equal psa_step's with a full-size kscr (the same splits).
+ * psa_step_noalloc_scratch gives the doubles a call with nw wires needs
+ * (16 max(Nr 4 nw, (2N-1) 3 nw, bnx 5 nw)); larger calls are refused with
+ * PSA_E_INVALID.  The array is shared: calls that overlap on one GPU race. */
+#define PSA_NOALLOC_SCRATCH_DOUBLES (4L << 20)
+int64_t psa_step_noalloc_scratch(const psa_group *g, int32_t nw);
+int psa_step_noalloc(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream);
+
 /* The same step with classic RK4 (the stage combination of _rk4,
  * integrator.py:143-157) in place of the leapfrog + average.  The reference
  * has no RK4 on the wire path; it integrates with the leapfrog of psa_step.

@@ -71,6 +71,7 @@ PSA_PSTD_ABC = 1
 PSA_PSTD_MAXW = 256
 PSA_RHS_MULTI_MAX = 16
 PSA_RHS_MULTI_QUEUE = 16
+PSA_NOALLOC_SCRATCH_DOUBLES = 4 << 20
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -93,6 +94,8 @@ SIGNATURES = {
     "psa_prop2qw": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_qw2prop": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_step": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
+    "psa_step_noalloc": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
+    "psa_step_noalloc_scratch": (ctypes.c_int64, [_GP, c_i32]),
     "psa_step_rk4": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),
     "psa_drive": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr, c_dbl, c_ptr]),
     "psa_record": (ctypes.c_int, [_GP, c_i32, c_i32, c_i32, c_ptr, c_ptr, c_i32, c_ptr]),

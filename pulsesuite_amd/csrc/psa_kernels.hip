@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 #include <string>
@@ -2302,6 +2303,72 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
   if ((rc = psa_sources(g, w0, nw, stream))) return rc;
   if ((rc = psa_qw2prop(g, w0, nw, stream))) return rc;
   return PSA_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Allocation-free step.  psa_step's split-K products (Prop2QW, the sources,
+// QW2Prop) keep their partial sums in a workspace, normally the caller's
+// psa_group.kscr.  psa_step_noalloc puts them in `scratch`, a file-scope
+// device array of PSA_NOALLOC_SCRATCH_DOUBLES doubles that every GPU the
+// library runs on holds until the process ends.  Its device address is
+// taken with hipGetSymbolAddress on the first call on each device (that
+// device current) and kept per device; host code uses only that address.
+// Nothing in it needs to start at zero: every split writes its whole
+// partial block before the fixed-order reduce reads it.  Calls that overlap
+// on one GPU share the array and race.
+// ===========================================================================
+namespace {
+
+static __device__ __align__(256) double scratch[PSA_NOALLOC_SCRATCH_DOUBLES];
+constexpr int kScratchDevices = 64;
+double *g_scratch_ptr[kScratchDevices];
+std::mutex g_scratch_mu;
+
+int scratch_address(double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices)
+    return fail(PSA_E_HIP, "psa_step_noalloc: no current device below index 64");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (int rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "psa_step_noalloc: hipGetSymbolAddress"))
+      return rc;
+    if (!p) return fail(PSA_E_HIP, "psa_step_noalloc: hipGetSymbolAddress gave a null address");
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PSA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t psa_step_noalloc_scratch(const psa_group *g, int32_t nw) {
+  if (!g || nw < 1) return -1;
+  const int64_t Nr = 2L * g->N + 2, Q = 2L * g->N - 1, bnx = std::max(g->bnx, 1);
+  return 16L * std::max({Nr * 4 * nw, Q * 3 * nw, bnx * 5 * nw});
+}
+
+int psa_step_noalloc(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  const int64_t need = psa_step_noalloc_scratch(g, nw);
+  if (need > PSA_NOALLOC_SCRATCH_DOUBLES) {
+    static thread_local char msg[200];
+    std::snprintf(msg, sizeof(msg),
+                  "psa_step_noalloc: %d wires of N=%d with bnx=%d need %lld doubles of split-K scratch, "
+                  "above the library's maximum of %lld",
+                  nw, g->N, g->bnx, (long long)need, (long long)PSA_NOALLOC_SCRATCH_DOUBLES);
+    return fail(PSA_E_INVALID, msg);
+  }
+  double *scr = nullptr;
+  if (int rc = scratch_address(&scr)) return rc;
+  psa_group gg = *g;
+  gg.kscr = scr;
+  gg.kscr_len = PSA_NOALLOC_SCRATCH_DOUBLES;
+  return psa_step(&gg, w0, nw, dt, stream);
 }
 
 int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *queue, double dt, void *stream) {

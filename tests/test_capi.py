@@ -103,3 +103,21 @@ def test_multi_rhs_packing_checks_without_the_gpu():
     zero = ctypes.create_string_buffer(len(buf.raw))
     q = (ctypes.c_uint32 * 2)()
     assert lib.psa_rhs_multi(cp(zero), cp(zero), cp(q), None) == -1
+
+
+def test_noalloc_step_refuses_calls_beyond_its_scratch_without_touching_the_gpu():
+    """psa_step_noalloc states the split-K scratch a call needs and refuses
+    one above the library's maximum before any HIP call."""
+    import ctypes
+    lib = capi.load()
+    g = capi.PsaGroup()
+    g.N, g.Nr, g.Nw, g.Nx, g.prec, g.bnx = 256, 514, 64, 4096, 0, 600
+    gp = ctypes.byref(g)
+    need = lib.psa_step_noalloc_scratch(gp, 64)
+    assert need == 16 * max(514 * 4 * 64, 511 * 3 * 64, 600 * 5 * 64)
+    assert need <= capi.PSA_NOALLOC_SCRATCH_DOUBLES
+    g.bnx, g.Nw = 4096, 200
+    assert lib.psa_step_noalloc_scratch(gp, 200) > capi.PSA_NOALLOC_SCRATCH_DOUBLES
+    assert lib.psa_step_noalloc(gp, 0, 200, 1e-17, None) == -1
+    assert b"above the library's maximum" in lib.psa_last_error()
+    assert lib.psa_step_noalloc(gp, 5, 200, 1e-17, None) == -1  # wire range outside the group
