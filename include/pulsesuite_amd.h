@@ -65,7 +65,7 @@ typedef struct psa_group {
   int32_t gx0, gnx; /* support of the Prop2QW transform in rr: [gx0, gx0+gnx) */
   int32_t bx0, bnx; /* rows of rr inside the wire for QW2Prop: [bx0, bx0+bnx) */
   int32_t nnz_mb[8];/* entries in each many-body scattering list */
-  int32_t tune[4];  /* launch choices, 0 = default: [0] split-BF16 RHS tile (0: 64, 1: 128),
+  int32_t tune[4];  /* launch choices, 0 = default: [0] reserved (0),
                        [1] transport lines per block, row pass (0: 4, else 2, 4 or 8),
                        [2] GEMM block order (0: batches grouped per XCD, 1: linear),
                        [3] transport lines per block, column pass (0: 4, else 2, 4 or 8;
@@ -121,10 +121,11 @@ typedef struct psa_group {
   double *tphase;                     /* [Nw][N] complex Transport phase exp(-i Y dk) */
   double *part;                       /* [Nw][256] scratch (reserved) */
   void *spl;                          /* bf16 split operands, PSA_PREC_BF16X3 only, zero-initialised:
-                                         [Nw][12 slots][NR/64][NK/32][4 planes][64][32] with
+                                         [Nw][14 slots][NR/64][NK/32][4 planes][64][32] with
                                          NR = 128 ceil(N/128), NK = 32 ceil(N/32); slot = 2 matrix +
-                                         transposed (p, C_o, D_o, Heh, Hee_o, Hhh_o), planes
-                                         re_hi, re_lo, im_hi, im_lo */
+                                         transposed (p, C_o, D_o, Heh, Hee_o, Hhh_o), then the
+                                         dephasing bands Dh, De; planes re_hi, re_lo, im_hi, im_lo;
+                                         the 8-k groups of a row XOR-swizzled by (row >> 2) & 3 */
   double *kscr;                       /* optional scratch for split-K partial sums of the source and
                                          propagation-grid products (NULL: no split); kscr_len doubles,
                                          8 x 2 x max(Nr x 4 Nw, (2N-1) x 3 Nw, bnx x 5 Nw) uses all splits */

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libpulsesuite_amd.so")
 SOURCES = ["psa_kernels.hip"]
-HEADERS = ["psa_dev.h", "psa_gemm.h", "psa_rhs_bf16.h", "psa_offdiag.h", "psa_calch.h", "psa_pstd.h",
+HEADERS = ["psa_dev.h", "psa_gemm.h", "psa_rhs_split.h", "psa_offdiag.h", "psa_calch.h", "psa_pstd.h",
            "psa_modules.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -409,55 +409,86 @@ struct RhsProb {
     }
   }
 
-  __device__ void store(int b, int row, int col, double re, double im) const {
+  // The per-element FP64 operands of the epilogue (loaded apart from the
+  // arithmetic, so that a caller can have many elements' loads in flight):
+  // x = the checkout level X (p, C or D) at the element, h = Heh[ke][kh]
+  // (dp) or H_o[k2][k1] (dC, dD; 0 on the diagonal), x1 = the level the
+  // leapfrog / RK4 stage adds to.
+  struct Pre {
+    cd x, h, x1;
+  };
+
+  __device__ Pre preload(int b, int row, int col) const {
+    const int N = g.N, e = b % 3, w = w0 + b / 3;
+    const long o = mat_off(g, w), ij = (long)row * N + col;
+    Pre q;
+    if (e == 0) {
+      q.x = ld(g.pB + o, ij);
+      q.h = ld(g.Heh + o, ij);  // Heh[ke][kh], stored transposed
+    } else {
+      q.x = ld((e == 1 ? g.CB : g.DB) + o, ij);
+      q.h = (row == col) ? cd{0.0, 0.0} : ld((e == 1 ? g.Heo : g.Hho) + o, ij);  // H_o[k2][k1]
+    }
+    q.x1 = mode == 0 ? cd{0.0, 0.0} : ld((e == 0 ? g.pA : (e == 1 ? g.CA : g.DA)) + o, ij);
+    return q;
+  }
+
+  // The diagonal quantities the epilogue needs at one index k, as a column
+  // (ke for dp, k2 for dC, dD) or a row (kh, k1): h = the band energy of
+  // that index (hdiag), n = the occupation, g = the diagonal dephasing
+  // rate (dp only).
+  struct Side {
+    cd h, n;
+    double g;
+  };
+
+  __device__ Side side(int e, int w, int k, bool col) const {
     const int N = g.N;
-    if (row >= N || col >= N) return;
+    const double *hd = g.hdiag + (long)w * 4 * N;  // [2][N] complex
+    const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+    if (e == 0)
+      return col ? Side{ld(hd, k), ld(pr + PSA_PREP_NE, k), pr[PSA_PREP_GAME(N) + k]}
+                 : Side{ld(hd, N + k), ld(pr + PSA_PREP_NH(N), k), pr[PSA_PREP_GAMH(N) + k]};
+    return Side{ld(hd, (e == 1 ? 0 : N) + k), ld(pr + (e == 1 ? PSA_PREP_NE : PSA_PREP_NH(N)), k), 0.0};
+  }
+
+  // dX/dt at one element from the contraction sum acc = (re, im):
+  // dp: row = kh, col = ke; the diagonal parts (he[ke] + hh[kh]) p, the
+  // Pauli factor Heh[ke,kh] (1 - ne[ke] - nh[kh]) from -Heh^T C - (Heh D)^T,
+  // and the dephasing -(GamE[ke] + GamH[kh]) p.  dC, dD:
+  // (h[k2] - h[k1]) X[k1,k2] + (n[k1] - n[k2]) H_o[k2,k1].
+  __device__ cd deriv(int e, double re, double im, const Pre &q, const Side &c, const Side &r) const {
+    const double ihb = 1.0 / g.hbar;  // z / (i hbar) = -i z / hbar
+    const cd acc = {re, im};
+    if (e == 0) {
+      const cd pauli = {1.0 - c.n.re - r.n.re, -c.n.im - r.n.im};
+      cd z = cadd(acc, cmul(cadd(c.h, r.h), q.x));
+      z = cadd(z, cmul(q.h, pauli));
+      const cd d = {z.im * ihb, -z.re * ihb};
+      return csub(d, cscale(q.x, c.g + r.g));
+    }
+    cd z = cadd(acc, cmul(csub(c.h, r.h), q.x));
+    z = cadd(z, cmul(csub(r.n, c.n), q.h));
+    return cd{z.im * ihb, -z.re * ihb};
+  }
+
+  __device__ void finish(int b, int row, int col, double re, double im, const Pre &q) const {
+    const int N = g.N;
     const int e = b % 3, w = w0 + b / 3;
     const long o = mat_off(g, w);
     const long ij = (long)row * N + col;
-    const double ihb = 1.0 / g.hbar;  // z / (i hbar) = -i z / hbar
-    cd acc = {re, im};
-    cd d;
-    const double *hd = g.hdiag + (long)w * 4 * N;  // [2][N] complex
-    const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-    if (e == 0) {
-      // row = kh, col = ke.  Diagonal parts: (he[ke] + hh[kh]) p and the
-      // Pauli factor Heh[ke,kh] (1 - ne[ke] - nh[kh]) from -Heh^T C - (Heh D)^T.
-      const cd p = ld(g.pB + o, ij);
-      const cd he = ld(hd, col), hh = ld(hd, N + row);
-      const cd heh = ld(g.Heh + o, ij);  // Heh[ke][kh], stored transposed
-      const cd ne = ld(pr + PSA_PREP_NE, col), nh = ld(pr + PSA_PREP_NH(N), row);
-      const cd pauli = {1.0 - ne.re - nh.re, -ne.im - nh.im};
-      cd z = cadd(acc, cmul(cadd(he, hh), p));
-      z = cadd(z, cmul(heh, pauli));
-      d = cd{z.im * ihb, -z.re * ihb};
-      const double gam = pr[PSA_PREP_GAME(N) + col] + pr[PSA_PREP_GAMH(N) + row];
-      d = csub(d, cscale(p, gam));
-    } else {
-      // (h[k2] - h[k1]) X[k1,k2] + (n[k1] - n[k2]) H_o[k2,k1]
-      const double *X = (e == 1 ? g.CB : g.DB) + o;
-      const double *Ho = (e == 1 ? g.Heo : g.Hho) + o;
-      const cd x = ld(X, ij);
-      const int sl = (e == 1) ? 0 : N;
-      const cd h2 = ld(hd, sl + col), h1 = ld(hd, sl + row);
-      const double *nn = pr + (e == 1 ? PSA_PREP_NE : PSA_PREP_NH(N));
-      const cd n1 = ld(nn, row), n2 = ld(nn, col);
-      const cd hot = (row == col) ? cd{0.0, 0.0} : ld(Ho, ij);  // H_o[k2][k1], stored transposed
-      cd z = cadd(acc, cmul(csub(h2, h1), x));
-      z = cadd(z, cmul(csub(n1, n2), hot));
-      d = cd{z.im * ihb, -z.re * ihb};
-    }
+    const cd d = deriv(e, re, im, q, side(e, w, col, true), side(e, w, row, false));
     double *out = (e == 0 ? g.pX : (e == 1 ? g.CX : g.DX)) + o;
     if (mode == 0) {
       st(out, ij, d);
     } else if (mode == 1) {
-      const double *X1 = (e == 0 ? g.pA : (e == 1 ? g.CA : g.DA)) + o;
-      const cd x1 = ld(X1, ij);
+      const cd x1 = q.x1;
       st(out, ij, cd{x1.re + d.re * dt * 2.0, x1.im + d.im * dt * 2.0});
     } else {
-      // RK4 stage mode - 1 (_rk4, integrator.py:143-157): A = step start,
-      // R = k1 + 2 k2 + 2 k3 so far; out may alias A (same element only)
-      const cd x0 = ld((e == 0 ? g.pA : (e == 1 ? g.CA : g.DA)) + o, ij);
+      // RK4 stage mode - 1 (_rk4, integrator.py:143-157): x1 = step start,
+      // R = k1 + 2 k2 + 2 k3 so far; out may alias the step start (same
+      // element only)
+      const cd x0 = q.x1;
       double *R = (e == 0 ? g.pR : (e == 1 ? g.CR : g.DR)) + o;
       if (mode == 5) {
         const cd r = ld(R, ij);
@@ -474,6 +505,11 @@ struct RhsProb {
         st(out, ij, cd{x0.re + c * d.re, x0.im + c * d.im});
       }
     }
+  }
+
+  __device__ void store(int b, int row, int col, double re, double im) const {
+    if (row >= g.N || col >= g.N) return;
+    finish(b, row, col, re, im, preload(b, row, col));
   }
 };
 

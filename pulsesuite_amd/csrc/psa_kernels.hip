@@ -11,7 +11,7 @@
 #include <string>
 
 #include "psa_gemm.h"
-#include "psa_rhs_bf16.h"
+#include "psa_rhs_split.h"
 #include "psa_offdiag.h"
 #include "psa_calch.h"
 #include "psa_pstd.h"
@@ -1696,31 +1696,36 @@ int launch_gemm_splitk(const P &p, int M, int ncol, int K, hipStream_t s, const 
   return PSA_OK;
 }
 
-template <class K, size_t SHM, int TILE>
-int launch_rhs_bf16(const RhsProb &p, int nw, hipStream_t s, K kern) {
-  static bool attr_set = false;  // per instantiation
-  if (!attr_set) {
-    int rc = hip_check(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)SHM),
-                       "hipFuncSetAttribute(k_rhs bf16)");
-    if (rc) return rc;
-    attr_set = true;
+// The split-bf16 operand planes of wires [w0, w0 + nw) (sp::k_split, and
+// sp::k_split_band for the off-diagonal dephasing bands).
+int split_planes(const psa_group &g, int w0, int nw, hipStream_t s) {
+  if (!g.spl) return fail(PSA_E_INVALID, "bf16x3 mode needs the spl operand buffer");
+  const int nt = (g.N + 63) / 64;
+  hipLaunchKernelGGL(sp::k_split, dim3(nt, nt, 6 * nw), dim3(256), 0, s, g, w0);
+  PSA_LAUNCH_CHECK("sp::k_split");
+  if (g.flags & PSA_F_OFFDIAGDPH) {
+    hipLaunchKernelGGL(sp::k_split_band, dim3(nt, 2, nw), dim3(256), 0, s, g, w0);
+    PSA_LAUNCH_CHECK("sp::k_split_band");
   }
-  const int nb = (p.g.N + TILE - 1) / TILE;
-  hipLaunchKernelGGL(kern, dim3(nb, nb, 3 * nw), dim3(256), SHM, s, p);
-  PSA_LAUNCH_CHECK("k_rhs bf16");
   return PSA_OK;
 }
 
-// Split-bf16 RHS: operand planes once per step, then the bf16 MFMA products
-// (64x64 or 128x128 tiles, tune[0]).
+// Split-bf16 RHS: operand planes once per step, then the bf16 MFMA
+// products over 128 x 128 tiles (sp::k_rhs).
 int rhs_bf16(const RhsProb &p, int nw, hipStream_t s) {
-  const psa_group *g = &p.g;
-  if (!g->spl) return fail(PSA_E_INVALID, "bf16x3 mode needs the spl operand buffer");
-  const int nt = (g->N + 63) / 64;
-  hipLaunchKernelGGL(k_split_bf16, dim3(nt, nt, 6 * nw), dim3(256), 0, s, *g, p.w0);
-  PSA_LAUNCH_CHECK("k_split_bf16");
-  if (g->tune[0] == 1) return launch_rhs_bf16<decltype(&t128::k_rhs), t128::lds_bytes(), t128::TILE>(p, nw, s, &t128::k_rhs);
-  return launch_rhs_bf16<decltype(&t64::k_rhs), t64::lds_bytes(), t64::TILE>(p, nw, s, &t64::k_rhs);
+  static bool attr_set = false;
+  if (!attr_set) {
+    int rc = hip_check(hipFuncSetAttribute((const void *)sp::k_rhs, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)sp::lds_bytes()),
+                       "hipFuncSetAttribute(sp::k_rhs)");
+    if (rc) return rc;
+    attr_set = true;
+  }
+  if (int rc = split_planes(p.g, p.w0, nw, s)) return rc;
+  const int nb = (p.g.N + sp::TILE - 1) / sp::TILE;
+  hipLaunchKernelGGL(sp::k_rhs, dim3(nb, nb, 3 * nw), dim3(256), sp::lds_bytes(), s, p);
+  PSA_LAUNCH_CHECK("sp::k_rhs");
+  return PSA_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1758,7 +1763,13 @@ struct RhsQueue {
 
 template <int KIND>
 constexpr size_t queue_tile_bytes() {
-  return KIND == 2 ? t64::lds_bytes() : (KIND == 0 ? gemm_lds_bytes<double>() : gemm_lds_bytes<float>());
+  return KIND == 2 ? sp::lds_bytes() : (KIND == 0 ? gemm_lds_bytes<double>() : gemm_lds_bytes<float>());
+}
+
+// output tile edge of the work items of each kind
+template <int KIND>
+constexpr int queue_tile() {
+  return KIND == 2 ? sp::TILE : GBM;
 }
 
 template <int KIND>  // 0: FP64 products, 1: FP32, 2: split-BF16
@@ -1787,9 +1798,9 @@ __global__ __launch_bounds__(256) void k_rhs_queue(const RhsProb *__restrict__ p
       const WireEnt e = ents[lo];
       const int gx = q.gx[e.gi], tpb = gx * gx, local = it - e.off;
       const int b = 3 * e.j + local / tpb, lin = local % tpb;
-      const int m0 = (lin / gx) * GBM, n0 = (lin % gx) * GBN;
+      const int m0 = (lin / gx) * queue_tile<KIND>(), n0 = (lin % gx) * queue_tile<KIND>();
       if constexpr (KIND == 2)
-        t64::rhs_tile(probs[e.gi], b, m0, n0, lds_raw);
+        sp::rhs_tile(probs[e.gi], b, m0, n0, lds_raw);
       else
         gemm_tile<typename std::conditional<KIND == 0, double, float>::type, RhsProb>(probs[e.gi], b, m0, n0,
                                                                                       lds_raw);
@@ -2001,7 +2012,8 @@ int psa_rhs_multi_pack(const psa_group *const *groups, const int32_t *w0, const 
     p.mode = mode;
     p.dt = dt;
     std::memcpy(&probs[j], &p, sizeof(p));
-    h.gx[j] = (groups[i]->N + GBM - 1) / GBM;
+    const int tile = h.kind == 2 ? sp::TILE : GBM;
+    h.gx[j] = (groups[i]->N + tile - 1) / tile;
     h.nw[j] = nw[i];
     for (int k = 0; k < nw[i]; ++k, ++ord) {
       const int x = ord % NXCD;
@@ -2030,14 +2042,12 @@ int psa_rhs_multi(const void *packed_host, const void *packed_dev, uint32_t *que
     return fail(PSA_E_INVALID, "psa_rhs_multi: not a packed description");
   const hipStream_t s = (hipStream_t)stream;
   const char *bh = static_cast<const char *>(packed_host), *bd = static_cast<const char *>(packed_dev);
-  const RhsProb *probs_h = reinterpret_cast<const RhsProb *>(bh + multi_hdr_bytes());
   if (h.kind == 2) {
+    const RhsProb *probs_h = reinterpret_cast<const RhsProb *>(bh + multi_hdr_bytes());
     for (int j = 0; j < h.n; ++j) {
       RhsProb p;
       std::memcpy(&p, &probs_h[j], sizeof(p));
-      const int nt = (p.g.N + 63) / 64;
-      hipLaunchKernelGGL(k_split_bf16, dim3(nt, nt, 6 * h.nw[j]), dim3(256), 0, s, p.g, p.w0);
-      PSA_LAUNCH_CHECK("k_split_bf16");
+      if (int rc = split_planes(p.g, p.w0, h.nw[j], s)) return rc;
     }
   }
   RhsQueue q;

@@ -1,0 +1,451 @@
+// Split-BF16 right-hand sides (PSA_PREC_BF16X3) for gfx950.
+//
+// Every FP64 operand x of the RHS contractions (RhsProb, SBEs.py:1941-1965,
+// 2308-2327, 2606-2625) is written once per step as two bf16 values,
+//   f = float(x), hi = bf16(f), lo = bf16(f - float(hi))
+// (v_cvt_pk_bf16_f32, round to nearest even; |x - hi - lo| <= ~2^-17 |x|),
+// for the real and the imaginary part: four bf16 planes per operand slot
+// (k_split).  A real product a b is a_hi b_hi + a_hi b_lo + a_lo b_hi on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation (the lo*lo term, ~2^-16
+// relative, is dropped); a complex product is four real ones, 12 MFMAs per
+// 16 x 16 x 32 block.  The diagonal parts, the epilogue and the state stay
+// in FP64 (RhsProb::store).
+//
+// Operand slots per wire: slot = 2 * matrix + orientation for the matrices
+// 0 p, 1 C_o, 2 D_o (diagonal zeroed), 3 Heh, 4 Hee_o, 5 Hhh_o
+// (orientation 0 the matrix, 1 its transpose); slot 12 the band
+// i hbar Dh[m - x][x] and slot 13 i hbar De[m - x][x] of the off-diagonal
+// dephasing (rows x, k = m; purely imaginary, real planes zero).  An A operand A(m, k)
+// reads slot rows m, a B operand B(k, n) slot rows n, both k-contiguous
+// as the MFMA lanes take them (A[l%16][8(l/16)+e], B[8(l/16)+e][l%16]).
+//
+// Layout: a slot is padded to NR = 128 ceil(N/128) rows and NK = 32
+// ceil(N/32) k (zeros) and stored as [row block of 64][k block of 32]
+// chunks of 16 KB = [4 planes][64 rows][32 k], with the 16-byte group of 8
+// k in a row at position (k >> 3) ^ ((row >> 1) & 3): the layout the RHS
+// kernel keeps in LDS, so a chunk is copied there byte for byte (direct
+// global-to-LDS loads), and the MFMA operand reads (8 lanes = 8
+// consecutive rows per 128-byte bank row) fall on distinct LDS banks.
+#pragma once
+
+#include "psa_gemm.h"
+
+namespace psa {
+namespace sp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TILE = 128, BK = 32, NSLOT = 14;
+
+__host__ __device__ inline int nr(int N) { return (N + 127) / 128 * 128; }
+__host__ __device__ inline int nk(int N) { return (N + 31) / 32 * 32; }
+__host__ __device__ inline long slot_elems(int N) { return 4L * nr(N) * nk(N); }
+__host__ __device__ inline long chunk(int N, int w, int slot, int rb, int kb) {
+  return ((long)w * NSLOT + slot) * slot_elems(N) + ((long)rb * (nk(N) / 32) + kb) * (4 * 64 * 32);
+}
+// position of (plane, row, k) inside a chunk
+__host__ __device__ inline int in_chunk(int pl, int row, int k) {
+  return pl * 2048 + (row & 63) * 32 + ((((k & 31) >> 3) ^ (((row & 63) >> 1) & 3)) << 3) + (k & 7);
+}
+
+// hi/lo split of two values (the bf16 pair of f = float(x), then of f - hi)
+__device__ __forceinline__ void split_pair(double x0, double x1, uint32_t &hi, uint32_t &lo) {
+  const float f0 = (float)x0, f1 = (float)x1;
+  const bf16x2 h = {(__bf16)f0, (__bf16)f1};
+  const bf16x2 l = {(__bf16)(f0 - (float)h[0]), (__bf16)(f1 - (float)h[1])};
+  hi = __builtin_bit_cast(uint32_t, h);
+  lo = __builtin_bit_cast(uint32_t, l);
+}
+
+// ---------------------------------------------------------------------------
+// k_split: block (64 x 64 tile of the stored FP64 buffer, matrix, wire).
+// Stored buffers: p, C, D as the matrix; Heh, Heo, Hho as its transpose.
+// The tile is read coalesced along the buffer's rows (two rows per thread
+// and pass), split, and kept in LDS as row pairs per column (dword = rows
+// 2r, 2r+1; column stride 33 dwords), from which both slots are written
+// with 16-byte stores of 8 consecutive k.  Values past N (the padding) are
+// written as zeros; groups of 8 k at or past NK are not written.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_split(psa_group g, int w0) {
+  __shared__ uint32_t t[4][64][33];  // [plane][column j][row pair]
+  const int N = g.N, NK = nk(N), mat = blockIdx.z % 6, w = w0 + blockIdx.z / 6;
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64, tid = threadIdx.x;
+  const long o = mat_off(g, w);
+  const double *X = (mat == 0 ? g.pB : mat == 1 ? g.CB : mat == 2 ? g.DB
+                     : mat == 3 ? g.Heh : mat == 4 ? g.Heo : g.Hho) + o;
+  const bool zd = (mat == 1 || mat == 2);
+  const bool stored_t = mat >= 3;  // the buffer holds the transpose of the matrix
+  uint16_t *spl = reinterpret_cast<uint16_t *>(g.spl);
+  // slot receiving the buffer as stored, and the one receiving its transpose
+  const int s_same = 2 * mat + (stored_t ? 1 : 0), s_tr = 2 * mat + (stored_t ? 0 : 1);
+  {
+    const int j = tid & 63, gj = c0 + j;
+    cd v[8][2];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int rp = (tid >> 6) + 4 * e;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int gi = r0 + 2 * rp + r;
+        v[e][r] = ld(X, (long)min(gi, N - 1) * N + min(gj, N - 1));
+        if (gi >= N || gj >= N || (zd && gi == gj)) v[e][r] = cd{0.0, 0.0};
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int rp = (tid >> 6) + 4 * e;
+      uint32_t rh, rl, ih, il;  // planes of the row pair (rows 2 rp, 2 rp + 1) at column j
+      split_pair(v[e][0].re, v[e][1].re, rh, rl);
+      split_pair(v[e][0].im, v[e][1].im, ih, il);
+      t[0][j][rp] = rh;
+      t[1][j][rp] = rl;
+      t[2][j][rp] = ih;
+      t[3][j][rp] = il;
+    }
+  }
+  __syncthreads();
+  const int x = tid >> 2, kq = tid & 3;
+  // the buffer as stored: row i = x, k = columns 16 kq .. 16 kq + 15
+  {
+    const int gi = r0 + x;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int k0 = c0 + 16 * kq + 8 * half;
+      if (k0 >= NK) continue;
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) {
+        uint16_t hv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) hv[q] = (uint16_t)(t[pl][16 * kq + 8 * half + q][x >> 1] >> (16 * (x & 1)));
+        *reinterpret_cast<u32x4 *>(&spl[chunk(N, w, s_same, gi >> 6, k0 >> 5) + in_chunk(pl, gi, k0)]) =
+            __builtin_bit_cast(u32x4, hv);
+      }
+    }
+  }
+  // transposed: row j = x (a column of the buffer), k = rows 16 kq .. 16 kq + 15
+  {
+    const int gj = c0 + x;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int k0 = r0 + 16 * kq + 8 * half;
+      if (k0 >= NK) continue;
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) {
+        u32x4 v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = t[pl][x][8 * kq + 4 * half + q];
+        *reinterpret_cast<u32x4 *>(&spl[chunk(N, w, s_tr, gj >> 6, k0 >> 5) + in_chunk(pl, gj, k0)]) = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_split_band: slots 12 (kind 1, Dh) and 13 (kind 0, De): rows x, k = m,
+// value i hbar D[m - x][x] from the tables dedh[w][kind][x][q + N - 1]
+// (q = m - x, q = 0 excluded).  Block (64 rows, kind, wire); thread: one
+// row, 8 k per pass; the imaginary planes only (the real planes stay as
+// allocated: zero).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_split_band(psa_group g, int w0) {
+  const int N = g.N, NK = nk(N), Q = 2 * N - 1;
+  const int kind = blockIdx.y, w = w0 + blockIdx.z, slot = 13 - kind;
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), gx = min(x, N - 1);
+  const double *tab = g.dedh + ((long)w * 2 + kind) * (long)N * Q + (long)gx * Q;
+  uint16_t *spl = reinterpret_cast<uint16_t *>(g.spl);
+  for (int k0 = 8 * (threadIdx.x >> 6); k0 < NK; k0 += 32) {
+    double v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = k0 + e;
+      const double raw = tab[min(max(k - gx + N - 1, 0), Q - 1)];
+      v[e] = (x < N && k < N && k != x) ? g.hbar * raw : 0.0;
+    }
+    u32x4 hi, lo;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint32_t a, b;
+      split_pair(v[2 * q], v[2 * q + 1], a, b);
+      hi[q] = a;
+      lo[q] = b;
+    }
+    const long c = chunk(N, w, slot, x >> 6, k0 >> 5);
+    *reinterpret_cast<u32x4 *>(&spl[c + in_chunk(2, x, k0)]) = hi;
+    *reinterpret_cast<u32x4 *>(&spl[c + in_chunk(3, x, k0)]) = lo;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The RHS products.  A workgroup (256 threads, 4 waves as 2 x 2) owns a
+// 128 x 128 output tile of one (wire, equation); each wave a 64 x 64
+// sub-tile of 4 x 4 MFMA blocks (128 accumulator registers).  K runs in
+// slabs of 32 over the equation's terms.  The slab of each operand (two
+// 16 KB chunks: rows 0-63 and 64-127) is copied from HBM into the
+// workgroup's LDS stage (64 KB) by direct global-to-LDS loads (1 KB per
+// wave instruction, no registers).  Two workgroups share a CU (64 KB of
+// LDS and at most 128 + 128 registers per lane each): while one waits for
+// its copies or runs its FP64 epilogue (memory bound, straight from the
+// accumulator registers), the other keeps the matrix cores busy.
+// ---------------------------------------------------------------------------
+
+// One term of an equation: A and B slots, conjugation and sign.
+struct Term {
+  int sa, sb;
+  bool ca, cb;
+  bool neg;
+};
+
+// Terms in the order each equation runs them.  The twelve workgroups of a
+// wire run together on one XCD, so the orders are aligned for L2 reuse:
+// at every position dC reads the same B slot as dp (p^T, Hee_o^T, C_o,
+// Heh^T) and dD reads dp's A slot as its B (Hhh_o, p, Heh^T, D_o^T).
+__device__ __forceinline__ Term term(int e, int t) {
+  if (e == 1) t = (t + 3) & 3;               // dC: 3, 0, 1, 2
+  else if (e == 2) t = (4 - t) & 3;          // dD: 0, 3, 2, 1
+  switch (e * 8 + t) {
+    case 0: return {10, 1, false, false, false};  // dp:  Hhh_o p
+    case 1: return {0, 8, false, false, false};   //      p Hee_o^T
+    case 2: return {7, 3, false, false, true};    //     -Heh^T C_o
+    case 3: return {5, 6, false, false, true};    //     -D_o^T Heh^T
+    case 4: return {12, 1, false, false, false};  //      (i hbar Dh band) p
+    case 5: return {0, 13, false, false, false};  //      p (i hbar De band)
+    case 8: return {2, 8, false, false, false};   // dC:  C_o Hee_o^T
+    case 9: return {9, 3, false, false, true};    //     -Hee_o^T C_o
+    case 10: return {1, 6, true, false, false};   //      p^H Heh^T
+    case 11: return {6, 1, true, false, true};    //     -conj(Heh) p
+    case 16: return {4, 10, false, false, false}; // dD:  D_o Hhh_o^T
+    case 17: return {11, 5, false, false, true};  //     -Hhh_o^T D_o
+    case 18: return {0, 7, true, false, false};   //      conj(p) Heh
+    default: return {7, 0, true, false, true};    //     -Heh^H p^T
+  }
+}
+
+struct Stage {
+  uint16_t a[2][4 * 64 * 32];  // [half: rows 0-63, 64-127][chunk layout]
+  uint16_t b[2][4 * 64 * 32];
+};
+
+constexpr size_t lds_bytes() { return sizeof(Stage); }
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Copy the slab (k block kb) of slot `slot`, rows of the 128-row tile at
+// x0, into dst[2][chunk]: wave v copies the 1 KB pieces v, v + 4, ... of
+// the 32 (band slots too: their real planes are zero).
+__device__ __forceinline__ void copy_slab(uint16_t (*dst)[4 * 64 * 32], const uint16_t *spl, int N, int w, int slot,
+                                          int x0, int kb) {
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int i = wave + 4 * j;
+    const int h = i >> 4, off = (i & 15) * 512;  // 64-row half of the tile, uint16 offset in the chunk
+    const uint16_t *src = spl + chunk(N, w, slot, (x0 >> 6) + h, kb) + off + lane * 8;
+    __builtin_amdgcn_global_load_lds(src, (lds_void *)&dst[h][off], 16, 0, 0);
+  }
+}
+
+__device__ __forceinline__ bf16x8 fread(const uint16_t *half, int pl, int row, int kc) {
+  return __builtin_bit_cast(
+      bf16x8, *reinterpret_cast<const u32x4 *>(&half[pl * 2048 + row * 32 + ((kc ^ ((row >> 1) & 3)) << 3)]));
+}
+
+__device__ __forceinline__ bf16x8 flip(bf16x8 x, uint32_t m) {
+  u32x4 u = __builtin_bit_cast(u32x4, x);
+  u ^= u32x4{m, m, m, m};
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ fv4 mfma(bf16x8 a, bf16x8 b, fv4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// acc[si][sj] += a[si] b[sj] over the three split products (hi hi, hi lo,
+// lo hi); products on the same accumulator are 16 issues apart.
+__device__ __forceinline__ void prod3(const bf16x8 (&ah)[4], const bf16x8 (&al)[4], const bf16x8 (&bh)[4],
+                                      const bf16x8 (&bl)[4], fv4 (&acc)[4][4]) {
+#pragma unroll
+  for (int si = 0; si < 4; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 4; ++sj) acc[si][sj] = mfma(ah[si], bh[sj], acc[si][sj]);
+#pragma unroll
+  for (int si = 0; si < 4; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 4; ++sj) acc[si][sj] = mfma(ah[si], bl[sj], acc[si][sj]);
+#pragma unroll
+  for (int si = 0; si < 4; ++si)
+#pragma unroll
+    for (int sj = 0; sj < 4; ++sj) acc[si][sj] = mfma(al[si], bh[sj], acc[si][sj]);
+}
+
+// The products of one staged slab of a term into this wave's 4 x 4
+// blocks.  The term's sign and conjugations are sign flips of the operand
+// planes (exact): A' = sg conj?(A), B' = conj?(B), then
+//   Re += Re A' Re B' - Im A' Im B',  Im += Re A' Im B' + Im A' Re B'
+// (the band operands go through the same products with their zero real
+// planes).  At most one A and one B plane pair (64 registers) are held at
+// a time: Re A' Re B' (into Re), Re A' Im B' (Im), -Im A' Im B' (Re),
+// Im A' Re B' (Im); each accumulator gets its products in that order.
+__device__ __forceinline__ void slab(const Stage &st, uint32_t fre, uint32_t fim, uint32_t gim, fv4 (&aR)[4][4],
+                                     fv4 (&aI)[4][4], int wm, int wn, int kl, int xl) {
+  const uint16_t *A = st.a[wm], *B = st.b[wn];
+  bf16x8 ah[4], al[4], bh[4], bl[4];
+  auto readA = [&](int pl, uint32_t m) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      ah[s] = flip(fread(A, pl, s * 16 + xl, kl), m);
+      al[s] = flip(fread(A, pl + 1, s * 16 + xl, kl), m);
+    }
+  };
+  auto readB = [&](int pl, uint32_t m) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      bh[s] = flip(fread(B, pl, s * 16 + xl, kl), m);
+      bl[s] = flip(fread(B, pl + 1, s * 16 + xl, kl), m);
+    }
+  };
+  readA(0, fre);
+  readB(0, 0u);
+  prod3(ah, al, bh, bl, aR);
+  __builtin_amdgcn_sched_barrier(0);
+  readB(2, gim);
+  prod3(ah, al, bh, bl, aI);
+  __builtin_amdgcn_sched_barrier(0);
+  readA(2, fim ^ 0x80008000u);  // -Im A'
+  prod3(ah, al, bh, bl, aR);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    ah[s] = flip(ah[s], 0x80008000u);
+    al[s] = flip(al[s], 0x80008000u);
+  }
+  readB(0, 0u);
+  prod3(ah, al, bh, bl, aI);
+}
+
+// One 128 x 128 output tile (batch b = 3 wire + equation; rows m0..,
+// columns n0..) of problem p; lds holds lds_bytes().
+__device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0, char *lds) {
+  Stage &stg = *reinterpret_cast<Stage *>(lds);
+  const int N = p.g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1, kl = lane >> 4, xl = lane & 15;
+  const int e = b % 3, w = p.w0 + b / 3;
+  const int nt = p.nterms(b), KS = nk(N) / BK;
+  const uint16_t *spl = reinterpret_cast<const uint16_t *>(p.g.spl);
+  fv4 aR[4][4], aI[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      aR[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
+      aI[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
+    }
+  // per K slab: copy, wait for everyone's copies, multiply, wait for
+  // everyone's products before the stage is overwritten
+  for (int t = 0; t < nt; ++t) {
+    const Term tm = term(e, t);
+    const uint32_t fre = tm.neg ? 0x80008000u : 0u, fim = (tm.neg != tm.ca) ? 0x80008000u : 0u;
+    const uint32_t gim = tm.cb ? 0x80008000u : 0u;
+    for (int kb = 0; kb < KS; ++kb) {
+      copy_slab(stg.a, spl, N, w, tm.sa, m0, kb);
+      copy_slab(stg.b, spl, N, w, tm.sb, n0, kb);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      slab(stg, fre, fim, gim, aR, aI, wm, wn, kl, xl);
+      __syncthreads();
+    }
+  }
+  // The FP64 epilogue, through the LDS stage in two halves of 64 rows: the
+  // waves holding a half (wm) leave their accumulators there as
+  // [re/im][64][128] floats, then every thread walks that half row-major
+  // with a fixed column (its column quantities read once), four rows at a
+  // time with their FP64 operands loaded first and without branches (the
+  // other half's accumulators are still held meanwhile).
+  float *acc = reinterpret_cast<float *>(lds);
+  const bool full = N % TILE == 0;
+  const int col = n0 + (tid & (TILE - 1));
+  const long o = mat_off(p.g, w);
+  const double *X = (e == 0 ? p.g.pB : (e == 1 ? p.g.CB : p.g.DB)) + o;
+  const double *H = (e == 0 ? p.g.Heh : (e == 1 ? p.g.Heo : p.g.Hho)) + o;
+  const double *X1 = (e == 0 ? p.g.pA : (e == 1 ? p.g.CA : p.g.DA)) + o;
+  double *out = (e == 0 ? p.g.pX : (e == 1 ? p.g.CX : p.g.DX)) + o;
+  const RhsProb::Side cs = p.side(e, w, min(col, N - 1), true);
+  // mode 1: X1 + 2 dt dX/dt (the leapfrog), mode 0: dX/dt
+  auto rows = [&](auto leap, int h) {
+    constexpr bool LEAP = decltype(leap)::value;
+    constexpr int EB = 4;
+    for (int j0 = 0; j0 < 32; j0 += EB) {
+      RhsProb::Pre q[EB];
+#pragma unroll
+      for (int u = 0; u < EB; ++u) {
+        const int row = m0 + h * 64 + (tid >> 7) + 2 * (j0 + u);
+        const long ij = (long)row * N + col;
+        q[u].x = ld(X, ij);
+        const cd hv = ld(H, ij);
+        q[u].h = (e != 0 && row == col) ? cd{0.0, 0.0} : hv;
+        if constexpr (LEAP) q[u].x1 = ld(X1, ij);
+      }
+#pragma unroll
+      for (int u = 0; u < EB; ++u) {
+        const int rr = (tid >> 7) + 2 * (j0 + u), row = m0 + h * 64 + rr, cc = tid & (TILE - 1);
+        const RhsProb::Side rs = p.side(e, w, row, false);
+        const cd d = p.deriv(e, (double)acc[rr * TILE + cc], (double)acc[64 * TILE + rr * TILE + cc], q[u], cs, rs);
+        if constexpr (LEAP)
+          st(out, (long)row * N + col, cd{q[u].x1.re + d.re * p.dt * 2.0, q[u].x1.im + d.im * p.dt * 2.0});
+        else
+          st(out, (long)row * N + col, d);
+      }
+    }
+  };
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (wm == h) {
+#pragma unroll
+      for (int si = 0; si < 4; ++si)
+#pragma unroll
+        for (int sj = 0; sj < 4; ++sj)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rr = si * 16 + 4 * kl + v, cc = wn * 64 + sj * 16 + xl;
+            acc[rr * TILE + cc] = aR[si][sj][v];
+            acc[64 * TILE + rr * TILE + cc] = aI[si][sj][v];
+          }
+    }
+    __syncthreads();
+    if (full && p.mode == 1) {
+      rows(std::true_type{}, h);
+    } else if (full && p.mode == 0) {
+      rows(std::false_type{}, h);
+    } else {
+      for (int i = tid; i < 64 * TILE; i += 256) {
+        const int rr = i / TILE, cc = i % TILE;
+        p.store(b, m0 + h * 64 + rr, n0 + cc, (double)acc[rr * TILE + cc], (double)acc[64 * TILE + rr * TILE + cc]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void k_rhs(RhsProb p) {
+  extern __shared__ char lds_raw[];
+  // XCD-aware order: the three equations of wire j run on XCD j % 8 (they
+  // read the same wire's planes, so those stay in that XCD's L2) when the
+  // wire count is a multiple of 8; consecutive workgroups go round robin
+  // over the XCDs.
+  const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
+  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  int b;
+  if (gridDim.z % 24 == 0 && p.g.tune[2] == 0) {
+    const unsigned xcd = lin % 8, q = lin / 8, per = 3 * tpb;
+    b = 3 * (xcd + 8 * (q / per)) + (q % per) / tpb;
+    lin = q % tpb;
+  } else {
+    b = lin / tpb;
+    lin %= tpb;
+  }
+  rhs_tile(p, b, (lin / gx) * TILE, (lin % gx) * TILE, lds_raw);
+}
+
+}  // namespace sp
+}  // namespace psa

@@ -115,12 +115,13 @@ class WireGroup:
         self.s["obs"] = z(nwz, 8, dt=F64)
         self.s["tphase"] = z(nwz, N)
         self.s["part"] = z(nwz, 256, dt=F64)
-        # split-bf16 operand planes (bf16x3 only): [Nw][12 slots] of 4 planes
-        # on a zero-padded (128 ceil(N/128)) x (32 ceil(N/32)) grid, blocked
         if integrator == "rk4":  # stage accumulators R = k1 + 2 k2 + 2 k3
             for k in ("pR", "CR", "DR"):
                 self.s[k] = z(nwz, N, N)
-        nspl = nwz * 12 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) if prec == "bf16x3" else 1
+        # split-bf16 operand planes (bf16x3 only): [Nw][14 slots][4 planes] on
+        # a zero-padded (128 ceil(N/128)) x (32 ceil(N/32)) grid, in 16 KB
+        # chunks of 64 rows x 32 k
+        nspl = nwz * 14 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) if prec == "bf16x3" else 1
         self.s["spl"] = torch.zeros(nspl, dtype=torch.int16, device=d)
         # split-K partial sums of the source / propagation-grid products
         self._kscr_len = 8 * 2 * max(Nr * 4 * nwz, (2 * N - 1) * 3 * nwz, max(tb["bnx"], 1) * 5 * nwz)
@@ -251,8 +252,8 @@ class WireGroup:
 
     def autotune(self, reps=3):
         """Pick the launch choices of this group by timing them on its own
-        buffers: tune[2] GEMM block order (XCD-grouped or linear), tune[0]
-        split-BF16 tile (64 or 128), tune[1] / tune[3] transport lines per
+        buffers: tune[2] GEMM and split-BF16 RHS block order (XCD-grouped or
+        linear), tune[1] / tune[3] transport lines per
         block for the row / column pass (4, 2, 8), timed inside full steps
         (the column pass carries the leapfrog average there).  The state is
         unchanged: only per-step scratch is written, and the averaged level
@@ -273,8 +274,6 @@ class WireGroup:
             capi.check(self.lib.psa_step(ctypes.byref(self.S), 0, self.nw, 1e-18, self._stream()))
 
         knobs = [(2, [0, 1], contractions)]
-        if self.prec == "bf16x3":
-            knobs.append((0, [0, 1], lambda: self.rhs(1, 1e-18)))
         saved = None
         if self.prm.flags.get("DCTrans") and self.prm.flags.get("LF"):
             saved = {k: self.s[k].clone() for k in ("pA", "CA", "DA")}

@@ -267,7 +267,7 @@ def slot_bytes(nk, nx, prec, maxsteps, integrator="leapfrog"):
     b += 2 * N * (2 * N - 1) * 8 + 11 * N * 8 + 3 * (2 * N - 1) * 16
     b += 3 * nx * 16 + 4 * nx * 16 + (9 * (2 * N + 2) + 8 * (2 * N - 1) + 8) * 8
     if prec == "bf16x3":
-        b += 12 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) * 2
+        b += 14 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) * 2
     b += 8 * 2 * max((2 * N + 2) * 4, (2 * N - 1) * 3, nx * 5) * 8  # split-K partial sums (bnx <= nx)
     b += maxsteps * 4 * 8 + 8 * 8
     return b
