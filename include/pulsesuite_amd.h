@@ -107,7 +107,9 @@ typedef struct psa_group {
                                          Heh[k1][k2] (resp. Hee, Hhh without their diagonal) */
   double *hdiag;                      /* [Nw][2][N] complex: diagonals of Hee, Hhh */
   double *prep;                       /* [Nw][PSA_PREP_STRIDE(N)] doubles, see PSA_PREP_* */
-  double *dedh;                       /* [Nw][2][N][2N-1] off-diagonal dephasing tables D[q][k] stored [k][q + N - 1] */
+  double *dedh;                       /* [Nw][2][N][2N-1] off-diagonal dephasing tables D[q][k] stored [k][q + N - 1];
+                                         allocate zeroed: entries with k + q outside [0, N) and q = 0 are
+                                         zero and the split-BF16 table kernel does not write them */
   double *wmb;                        /* 11 N Nw doubles: [Nw][4][N] many-body Win_e, Wout_e, Win_h, Wout_h
                                          (after psa_relaxation: the total rates times the occupation
                                          factors Win_e (1 - ne), Wout_e ne, Win_h (1 - nh), Wout_h nh);

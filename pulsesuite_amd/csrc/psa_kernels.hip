@@ -1937,7 +1937,11 @@ int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream)
     // Lorentzian generated in registers
     if (g->N > odg::NMAX) return fail(PSA_E_UNSUPPORTED, "psa_offdiag_tables: N > 512 needs the lor table");
     const dim3 grid((g->N + odg::ROWS - 1) / odg::ROWS, (2 * nw + odg::COLS - 1) / odg::COLS, 2 * (2 * g->N - 1));
-    if (prec == PSA_PREC_FP64)
+    if (g->prec == PSA_PREC_BF16X3)
+      hipLaunchKernelGGL(odg::k_offdiag_bf,
+                         dim3((g->N + odg::bfx::ROWS - 1) / odg::bfx::ROWS, (nw + 63) / 64, 2 * (2 * g->N - 1)),
+                         dim3(odg::bfx::THREADS), 0, s, *g, w0, nw);
+    else if (prec == PSA_PREC_FP64)
       hipLaunchKernelGGL(odg::k_offdiag<double>, grid, dim3(odg::Cfg<double>::THREADS), 0, s, *g, w0, nw, 0);
     else
       hipLaunchKernelGGL(odg::k_offdiag<float>, grid, dim3(odg::Cfg<float>::THREADS), 0, s, *g, w0, nw, 1);

@@ -261,5 +261,235 @@ __global__ __launch_bounds__(Cfg<T>::THREADS, 2) void k_offdiag(psa_group g, int
       }
 }
 
+// ---------------------------------------------------------------------------
+// Split-BF16 tables (PSA_PREC_BF16X3): the same sums on
+// v_mfma_f32_16x16x32_bf16.  Both operands are split as
+// hi = bf16(f), lo = bf16(f - hi) of their FP32 value f, and every product
+// is hi hi + hi lo + lo hi with FP32 accumulation (relative error of a
+// product ~2^-16; the terms are positive, so the sums keep that bound).
+//
+// Block = 8 waves of 32 rows over 256 rows starting at the first row rb
+// with k + q in [0, N), x 128 columns (64 wires x the A and B sums).  Rows
+// past re are skipped wave by wave; blocks past re exit at once, so the
+// shrinking (q, k) triangle costs no idle blocks.  Entries with k + q
+// outside [0, N) and q = 0 are not written (the tables are allocated
+// zeroed and no kernel writes them).
+//
+// The K index runs over the same-carrier p, zero padding to a multiple of
+// the 32-K chunk, then the other-carrier p, so every chunk is of one
+// carrier.  B [K][128] (the occupation columns times V^2 / (pi hg)) is built
+// per chunk by all threads (thread = wire, 4 consecutive K: one pair of
+// occupation loads gives both columns of the wire), split, and stored as
+// [hi/lo][column][32 K] bf16 with the 16-byte group of 8 K at
+// (K >> 3) ^ ((column >> 1) & 3) (the MFMA reads, 16 columns x 4 groups per
+// instruction, fall on distinct banks), double-buffered: chunk ch + 1 is
+// built between chunk ch's Lorentzian and its products.  The Lorentzian
+// 1 / (x^2 + 1) is made per lane in FP32 (8 consecutive K of one row per
+// MFMA) from x = (Es[k + q] - Es[k]) / hg (registers) + (e1 - e2) / hg
+// (LDS), summed in FP64, then split.  The occupation columns are the MFMA A
+// operand and the Lorentzian B, so that each lane ends with both sums (A,
+// B) of two wires for one row k.
+// ---------------------------------------------------------------------------
+namespace bfx {
+constexpr int WAVES = 8, THREADS = 64 * WAVES, ROWS = 32 * WAVES, COLS = 128, KC = 32;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+}  // namespace bfx
+
+__global__ __launch_bounds__(bfx::THREADS, 4) void k_offdiag_bf(psa_group g, int w0, int nw) {
+  constexpr int THREADS = bfx::THREADS, ROWS = bfx::ROWS, COLS = bfx::COLS, KC = bfx::KC;
+  using bfx::bf16x4;
+  using bfx::bf16x8;
+  using bfx::u32x2;
+  using bfx::u32x4;
+  __shared__ double dvk[KMAX];                                              // (e1 - e2) / hg per K index
+  __shared__ __attribute__((aligned(16))) uint16_t bsp[2][2][COLS * KC];  // [buffer][hi, lo][column][K]
+
+  const int N = g.N, W = g.Nw, Q = 2 * N - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int pr = blockIdx.z >> 1, kind = blockIdx.z & 1;
+  const int q = (pr & 1) ? (pr + 1) >> 1 : -(pr >> 1), qi = q + N - 1;
+  const int aq = q < 0 ? -q : q;
+  const int rb = max(0, -q), re = min(N, N - q);
+  const int row0 = rb + ROWS * blockIdx.x;
+  // K layout: same-carrier p in [0, K0), zero padding to K0p (a multiple
+  // of the chunk, so that every chunk is of one carrier), other-carrier p
+  // in [K0p, K)
+  const int kb0 = max(0, q + 1), K0 = max(0, min(N, N + q) - kb0), kb1 = rb;
+  const int K0p = (K0 + KC - 1) / KC * KC;
+  const int K = q == 0 ? 0 : K0p + (re - rb);
+  if (K == 0 || row0 >= re) return;  // block-uniform, before any barrier
+  const int nch = (K + KC - 1) / KC;
+  const double *Es = kind == 0 ? g.Ee : g.Eh;
+  const double *Eo = kind == 0 ? g.Eh : g.Ee;
+  // Lrtz(Ed, hg) = (1 / (pi hg)) / (x^2 + 1), x = Ed / hg, hg of the
+  // same-carrier (hgS) or other-carrier (hgO) K value:
+  // x = (Es[k + q] - Es[k]) / hg + (e1 - e2) / hg in FP64; the factor
+  // 1 / (pi hg) goes into the occupation operand
+  const double hgS = g.hbar * (kind == 0 ? g.gam_eh : g.gam_e), hgO = g.hbar * g.gam_h;
+  for (int kk = tid; kk < nch * KC; kk += THREADS) {
+    double d = 0.0;
+    if (kk < K0) {
+      const int p = kb0 + kk;
+      d = (Es[p - q] - Es[p]) / hgS;
+    } else if (kk >= K0p && kk < K) {
+      const int p = kb1 + kk - K0p;
+      d = (Eo[p + q] - Eo[p]) / hgO;
+    }
+    dvk[kk] = d;
+  }
+
+  // B builder: wire wl of the block's 64, K values 4 kq .. 4 kq + 3 of each chunk
+  const int wl = tid & 63, kq = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+  const int wcol = blockIdx.y * 64 + wl;
+  const bool wok = wcol < nw;
+  const int wB = w0 + min(wcol, nw - 1);
+  const double *occ = g.wmb + 9L * N * W;  // |n| of the checkout level [2][N][W]
+  const double *vsq = g.wmb + 6L * N * W;  // VC(d)^2 [3][N][W]
+  const double *nnS = occ + (kind == 0 ? 0L : (long)N * W);  // wire wB at offset wB
+  const double *nnO = occ + (kind == 0 ? (long)N * W : 0L);
+  const int aqc = min(aq, N - 1);
+  // V^2 / (pi hg), 2^100-scaled, as FP32: its rounding is far below the split's
+  const float v2S = wok ? (float)(0x1p100 * vsq[((long)(kind == 0 ? 1 : 2) * N + aqc) * W + wB] / (PI * hgS)) : 0.0f;
+  const float v2O = wok ? (float)(0x1p100 * vsq[(long)aqc * W + wB] / (PI * hgO)) : 0.0f;
+  // the occupations are read as the upper 32 bits of their FP64 values
+  // (21 significant bits; the split keeps 16), one register each
+  uint32_t ra[4], rc[4];
+  auto issue = [&](int ch) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int kk = ch * KC + 4 * kq + e;
+      int p = 0, j = 0;
+      const double *nn = nnS;
+      if (kk < K0) {
+        p = kb0 + kk;
+        j = p - q;
+      } else if (kk >= K0p && kk < K) {
+        p = kb1 + kk - K0p;
+        j = p + q;
+        nn = nnO;
+      }
+      ra[e] = reinterpret_cast<const uint32_t *>(nn + (long)j * W + wB)[1];
+      rc[e] = reinterpret_cast<const uint32_t *>(nn + (long)p * W + wB)[1];
+    }
+  };
+  // columns 2 wl (A: n[j] (1 - n[p])) and 2 wl + 1 (B: (1 - n[j]) n[p])
+  const int cA = 2 * wl, sw = (wl & 3);
+  const int boff = cA * KC + (((kq >> 1) ^ sw) << 3) + 4 * (kq & 1);
+  auto commit = [&](int ch, int buf) {
+    bf16x4 hA, lA, hB, lB;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int kk = ch * KC + 4 * kq + e;
+      const double v = kk < K0 ? (double)v2S : ((kk >= K0p && kk < K) ? (double)v2O : 0.0);
+      const double na = __builtin_bit_cast(double, (uint64_t)ra[e] << 32);
+      const double nc = __builtin_bit_cast(double, (uint64_t)rc[e] << 32);
+      const float fa = (float)(v * (na * (1.0 - nc))), fb = (float)(v * ((1.0 - na) * nc));
+      hA[e] = (__bf16)fa;
+      lA[e] = (__bf16)(fa - (float)hA[e]);
+      hB[e] = (__bf16)fb;
+      lB[e] = (__bf16)(fb - (float)hB[e]);
+    }
+    uint16_t *h = bsp[buf][0], *l = bsp[buf][1];
+    *reinterpret_cast<u32x2 *>(&h[boff]) = __builtin_bit_cast(u32x2, hA);
+    *reinterpret_cast<u32x2 *>(&l[boff]) = __builtin_bit_cast(u32x2, lA);
+    *reinterpret_cast<u32x2 *>(&h[boff + KC]) = __builtin_bit_cast(u32x2, hB);
+    *reinterpret_cast<u32x2 *>(&l[boff + KC]) = __builtin_bit_cast(u32x2, lB);
+  };
+
+  fv4 acc[2][8];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[i][c] = fv4{0.f, 0.f, 0.f, 0.f};
+
+  const int g4 = lane >> 4, r16 = lane & 15;
+  const int wrow0 = row0 + 32 * wv;
+  const bool wact = wrow0 < re;  // wave-uniform
+  double udS[2], udO[2];  // (Es[k + q] - Es[k]) / hg of this lane's rows
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int k = min(wrow0 + 16 * i + r16, N - 1);
+    const double ud = Es[min(max(k + q, 0), N - 1)] - Es[k];
+    udS[i] = ud / hgS;
+    udO[i] = ud / hgO;
+  }
+
+  issue(0);
+  commit(0, 0);
+  __syncthreads();
+  // per chunk: this chunk's Lorentzian, the next chunk's B into the other
+  // buffer (last read before the previous barrier), then the products
+  for (int ch = 0; ch < nch; ++ch) {
+    const bool more = ch + 1 < nch;
+    bf16x8 ah[2], al[2];
+    if (wact) {
+      const bool same = ch * KC < K0p;  // chunk-uniform
+      const int kk0 = ch * KC + 8 * g4;  // this lane's 8 K values
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const double ud = same ? udS[i] : udO[i];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float x = (float)(ud + dvk[kk0 + e]);
+          const float a = __builtin_amdgcn_rcpf(fmaf(x, x, 1.0f));
+          ah[i][e] = (__bf16)a;
+          al[i][e] = (__bf16)(a - (float)ah[i][e]);
+        }
+      }
+    }
+    if (more) {
+      issue(ch + 1);
+      commit(ch + 1, (ch + 1) & 1);
+    }
+    if (wact) {
+      // occupation columns as the A operand (rows m = columns), Lorentzian
+      // as B (columns n = rows k): D[m][n] lands in lane n % 16, m = 4 (l / 16) + v
+      const uint16_t *Bh = bsp[ch & 1][0], *Bl = bsp[ch & 1][1];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int col = 16 * c + r16;
+        const int off = col * KC + ((g4 ^ ((col >> 1) & 3)) << 3);
+        const bf16x8 bh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(&Bh[off]));
+        const bf16x8 bl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(&Bl[off]));
+#pragma unroll
+        for (int i = 0; i < 2; ++i) acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah[i], acc[i][c], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah[i], acc[i][c], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al[i], acc[i][c], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    __syncthreads();
+  }
+  if (!wact) return;
+
+  // epilogue: acc[i][c][v] is row k = wrow0 + 16 i + lane % 16 of column
+  // 16 c + 4 (lane / 16) + v, i.e. the A (v even) and B (v odd) sums of
+  // wire 8 c + 2 (lane / 16) + v / 2: both sums of a wire in one lane
+  const double pih = PI / g.hbar * 0x1p-100;
+  const long aoff = kind == 0 ? PSA_PREP_ANE(N) : PSA_PREP_ANH(N);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int kr = wrow0 + 16 * i + r16;
+    if (kr >= re) continue;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int wc = blockIdx.y * 64 + 8 * c + 2 * g4 + h;
+        if (wc >= nw) continue;
+        const int w = w0 + wc;
+        const double nk = g.prep[(long)w * PSA_PREP_STRIDE(N) + aoff + kr];
+        const double rA = (double)acc[i][c][2 * h], rB = (double)acc[i][c][2 * h + 1];
+        g.dedh[(((long)w * 2 + kind) * N + kr) * Q + qi] = ((1.0 - nk) * rA + nk * rB) * pih;
+        if (h) __builtin_amdgcn_sched_barrier(0);
+      }
+  }
+}
+
 }  // namespace odg
 }  // namespace psa
