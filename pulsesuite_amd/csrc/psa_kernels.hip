@@ -1885,6 +1885,15 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   const int prec = g->prec == PSA_PREC_FP64 ? PSA_PREC_FP64 : PSA_PREC_FP32;
   // FP32: the shifted-Toeplitz kernel; FP64 runs faster on the GEMM engine
   // (the FP64 MFMA rate leaves the shifted kernel's larger row count behind)
+  if (g->prec == PSA_PREC_BF16X3 && g->N <= chx::NMAX) {
+    // split-BF16 products on the same shifted-Toeplitz blocks
+    const int cols = (g->N + 31) / 32;
+    constexpr int R = chx::bfx::ROWS;
+    hipLaunchKernelGGL(chx::k_calch_bf, dim3(3 * nw, (g->N + R - 1) / R, cols), dim3(chx::bfx::THREADS), 0, s, *g,
+                       w0);
+    PSA_LAUNCH_CHECK("k_calch_bf");
+    return PSA_OK;
+  }
   if (prec == PSA_PREC_FP32 && g->N <= chx::NMAX) {
     const int cols = (g->N + 31) / 32;  // column blocks: 32 positive and 32 non-positive d each
     constexpr int R = chx::rows_per_block<float>();

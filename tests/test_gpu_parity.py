@@ -129,11 +129,13 @@ def test_calc_h_stage(case64):
         assert rel(hd[w, 1], np.diag(Hhh)) < 1e-14
 
 
-@pytest.mark.parametrize("N", [256, 384])
+@pytest.mark.parametrize("N", [256, 384, 512])
 def test_calc_h_large_n(N):
     """Exchange sums at the serving sizes against the oracle: FP64 (GEMM
-    engine) within 1e-12; FP32 (the reduced modes: shifted-Toeplitz kernel,
-    N = 384 with two row groups and a partial column block) within 1e-5."""
+    engine) within 1e-12; FP32 (shifted-Toeplitz kernel on FP32 MFMA) and
+    split-BF16 (the same blocks on bf16 MFMA, operands split hi/lo) within
+    1e-5 of the largest entry; N = 384 has two row groups and a partial
+    column block, N = 512 the largest Toeplitz row."""
     import torch
     from pulsesuite_amd.engine import WireGroup
     L = wire_length(N)
@@ -154,7 +156,7 @@ def test_calc_h_large_n(N):
         Meh = O.calc_meh(og, ot, *oq, w + 1)
         ref.append(O.calc_h(og, Meh, C3, D3, p3, VC, True, True))
     off = lambda m: m - np.diag(np.diag(m))  # noqa: E731
-    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-5), ("bf16x3", 1e-5)):
         print(f"[calc_h] N={N} prec={prec}", flush=True)
         wg = WireGroup(prm, nw, rr, prec=prec)
         wg.set_state(*st)
