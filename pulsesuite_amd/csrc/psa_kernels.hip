@@ -498,43 +498,54 @@ __global__ __launch_bounds__(MB_THREADS) void k_mb(psa_group g, int w0, int nw) 
 
 // The same sums for the reduced-precision modes: occupations and V^2 staged
 // in FP32 (V^2 scaled by 2^100, exact, to stay in the FP32 normal range;
-// unscaled with the FP64 result) for 16 wires per block, so a lane (entry
-// group lane >> 3, wires 2c and 2c + 1 for c = lane & 7) decodes each entry
-// once for two wires and reads each table with one 8-byte LDS load.  The
-// occupation and V^2 products run in FP32; the table weight (~1e-270 ..
-// 1e57) multiplies them in FP64 into FP64 sums.
+// unscaled with the FP64 result) for 16 wires per block.  A lane is (entry
+// group lane >> 2, wires 4 (lane & 3) .. 4 (lane & 3) + 3): the four lanes
+// of a group load the same list entry (one request), decode it once for
+// their four wires and read each table row with one 16-byte LDS load; the
+// sixteen groups of a wave take consecutive entries, and a lane loads the
+// entries of four block passes before using them.  The occupation and
+// V^2 products run in FP32; the table weight (~1e-270 .. 1e57) multiplies
+// them in FP64 into FP64 sums.
 constexpr int MB2_WC = 16;
 
 template <int L>
 __device__ __forceinline__ void mb_list_sum2(const uint32_t *ix, const double *vl, int beg, int end,
                                              const float *s_e, const float *s_h, const float *vv,
-                                             double &acc0, double &acc1) {
+                                             double (&acc)[4]) {
   using T = MbList<L>;
   const float *t1 = T::c1 ? s_h : s_e, *t2 = T::c2 ? s_h : s_e, *t3 = T::c3 ? s_h : s_e;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, u = lane >> 3, c2 = 2 * (lane & 7);
-  for (int e0 = beg + wv * 64; e0 < end; e0 += MB_THREADS) {
-    const int e = e0 + lane;
-    const uint32_t pk_l = e < end ? ix[e] : 0u;
-    const double val_l = e < end ? vl[e] : 0.0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, c4 = 4 * (lane & 3);
+  constexpr int PASS = MB_THREADS / 4, B = 4;  // entries per block pass; passes loaded together
+  for (int e0 = beg + 16 * wv + (lane >> 2); e0 < end; e0 += B * PASS) {
+    uint32_t pks[B];
+    double vals[B];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int src = 8 * r + u;
-      const uint32_t pk = (uint32_t)__shfl((int)pk_l, src, 64);
-      const double val = __shfl(val_l, src, 64);
+    for (int u = 0; u < B; ++u) {  // the entries of B passes in flight together (padding: weight 0, index 0)
+      const int e = e0 + u * PASS;
+      pks[u] = e < end ? ix[e] : 0u;
+      vals[u] = e < end ? vl[e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const uint32_t pk = pks[u];
+      const double val = vals[u];
       int a, cc, r1, r2, r3;
       mb_indices<L>(pk, a, cc, r1, r2, r3);
       // VC is lower-triangular: V(a, c) = 0 for a < c
-      const float2 v2 = *reinterpret_cast<const float2 *>(&vv[(a >= cc ? a - cc : 0) * MB2_WC + c2]);
-      // the table weights span ~1e-270 .. 1e57: they stay FP64
+      const float4 v2 = *reinterpret_cast<const float4 *>(&vv[(a >= cc ? a - cc : 0) * MB2_WC + c4]);
       const double w = (a >= cc) ? val : 0.0;
-      const float2 p1 = *reinterpret_cast<const float2 *>(&t1[r1 * MB2_WC + c2]);
-      const float2 p2 = *reinterpret_cast<const float2 *>(&t2[r2 * MB2_WC + c2]);
-      const float2 p3 = *reinterpret_cast<const float2 *>(&t3[r3 * MB2_WC + c2]);
-      const float f1x = T::x1 ? 1.0f - p1.x : p1.x, f1y = T::x1 ? 1.0f - p1.y : p1.y;
-      const float f2x = T::x2 ? 1.0f - p2.x : p2.x, f2y = T::x2 ? 1.0f - p2.y : p2.y;
-      const float f3x = T::x3 ? 1.0f - p3.x : p3.x, f3y = T::x3 ? 1.0f - p3.y : p3.y;
-      acc0 = fma((double)(v2.x * (f1x * f2x * f3x)), w, acc0);
-      acc1 = fma((double)(v2.y * (f1y * f2y * f3y)), w, acc1);
+      const float4 p1 = *reinterpret_cast<const float4 *>(&t1[r1 * MB2_WC + c4]);
+      const float4 p2 = *reinterpret_cast<const float4 *>(&t2[r2 * MB2_WC + c4]);
+      const float4 p3 = *reinterpret_cast<const float4 *>(&t3[r3 * MB2_WC + c4]);
+      const float pv[4][4] = {{v2.x, p1.x, p2.x, p3.x}, {v2.y, p1.y, p2.y, p3.y},
+                              {v2.z, p1.z, p2.z, p3.z}, {v2.w, p1.w, p2.w, p3.w}};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float f1 = T::x1 ? 1.0f - pv[q][1] : pv[q][1];
+        const float f2 = T::x2 ? 1.0f - pv[q][2] : pv[q][2];
+        const float f3 = T::x3 ? 1.0f - pv[q][3] : pv[q][3];
+        acc[q] = fma((double)(pv[q][0] * (f1 * f2 * f3)), w, acc[q]);
+      }
     }
   }
 }
@@ -561,7 +572,7 @@ __global__ __launch_bounds__(MB_THREADS) void k_mb2(psa_group g, int w0, int nw)
   for (int kk = 0; kk < MB_KG; ++kk) {
     const int k = kb + kk;
     if (k >= N) break;
-    double acc0 = 0.0, acc1 = 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int half = 0; half < 2; ++half) {
       // slot 0 WinE <- lists 0, 2; 1 WoutE <- 1, 3; 2 WinH <- 4, 6; 3 WoutH <- 5, 7
       const int L = (slot >> 1) * 4 + (slot & 1) + 2 * half;
@@ -571,27 +582,25 @@ __global__ __launch_bounds__(MB_THREADS) void k_mb2(psa_group g, int w0, int nw)
       const double *vl = g.mb_val[L];
       const float *vv = half == 0 ? s_v0 : s_v1;
       switch (L) {
-        case 0: mb_list_sum2<0>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
-        case 1: mb_list_sum2<1>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
-        case 2: mb_list_sum2<2>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
-        case 3: mb_list_sum2<3>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
-        case 4: mb_list_sum2<4>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
-        case 5: mb_list_sum2<5>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
-        case 6: mb_list_sum2<6>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
-        default: mb_list_sum2<7>(ix, vl, beg, end, s_ne, s_nh, vv, acc0, acc1); break;
+        case 0: mb_list_sum2<0>(ix, vl, beg, end, s_ne, s_nh, vv, acc); break;
+        case 1: mb_list_sum2<1>(ix, vl, beg, end, s_ne, s_nh, vv, acc); break;
+        case 2: mb_list_sum2<2>(ix, vl, beg, end, s_ne, s_nh, vv, acc); break;
+        case 3: mb_list_sum2<3>(ix, vl, beg, end, s_ne, s_nh, vv, acc); break;
+        case 4: mb_list_sum2<4>(ix, vl, beg, end, s_ne, s_nh, vv, acc); break;
+        case 5: mb_list_sum2<5>(ix, vl, beg, end, s_ne, s_nh, vv, acc); break;
+        case 6: mb_list_sum2<6>(ix, vl, beg, end, s_ne, s_nh, vv, acc); break;
+        default: mb_list_sum2<7>(ix, vl, beg, end, s_ne, s_nh, vv, acc); break;
       }
     }
-    // reduce over the eight entry groups of a wave (lane bits 3..5), then
+    // reduce over the sixteen entry groups of a wave (lane bits 2..5), then
     // over the waves, in a fixed order
 #pragma unroll
-    for (int o = 8; o < 64; o *= 2) {
-      acc0 += __shfl_xor(acc0, o, 64);
-      acc1 += __shfl_xor(acc1, o, 64);
-    }
-    if (lane < 8) {
-      red2[wv][2 * lane] = acc0;
-      red2[wv][2 * lane + 1] = acc1;
-    }
+    for (int o = 4; o < 64; o *= 2)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] += __shfl_xor(acc[q], o, 64);
+    if (lane < 4)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) red2[wv][4 * lane + q] = acc[q];
     __syncthreads();
     if (tid < MB2_WC && wc0 + tid < nw) {
       double sum = 0.0;
