@@ -145,15 +145,42 @@ def accuracy_vs_fp64(prec, N, nx, nw_err, steps, dt, seed):
     return out
 
 
-def cpu_baseline(wg, prm, rr, dt, budget_s=20.0):
-    """The oracle (NumPy FP64 restatement of the reference) on the host:
-    one wire-step of the same workload, repeated until ~budget_s."""
+def host_cpu():
+    """CPU model, physical cores of the CPUs this process may run on, and
+    the logical CPUs it may run on (from /proc/cpuinfo and the affinity mask)."""
+    allowed = sorted(os.sched_getaffinity(0))
+    model, phys, cur = "unknown", set(), {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    if "processor" in cur and int(cur["processor"]) in allowed:
+                        phys.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                    cur = {}
+                    continue
+                k, v = (t.strip() for t in line.split(":", 1))
+                cur[k] = v
+                if k == "model name":
+                    model = v
+        if "processor" in cur and int(cur["processor"]) in allowed:
+            phys.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+    except OSError:  # pragma: no cover
+        pass
+    return model, len(phys) or len(allowed), len(allowed)
+
+
+def cpu_baseline(wg, prm, rr, dt, steps=10):
+    """The oracle (NumPy FP64 restatement of the reference) on the host: one
+    warm-up wire-step of the same workload (it also indexes the many-body
+    tables' non-zero entries), then `steps` timed wire-steps with the field
+    on (active steps)."""
     from oracle import sbe_oracle as O
     try:
         from threadpoolctl import threadpool_info
         threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
     except Exception:  # pragma: no cover
         threads = 1
+    model, phys, logical = host_cpu()
     oprm = O.Params(L=prm.L, me_rel=0.067)
     g = O.Grid(oprm)
     vt = np.zeros((3, g.Nr))
@@ -167,19 +194,17 @@ def cpu_baseline(wg, prm, rr, dt, budget_s=20.0):
     Eyy = wg.s["efield"][1, 0].cpu().numpy()
     Ezz = wg.s["efield"][2, 0].cpu().numpy()
     s._mb()  # the many-body tables are built once, as in the reference init
-    n = 0
+    s.qw_calculator(Exx, Eyy, Ezz, dt, 1)  # warm-up
     t0 = time.perf_counter()
-    while True:
+    for _ in range(steps):
         s.qw_calculator(Exx, Eyy, Ezz, dt, 1)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or n >= 3:
-            break
-    per_wire_step = el / n
+    el = time.perf_counter() - t0
+    per_wire_step = el / steps
     return dict(value=1.0 / (per_wire_step * wg.nw), unit="steps/s", cores=int(threads),
-                kind="port",
-                sample=f"{n} single-wire QWCalculator step(s) of the Nk=256 workload with the "
-                       f"NumPy oracle ({per_wire_step:.2f} s/wire-step), scaled to 64 wires")
+                kind="port", cpu_model=model, physical_cores=phys, logical_cpus=logical,
+                sample=f"{steps} active single-wire QWCalculator steps of the Nk=256 workload (after one "
+                       f"warm-up step) with the NumPy oracle on {threads} BLAS threads "
+                       f"({per_wire_step:.2f} s/wire-step), scaled to {wg.nw} wires")
 
 
 def serve_sample(prec, seed, nreq=150, rate=30.0, nks=(64, 128, 256, 384, 512), steps=(100, 500),
@@ -218,6 +243,84 @@ def serve_sample(prec, seed, nreq=150, rate=30.0, nks=(64, 128, 256, 384, 512), 
     del srv
     torch.cuda.empty_cache()
     return out
+
+
+def precision_sample(seed, nw=4, nk=512, steps=10000, dt=10e-18, fwhm=20e-15, nx=1024,
+                     modes=("fp64", "fp32", "bf16x3")):
+    """Error of the reduced-precision modes against FP64 over a long, strongly
+    driven run: `nw` wires of Nk = `nk` from the initial state (Fermi
+    populations, no coherence), each driven by the sbetestprop pulse on the
+    device (psa_drive) with E0 log-uniform in [1e7, 1e8] V/m, FWHM `fwhm`,
+    centred 50 fs into the run, for `steps` leapfrog steps of `dt`.  Per
+    observable: Pxx at the middle grid point and the carrier totals sum ne,
+    sum nh every step (psa_record), max over time of |mode - FP64| over max
+    over time of |FP64|; the final p, C, D levels and Rho normwise."""
+    import ctypes
+    import torch
+    from scipy.constants import hbar
+    from pulsesuite_amd import capi
+    from pulsesuite_amd.engine import WireGroup
+    from pulsesuite_amd.tables import QWParams
+    rng = np.random.default_rng(seed)
+    e0 = 10 ** rng.uniform(7, 8, nw)
+    rr = (np.arange(nx) - nx // 2) * 1e-9
+    prm = QWParams(L=wire_length(nk), me_rel=0.067)
+    tw = fwhm / (2 * np.sqrt(np.log(2)))
+    tp = 3 * fwhm
+    omega0 = prm.gap / hbar  # centred on the gap
+    rows = [[e0[w], 2 * e0[w], 0.0, omega0, tw, tp, tp - 50e-15, 0.0] for w in range(nw)]
+    lib = capi.load()
+    res, traj = {}, {}
+    for mode in modes:
+        print(f"[bench] precision run {mode}: {nw} wires Nk={nk}, {steps} steps", file=sys.stderr, flush=True)
+        wg = WireGroup(prm, nw, rr, prec=mode)
+        wg.init_state()
+        drive = torch.tensor(rows, dtype=torch.float64, device=wg.dev)
+        trace = torch.zeros(nw, steps, 4, dtype=torch.float64, device=wg.dev)
+        count = torch.zeros(nw, dtype=torch.int32, device=wg.dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for n in range(steps):
+            capi.check(lib.psa_drive(ctypes.byref(wg.S), 0, nw, drive.data_ptr(), ctypes.c_double(dt),
+                                     wg._stream()))
+            wg.step(dt)
+            capi.check(lib.psa_record(ctypes.byref(wg.S), 0, nw, nx // 2, trace.data_ptr(), count.data_ptr(),
+                                      steps, wg._stream()))
+            if n % 2000 == 1999:
+                print(f"[bench]   {mode} step {n + 1}", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = wg.get_state()
+        traj[mode] = dict(trace=trace.cpu().numpy(), C=st[3], D=st[4], p=st[5], rho=wg.outputs()[3])
+        res[f"{mode}_ms_per_step"] = 1e3 * wall / steps
+        del wg, drive, trace, count
+        torch.cuda.empty_cache()
+    ref = traj["fp64"]
+    tr = ref["trace"]
+    res["fp64_peak_sum_ne"] = float(np.max(tr[:, :, 2]))
+    res["fp64_peak_abs_pxx"] = float(np.max(np.hypot(tr[:, :, 0], tr[:, :, 1])))
+
+    def over_time(a, b):  # a, b: [nw, steps]
+        return float(max(np.max(np.abs(a[w] - b[w])) / max(np.max(np.abs(b[w])), 1e-300) for w in range(nw)))
+
+    def normwise(a, b):  # [n, n, nw] or [nw, Nx]
+        if a.ndim == 3:
+            return float(max(np.linalg.norm(a[:, :, w] - b[:, :, w]) / max(np.linalg.norm(b[:, :, w]), 1e-300)
+                             for w in range(nw)))
+        return float(max(np.linalg.norm(a[w] - b[w]) / max(np.linalg.norm(b[w]), 1e-300) for w in range(nw)))
+
+    for mode in modes[1:]:
+        t = traj[mode]
+        res[f"{mode}_error_vs_fp64"] = dict(
+            polarization_pxx=over_time(t["trace"][:, :, 0] + 1j * t["trace"][:, :, 1], tr[:, :, 0] + 1j * tr[:, :, 1]),
+            density_sum_ne=over_time(t["trace"][:, :, 2], tr[:, :, 2]),
+            density_sum_nh=over_time(t["trace"][:, :, 3], tr[:, :, 3]),
+            p_final=normwise(t["p"], ref["p"]), C_final=normwise(t["C"], ref["C"]),
+            D_final=normwise(t["D"], ref["D"]), rho_final=normwise(t["rho"], ref["rho"]))
+    res["workload"] = (f"{nw} wires x Nk={nk}, {steps} leapfrog steps of {dt * 1e18:.0f} as from the initial state, "
+                       f"sbetestprop pulse E0 = {', '.join(f'{x:.2e}' for x in e0)} V/m (log-uniform 1e7-1e8), "
+                       f"FWHM {fwhm * 1e15:.0f} fs, peak 50 fs into the run, Nx={nx}")
+    return res
 
 
 def transmission_sample(seed, nw=16, nk=256, nx=4096, npts=4096, steps=20, warmup=3, dt=10e-18):
@@ -307,13 +410,17 @@ def uneven_sample(seed, sizes=(64, 128, 192, 256), per=16, nx=4096, steps=20, wa
     """64 wires of uneven Nk (16 each of 64, 128, 192, 256), as one wire group
     per Nk: HIP-event ms per step with every group stepped on its own
     (psa_step) and with the groups' RHS as one persistent launch over the
-    per-XCD work queues (psa_step_multi), in FP64 and split-BF16."""
+    per-XCD work queues (psa_step_multi), in FP64, FP32 and split-BF16.  The
+    modes run the same step sequence from the same seeded state, so each
+    reduced mode's error against FP64 is stated per Nk after the 2 (warmup +
+    steps) steps: the densities (diagonal of C), Pxx and the p level."""
     import torch
     from pulsesuite_amd.engine import MultiStep, WireGroup
     from pulsesuite_amd.tables import QWParams
     out = dict(workload=f"{per * len(sizes)} wires, {per} each of Nk = "
                + ", ".join(str(n) for n in sizes) + f", Nx={nx}")
-    for prec in ("fp64", "bf16x3"):
+    final = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
         rng = np.random.default_rng(seed)
         gs = []
         for nk in sizes:
@@ -340,8 +447,17 @@ def uneven_sample(seed, sizes=(64, 128, 192, 256), per=16, nx=4096, steps=20, wa
             ev[1].record()
             torch.cuda.synchronize()
             out[f"{prec}_{name}_ms"] = ev[0].elapsed_time(ev[1]) / steps
+        final[prec] = [(g.get_state(), g.outputs()[0]) for g in gs]
         del gs, ms
         torch.cuda.empty_cache()
+    rel = lambda x, y: float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))  # noqa: E731
+    for prec in ("fp32", "bf16x3"):
+        err = {}
+        for nk, (st, px), (st0, px0) in zip(sizes, final[prec], final["fp64"]):
+            ne = np.real(np.einsum("iiw->iw", st[3]))
+            ne0 = np.real(np.einsum("iiw->iw", st0[3]))
+            err[str(nk)] = dict(density=rel(ne, ne0), polarization=rel(px, px0), p_matrix=rel(st[5], st0[5]))
+        out[f"{prec}_error_vs_fp64_after_{2 * (warmup + steps)}_steps"] = err
     return out
 
 
@@ -363,6 +479,12 @@ def main():
     ap.add_argument("--no-rk4", action="store_true")
     ap.add_argument("--no-qw3d", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-wires", type=int, default=4)
+    ap.add_argument("--precision-nk", type=int, default=512)
+    ap.add_argument("--precision-steps", type=int, default=10000)
+    ap.add_argument("--precision-only", action="store_true",
+                    help="run only the long-run precision leg and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only (the SBE wire batch is stepped on a single "
@@ -371,6 +493,10 @@ def main():
     import torch
     dt = 10e-18
     seed = 1234
+    if args.precision_only:
+        print(json.dumps(precision_sample(seed, nw=args.precision_wires, nk=args.precision_nk,
+                                          steps=args.precision_steps)), flush=True)
+        return
     modes = ["fp64"] + ([] if args.no_mixed else ["fp32", "bf16x3"])
     if args.modes:
         modes = ["fp64"] + [m for m in args.modes.split(",") if m in ("fp32", "bf16x3")]
@@ -453,6 +579,13 @@ def main():
             res["qw3d"] = qw3d_sample(seed)
         except Exception as ex:  # report, keep the batch number
             res["qw3d"] = {"error": repr(ex)}
+    if not args.no_precision:
+        try:
+            print("[bench] long-run precision", file=sys.stderr, flush=True)
+            res["precision"] = precision_sample(seed, nw=args.precision_wires, nk=args.precision_nk,
+                                                steps=args.precision_steps)
+        except Exception as ex:  # report, keep the batch number
+            res["precision"] = {"error": repr(ex)}
     if not args.no_uneven:
         try:
             print("[bench] uneven Nk (persistent multi-group RHS)", file=sys.stderr, flush=True)
@@ -479,7 +612,9 @@ def main():
         "config": {"workload": f"{args.wires} quantum wires x Nk={args.nk}, Nx={args.nx} "
                                "propagation grid, shipped mb.params, dt=10 as, "
                                "one QWCalculator step of every wire per step",
-                   "wires": args.wires, "Nk": N, "Nx": args.nx},
+                   "wires": args.wires, "Nk": N, "Nx": args.nx,
+                   "serving": (f"a sample of the 1000-request stream: {args.serve_requests} requests with "
+                               "100-500 active steps (the stream names 1k-20k)")},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "roofline": {"bound": "mfma",
                      "kernel": f"rhs (dp/dC/dD + leapfrog), {best} contractions",

@@ -713,6 +713,33 @@ def mb_tables(g, kind, ge, gh, lorentz=False):
     return S
 
 
+# The many-body sums below run over the non-zero entries of each term only:
+# the 3-D weight of a term (the stored table read at the loop indices, zero
+# where the shifted index leaves [0, N)) does not change during a run, so its
+# non-zero entries are found once per table and kept.  Each entry's product
+# is formed in the reference's factor order; the per-k sums are taken in
+# entry order (np.bincount) instead of numpy's pairwise order.
+_MB_TERMS = {}
+
+
+def _mb_term(S, key, build):
+    """Entries (out, i1, i2, kk, w) of one term of table S, cached per table
+    object (the table itself is held so that its id stays valid)."""
+    hit = _MB_TERMS.get((id(S), key))
+    if hit is None or hit[0] is not S:
+        W, kk, out_axis = build()
+        nz = np.nonzero(W)
+        idx = [np.broadcast_to(a, W.shape)[nz].astype(np.int64) for a in np.ix_(*(np.arange(n) for n in W.shape))]
+        hit = (S, (idx[out_axis], idx[1], idx[2], np.broadcast_to(kk, W.shape)[nz].astype(np.int64), W[nz]))
+        _MB_TERMS[(id(S), key)] = hit
+    return hit[1]
+
+
+def _term_weight(S, sel, shifted, N):
+    ok = (shifted >= 0) & (shifted < N)
+    return np.where(ok, S[sel], 0.0), np.clip(shifted, 0, N - 1)
+
+
 def mbce(g, S_eh, S_ee, ne0, nh0, VC):
     """coulomb.py:956-1012, 1298-1311 (reads the stored tables with the
     0-based loop indices, i.e. shifted by one)."""
@@ -721,72 +748,52 @@ def mbce(g, S_eh, S_ee, ne0, nh0, VC):
     nh = np.abs(nh0)
     Veh2 = VC[:, :, 0] ** 2
     Vee2 = VC[:, :, 1] ** 2
-    Win = np.zeros(N)
-    Wout = np.zeros(N)
     K = np.arange(N)[:, None, None]
     P = np.arange(N)[None, :, None]
     Q = np.arange(N)[None, None, :]
     # term A: k1p = P+Q-K
-    k1p = P + Q - K
-    ok = (k1p >= 0) & (k1p < N)
-    kk = np.clip(k1p, 0, N - 1)
-    Win += np.sum(np.where(ok, Veh2[K, Q] * (1.0 - nh[P]) * nh[kk] * ne[Q]
-                           * S_eh[K, P, Q], 0.0), axis=(1, 2))
+    k, p, q, kk, w = _mb_term(S_eh, "eA", lambda: _term_weight(S_eh, (K, P, Q), P + Q - K, N) + (0,))
+    Win = np.bincount(k, Veh2[k, q] * (1.0 - nh[p]) * nh[kk] * ne[q] * w, minlength=N)
     # term B: k1p = P+K-Q
-    k1p = P + K - Q
-    ok = (k1p >= 0) & (k1p < N)
-    kk = np.clip(k1p, 0, N - 1)
-    Wout += np.sum(np.where(ok, Veh2[Q, K] * (1.0 - ne[Q]) * (1.0 - nh[P])
-                            * nh[kk] * S_eh[Q, P, K], 0.0), axis=(1, 2))
+    k, p, q, kk, w = _mb_term(S_eh, "eB", lambda: _term_weight(S_eh, (Q, P, K), P + K - Q, N) + (0,))
+    Wout = np.bincount(k, Veh2[q, k] * (1.0 - ne[q]) * (1.0 - nh[p]) * nh[kk] * w, minlength=N)
     # term C (P plays k2, Q plays k4): k30 = K+P-Q
-    k30 = K + P - Q
-    ok = (k30 >= 0) & (k30 < N)
-    kk = np.clip(k30, 0, N - 1)
-    Win += np.sum(np.where(ok, Vee2[K, Q] * (1.0 - ne[P]) * ne[kk] * ne[Q]
-                           * S_ee[K, P, Q], 0.0), axis=(1, 2))
+    k, p, q, kk, w = _mb_term(S_ee, "eC", lambda: _term_weight(S_ee, (K, P, Q), K + P - Q, N) + (0,))
+    Win += np.bincount(k, Vee2[k, q] * (1.0 - ne[p]) * ne[kk] * ne[q] * w, minlength=N)
     # term D: k30 = Q+P-K
-    k30 = Q + P - K
-    ok = (k30 >= 0) & (k30 < N)
-    kk = np.clip(k30, 0, N - 1)
-    Wout += np.sum(np.where(ok, Vee2[Q, K] * (1.0 - ne[Q]) * (1.0 - ne[P])
-                            * ne[kk] * S_ee[Q, P, K], 0.0), axis=(1, 2))
+    k, p, q, kk, w = _mb_term(S_ee, "eD", lambda: _term_weight(S_ee, (Q, P, K), Q + P - K, N) + (0,))
+    Wout += np.bincount(k, Vee2[q, k] * (1.0 - ne[q]) * (1.0 - ne[p]) * ne[kk] * w, minlength=N)
     return Win, Wout
 
 
 def mbch(g, S_eh, S_hh, ne0, nh0, VC):
-    """coulomb.py:1015-1071, 1313-1326."""
+    """coulomb.py:1015-1071, 1313-1326 (sums over axes 1 and 2 into the
+    axis-0 index KP)."""
     N = g.Nk
     ne = np.abs(ne0)
     nh = np.abs(nh0)
     Veh2 = VC[:, :, 0] ** 2
     Vhh2 = VC[:, :, 2] ** 2
-    Win = np.zeros(N)
-    Wout = np.zeros(N)
     KP = np.arange(N)[:, None, None]
     K = np.arange(N)[None, :, None]
     Q = np.arange(N)[None, None, :]
-    k1p = KP + Q - K
-    ok = (k1p >= 0) & (k1p < N)
-    kk = np.clip(k1p, 0, N - 1)
-    Win += np.sum(np.where(ok, Veh2[K, Q] * (1.0 - ne[K]) * nh[kk] * ne[Q]
-                           * S_eh[K, KP, Q], 0.0), axis=(1, 2))
-    k1p = KP + K - Q
-    ok = (k1p >= 0) & (k1p < N)
-    kk = np.clip(k1p, 0, N - 1)
-    Wout += np.sum(np.where(ok, Veh2[Q, K] * (1.0 - ne[K]) * (1.0 - nh[kk])
-                            * ne[Q] * S_eh[K, kk, Q], 0.0), axis=(1, 2))
-    B2 = K
-    F = Q
-    k3p = KP + B2 - F
-    ok = (k3p >= 0) & (k3p < N)
-    kk = np.clip(k3p, 0, N - 1)
-    Win += np.sum(np.where(ok, Vhh2[KP, F] * (1.0 - nh[B2]) * nh[kk] * nh[F]
-                           * S_hh[KP, B2, F], 0.0), axis=(1, 2))
-    k3p = F + B2 - KP
-    ok = (k3p >= 0) & (k3p < N)
-    kk = np.clip(k3p, 0, N - 1)
-    Wout += np.sum(np.where(ok, Vhh2[F, KP] * (1.0 - nh[F]) * (1.0 - nh[B2])
-                            * nh[kk] * S_hh[F, B2, KP], 0.0), axis=(1, 2))
+    # k1p = KP+Q-K
+    o, k, q, kk, w = _mb_term(S_eh, "hA", lambda: _term_weight(S_eh, (K, KP, Q), KP + Q - K, N) + (0,))
+    Win = np.bincount(o, Veh2[k, q] * (1.0 - ne[k]) * nh[kk] * ne[q] * w, minlength=N)
+    # k1p = KP+K-Q, the table read at (K, k1p, Q)
+    def build_b():
+        k1p = KP + K - Q
+        ok = (k1p >= 0) & (k1p < N)
+        kk = np.clip(k1p, 0, N - 1)
+        return np.where(ok, S_eh[K, kk, Q], 0.0), kk, 0
+    o, k, q, kk, w = _mb_term(S_eh, "hB", build_b)
+    Wout = np.bincount(o, Veh2[q, k] * (1.0 - ne[k]) * (1.0 - nh[kk]) * ne[q] * w, minlength=N)
+    # B2 = K, F = Q: k3p = KP+B2-F
+    o, b2, f, kk, w = _mb_term(S_hh, "hC", lambda: _term_weight(S_hh, (KP, K, Q), KP + K - Q, N) + (0,))
+    Win += np.bincount(o, Vhh2[o, f] * (1.0 - nh[b2]) * nh[kk] * nh[f] * w, minlength=N)
+    # k3p = F+B2-KP
+    o, b2, f, kk, w = _mb_term(S_hh, "hD", lambda: _term_weight(S_hh, (Q, K, KP), Q + K - KP, N) + (0,))
+    Wout += np.bincount(o, Vhh2[f, o] * (1.0 - nh[f]) * (1.0 - nh[b2]) * nh[kk] * w, minlength=N)
     return Win, Wout
 
 
