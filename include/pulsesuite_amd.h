@@ -66,10 +66,11 @@ typedef struct psa_group {
   int32_t bx0, bnx; /* rows of rr inside the wire for QW2Prop: [bx0, bx0+bnx) */
   int32_t nnz_mb[8];/* entries in each many-body scattering list */
   int32_t tune[4];  /* launch choices, 0 = default: [0] reserved (0),
-                       [1] transport lines per block, row pass (0: 4, else 2, 4 or 8),
+                       [1] transport lines per block, row pass (0: 4, else 2, 4, 8 or 16),
                        [2] GEMM block order (0: batches grouped per XCD, 1: linear),
-                       [3] transport lines per block, column pass (0: 4, else 2, 4 or 8;
-                       N known at compile time: 64, 96, 128, 192, 256, 384, 512) */
+                       [3] transport lines per block, column pass (0: 4, else 2, 4, 8 or 16;
+                       16 and the column choice only for N known at compile time: 64, 96,
+                       128, 192, 256, 384, 512; other N take 4 for 16) */
   int32_t pad0;
   /* ---- scalars (SI units) ---- */
   double hbar, e0, gap, L, ehint, dcv, area, dkr, dr_r, drr;

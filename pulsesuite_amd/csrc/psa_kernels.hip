@@ -650,6 +650,7 @@ __global__ void k_wire_tables(psa_group g, int w0) {
 constexpr int RL_WAVES = 16;
 __global__ __launch_bounds__(64 * RL_WAVES) void k_relax(psa_group g, int w0, double dt) {
   __shared__ double s_ner[1024], s_nhr[1024];
+  __shared__ double s_vep[1024], s_vhp[1024];  // VC / E1D * Vscale per |k - i|, electrons and holes
   __shared__ double part[4][RL_WAVES][64];
   const int w = w0 + blockIdx.x, N = g.N, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6, k = blockIdx.y * 64 + lane;
@@ -663,18 +664,23 @@ __global__ __launch_bounds__(64 * RL_WAVES) void k_relax(psa_group g, int w0, do
   for (int i = tid; i < N; i += blockDim.x) {
     s_ner[i] = ld(C3, (long)i * N + i).re;
     s_nhr[i] = ld(D3, (long)i * N + i).re;
+    // the interaction factors depend on |k - i| only: one division per distance
+    s_vep[i] = vc[N + i] / E1D[i] * g.Vscale;
+    s_vhp[i] = vc[2 * N + i] / E1D[i] * g.Vscale;
   }
   __syncthreads();
   double a = 0, b = 0, c = 0, d = 0;
   if (phon && k < N) {
     // Vep[k][i] = VC[k,i,1] / E1D[k,i] * Vscale, VC lower-triangular
+#pragma unroll 4
     for (int i = wv; i <= k; i += RL_WAVES) {
-      const double vep = vc[N + k - i] / E1D[k - i] * g.Vscale;
+      const double vep = s_vep[k - i];
       a += vep * s_ner[i] * g.EPT[(long)i * N + k];
       b += vep * (1.0 - s_ner[i]) * g.EP[(long)i * N + k];
     }
+#pragma unroll 4
     for (int i = k + wv; i < N; i += RL_WAVES) {
-      const double vhp = vc[2 * N + i - k] / E1D[i - k] * g.Vscale;
+      const double vhp = s_vhp[i - k];
       c += vhp * s_nhr[i] * g.HPT[(long)i * N + k];
       d += vhp * (1.0 - s_nhr[i]) * g.HP[(long)i * N + k];
     }
@@ -1437,10 +1443,13 @@ __global__ __launch_bounds__(256) void k_src_norm(psa_group g, int w0) {
   const double vee0 = s_vee[0], vhh0 = s_vhh[0];
   for (int k = tid; k < N; k += blockDim.x) {
     double se = 0, shh = 0;
+    // unrolled so that the LDS reads of several i are in flight together
+#pragma unroll 8
     for (int i = 0; i <= k; ++i) {  // V[k][i] nonzero for i <= k (EkReNorm, dcfield.py:338-345)
       se += s_ne2[i] * (vee0 - s_vee[k - i]);
       shh += s_nh2[i] * (vhh0 - s_vhh[k - i]);
     }
+#pragma unroll 8
     for (int i = k + 1; i < N; ++i) {
       se += s_ne2[i] * vee0;
       shh += s_nh2[i] * vhh0;
@@ -2162,6 +2171,7 @@ int launch_transport_fpass(const psa_group *g, int32_t w0, int nw, hipStream_t s
   switch (pass == 0 ? g->tune[1] : g->tune[3]) {
     case 2: return launch_transport_fp<NT, 2>(g, w0, nw, s, pass, avg);
     case 8: return launch_transport_fp<NT, 8>(g, w0, nw, s, pass, avg);
+    case 16: return launch_transport_fp<NT, 16>(g, w0, nw, s, pass, avg);
     default: return launch_transport_fp<NT, 4>(g, w0, nw, s, pass, avg);
   }
 }

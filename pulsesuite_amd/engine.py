@@ -254,7 +254,7 @@ class WireGroup:
         """Pick the launch choices of this group by timing them on its own
         buffers: tune[2] GEMM and split-BF16 RHS block order (XCD-grouped or
         linear), tune[1] / tune[3] transport lines per
-        block for the row / column pass (4, 2, 8), timed inside full steps
+        block for the row / column pass (4, 2, 8, 16), timed inside full steps
         (the column pass carries the leapfrog average there).  The state is
         unchanged: only per-step scratch is written, and the averaged level
         that the steps overwrite is saved and restored.  Returns {knob:
@@ -277,8 +277,8 @@ class WireGroup:
         saved = None
         if self.prm.flags.get("DCTrans") and self.prm.flags.get("LF"):
             saved = {k: self.s[k].clone() for k in ("pA", "CA", "DA")}
-            knobs.append((1, [4, 2, 8], full_step))
-            knobs.append((3, [4, 2, 8], full_step))
+            knobs.append((1, [4, 2, 8, 16], full_step))
+            knobs.append((3, [4, 2, 8, 16], full_step))
         out = {}
         for idx, cands, fn in knobs:
             times = []
