@@ -240,13 +240,26 @@ __global__ __launch_bounds__(64 * DS_WAVES) void k_diagsums(psa_group g, int w0,
     st(ds, 3 * Q + si, t[3]);
   }
 }
-__global__ __launch_bounds__(256) void k_prep_b(psa_group g, int w0) {
+// Block = PB_WAVES rates k of one wire (a wave per k); the wire's
+// occupations (real parts) and screened interactions are staged in LDS once
+// per block, so the index-table gathers read LDS.
+constexpr int PB_WAVES = 16;
+__global__ __launch_bounds__(64 * PB_WAVES) void k_prep_b(psa_group g, int w0) {
+  __shared__ double s_ne[1024], s_nh[1024], s_vc[3 * 1024];
   const int w = w0 + blockIdx.x, N = g.N, lane = threadIdx.x & 63;
-  const int k = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (k >= N) return;  // whole wave; only wave-level reductions below
+  const int k = blockIdx.y * PB_WAVES + (threadIdx.x >> 6);
   double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-  const double *nev = pr + PSA_PREP_NE, *nhv = pr + PSA_PREP_NH(N);
-  const double *vc = pr + PSA_PREP_VC(N);
+  {
+    const double *nev = pr + PSA_PREP_NE, *nhv = pr + PSA_PREP_NH(N), *vcg = pr + PSA_PREP_VC(N);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      s_ne[i] = nev[2 * i];
+      s_nh[i] = nhv[2 * i];
+    }
+    for (int i = threadIdx.x; i < 3 * N; i += blockDim.x) s_vc[i] = vcg[i];
+  }
+  __syncthreads();
+  if (k >= N) return;  // whole wave; only wave-level reductions below
+  const double *vc = s_vc;
   double *GE = pr + PSA_PREP_GAME(N), *GH = pr + PSA_PREP_GAMH(N), *Rsp = pr + PSA_PREP_RSP(N);
   if (!(g.flags & PSA_F_DIAGDPH)) {
     if (lane == 0) {
@@ -267,20 +280,20 @@ __global__ __launch_bounds__(256) void k_prep_b(psa_group g, int w0) {
       const double Vee = vc[N + iq], Veh = vc[iq], Vhh = vc[2 * N + iq];
       const int i = kpq[qk];
       if (i >= 0 && i < N) {
-        const double ni = nev[2 * i], hi = nhv[2 * i];
+        const double ni = s_ne[i], hi = s_nh[i];
         ge += ph * (Vee * Vee) * ni * (1.0 - ni) * fabs(g.xe[q]);
         gh += ph * (Vhh * Vhh) * hi * (1.0 - hi) * fabs(g.xh[q]);
       }
       const int c = kmq[qk];
       int a = k1pmq[qk], b = k1p[qk];
       if (a >= 0 && a < N && b >= 0 && b < N && c >= 0 && c < N) {
-        const double na = nhv[2 * a], nb = nhv[2 * b], ec = nev[2 * c];
+        const double na = s_nh[a], nb = s_nh[b], ec = s_ne[c];
         ge += ph * (Veh * Veh) * (na * (1.0 - nb) * ec + (1.0 - na) * nb * (1.0 - ec)) * fabs(g.xh[q]);
       }
       a = k1mq[qk];
       b = k1[qk];
       if (a >= 0 && a < N && b >= 0 && b < N && c >= 0 && c < N) {
-        const double ea = nev[2 * a], eb = nev[2 * b], hc = nhv[2 * c];
+        const double ea = s_ne[a], eb = s_ne[b], hc = s_nh[c];
         gh += ph * (Veh * Veh) * (ea * (1.0 - eb) * hc + (1.0 - ea) * eb * (1.0 - hc)) * fabs(g.xe[q]);
       }
     }
@@ -299,7 +312,7 @@ __global__ __launch_bounds__(256) void k_prep_b(psa_group g, int w0) {
   const double veh0 = vc[0], vee0 = vc[N], vhh0 = vc[2 * N];
   double s1 = 0.0;
   for (int i = lane; i < N; i += 64) {
-    const double ne = nev[2 * i], nh = nhv[2 * i];
+    const double ne = s_ne[i], nh = s_nh[i];
     const double vee_ik = i >= k ? vc[N + i - k] : 0.0;
     const double vhh_ik = i >= k ? vc[2 * N + i - k] : 0.0;
     s1 += ne * (vee0 - vee_ik) + nh * (vhh0 - vhh_ik);
@@ -1882,7 +1895,7 @@ int psa_prep(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   PSA_LAUNCH_CHECK("k_prep_a");
   hipLaunchKernelGGL(k_diagsums, dim3(nw, (2 * g->N - 1 + 63) / 64), dim3(64 * DS_WAVES), 0, s, *g, w0, 0);
   PSA_LAUNCH_CHECK("k_diagsums");
-  hipLaunchKernelGGL(k_prep_b, dim3(nw, (g->N + 3) / 4), dim3(256), 0, s, *g, w0);
+  hipLaunchKernelGGL(k_prep_b, dim3(nw, (g->N + PB_WAVES - 1) / PB_WAVES), dim3(64 * PB_WAVES), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_prep_b");
   hipLaunchKernelGGL(k_wire_tables, dim3(nw), dim3(256), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_wire_tables");
