@@ -1969,8 +1969,9 @@ int psa_lorentz_table(const psa_group *g, void *stream) {
 
 int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
-  // FP64, or FP32 MFMA in the reduced-precision modes on 2^100-scaled
-  // occupation columns (positive terms only: no cancellation)
+  // FP64, or (reduced-precision modes) FP32 MFMA or split-BF16 MFMA on
+  // 2^100-scaled occupation columns (positive terms only: no cancellation);
+  // the stored-table form runs FP32 in both reduced modes
   const int prec = g->prec == PSA_PREC_FP64 ? PSA_PREC_FP64 : PSA_PREC_FP32;
   const hipStream_t s = (hipStream_t)stream;
   if (!g->lor) {
