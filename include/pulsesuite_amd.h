@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PSA_ABI_VERSION 3
+#define PSA_ABI_VERSION 4
 
 #define PSA_OK 0
 #define PSA_E_INVALID (-1)   /* bad argument / shape */
@@ -67,7 +67,9 @@ typedef struct psa_group {
   int32_t nnz_mb[8];/* entries in each many-body scattering list */
   int32_t tune[4];  /* launch choices, 0 = default: [0] reserved (0),
                        [1] transport lines per block, row pass (0: 4, else 2, 4, 8 or 16),
-                       [2] GEMM block order (0: batches grouped per XCD, 1: linear),
+                       [2] GEMM block order (0: batches grouped per XCD, 1: linear; 2: for the
+                       split-BF16 RHS, grouped per XCD with every dp tile before the dC and dD
+                       tiles, for the GEMMs linear),
                        [3] transport lines per block, column pass (0: 4, else 2, 4, 8 or 16;
                        16 and the column choice only for N known at compile time: 64, 96,
                        128, 192, 256, 384, 512; other N take 4 for 16) */
@@ -136,6 +138,12 @@ typedef struct psa_group {
   const int32_t *wid_host;            /* [Nw] host copy of wire_id */
   double *pR, *CR, *DR;               /* [Nw][N][N] complex RK4 stage accumulators (psa_step_rk4,
                                          psa_rhs modes 2-5 only; NULL otherwise) */
+  uint64_t *wtag;                     /* [Nw][2] per-wire reuse tags, zero-initialised, owned by the
+                                         library (NULL: nothing is reused): [0] the device address of
+                                         the p buffer (+ the wire's offset) whose p, C, D planes spl
+                                         holds, [1] that of the C buffer whose diagonal sums the src
+                                         block holds.  A caller that writes p, C or D of a wire other
+                                         than through the step entry points zeroes both of its tags. */
 } psa_group;
 
 /* layout of the per-wire prep block (doubles) */

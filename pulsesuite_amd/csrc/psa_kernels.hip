@@ -162,6 +162,17 @@ __global__ __launch_bounds__(64 * DS_WAVES) void k_diagsums(psa_group g, int w0,
   const int kmin = max(0, -(s0 + 63)), kmax = N - max(0, s0);
   const long o = mat_off(g, w);
   constexpr int ST = DS_WAVES;
+  if (mode == 0 && g.wtag && g.wtag[2 * w + 1] == (uint64_t)(g.CB + o)) {
+    // the previous step's sources summed this level already (mode 1 sums
+    // C3 and conj(D3) in the same order): copy them (block-uniform)
+    if (wv == 0 && si < Q) {
+      const double *ds = g.src + (long)w * PSA_SRC_STRIDE(N, g.Nr) + PSA_SRC_DS(g.Nr);
+      double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+      st(pr + PSA_PREP_DSC(N), si, ld(ds, 2 * Q + si));
+      st(pr + PSA_PREP_DSD(N), si, cconj(ld(ds, 3 * Q + si)));
+    }
+    return;
+  }
   cd a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0}, a3 = {0, 0};
   if (mode == 0) {
     const double *C2 = g.CB + o, *D2 = g.DB + o;
@@ -238,6 +249,9 @@ __global__ __launch_bounds__(64 * DS_WAVES) void k_diagsums(psa_group g, int w0,
     st(ds, Q + si, cscale(t[1], g.dcv));
     st(ds, 2 * Q + si, t[2]);
     st(ds, 3 * Q + si, t[3]);
+    // src now holds the diagonal sums of C and D of this level (read by the
+    // next step's prep once the level has become its level 3)
+    if (blockIdx.y == 0 && lane == 0 && g.wtag) g.wtag[2 * w + 1] = (uint64_t)(g.CX + o);
   }
 }
 // Block = PB_WAVES rates k of one wire (a wave per k); the wire's
@@ -1243,8 +1257,12 @@ __device__ __forceinline__ void fused_middle(cd *x, const cd *tw, int lane, cons
   wave_lds_sync();
 }
 
+// Column pass with `planes` (split-BF16 step, LINES 8 or 16): the final
+// p, C, D of the new level also leave as the bf16 hi/lo operand planes of
+// the next step's RHS (the layout of sp::k_split, both orientations, C and
+// D with their diagonals zeroed), and the p tag of the wire records it.
 template <int NT, int LINES>
-__global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0, int pass, int avg) {
+__global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0, int pass, int avg, int planes) {
   extern __shared__ double smem[];
   constexpr int N = NT, S = radix_plan(N).S;
   constexpr int RL = plan_r(N, 0, S - 1), ITL = (N / RL + 63) / 64;  // the fused radix
@@ -1254,7 +1272,12 @@ __global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0,
   const int LS = tr_line_stride_s(N, LINES);
   const int mat = blockIdx.y;  // 0 C, 1 D, 2 p
   const int w = w0 + blockIdx.z;
-  const int line0 = blockIdx.x * LINES, line = line0 + wv;
+  // column pass with planes: the blocks of neighbouring columns run on one
+  // XCD (block i runs on XCD i % 8), so the partial 64-byte rows of the
+  // matrix-orientation planes that they write meet in one L2
+  int bx = blockIdx.x;
+  if (pass == 1 && planes && gridDim.x % 8 == 0) bx = (bx % 8) * (gridDim.x / 8) + bx / 8;
+  const int line0 = bx * LINES, line = line0 + wv;
   cd *base = reinterpret_cast<cd *>(smem);
   cd *tw = base + LINES * LS;
   cd *x = base + wv * LS;
@@ -1333,6 +1356,54 @@ __global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0,
           const cd a = ld(A, ij);
           st(A, ij, cd{(a.re + v.re) / 2.0, (a.im + v.im) / 2.0});
         }
+      }
+    }
+    if constexpr (LINES % 8 == 0 && NT % LINES == 0) {
+      if (planes) {
+        // items: (row j, 8-column group) of the matrix slot, then (column,
+        // 8-row group) of the transposed slot; 16-byte stores of 8 k
+        const int sm = mat == 2 ? 0 : mat + 1;  // slot matrix: p, C_o, D_o
+        const bool zd = mat < 2;
+        uint16_t *spl = reinterpret_cast<uint16_t *>(g.spl);
+        constexpr int NA = N * (LINES / 8), NB = LINES * (N / 8);
+        for (int it = tid; it < NA + NB; it += blockDim.x) {
+          int row, k0, cb, jb, dc, dj;  // value q at line cb + q dc, row jb + q dj
+          if (it < NA) {
+            row = it / (LINES / 8);
+            const int gq = it % (LINES / 8);
+            k0 = line0 + 8 * gq;
+            cb = 8 * gq, jb = row, dc = 1, dj = 0;
+          } else {
+            const int u = it - NA, c = u % LINES;
+            row = line0 + c;
+            k0 = 8 * (u / LINES);
+            cb = c, jb = k0, dc = 0, dj = 1;
+          }
+          sp::u32x4 ph[4];
+#pragma unroll
+          for (int q = 0; q < 8; q += 2) {
+            cd v2[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+              const int cc = cb + (q + r) * dc, jj = jb + (q + r) * dj;
+              v2[r] = cscale(base[cc * LS + fswz(jj)], inv);
+              if (zd && jj == line0 + cc) v2[r] = cd{0.0, 0.0};
+            }
+            uint32_t rh, rl, ih, il;
+            sp::split_pair(v2[0].re, v2[1].re, rh, rl);
+            sp::split_pair(v2[0].im, v2[1].im, ih, il);
+            ph[0][q >> 1] = rh;
+            ph[1][q >> 1] = rl;
+            ph[2][q >> 1] = ih;
+            ph[3][q >> 1] = il;
+          }
+          const int slot = 2 * sm + (it < NA ? 0 : 1);
+          const long cbase = sp::chunk(N, w, slot, row >> 6, k0 >> 5);
+#pragma unroll
+          for (int pl = 0; pl < 4; ++pl)
+            *reinterpret_cast<sp::u32x4 *>(&spl[cbase + sp::in_chunk(pl, row, k0)]) = ph[pl];
+        }
+        if (bx == 0 && mat == 2 && tid == 0) g.wtag[2 * w] = (uint64_t)(g.pX + o);
       }
     }
   }
@@ -1804,7 +1875,7 @@ constexpr int queue_tile() {
 }
 
 template <int KIND>  // 0: FP64 products, 1: FP32, 2: split-BF16
-__global__ __launch_bounds__(256) void k_rhs_queue(const RhsProb *__restrict__ probs, const WireEnt *__restrict__ ents,
+__global__ __launch_bounds__(256, 2) void k_rhs_queue(const RhsProb *__restrict__ probs, const WireEnt *__restrict__ ents,
                                                    RhsQueue q, unsigned *queue) {
   // Dynamic LDS only: the operand tiles at offset 0 (aligned as the tile
   // bodies assume), the broadcast item index after them.
@@ -2163,7 +2234,7 @@ namespace {
 
 // one pass of the compile-time-N transport (0 rows, 1 columns) with LINES lines per block
 template <int NT, int LINES>
-int launch_transport_fp(const psa_group *g, int32_t w0, int nw, hipStream_t s, int pass, bool avg) {
+int launch_transport_fp(const psa_group *g, int32_t w0, int nw, hipStream_t s, int pass, bool avg, bool planes) {
   const size_t shm = ((size_t)LINES * tr_line_stride_s(NT, LINES) + plan_twiddles(NT)) * sizeof(cd);
   static bool attr_set = false;  // per instantiation
   if (shm > 65536 && !attr_set) {
@@ -2175,18 +2246,18 @@ int launch_transport_fp(const psa_group *g, int32_t w0, int nw, hipStream_t s, i
   }
   const dim3 grid((NT + LINES - 1) / LINES, 3, nw);
   hipLaunchKernelGGL((k_transport_f<NT, LINES>), grid, dim3(64 * LINES), shm, s, *g, w0, pass,
-                     (pass == 1 && avg) ? 1 : 0);
+                     (pass == 1 && avg) ? 1 : 0, (pass == 1 && planes && LINES % 8 == 0) ? 1 : 0);
   PSA_LAUNCH_CHECK("k_transport_f");
   return PSA_OK;
 }
 
 template <int NT>
-int launch_transport_fpass(const psa_group *g, int32_t w0, int nw, hipStream_t s, int pass, bool avg) {
+int launch_transport_fpass(const psa_group *g, int32_t w0, int nw, hipStream_t s, int pass, bool avg, bool planes) {
   switch (pass == 0 ? g->tune[1] : g->tune[3]) {
-    case 2: return launch_transport_fp<NT, 2>(g, w0, nw, s, pass, avg);
-    case 8: return launch_transport_fp<NT, 8>(g, w0, nw, s, pass, avg);
-    case 16: return launch_transport_fp<NT, 16>(g, w0, nw, s, pass, avg);
-    default: return launch_transport_fp<NT, 4>(g, w0, nw, s, pass, avg);
+    case 2: return launch_transport_fp<NT, 2>(g, w0, nw, s, pass, avg, planes);
+    case 8: return launch_transport_fp<NT, 8>(g, w0, nw, s, pass, avg, planes);
+    case 16: return launch_transport_fp<NT, 16>(g, w0, nw, s, pass, avg, planes);
+    default: return launch_transport_fp<NT, 4>(g, w0, nw, s, pass, avg, planes);
   }
 }
 
@@ -2215,11 +2286,11 @@ int launch_transport_l(const psa_group *g, int32_t w0, int nw, hipStream_t s, bo
 }
 
 template <int NT>
-int launch_transport(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg) {
+int launch_transport(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool avg, bool planes) {
   if constexpr (NT > 0) {
     // row pass with tune[1] lines per block, column pass with tune[3]
-    if (int rc = launch_transport_fpass<NT>(g, w0, nw, s, 0, avg)) return rc;
-    return launch_transport_fpass<NT>(g, w0, nw, s, 1, avg);
+    if (int rc = launch_transport_fpass<NT>(g, w0, nw, s, 0, avg, false)) return rc;
+    return launch_transport_fpass<NT>(g, w0, nw, s, 1, avg, planes);
   } else {
     switch (g->tune[1]) {
       case 2: return launch_transport_l<NT, 2>(g, w0, nw, s, avg);
@@ -2229,7 +2300,12 @@ int launch_transport(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool
   }
 }
 
-int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStream_t s, bool avg) {
+// planes: the leapfrog step in split-BF16 mode may leave the next step's
+// p, C, D operand planes behind (compile-time N, 8 or 16 column lines per
+// block); otherwise the next RHS splits them itself.
+int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStream_t s, bool avg,
+                   bool planes = false) {
+  planes = planes && g->prec == PSA_PREC_BF16X3 && g->spl && g->wtag;
   if (!(g->flags & PSA_F_LF)) return fail(PSA_E_UNSUPPORTED, "Transport with LF = 0 (diagonal-only shift) is not supported");
   if (g->N > 512) return fail(PSA_E_UNSUPPORTED, "Transport needs N <= 512 (LDS lines)");
   int n = g->N;
@@ -2239,14 +2315,14 @@ int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStr
   hipLaunchKernelGGL(k_tphase, dim3(nw), dim3(256), 0, s, *g, w0, dt);
   PSA_LAUNCH_CHECK("k_tphase");
   switch (g->N) {
-    case 64: return launch_transport<64>(g, w0, nw, s, avg);
-    case 96: return launch_transport<96>(g, w0, nw, s, avg);
-    case 128: return launch_transport<128>(g, w0, nw, s, avg);
-    case 192: return launch_transport<192>(g, w0, nw, s, avg);
-    case 256: return launch_transport<256>(g, w0, nw, s, avg);
-    case 384: return launch_transport<384>(g, w0, nw, s, avg);
-    case 512: return launch_transport<512>(g, w0, nw, s, avg);
-    default: return launch_transport<0>(g, w0, nw, s, avg);
+    case 64: return launch_transport<64>(g, w0, nw, s, avg, planes);
+    case 96: return launch_transport<96>(g, w0, nw, s, avg, planes);
+    case 128: return launch_transport<128>(g, w0, nw, s, avg, planes);
+    case 192: return launch_transport<192>(g, w0, nw, s, avg, planes);
+    case 256: return launch_transport<256>(g, w0, nw, s, avg, planes);
+    case 384: return launch_transport<384>(g, w0, nw, s, avg, planes);
+    case 512: return launch_transport<512>(g, w0, nw, s, avg, planes);
+    default: return launch_transport<0>(g, w0, nw, s, avg, false);
   }
 }
 
@@ -2354,7 +2430,7 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
   // Transport's column pass also writes the leapfrog average (all but the
   // C/D diagonals, which finish averages after the renormalisation)
   const bool fused = (g->flags & PSA_F_DCTRANS) != 0;
-  if (fused && (rc = transport_impl(g, w0, nw, dt, (hipStream_t)stream, true))) return rc;
+  if (fused && (rc = transport_impl(g, w0, nw, dt, (hipStream_t)stream, true, true))) return rc;
   if ((rc = finish_impl(g, w0, nw, (hipStream_t)stream, fused))) return rc;
   if ((rc = psa_sources(g, w0, nw, stream))) return rc;
   if ((rc = psa_qw2prop(g, w0, nw, stream))) return rc;
@@ -2460,7 +2536,7 @@ int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *qu
     const psa_group *g = &gs[j];
     if ((rc = psa_relaxation(g, w0[j], h.nw[j], dt, stream))) return rc;
     const bool fused = (g->flags & PSA_F_DCTRANS) != 0;
-    if (fused && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)stream, true))) return rc;
+    if (fused && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)stream, true, true))) return rc;
     if ((rc = finish_impl(g, w0[j], h.nw[j], (hipStream_t)stream, fused))) return rc;
     if ((rc = psa_sources(g, w0[j], h.nw[j], stream))) return rc;
     if ((rc = psa_qw2prop(g, w0[j], h.nw[j], stream))) return rc;

@@ -73,6 +73,9 @@ __global__ __launch_bounds__(256) void k_split(psa_group g, int w0) {
   const int N = g.N, NK = nk(N), mat = blockIdx.z % 6, w = w0 + blockIdx.z / 6;
   const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64, tid = threadIdx.x;
   const long o = mat_off(g, w);
+  // p, C, D planes already hold this level (written by the previous step's
+  // transport): nothing to do for them (block-uniform, before any barrier)
+  if (mat < 3 && g.wtag && g.wtag[2 * w] == (uint64_t)(g.pB + o)) return;
   const double *X = (mat == 0 ? g.pB : mat == 1 ? g.CB : mat == 2 ? g.DB
                      : mat == 3 ? g.Heh : mat == 4 ? g.Heo : g.Hho) + o;
   const bool zd = (mat == 1 || mat == 2);
@@ -233,13 +236,16 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 // Copy the slab (k block kb) of slot `slot`, rows of the 128-row tile at
 // x0, into dst[2][chunk]: wave v copies the 1 KB pieces v, v + 4, ... of
-// the 32 (band slots too: their real planes are zero).
+// the 32.  IM_ONLY (the band slots, whose real planes are zero): only the
+// 16 pieces of the imaginary planes.
+template <bool IM_ONLY>
 __device__ __forceinline__ void copy_slab(uint16_t (*dst)[4 * 64 * 32], const uint16_t *spl, int N, int w, int slot,
                                           int x0, int kb) {
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int i = wave + 4 * j;
+  for (int j = 0; j < (IM_ONLY ? 4 : 8); ++j) {
+    // IM_ONLY: pieces 8..15 of each half (planes 2 and 3)
+    const int i = IM_ONLY ? 16 * (j >> 1) + 8 + wave + 4 * (j & 1) : wave + 4 * j;
     const int h = i >> 4, off = (i & 15) * 512;  // 64-row half of the tile, uint16 offset in the chunk
     const uint16_t *src = spl + chunk(N, w, slot, (x0 >> 6) + h, kb) + off + lane * 8;
     __builtin_amdgcn_global_load_lds(src, (lds_void *)&dst[h][off], 16, 0, 0);
@@ -282,11 +288,13 @@ __device__ __forceinline__ void prod3(const bf16x8 (&ah)[4], const bf16x8 (&al)[
 // The products of one staged slab of a term into this wave's 4 x 4
 // blocks.  The term's sign and conjugations are sign flips of the operand
 // planes (exact): A' = sg conj?(A), B' = conj?(B), then
-//   Re += Re A' Re B' - Im A' Im B',  Im += Re A' Im B' + Im A' Re B'
-// (the band operands go through the same products with their zero real
-// planes).  At most one A and one B plane pair (64 registers) are held at
-// a time: Re A' Re B' (into Re), Re A' Im B' (Im), -Im A' Im B' (Re),
+//   Re += Re A' Re B' - Im A' Im B',  Im += Re A' Im B' + Im A' Re B'.
+// At most one A and one B plane pair (64 registers) are held at a time:
+// Re A' Re B' (into Re), Re A' Im B' (Im), -Im A' Im B' (Re),
 // Im A' Re B' (Im); each accumulator gets its products in that order.
+// BAND 1 (A is a band slot) and 2 (B is): the products with the zero real
+// planes of the band operand are skipped (they would add exact zeros).
+template <int BAND>
 __device__ __forceinline__ void slab(const Stage &st, uint32_t fre, uint32_t fim, uint32_t gim, fv4 (&aR)[4][4],
                                      fv4 (&aI)[4][4], int wm, int wn, int kl, int xl) {
   const uint16_t *A = st.a[wm], *B = st.b[wn];
@@ -305,23 +313,32 @@ __device__ __forceinline__ void slab(const Stage &st, uint32_t fre, uint32_t fim
       bl[s] = flip(fread(B, pl + 1, s * 16 + xl, kl), m);
     }
   };
-  readA(0, fre);
-  readB(0, 0u);
-  prod3(ah, al, bh, bl, aR);
-  __builtin_amdgcn_sched_barrier(0);
-  readB(2, gim);
-  prod3(ah, al, bh, bl, aI);
-  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (BAND == 0) {
+    readA(0, fre);
+    readB(0, 0u);
+    prod3(ah, al, bh, bl, aR);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if constexpr (BAND != 1) {
+    if constexpr (BAND == 2) readA(0, fre);
+    readB(2, gim);
+    prod3(ah, al, bh, bl, aI);
+    __builtin_amdgcn_sched_barrier(0);
+  } else {
+    readB(2, gim);
+  }
   readA(2, fim ^ 0x80008000u);  // -Im A'
   prod3(ah, al, bh, bl, aR);
-  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (BAND != 2) {
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    ah[s] = flip(ah[s], 0x80008000u);
-    al[s] = flip(al[s], 0x80008000u);
+    for (int s = 0; s < 4; ++s) {
+      ah[s] = flip(ah[s], 0x80008000u);
+      al[s] = flip(al[s], 0x80008000u);
+    }
+    readB(0, 0u);
+    prod3(ah, al, bh, bl, aI);
   }
-  readB(0, 0u);
-  prod3(ah, al, bh, bl, aI);
 }
 
 // One 128 x 128 output tile (batch b = 3 wire + equation; rows m0..,
@@ -333,6 +350,11 @@ __device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0
   const int e = b % 3, w = p.w0 + b / 3;
   const int nt = p.nterms(b), KS = nk(N) / BK;
   const uint16_t *spl = reinterpret_cast<const uint16_t *>(p.g.spl);
+  // the p, C, D planes in spl now hold the level this RHS reads (pB); the
+  // RK4 stages (modes 2-5) rewrite their inputs in place, so nothing is
+  // kept from them
+  if (tid == 0 && e == 0 && m0 == 0 && n0 == 0 && p.g.wtag)
+    p.g.wtag[2 * w] = p.mode <= 1 ? (uint64_t)(p.g.pB + mat_off(p.g, w)) : 0ull;
   fv4 aR[4][4], aI[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -342,19 +364,26 @@ __device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0
       aI[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
     }
   // per K slab: copy, wait for everyone's copies, multiply, wait for
-  // everyone's products before the stage is overwritten
-  for (int t = 0; t < nt; ++t) {
+  // everyone's products before the stage is overwritten.  The terms with a
+  // band operand (dp's last two) run in loops of their own.
+  auto run_term = [&](auto band_c, int t) {
+    constexpr int BAND = decltype(band_c)::value;
     const Term tm = term(e, t);
     const uint32_t fre = tm.neg ? 0x80008000u : 0u, fim = (tm.neg != tm.ca) ? 0x80008000u : 0u;
     const uint32_t gim = tm.cb ? 0x80008000u : 0u;
     for (int kb = 0; kb < KS; ++kb) {
-      copy_slab(stg.a, spl, N, w, tm.sa, m0, kb);
-      copy_slab(stg.b, spl, N, w, tm.sb, n0, kb);
+      copy_slab<BAND == 1>(stg.a, spl, N, w, tm.sa, m0, kb);
+      copy_slab<BAND == 2>(stg.b, spl, N, w, tm.sb, n0, kb);
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
-      slab(stg, fre, fim, gim, aR, aI, wm, wn, kl, xl);
+      slab<BAND>(stg, fre, fim, gim, aR, aI, wm, wn, kl, xl);
       __syncthreads();
     }
+  };
+  for (int t = 0; t < nt && t < 4; ++t) run_term(std::integral_constant<int, 0>{}, t);
+  if (nt > 4) {
+    run_term(std::integral_constant<int, 1>{}, 4);  // (i hbar Dh band) p
+    run_term(std::integral_constant<int, 2>{}, 5);  // p (i hbar De band)
   }
   // The FP64 epilogue, through the LDS stage in two halves of 64 rows: the
   // waves holding a half (wm) leave their accumulators there as
@@ -440,6 +469,24 @@ __global__ __launch_bounds__(256, 2) void k_rhs(RhsProb p) {
     const unsigned xcd = lin % 8, q = lin / 8, per = 3 * tpb;
     b = 3 * (xcd + 8 * (q / per)) + (q % per) / tpb;
     lin = q % tpb;
+  } else if (gridDim.z % 24 == 0 && p.g.tune[2] == 2) {
+    // XCD-grouped, longest first: on each XCD the dp tiles of all its wires
+    // (six terms each) before the dC and dD tiles (four terms), so the
+    // tail of the launch is not left to a few six-term tiles
+    const unsigned xcd = lin % 8, q = lin / 8, wx = gridDim.z / 24;  // wires per XCD
+    const unsigned ndp = wx * tpb;
+    unsigned j, e;
+    if (q < ndp) {
+      j = q / tpb;
+      e = 0;
+      lin = q % tpb;
+    } else {
+      const unsigned r = q - ndp;
+      j = r / (2 * tpb);
+      e = 1 + (r % (2 * tpb)) / tpb;
+      lin = r % tpb;
+    }
+    b = 3 * (xcd + 8 * j) + e;
   } else {
     b = lin / tpb;
     lin %= tpb;

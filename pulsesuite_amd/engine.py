@@ -44,8 +44,11 @@ class WireGroup:
     """
 
     def __init__(self, prm, nw, rr, wire_ids=None, prec="fp64", device="cuda", host_mb=False,
-                 lor_table=False, integrator="leapfrog"):
+                 lor_table=False, integrator="leapfrog", carry=True):
         self.lib = capi.load()
+        # carry: let the library reuse, from one step to the next, the
+        # operand planes and diagonal sums it left behind (psa_group.wtag)
+        self.carry = bool(carry)
         if not torch.cuda.is_available():
             raise capi.PsaError("no GPU visible: the device path has no CPU fallback")
         self.dev = torch.device(device)
@@ -123,6 +126,10 @@ class WireGroup:
         # chunks of 64 rows x 32 k
         nspl = nwz * 14 * 4 * (-(-N // 128) * 128) * (-(-N // 32) * 32) if prec == "bf16x3" else 1
         self.s["spl"] = torch.zeros(nspl, dtype=torch.int16, device=d)
+        # per-wire reuse tags (psa_group.wtag): which level the p, C, D planes
+        # and the carried diagonal sums belong to; zeroed on every host write
+        # of the state
+        self.s["wtag"] = torch.zeros(2 * nwz, dtype=torch.int64, device=d)
         # split-K partial sums of the source / propagation-grid products
         self._kscr_len = 8 * 2 * max(Nr * 4 * nwz, (2 * N - 1) * 3 * nwz, max(tb["bnx"], 1) * 5 * nwz)
         self.s["kscr"] = torch.empty(self._kscr_len, dtype=F64, device=d)
@@ -214,6 +221,8 @@ class WireGroup:
             setattr(S, k, v.data_ptr())
         S.kscr_len = self._kscr_len
         S.wid_host = self._wid_host.ctypes.data
+        if not self.carry:
+            S.wtag = None
         self.S = S
 
     # ------------------------------------------------------------------
@@ -231,6 +240,16 @@ class WireGroup:
             self.s[k].copy_(diag.expand(self.nw, N, N))
         self.s["CX"].zero_()
         self.s["DX"].zero_()
+        self.invalidate()
+
+    def invalidate(self, w=None):
+        """Forget what the library carries over between steps (the p, C, D
+        operand planes and the diagonal sums) for wire w, or every wire:
+        needed after any write of p, C or D outside the step entry points."""
+        if w is None:
+            self.s["wtag"].zero_()
+        else:
+            self.s["wtag"][2 * w:2 * w + 2].zero_()
 
     def reset_wire(self, w):
         """Initial state (init_state) for wire w only, levels 2 and 3."""
@@ -241,6 +260,7 @@ class WireGroup:
             self.s[k][w].zero_()
         for k in ("CA", "CB", "DA", "DB"):
             self.s[k][w].copy_(diag)
+        self.invalidate(w)
 
     def swap_levels(self):
         """Level 3 <- the new level for every wire of the group (pointer
@@ -252,8 +272,9 @@ class WireGroup:
 
     def autotune(self, reps=3):
         """Pick the launch choices of this group by timing them on its own
-        buffers: tune[2] GEMM and split-BF16 RHS block order (XCD-grouped or
-        linear), tune[1] / tune[3] transport lines per
+        buffers: tune[2] GEMM and split-BF16 RHS block order (XCD-grouped,
+        linear, or for the split-BF16 RHS XCD-grouped with the longest tiles
+        first), tune[1] / tune[3] transport lines per
         block for the row / column pass (4, 2, 8, 16), timed inside full steps
         (the column pass carries the leapfrog average there).  The state is
         unchanged: only per-step scratch is written, and the averaged level
@@ -273,7 +294,9 @@ class WireGroup:
         def full_step():  # psa_step without the level swap
             capi.check(self.lib.psa_step(ctypes.byref(self.S), 0, self.nw, 1e-18, self._stream()))
 
-        knobs = [(2, [0, 1], contractions)]
+        # knob 2: 0 XCD-grouped, 1 linear, 2 XCD-grouped with the longest
+        # (six-term dp) RHS tiles first (split-BF16 RHS only)
+        knobs = [(2, [0, 1, 2] if self.prec == "bf16x3" else [0, 1], contractions)]
         saved = None
         if self.prm.flags.get("DCTrans") and self.prm.flags.get("LF"):
             saved = {k: self.s[k].clone() for k in ("pA", "CA", "DA")}
@@ -295,11 +318,13 @@ class WireGroup:
                 torch.cuda.synchronize()
                 times.append(a.elapsed_time(b) / reps)
             best = int(np.argmin(times))
-            # the XCD-grouped block order (candidate 0 of knob 2) reads about
-            # half the HBM bytes of the linear one; it is kept unless the
-            # linear order is clearly faster (timing noise aside)
-            if idx == 2 and times[0] <= 1.03 * times[best]:
-                best = 0
+            # the XCD-grouped block orders (candidates 0 and 2 of knob 2) read
+            # about half the HBM bytes of the linear one; one of them is kept
+            # unless the linear order is clearly faster (timing noise aside)
+            if idx == 2 and best == 1:
+                grouped = min((i for i in range(len(cands)) if i != 1), key=lambda i: times[i])
+                if times[grouped] <= 1.03 * times[1]:
+                    best = grouped
             self.tune[idx] = cands[best]
             self._sync_struct()
             out[idx] = (self.tune[idx], times)
@@ -323,6 +348,7 @@ class WireGroup:
             if a.ndim == 2:
                 a = a[:, :, None]
             self.s[k].copy_(torch.as_tensor(np.ascontiguousarray(np.moveaxis(a, 2, 0)), device=self.dev))
+        self.invalidate()
 
     def get_state(self):
         """(C2, D2, p2, C3, D3, p3) in the reference layout [N, N, Nw]."""

@@ -156,6 +156,7 @@ def test_nonfinite_request_is_isolated_on_the_device():
             with torch.cuda.stream(pool.stream):
                 pool.group.s["CB"][i, 3, 3] = float("nan")
                 pool.group.s["CA"][i, 3, 3] = float("nan")
+                pool.group.invalidate(i)
 
     print("[serve] mix with a NaN injected into request 2", flush=True)
     got, srv = _serve(mk(), on_tick=inject, nan_check=8)
