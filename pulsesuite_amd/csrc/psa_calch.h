@@ -118,13 +118,12 @@ __device__ __forceinline__ void calch_epilogue(const psa_group &g, int w, int wh
         const double re = (double)aR[i][c][v], im = (double)aI[i][c][v];
         if (which == 0) {
           const cd m = (a & 1) ? k1c[c] : k0[c];
-          st(g.Heh + ow, ij, cd{m.re + re, m.im + im});
+          st_h(g, g.Heh, ow, ij, cd{m.re + re, m.im + im});
         } else if (d == 0) {
           const double e0 = ee ? g.Ee[a] + g.gap : g.Eh[a];
           st(g.hdiag + (long)w * 4 * N, (ee ? 0 : N) + a, cd{e0 - re, -im});
         } else {
-          double *Hout = (ee ? g.Heo : g.Hho) + ow;
-          st(Hout, ij, lf ? cd{-re + k0[c].re, -im + k0[c].im} : cd{0.0, 0.0});
+          st_h(g, ee ? g.Heo : g.Hho, ow, ij, lf ? cd{-re + k0[c].re, -im + k0[c].im} : cd{0.0, 0.0});
         }
       }
   }

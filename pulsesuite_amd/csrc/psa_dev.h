@@ -82,4 +82,24 @@ __device__ __forceinline__ long mat_off(const psa_group &g, int w) {
   return (long)w * g.N * g.N * 2;  // doubles
 }
 
+// The Hamiltonian buffers Heh, Heo, Hho hold complex64 in the split-BF16
+// mode (packed from the start of the buffer: wire w at float offset
+// mat_off(g, w)), complex128 otherwise.
+__device__ __forceinline__ bool h32(const psa_group &g) { return g.prec == PSA_PREC_BF16X3; }
+__device__ __forceinline__ cd ld_h32(const double *base, long o, long ij) {
+  const float2 v = reinterpret_cast<const float2 *>(reinterpret_cast<const float *>(base) + o)[ij];
+  return cd{(double)v.x, (double)v.y};
+}
+__device__ __forceinline__ void st_h32(double *base, long o, long ij, cd v) {
+  reinterpret_cast<float2 *>(reinterpret_cast<float *>(base) + o)[ij] = make_float2((float)v.re, (float)v.im);
+}
+// element ij of wire offset o, in the group's storage precision
+__device__ __forceinline__ cd ld_h(const psa_group &g, const double *base, long o, long ij) {
+  return h32(g) ? ld_h32(base, o, ij) : ld(base + o, ij);
+}
+__device__ __forceinline__ void st_h(const psa_group &g, double *base, long o, long ij, cd v) {
+  if (h32(g)) st_h32(base, o, ij, v);
+  else st(base + o, ij, v);
+}
+
 }  // namespace psa

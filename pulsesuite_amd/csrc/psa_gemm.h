@@ -424,10 +424,10 @@ struct RhsProb {
     Pre q;
     if (e == 0) {
       q.x = ld(g.pB + o, ij);
-      q.h = ld(g.Heh + o, ij);  // Heh[ke][kh], stored transposed
+      q.h = ld_h(g, g.Heh, o, ij);  // Heh[ke][kh], stored transposed
     } else {
       q.x = ld((e == 1 ? g.CB : g.DB) + o, ij);
-      q.h = (row == col) ? cd{0.0, 0.0} : ld((e == 1 ? g.Heo : g.Hho) + o, ij);  // H_o[k2][k1]
+      q.h = (row == col) ? cd{0.0, 0.0} : ld_h(g, e == 1 ? g.Heo : g.Hho, o, ij);  // H_o[k2][k1]
     }
     q.x1 = mode == 0 ? cd{0.0, 0.0} : ld((e == 0 ? g.pA : (e == 1 ? g.CA : g.DA)) + o, ij);
     return q;
@@ -543,7 +543,7 @@ __device__ inline void calch_store(const psa_group &g, int w, int which, int a, 
       meh.re = a1 * ex.re + (a2 * ey.re) * yv + (a3 * ez.re) * zs;
       meh.im = a1 * ex.im + (a2 * ey.im) * yv + (a3 * ez.im) * zs;
     }
-    st(g.Heh + o, ij, cd{meh.re + re, meh.im + im});
+    st_h(g, g.Heh, o, ij, cd{meh.re + re, meh.im + im});
   } else {
     const bool ee = (which == 1);
     if (d == 0) {
@@ -551,9 +551,9 @@ __device__ inline void calch_store(const psa_group &g, int w, int which, int a, 
       st(hd, (ee ? 0 : N) + a, cd{e0 - re, -im});
       return;
     }
-    double *Hout = (ee ? g.Heo : g.Hho) + o;
+    double *Hout = ee ? g.Heo : g.Hho;
     if (!lf) {
-      st(Hout, ij, cd{0.0, 0.0});
+      st_h(g, Hout, o, ij, cd{0.0, 0.0});
       return;
     }
     const int ad = d > 0 ? d : -d;
@@ -568,7 +568,7 @@ __device__ inline void calch_store(const psa_group &g, int w, int which, int a, 
       h = cd{-re + vself * dsC.re - veh * dsDm.re, -im + vself * dsC.im - veh * dsDm.im};
     else     // - exchange + Vhh(qd) sum_k D[k,k+qd] - Veh(qd) sum_k C[k+qd,k]
       h = cd{-re + vself * dsD.re - veh * dsCm.re, -im + vself * dsD.im - veh * dsCm.im};
-    st(Hout, ij, h);
+    st_h(g, Hout, o, ij, h);
   }
 }
 

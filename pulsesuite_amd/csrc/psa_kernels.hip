@@ -362,9 +362,9 @@ __global__ __launch_bounds__(256) void k_h_simple(psa_group g, int w0) {
       meh.re = a1 * ex.re + (a2 * ey.re) * yv + (a3 * ez.re) * zs;
       meh.im = a1 * ex.im + (a2 * ey.im) * yv + (a3 * ez.im) * zs;
     }
-    st(g.Heh + o, ij, meh);
-    st(g.Heo + o, ij, cd{0, 0});
-    st(g.Hho + o, ij, cd{0, 0});
+    st_h(g, g.Heh, o, ij, meh);
+    st_h(g, g.Heo, o, ij, cd{0, 0});
+    st_h(g, g.Hho, o, ij, cd{0, 0});
     if (k1 == k2) {
       double *hd = g.hdiag + (long)w * 4 * N;
       st(hd, k1, cd{g.Ee[k1] + g.gap, 0.0});
@@ -1463,88 +1463,119 @@ __global__ __launch_bounds__(256) void k_average(psa_group g, int w0, int nw) {
 // scaling (SBEs.py:5749-5750) and QW2Prop's |.| / common-total step
 // (qwoptics.py:785-790), and evaluates CalcI0 (dcfield.py:1110-1152).
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_src_norm(psa_group g, int w0) {
-  __shared__ double sh[8];
-  __shared__ double s_ne2[1024], s_nh2[1024], ecE[1024], ecH[1024], s_vee[1024], s_vhh[1024];
+// One block of SN_THREADS per wire.  Every global load is issued up front
+// (one round of latency): the source lines into registers, the diagonals of
+// the new and the averaged level and the interaction rows into LDS.  The
+// O(N^2) EkReNorm sums of CalcI0 split each k over SN_SPLIT neighbouring
+// lanes (their partial sums added in lane order).
+constexpr int SN_THREADS = 1024, SN_RC = 2, SN_SPLIT = 4;  // SN_RC * SN_THREADS >= Nr for N <= 1023
+__global__ __launch_bounds__(SN_THREADS) void k_src_norm(psa_group g, int w0) {
+  __shared__ double sh[SN_THREADS / 64];
+  __shared__ double s_ne2[1024], s_nh2[1024], s_vee[1024], s_vhh[1024], ecE[1024], ecH[1024];
   const int w = w0 + blockIdx.x, N = g.N, Nr = g.Nr, tid = threadIdx.x;
   const long o = mat_off(g, w);
   double *src = g.src + (long)w * PSA_SRC_STRIDE(N, Nr);
-  cd *rcv = reinterpret_cast<cd *>(src + PSA_SRC_RC(Nr));  // [2][Nr]
+  const cd *rcv = reinterpret_cast<const cd *>(src + PSA_SRC_RC(Nr));  // [2][Nr]
   double *re_ = src + 3 * Nr, *rh_ = src + 4 * Nr;
-  // boundary mean of the real parts (qwoptics.py:1062-1064)
-  double bpe = 0, bph = 0;
-  for (int r = tid; r < Nr; r += blockDim.x)
-  {
-    if (r < 10) {
-      bpe += rcv[r].re;
-      bph += rcv[Nr + r].re;
-    }
-    if (r >= Nr - 10) {
-      bpe += rcv[r].re;
-      bph += rcv[Nr + r].re;
-    }
-  }
-  const double be = block_sum(bpe, sh) / 20.0, bh = block_sum(bph, sh) / 20.0;
-  double ne_t = 0, nh_t = 0;
-  for (int k = tid; k < N; k += blockDim.x) {
-    ne_t += cabs_(ld(g.CX + o, (long)k * N + k));
-    nh_t += cabs_(ld(g.DX + o, (long)k * N + k));
-  }
-  const double NeT = block_sum(ne_t, sh) + 1e-50, NhT = block_sum(nh_t, sh) + 1e-50;
-  const double dr = g.rq[1] - g.rq[0];
-  double ae = 0, ah = 0;
-  for (int r = tid; r < Nr; r += blockDim.x) {
-    ae += hypot(rcv[r].re - be, rcv[r].im);
-    ah += hypot(rcv[Nr + r].re - bh, rcv[Nr + r].im);
-  }
-  const double Ae = block_sum(ae, sh), Ah = block_sum(ah, sh);
-  const double fe = NeT / (Ae * dr + 1e-100), fh = NhT / (Ah * dr + 1e-100);
-  const double sc = 2.0 * g.e0 / g.area * g.ehint;
-  double s1 = 0, s2 = 0;
-  for (int r = tid; r < Nr; r += blockDim.x) {
-    const double a = hypot((rcv[r].re - be) * fe * sc, rcv[r].im * fe * sc);
-    const double b = hypot((rcv[Nr + r].re - bh) * fh * sc, rcv[Nr + r].im * fh * sc);
-    re_[r] = a;
-    rh_[r] = b;
-    s1 += a;
-    s2 += b;
-  }
-  const double S1 = block_sum(s1, sh), S2 = block_sum(s2, sh);
-  const double total = (S1 * dr + S2 * dr + 1e-100) / 2.0;
-  for (int r = tid; r < Nr; r += blockDim.x) {
-    re_[r] = re_[r] * total / (S1 * dr + 1e-100);
-    rh_[r] = rh_[r] * total / (S2 * dr + 1e-100);
-  }
-  // CalcI0 on ne2, nh2 = |diag of the averaged level|
   const double *vc = g.prep + (long)w * PSA_PREP_STRIDE(N) + PSA_PREP_VC(N);
-  for (int k = tid; k < N; k += blockDim.x) {
-    s_ne2[k] = cabs_(ld(g.CA + o, (long)k * N + k));
-    s_nh2[k] = cabs_(ld(g.DA + o, (long)k * N + k));
+  // --- loads ---
+  cd ce[SN_RC], ch[SN_RC];
+#pragma unroll
+  for (int u = 0; u < SN_RC; ++u) {
+    const int r = tid + u * SN_THREADS;
+    ce[u] = r < Nr ? rcv[r] : cd{0.0, 0.0};
+    ch[u] = r < Nr ? rcv[Nr + r] : cd{0.0, 0.0};
+  }
+  double ne_t = 0, nh_t = 0;
+  for (int k = tid; k < N; k += SN_THREADS) {
+    const long kk = (long)k * N + k;
+    ne_t += cabs_(ld(g.CX + o, kk));
+    nh_t += cabs_(ld(g.DX + o, kk));
+    // CalcI0 on ne2, nh2 = |diag of the averaged level|
+    s_ne2[k] = cabs_(ld(g.CA + o, kk));
+    s_nh2[k] = cabs_(ld(g.DA + o, kk));
     s_vee[k] = vc[N + k];
     s_vhh[k] = vc[2 * N + k];
   }
-  __syncthreads();
+  // boundary mean of the real parts (qwoptics.py:1062-1064)
+  double bpe = 0, bph = 0;
+#pragma unroll
+  for (int u = 0; u < SN_RC; ++u) {
+    const int r = tid + u * SN_THREADS;
+    if (r < 10 || (r >= Nr - 10 && r < Nr)) {
+      bpe += ce[u].re;
+      bph += ch[u].re;
+    }
+  }
+  const double be = block_sum(bpe, sh) / 20.0, bh = block_sum(bph, sh) / 20.0;
+  const double NeT = block_sum(ne_t, sh) + 1e-50, NhT = block_sum(nh_t, sh) + 1e-50;
+  const double dr = g.rq[1] - g.rq[0];
+  double ae = 0, ah = 0;
+#pragma unroll
+  for (int u = 0; u < SN_RC; ++u)
+    if (tid + u * SN_THREADS < Nr) {
+      ae += hypot(ce[u].re - be, ce[u].im);
+      ah += hypot(ch[u].re - bh, ch[u].im);
+    }
+  const double Ae = block_sum(ae, sh), Ah = block_sum(ah, sh);
+  const double fe = NeT / (Ae * dr + 1e-100), fh = NhT / (Ah * dr + 1e-100);
+  const double sc = 2.0 * g.e0 / g.area * g.ehint;
+  double av[SN_RC], bv[SN_RC], s1 = 0, s2 = 0;
+#pragma unroll
+  for (int u = 0; u < SN_RC; ++u) {
+    av[u] = bv[u] = 0.0;
+    if (tid + u * SN_THREADS < Nr) {
+      av[u] = hypot((ce[u].re - be) * fe * sc, ce[u].im * fe * sc);
+      bv[u] = hypot((ch[u].re - bh) * fh * sc, ch[u].im * fh * sc);
+      s1 += av[u];
+      s2 += bv[u];
+    }
+  }
+  const double S1 = block_sum(s1, sh), S2 = block_sum(s2, sh);
+  const double total = (S1 * dr + S2 * dr + 1e-100) / 2.0;
+#pragma unroll
+  for (int u = 0; u < SN_RC; ++u) {
+    const int r = tid + u * SN_THREADS;
+    if (r < Nr) {
+      re_[r] = av[u] * total / (S1 * dr + 1e-100);
+      rh_[r] = bv[u] * total / (S2 * dr + 1e-100);
+    }
+  }
+  // EkReNorm (dcfield.py:338-345): V[k][i] nonzero for i <= k; the
+  // SN_SPLIT neighbouring lanes of a k sum i = part, part + SN_SPLIT, ...
+  // and their partial sums are added in lane order
   const double vee0 = s_vee[0], vhh0 = s_vhh[0];
-  for (int k = tid; k < N; k += blockDim.x) {
+  const int lane = tid & 63;
+  for (int kb = 0; kb < N; kb += SN_THREADS / SN_SPLIT) {
+    const int pt = tid % SN_SPLIT, k = kb + tid / SN_SPLIT;
     double se = 0, shh = 0;
-    // unrolled so that the LDS reads of several i are in flight together
+    if (k < N) {
 #pragma unroll 8
-    for (int i = 0; i <= k; ++i) {  // V[k][i] nonzero for i <= k (EkReNorm, dcfield.py:338-345)
-      se += s_ne2[i] * (vee0 - s_vee[k - i]);
-      shh += s_nh2[i] * (vhh0 - s_vhh[k - i]);
-    }
+      for (int i = pt; i <= k; i += SN_SPLIT) {
+        se += s_ne2[i] * (vee0 - s_vee[k - i]);
+        shh += s_nh2[i] * (vhh0 - s_vhh[k - i]);
+      }
 #pragma unroll 8
-    for (int i = k + 1; i < N; ++i) {
-      se += s_ne2[i] * vee0;
-      shh += s_nh2[i] * vhh0;
+      for (int i = k + 1 + (pt - (k + 1) % SN_SPLIT + SN_SPLIT) % SN_SPLIT; i < N; i += SN_SPLIT) {
+        se += s_ne2[i] * vee0;
+        shh += s_nh2[i] * vhh0;
+      }
     }
-    ecE[k] = g.Ee[k] + se / 2.0;
-    ecH[k] = g.Eh[k] + shh / 2.0;
+    double a = 0, b = 0;
+#pragma unroll
+    for (int q = 0; q < SN_SPLIT; ++q) {
+      a += __shfl(se, (lane & ~(SN_SPLIT - 1)) + q, 64);
+      b += __shfl(shh, (lane & ~(SN_SPLIT - 1)) + q, 64);
+    }
+    if (pt == 0 && k < N) {
+      ecE[k] = g.Ee[k] + a / 2.0;
+      ecH[k] = g.Eh[k] + b / 2.0;
+    }
   }
   __syncthreads();
   double pe = 0, phh = 0, sne = 0, snh = 0;
   const double dk = g.dkr;
-  for (int i = tid; i < N; i += blockDim.x) {  // DriftVt (dcfield.py:347-360)
+  for (int i = tid; i < N; i += SN_THREADS) {  // DriftVt (dcfield.py:347-360)
     double de, dh;
     if (i >= 1 && i <= N - 2) {
       de = (ecE[i + 1] - ecE[i - 1]) / 2.0 / dk;
@@ -2364,7 +2395,7 @@ int psa_sources(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
     p.nw = nw;
     if (int rc = launch_gemm_splitk(p, g->Nr, 4 * nw, 2 * g->N - 1, s, "sources gemm")) return rc;
   }
-  hipLaunchKernelGGL(k_src_norm, dim3(nw), dim3(256), 0, s, *g, w0);
+  hipLaunchKernelGGL(k_src_norm, dim3(nw), dim3(SN_THREADS), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_src_norm");
   return PSA_OK;
 }

@@ -76,8 +76,9 @@ __global__ __launch_bounds__(256) void k_split(psa_group g, int w0) {
   // p, C, D planes already hold this level (written by the previous step's
   // transport): nothing to do for them (block-uniform, before any barrier)
   if (mat < 3 && g.wtag && g.wtag[2 * w] == (uint64_t)(g.pB + o)) return;
-  const double *X = (mat == 0 ? g.pB : mat == 1 ? g.CB : mat == 2 ? g.DB
-                     : mat == 3 ? g.Heh : mat == 4 ? g.Heo : g.Hho) + o;
+  // p, C, D complex128; the Hamiltonians complex64 in this mode
+  const double *X = mat == 0 ? g.pB : mat == 1 ? g.CB : mat == 2 ? g.DB
+                    : mat == 3 ? g.Heh : mat == 4 ? g.Heo : g.Hho;
   const bool zd = (mat == 1 || mat == 2);
   const bool stored_t = mat >= 3;  // the buffer holds the transpose of the matrix
   uint16_t *spl = reinterpret_cast<uint16_t *>(g.spl);
@@ -92,7 +93,8 @@ __global__ __launch_bounds__(256) void k_split(psa_group g, int w0) {
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
         const int gi = r0 + 2 * rp + r;
-        v[e][r] = ld(X, (long)min(gi, N - 1) * N + min(gj, N - 1));
+        const long ij = (long)min(gi, N - 1) * N + min(gj, N - 1);
+        v[e][r] = mat < 3 ? ld(X + o, ij) : ld_h32(X, o, ij);
         if (gi >= N || gj >= N || (zd && gi == gj)) v[e][r] = cd{0.0, 0.0};
       }
     }
@@ -148,22 +150,24 @@ __global__ __launch_bounds__(256) void k_split(psa_group g, int w0) {
 // ---------------------------------------------------------------------------
 // k_split_band: slots 12 (kind 1, Dh) and 13 (kind 0, De): rows x, k = m,
 // value i hbar D[m - x][x] from the tables dedh[w][kind][x][q + N - 1]
-// (q = m - x, q = 0 excluded).  Block (64 rows, kind, wire); thread: one
-// row, 8 k per pass; the imaginary planes only (the real planes stay as
-// allocated: zero).
+// (q = m - x, q = 0 excluded).  Block (64 rows, kind, wire); a thread takes
+// one 8-k group of a row, so the lanes of a wave read consecutive table
+// entries of a row and write consecutive 16-byte groups of it; the
+// imaginary planes only (the real planes stay as allocated: zero).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_split_band(psa_group g, int w0) {
-  const int N = g.N, NK = nk(N), Q = 2 * N - 1;
+  const int N = g.N, NK = nk(N), Q = 2 * N - 1, G = NK / 8;  // 8-k groups per row
   const int kind = blockIdx.y, w = w0 + blockIdx.z, slot = 13 - kind;
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63), gx = min(x, N - 1);
-  const double *tab = g.dedh + ((long)w * 2 + kind) * (long)N * Q + (long)gx * Q;
   uint16_t *spl = reinterpret_cast<uint16_t *>(g.spl);
-  for (int k0 = 8 * (threadIdx.x >> 6); k0 < NK; k0 += 32) {
+  const int rows = 256 / G, gq = threadIdx.x % G, k0 = 8 * gq;
+  for (int r = threadIdx.x / G; r < 64 && threadIdx.x < rows * G; r += rows) {
+    const int x = blockIdx.x * 64 + r, gx = min(x, N - 1);
+    const double *tab = g.dedh + ((long)w * 2 + kind) * (long)N * Q + (long)gx * Q + (N - 1 - gx);
     double v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int k = k0 + e;
-      const double raw = tab[min(max(k - gx + N - 1, 0), Q - 1)];
+      const double raw = tab[min(k, N - 1)];  // q = k - x
       v[e] = (x < N && k < N && k != x) ? g.hbar * raw : 0.0;
     }
     u32x4 hi, lo;
@@ -396,7 +400,7 @@ __device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0
   const int col = n0 + (tid & (TILE - 1));
   const long o = mat_off(p.g, w);
   const double *X = (e == 0 ? p.g.pB : (e == 1 ? p.g.CB : p.g.DB)) + o;
-  const double *H = (e == 0 ? p.g.Heh : (e == 1 ? p.g.Heo : p.g.Hho)) + o;
+  const double *H = e == 0 ? p.g.Heh : (e == 1 ? p.g.Heo : p.g.Hho);  // complex64 in this mode
   const double *X1 = (e == 0 ? p.g.pA : (e == 1 ? p.g.CA : p.g.DA)) + o;
   double *out = (e == 0 ? p.g.pX : (e == 1 ? p.g.CX : p.g.DX)) + o;
   const RhsProb::Side cs = p.side(e, w, min(col, N - 1), true);
@@ -411,7 +415,7 @@ __device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0
         const int row = m0 + h * 64 + (tid >> 7) + 2 * (j0 + u);
         const long ij = (long)row * N + col;
         q[u].x = ld(X, ij);
-        const cd hv = ld(H, ij);
+        const cd hv = ld_h32(H, o, ij);
         q[u].h = (e != 0 && row == col) ? cd{0.0, 0.0} : hv;
         if constexpr (LEAP) q[u].x1 = ld(X1, ij);
       }

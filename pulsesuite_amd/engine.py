@@ -350,6 +350,20 @@ class WireGroup:
             self.s[k].copy_(torch.as_tensor(np.ascontiguousarray(np.moveaxis(a, 2, 0)), device=self.dev))
         self.invalidate()
 
+    def hamiltonians(self):
+        """(Heh, Heo, Hho) as complex128 arrays [Nw][N][N] in the device
+        order (transposed, [k2][k1]); the split-BF16 mode stores them as
+        complex64 packed from the start of each buffer."""
+        torch.cuda.synchronize()
+        n = self.nw * self.N * self.N
+        out = []
+        for k in ("Heh", "Heo", "Hho"):
+            t = self.s[k]
+            if self.prec == "bf16x3":
+                t = t.reshape(-1).view(torch.complex64)[:n].reshape(self.nw, self.N, self.N)
+            out.append(t.cpu().numpy().astype(np.complex128))
+        return tuple(out)
+
     def get_state(self):
         """(C2, D2, p2, C3, D3, p3) in the reference layout [N, N, Nw]."""
         torch.cuda.synchronize()

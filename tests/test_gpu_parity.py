@@ -111,9 +111,7 @@ def test_calc_h_stage(case64):
     c.wg.calc_h()
     torch.cuda.synchronize()
     Ex, Ey, Ez = c.oracle_fields()
-    Heh_g = c.wg.s["Heh"].cpu().numpy()
-    Heo = c.wg.s["Heo"].cpu().numpy()
-    Hho = c.wg.s["Hho"].cpu().numpy()
+    Heh_g, Heo, Hho = c.wg.hamiltonians()
     hd = c.wg.s["hdiag"].cpu().numpy()
     for w in range(c.nw):
         C3, D3, p3 = c.state[3][:, :, w], c.state[4][:, :, w], c.state[5][:, :, w]
@@ -165,9 +163,7 @@ def test_calc_h_large_n(N):
         wg.prep()
         wg.calc_h()
         torch.cuda.synchronize()
-        Heh_g = wg.s["Heh"].cpu().numpy()
-        Heo = wg.s["Heo"].cpu().numpy()
-        Hho = wg.s["Hho"].cpu().numpy()
+        Heh_g, Heo, Hho = wg.hamiltonians()  # complex64 storage in the split-BF16 mode
         hd = wg.s["hdiag"].cpu().numpy()
         for w in range(nw):
             Heh, Hee, Hhh = ref[w]
