@@ -113,13 +113,20 @@ __global__ __launch_bounds__(256) void k_prep_a(psa_group g, int w0) {
   const long o = mat_off(g, w);
   const double *C2 = g.CB + o, *D2 = g.DB + o;
   double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
+  // wire-fastest copies for k_mb and the off-diagonal dephasing products:
+  // |ne|, |nh| of the checkout level and VC(d)^2 per channel
+  const int W = g.Nw;
+  double *vsq = g.wmb + 6L * N * W, *aocc = g.wmb + 9L * N * W;
   double part = 0.0;
   for (int k = tid; k < N; k += blockDim.x) {
     const cd a = ld(C2, (long)k * N + k), b = ld(D2, (long)k * N + k);
     st(pr + PSA_PREP_NE, k, a);
     st(pr + PSA_PREP_NH(N), k, b);
-    pr[PSA_PREP_ANE(N) + k] = cabs_(a);
-    pr[PSA_PREP_ANH(N) + k] = cabs_(b);
+    const double ane = cabs_(a), anh = cabs_(b);
+    pr[PSA_PREP_ANE(N) + k] = ane;
+    pr[PSA_PREP_ANH(N) + k] = anh;
+    aocc[(long)k * W + w] = ane;
+    aocc[(long)(N + k) * W + w] = anh;
     part += a.re + b.re;
   }
   const double tot = block_sum(part, sh);
@@ -135,7 +142,11 @@ __global__ __launch_bounds__(256) void k_prep_a(psa_group g, int w0) {
           g.chi[N + d] * 2.0 * log(fabs((qh - PI * n1) / (qh + n1)));
     }
     E1D[d] = e;
-    for (int c = 0; c < 3; ++c) vc[c * N + d] = screened ? g.vtab[c * N + d] / e : g.vtab[c * N + d];
+    for (int c = 0; c < 3; ++c) {
+      const double v = screened ? g.vtab[c * N + d] / e : g.vtab[c * N + d];
+      vc[c * N + d] = v;
+      vsq[((long)c * N + d) * W + w] = v * v;
+    }
   }
 }
 
@@ -380,7 +391,7 @@ __global__ __launch_bounds__(256) void k_h_simple(psa_group g, int w0) {
 // feeds (see the slot mapping below).  Block = (MB_KG target k, slot,
 // MB_WC wires): the block stages the occupations and the two V^2 channels
 // of its slot for its wires in LDS (rows of MB_WC wires, from the
-// wire-fastest tables built by k_diag_gather / k_wire_tables); a lane is
+// wire-fastest tables built by k_diag_gather / k_prep_a); a lane is
 // (entry group, wire) and the eight lanes of a group read one 64-byte LDS
 // row.  The sums are reduced over the block in a fixed order.
 //   wmb + 4 N Nw: occ[2][N][Nw]  |Re ne3|, |Re nh3|
@@ -648,22 +659,6 @@ __global__ void k_diag_gather(psa_group g, int w0) {
   for (int k = threadIdx.x; k < N; k += blockDim.x) {
     occ[(long)k * W + w] = fabs(ld(g.CX + o, (long)k * N + k).re);
     occ[(long)(N + k) * W + w] = fabs(ld(g.DX + o, (long)k * N + k).re);
-  }
-}
-
-// Wire-fastest tables from prep for k_mb and the off-diagonal dephasing
-// products: VC(d)^2 per channel and |ne|, |nh| of the checkout level.
-__global__ void k_wire_tables(psa_group g, int w0) {
-  const int w = w0 + blockIdx.x, N = g.N, W = g.Nw;
-  const double *pr = g.prep + (long)w * PSA_PREP_STRIDE(N);
-  double *vsq = g.wmb + 6L * N * W, *aocc = g.wmb + 9L * N * W;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) {
-    for (int ch = 0; ch < 3; ++ch) {
-      const double v = pr[PSA_PREP_VC(N) + ch * N + k];
-      vsq[((long)ch * N + k) * W + w] = v * v;
-    }
-    aocc[(long)k * W + w] = pr[PSA_PREP_ANE(N) + k];
-    aocc[(long)(N + k) * W + w] = pr[PSA_PREP_ANH(N) + k];
   }
 }
 
@@ -1999,8 +1994,6 @@ int psa_prep(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   PSA_LAUNCH_CHECK("k_diagsums");
   hipLaunchKernelGGL(k_prep_b, dim3(nw, (g->N + PB_WAVES - 1) / PB_WAVES), dim3(64 * PB_WAVES), 0, s, *g, w0);
   PSA_LAUNCH_CHECK("k_prep_b");
-  hipLaunchKernelGGL(k_wire_tables, dim3(nw), dim3(256), 0, s, *g, w0);
-  PSA_LAUNCH_CHECK("k_wire_tables");
   return PSA_OK;
 }
 
