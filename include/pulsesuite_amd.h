@@ -196,15 +196,17 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream);
 int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream);
 
 /* Many-body scattering list `list` (0..7) on the device: the non-zero
- * entries (i, j, l) of the stored Gaussian tables of CalcMBArrays
- * (coulomb.py:367-499) in the order _MBCE_core/_MBCH_core consume them
- * (coulomb.py:956-1071), grouped by target index (the slot mapping of
- * psa_group.mb_*).  Two calls: off == NULL counts the entries per target
- * into counts[N]; then, with off[N+1] the exclusive prefix sum, fills
- * idx (packed i | j << 10 | l << 20) and val.  Init-time. */
+ * entries (i, j, l) of the stored tables of CalcMBArrays (coulomb.py:367-499;
+ * the Gaussian branch 448-497, or with lorentz != 0 the LorentzDelta branch
+ * 405-446, whose entries are all non-zero) in the order
+ * _MBCE_core/_MBCH_core consume them (coulomb.py:956-1071), grouped by
+ * target index (the slot mapping of psa_group.mb_*).  Two calls: off ==
+ * NULL counts the entries per target into counts[N]; then, with off[N+1]
+ * the exclusive prefix sum, fills idx (packed i | j << 10 | l << 20) and
+ * val.  Init-time. */
 int psa_mb_list(int32_t list, int32_t N, const double *Ee, const double *Eh, double ge, double gh,
-                double hbar, const int32_t *off, int32_t *counts, uint32_t *idx, double *val,
-                void *stream);
+                double hbar, int32_t lorentz, const int32_t *off, int32_t *counts, uint32_t *idx,
+                double *val, void *stream);
 
 /* The state-independent Lorentzian factors Lrtz(dE, hbar g) of
  * CalcOffDiagDeph_E2/H2 (dephasing.py:226-317; Lrtz dephasing.py:220-224),

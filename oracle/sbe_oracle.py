@@ -813,10 +813,17 @@ def mbph(t, nh, VC, E1D):
     return Win, Wout
 
 
-def transport(t, X, Edc, Eac, dt):
-    """dcfield.py:1176-1236 (k1nek2 = True)."""
+def transport(t, X, Edc, Eac, dt, lf=True):
+    """dcfield.py:1176-1236: ShiftN2D of the whole matrix (k1nek2 = LF
+    True, 1188-1199), else ShiftN1D of its diagonal only (1154-1174,
+    1229-1232), the off-diagonal entries kept."""
     dk = -e0 * (Edc + Eac) / hbar * dt
     Y = t.Ytr
+    if not lf:
+        out = np.array(X, dtype=complex, copy=True)
+        nk = np.fft.ifft(np.fft.fft(np.diag(X)) * np.exp(-1j * Y * dk))
+        out[np.diag_indices(len(nk))] = nk
+        return out
     F = np.fft.fft2(X)
     F = F * np.exp(-1j * (Y[:, None] + Y[None, :]) * dk)
     return np.fft.ifft2(F)
@@ -1059,9 +1066,10 @@ class Solver:
     def _mb(self):
         if self.S_eh is None:
             g, prm = self.g, self.prm
-            self.S_eh = mb_tables(g, "eh", prm.gam_e, prm.gam_h)
-            self.S_ee = mb_tables(g, "ee", prm.gam_e, prm.gam_h)
-            self.S_hh = mb_tables(g, "hh", prm.gam_e, prm.gam_h)
+            lz = bool(prm.LorentzDelta)  # mb.params line 15 (SBEs.py:5015-5022)
+            self.S_eh = mb_tables(g, "eh", prm.gam_e, prm.gam_h, lz)
+            self.S_ee = mb_tables(g, "ee", prm.gam_e, prm.gam_h, lz)
+            self.S_hh = mb_tables(g, "hh", prm.gam_e, prm.gam_h, lz)
 
     def preparation(self, p2, C2, D2, Ex, Ey, Ez, w):
         """SBEs.py:5173-5226."""
@@ -1156,9 +1164,9 @@ class Solver:
             D3[np.diag_indices(N)] = nh3
         if prm.DCTrans:
             Eac = np.real(Ex[g.NQ0])
-            C3 = transport(t, C3, prm.Edc, Eac, dt)
-            D3 = transport(t, D3, prm.Edc, Eac, dt)
-            p3 = transport(t, p3, prm.Edc, Eac, dt)
+            C3 = transport(t, C3, prm.Edc, Eac, dt, prm.LF)
+            D3 = transport(t, D3, prm.Edc, Eac, dt, prm.LF)
+            p3 = transport(t, p3, prm.Edc, Eac, dt, prm.LF)
         ne3 = np.abs(np.diag(C3)).astype(complex)
         nh3 = np.abs(np.diag(D3)).astype(complex)
         total = (np.sum(np.abs(ne3)) + np.sum(np.abs(nh3))) / 2.0
@@ -1225,9 +1233,9 @@ class Solver:
             D3[np.diag_indices(N)] = nh3
         if prm.DCTrans:
             Eac = np.real(Ex[g.NQ0])
-            C3 = transport(t, C3, prm.Edc, Eac, dt)
-            D3 = transport(t, D3, prm.Edc, Eac, dt)
-            p3 = transport(t, p3, prm.Edc, Eac, dt)
+            C3 = transport(t, C3, prm.Edc, Eac, dt, prm.LF)
+            D3 = transport(t, D3, prm.Edc, Eac, dt, prm.LF)
+            p3 = transport(t, p3, prm.Edc, Eac, dt, prm.LF)
         ne3 = np.abs(np.diag(C3)).astype(complex)
         nh3 = np.abs(np.diag(D3)).astype(complex)
         total = (np.sum(np.abs(ne3)) + np.sum(np.abs(nh3))) / 2.0

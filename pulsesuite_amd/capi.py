@@ -83,8 +83,8 @@ SIGNATURES = {
     "psa_prep": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_calc_h": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_lorentz_table": (ctypes.c_int, [_GP, c_ptr]),
-    "psa_mb_list": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_dbl, c_dbl, c_dbl, c_ptr, c_ptr,
-                                   c_ptr, c_ptr, c_ptr]),
+    "psa_mb_list": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_dbl, c_dbl, c_dbl, c_i32, c_ptr,
+                                   c_ptr, c_ptr, c_ptr, c_ptr]),
     "psa_offdiag_tables": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_rhs": (ctypes.c_int, [_GP, c_i32, c_i32, c_i32, c_dbl, c_ptr]),
     "psa_relaxation": (ctypes.c_int, [_GP, c_i32, c_i32, c_dbl, c_ptr]),

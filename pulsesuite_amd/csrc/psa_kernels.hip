@@ -1019,6 +1019,45 @@ __global__ void k_tphase(psa_group g, int w0, double dt) {
   }
 }
 
+// Transport with LF = 0 (dcfield.py:1229-1232): only the diagonal of each
+// matrix moves, ShiftN1D (dcfield.py:1154-1174) = ifft(fft(n) exp(-i Y dk)).
+// Block (wire, matrix): the diagonal and the twiddles in LDS, the forward
+// and inverse transforms as direct sums (twiddle index m j mod N), FP64.
+__global__ __launch_bounds__(256) void k_transport_diag(psa_group g, int w0) {
+  __shared__ cd s_n[1024], s_f[1024], s_tw[1024];
+  const int w = w0 + blockIdx.x, N = g.N, mat = blockIdx.y;  // 0 C, 1 D, 2 p
+  const long o = mat_off(g, w);
+  double *X = (mat == 0 ? g.CX : (mat == 1 ? g.DX : g.pX)) + o;
+  const double *ph = g.tphase + (long)w * 2 * N;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    s_n[j] = ld(X, (long)j * N + j);
+    s_tw[j] = ld(g.twid, j);  // exp(-2 pi i j / N)
+  }
+  __syncthreads();
+  for (int m = threadIdx.x; m < N; m += blockDim.x) {
+    cd a = {0.0, 0.0};
+    int t = 0;  // m j mod N
+    for (int j = 0; j < N; ++j) {
+      a = cadd(a, cmul(s_n[j], s_tw[t]));
+      t += m;
+      if (t >= N) t -= N;
+    }
+    s_f[m] = cmul(a, ld(ph, m));
+  }
+  __syncthreads();
+  const double inv = 1.0 / N;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    cd a = {0.0, 0.0};
+    int t = 0;  // m j mod N
+    for (int m = 0; m < N; ++m) {
+      a = cadd(a, cmul(s_f[m], cconj(s_tw[t])));
+      t += j;
+      if (t >= N) t -= N;
+    }
+    st(X, (long)j * N + j, cscale(a, inv));
+  }
+}
+
 // pass 0: rows, pass 1: columns.  With avg set (column pass only), also
 // writes the leapfrog average A = (A + X3) / 2 (SBEs.py:5704-5706) for
 // every element except the diagonals of C and D, which the e/h
@@ -1528,12 +1567,14 @@ __global__ __launch_bounds__(SN_THREADS) void k_src_norm(psa_group g, int w0) {
   }
   const double S1 = block_sum(s1, sh), S2 = block_sum(s2, sh);
   const double total = (S1 * dr + S2 * dr + 1e-100) / 2.0;
+  // without LF the reference never calls QWRho5: re = rh = 0 (SBEs.py:5732)
+  const bool lf = (g.flags & PSA_F_LF) != 0;
 #pragma unroll
   for (int u = 0; u < SN_RC; ++u) {
     const int r = tid + u * SN_THREADS;
     if (r < Nr) {
-      re_[r] = av[u] * total / (S1 * dr + 1e-100);
-      rh_[r] = bv[u] * total / (S2 * dr + 1e-100);
+      re_[r] = lf ? av[u] * total / (S1 * dr + 1e-100) : 0.0;
+      rh_[r] = lf ? bv[u] * total / (S2 * dr + 1e-100) : 0.0;
     }
   }
   // EkReNorm (dcfield.py:338-345): V[k][i] nonzero for i <= k; the
@@ -1681,8 +1722,9 @@ __global__ void k_record(psa_group g, int w0, int nw, int xmid, double *trace, i
 // ===========================================================================
 // Many-body scattering lists on the device (the tables of CalcMBArrays,
 // coulomb.py:367-499, as _MBCE_core/_MBCH_core read them, coulomb.py:956-1071):
-// entries (i, j, l) of the stored (N+1)^3 Gaussian table (stored index =
-// physical index + 1; index 0 planes are zero) whose weight is not zero,
+// entries (i, j, l) of the stored (N+1)^3 table, Gaussian or (LorentzDelta)
+// Lorentzian (stored index = physical index + 1; index 0 planes are zero)
+// whose weight is not zero,
 // CSR-grouped by the rate index they feed and ordered (i, j, l) within a
 // target.  Block = one target index; the threads walk the two free indices
 // in that order, 256 at a time, and a block-wide ballot scan keeps the order.
@@ -1697,31 +1739,33 @@ __global__ void k_record(psa_group g, int w0, int nw, int xmid, double *trace, i
 //   7     hh    l          i+j-l in [0,N), i >= l
 // ===========================================================================
 __device__ __forceinline__ double mb_weight(int kind, int i, int j, int l, int N, const double *Ee,
-                                            const double *Eh, double ge, double gh, double hbar) {
+                                            const double *Eh, double ge, double gh, double hbar, int lorentz) {
   if (i < 1 || j < 1 || l < 1) return 0.0;
   const int a = i - 1, b = j - 1, c = l - 1;
-  double Ed, hb;
+  double Ed, gam;
   int k30;
   if (kind == 0) {  // eh
     k30 = c + b - a;
     if (k30 < 0 || k30 >= N) return 0.0;
-    hb = hbar * ((ge + gh) / 2.0);
+    gam = (ge + gh) / 2.0;
     Ed = Ee[a] + Eh[b] - Eh[k30] - Ee[c];
   } else {
     const double *E = kind == 1 ? Ee : Eh;
     k30 = a + b - c;
     if (k30 < 0 || k30 >= N) return 0.0;
-    hb = hbar * (kind == 1 ? ge : gh);
+    gam = kind == 1 ? ge : gh;
     Ed = E[a] + E[b] - E[k30] - E[c];
   }
   if (a == c || b == k30) return 0.0;
+  const double hb = hbar * gam;
+  if (lorentz) return 2.0 * gam / (Ed * Ed + hb * hb);  // LorentzDelta branch, coulomb.py:405-446
   const double x = Ed / hb;
   return 2.0 * PI / hbar * (1.0 / (sqrt(PI) * hb) * exp(-(x * x)));
 }
 
 template <int L>
 __global__ __launch_bounds__(256) void k_mb_list(int N, const double *Ee, const double *Eh, double ge,
-                                                 double gh, double hbar, const int *off, int *counts,
+                                                 double gh, double hbar, int lorentz, const int *off, int *counts,
                                                  uint32_t *idx, double *val) {
   __shared__ int wtot[4];
   const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1740,7 +1784,7 @@ __global__ __launch_bounds__(256) void k_mb_list(int N, const double *Ee, const 
     bool ok = f < NN && l >= 0 && l < N;
     const int i3 = jli ? j + l - i : i + j - l;
     ok = ok && i3 >= 0 && i3 < N && (L == 5 ? l >= i : i >= l);
-    const double v = ok ? mb_weight(kind, i, j, l, N, Ee, Eh, ge, gh, hbar) : 0.0;
+    const double v = ok ? mb_weight(kind, i, j, l, N, Ee, Eh, ge, gh, hbar, lorentz) : 0.0;
     ok = ok && v > 0.0;
     const unsigned long long m = __ballot(ok);
     const int pre = __popcll(m & ((1ull << lane) - 1ull));
@@ -2035,7 +2079,7 @@ int psa_calc_h(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
 }
 
 int psa_mb_list(int32_t list, int32_t N, const double *Ee, const double *Eh, double ge, double gh,
-                double hbar, const int32_t *off, int32_t *counts, uint32_t *idx, double *val,
+                double hbar, int32_t lorentz, const int32_t *off, int32_t *counts, uint32_t *idx, double *val,
                 void *stream) {
   if (list < 0 || list > 7 || N < 2 || N > 1023 || !Ee || !Eh)
     return fail(PSA_E_INVALID, "psa_mb_list: bad arguments");
@@ -2044,8 +2088,8 @@ int psa_mb_list(int32_t list, int32_t N, const double *Ee, const double *Eh, dou
   switch (list) {
 #define PSA_MBL(L)                                                                                \
   case L:                                                                                         \
-    hipLaunchKernelGGL(k_mb_list<L>, dim3(N), dim3(256), 0, s, N, Ee, Eh, ge, gh, hbar, off, counts, \
-                       idx, val);                                                                 \
+    hipLaunchKernelGGL(k_mb_list<L>, dim3(N), dim3(256), 0, s, N, Ee, Eh, ge, gh, hbar, lorentz, off,  \
+                       counts, idx, val);                                                         \
     break;
     PSA_MBL(0) PSA_MBL(1) PSA_MBL(2) PSA_MBL(3) PSA_MBL(4) PSA_MBL(5) PSA_MBL(6) PSA_MBL(7)
 #undef PSA_MBL
@@ -2330,7 +2374,13 @@ int launch_transport(const psa_group *g, int32_t w0, int nw, hipStream_t s, bool
 int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStream_t s, bool avg,
                    bool planes = false) {
   planes = planes && g->prec == PSA_PREC_BF16X3 && g->spl && g->wtag;
-  if (!(g->flags & PSA_F_LF)) return fail(PSA_E_UNSUPPORTED, "Transport with LF = 0 (diagonal-only shift) is not supported");
+  if (!(g->flags & PSA_F_LF)) {  // diagonal-only shift; the caller averages
+    hipLaunchKernelGGL(k_tphase, dim3(nw), dim3(256), 0, s, *g, w0, dt);
+    PSA_LAUNCH_CHECK("k_tphase");
+    hipLaunchKernelGGL(k_transport_diag, dim3(nw, 3), dim3(256), 0, s, *g, w0);
+    PSA_LAUNCH_CHECK("k_transport_diag");
+    return PSA_OK;
+  }
   if (g->N > 512) return fail(PSA_E_UNSUPPORTED, "Transport needs N <= 512 (LDS lines)");
   int n = g->N;
   while (n % 2 == 0) n /= 2;
@@ -2452,9 +2502,10 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
   if ((rc = psa_rhs(g, w0, nw, 1, dt, stream))) return rc;
   if ((rc = psa_relaxation(g, w0, nw, dt, stream))) return rc;
   // Transport's column pass also writes the leapfrog average (all but the
-  // C/D diagonals, which finish averages after the renormalisation)
-  const bool fused = (g->flags & PSA_F_DCTRANS) != 0;
-  if (fused && (rc = transport_impl(g, w0, nw, dt, (hipStream_t)stream, true, true))) return rc;
+  // C/D diagonals, which finish averages after the renormalisation); the
+  // diagonal-only transport of LF = 0 leaves the average to finish
+  const bool trans = (g->flags & PSA_F_DCTRANS) != 0, fused = trans && (g->flags & PSA_F_LF);
+  if (trans && (rc = transport_impl(g, w0, nw, dt, (hipStream_t)stream, true, true))) return rc;
   if ((rc = finish_impl(g, w0, nw, (hipStream_t)stream, fused))) return rc;
   if ((rc = psa_sources(g, w0, nw, stream))) return rc;
   if ((rc = psa_qw2prop(g, w0, nw, stream))) return rc;
@@ -2559,8 +2610,8 @@ int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *qu
   for (int j = 0; j < h.n; ++j) {
     const psa_group *g = &gs[j];
     if ((rc = psa_relaxation(g, w0[j], h.nw[j], dt, stream))) return rc;
-    const bool fused = (g->flags & PSA_F_DCTRANS) != 0;
-    if (fused && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)stream, true, true))) return rc;
+    const bool trans = (g->flags & PSA_F_DCTRANS) != 0, fused = trans && (g->flags & PSA_F_LF);
+    if (trans && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)stream, true, true))) return rc;
     if ((rc = finish_impl(g, w0[j], h.nw[j], (hipStream_t)stream, fused))) return rc;
     if ((rc = psa_sources(g, w0[j], h.nw[j], stream))) return rc;
     if ((rc = psa_qw2prop(g, w0[j], h.nw[j], stream))) return rc;

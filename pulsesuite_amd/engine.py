@@ -155,11 +155,12 @@ class WireGroup:
         Ee = torch.as_tensor(g["Ee"], dtype=F64, device=d)
         Eh = torch.as_tensor(g["Eh"], dtype=F64, device=d)
         st = self._stream()
+        lz = 1 if prm.flags.get("LorentzDelta") else 0
         out, sizes = [], []
         for L in range(8):
             cnt = torch.zeros(N, dtype=torch.int32, device=d)
             capi.check(self.lib.psa_mb_list(L, N, Ee.data_ptr(), Eh.data_ptr(), prm.gam_e, prm.gam_h,
-                                            hbar, None, cnt.data_ptr(), None, None, st))
+                                            hbar, lz, None, cnt.data_ptr(), None, None, st))
             c = cnt.cpu().numpy().astype(np.int64)
             off = np.zeros(N + 1, dtype=np.int64)
             off[1:] = np.cumsum(c)
@@ -171,7 +172,7 @@ class WireGroup:
             val = torch.zeros(max(nnz, 1), dtype=F64, device=d)
             if nnz:
                 capi.check(self.lib.psa_mb_list(L, N, Ee.data_ptr(), Eh.data_ptr(), prm.gam_e, prm.gam_h,
-                                                hbar, offt.data_ptr(), None, idx.data_ptr(),
+                                                hbar, lz, offt.data_ptr(), None, idx.data_ptr(),
                                                 val.data_ptr(), st))
             out.append((offt, idx, val))
             sizes.append(nnz)

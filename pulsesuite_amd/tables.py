@@ -262,8 +262,9 @@ def spline_matrix(x0, x1):
 # _MBCE_core/_MBCH_core, coulomb.py:956-1071)
 # ---------------------------------------------------------------------------
 
-def _stored_table_chunk(kind, i, N, Ee, Eh, ge, gh):
-    """Row i (stored index, 1..N) of the (N+1)^3 Gaussian table:
+def _stored_table_chunk(kind, i, N, Ee, Eh, ge, gh, lorentz=False):
+    """Row i (stored index, 1..N) of the (N+1)^3 table, Gaussian
+    (coulomb.py:448-497) or with ``lorentz`` Lorentzian (405-446):
     S[i][j][l] for j, l in [0, N] (index 0 planes are zero)."""
     geh = (ge + gh) / 2.0
     out = np.zeros((N + 1, N + 1))
@@ -274,24 +275,28 @@ def _stored_table_chunk(kind, i, N, Ee, Eh, ge, gh):
     c = np.arange(N)[None, :]
     if kind == "eh":
         k30 = c + b - a
-        hb = hbar * geh
+        gam = geh
         ok = (k30 >= 0) & (k30 < N)
         kk = np.clip(k30, 0, N - 1)
         Ed = Ee[a] + Eh[b] - Eh[kk] - Ee[c]
     else:
         E = Ee if kind == "ee" else Eh
-        hb = hbar * (ge if kind == "ee" else gh)
+        gam = ge if kind == "ee" else gh
         k30 = a + b - c
         ok = (k30 >= 0) & (k30 < N)
         kk = np.clip(k30, 0, N - 1)
         Ed = E[a] + E[b] - E[kk] - E[c]
-    val = (TWOPI / hbar * (a != c) * (b != k30)
-           * (1.0 / (np.sqrt(np.pi) * hb) * np.exp(-((Ed / hb) ** 2))))
+    hb = hbar * gam
+    if lorentz:
+        val = 2.0 * gam * (a != c) * (b != k30) / (Ed**2 + hb**2)
+    else:
+        val = (TWOPI / hbar * (a != c) * (b != k30)
+               * (1.0 / (np.sqrt(np.pi) * hb) * np.exp(-((Ed / hb) ** 2))))
     out[1:, 1:] = np.where(ok, val, 0.0)
     return out
 
 
-def many_body_lists(N, Ee, Eh, ge, gh, rel_cut=0.0):
+def many_body_lists(N, Ee, Eh, ge, gh, rel_cut=0.0, lorentz=False):
     """Eight CSR lists of the non-negligible stored-table entries (i, j, l)
     grouped by the rate index they feed:
       0 eh->WinE[i]   1 eh->WoutE[l]   2 ee->WinE[i]   3 ee->WoutE[l]
@@ -307,7 +312,7 @@ def many_body_lists(N, Ee, Eh, ge, gh, rel_cut=0.0):
     for kind in ("eh", "ee", "hh"):
         cut = rel_cut * peak[kind]
         for i in range(1, N):
-            S = _stored_table_chunk(kind, i, N, Ee, Eh, ge, gh)[:N, :N]
+            S = _stored_table_chunk(kind, i, N, Ee, Eh, ge, gh, lorentz)[:N, :N]
             m = S > cut
             if m.any():
                 jj = J[:N, :N][m]
@@ -445,12 +450,10 @@ def build_tables(prm, rr, vint_fn, host_mb=True):
     t["m2"] = np.ascontiguousarray(M2[bx0:bx1])
     # many-body scattering lists
     if prm.flags.get("EHs"):
-        if prm.flags.get("LorentzDelta"):
-            raise NotImplementedError(
-                "LorentzDelta = 1 (Lorentzian many-body tables) is not supported "
-                "by the device path")
-        # host_mb False: the caller builds the lists on the device
-        t["mb"] = many_body_lists(N, Ee, Eh, prm.gam_e, prm.gam_h) if host_mb else None
+        # host_mb False: the caller builds the lists on the device; with
+        # LorentzDelta = 1 every entry of the tables is non-zero (dense lists)
+        lz = bool(prm.flags.get("LorentzDelta"))
+        t["mb"] = many_body_lists(N, Ee, Eh, prm.gam_e, prm.gam_h, lorentz=lz) if host_mb else None
     else:
         z = np.zeros(N + 1, dtype=np.int32)
         t["mb"] = [(z, np.zeros(0, np.uint32), np.zeros(0))] * 8

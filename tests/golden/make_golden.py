@@ -67,6 +67,7 @@ MB_SHIPPED = [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 0, 1, 0]
 MB_RHS_ONLY = [1, 1, 0, 1, 0, 0, 1, 0, 1, 1, 0, 0, 0, 0, 0]
 MB_IGNORE = [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 1, 1, 0]
 MB_LORENTZ = [1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 0, 1, 1]
+MB_NOLF = [1, 1, 1, 1, 1, 1, 0, 0, 1, 1, 1, 0, 0, 1, 0]
 
 
 def _install_shims():
@@ -500,6 +501,8 @@ def xqw_case(nk=16, seed=5, nx=64, drr=10e-9):
 def main():
     if not os.path.isdir(REF_SRC):
         raise SystemExit("reference tree not found; run in the build container")
+    # the reference tree is read only: no bytecode caches written next to it
+    sys.dont_write_bytecode = True
     _install_shims()
     sys.path.insert(0, REF_SRC)
     if sys.argv[1:] == ["xqw"]:
@@ -515,6 +518,9 @@ def main():
     if sys.argv[1:] == ["qw3d"]:
         qw3d_case()
         return
+    if sys.argv[1:] == ["flags"]:
+        flag_cases()
+        return
     run_case(16, seed=1, mb=MB_SHIPPED, nsteps=4, tag="n16_mb")
     run_case(16, seed=2, mb=MB_RHS_ONLY, nsteps=3, tag="n16_rhs")
     run_case(32, seed=3, mb=MB_SHIPPED, nsteps=3, tag="n32_mb")
@@ -523,6 +529,16 @@ def main():
     xqw_case()
     lorentz_case()
     qw3d_case()
+    flag_cases()
+
+
+def flag_cases():
+    """QWCalculator trajectories with the mb.params options that change the
+    fused step: LF = 0 (diagonal-only DC transport, dcfield.py:1229-1232, no
+    QWRho5, Hartree terms off) and LorentzDelta = 1 (Lorentzian many-body
+    tables, coulomb.py:405-446)."""
+    run_case(16, seed=21, mb=MB_NOLF, nsteps=3, tag="n16_nolf")
+    run_case(16, seed=22, mb=MB_LORENTZ, nsteps=3, tag="n16_lztraj")
 
 
 if __name__ == "__main__":

@@ -245,7 +245,7 @@ def test_rhs_fp64(case64):
         assert rel(dD_g[w], dD) < RHS_TOL
 
 
-@pytest.mark.parametrize("tag", ["n16_mb", "n16_rhs", "n32_mb"])
+@pytest.mark.parametrize("tag", ["n16_mb", "n16_rhs", "n32_mb", "n16_nolf", "n16_lztraj"])
 def test_trajectory_vs_reference_golden(tag):
     """Full QWCalculator steps from the reference's own seeded state and
     fields (golden vectors written from the reference)."""
@@ -295,6 +295,44 @@ def test_step_vs_oracle_n64():
         got = c.wg.get_state()
         for v, r in zip(got, (c.os.CC2, c.os.DD2, c.os.YY2, c.os.CC3, c.os.DD3, c.os.YY3)):
             assert rel(v, r) < TRAJ_TOL, it
+
+
+def _flags(**kw):
+    names = O.FLAG_NAMES
+    f = list(O.SHIPPED_FLAGS)
+    for k, v in kw.items():
+        f[names.index(k)] = v
+    return tuple(f)
+
+
+@pytest.mark.parametrize("what,flags", [("LF=0", _flags(LF=0)), ("LorentzDelta=1", _flags(LorentzDelta=1))])
+def test_step_vs_oracle_mb_options(what, flags):
+    """Three full steps of two wires at N = 64 with the mb.params options
+    that change the fused step: LF = 0 (diagonal-only transport by direct
+    DFTs, no QWRho5, no Hartree terms) and LorentzDelta = 1 (dense
+    Lorentzian many-body lists); the oracle is pinned for both by the
+    reference's trajectories (n16_nolf, n16_lztraj)."""
+    import torch
+    print(f"[mb options] {what}", flush=True)
+    c = Case(64, 2, seed=17, flags=flags)
+    dt = 10e-18
+    for it in range(3):
+        c.wg.step(dt)
+        torch.cuda.synchronize()
+        Pxx, Pyy, Pzz, Rho = c.wg.outputs()
+        for w in range(c.nw):
+            ref = c.os.qw_calculator(c.Exx, c.Eyy, c.Ezz, dt, w + 1)
+            assert rel(Pxx[w], ref[0]) < TRAJ_TOL, (it, w)
+            assert rel(Pyy[w], ref[1]) < TRAJ_TOL, (it, w)
+            assert rel(Pzz[w], ref[2]) < TRAJ_TOL, (it, w)
+            if np.max(np.abs(ref[3])) > 0:
+                assert rel(Rho[w], ref[3]) < TRAJ_TOL, (it, w)
+            else:
+                assert np.max(np.abs(Rho[w])) == 0.0, (it, w)
+        got = c.wg.get_state()
+        for v, r in zip(got, (c.os.CC2, c.os.DD2, c.os.YY2, c.os.CC3, c.os.DD3, c.os.YY3)):
+            assert rel(v, r) < TRAJ_TOL, it
+        assert rel(c.wg.observables()["I0"], c.os.I0) < TRAJ_TOL
 
 
 @pytest.mark.parametrize("N", [48, 64, 96, 128, 192, 256])

@@ -114,7 +114,9 @@ def run_golden_trajectory(z):
     return outs, states
 
 
-@pytest.mark.parametrize("tag", TAGS)
+# trajectories also with LF = 0 (diagonal-only transport, no QWRho5, no
+# Hartree terms) and LorentzDelta = 1 (Lorentzian many-body tables)
+@pytest.mark.parametrize("tag", TAGS + ["n16_nolf", "n16_lztraj"])
 def test_qwcalculator_trajectory_matches_reference(golden_dir, tag):
     z = load(golden_dir, tag)
     outs, states = run_golden_trajectory(z)
@@ -132,6 +134,36 @@ def test_qwcalculator_trajectory_matches_reference(golden_dir, tag):
 # ---------------------------------------------------------------------------
 # Closed forms, library routines, brute force
 # ---------------------------------------------------------------------------
+
+def test_transport_lf0_diagonal_shift(golden_dir):
+    """LF = 0 transport (ShiftN1D of the diagonal, dcfield.py:1154-1174,
+    1229-1232) against the 2-D shift the golden trajectories pin: the 2-D
+    shift of a rank-one matrix u v^T is S(u) S(v)^T with S the 1-D shift, so
+    a wrong sign or bin order in S fails; off-diagonal entries are kept, a
+    zero field is the identity and S is unitary."""
+    z = load(golden_dir, "n16_mb")
+    prm, g, t = setup(z)
+    N = g.Nk
+    rng = np.random.default_rng(5)
+    Edc, Eac, dt = 1e6, 3e7, 2e-15
+
+    def S(u):
+        return np.diag(O.transport(t, np.diag(u), Edc, Eac, dt, lf=False))
+
+    u = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    v = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    two_d = O.transport(t, np.outer(u, v), Edc, Eac, dt, lf=True)
+    assert rel(two_d, np.outer(S(u), S(v))) < 1e-13
+    assert abs(np.linalg.norm(S(u)) / np.linalg.norm(u) - 1.0) < 1e-13
+    X = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    Y = O.transport(t, X, Edc, Eac, dt, lf=False)
+    off = ~np.eye(N, dtype=bool)
+    assert np.array_equal(Y[off], X[off])
+    assert rel(np.diag(Y), S(np.diag(X))) < 1e-15
+    assert rel(O.transport(t, X, 0.0, 0.0, dt, lf=False), X) < 1e-14
+    # the shift moves weight: not the identity at a finite field
+    assert rel(np.diag(Y), np.diag(X)) > 1e-3
+
 
 def test_k0_accurate_vs_scipy():
     # Zhang-Jin CIK01: series for x <= 9 loses ~1e-9 to cancellation near
