@@ -67,9 +67,8 @@ typedef struct psa_group {
   int32_t nnz_mb[8];/* entries in each many-body scattering list */
   int32_t tune[4];  /* launch choices, 0 = default: [0] reserved (0),
                        [1] transport lines per block, row pass (0: 4, else 2, 4, 8 or 16),
-                       [2] GEMM block order (0: batches grouped per XCD, 1: linear; 2: for the
-                       split-BF16 RHS, grouped per XCD with every dp tile before the dC and dD
-                       tiles, for the GEMMs linear),
+                       [2] GEMM block order (0: batches grouped per XCD, 1: linear; the
+                       split-BF16 RHS has a static schedule of its own),
                        [3] transport lines per block, column pass (0: 4, else 2, 4, 8 or 16;
                        16 and the column choice only for N known at compile time: 64, 96,
                        128, 192, 256, 384, 512; other N take 4 for 16) */

@@ -1883,20 +1883,22 @@ int split_planes(const psa_group &g, int w0, int nw, hipStream_t s) {
 }
 
 // Split-bf16 RHS: operand planes once per step, then the bf16 MFMA
-// products over 128 x 128 tiles (sp::k_rhs).
+// products (sp::k_rhs_pp, one persistent workgroup of 8 waves per CU).
 int rhs_bf16(const RhsProb &p, int nw, hipStream_t s) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    int rc = hip_check(hipFuncSetAttribute((const void *)sp::k_rhs, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)sp::lds_bytes()),
-                       "hipFuncSetAttribute(sp::k_rhs)");
+  static int cus = 0;
+  if (!cus) {
+    int rc = hip_check(hipFuncSetAttribute((const void *)sp::k_rhs_pp, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)sp::pp_lds_bytes()),
+                       "hipFuncSetAttribute(sp::k_rhs_pp)");
     if (rc) return rc;
-    attr_set = true;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
   }
   if (int rc = split_planes(p.g, p.w0, nw, s)) return rc;
-  const int nb = (p.g.N + sp::TILE - 1) / sp::TILE;
-  hipLaunchKernelGGL(sp::k_rhs, dim3(nb, nb, 3 * nw), dim3(256), sp::lds_bytes(), s, p);
-  PSA_LAUNCH_CHECK("sp::k_rhs");
+  hipLaunchKernelGGL(sp::k_rhs_pp, dim3(cus), dim3(sp::PP_THREADS), sp::pp_lds_bytes(), s, p, nw);
+  PSA_LAUNCH_CHECK("sp::k_rhs_pp");
   return PSA_OK;
 }
 
@@ -1935,7 +1937,17 @@ struct RhsQueue {
 
 template <int KIND>
 constexpr size_t queue_tile_bytes() {
-  return KIND == 2 ? sp::lds_bytes() : (KIND == 0 ? gemm_lds_bytes<double>() : gemm_lds_bytes<float>());
+  return KIND == 2 ? sp::pp_lds_bytes() : (KIND == 0 ? gemm_lds_bytes<double>() : gemm_lds_bytes<float>());
+}
+
+// threads per workgroup and workgroups per CU of each kind
+template <int KIND>
+constexpr int queue_threads() {
+  return KIND == 2 ? sp::PP_THREADS : 256;
+}
+template <int KIND>
+constexpr int queue_occupancy() {
+  return KIND == 2 ? 1 : 2;
 }
 
 // output tile edge of the work items of each kind
@@ -1945,7 +1957,7 @@ constexpr int queue_tile() {
 }
 
 template <int KIND>  // 0: FP64 products, 1: FP32, 2: split-BF16
-__global__ __launch_bounds__(256, 2) void k_rhs_queue(const RhsProb *__restrict__ probs, const WireEnt *__restrict__ ents,
+__global__ __launch_bounds__(queue_threads<KIND>(), queue_occupancy<KIND>()) void k_rhs_queue(const RhsProb *__restrict__ probs, const WireEnt *__restrict__ ents,
                                                    RhsQueue q, unsigned *queue) {
   // Dynamic LDS only: the operand tiles at offset 0 (aligned as the tile
   // bodies assume), the broadcast item index after them.
@@ -1972,7 +1984,7 @@ __global__ __launch_bounds__(256, 2) void k_rhs_queue(const RhsProb *__restrict_
       const int b = 3 * e.j + local / tpb, lin = local % tpb;
       const int m0 = (lin / gx) * queue_tile<KIND>(), n0 = (lin % gx) * queue_tile<KIND>();
       if constexpr (KIND == 2)
-        sp::rhs_tile(probs[e.gi], b, m0, n0, lds_raw);
+        sp::pp_tile(probs[e.gi], b, m0, n0, lds_raw);
       else
         gemm_tile<typename std::conditional<KIND == 0, double, float>::type, RhsProb>(probs[e.gi], b, m0, n0,
                                                                                       lds_raw);
@@ -1997,7 +2009,7 @@ int launch_rhs_queue(const RhsProb *probs_dev, const WireEnt *ents_dev, const Rh
                        "hipFuncSetAttribute(k_rhs_queue)");
     if (rc) return rc;
     int per = 0, dev = 0, cus = 0;
-    rc = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_queue<KIND>, 256, shm),
+    rc = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_queue<KIND>, queue_threads<KIND>(), shm),
                    "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (rc) return rc;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -2005,7 +2017,8 @@ int launch_rhs_queue(const RhsProb *probs_dev, const WireEnt *ents_dev, const Rh
     blocks = std::max(1, per) * std::max(1, cus);
   }
   const int grid = std::min(blocks, std::max(total, NXCD));
-  hipLaunchKernelGGL(k_rhs_queue<KIND>, dim3(grid), dim3(256), shm, s, probs_dev, ents_dev, q, queue);
+  hipLaunchKernelGGL(k_rhs_queue<KIND>, dim3(grid), dim3(queue_threads<KIND>()), shm, s, probs_dev, ents_dev, q,
+                     queue);
   PSA_LAUNCH_CHECK("k_rhs_queue");
   return PSA_OK;
 }

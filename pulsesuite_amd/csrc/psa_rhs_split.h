@@ -185,16 +185,12 @@ __global__ __launch_bounds__(256) void k_split_band(psa_group g, int w0) {
 }
 
 // ---------------------------------------------------------------------------
-// The RHS products.  A workgroup (256 threads, 4 waves as 2 x 2) owns a
-// 128 x 128 output tile of one (wire, equation); each wave a 64 x 64
-// sub-tile of 4 x 4 MFMA blocks (128 accumulator registers).  K runs in
-// slabs of 32 over the equation's terms.  The slab of each operand (two
-// 16 KB chunks: rows 0-63 and 64-127) is copied from HBM into the
-// workgroup's LDS stage (64 KB) by direct global-to-LDS loads (1 KB per
-// wave instruction, no registers).  Two workgroups share a CU (64 KB of
-// LDS and at most 128 + 128 registers per lane each): while one waits for
-// its copies or runs its FP64 epilogue (memory bound, straight from the
-// accumulator registers), the other keeps the matrix cores busy.
+// The RHS products.  The output tiles are 128 x 128 of one (wire,
+// equation); four waves (2 x 2) cover a tile, each a 64 x 64 sub-tile of
+// 4 x 4 MFMA blocks (128 accumulator registers).  K runs in slabs of 32
+// over the equation's terms.  The slab of each operand (two 16 KB chunks:
+// rows 0-63 and 64-127) is copied from HBM into an LDS stage (64 KB) by
+// direct global-to-LDS loads (1 KB per wave instruction, no registers).
 // ---------------------------------------------------------------------------
 
 // One term of an equation: A and B slots, conjugation and sign.
@@ -234,7 +230,6 @@ struct Stage {
   uint16_t b[2][4 * 64 * 32];
 };
 
-constexpr size_t lds_bytes() { return sizeof(Stage); }
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -244,8 +239,9 @@ typedef __attribute__((address_space(3))) void lds_void;
 // 16 pieces of the imaginary planes.
 template <bool IM_ONLY>
 __device__ __forceinline__ void copy_slab(uint16_t (*dst)[4 * 64 * 32], const uint16_t *spl, int N, int w, int slot,
-                                          int x0, int kb) {
-  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+                                          int x0, int kb, int wave_ = -1) {
+  const int lane = threadIdx.x & 63;
+  const int wave = wave_ >= 0 ? wave_ : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 #pragma unroll
   for (int j = 0; j < (IM_ONLY ? 4 : 8); ++j) {
     // IM_ONLY: pieces 8..15 of each half (planes 2 and 3)
@@ -345,18 +341,55 @@ __device__ __forceinline__ void slab(const Stage &st, uint32_t fre, uint32_t fim
   }
 }
 
-// One 128 x 128 output tile (batch b = 3 wire + equation; rows m0..,
-// columns n0..) of problem p; lds holds lds_bytes().
-__device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0, char *lds) {
-  Stage &stg = *reinterpret_cast<Stage *>(lds);
-  const int N = p.g.N, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1, kl = lane >> 4, xl = lane & 15;
+// ---------------------------------------------------------------------------
+// The products, persistent and ping-pong.
+// One workgroup of 8 waves per CU, two groups of 4 waves (the 2 x 2 layout
+// of the tile each), two LDS stages (128 KB).  Both groups work on the same
+// 128 x 128 tile: group g takes the K slabs s = g, g + 2, ...  In phase s
+// the group of slab s multiplies from its stage while the other group
+// copies slab s + 1 into its own stage and waits for it, so on every SIMD
+// one wave issues MFMAs while the other one loads.  After the last slab the
+// two partial sums are added in LDS (even slabs + odd slabs, fixed order)
+// and all 512 threads run the FP64 epilogue.  Tiles are assigned
+// statically: the tiles of wire w belong to XCD w % 8, listed dp tiles
+// first (six terms) and then the dC, dD tiles; workgroup j of an XCD takes
+// items j, j + P, j + 2P, ... of its list (P workgroups per XCD), which
+// balances the term count per CU when the XCD has 32 CUs and 4 tiles per
+// equation.
+// ---------------------------------------------------------------------------
+constexpr int PP_THREADS = 512;
+
+constexpr size_t pp_lds_bytes() { return 2 * sizeof(Stage); }
+
+// Item it of the list of XCD x: batch b (3 wire + equation) and tile index
+// in the equation, or false past the list's end.
+__device__ __forceinline__ bool pp_item(int x, int it, int nw, int tpb, int &b, int &tile) {
+  const int nwx = nw > x ? (nw - x + 7) / 8 : 0;  // wires x, x + 8, ...
+  const int ndp = nwx * tpb;
+  if (it >= 3 * ndp) return false;
+  int u, e;
+  if (it < ndp) {
+    u = it / tpb;
+    e = 0;
+    tile = it % tpb;
+  } else {
+    const int r = it - ndp;
+    u = r / (2 * tpb);
+    e = 1 + (r % (2 * tpb)) / tpb;
+    tile = r % tpb;
+  }
+  b = 3 * (x + 8 * u) + e;
+  return true;
+}
+
+__device__ __forceinline__ void pp_tile(const RhsProb &p, int b, int m0, int n0, char *lds) {
+  Stage *stg = reinterpret_cast<Stage *>(lds);
+  const int N = p.g.N, tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), grp = wave >> 2, wv4 = wave & 3;
+  const int wm = wv4 >> 1, wn = wv4 & 1, kl = lane >> 4, xl = lane & 15;
   const int e = b % 3, w = p.w0 + b / 3;
-  const int nt = p.nterms(b), KS = nk(N) / BK;
+  const int nt = p.nterms(b), KS = nk(N) / BK, S = nt * KS;
   const uint16_t *spl = reinterpret_cast<const uint16_t *>(p.g.spl);
-  // the p, C, D planes in spl now hold the level this RHS reads (pB); the
-  // RK4 stages (modes 2-5) rewrite their inputs in place, so nothing is
-  // kept from them
   if (tid == 0 && e == 0 && m0 == 0 && n0 == 0 && p.g.wtag)
     p.g.wtag[2 * w] = p.mode <= 1 ? (uint64_t)(p.g.pB + mat_off(p.g, w)) : 0ull;
   fv4 aR[4][4], aI[4][4];
@@ -367,20 +400,32 @@ __device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0
       aR[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
       aI[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
     }
-  // per K slab: copy, wait for everyone's copies, multiply, wait for
-  // everyone's products before the stage is overwritten.  The terms with a
-  // band operand (dp's last two) run in loops of their own.
+  // slab s of this tile into stage st by the calling group (its four waves)
+  auto copy = [&](Stage &st, int s) {
+    const Term tm = term(e, s / KS);
+    const int kb = s % KS;
+    if (tm.sa >= 12) copy_slab<true>(st.a, spl, N, w, tm.sa, m0, kb, wv4);
+    else copy_slab<false>(st.a, spl, N, w, tm.sa, m0, kb, wv4);
+    if (tm.sb >= 12) copy_slab<true>(st.b, spl, N, w, tm.sb, n0, kb, wv4);
+    else copy_slab<false>(st.b, spl, N, w, tm.sb, n0, kb, wv4);
+  };
+  if (grp == 0) copy(stg[0], 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  // phases of term t; the terms with a band operand run in loops of their own
   auto run_term = [&](auto band_c, int t) {
     constexpr int BAND = decltype(band_c)::value;
     const Term tm = term(e, t);
     const uint32_t fre = tm.neg ? 0x80008000u : 0u, fim = (tm.neg != tm.ca) ? 0x80008000u : 0u;
     const uint32_t gim = tm.cb ? 0x80008000u : 0u;
     for (int kb = 0; kb < KS; ++kb) {
-      copy_slab<BAND == 1>(stg.a, spl, N, w, tm.sa, m0, kb);
-      copy_slab<BAND == 2>(stg.b, spl, N, w, tm.sb, n0, kb);
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      slab<BAND>(stg, fre, fim, gim, aR, aI, wm, wn, kl, xl);
+      const int s = t * KS + kb, c = s & 1;
+      if (grp == c) {
+        slab<BAND>(stg[c], fre, fim, gim, aR, aI, wm, wn, kl, xl);
+      } else {
+        if (s + 1 < S) copy(stg[1 - c], s + 1);
+        __builtin_amdgcn_s_waitcnt(0);
+      }
       __syncthreads();
     }
   };
@@ -389,30 +434,43 @@ __device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0
     run_term(std::integral_constant<int, 1>{}, 4);  // (i hbar Dh band) p
     run_term(std::integral_constant<int, 2>{}, 5);  // p (i hbar De band)
   }
-  // The FP64 epilogue, through the LDS stage in two halves of 64 rows: the
-  // waves holding a half (wm) leave their accumulators there as
-  // [re/im][64][128] floats, then every thread walks that half row-major
-  // with a fixed column (its column quantities read once), four rows at a
-  // time with their FP64 operands loaded first and without branches (the
-  // other half's accumulators are still held meanwhile).
+  // even + odd slabs in LDS as [re/im][128 rows][128 columns]
   float *acc = reinterpret_cast<float *>(lds);
+  auto put = [&](bool add) {
+#pragma unroll
+    for (int si = 0; si < 4; ++si)
+#pragma unroll
+      for (int sj = 0; sj < 4; ++sj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rr = wm * 64 + si * 16 + 4 * kl + v, cc = wn * 64 + sj * 16 + xl;
+          float *re = &acc[rr * TILE + cc], *im = &acc[TILE * TILE + rr * TILE + cc];
+          *re = add ? *re + aR[si][sj][v] : aR[si][sj][v];
+          *im = add ? *im + aI[si][sj][v] : aI[si][sj][v];
+        }
+  };
+  if (grp == 0) put(false);
+  __syncthreads();
+  if (grp == 1) put(true);
+  __syncthreads();
+  // the FP64 epilogue: a fixed column per thread, four rows at a time with
+  // their FP64 operands loaded first
   const bool full = N % TILE == 0;
-  const int col = n0 + (tid & (TILE - 1));
+  const int col = n0 + (tid & (TILE - 1)), r0 = tid >> 7;  // 4 row phases
   const long o = mat_off(p.g, w);
   const double *X = (e == 0 ? p.g.pB : (e == 1 ? p.g.CB : p.g.DB)) + o;
   const double *H = e == 0 ? p.g.Heh : (e == 1 ? p.g.Heo : p.g.Hho);  // complex64 in this mode
   const double *X1 = (e == 0 ? p.g.pA : (e == 1 ? p.g.CA : p.g.DA)) + o;
   double *out = (e == 0 ? p.g.pX : (e == 1 ? p.g.CX : p.g.DX)) + o;
   const RhsProb::Side cs = p.side(e, w, min(col, N - 1), true);
-  // mode 1: X1 + 2 dt dX/dt (the leapfrog), mode 0: dX/dt
-  auto rows = [&](auto leap, int h) {
+  auto rows = [&](auto leap) {
     constexpr bool LEAP = decltype(leap)::value;
     constexpr int EB = 4;
-    for (int j0 = 0; j0 < 32; j0 += EB) {
+    for (int j0 = 0; j0 < TILE / 4; j0 += EB) {
       RhsProb::Pre q[EB];
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
-        const int row = m0 + h * 64 + (tid >> 7) + 2 * (j0 + u);
+        const int row = m0 + r0 + 4 * (j0 + u);
         const long ij = (long)row * N + col;
         q[u].x = ld(X, ij);
         const cd hv = ld_h32(H, o, ij);
@@ -421,9 +479,9 @@ __device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0
       }
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
-        const int rr = (tid >> 7) + 2 * (j0 + u), row = m0 + h * 64 + rr, cc = tid & (TILE - 1);
+        const int rr = r0 + 4 * (j0 + u), row = m0 + rr, cc = tid & (TILE - 1);
         const RhsProb::Side rs = p.side(e, w, row, false);
-        const cd d = p.deriv(e, (double)acc[rr * TILE + cc], (double)acc[64 * TILE + rr * TILE + cc], q[u], cs, rs);
+        const cd d = p.deriv(e, (double)acc[rr * TILE + cc], (double)acc[TILE * TILE + rr * TILE + cc], q[u], cs, rs);
         if constexpr (LEAP)
           st(out, (long)row * N + col, cd{q[u].x1.re + d.re * p.dt * 2.0, q[u].x1.im + d.im * p.dt * 2.0});
         else
@@ -431,71 +489,26 @@ __device__ __forceinline__ void rhs_tile(const RhsProb &p, int b, int m0, int n0
       }
     }
   };
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    if (wm == h) {
-#pragma unroll
-      for (int si = 0; si < 4; ++si)
-#pragma unroll
-        for (int sj = 0; sj < 4; ++sj)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int rr = si * 16 + 4 * kl + v, cc = wn * 64 + sj * 16 + xl;
-            acc[rr * TILE + cc] = aR[si][sj][v];
-            acc[64 * TILE + rr * TILE + cc] = aI[si][sj][v];
-          }
+  if (full && p.mode == 1) {
+    rows(std::true_type{});
+  } else if (full && p.mode == 0) {
+    rows(std::false_type{});
+  } else {
+    for (int i = tid; i < TILE * TILE; i += PP_THREADS) {
+      const int rr = i / TILE, cc = i % TILE;
+      p.store(b, m0 + rr, n0 + cc, (double)acc[rr * TILE + cc], (double)acc[TILE * TILE + rr * TILE + cc]);
     }
-    __syncthreads();
-    if (full && p.mode == 1) {
-      rows(std::true_type{}, h);
-    } else if (full && p.mode == 0) {
-      rows(std::false_type{}, h);
-    } else {
-      for (int i = tid; i < 64 * TILE; i += 256) {
-        const int rr = i / TILE, cc = i % TILE;
-        p.store(b, m0 + h * 64 + rr, n0 + cc, (double)acc[rr * TILE + cc], (double)acc[64 * TILE + rr * TILE + cc]);
-      }
-    }
-    __syncthreads();
   }
+  __syncthreads();  // the next tile's first copy overwrites the accumulators
 }
 
-__global__ __launch_bounds__(256, 2) void k_rhs(RhsProb p) {
+__global__ __launch_bounds__(PP_THREADS, 1) void k_rhs_pp(RhsProb p, int nw) {
   extern __shared__ char lds_raw[];
-  // XCD-aware order: the three equations of wire j run on XCD j % 8 (they
-  // read the same wire's planes, so those stay in that XCD's L2) when the
-  // wire count is a multiple of 8; consecutive workgroups go round robin
-  // over the XCDs.
-  const unsigned gx = gridDim.x, gy = gridDim.y, tpb = gx * gy;
-  unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  int b;
-  if (gridDim.z % 24 == 0 && p.g.tune[2] == 0) {
-    const unsigned xcd = lin % 8, q = lin / 8, per = 3 * tpb;
-    b = 3 * (xcd + 8 * (q / per)) + (q % per) / tpb;
-    lin = q % tpb;
-  } else if (gridDim.z % 24 == 0 && p.g.tune[2] == 2) {
-    // XCD-grouped, longest first: on each XCD the dp tiles of all its wires
-    // (six terms each) before the dC and dD tiles (four terms), so the
-    // tail of the launch is not left to a few six-term tiles
-    const unsigned xcd = lin % 8, q = lin / 8, wx = gridDim.z / 24;  // wires per XCD
-    const unsigned ndp = wx * tpb;
-    unsigned j, e;
-    if (q < ndp) {
-      j = q / tpb;
-      e = 0;
-      lin = q % tpb;
-    } else {
-      const unsigned r = q - ndp;
-      j = r / (2 * tpb);
-      e = 1 + (r % (2 * tpb)) / tpb;
-      lin = r % tpb;
-    }
-    b = 3 * (xcd + 8 * j) + e;
-  } else {
-    b = lin / tpb;
-    lin %= tpb;
-  }
-  rhs_tile(p, b, (lin / gx) * TILE, (lin % gx) * TILE, lds_raw);
+  const int nb = (p.g.N + TILE - 1) / TILE, tpb = nb * nb;
+  const int x = blockIdx.x % 8, P = (gridDim.x - x + 7) / 8;  // workgroups of XCD x
+  int b, tile;
+  for (int it = blockIdx.x / 8; pp_item(x, it, nw, tpb, b, tile); it += P)
+    pp_tile(p, b, (tile / nb) * TILE, (tile % nb) * TILE, lds_raw);
 }
 
 }  // namespace sp

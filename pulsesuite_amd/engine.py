@@ -273,9 +273,9 @@ class WireGroup:
 
     def autotune(self, reps=3):
         """Pick the launch choices of this group by timing them on its own
-        buffers: tune[2] GEMM and split-BF16 RHS block order (XCD-grouped,
-        linear, or for the split-BF16 RHS XCD-grouped with the longest tiles
-        first), tune[1] / tune[3] transport lines per
+        buffers: tune[2] GEMM block order (XCD-grouped or linear; the
+        split-BF16 RHS has a static schedule of its own), tune[1] / tune[3]
+        transport lines per
         block for the row / column pass (4, 2, 8, 16), timed inside full steps
         (the column pass carries the leapfrog average there).  The state is
         unchanged: only per-step scratch is written, and the averaged level
@@ -295,9 +295,7 @@ class WireGroup:
         def full_step():  # psa_step without the level swap
             capi.check(self.lib.psa_step(ctypes.byref(self.S), 0, self.nw, 1e-18, self._stream()))
 
-        # knob 2: 0 XCD-grouped, 1 linear, 2 XCD-grouped with the longest
-        # (six-term dp) RHS tiles first (split-BF16 RHS only)
-        knobs = [(2, [0, 1, 2] if self.prec == "bf16x3" else [0, 1], contractions)]
+        knobs = [(2, [0, 1], contractions)]
         saved = None
         if self.prm.flags.get("DCTrans") and self.prm.flags.get("LF"):
             saved = {k: self.s[k].clone() for k in ("pA", "CA", "DA")}
@@ -319,13 +317,11 @@ class WireGroup:
                 torch.cuda.synchronize()
                 times.append(a.elapsed_time(b) / reps)
             best = int(np.argmin(times))
-            # the XCD-grouped block orders (candidates 0 and 2 of knob 2) read
-            # about half the HBM bytes of the linear one; one of them is kept
-            # unless the linear order is clearly faster (timing noise aside)
-            if idx == 2 and best == 1:
-                grouped = min((i for i in range(len(cands)) if i != 1), key=lambda i: times[i])
-                if times[grouped] <= 1.03 * times[1]:
-                    best = grouped
+            # the XCD-grouped block order (candidate 0 of knob 2) reads about
+            # half the HBM bytes of the linear one; it is kept unless the
+            # linear order is clearly faster (timing noise aside)
+            if idx == 2 and times[0] <= 1.03 * times[best]:
+                best = 0
             self.tune[idx] = cands[best]
             self._sync_struct()
             out[idx] = (self.tune[idx], times)
