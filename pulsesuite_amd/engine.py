@@ -275,12 +275,12 @@ class WireGroup:
         """Pick the launch choices of this group by timing them on its own
         buffers: tune[2] GEMM block order (XCD-grouped or linear; the
         split-BF16 RHS has a static schedule of its own), tune[1] / tune[3]
-        transport lines per
-        block for the row / column pass (4, 2, 8, 16), timed inside full steps
-        (the column pass carries the leapfrog average there).  The state is
-        unchanged: only per-step scratch is written, and the averaged level
-        that the steps overwrite is saved and restored.  Returns {knob:
-        (choice, ms per candidate)}."""
+        transport lines per block for the row / column pass (4, 2, 8, 16),
+        timed over full steps with the level swap (the column pass carries
+        the leapfrog average there, and in the split-BF16 mode writes the
+        next step's operand planes).  The state is unchanged: the levels the
+        steps overwrite are saved and restored.  Returns {knob: (choice, ms
+        per candidate)}."""
         od = bool(self.prm.flags.get("OffDiagDph"))
         self.prop2qw()
         self.prep()
@@ -292,13 +292,13 @@ class WireGroup:
                 self.offdiag()
             self.rhs(1, 1e-18)
 
-        def full_step():  # psa_step without the level swap
-            capi.check(self.lib.psa_step(ctypes.byref(self.S), 0, self.nw, 1e-18, self._stream()))
+        def full_step():  # a step as the time loop runs it (with the level swap,
+            self.step(1e-18)  # so what one step leaves for the next counts)
 
         knobs = [(2, [0, 1], contractions)]
         saved = None
         if self.prm.flags.get("DCTrans") and self.prm.flags.get("LF"):
-            saved = {k: self.s[k].clone() for k in ("pA", "CA", "DA")}
+            saved = {k: self.s[k].clone() for k in ("pA", "pB", "pX", "CA", "CB", "CX", "DA", "DB", "DX")}
             knobs.append((1, [4, 2, 8, 16], full_step))
             knobs.append((3, [4, 2, 8, 16], full_step))
         out = {}
@@ -325,9 +325,10 @@ class WireGroup:
             self.tune[idx] = cands[best]
             self._sync_struct()
             out[idx] = (self.tune[idx], times)
-        if saved is not None:
+        if saved is not None:  # the levels by role (the buffers may have swapped)
             for k, v in saved.items():
                 self.s[k].copy_(v)
+            self.invalidate()
             torch.cuda.synchronize()
         return out
 
