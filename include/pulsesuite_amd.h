@@ -578,6 +578,39 @@ int psa_mod_spont(int32_t N, int32_t Nhw, const double *ne0, const double *nh0, 
                   const double *square, double dhw, double rden, double *Rsp, double *ec,
                   void *stream);
 
+/* CalcMVeh (coulomb.py:1234-1241, _CalcMVeh_core 786-810):
+ * MVeh[k][kp][f] = sum_q p[q][qp][f] VC[k][q][0] UnDel[k+1][q+1] UnDel[kp+1][qp+1]
+ * with qp + 1 = k3[kp][q][k] (0: no term).  p, MVeh complex [N][N][Nf];
+ * k3 int32 [N][N][N]; UnDel [N+1][N+1]. */
+int psa_mod_mveh(int32_t N, int32_t Nf, const double *p, const double *VC, const int32_t *k3,
+                 const double *UnDel, double *MVeh, void *stream);
+
+/* BGRenorm (which 0, on ne = diag C, nh = diag D), EeRenorm (1, ne) and
+ * EhRenorm (2, nh) (coulomb.py:1248-1280; cores 813-897) into BGR complex
+ * [N][N].  ne, nh complex [N] (the one a mode does not read may be NULL). */
+int psa_mod_renorm(int32_t N, int32_t which, const double *ne, const double *nh, const double *VC,
+                   const double *UnDel, double *BGR, void *stream);
+
+/* GetVn1n2 (qwoptics.py:1149-1186, _GetVn1n2_jit 522-542): Vcc, Vvv, Vcv and
+ * Vvc = Vcv^H from rcv complex [N] and Hcc, Hhh, Hcv complex [N][N]. */
+int psa_mod_vn1n2(int32_t N, const double *rcv, const double *Hcc, const double *Hhh,
+                  const double *Hcv, double hbar, double *Vcc, double *Vvv, double *Vcv,
+                  double *Vvc, void *stream);
+
+/* PLSpectrum (emission.py:291-361): PLS[w] = (PLS[w] + rscale sum_j ...) tfac
+ * over Nhw photon energies hw, with hgeh = hbar geh, rden = c0^3 pi^2 hbar^3
+ * (rho0), tfac = softtheta(t, geh); ne, nh complex [N]; work: device scratch
+ * of N + 1 doubles (transition energies and the carrier temperature). */
+int psa_mod_plspectrum(int32_t N, int32_t Nhw, const double *ne, const double *nh,
+                       const double *Ee, const double *Eh, double gap, double hgeh,
+                       const double *VC, const double *hw, double rden, double kB,
+                       double rscale, double tfac, double *work, double *PLS, void *stream);
+
+/* Vehint (coulomb.py:180-242) with dky = ky[k-1] - ky[q-1]: the Vint double
+ * sum with the accurate K0 (usefulsubs.K03) into out[0]. */
+int psa_mod_vehint(double dky, const double *y, int32_t Ny, double alphae, double alphah,
+                   double Delta0, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -137,6 +137,13 @@ SIGNATURES = {
                                         c_dbl, c_ptr, c_ptr, c_ptr]),
     "psa_mod_spont": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_ptr, c_ptr,
                                      c_dbl, c_dbl, c_ptr, c_ptr, c_ptr]),
+    "psa_mod_mveh": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "psa_mod_renorm": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "psa_mod_vn1n2": (ctypes.c_int, [c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_ptr, c_ptr, c_ptr, c_ptr,
+                                     c_ptr]),
+    "psa_mod_plspectrum": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_dbl, c_ptr, c_ptr,
+                                          c_dbl, c_dbl, c_dbl, c_dbl, c_ptr, c_ptr, c_ptr]),
+    "psa_mod_vehint": (ctypes.c_int, [c_dbl, c_ptr, c_i32, c_dbl, c_dbl, c_dbl, c_ptr, c_ptr]),
 }
 
 _lib = None

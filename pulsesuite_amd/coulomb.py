@@ -19,10 +19,13 @@ device; the attributes ``k3``, ``Ceh``, ``Cee``, ``Chi1De`` ... return host
 arrays.  There is no CPU fallback: without the HIP library or a GPU the
 constructor raises.
 
-Not provided (not on the SBE path): ``CalcMVeh``, ``BGRenorm``,
-``EeRenorm``, ``EhRenorm``, ``Vehint``, ``GetChi1Dqw``, ``GetEps1Dqw`` (the
-chi(q, w) record of the SBE drop-in runs on the device through
-``psa_xqw_table`` instead).
+Off the SBE step, also on the device: ``CalcMVeh`` (``psa_mod_mveh``),
+``BGRenorm`` / ``EeRenorm`` / ``EhRenorm`` (``psa_mod_renorm``), ``Vint``
+(``psa_vint_table`` for one momentum) and ``Vehint`` (``psa_mod_vehint``,
+with the accurate K0 as the reference's Vehint uses).  ``GetChi1Dqw`` and
+``GetEps1Dqw`` are scalar diagnostics of one (q, w) and are evaluated on the
+host, as ``CalcChi1D`` is (the chi(q, w) record of the SBE drop-in runs on
+the device through ``psa_xqw_table``).
 """
 
 import ctypes
@@ -156,6 +159,96 @@ def CalcCoulombArrays(y, ky, er, alphae, alphah, L, Delta0, Qy, kkp, ReadArrays=
     return tuple(V)
 
 
+def Vint(Qyk, y, alphae, alphah, Delta0):
+    """coulomb.py:107-177: the interaction integral of one momentum Qyk (the
+    reference's JIT path, crude K0), on the device (psa_vint_table's eh
+    channel with unit prefactor)."""
+    dev = device()
+    lib = capi.load()
+    out = torch.zeros(3, 1, dtype=F64, device=dev)
+    yd, Qd = to_dev(y, F64, dev), to_dev([float(Qyk)], F64, dev)
+    capi.check(lib.psa_vint_table(Qd.data_ptr(), 1, yd.data_ptr(), len(y), float(alphae), float(alphah),
+                                  float(Delta0), 1.0, out.data_ptr(), stream(dev)))
+    return float(to_host(out)[0, 0])
+
+
+def Vehint(k, q, y, ky, alphae, alphah, Delta0):
+    """coulomb.py:180-242: the integral at momentum ky[k-1] - ky[q-1]
+    (1-based k, q; 0 reads index 0) with the accurate K0, on the device."""
+    if len(y) < 4:
+        raise ValueError("Vehint: y needs at least 4 points")
+    ki = k - 1 if k > 0 else 0
+    qi = q - 1 if q > 0 else 0
+    dev = device()
+    lib = capi.load()
+    out = torch.zeros(1, dtype=F64, device=dev)
+    yd = to_dev(y, F64, dev)
+    capi.check(lib.psa_mod_vehint(float(ky[ki] - ky[qi]), yd.data_ptr(), len(y), float(alphae), float(alphah),
+                                  float(Delta0), out.data_ptr(), stream(dev)))
+    return float(to_host(out)[0])
+
+
+def _k0(x):
+    return float(k0_zhang_jin(np.array([x]))[0])
+
+
+def GetChi1Dqw(alphae, alphah, Delta0, epsr, game, gamh, ky, Ee, Eh, ne, nh, qq, w):
+    """coulomb.py:633-709: chi(q, w) of the wire at one (q, w); returns
+    (Re chi, Im chi).  Host scalar diagnostic (one pass over Nk)."""
+    ky = np.asarray(ky, dtype=float)
+    Nk = len(ky)
+    dk = ky[1] - ky[0]
+    hge = np.maximum(hbar * np.asarray(game), 1e-4 * e0)
+    hgh = np.maximum(hbar * np.asarray(gamh), 1e-4 * e0)
+    Re = np.sqrt((2.0 / alphae) ** 2 + Delta0**2)
+    Rh = np.sqrt((2.0 / alphah) ** 2 + Delta0**2)
+    beta = e0**2 / (4 * pi * eps0 * epsr)
+    ql = abs(qq)
+    Ke = _k0(max(ql, alphah / 2.0) * Re) * beta * 2
+    Kh = _k0(max(ql, alphae / 2.0) * Rh) * beta * 2
+    q = int(round(qq / dk))
+    k = np.arange(max(0, -q), min(Nk, Nk - q))
+    kq = k + q
+    chi = 0.0 + 0.0j
+    for a, b in zip(k, kq):
+        de = hbar * w - Ee[b] + Ee[a] + 1j * (hge[b] + hge[a])
+        if abs(de) > 1e-300:
+            chi -= Ke / pi * (ne[a] - ne[b]) / de * dk
+        dh = hbar * w - Eh[b] + Eh[a] + 1j * (hgh[b] + hgh[a])
+        if abs(dh) > 1e-300:
+            chi -= Kh / pi * (nh[a] - nh[b]) / dh * dk
+    return np.real(chi), np.imag(chi)
+
+
+def GetEps1Dqw(alphae, alphah, Delta0, epsr, me, mh, n1D, q, w):
+    """coulomb.py:712-776: the plasmon-pole dielectric function at one
+    (q, w); returns (Re eps, Im eps).  Host scalar diagnostic."""
+    Re = np.sqrt((2.0 / alphae) ** 2 + Delta0**2)
+    Rh = np.sqrt((2.0 / alphah) ** 2 + Delta0**2)
+    beta = e0**2 / (4 * pi * eps0 * epsr)
+    if abs(q) < 1.0:
+        q = 1.0
+    ql = abs(q)
+    Ce = 2 * beta * me / pi / hbar**2 / ql
+    Ch = 2 * beta * mh / pi / hbar**2 / ql
+    Ke = _k0(max(ql, alphah / 2.0) * Re)
+    Kh = _k0(max(ql, alphae / 2.0) * Rh)
+    oep = hbar * ql / 2.0 / me * abs(ql + pi * n1D)
+    oem = hbar * ql / 2.0 / me * abs(ql - pi * n1D)
+    ohp = hbar * ql / 2.0 / mh * abs(ql + pi * n1D)
+    ohm = hbar * ql / 2.0 / mh * abs(ql - pi * n1D)
+    epr = (1.0 - Ce * Ke * np.log(abs((w**2 - oem**2) / (w**2 - oep**2)))
+           - Ch * Kh * np.log(abs((w**2 - ohm**2) / (w**2 - ohp**2))))
+    epi = 0.0
+    if min(oem, oep) < w < max(oem, oep) and min(ohm, ohp) < w < max(ohm, ohp):
+        epi = -Ce * Ke - Ch * Kh
+    if np.isnan(epr) or np.isnan(epi):
+        print(f"NAN in EpsL(q,w) at (q,w) = {q}, {w}")
+        print(f"Re[EpsL(q,w)] = {epr}")
+        print(f"Im[EpsL(q,w)] = {epi}")
+    return epr, epi
+
+
 def CalcChi1D(ky, alphae, alphah, Delta0, epsr, me, mh, qe, qh):
     """coulomb.py:580-630 with the accurate K0 (usefulsubs.K03)."""
     Re = np.sqrt((2.0 / alphae) ** 2 + Delta0**2)
@@ -198,6 +291,7 @@ class CoulombModule:
             "V0": to_dev(np.stack([self.Veh0, self.Vee0, self.Vhh0]), F64, dev),
             "chi": to_dev(np.stack([self.Chi1De, self.Chi1Dh]), F64, dev),
             "q": to_dev(np.stack([self.qe, self.qh]), F64, dev),
+            "ud": to_dev(self.UnDel, F64, dev),
         }
         self._nmax = min(self.qe[1, 1], self.qh[1, 1]) / pi * 0.99
         self._host = {}
@@ -241,6 +335,48 @@ class CoulombModule:
     def undell(self, k, q):
         """coulomb.py:1244-1246: UnDel(k, q) with 1-based indices."""
         return self.UnDel[k, q]
+
+    def CalcMVeh(self, p, VC, MVeh):
+        """coulomb.py:1234-1241: MVeh[k, kp, f] = sum_q p[q, qp, f] Veh[k, q]
+        UnDel[k+1, q+1] UnDel[kp+1, qp+1] over the k3 momentum conservation;
+        p, MVeh (N, N, Nf)."""
+        N, dev = self.N, self.dev
+        p = np.asarray(p)
+        if p.ndim != 3 or p.shape[:2] != (N, N) or np.shape(MVeh) != p.shape or np.shape(VC) != (N, N, 3):
+            raise ValueError("p and MVeh need shape (N, N, Nf) and VC shape (N, N, 3)")
+        pd, vcd = to_dev(p, C128, dev), to_dev(VC, F64, dev)
+        M = torch.empty(p.shape, dtype=C128, device=dev)
+        capi.check(self.lib.psa_mod_mveh(N, p.shape[2], pd.data_ptr(), vcd.data_ptr(), self._d["k3"].data_ptr(),
+                                         self._d["ud"].data_ptr(), M.data_ptr(), stream(dev)))
+        MVeh[...] = to_host(M)
+
+    def _renorm(self, which, ne, nh, VC, BGR):
+        N, dev = self.N, self.dev
+        if np.shape(VC) != (N, N, 3) or np.shape(BGR) != (N, N):
+            raise ValueError("VC needs shape (N, N, 3) and BGR (N, N)")
+        for v in (ne, nh):
+            if v is not None and len(v) != N:
+                raise ValueError("occupations need N entries")
+        ned = None if ne is None else to_dev(ne, C128, dev)
+        nhd = None if nh is None else to_dev(nh, C128, dev)
+        vcd = to_dev(VC, F64, dev)
+        B = torch.empty(N, N, dtype=C128, device=dev)
+        capi.check(self.lib.psa_mod_renorm(N, which, None if ned is None else ned.data_ptr(),
+                                           None if nhd is None else nhd.data_ptr(), vcd.data_ptr(),
+                                           self._d["ud"].data_ptr(), B.data_ptr(), stream(dev)))
+        BGR[...] = to_host(B)
+
+    def BGRenorm(self, C, D, VC, BGR):
+        """coulomb.py:1248-1259: band-gap renormalisation from diag C, diag D."""
+        self._renorm(0, np.diag(C), np.diag(D), VC, BGR)
+
+    def EeRenorm(self, ne, VC, BGR):
+        """coulomb.py:1261-1269: electron energy renormalisation."""
+        self._renorm(1, ne, None, VC, BGR)
+
+    def EhRenorm(self, nh, VC, BGR):
+        """coulomb.py:1271-1280: hole energy renormalisation."""
+        self._renorm(2, None, nh, VC, BGR)
 
     def _mbc(self, hole, ne0, nh0, VC, Win, Wout):
         N, d, dev = self.N, self._d, self.dev
@@ -305,6 +441,22 @@ def Eps1D(n1D, Nk):
 
 def undell(k, q):
     return _require_instance().undell(k, q)
+
+
+def CalcMVeh(p, VC, MVeh, k3=None, UnDel=None):
+    _require_instance().CalcMVeh(p, VC, MVeh)
+
+
+def BGRenorm(C, D, VC, BGR, UnDel=None):
+    _require_instance().BGRenorm(C, D, VC, BGR)
+
+
+def EeRenorm(ne, VC, BGR, UnDel=None):
+    _require_instance().EeRenorm(ne, VC, BGR)
+
+
+def EhRenorm(nh, VC, BGR, UnDel=None):
+    _require_instance().EhRenorm(nh, VC, BGR)
 
 
 def MBCE2(ne0, nh0, ky, Ee, Eh, VC, geh, ge, Win, Wout, k3=None, Ceh=None, Cee=None):

@@ -3312,4 +3312,68 @@ int psa_mod_spont(int32_t N, int32_t Nhw, const double *ne0, const double *nh0, 
   return PSA_OK;
 }
 
+static int mod_blocks(long n) { return (int)((n + mod::RED_T - 1) / mod::RED_T); }
+
+int psa_mod_mveh(int32_t N, int32_t Nf, const double *p, const double *VC, const int32_t *k3, const double *UnDel,
+                 double *MVeh, void *stream) {
+  if (N < 1 || N > 4096 || Nf < 1 || !p || !VC || !k3 || !UnDel || !MVeh)
+    return fail(PSA_E_INVALID, "psa_mod_mveh: bad arguments");
+  hipLaunchKernelGGL(mod::k_mveh, dim3(mod_blocks((long)N * N * Nf)), dim3(mod::RED_T), 0, (hipStream_t)stream, N,
+                     Nf, reinterpret_cast<const cd *>(p), VC, k3, UnDel, reinterpret_cast<cd *>(MVeh));
+  PSA_LAUNCH_CHECK("k_mveh");
+  return PSA_OK;
+}
+
+int psa_mod_renorm(int32_t N, int32_t which, const double *ne, const double *nh, const double *VC,
+                   const double *UnDel, double *BGR, void *stream) {
+  if (N < 1 || N > 4096 || which < 0 || which > 2 || !VC || !UnDel || !BGR || (which != 2 && !ne) ||
+      (which != 1 && !nh))
+    return fail(PSA_E_INVALID, "psa_mod_renorm: bad arguments");
+  hipLaunchKernelGGL(mod::k_renorm, dim3(mod_blocks((long)N * N)), dim3(mod::RED_T), 0, (hipStream_t)stream, N,
+                     which, reinterpret_cast<const cd *>(ne), reinterpret_cast<const cd *>(nh), VC, UnDel,
+                     reinterpret_cast<cd *>(BGR));
+  PSA_LAUNCH_CHECK("k_renorm");
+  return PSA_OK;
+}
+
+int psa_mod_vn1n2(int32_t N, const double *rcv, const double *Hcc, const double *Hhh, const double *Hcv, double hbar,
+                  double *Vcc, double *Vvv, double *Vcv, double *Vvc, void *stream) {
+  if (N < 1 || N > 8192 || !rcv || !Hcc || !Hhh || !Hcv || !Vcc || !Vvv || !Vcv || !Vvc || !(hbar > 0.0))
+    return fail(PSA_E_INVALID, "psa_mod_vn1n2: bad arguments");
+  using C = cd;
+  hipLaunchKernelGGL(mod::k_vn1n2, dim3(mod_blocks((long)N * N)), dim3(mod::RED_T), 0, (hipStream_t)stream, N,
+                     reinterpret_cast<const C *>(rcv), reinterpret_cast<const C *>(Hcc),
+                     reinterpret_cast<const C *>(Hhh), reinterpret_cast<const C *>(Hcv), hbar,
+                     reinterpret_cast<C *>(Vcc), reinterpret_cast<C *>(Vvv), reinterpret_cast<C *>(Vcv),
+                     reinterpret_cast<C *>(Vvc));
+  PSA_LAUNCH_CHECK("k_vn1n2");
+  return PSA_OK;
+}
+
+int psa_mod_plspectrum(int32_t N, int32_t Nhw, const double *ne, const double *nh, const double *Ee,
+                       const double *Eh, double gap, double hgeh, const double *VC, const double *hw, double rden,
+                       double kB, double rscale, double tfac, double *work, double *PLS, void *stream) {
+  if (N < 2 || Nhw < 1 || !ne || !nh || !Ee || !Eh || !VC || !hw || !work || !PLS || !(hgeh > 0.0) ||
+      !(rden > 0.0) || !(kB > 0.0))
+    return fail(PSA_E_INVALID, "psa_mod_plspectrum: bad arguments");
+  const cd *n1 = reinterpret_cast<const cd *>(ne), *n2 = reinterpret_cast<const cd *>(nh);
+  hipLaunchKernelGGL(mod::k_pl_prep, dim3(N + 1), dim3(mod::RED_T), 0, (hipStream_t)stream, N, n1, n2, Ee, Eh, gap,
+                     VC, kB, work);
+  PSA_LAUNCH_CHECK("k_pl_prep");
+  hipLaunchKernelGGL(mod::k_pl, dim3(Nhw), dim3(mod::RED_T), 0, (hipStream_t)stream, N, n1, n2, hw, work, hgeh,
+                     rden, kB, rscale, tfac, PLS);
+  PSA_LAUNCH_CHECK("k_pl");
+  return PSA_OK;
+}
+
+int psa_mod_vehint(double dky, const double *y, int32_t Ny, double alphae, double alphah, double Delta0, double *out,
+                   void *stream) {
+  if (Ny < 4 || !y || !out || !(alphae > 0.0) || !(alphah > 0.0))
+    return fail(PSA_E_INVALID, "psa_mod_vehint: bad arguments");
+  hipLaunchKernelGGL(mod::k_vehint, dim3(1), dim3(mod::RED_T), 0, (hipStream_t)stream, dky, y, Ny, alphae, alphah,
+                     Delta0, out);
+  PSA_LAUNCH_CHECK("k_vehint");
+  return PSA_OK;
+}
+
 }  // extern "C"

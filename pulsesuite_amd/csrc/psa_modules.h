@@ -511,5 +511,290 @@ __global__ __launch_bounds__(RED_T) void k_spont(int N, int Nhw, const cd *ne0, 
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Module functions off the SBE step (the coulomb, emission and qwoptics
+// drop-ins).  Sequential sums run in the reference's order, and contraction
+// into FMAs is off, so these give the reference's pure-Python results.
+// ---------------------------------------------------------------------------
+
+// CalcMVeh (coulomb.py:1234-1241, _CalcMVeh_core 786-810): thread per
+// (k, kp, f), t = (k N + kp) Nf + f;
+//   MVeh[k][kp][f] = sum_q p[q][qp][f] Veh[k][q] U[k+1][q+1] U[kp+1][qp+1],
+// qp + 1 = k3[kp][q][k] (terms with 0 skipped), q ascending.
+__global__ __launch_bounds__(RED_T) void k_mveh(int N, int Nf, const cd *p, const double *VC, const int32_t *k3,
+                                                const double *U, cd *M) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * RED_T + threadIdx.x;
+  if (t >= (long)N * N * Nf) return;
+  const int f = (int)(t % Nf);
+  const long r = t / Nf;
+  const int kp = (int)(r % N), k = (int)(r / N);
+  const long UN = N + 1;
+  cd acc{0.0, 0.0};
+  for (int q = 0; q < N; ++q) {
+    const int qp1 = k3[((long)kp * N + q) * N + k];
+    if (qp1 > 0 && qp1 <= N) {
+      const cd v = p[((long)q * N + (qp1 - 1)) * Nf + f];
+      const double veh = VC[((long)k * N + q) * 3], u1 = U[(k + 1) * UN + q + 1], u2 = U[(kp + 1) * UN + qp1];
+      acc.re = acc.re + ((v.re * veh) * u1) * u2;
+      acc.im = acc.im + ((v.im * veh) * u1) * u2;
+    }
+  }
+  M[t] = acc;
+}
+
+// BGRenorm / EeRenorm / EhRenorm (coulomb.py:1248-1280; _BGRenorm_core
+// 813-831, _EeRenorm_core 834-863, _EhRenorm_core 866-897): thread per
+// (k, kp), t = k N + kp, sums over i ascending.
+//   which 0: BGR = -sum_i nh_i Vhh[i][kp] U[i+1][kp+1] - sum_i ne_i Vee[i][k] U[i+1][k+1]
+//   which 1, 2 (n = ne, V = Vee / n = nh, V = Vhh): with a = sum_i n_i V[kp][kp],
+//   b = sum_i n_i V[i][kp] U[i+1][kp+1], c = sum_i n_i V[k][k],
+//   d = sum_i n_i V[i][k] U[i+1][k+1]: BGR = 2a - b + 2c - d - 2a - c, left to right.
+__global__ __launch_bounds__(RED_T) void k_renorm(int N, int which, const cd *ne, const cd *nh, const double *VC,
+                                                  const double *U, cd *B) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * RED_T + threadIdx.x;
+  if (t >= (long)N * N) return;
+  const int kp = (int)(t % N), k = (int)(t / N);
+  const long UN = N + 1;
+  auto V = [&](int a, int b, int c) { return VC[((long)a * N + b) * 3 + c]; };
+  if (which == 0) {
+    cd sh{0.0, 0.0}, se{0.0, 0.0};
+    for (int i = 0; i < N; ++i) {
+      const double wh = V(i, kp, 2), we = V(i, k, 1);
+      const double uh = U[(i + 1) * UN + kp + 1], ue = U[(i + 1) * UN + k + 1];
+      sh.re = sh.re + (nh[i].re * wh) * uh;
+      sh.im = sh.im + (nh[i].im * wh) * uh;
+      se.re = se.re + (ne[i].re * we) * ue;
+      se.im = se.im + (ne[i].im * we) * ue;
+    }
+    B[t] = cd{-sh.re - se.re, -sh.im - se.im};
+    return;
+  }
+  const cd *n = (which == 1) ? ne : nh;
+  const int c = (which == 1) ? 1 : 2;
+  const double vkpkp = V(kp, kp, c), vkk = V(k, k, c);
+  cd a{0.0, 0.0}, b{0.0, 0.0}, cc{0.0, 0.0}, d{0.0, 0.0};
+  for (int i = 0; i < N; ++i) {
+    const cd x = n[i];
+    const double wkp = V(i, kp, c), wk = V(i, k, c);
+    const double ukp = U[(i + 1) * UN + kp + 1], uk = U[(i + 1) * UN + k + 1];
+    a.re = a.re + x.re * vkpkp;
+    a.im = a.im + x.im * vkpkp;
+    b.re = b.re + (x.re * wkp) * ukp;
+    b.im = b.im + (x.im * wkp) * ukp;
+    cc.re = cc.re + x.re * vkk;
+    cc.im = cc.im + x.im * vkk;
+    d.re = d.re + (x.re * wk) * uk;
+    d.im = d.im + (x.im * wk) * uk;
+  }
+  cd r;
+  r.re = ((((2.0 * a.re - b.re) + 2.0 * cc.re) - d.re) - 2.0 * a.re) - cc.re;
+  r.im = ((((2.0 * a.im - b.im) + 2.0 * cc.im) - d.im) - 2.0 * a.im) - cc.im;
+  B[t] = r;
+}
+
+// complex product as (ar br - ai bi, ar bi + ai br), no contraction
+__device__ __forceinline__ cd cmul_rn(cd a, cd b) {
+#pragma clang fp contract(off)
+  return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// GetVn1n2 (qwoptics.py:1149-1186, _GetVn1n2_jit 522-542): thread per
+// (k1, k2), t = k1 N + k2; Hvv = -Hhh^T, Hvc = Hcv^H, rvc = conj(rcv),
+// m = -i / hbar = (-0, -1/hbar):
+//   Vcv = m (rcv[k1] Hvv[k1][k2] - Hcc[k1][k2] rcv[k2])
+//   Vcc = m (rcv[k1] Hvc[k1][k2] - Hcv[k1][k2] rvc[k2])
+//   Vvv = m (rvc[k1] Hcv[k1][k2] - Hvc[k1][k2] rcv[k2]),  Vvc = Vcv^H.
+__global__ __launch_bounds__(RED_T) void k_vn1n2(int N, const cd *rcv, const cd *Hcc, const cd *Hhh, const cd *Hcv,
+                                                 double hbar, cd *Vcc, cd *Vvv, cd *Vcv, cd *Vvc) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * RED_T + threadIdx.x;
+  if (t >= (long)N * N) return;
+  const int k2 = (int)(t % N), k1 = (int)(t / N);
+  const long tt = (long)k2 * N + k1;
+  const cd r1 = rcv[k1], r2 = rcv[k2];
+  const cd v1 = cconj(r1), v2 = cconj(r2);
+  const cd hh = Hhh[tt], hvv{-hh.re, -hh.im};
+  const cd hcvT = Hcv[tt], hvc{hcvT.re, -hcvT.im};
+  const cd hcc = Hcc[t], hcv = Hcv[t];
+  const double mi = -1.0 / hbar;
+  // m z = (-0 zr - mi zi, -0 zi + mi zr) = (-(mi zi), mi zr) exactly
+  auto mz = [&](cd z) { return cd{-(mi * z.im), mi * z.re}; };
+  const cd zcv = csub(cmul_rn(r1, hvv), cmul_rn(hcc, r2));
+  const cd zcc = csub(cmul_rn(r1, hvc), cmul_rn(hcv, v2));
+  const cd zvv = csub(cmul_rn(v1, hcv), cmul_rn(hvc, r2));
+  const cd vcv = mz(zcv);
+  Vcv[t] = vcv;
+  Vcc[t] = mz(zcc);
+  Vvv[t] = mz(zvv);
+  Vvc[tt] = cconj(vcv);
+}
+
+// PLSpectrum (emission.py:291-361), part 1, grid N + 1: block k < N writes
+// work[k] = gap + Ee[k] + Eh[k] + Ec[k] on |ne|, |nh| (Ec, emission.py:133-171);
+// block N writes work[N] = (Te + Th) / 2 with T = 2 Re(sum n E / (sum n + 1e-100)) / kB
+// (Temperature, AvgEnergy: usefulsubs.py:1203-1253).
+__global__ __launch_bounds__(RED_T) void k_pl_prep(int N, const cd *ne, const cd *nh, const double *Ee,
+                                                   const double *Eh, double gap, const double *VC, double kB,
+                                                   double *work) {
+  __shared__ double sh[2][RED_T];
+  const int k = blockIdx.x;
+  if (k == N) {
+    double nr = 0.0, ni = 0.0, dr = 0.0, di = 0.0, hr = 0.0, hi = 0.0, gr = 0.0, gi = 0.0;
+    for (int i = threadIdx.x; i < N; i += RED_T) {
+      nr += ne[i].re * Ee[i];
+      ni += ne[i].im * Ee[i];
+      dr += ne[i].re;
+      di += ne[i].im;
+      hr += nh[i].re * Eh[i];
+      hi += nh[i].im * Eh[i];
+      gr += nh[i].re;
+      gi += nh[i].im;
+    }
+    block_sum2(nr, ni, sh);
+    block_sum2(dr, di, sh);
+    block_sum2(hr, hi, sh);
+    block_sum2(gr, gi, sh);
+    if (threadIdx.x == 0) {
+      dr += 1e-100;
+      gr += 1e-100;
+      const double ae = (nr * dr + ni * di) / (dr * dr + di * di);
+      const double ah = (hr * gr + hi * gi) / (gr * gr + gi * gi);
+      work[N] = (2.0 * ae / kB + 2.0 * ah / kB) / 2.0;
+    }
+    return;
+  }
+  auto V = [&](int a, int b, int c) { return VC[((long)a * N + b) * 3 + c]; };
+  const double vkk_eh = V(k, k, 0), vkk_ee = V(k, k, 1), vkk_hh = V(k, k, 2);
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = threadIdx.x; i < N; i += RED_T) {
+    const double a = cabs_(ne[i]), b = cabs_(nh[i]), id = (i == k) ? 0.0 : 1.0;
+    s1 += a * (vkk_ee - V(i, k, 1)) + b * (vkk_hh - V(i, k, 2));
+    s2 += a * vkk_eh * id - b * vkk_eh * id;
+  }
+  block_sum2(s1, s2, sh);
+  if (threadIdx.x == 0) work[k] = ((gap + Ee[k]) + Eh[k]) + ((s1 + s2) - vkk_eh);
+}
+
+// LinearInterp on ky = 0, 1, ..., N-1 (libpulsesuite/helpers.py:951-995,
+// locator usefulsubs.py:1803-1830): x0 = j / 100, i = min(floor(x0), N - 2)
+__device__ __forceinline__ void pl_interp(int N, int j, int &i, double &x0) {
+  x0 = (double)j / 100.0;
+  i = min(j / 100, N - 2);
+}
+
+// PLSpectrum part 2: block per photon energy w;
+//   PLS[w] = (PLS[w] + RScale sum_j hw rho0(hw) nenh_j exp(-|hw - E_j| / kB / T)
+//             Lrtz(hw - E_j, hbar geh) softtheta(hw - E_mid, hbar geh)) softtheta(t, geh)
+// over the 100 N interpolated points j (E_j from work, nenh_j = |(ne nh)(q_j)|).
+__global__ __launch_bounds__(RED_T) void k_pl(int N, const cd *ne, const cd *nh, const double *hw,
+                                              const double *work, double hg, double rden, double kB,
+                                              double rscale, double tfac, double *PLS) {
+  __shared__ double sh[2][RED_T];
+  const int w = blockIdx.x, X = 100, M = X * N;
+  const double h = hw[w];
+  const double hr = h * ((h * h) / rden);
+  const double T = work[N];
+  auto E_at = [&](int j) {
+    int i;
+    double x0;
+    pl_interp(N, j, i, x0);
+    return work[i] * ((double)(i + 1) - x0) + work[i + 1] * (x0 - (double)i);
+  };
+  const double st = 0.5 * (1.0 + 2.0 / PI * atan((h - E_at(M / 2)) / hg));
+  const double hg2 = hg * hg;
+  double acc = 0.0, dummy = 0.0;
+  for (int j = threadIdx.x; j < M; j += RED_T) {
+    int i;
+    double x0;
+    pl_interp(N, j, i, x0);
+    const double a = (double)(i + 1) - x0, b = x0 - (double)i;
+    const double E = work[i] * a + work[i + 1] * b;
+    const cd f0 = cmul(ne[i], nh[i]), f1 = cmul(ne[i + 1], nh[i + 1]);
+    const double nn = hypot(f0.re * a + f1.re * b, f0.im * a + f1.im * b);
+    const double d = h - E;
+    const double lz = (hg / PI) / (d * d + hg2);
+    acc += (((hr * nn) * exp(-fabs(d) / kB / T)) * lz) * st;
+  }
+  block_sum2(acc, dummy, sh);
+  if (threadIdx.x == 0) PLS[w] = (PLS[w] + rscale * acc) * tfac;
+}
+
+// Accurate K0(x) for x > 0 (the reference's usefulsubs.K03 through CIK01):
+// Zhang & Jin, Computation of Special Functions (1996), routine CIK01 --
+// power series of I0 for x <= 18 (asymptotic series above), K0 by its power
+// series for x <= 9 and from the asymptotic expansion of I0 K0 above; zero
+// for x > 100 as K03 returns.  The asymptotic coefficients are generated
+// from their closed forms: I0: ((2k-1)!!)^2 / (k! 8^k);
+// I0 K0: c_k = c_{k-1} (2k-1)^3 / (8k).
+__device__ double k0_accurate(double x) {
+  if (x > 100.0) return 0.0;
+  const double z2 = x * x;
+  double i0;
+  if (x <= 18.0) {
+    i0 = 1.0;
+    double cr = 1.0;
+    for (int k = 1; k <= 50; ++k) {
+      cr = 0.25 * cr * z2 / (double)(k * k);
+      i0 += cr;
+      if (fabs(cr / i0) < 1e-15) break;
+    }
+  } else {
+    const int kn = x < 35.0 ? 12 : (x < 50.0 ? 9 : 7);
+    double s = 1.0, num = 1.0, fact = 1.0, p8 = 1.0;
+    for (int k = 1; k <= kn; ++k) {
+      num *= (double)((2 * k - 1) * (2 * k - 1));
+      fact *= (double)k;
+      p8 *= 8.0;
+      s += num / (fact * p8) * pow(x, (double)(-k));
+    }
+    i0 = exp(x) / sqrt(2.0 * PI * x) * s;
+  }
+  if (x <= 9.0) {
+    double cs = 0.0, cw = 0.0, w0 = 0.0, cr = 1.0;
+    const double ct = -log(0.5 * x) - 0.5772156649015329;
+    for (int k = 1; k <= 50; ++k) {
+      w0 += 1.0 / k;
+      cr = 0.25 * cr / (double)(k * k) * z2;
+      cs += cr * (w0 + ct);
+      if (fabs((cs - cw) / cs) < 1e-15) break;
+      cw = cs;
+    }
+    return ct + cs;
+  }
+  double s = 1.0, v = 1.0;
+  for (int k = 1; k <= 10; ++k) {
+    const double m = (double)(2 * k - 1);
+    v *= m * m * m / (8.0 * k);
+    s += v * pow(z2, (double)(-k));
+  }
+  return 0.5 / x * s / i0;
+}
+
+// Vehint (coulomb.py:180-242): one block; the double sum over the middle
+// half of y (N1 = Ny/4 .. N2 = 3Ny/4) of exp(-(ae y_i)^2 - (ah y_j)^2)
+// (ae ah / pi dy^2) K0(dk r_ij), r_ij = sqrt((y_i - y_j)^2 + Delta0^2),
+// dk = max(|dky|, (ae + ah)/4), with the accurate K0.
+__global__ __launch_bounds__(RED_T) void k_vehint(double dky, const double *y, int Ny, double ae, double ah,
+                                                  double Delta0, double *out) {
+  __shared__ double sh[2][RED_T];
+  const int N1 = Ny / 4, N2 = 3 * Ny / 4, M = N2 - N1 + 1;
+  const double dk = fmax(fabs(dky), (ae + ah) / 4.0);
+  const double dy = y[1] - y[0];
+  const double mult = ae * ah / PI * (dy * dy);
+  double acc = 0.0, dummy = 0.0;
+  for (long t = threadIdx.x; t < (long)M * M; t += RED_T) {
+    const int i = N1 + (int)(t / M), j = N1 + (int)(t % M);
+    const double yi = y[i], yj = y[j];
+    const double r = sqrt((yi - yj) * (yi - yj) + Delta0 * Delta0);
+    const double ex = exp(-((ae * yi) * (ae * yi)) - (ah * yj) * (ah * yj));
+    acc += ex * mult * k0_accurate(dk * r);
+  }
+  block_sum2(acc, dummy, sh);
+  if (threadIdx.x == 0) out[0] = acc;
+}
+
 }  // namespace mod
 }  // namespace psa

@@ -9,8 +9,9 @@ with ``SpontEmission`` on the GPU.
   (``psa_mod_spont``).
 * ``SpontIntegral``, ``rho0`` (``emission.py:174-217``): scalar helpers on
   the host.
-
-Not provided (not on the SBE step): ``Calchw``, ``PLSpectrum``.
+* ``Calchw(hw, PLS, Estart, Emax)`` (``emission.py:258-288``): the photon
+  energy grid, and ``PLSpectrum(ne, nh, Ee, Eh, gap, geh, VC, hw, t, PLS)``
+  (``emission.py:291-361``) on the device (``psa_mod_plspectrum``).
 """
 
 import numpy as np
@@ -104,3 +105,36 @@ def Ec(ne, nh, VC):
     if _HOmega is None:
         raise ValueError("HOmega not initialized. Call InitializeEmission first.")
     return _spont(ne, nh, z, z, 0.0, VC, None, True)
+
+
+def Calchw(hw, PLS, Estart, Emax):
+    """emission.py:258-288: hw[w] = Estart + w (Emax - Estart) / Nw; PLS = 0."""
+    Nw = len(hw)
+    hw[:] = 0.0
+    PLS[:] = 0.0
+    dhw = (Emax - Estart) / float(Nw)
+    for w in range(Nw):
+        hw[w] = Estart + w * dhw
+
+
+def PLSpectrum(ne, nh, Ee, Eh, gap, geh, VC, hw, t, PLS):
+    """emission.py:291-361: adds the photoluminescence of the occupations ne,
+    nh (interpolated 100x in k) at the photon energies hw to PLS, then
+    scales PLS by softtheta(t, geh); on the device."""
+    N = len(ne)
+    if N < 2 or len(nh) != N or np.shape(VC) != (N, N, 3) or len(Ee) != N or len(Eh) != N:
+        raise ValueError("ne, nh, Ee, Eh need N >= 2 entries and VC shape (N, N, 3)")
+    if len(PLS) != len(hw):
+        raise ValueError("PLS and hw need the same length")
+    lib, dev = capi.load(), device()
+    ned, nhd = to_dev(ne, C128, dev), to_dev(nh, C128, dev)
+    Ed, Hd, vcd = to_dev(Ee, F64, dev), to_dev(Eh, F64, dev), to_dev(VC, F64, dev)
+    hwd = to_dev(hw, F64, dev)
+    pls = to_dev(np.real(np.asarray(PLS)), F64, dev)
+    work = torch.empty(N + 1, dtype=F64, device=dev)
+    tfac = 0.5 * (1.0 + 2.0 / pi * np.arctan(t / geh))
+    capi.check(lib.psa_mod_plspectrum(N, len(hw), ned.data_ptr(), nhd.data_ptr(), Ed.data_ptr(), Hd.data_ptr(),
+                                      float(gap), hbar * geh, vcd.data_ptr(), hwd.data_ptr(),
+                                      c0**3 * pi**2 * hbar**3, kB, float(_RScale), float(tfac),
+                                      work.data_ptr(), pls.data_ptr(), stream(dev)))
+    PLS[...] = to_host(pls)

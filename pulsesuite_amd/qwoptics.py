@@ -12,16 +12,18 @@ reference's call signatures and in-place outputs.
   propagation grid) through ``psa_mod_*``.  The cubic spline is applied as
   the linear map it is (``rescale_1D`` is linear in the data): its matrix
   is built once per pair of grids on the host and kept on the device.
-* ``yw``, ``WriteQWFields`` (``qwoptics.py:1093-1106, 1316-1349``).
+* ``yw``, ``QWChi1`` (host scalar), ``GetVn1n2`` (on the device,
+  ``psa_mod_vn1n2``), and the file writers ``WriteQWFields``,
+  ``WriteSBESolns``, ``WritePLSpectrum``, ``WritePropFields``,
+  ``printITReal2``, ``printITReal``, ``printIT3D`` in the reference's formats
+  (``qwoptics.py:1093-1386``).  ``printITReal2`` writes the coordinate column
+  into the first file of the process only, a state it shares with the SBE
+  drop-in's Prop snapshots (the reference's ``qwoptics._firsttime``).
 
 The tables the methods read are the instance attributes at call time (the
 reference's methods return early when they are None, and so do these).
 Unlike the reference, constructing ``QWOptics`` writes no ``Envl.y`` file;
 the SBE drop-in writes its start-up files itself.
-
-Not provided (not on the SBE step): ``QWChi1``, ``GetVn1n2``,
-``WriteSBESolns``, ``WritePLSpectrum``, ``WritePropFields`` and the
-``printIT*`` helpers.
 """
 
 import hashlib
@@ -48,6 +50,86 @@ def WriteQWFields(QY, Ex, Ey, Ez, Vr, Px, Py, Pz, Re, Rh, sp, w, xxx):
     for nm, v in (("Ex", Ex), ("Ey", Ey), ("Ez", Ez), ("Vr", Vr), ("Px", Px), ("Py", Py), ("Pz", Pz),
                   ("Re", Re), ("Rh", Rh), ("Rho", Rh - Re)):
         _print_it(np.asarray(v), QY, xxx, f"Wire/{nm}/{nm}.{ws}.{sp}.")
+
+
+def QWChi1(lam, dky, Ee, Eh, area, geh, dcv):
+    """qwoptics.py:1109-1146: linear susceptibility of the wire."""
+    from .SBEs import QWChi1 as _chi1
+    return _chi1(lam, dky, Ee, Eh, area, geh, dcv)
+
+
+def GetVn1n2(kr, rcv, Hcc, Hhh, Hcv, Vcc, Vvv, Vcv, Vvc):
+    """qwoptics.py:1149-1186: the interaction matrices Vcc, Vvv, Vcv and
+    Vvc = Vcv^H, written in place; on the device."""
+    from scipy.constants import hbar
+    N = len(kr)
+    for a in (Hcc, Hhh, Hcv, Vcc, Vvv, Vcv, Vvc):
+        if np.shape(a) != (N, N):
+            raise ValueError("GetVn1n2: matrices need shape (Nk, Nk)")
+    if len(rcv) != N:
+        raise ValueError("GetVn1n2: rcv needs Nk entries")
+    lib, dev = capi.load(), device()
+    ins = [to_dev(a, C128, dev) for a in (rcv, Hcc, Hhh, Hcv)]
+    outs = [torch.empty(N, N, dtype=C128, device=dev) for _ in range(4)]
+    capi.check(lib.psa_mod_vn1n2(N, *[t.data_ptr() for t in ins], hbar, *[t.data_ptr() for t in outs],
+                                 stream(dev)))
+    for dst, t in zip((Vcc, Vvv, Vcv, Vvc), outs):
+        dst[...] = to_host(t)
+
+
+def printITReal2(Dx, z, n, file):
+    """qwoptics.py:1189-1215: Re Dx per line, with z as a second column in
+    the first file of the process; dataQW/<file><n:06d>.dat."""
+    from .SBEs import _print_it_real2
+    _print_it_real2(Dx, z, n, file)
+
+
+def printITReal(Dx, z, n, file):
+    """qwoptics.py:1218-1237: z and Re Dx per line, as float32."""
+    import os
+    fn = f"dataQW/{file}{n:06d}.dat"
+    os.makedirs(os.path.dirname(fn), exist_ok=True)
+    with open(fn, "w", encoding="utf-8") as f:
+        f.write("".join(f"{np.float32(a)} {np.float32(np.real(b))}\n" for a, b in zip(z, Dx)))
+
+
+def printIT3D(Dx, z, n, file):
+    """qwoptics.py:1240-1260: Re and Im of the flattened array, "%.6e"."""
+    import os
+    fn = f"dataQW/{file}{n:06d}.dat"
+    os.makedirs(os.path.dirname(fn), exist_ok=True)
+    Dx = np.asarray(Dx)
+    np.savetxt(fn, np.column_stack([np.real(Dx.flatten()), np.imag(Dx.flatten())]), fmt="%.6e")
+
+
+def WriteSBESolns(ky, ne, nh, C, D, P, Ee, Eh, w, xxx):
+    """qwoptics.py:1263-1291: |C|, |D|, |P| (printIT2D) and ne, nh, Ee, Eh
+    (printIT) of wire w at index xxx under dataQW/Wire/."""
+    from .SBEs import _print_it, _print_it2d
+    ws = f"{w:02d}"
+    for nm, v in (("C", C), ("D", D), ("P", P)):
+        _print_it2d(v, xxx, f"Wire/{nm}/{nm}.{ws}.k.kp.")
+    for nm, v in (("ne", ne), ("nh", nh), ("Ee", Ee), ("Eh", Eh)):
+        _print_it(v, ky, xxx, f"Wire/{nm}/{nm}.{ws}.k.")
+
+
+def WritePLSpectrum(hw, PLS, w, xxx):
+    """qwoptics.py:1294-1313: dataQW/Wire/PL/pl.<ww>.hw.<xxx>.dat against hw / e."""
+    from scipy.constants import e as e0
+
+    from .SBEs import _print_it
+    _print_it(np.asarray(PLS, dtype=complex), np.asarray(hw) / e0, xxx, f"Wire/PL/pl.{w:02d}.hw.")
+
+
+def WritePropFields(y, Ex, Ey, Ez, Vr, Px, Py, Pz, Re, Rh, sp, w, xxx):
+    """qwoptics.py:1352-1386: the propagation-grid fields (printITReal2) and
+    Rho = Rh - Re (printIT) under dataQW/Prop/."""
+    from .SBEs import _print_it, _print_it_real2
+    ws = f"{w:02d}"
+    for nm, v in (("Ex", Ex), ("Ey", Ey), ("Ez", Ez), ("Vr", Vr), ("Px", Px), ("Py", Py), ("Pz", Pz),
+                  ("Re", Re), ("Rh", Rh)):
+        _print_it_real2(v, y, xxx, f"Prop/{nm}/{nm}.{ws}.{sp}.")
+    _print_it(np.asarray(Rh) - np.asarray(Re), y, xxx, f"Prop/Rho/Rho.{ws}.{sp}.")
 
 
 def _key(a):

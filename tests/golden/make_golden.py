@@ -40,13 +40,15 @@ recorded (composed and index-heavy behaviour):
 * ``CalcMBArrays`` with ``LorentzDelta`` = 1 (``coulomb.py:367-499``) and
   ``MBCE``/``MBCH`` on those tables;
 * ``rhoPJ.QuantumWire`` (``rhoPJ.py:117-449``): two wires on a small 3-D
-  grid, three calls with non-zero incoming J and Rho grids.
+  grid, three calls with non-zero incoming J and Rho grids;
+* the module functions off the QWCalculator path (``extras_case``).
 
 Usage: ``python tests/golden/make_golden.py`` (writes ``tests/golden/*.npz``;
 ``python tests/golden/make_golden.py xqw`` writes only ``ref_xqw.npz``,
 ``... snap`` only ``ref_n16_snap.npz`` and ``ref_n16_ignore.npz``,
 ``... lorentz`` only ``ref_n16_lorentz.npz``, ``... qw3d`` only
-``ref_qw3d.npz``).
+``ref_qw3d.npz``, ``... flags`` the LF = 0 and LorentzDelta = 1
+trajectories, ``... extras`` only ``ref_modx.npz``).
 """
 
 import contextlib
@@ -498,6 +500,144 @@ def xqw_case(nk=16, seed=5, nx=64, drr=10e-9):
     print("wrote", path, "N =", nk)
 
 
+def extras_case(nk=16, seed=17, nx=64, drr=10e-9):
+    """The module functions off the QWCalculator path, on a solver's own
+    grid and tables: CalcMVeh, BGRenorm, EeRenorm, EhRenorm
+    (coulomb.py:1234-1280), Vint and Vehint (coulomb.py:107-242),
+    GetChi1Dqw and GetEps1Dqw (coulomb.py:633-776), Calchw and PLSpectrum
+    (emission.py:258-357), QWChi1 and GetVn1n2 (qwoptics.py:1109-1186), and
+    the files of printITReal2, printITReal, printIT3D, WriteSBESolns,
+    WritePLSpectrum and WritePropFields (qwoptics.py:1189-1386) as text."""
+    rng = np.random.default_rng(seed)
+    L = wire_length(nk)
+    out = {}
+
+    def crand(*shape):
+        return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+    with tempfile.TemporaryDirectory() as d:
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            write_params(d, L, MB_SHIPPED)
+            buf = io.StringIO()
+            with contextlib.redirect_stdout(buf):
+                from pulsesuite.PSTD3D import SBEs, coulomb, emission, qwoptics
+                from pulsesuite.PSTD3D.typespace import GetKArray, GetSpaceArray
+                rr = GetSpaceArray(nx, (nx - 1) * drr)
+                qrr = GetKArray(nx, nx * drr)
+                s = SBEs.SBESolver(qrr, rr, 0.0, 1e7, 800e-9, 1, True)
+            n = s.Nk
+            assert n == nk
+            C, D, p = random_state(rng, n)
+            ne = np.diag(C).copy()
+            nh = np.diag(D).copy()
+            VC = np.zeros((n, n, 3))
+            E1D = np.zeros((n, n))
+            with contextlib.redirect_stdout(buf):
+                coulomb.CalcScreenedArrays(True, s.L, ne, nh, VC, E1D)
+            out.update(nk=n, L=L, kr=s.kr, r=s.r, Ee=s.Ee, Eh=s.Eh, gap=s.gap, area=s.area,
+                       dcv=s.dcv, ehint=s.ehint, epsr=s.epsr, me=s.me, mh=s.mh, Delta0=s.Delta0,
+                       alphae=s.alphae, alphah=s.alphah, gam_e=s.gam_e, gam_h=s.gam_h,
+                       gam_eh=s.gam_eh, fC=C, fD=D, fp=p, fVC=VC)
+            # CalcMVeh on a three-frequency p[q, qp, f]
+            p3 = crand(n, n, 3)
+            MV = np.zeros((n, n, 3), dtype=complex)
+            coulomb.CalcMVeh(p3, VC, MV)
+            out.update(p3=p3, MVeh=MV)
+            B = np.zeros((n, n), dtype=complex)
+            coulomb.BGRenorm(C, D, VC, B)
+            out["BGR"] = B.copy()
+            coulomb.EeRenorm(ne, VC, B)
+            out["EeR"] = B.copy()
+            coulomb.EhRenorm(nh, VC, B)
+            out["EhR"] = B.copy()
+            # Vint / Vehint on the solver's r grid
+            qs = np.array([0.0, 1e5, 0.37 * s.kr[-1], -2.0 * s.kr[3], 5e9])
+            out["vint_q"] = qs
+            out["vint"] = np.array([coulomb.Vint(q, s.r, s.alphae, s.alphah, s.Delta0) for q in qs])
+            kq = np.array([[0, 0], [1, 1], [1, 5], [7, 2], [n, 1], [3, n]])
+            out["vehint_kq"] = kq
+            out["vehint"] = np.array([coulomb.Vehint(int(a), int(b), s.r, s.kr, s.alphae, s.alphah,
+                                                     s.Delta0) for a, b in kq])
+            # chi(q, w) and eps(q, w) diagnostics
+            game = rng.uniform(1e11, 1e13, n)
+            gamh = rng.uniform(1e11, 1e13, n)
+            game[0] = 1.0
+            fe = rng.uniform(0.0, 0.6, n)
+            fh = rng.uniform(0.0, 0.6, n)
+            dkr = s.kr[1] - s.kr[0]
+            wg = s.gap / SBEs.hbar
+            qw = [(0.0, 0.3 * wg), (dkr, 0.01 * wg), (3.2 * dkr, 0.2 * wg), (-4.0 * dkr, 0.05 * wg),
+                  (0.5, 1.0 * wg), (n * dkr, 0.7 * wg)]
+            chi = [coulomb.GetChi1Dqw(s.alphae, s.alphah, s.Delta0, s.epsr, game, gamh, s.kr, s.Ee,
+                                      s.Eh, fe, fh, q, w) for q, w in qw]
+            n1d = 2.0e7
+            qw2 = [(0.5, 1e12), (1.0e6, 1e12), (3.0e7, 2e13), (8.0e7, 5e12), (2.0e8, 1e14), (6.3e7, 4.0e12),
+                   (1.0e8, 9.0e13)]
+            eps = [coulomb.GetEps1Dqw(s.alphae, s.alphah, s.Delta0, s.epsr, s.me, s.mh, n1d, q, w)
+                   for q, w in qw2]
+            out.update(game=game, gamh=gamh, fe=fe, fh=fh, chi_qw=np.array(qw), chi=np.array(chi),
+                       eps_n1d=n1d, eps_qw=np.array(qw2), eps=np.array(eps))
+            # photoluminescence
+            hw = np.zeros(40)
+            PLS = np.zeros(40)
+            emission.Calchw(hw, PLS, s.gap - 0.05 * SBEs.eV, s.gap + 0.25 * SBEs.eV)
+            out["hw"] = hw.copy()
+            PLS[:] = rng.uniform(0.0, 1e-7, 40)
+            out["pls0"] = PLS.copy()
+            nep = np.abs(ne) * np.exp(0.3j * np.arange(n))
+            nhp = np.abs(nh) * np.exp(-0.2j * np.arange(n))
+            # t enters only through softtheta(t, geh) (emission.py:361)
+            t_pl = 0.37 * s.gam_eh
+            emission.PLSpectrum(nep, nhp, s.Ee, s.Eh, s.gap, s.gam_eh, VC, hw, t_pl, PLS)
+            out.update(pl_ne=nep, pl_nh=nhp, pl_t=t_pl, pls=PLS.copy(), rscale=emission._RScale)
+            out["qwchi1"] = qwoptics.QWChi1(800e-9, dkr, s.Ee, s.Eh, s.area, s.gam_eh, s.dcv)
+            # GetVn1n2
+            rcv = crand(n)
+            Hcc, Hhh, Hcv = crand(n, n), crand(n, n), crand(n, n)
+            V = [np.zeros((n, n), dtype=complex) for _ in range(4)]
+            qwoptics.GetVn1n2(s.kr, rcv, Hcc, Hhh, Hcv, *V)
+            out.update(rcv=rcv, Hcc=Hcc, Hhh=Hhh, Hcv=Hcv, Vcc=V[0], Vvv=V[1], Vcv=V[2], Vvc=V[3])
+            # file writers (qwoptics._firsttime starts True: printITReal2's first file has two columns)
+            f1 = crand(8)
+            y8 = np.linspace(-1e-7, 1e-7, 8)
+            qwoptics._firsttime = True
+            qwoptics.printITReal2(f1, y8, 3, "t/r2a.")
+            qwoptics.printITReal2(f1, y8, 4, "t/r2b.")
+            qwoptics.printITReal(f1, y8, 5, "t/r1.")
+            c3 = crand(3, 4, 2)
+            qwoptics.printIT3D(c3, y8, 6, "t/c3.")
+            qwoptics._firsttime = True
+            F = [crand(8) for _ in range(9)]
+            qwoptics.WritePropFields(y8, *F, "r", 2, 7)
+            qwoptics.WriteSBESolns(s.kr, ne, nh, C, D, p, s.Ee + 0j, s.Eh + 0j, 2, 8)
+            qwoptics.WritePLSpectrum(hw, PLS, 2, 9)
+            out.update(f1=f1, y8=y8, F=np.array(F), c3=c3)
+            names, texts = [], []
+            for root, _, files in os.walk("dataQW"):
+                for fn in sorted(files):
+                    pth = os.path.join(root, fn)
+                    if not (pth.startswith(os.path.join("dataQW", "t")) or fn.endswith(("0000007.dat",
+                            "000007.dat", "000008.dat", "0000008.dat", "000009.dat"))):
+                        continue
+                    if "Wire" in pth and not any(t in pth for t in ("/C/", "/D/", "/P/", "/ne/", "/nh/",
+                                                                     "/Ee/", "/Eh/", "/PL/")):
+                        continue
+                    names.append(os.path.relpath(pth, "dataQW"))
+                    with open(pth) as fh_:
+                        texts.append(fh_.read())
+            order = np.argsort(names)
+            out["file_names"] = np.array(names)[order]
+            out["file_texts"] = np.array(texts)[order]
+            s.close()
+        finally:
+            os.chdir(cwd)
+    path = os.path.join(OUT_DIR, "ref_modx.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "N =", nk, "files", len(out["file_names"]))
+
+
 def main():
     if not os.path.isdir(REF_SRC):
         raise SystemExit("reference tree not found; run in the build container")
@@ -521,6 +661,9 @@ def main():
     if sys.argv[1:] == ["flags"]:
         flag_cases()
         return
+    if sys.argv[1:] == ["extras"]:
+        extras_case()
+        return
     run_case(16, seed=1, mb=MB_SHIPPED, nsteps=4, tag="n16_mb")
     run_case(16, seed=2, mb=MB_RHS_ONLY, nsteps=3, tag="n16_rhs")
     run_case(32, seed=3, mb=MB_SHIPPED, nsteps=3, tag="n32_mb")
@@ -530,6 +673,7 @@ def main():
     lorentz_case()
     qw3d_case()
     flag_cases()
+    extras_case()
 
 
 def flag_cases():
