@@ -351,11 +351,12 @@ __device__ __forceinline__ void slab(const Stage &st, uint32_t fre, uint32_t fim
 // one wave issues MFMAs while the other one loads.  After the last slab the
 // two partial sums are added in LDS (even slabs + odd slabs, fixed order)
 // and all 512 threads run the FP64 epilogue.  Tiles are assigned
-// statically: the tiles of wire w belong to XCD w % 8, listed dp tiles
-// first (six terms) and then the dC, dD tiles; workgroup j of an XCD takes
-// items j, j + P, j + 2P, ... of its list (P workgroups per XCD), which
-// balances the term count per CU when the XCD has 32 CUs and 4 tiles per
-// equation.
+// statically: the tiles of wire w belong to XCD w % 8, listed wire by wire
+// (the dp, dC, dD tiles of a wire together, so the tiles that read the same
+// operand slabs run at the same time and meet in the XCD's L2); workgroup j
+// of an XCD takes items j, j + P, j + 2P, ... of its list (P workgroups per
+// XCD).  With 32 CUs per XCD and 4 tiles per equation, every CU gets one
+// dp tile (six terms) and one dC and one dD tile (four terms each).
 // ---------------------------------------------------------------------------
 constexpr int PP_THREADS = 512;
 
@@ -365,20 +366,10 @@ constexpr size_t pp_lds_bytes() { return 2 * sizeof(Stage); }
 // in the equation, or false past the list's end.
 __device__ __forceinline__ bool pp_item(int x, int it, int nw, int tpb, int &b, int &tile) {
   const int nwx = nw > x ? (nw - x + 7) / 8 : 0;  // wires x, x + 8, ...
-  const int ndp = nwx * tpb;
-  if (it >= 3 * ndp) return false;
-  int u, e;
-  if (it < ndp) {
-    u = it / tpb;
-    e = 0;
-    tile = it % tpb;
-  } else {
-    const int r = it - ndp;
-    u = r / (2 * tpb);
-    e = 1 + (r % (2 * tpb)) / tpb;
-    tile = r % tpb;
-  }
-  b = 3 * (x + 8 * u) + e;
+  if (it >= 3 * nwx * tpb) return false;
+  const int u = it / (3 * tpb), r = it % (3 * tpb);
+  tile = r % tpb;
+  b = 3 * (x + 8 * u) + r / tpb;
   return true;
 }
 
