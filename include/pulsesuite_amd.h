@@ -464,7 +464,12 @@ int psa_rhs_multi(const void *packed_host, const void *packed_dev, uint32_t *que
  * with the RHS of all of them as the one psa_rhs_multi launch; the
  * description must be packed with mode 1 and this dt (and repacked after
  * the caller exchanges the levels: one description per level parity).
- * Results equal psa_step's per group. */
+ * Results equal psa_step's per group.  With more than one group, each
+ * group's stages before and after the RHS run on a library-owned stream
+ * per group (default priority, created on the first call per device),
+ * forked from `stream` and joined back into it by events; under graph
+ * capture they do so once an eager call has created the streams, and run
+ * on `stream` in group order otherwise. */
 int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *queue, double dt,
                    void *stream);
 

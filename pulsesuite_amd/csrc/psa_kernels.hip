@@ -2231,15 +2231,19 @@ int psa_rhs_multi_pack(const psa_group *const *groups, const int32_t *w0, const 
   return PSA_OK;
 }
 
-int psa_rhs_multi(const void *packed_host, const void *packed_dev, uint32_t *queue, void *stream) {
+}  // extern "C"
+
+namespace {
+// psa_rhs_multi; with splits false the caller has written every group's
+// bf16 operand planes already
+int rhs_multi_impl(const void *packed_host, const void *packed_dev, uint32_t *queue, hipStream_t s, bool splits) {
   if (!packed_host || !packed_dev || !queue) return fail(PSA_E_INVALID, "psa_rhs_multi: null argument");
   MultiHdr h;
   std::memcpy(&h, packed_host, sizeof(h));
   if (h.n < 1 || h.n > PSA_RHS_MULTI_MAX || h.nwires < 1 || h.kind < 0 || h.kind > 2 || h.xstart[NXCD] != h.nwires)
     return fail(PSA_E_INVALID, "psa_rhs_multi: not a packed description");
-  const hipStream_t s = (hipStream_t)stream;
   const char *bh = static_cast<const char *>(packed_host), *bd = static_cast<const char *>(packed_dev);
-  if (h.kind == 2) {
+  if (h.kind == 2 && splits) {
     const RhsProb *probs_h = reinterpret_cast<const RhsProb *>(bh + multi_hdr_bytes());
     for (int j = 0; j < h.n; ++j) {
       RhsProb p;
@@ -2262,6 +2266,13 @@ int psa_rhs_multi(const void *packed_host, const void *packed_dev, uint32_t *que
   if (h.kind == 0) return launch_rhs_queue<0>(probs_d, ents_d, q, tot, qu, s);
   if (h.kind == 1) return launch_rhs_queue<1>(probs_d, ents_d, q, tot, qu, s);
   return launch_rhs_queue<2>(probs_d, ents_d, q, tot, qu, s);
+}
+}  // namespace
+
+extern "C" {
+
+int psa_rhs_multi(const void *packed_host, const void *packed_dev, uint32_t *queue, void *stream) {
+  return rhs_multi_impl(packed_host, packed_dev, queue, (hipStream_t)stream, true);
 }
 
 int psa_relaxation(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
@@ -2411,6 +2422,50 @@ int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStr
     case 512: return launch_transport<512>(g, w0, nw, s, avg, planes);
     default: return launch_transport<0>(g, w0, nw, s, avg, false);
   }
+}
+
+// Library-owned streams for psa_step_multi's groups: one set per device,
+// created on first use with the default priority and kept until the
+// process ends, with the fork and join events.
+struct AuxStreams {
+  hipStream_t st[PSA_RHS_MULTI_MAX];
+  hipEvent_t fork, mid, join[2][PSA_RHS_MULTI_MAX];
+};
+constexpr int kAuxDevices = 64;
+AuxStreams *g_aux[kAuxDevices];
+std::mutex g_aux_mu;
+
+int aux_streams(AuxStreams **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kAuxDevices)
+    return fail(PSA_E_HIP, "psa_step_multi: no current device below index 64");
+  std::lock_guard<std::mutex> lock(g_aux_mu);
+  if (!g_aux[dev]) {
+    AuxStreams *a = new AuxStreams();
+    for (int j = 0; j < PSA_RHS_MULTI_MAX; ++j) {
+      if (int rc = hip_check(hipStreamCreateWithFlags(&a->st[j], hipStreamNonBlocking),
+                             "psa_step_multi: hipStreamCreateWithFlags"))
+        return rc;
+      for (int k = 0; k < 2; ++k)
+        if (int rc = hip_check(hipEventCreateWithFlags(&a->join[k][j], hipEventDisableTiming),
+                               "psa_step_multi: hipEventCreateWithFlags"))
+          return rc;
+    }
+    if (int rc = hip_check(hipEventCreateWithFlags(&a->fork, hipEventDisableTiming), "psa_step_multi: hipEventCreate"))
+      return rc;
+    if (int rc = hip_check(hipEventCreateWithFlags(&a->mid, hipEventDisableTiming), "psa_step_multi: hipEventCreate"))
+      return rc;
+    g_aux[dev] = a;
+  }
+  *out = g_aux[dev];
+  return PSA_OK;
+}
+
+AuxStreams *aux_streams_existing() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kAuxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_aux_mu);
+  return g_aux[dev];
 }
 
 int finish_impl(const psa_group *g, int32_t w0, int32_t nw, hipStream_t s, bool averaged) {
@@ -2609,27 +2664,66 @@ int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *qu
     gs[j] = p.g;
     w0[j] = p.w0;
   }
+  const hipStream_t s = (hipStream_t)stream;
+  // The groups' stages before and after the RHS run on a stream per group
+  // (forked from and joined back into the caller's stream by events), so
+  // the small groups' kernels fill the GPU together.  A single group runs
+  // on the caller's stream; so do all groups under graph capture when no
+  // eager call has created the streams yet (streams are not created while
+  // a capture is open).
+  AuxStreams *aux = nullptr;
+  if (h.n > 1) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (int rc = hip_check(hipStreamIsCapturing(s, &cap), "psa_step_multi: hipStreamIsCapturing")) return rc;
+    if (cap == hipStreamCaptureStatusNone) {
+      if (int rc = aux_streams(&aux)) return rc;
+    } else {
+      aux = aux_streams_existing();
+    }
+  }
+  auto on = [&](int j) { return aux ? aux->st[j] : s; };
+  auto fork = [&](hipEvent_t ev) -> int {
+    if (!aux) return PSA_OK;
+    if (int rc = hip_check(hipEventRecord(ev, s), "psa_step_multi: hipEventRecord")) return rc;
+    for (int j = 0; j < h.n; ++j)
+      if (int rc = hip_check(hipStreamWaitEvent(aux->st[j], ev, 0), "psa_step_multi: hipStreamWaitEvent")) return rc;
+    return PSA_OK;
+  };
+  auto join = [&](hipEvent_t *evs) -> int {
+    if (!aux) return PSA_OK;
+    for (int j = 0; j < h.n; ++j) {
+      if (int rc = hip_check(hipEventRecord(evs[j], aux->st[j]), "psa_step_multi: hipEventRecord")) return rc;
+      if (int rc = hip_check(hipStreamWaitEvent(s, evs[j], 0), "psa_step_multi: hipStreamWaitEvent")) return rc;
+    }
+    return PSA_OK;
+  };
   int rc;
   // the single-group step (psa_step) up to its RHS, for every group
+  if ((rc = fork(aux ? aux->fork : nullptr))) return rc;
   for (int j = 0; j < h.n; ++j) {
     const psa_group *g = &gs[j];
-    if ((rc = psa_prop2qw(g, w0[j], h.nw[j], stream))) return rc;
-    if ((rc = psa_prep(g, w0[j], h.nw[j], stream))) return rc;
-    if ((rc = psa_calc_h(g, w0[j], h.nw[j], stream))) return rc;
+    void *sj = on(j);
+    if ((rc = psa_prop2qw(g, w0[j], h.nw[j], sj))) return rc;
+    if ((rc = psa_prep(g, w0[j], h.nw[j], sj))) return rc;
+    if ((rc = psa_calc_h(g, w0[j], h.nw[j], sj))) return rc;
     if (g->flags & PSA_F_OFFDIAGDPH)
-      if ((rc = psa_offdiag_tables(g, w0[j], h.nw[j], stream))) return rc;
+      if ((rc = psa_offdiag_tables(g, w0[j], h.nw[j], sj))) return rc;
+    if (g->prec == PSA_PREC_BF16X3 && (rc = split_planes(*g, w0[j], h.nw[j], on(j)))) return rc;
   }
-  if ((rc = psa_rhs_multi(packed_host, packed_dev, queue, stream))) return rc;
+  if ((rc = join(aux ? aux->join[0] : nullptr))) return rc;
+  if ((rc = rhs_multi_impl(packed_host, packed_dev, queue, s, false))) return rc;
+  if ((rc = fork(aux ? aux->mid : nullptr))) return rc;
   for (int j = 0; j < h.n; ++j) {
     const psa_group *g = &gs[j];
-    if ((rc = psa_relaxation(g, w0[j], h.nw[j], dt, stream))) return rc;
+    void *sj = on(j);
+    if ((rc = psa_relaxation(g, w0[j], h.nw[j], dt, sj))) return rc;
     const bool trans = (g->flags & PSA_F_DCTRANS) != 0, fused = trans && (g->flags & PSA_F_LF);
-    if (trans && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)stream, true, true))) return rc;
-    if ((rc = finish_impl(g, w0[j], h.nw[j], (hipStream_t)stream, fused))) return rc;
-    if ((rc = psa_sources(g, w0[j], h.nw[j], stream))) return rc;
-    if ((rc = psa_qw2prop(g, w0[j], h.nw[j], stream))) return rc;
+    if (trans && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)sj, true, true))) return rc;
+    if ((rc = finish_impl(g, w0[j], h.nw[j], (hipStream_t)sj, fused))) return rc;
+    if ((rc = psa_sources(g, w0[j], h.nw[j], sj))) return rc;
+    if ((rc = psa_qw2prop(g, w0[j], h.nw[j], sj))) return rc;
   }
-  return PSA_OK;
+  return join(aux ? aux->join[1] : nullptr);
 }
 
 int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {

@@ -2,7 +2,9 @@
 wire groups of different N in one launch over a work queue give exactly the
 per-group psa_rhs results, in FP64, FP32 and split-BF16, for the raw RHS
 (mode 0) and the leapfrog write (mode 1); repeated launches (the queue
-resets itself) and a graph replay give the same bits.  Uneven Nk
+resets itself) and a graph replay give the same bits; the multi-group
+step (its groups on streams of their own) equals the per-group steps, also
+replayed from a graph.  Uneven Nk
 {64, 128, 192, 256} stepped together in the reduced modes stay within
 1e-6 of FP64 in density and polarization."""
 
@@ -97,6 +99,51 @@ def test_multi_step_equals_per_group_steps(prec):
         ms()
     torch.cuda.synchronize()
     assert len(ms._packs) == 2  # one description per level parity
+    for ga, gb in zip(a, b):
+        for x, y in zip(ga.get_state(), gb.get_state()):
+            assert np.array_equal(x, y)
+        for x, y in zip(ga.outputs(), gb.outputs()):
+            assert np.array_equal(x, y)
+
+
+def _noise():
+    import torch
+    x = torch.randn(1 << 16, device="cuda", dtype=torch.float64)
+    return float((x * x).sum())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_multi_step_graph_replay_matches_eager(prec):
+    """psa_step_multi runs each group's stages on a stream of its own; a
+    graph captured after eager calls (the streams exist, so the capture
+    forks into them and joins back) and replayed three times, with eager
+    work and synchronisations between the replays, gives the eager run's
+    bits."""
+    import torch
+    from pulsesuite_amd.engine import MultiStep
+    print(f"[step_multi graph] prec={prec}", flush=True)
+    dt = 10e-18
+    a = _groups(prec, seed=7)
+    ma = MultiStep(a, dt)
+    for _ in range(8):
+        ma()
+    torch.cuda.synchronize()
+    b = _groups(prec, seed=7)
+    mb = MultiStep(b, dt)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        for _ in range(2):
+            mb()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):  # two steps: back to this level parity
+        mb()
+        mb()
+    for _ in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        _noise()
+    torch.cuda.synchronize()
     for ga, gb in zip(a, b):
         for x, y in zip(ga.get_state(), gb.get_state()):
             assert np.array_equal(x, y)
