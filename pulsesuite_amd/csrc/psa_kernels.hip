@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -1870,21 +1871,34 @@ int launch_gemm_splitk(const P &p, int M, int ncol, int K, hipStream_t s, const 
 
 // The split-bf16 operand planes of wires [w0, w0 + nw) (sp::k_split, and
 // sp::k_split_band for the off-diagonal dephasing bands).
-int split_planes(const psa_group &g, int w0, int nw, hipStream_t s) {
+// the operand planes of p, C, D (unless carried) and of the Hamiltonians
+int split_main(const psa_group &g, int w0, int nw, hipStream_t s) {
   if (!g.spl) return fail(PSA_E_INVALID, "bf16x3 mode needs the spl operand buffer");
   const int nt = (g.N + 63) / 64;
   hipLaunchKernelGGL(sp::k_split, dim3(nt, nt, 6 * nw), dim3(256), 0, s, g, w0);
   PSA_LAUNCH_CHECK("sp::k_split");
+  return PSA_OK;
+}
+
+// the band planes of the off-diagonal dephasing tables
+int split_band(const psa_group &g, int w0, int nw, hipStream_t s) {
+  if (!g.spl) return fail(PSA_E_INVALID, "bf16x3 mode needs the spl operand buffer");
   if (g.flags & PSA_F_OFFDIAGDPH) {
+    const int nt = (g.N + 63) / 64;
     hipLaunchKernelGGL(sp::k_split_band, dim3(nt, 2, nw), dim3(256), 0, s, g, w0);
     PSA_LAUNCH_CHECK("sp::k_split_band");
   }
   return PSA_OK;
 }
 
+int split_planes(const psa_group &g, int w0, int nw, hipStream_t s) {
+  if (int rc = split_main(g, w0, nw, s)) return rc;
+  return split_band(g, w0, nw, s);
+}
+
 // Split-bf16 RHS: operand planes once per step, then the bf16 MFMA
 // products (sp::k_rhs_pp, one persistent workgroup of 8 waves per CU).
-int rhs_bf16(const RhsProb &p, int nw, hipStream_t s) {
+int rhs_bf16(const RhsProb &p, int nw, hipStream_t s, bool split = true) {
   static int cus = 0;
   if (!cus) {
     int rc = hip_check(hipFuncSetAttribute((const void *)sp::k_rhs_pp, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1896,7 +1910,8 @@ int rhs_bf16(const RhsProb &p, int nw, hipStream_t s) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
   }
-  if (int rc = split_planes(p.g, p.w0, nw, s)) return rc;
+  if (split)
+    if (int rc = split_planes(p.g, p.w0, nw, s)) return rc;
   hipLaunchKernelGGL(sp::k_rhs_pp, dim3(cus), dim3(sp::PP_THREADS), sp::pp_lds_bytes(), s, p, nw);
   PSA_LAUNCH_CHECK("sp::k_rhs_pp");
   return PSA_OK;
@@ -2149,7 +2164,12 @@ int psa_offdiag_tables(const psa_group *g, int32_t w0, int32_t nw, void *stream)
   return launch_gemm(p, prec, g->N, 2 * nw, 2 * (2 * g->N - 1), s, "offdiag gemm");
 }
 
-int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt, void *stream) {
+}  // extern "C"
+
+namespace {
+// psa_rhs; with split false (split-BF16) the caller has written the
+// operand planes already
+int rhs_launch(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt, hipStream_t s, bool split) {
   if (int rc = check_group(g, w0, nw)) return rc;
   if (mode < 0 || mode > 5) return fail(PSA_E_INVALID, "psa_rhs: mode must be in [0, 5]");
   if (mode >= 2 && (!g->pR || !g->CR || !g->DR))
@@ -2159,8 +2179,15 @@ int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt,
   p.w0 = w0;
   p.mode = mode;
   p.dt = dt;
-  if (g->prec == PSA_PREC_BF16X3) return rhs_bf16(p, nw, (hipStream_t)stream);
-  return launch_gemm(p, g->prec, g->N, g->N, 3 * nw, (hipStream_t)stream, "rhs gemm");
+  if (g->prec == PSA_PREC_BF16X3) return rhs_bf16(p, nw, s, split);
+  return launch_gemm(p, g->prec, g->N, g->N, 3 * nw, s, "rhs gemm");
+}
+}  // namespace
+
+extern "C" {
+
+int psa_rhs(const psa_group *g, int32_t w0, int32_t nw, int32_t mode, double dt, void *stream) {
+  return rhs_launch(g, w0, nw, mode, dt, (hipStream_t)stream, true);
 }
 
 int64_t psa_rhs_multi_bytes(const int32_t *nw, int32_t n) {
@@ -2424,48 +2451,75 @@ int transport_impl(const psa_group *g, int32_t w0, int32_t nw, double dt, hipStr
   }
 }
 
-// Library-owned streams for psa_step_multi's groups: one set per device,
-// created on first use with the default priority and kept until the
-// process ends, with the fork and join events.
+// Library-owned streams that the steps fork into: one set per (device,
+// caller stream), created on first use with the default priority and kept
+// until the process ends, with the events of the forks and joins.  A set
+// serves one caller stream, so steps on different caller streams (or host
+// threads) never share streams or events.
 struct AuxStreams {
-  hipStream_t st[PSA_RHS_MULTI_MAX];
+  hipStream_t st[PSA_RHS_MULTI_MAX];  // created on first use each
   hipEvent_t fork, mid, join[2][PSA_RHS_MULTI_MAX];
 };
-constexpr int kAuxDevices = 64;
-AuxStreams *g_aux[kAuxDevices];
+std::map<std::pair<int, hipStream_t>, AuxStreams *> g_aux;
 std::mutex g_aux_mu;
 
-int aux_streams(AuxStreams **out) {
+// The set of caller stream s with streams 0 .. n-1 created; with create
+// false (under graph capture, where no stream may be created) only a set
+// whose streams exist already, else nullptr.
+int aux_streams(hipStream_t s, int n, bool create, AuxStreams **out) {
+  *out = nullptr;
   int dev = -1;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kAuxDevices)
-    return fail(PSA_E_HIP, "psa_step_multi: no current device below index 64");
+  if (hipGetDevice(&dev) != hipSuccess) return fail(PSA_E_HIP, "aux streams: no current device");
   std::lock_guard<std::mutex> lock(g_aux_mu);
-  if (!g_aux[dev]) {
-    AuxStreams *a = new AuxStreams();
-    for (int j = 0; j < PSA_RHS_MULTI_MAX; ++j) {
-      if (int rc = hip_check(hipStreamCreateWithFlags(&a->st[j], hipStreamNonBlocking),
-                             "psa_step_multi: hipStreamCreateWithFlags"))
-        return rc;
-      for (int k = 0; k < 2; ++k)
-        if (int rc = hip_check(hipEventCreateWithFlags(&a->join[k][j], hipEventDisableTiming),
-                               "psa_step_multi: hipEventCreateWithFlags"))
-          return rc;
-    }
-    if (int rc = hip_check(hipEventCreateWithFlags(&a->fork, hipEventDisableTiming), "psa_step_multi: hipEventCreate"))
+  auto it = g_aux.find({dev, s});
+  AuxStreams *a = it == g_aux.end() ? nullptr : it->second;
+  if (!a) {
+    if (!create) return PSA_OK;
+    a = new AuxStreams();
+    if (int rc = hip_check(hipEventCreateWithFlags(&a->fork, hipEventDisableTiming), "aux streams: hipEventCreate"))
       return rc;
-    if (int rc = hip_check(hipEventCreateWithFlags(&a->mid, hipEventDisableTiming), "psa_step_multi: hipEventCreate"))
+    if (int rc = hip_check(hipEventCreateWithFlags(&a->mid, hipEventDisableTiming), "aux streams: hipEventCreate"))
       return rc;
-    g_aux[dev] = a;
+    g_aux[{dev, s}] = a;
   }
-  *out = g_aux[dev];
+  for (int j = 0; j < n; ++j) {
+    if (a->st[j]) continue;
+    if (!create) return PSA_OK;
+    if (int rc = hip_check(hipStreamCreateWithFlags(&a->st[j], hipStreamNonBlocking), "aux streams: hipStreamCreate"))
+      return rc;
+    for (int k = 0; k < 2; ++k)
+      if (int rc = hip_check(hipEventCreateWithFlags(&a->join[k][j], hipEventDisableTiming),
+                             "aux streams: hipEventCreate"))
+        return rc;
+  }
+  *out = a;
   return PSA_OK;
 }
 
-AuxStreams *aux_streams_existing() {
-  int dev = -1;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kAuxDevices) return nullptr;
-  std::lock_guard<std::mutex> lock(g_aux_mu);
-  return g_aux[dev];
+// aux_streams for a step on s: creates outside graph capture, and under
+// capture uses the streams an eager call created (or none)
+int step_aux(hipStream_t s, int n, AuxStreams **out) {
+  *out = nullptr;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (int rc = hip_check(hipStreamIsCapturing(s, &cap), "hipStreamIsCapturing")) return rc;
+  return aux_streams(s, n, cap == hipStreamCaptureStatusNone, out);
+}
+
+// fork: streams 0 .. n-1 of a wait for what s has queued so far
+int aux_fork(AuxStreams *a, hipEvent_t ev, hipStream_t s, int n) {
+  if (int rc = hip_check(hipEventRecord(ev, s), "hipEventRecord")) return rc;
+  for (int j = 0; j < n; ++j)
+    if (int rc = hip_check(hipStreamWaitEvent(a->st[j], ev, 0), "hipStreamWaitEvent")) return rc;
+  return PSA_OK;
+}
+
+// join: s waits for what streams 0 .. n-1 of a have queued so far
+int aux_join(AuxStreams *a, int k, hipStream_t s, int n) {
+  for (int j = 0; j < n; ++j) {
+    if (int rc = hip_check(hipEventRecord(a->join[k][j], a->st[j]), "hipEventRecord")) return rc;
+    if (int rc = hip_check(hipStreamWaitEvent(s, a->join[k][j], 0), "hipStreamWaitEvent")) return rc;
+  }
+  return PSA_OK;
 }
 
 int finish_impl(const psa_group *g, int32_t w0, int32_t nw, hipStream_t s, bool averaged) {
@@ -2672,34 +2726,14 @@ int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *qu
   // eager call has created the streams yet (streams are not created while
   // a capture is open).
   AuxStreams *aux = nullptr;
-  if (h.n > 1) {
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    if (int rc = hip_check(hipStreamIsCapturing(s, &cap), "psa_step_multi: hipStreamIsCapturing")) return rc;
-    if (cap == hipStreamCaptureStatusNone) {
-      if (int rc = aux_streams(&aux)) return rc;
-    } else {
-      aux = aux_streams_existing();
-    }
-  }
+  if (h.n > 1)
+    if (int rc = step_aux(s, h.n, &aux)) return rc;
   auto on = [&](int j) { return aux ? aux->st[j] : s; };
-  auto fork = [&](hipEvent_t ev) -> int {
-    if (!aux) return PSA_OK;
-    if (int rc = hip_check(hipEventRecord(ev, s), "psa_step_multi: hipEventRecord")) return rc;
-    for (int j = 0; j < h.n; ++j)
-      if (int rc = hip_check(hipStreamWaitEvent(aux->st[j], ev, 0), "psa_step_multi: hipStreamWaitEvent")) return rc;
-    return PSA_OK;
-  };
-  auto join = [&](hipEvent_t *evs) -> int {
-    if (!aux) return PSA_OK;
-    for (int j = 0; j < h.n; ++j) {
-      if (int rc = hip_check(hipEventRecord(evs[j], aux->st[j]), "psa_step_multi: hipEventRecord")) return rc;
-      if (int rc = hip_check(hipStreamWaitEvent(s, evs[j], 0), "psa_step_multi: hipStreamWaitEvent")) return rc;
-    }
-    return PSA_OK;
-  };
+  auto fork = [&](bool mid) { return aux ? aux_fork(aux, mid ? aux->mid : aux->fork, s, h.n) : PSA_OK; };
+  auto join = [&](int k) { return aux ? aux_join(aux, k, s, h.n) : PSA_OK; };
   int rc;
   // the single-group step (psa_step) up to its RHS, for every group
-  if ((rc = fork(aux ? aux->fork : nullptr))) return rc;
+  if ((rc = fork(false))) return rc;
   for (int j = 0; j < h.n; ++j) {
     const psa_group *g = &gs[j];
     void *sj = on(j);
@@ -2710,9 +2744,9 @@ int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *qu
       if ((rc = psa_offdiag_tables(g, w0[j], h.nw[j], sj))) return rc;
     if (g->prec == PSA_PREC_BF16X3 && (rc = split_planes(*g, w0[j], h.nw[j], on(j)))) return rc;
   }
-  if ((rc = join(aux ? aux->join[0] : nullptr))) return rc;
+  if ((rc = join(0))) return rc;
   if ((rc = rhs_multi_impl(packed_host, packed_dev, queue, s, false))) return rc;
-  if ((rc = fork(aux ? aux->mid : nullptr))) return rc;
+  if ((rc = fork(true))) return rc;
   for (int j = 0; j < h.n; ++j) {
     const psa_group *g = &gs[j];
     void *sj = on(j);
@@ -2723,7 +2757,7 @@ int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *qu
     if ((rc = psa_sources(g, w0[j], h.nw[j], sj))) return rc;
     if ((rc = psa_qw2prop(g, w0[j], h.nw[j], sj))) return rc;
   }
-  return join(aux ? aux->join[1] : nullptr);
+  return join(1);
 }
 
 int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream) {
