@@ -271,16 +271,19 @@ class WireGroup:
             self.s[a], self.s[b] = self.s[b], self.s[a]
         self._sync_struct()
 
-    def autotune(self, reps=3):
+    def autotune(self, reps=2, rounds=3):
         """Pick the launch choices of this group by timing them on its own
         buffers: tune[2] GEMM block order (XCD-grouped or linear; the
         split-BF16 RHS has a static schedule of its own), tune[1] / tune[3]
         transport lines per block for the row / column pass (4, 2, 8, 16),
         timed over full steps with the level swap (the column pass carries
         the leapfrog average there, and in the split-BF16 mode writes the
-        next step's operand planes).  The state is unchanged: the levels the
-        steps overwrite are saved and restored.  Returns {knob: (choice, ms
-        per candidate)}."""
+        next step's operand planes).  The candidates of a knob are timed in
+        turn, `rounds` times over (reps calls each), and each keeps its
+        fastest round, so a clock or thermal drift during the timing does
+        not pick a choice.  The state is unchanged: the levels the steps
+        overwrite are saved and restored.  Returns {knob: (choice, ms per
+        candidate)}."""
         od = bool(self.prm.flags.get("OffDiagDph"))
         self.prop2qw()
         self.prep()
@@ -303,19 +306,21 @@ class WireGroup:
             knobs.append((3, [4, 2, 8, 16], full_step))
         out = {}
         for idx, cands, fn in knobs:
-            times = []
-            for c in cands:
-                self.tune[idx] = c
-                self._sync_struct()
-                fn()  # first launch of this choice (attribute set-up)
-                torch.cuda.synchronize()
-                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                a.record()
-                for _ in range(reps):
-                    fn()
-                b.record()
-                torch.cuda.synchronize()
-                times.append(a.elapsed_time(b) / reps)
+            times = [float("inf")] * len(cands)
+            for r in range(rounds):
+                for i, c in enumerate(cands):
+                    self.tune[idx] = c
+                    self._sync_struct()
+                    if r == 0:
+                        fn()  # first launch of this choice (attribute set-up)
+                    torch.cuda.synchronize()
+                    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    a.record()
+                    for _ in range(reps):
+                        fn()
+                    b.record()
+                    torch.cuda.synchronize()
+                    times[i] = min(times[i], a.elapsed_time(b) / reps)
             best = int(np.argmin(times))
             # the XCD-grouped block order (candidate 0 of knob 2) reads about
             # half the HBM bytes of the linear one; it is kept unless the
