@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PSA_ABI_VERSION 4
+#define PSA_ABI_VERSION 5
 
 #define PSA_OK 0
 #define PSA_E_INVALID (-1)   /* bad argument / shape */
@@ -143,6 +143,17 @@ typedef struct psa_group {
                                          holds, [1] that of the C buffer whose diagonal sums the src
                                          block holds.  A caller that writes p, C or D of a wire other
                                          than through the step entry points zeroes both of its tags. */
+  int32_t *lazy;                      /* [Nw] deferred leapfrog average, zero-initialised (NULL: the
+                                         steps average in place, as the reference does).  With it set,
+                                         psa_step, psa_step_multi and psa_step_noalloc leave level A
+                                         (pA, CA, DA) holding the level the step added to (X1) and set
+                                         lazy[w] = 1 instead of writing X2 = (X1 + X3) / 2 there: the
+                                         averaged level is (A + B) / 2 once the caller has made the new
+                                         level B.  The next leapfrog RHS forms it where it reads X1 and
+                                         writes it back, and psa_average_levels writes it for host
+                                         reads; psa_finish and the RK4 step leave lazy[w] = 0.  A
+                                         caller that writes level A or B of a wire sets lazy[w] = 0
+                                         after calling psa_average_levels (or overwriting both). */
 } psa_group;
 
 /* layout of the per-wire prep block (doubles) */
@@ -472,6 +483,12 @@ int psa_rhs_multi(const void *packed_host, const void *packed_dev, uint32_t *que
  * on `stream` in group order otherwise. */
 int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *queue, double dt,
                    void *stream);
+
+/* The deferred leapfrog average of psa_group.lazy written out: for every
+ * wire w in [w0, w0 + nw) with lazy[w] set, A <- (A + B) / 2 for p, C, D
+ * (SBEs.py:5704-5706), then lazy[w] = 0.  No counterpart in the reference,
+ * which averages in place every step; a no-op when lazy is NULL. */
+int psa_average_levels(const psa_group *g, int32_t w0, int32_t nw, void *stream);
 
 /* ---- Module API on the caller's dense arrays (coulomb.py, phonons.py) ----
  *

@@ -400,6 +400,7 @@ class SBESolver:
         if not self.keep_level1:
             return
         s = self.group.s
+        self.group.average_levels(w0, nw)  # level A as the reference keeps it
         if self._lvl1 is None:
             self._lvl1 = {k: torch.zeros_like(s[k]) for k in ("pA", "CA", "DA")}
         for k in ("pA", "CA", "DA"):
@@ -500,6 +501,7 @@ class SBESolver:
         i = w - 1
         g = self.group
         N, Nr = self.Nk, self.Nr
+        g.average_levels(i, 1)  # level 2 as the reference keeps it
         C2 = g.s["CA"][i].diagonal().cpu().numpy()
         D2 = g.s["DA"][i].diagonal().cpu().numpy()
         ne2 = np.abs(C2).astype(complex)

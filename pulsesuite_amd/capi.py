@@ -10,7 +10,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libpulsesuite_amd.so")
 
-PSA_ABI_VERSION = 4
+PSA_ABI_VERSION = 5
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 c_i32 = ctypes.c_int32
@@ -37,7 +37,7 @@ class PsaGroup(ctypes.Structure):
                 + [("mb_off", c_ptr * 8), ("mb_idx", c_ptr * 8), ("mb_val", c_ptr * 8)]
                 + [(n, c_ptr) for n in _WIRE]
                 + [("kscr_len", ctypes.c_int64), ("wid_host", c_ptr)]
-                + [(n, c_ptr) for n in ("pR", "CR", "DR", "wtag")])
+                + [(n, c_ptr) for n in ("pR", "CR", "DR", "wtag", "lazy")])
 
 
 _GP = ctypes.POINTER(PsaGroup)
@@ -116,6 +116,7 @@ SIGNATURES = {
     "psa_rhs_multi_pack": (ctypes.c_int, [c_ptr, c_ptr, c_ptr, c_i32, c_i32, c_dbl, c_ptr]),
     "psa_rhs_multi": (ctypes.c_int, [c_ptr, c_ptr, c_ptr, c_ptr]),
     "psa_step_multi": (ctypes.c_int, [c_ptr, c_ptr, c_ptr, c_dbl, c_ptr]),
+    "psa_average_levels": (ctypes.c_int, [_GP, c_i32, c_i32, c_ptr]),
     "psa_mod_mb_arrays": (ctypes.c_int, [c_i32, c_ptr, c_ptr, c_dbl, c_dbl, c_dbl, c_i32, c_ptr, c_ptr,
                                          c_ptr, c_ptr, c_ptr]),
     "psa_mod_screen": (ctypes.c_int, [c_i32, c_i32, c_dbl, c_dbl, c_ptr, c_ptr, c_dbl, c_ptr, c_ptr,

@@ -430,6 +430,9 @@ struct RhsProb {
       q.h = (row == col) ? cd{0.0, 0.0} : ld_h(g, e == 1 ? g.Heo : g.Hho, o, ij);  // H_o[k2][k1]
     }
     q.x1 = mode == 0 ? cd{0.0, 0.0} : ld((e == 0 ? g.pA : (e == 1 ? g.CA : g.DA)) + o, ij);
+    // deferred leapfrog average (psa_group.lazy): level A holds X1 of the
+    // step that made level B; the step adds to their average
+    if (mode == 1 && g.lazy && g.lazy[w]) q.x1 = cd{(q.x1.re + q.x.re) / 2.0, (q.x1.im + q.x.im) / 2.0};
     return q;
   }
 
@@ -484,6 +487,8 @@ struct RhsProb {
     } else if (mode == 1) {
       const cd x1 = q.x1;
       st(out, ij, cd{x1.re + d.re * dt * 2.0, x1.im + d.im * dt * 2.0});
+      // the average formed in preload becomes level A (the X1 of this step)
+      if (g.lazy && g.lazy[w]) st((e == 0 ? g.pA : (e == 1 ? g.CA : g.DA)) + o, ij, x1);
     } else {
       // RK4 stage mode - 1 (_rk4, integrator.py:143-157): x1 = step start,
       // R = k1 + 2 k2 + 2 k3 so far; out may alias the step start (same

@@ -1447,7 +1447,11 @@ __global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0,
 // ===========================================================================
 // finish: diagonal |.| and e/h renormalisation (SBEs.py:5692-5702)
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_finish_diag(psa_group g, int w0, int avg) {
+// avg: also the diagonal part of the leapfrog average (the rest was fused
+// into the transport); lazy: the step defers the average (psa_group.lazy),
+// so this wire's flag is set; otherwise an existing flag is cleared (the
+// caller averages now or the level is not a leapfrog level).
+__global__ __launch_bounds__(256) void k_finish_diag(psa_group g, int w0, int avg, int lazy) {
   __shared__ double sh[8];
   const int w = w0 + blockIdx.x, N = g.N, tid = threadIdx.x;
   const long o = mat_off(g, w);
@@ -1470,12 +1474,15 @@ __global__ __launch_bounds__(256) void k_finish_diag(psa_group g, int w0, int av
       st(g.DA + o, (long)k * N + k, cd{(da.re + d3.re) / 2.0, (da.im + d3.im) / 2.0});
     }
   }
+  if (g.lazy && tid == 0) g.lazy[w] = lazy ? 1 : 0;
 }
 
 // X2 = (X1 + X3) / 2 for p, C, D of every wire (SBEs.py:5704-5706)
 __global__ __launch_bounds__(256) void k_average(psa_group g, int w0, int nw) {
   const long per = (long)g.N * g.N;
   const long total = per * nw * 3;
+  if (g.lazy && blockIdx.x == 0)
+    for (int t = threadIdx.x; t < nw; t += blockDim.x) g.lazy[w0 + t] = 0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const int m = i / (per * nw);
     const long r = i % (per * nw);
@@ -1486,6 +1493,29 @@ __global__ __launch_bounds__(256) void k_average(psa_group g, int w0, int nw) {
     const double2 x = reinterpret_cast<const double2 *>(X)[off];
     reinterpret_cast<double2 *>(A)[off] = make_double2((a.x + x.x) / 2.0, (a.y + x.y) / 2.0);
   }
+}
+
+// The deferred leapfrog average written out (psa_average_levels): for the
+// wires with lazy[w] set, A <- (A + B) / 2 for p, C, D; k_lazy_clear then
+// clears their flags (after every block has read them).
+__global__ __launch_bounds__(256) void k_lazy_average(psa_group g, int w0, int nw) {
+  const long per = (long)g.N * g.N;
+  const long total = per * nw * 3;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int m = i / (per * nw);
+    const long r = i % (per * nw);
+    if (!g.lazy[w0 + r / per]) continue;
+    const long off = (long)w0 * per + r;
+    double *A = m == 0 ? g.pA : (m == 1 ? g.CA : g.DA);
+    const double *B = m == 0 ? g.pB : (m == 1 ? g.CB : g.DB);
+    const double2 a = reinterpret_cast<double2 *>(A)[off];
+    const double2 b = reinterpret_cast<const double2 *>(B)[off];
+    reinterpret_cast<double2 *>(A)[off] = make_double2((a.x + b.x) / 2.0, (a.y + b.y) / 2.0);
+  }
+}
+
+__global__ void k_lazy_clear(psa_group g, int w0, int nw) {
+  for (int t = threadIdx.x; t < nw; t += blockDim.x) g.lazy[w0 + t] = 0;
 }
 
 // ===========================================================================
@@ -1513,6 +1543,7 @@ __global__ __launch_bounds__(SN_THREADS) void k_src_norm(psa_group g, int w0) {
   const cd *rcv = reinterpret_cast<const cd *>(src + PSA_SRC_RC(Nr));  // [2][Nr]
   double *re_ = src + 3 * Nr, *rh_ = src + 4 * Nr;
   const double *vc = g.prep + (long)w * PSA_PREP_STRIDE(N) + PSA_PREP_VC(N);
+  const bool lzw = g.lazy && g.lazy[w];
   // --- loads ---
   cd ce[SN_RC], ch[SN_RC];
 #pragma unroll
@@ -1526,9 +1557,16 @@ __global__ __launch_bounds__(SN_THREADS) void k_src_norm(psa_group g, int w0) {
     const long kk = (long)k * N + k;
     ne_t += cabs_(ld(g.CX + o, kk));
     nh_t += cabs_(ld(g.DX + o, kk));
-    // CalcI0 on ne2, nh2 = |diag of the averaged level|
-    s_ne2[k] = cabs_(ld(g.CA + o, kk));
-    s_nh2[k] = cabs_(ld(g.DA + o, kk));
+    // CalcI0 on ne2, nh2 = |diag of the averaged level| (with the average
+    // deferred: (A + X) / 2 of the new level X)
+    cd a2 = ld(g.CA + o, kk), b2 = ld(g.DA + o, kk);
+    if (lzw) {
+      const cd ax = ld(g.CX + o, kk), bx = ld(g.DX + o, kk);
+      a2 = cd{(a2.re + ax.re) / 2.0, (a2.im + ax.im) / 2.0};
+      b2 = cd{(b2.re + bx.re) / 2.0, (b2.im + bx.im) / 2.0};
+    }
+    s_ne2[k] = cabs_(a2);
+    s_nh2[k] = cabs_(b2);
     s_vee[k] = vc[N + k];
     s_vhh[k] = vc[2 * N + k];
   }
@@ -2475,22 +2513,36 @@ int aux_streams(hipStream_t s, int n, bool create, AuxStreams **out) {
   AuxStreams *a = it == g_aux.end() ? nullptr : it->second;
   if (!a) {
     if (!create) return PSA_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (int rc = hip_check(hipEventCreateWithFlags(&e0, hipEventDisableTiming), "aux streams: hipEventCreate"))
+      return rc;
+    if (int rc = hip_check(hipEventCreateWithFlags(&e1, hipEventDisableTiming), "aux streams: hipEventCreate")) {
+      (void)hipEventDestroy(e0);
+      return rc;
+    }
     a = new AuxStreams();
-    if (int rc = hip_check(hipEventCreateWithFlags(&a->fork, hipEventDisableTiming), "aux streams: hipEventCreate"))
-      return rc;
-    if (int rc = hip_check(hipEventCreateWithFlags(&a->mid, hipEventDisableTiming), "aux streams: hipEventCreate"))
-      return rc;
+    a->fork = e0;
+    a->mid = e1;
     g_aux[{dev, s}] = a;
   }
+  // stream j is usable once it and both its join events exist; a failed
+  // creation leaves it null, to be created again by a later call
   for (int j = 0; j < n; ++j) {
     if (a->st[j]) continue;
     if (!create) return PSA_OK;
-    if (int rc = hip_check(hipStreamCreateWithFlags(&a->st[j], hipStreamNonBlocking), "aux streams: hipStreamCreate"))
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t st = nullptr;
+    int rc = hip_check(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), "aux streams: hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), "aux streams: hipEventCreate");
+    if (!rc) rc = hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "aux streams: hipStreamCreate");
+    if (rc) {
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
       return rc;
-    for (int k = 0; k < 2; ++k)
-      if (int rc = hip_check(hipEventCreateWithFlags(&a->join[k][j], hipEventDisableTiming),
-                             "aux streams: hipEventCreate"))
-        return rc;
+    }
+    a->join[0][j] = ev[0];
+    a->join[1][j] = ev[1];
+    a->st[j] = st;
   }
   *out = a;
   return PSA_OK;
@@ -2522,10 +2574,11 @@ int aux_join(AuxStreams *a, int k, hipStream_t s, int n) {
   return PSA_OK;
 }
 
-int finish_impl(const psa_group *g, int32_t w0, int32_t nw, hipStream_t s, bool averaged) {
-  hipLaunchKernelGGL(k_finish_diag, dim3(nw), dim3(256), 0, s, *g, w0, averaged ? 1 : 0);
+// lazy: the leapfrog average is deferred (psa_group.lazy), nothing is averaged
+int finish_impl(const psa_group *g, int32_t w0, int32_t nw, hipStream_t s, bool averaged, bool lazy = false) {
+  hipLaunchKernelGGL(k_finish_diag, dim3(nw), dim3(256), 0, s, *g, w0, (averaged && !lazy) ? 1 : 0, lazy ? 1 : 0);
   PSA_LAUNCH_CHECK("k_finish_diag");
-  if (averaged) return PSA_OK;
+  if (averaged || lazy) return PSA_OK;
   const long total = (long)g->N * g->N * nw * 3;
   const int blocks = (int)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
   hipLaunchKernelGGL(k_average, dim3(blocks), dim3(256), 0, s, *g, w0, nw);
@@ -2546,6 +2599,19 @@ int psa_transport(const psa_group *g, int32_t w0, int32_t nw, double dt, void *s
 int psa_finish(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
   if (int rc = check_group(g, w0, nw)) return rc;
   return finish_impl(g, w0, nw, (hipStream_t)stream, false);
+}
+
+int psa_average_levels(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
+  if (int rc = check_group(g, w0, nw)) return rc;
+  if (!g->lazy) return PSA_OK;
+  const hipStream_t s = (hipStream_t)stream;
+  const long total = (long)g->N * g->N * nw * 3;
+  const int blocks = (int)((total + 255) / 256 < 8192 ? (total + 255) / 256 : 8192);
+  hipLaunchKernelGGL(k_lazy_average, dim3(blocks), dim3(256), 0, s, *g, w0, nw);
+  PSA_LAUNCH_CHECK("k_lazy_average");
+  hipLaunchKernelGGL(k_lazy_clear, dim3(1), dim3(256), 0, s, *g, w0, nw);
+  PSA_LAUNCH_CHECK("k_lazy_clear");
+  return PSA_OK;
 }
 
 int psa_sources(const psa_group *g, int32_t w0, int32_t nw, void *stream) {
@@ -2627,8 +2693,9 @@ int psa_step(const psa_group *g, int32_t w0, int32_t nw, double dt, void *stream
   // C/D diagonals, which finish averages after the renormalisation); the
   // diagonal-only transport of LF = 0 leaves the average to finish
   const bool trans = (g->flags & PSA_F_DCTRANS) != 0, fused = trans && (g->flags & PSA_F_LF);
-  if (trans && (rc = transport_impl(g, w0, nw, dt, (hipStream_t)stream, true, true))) return rc;
-  if ((rc = finish_impl(g, w0, nw, (hipStream_t)stream, fused))) return rc;
+  const bool lazy = g->lazy != nullptr;  // the average deferred to the next step's RHS
+  if (trans && (rc = transport_impl(g, w0, nw, dt, (hipStream_t)stream, !lazy, true))) return rc;
+  if ((rc = finish_impl(g, w0, nw, (hipStream_t)stream, fused, lazy))) return rc;
   if ((rc = psa_sources(g, w0, nw, stream))) return rc;
   if ((rc = psa_qw2prop(g, w0, nw, stream))) return rc;
   return PSA_OK;
@@ -2752,8 +2819,9 @@ int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *qu
     void *sj = on(j);
     if ((rc = psa_relaxation(g, w0[j], h.nw[j], dt, sj))) return rc;
     const bool trans = (g->flags & PSA_F_DCTRANS) != 0, fused = trans && (g->flags & PSA_F_LF);
-    if (trans && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)sj, true, true))) return rc;
-    if ((rc = finish_impl(g, w0[j], h.nw[j], (hipStream_t)sj, fused))) return rc;
+    const bool lazy = g->lazy != nullptr;
+    if (trans && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)sj, !lazy, true))) return rc;
+    if ((rc = finish_impl(g, w0[j], h.nw[j], (hipStream_t)sj, fused, lazy))) return rc;
     if ((rc = psa_sources(g, w0[j], h.nw[j], sj))) return rc;
     if ((rc = psa_qw2prop(g, w0[j], h.nw[j], sj))) return rc;
   }
@@ -2772,6 +2840,7 @@ int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *st
                 *const A[3] = {g->pA, g->CA, g->DA};
   double *const *in[4] = {S, X, A, X}, *const *out[4] = {X, A, X, S};
   psa_group q = *g;
+  q.lazy = nullptr;  // no leapfrog levels; k_finish_diag below clears the flags
   for (int st = 0; st < 4; ++st) {
     q.pA = S[0], q.CA = S[1], q.DA = S[2];
     q.pB = in[st][0], q.CB = in[st][1], q.DB = in[st][2];
@@ -2786,8 +2855,12 @@ int psa_step_rk4(const psa_group *g, int32_t w0, int32_t nw, double dt, void *st
   // diagonal is the state, and CalcI0 reads it through A.
   if ((rc = psa_relaxation(&q, w0, nw, dt, stream))) return rc;
   if ((q.flags & PSA_F_DCTRANS) && (rc = transport_impl(&q, w0, nw, dt, s, false))) return rc;
-  hipLaunchKernelGGL(k_finish_diag, dim3(nw), dim3(256), 0, s, q, w0, 0);
-  PSA_LAUNCH_CHECK("k_finish_diag");
+  {
+    psa_group qf = q;
+    qf.lazy = g->lazy;
+    hipLaunchKernelGGL(k_finish_diag, dim3(nw), dim3(256), 0, s, qf, w0, 0, 0);
+    PSA_LAUNCH_CHECK("k_finish_diag");
+  }
   if ((rc = psa_sources(&q, w0, nw, stream))) return rc;
   if ((rc = psa_qw2prop(&q, w0, nw, stream))) return rc;
   return PSA_OK;

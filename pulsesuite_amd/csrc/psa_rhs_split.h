@@ -451,8 +451,11 @@ __device__ __forceinline__ void pp_tile(const RhsProb &p, int b, int m0, int n0,
   const long o = mat_off(p.g, w);
   const double *X = (e == 0 ? p.g.pB : (e == 1 ? p.g.CB : p.g.DB)) + o;
   const double *H = e == 0 ? p.g.Heh : (e == 1 ? p.g.Heo : p.g.Hho);  // complex64 in this mode
-  const double *X1 = (e == 0 ? p.g.pA : (e == 1 ? p.g.CA : p.g.DA)) + o;
+  double *X1 = (e == 0 ? p.g.pA : (e == 1 ? p.g.CA : p.g.DA)) + o;
   double *out = (e == 0 ? p.g.pX : (e == 1 ? p.g.CX : p.g.DX)) + o;
+  // deferred leapfrog average (psa_group.lazy): X1 = (A + B) / 2, written
+  // back to A as this step's X1
+  const bool lz = p.g.lazy && p.g.lazy[w];
   const RhsProb::Side cs = p.side(e, w, min(col, N - 1), true);
   auto rows = [&](auto leap) {
     constexpr bool LEAP = decltype(leap)::value;
@@ -466,16 +469,20 @@ __device__ __forceinline__ void pp_tile(const RhsProb &p, int b, int m0, int n0,
         q[u].x = ld(X, ij);
         const cd hv = ld_h32(H, o, ij);
         q[u].h = (e != 0 && row == col) ? cd{0.0, 0.0} : hv;
-        if constexpr (LEAP) q[u].x1 = ld(X1, ij);
+        if constexpr (LEAP) {
+          q[u].x1 = ld(X1, ij);
+          if (lz) q[u].x1 = cd{(q[u].x1.re + q[u].x.re) / 2.0, (q[u].x1.im + q[u].x.im) / 2.0};
+        }
       }
 #pragma unroll
       for (int u = 0; u < EB; ++u) {
         const int rr = r0 + 4 * (j0 + u), row = m0 + rr, cc = tid & (TILE - 1);
         const RhsProb::Side rs = p.side(e, w, row, false);
         const cd d = p.deriv(e, (double)acc[rr * TILE + cc], (double)acc[TILE * TILE + rr * TILE + cc], q[u], cs, rs);
-        if constexpr (LEAP)
+        if constexpr (LEAP) {
           st(out, (long)row * N + col, cd{q[u].x1.re + d.re * p.dt * 2.0, q[u].x1.im + d.im * p.dt * 2.0});
-        else
+          if (lz) st(X1, (long)row * N + col, q[u].x1);
+        } else
           st(out, (long)row * N + col, d);
       }
     }

@@ -44,11 +44,15 @@ class WireGroup:
     """
 
     def __init__(self, prm, nw, rr, wire_ids=None, prec="fp64", device="cuda", host_mb=False,
-                 lor_table=False, integrator="leapfrog", carry=True):
+                 lor_table=False, integrator="leapfrog", carry=True, lazy=True):
         self.lib = capi.load()
         # carry: let the library reuse, from one step to the next, the
         # operand planes and diagonal sums it left behind (psa_group.wtag)
         self.carry = bool(carry)
+        # lazy: the leapfrog average X2 = (X1 + X3) / 2 is formed where the
+        # next step reads it (psa_group.lazy) instead of in place; the
+        # state reads (get_state) write it out first
+        self.lazy = bool(lazy)
         if not torch.cuda.is_available():
             raise capi.PsaError("no GPU visible: the device path has no CPU fallback")
         self.dev = torch.device(device)
@@ -130,6 +134,8 @@ class WireGroup:
         # and the carried diagonal sums belong to; zeroed on every host write
         # of the state
         self.s["wtag"] = torch.zeros(2 * nwz, dtype=torch.int64, device=d)
+        # per-wire flags of the deferred leapfrog average (psa_group.lazy)
+        self.s["lazy"] = torch.zeros(nwz, dtype=torch.int32, device=d)
         # split-K partial sums of the source / propagation-grid products
         self._kscr_len = 8 * 2 * max(Nr * 4 * nwz, (2 * N - 1) * 3 * nwz, max(tb["bnx"], 1) * 5 * nwz)
         self.s["kscr"] = torch.empty(self._kscr_len, dtype=F64, device=d)
@@ -224,6 +230,8 @@ class WireGroup:
         S.wid_host = self._wid_host.ctypes.data
         if not self.carry:
             S.wtag = None
+        if not self.lazy:
+            S.lazy = None
         self.S = S
 
     # ------------------------------------------------------------------
@@ -241,6 +249,7 @@ class WireGroup:
             self.s[k].copy_(diag.expand(self.nw, N, N))
         self.s["CX"].zero_()
         self.s["DX"].zero_()
+        self.s["lazy"].zero_()
         self.invalidate()
 
     def invalidate(self, w=None):
@@ -261,6 +270,7 @@ class WireGroup:
             self.s[k][w].zero_()
         for k in ("CA", "CB", "DA", "DB"):
             self.s[k][w].copy_(diag)
+        self.s["lazy"][w] = 0
         self.invalidate(w)
 
     def swap_levels(self):
@@ -301,7 +311,8 @@ class WireGroup:
         knobs = [(2, [0, 1], contractions)]
         saved = None
         if self.prm.flags.get("DCTrans") and self.prm.flags.get("LF"):
-            saved = {k: self.s[k].clone() for k in ("pA", "pB", "pX", "CA", "CB", "CX", "DA", "DB", "DX")}
+            saved = {k: self.s[k].clone() for k in ("pA", "pB", "pX", "CA", "CB", "CX", "DA", "DB", "DX",
+                                                      "lazy")}
             knobs.append((1, [4, 2, 8, 16], full_step))
             knobs.append((3, [4, 2, 8, 16], full_step))
         out = {}
@@ -351,6 +362,7 @@ class WireGroup:
             if a.ndim == 2:
                 a = a[:, :, None]
             self.s[k].copy_(torch.as_tensor(np.ascontiguousarray(np.moveaxis(a, 2, 0)), device=self.dev))
+        self.s["lazy"].zero_()
         self.invalidate()
 
     def hamiltonians(self):
@@ -367,8 +379,16 @@ class WireGroup:
             out.append(t.cpu().numpy().astype(np.complex128))
         return tuple(out)
 
+    def average_levels(self, w0=0, nw=None):
+        """Write out the deferred leapfrog average of wires [w0, w0+nw)
+        (psa_average_levels): level A then holds X2 as the reference keeps
+        it.  Host code reads or writes level A only after this."""
+        w0, nw = self._range(w0, nw)
+        capi.check(self.lib.psa_average_levels(ctypes.byref(self.S), w0, nw, self._stream()))
+
     def get_state(self):
         """(C2, D2, p2, C3, D3, p3) in the reference layout [N, N, Nw]."""
+        self.average_levels()
         torch.cuda.synchronize()
         out = []
         for k in ("CA", "DA", "pA", "CB", "DB", "pB"):
