@@ -322,8 +322,11 @@ class WireGroup:
                 for i, c in enumerate(cands):
                     self.tune[idx] = c
                     self._sync_struct()
-                    if r == 0:
-                        fn()  # first launch of this choice (attribute set-up)
+                    # an untimed call first: the first launch of a choice sets
+                    # its attributes, and a step leaves state for the next
+                    # (the column pass's planes), so the timed calls start
+                    # from what this choice itself leaves
+                    fn()
                     torch.cuda.synchronize()
                     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                     a.record()
