@@ -1396,7 +1396,11 @@ __global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0,
     if constexpr (LINES % 8 == 0 && NT % LINES == 0) {
       if (planes) {
         // items: (row j, 8-column group) of the matrix slot, then (column,
-        // 8-row group) of the transposed slot; 16-byte stores of 8 k
+        // 8-row group) of the transposed slot; 16-byte stores of 8 k.  In
+        // the transposed slot the four 8-row groups of a 32-k block are
+        // neighbouring items, so four lanes write a slot row's whole 64
+        // bytes of a plane in one instruction
+        static_assert(NT % 32 == 0, "the plane-writing column pass needs N % 32 == 0");
         const int sm = mat == 2 ? 0 : mat + 1;  // slot matrix: p, C_o, D_o
         const bool zd = mat < 2;
         uint16_t *spl = reinterpret_cast<uint16_t *>(g.spl);
@@ -1409,9 +1413,9 @@ __global__ __launch_bounds__(64 * LINES) void k_transport_f(psa_group g, int w0,
             k0 = line0 + 8 * gq;
             cb = 8 * gq, jb = row, dc = 1, dj = 0;
           } else {
-            const int u = it - NA, c = u % LINES;
+            const int u = it - NA, c = (u >> 2) % LINES;
             row = line0 + c;
-            k0 = 8 * (u / LINES);
+            k0 = 32 * ((u >> 2) / LINES) + 8 * (u & 3);
             cb = c, jb = k0, dc = 0, dj = 1;
           }
           sp::u32x4 ph[4];
