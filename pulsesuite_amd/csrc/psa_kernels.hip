@@ -2804,16 +2804,22 @@ int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *qu
   auto join = [&](int k) { return aux ? aux_join(aux, k, s, h.n) : PSA_OK; };
   int rc;
   // the single-group step (psa_step) up to its RHS, for every group
+  // a failure after a fork still joins the forked streams back (their
+  // queued work then precedes anything the caller queues next)
+  auto fail_joined = [&](int k, int err) {
+    join(k);
+    return err;
+  };
   if ((rc = fork(false))) return rc;
   for (int j = 0; j < h.n; ++j) {
     const psa_group *g = &gs[j];
     void *sj = on(j);
-    if ((rc = psa_prop2qw(g, w0[j], h.nw[j], sj))) return rc;
-    if ((rc = psa_prep(g, w0[j], h.nw[j], sj))) return rc;
-    if ((rc = psa_calc_h(g, w0[j], h.nw[j], sj))) return rc;
+    if ((rc = psa_prop2qw(g, w0[j], h.nw[j], sj))) return fail_joined(0, rc);
+    if ((rc = psa_prep(g, w0[j], h.nw[j], sj))) return fail_joined(0, rc);
+    if ((rc = psa_calc_h(g, w0[j], h.nw[j], sj))) return fail_joined(0, rc);
     if (g->flags & PSA_F_OFFDIAGDPH)
-      if ((rc = psa_offdiag_tables(g, w0[j], h.nw[j], sj))) return rc;
-    if (g->prec == PSA_PREC_BF16X3 && (rc = split_planes(*g, w0[j], h.nw[j], on(j)))) return rc;
+      if ((rc = psa_offdiag_tables(g, w0[j], h.nw[j], sj))) return fail_joined(0, rc);
+    if (g->prec == PSA_PREC_BF16X3 && (rc = split_planes(*g, w0[j], h.nw[j], on(j)))) return fail_joined(0, rc);
   }
   if ((rc = join(0))) return rc;
   if ((rc = rhs_multi_impl(packed_host, packed_dev, queue, s, false))) return rc;
@@ -2821,13 +2827,14 @@ int psa_step_multi(const void *packed_host, const void *packed_dev, uint32_t *qu
   for (int j = 0; j < h.n; ++j) {
     const psa_group *g = &gs[j];
     void *sj = on(j);
-    if ((rc = psa_relaxation(g, w0[j], h.nw[j], dt, sj))) return rc;
+    if ((rc = psa_relaxation(g, w0[j], h.nw[j], dt, sj))) return fail_joined(1, rc);
     const bool trans = (g->flags & PSA_F_DCTRANS) != 0, fused = trans && (g->flags & PSA_F_LF);
     const bool lazy = g->lazy != nullptr;
-    if (trans && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)sj, !lazy, true))) return rc;
-    if ((rc = finish_impl(g, w0[j], h.nw[j], (hipStream_t)sj, fused, lazy))) return rc;
-    if ((rc = psa_sources(g, w0[j], h.nw[j], sj))) return rc;
-    if ((rc = psa_qw2prop(g, w0[j], h.nw[j], sj))) return rc;
+    if (trans && (rc = transport_impl(g, w0[j], h.nw[j], dt, (hipStream_t)sj, !lazy, true)))
+      return fail_joined(1, rc);
+    if ((rc = finish_impl(g, w0[j], h.nw[j], (hipStream_t)sj, fused, lazy))) return fail_joined(1, rc);
+    if ((rc = psa_sources(g, w0[j], h.nw[j], sj))) return fail_joined(1, rc);
+    if ((rc = psa_qw2prop(g, w0[j], h.nw[j], sj))) return fail_joined(1, rc);
   }
   return join(1);
 }
