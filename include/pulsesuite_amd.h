@@ -628,6 +628,23 @@ int psa_mod_plspectrum(int32_t N, int32_t Nhw, const double *ne, const double *n
                        const double *VC, const double *hw, double rden, double kB,
                        double rscale, double tfac, double *work, double *PLS, void *stream);
 
+/* CalcOffDiagDeph_E (hole 0) / CalcOffDiagDeph_H (hole 1) (dephasing.py:
+ * 107-190, 627-685), the version-1 tables D [N][N] (k, q) from the complex
+ * occupations ne0, nh0 [N], VC, the band energies and the index tables kpq,
+ * kmq (int32 [N][N], 0-based: the reference's _k_p_q, _k_m_q; an index
+ * outside [0, N) drops the term), widths hbar gss (same carrier) and
+ * hbar gso (electron-hole). */
+int psa_mod_offdiag1(int32_t N, int32_t hole, const double *ne0, const double *nh0, const double *VC,
+                     const double *Ee, const double *Eh, const int32_t *kpq, const int32_t *kmq,
+                     double gss, double gso, double hbar, double *D, void *stream);
+
+/* OffDiagDephasing (dephasing.py:193-218, 687-712), the version-1
+ * contraction of p (complex [N][N]) with the tables DE, DH of
+ * psa_mod_offdiag1 into x (complex [N][N]); reads p with the reference's
+ * one-index shift of the second index. */
+int psa_mod_offdiag1_apply(int32_t N, const double *DE, const double *DH, const double *p,
+                           const int32_t *kpq, const double *ky, double hbar, double *x, void *stream);
+
 /* Vehint (coulomb.py:180-242) with dky = ky[k-1] - ky[q-1]: the Vint double
  * sum with the accurate K0 (usefulsubs.K03) into out[0]. */
 int psa_mod_vehint(double dky, const double *y, int32_t Ny, double alphae, double alphah,

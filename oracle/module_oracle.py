@@ -214,5 +214,77 @@ def pl_spectrum(ne, nh, Ee, Eh, gap, geh, VC, hw, t, PLS, rscale):
     return out * _softtheta(t, geh)
 
 
+def offdiag_deph1(ne0, nh0, VC, Ee, Eh, kpq, kmq, gss, gso, hole):
+    """CalcOffDiagDeph_E (hole False) / CalcOffDiagDeph_H (hole True)
+    (dephasing.py:107-190, 627-685) on 0-based index tables (an entry
+    outside [0, N) drops the term, as the reference's extended arrays do):
+    D[k, q] = pi/hbar [sum_k1 Vss(k1, a)^2 L(Es[a] + Es[k] - Es[k1] - Es[b], hbar gss)
+              (ns[a] ns[k] (1 - ns[k1]) + (1 - ns[a]) (1 - ns[k]) ns[k1])
+            + sum_k1 Vso^2 L(Eo[c] + Es[k] - Eo[k1] - Es[b], hbar gso)
+              (no[c] (1 - no[k1]) ns[k] + (1 - no[c]) no[k1] (1 - ns[k]))]
+    with b = kpq[k, q], a = kpq[k1, q], c = kmq[k1, q]; Vso = Veh at (k, b)
+    for electrons and at (k1, c) for holes; the same-carrier sum runs first,
+    both over k1 ascending."""
+    N = len(ne0)
+    ns, no = (np.abs(nh0), np.abs(ne0)) if hole else (np.abs(ne0), np.abs(nh0))
+    Es, Eo = (Eh, Ee) if hole else (Ee, Eh)
+    Vss2 = VC[:, :, 2 if hole else 1] ** 2
+    Veh2 = VC[:, :, 0] ** 2
+    hs, ho = hbar * gss, hbar * gso
+    D = np.zeros((N, N))
+    k = np.arange(N)[:, None]
+    b = kpq                        # [k, q]
+    okb = (b >= 0) & (b < N)
+    bb = np.where(okb, b, 0)
+    for k1 in range(N):
+        a = kpq[k1][None, :]       # [1, q]
+        ok = okb & (a >= 0) & (a < N)
+        aa = np.where(ok, a, 0)
+        Ed = Es[aa] + Es[k] - Es[k1] - Es[bb]
+        occ = ns[aa] * ns[k] * (1.0 - ns[k1]) + (1.0 - ns[aa]) * (1.0 - ns[k]) * ns[k1]
+        D = np.where(ok, D + Vss2[k1, aa] * _lrtz(Ed, hs) * occ, D)
+    for k1 in range(N):
+        c = kmq[k1][None, :]
+        ok = okb & (c >= 0) & (c < N)
+        cc = np.where(ok, c, 0)
+        Ed = Eo[cc] + Es[k] - Eo[k1] - Es[bb]
+        occ = no[cc] * (1.0 - no[k1]) * ns[k] + (1.0 - no[cc]) * no[k1] * (1.0 - ns[k])
+        v2 = Veh2[k1, cc] if hole else Veh2[k, bb]
+        D = np.where(ok, D + v2 * _lrtz(Ed, ho) * occ, D)
+    return D * pi / hbar
+
+
+def offdiag_dephasing1(ne, nh, p, ky, Ee, Eh, g, VC, kpq, kmq):
+    """OffDiagDephasing (dephasing.py:193-218, 687-712):
+    x[k, kp] = i hbar [sum_qp DH[kp, qp] pt[kpq[qp, kp] + 1, k] u[qp]
+                       + sum_q DE[k, q] pp[kpq[q, k] + 1, kp] u[q]],
+    u = |ky| / (|ky| + 1e-10), DE, DH the tables with (gee, geh), (ghh, geh).
+    pt and pp are p and p^T placed at [1:N+1, 1:N+1] of (N+2)^2 arrays and
+    read with the 0-based column index, as the reference does: the second
+    index is shifted by one, pt[i + 1, k] = p[i, k - 1] (0 for k = 0) and
+    pp[i + 1, kp] = p[kp - 1, i] (0 for kp = 0)."""
+    N = len(ky)
+    DE = offdiag_deph1(ne, nh, VC, Ee, Eh, kpq, kmq, g[0], g[2], False)
+    DH = offdiag_deph1(ne, nh, VC, Ee, Eh, kpq, kmq, g[1], g[2], True)
+    u = np.abs(ky) / (np.abs(ky) + 1e-10)
+    # p padded with a zero row and column in front: ps[i + 1, j + 1] = p[i, j]
+    ps = np.zeros((N + 1, N + 1), dtype=complex)
+    ps[1:, 1:] = p
+    x = np.zeros((N, N), dtype=complex)
+    k = np.arange(N)[:, None]
+    kp = np.arange(N)[None, :]
+    for qp in range(N):
+        r = kpq[qp][None, :]       # kpq[qp, kp]
+        ok = (r >= 0) & (r < N)
+        rr = np.where(ok, r, 0)
+        x = np.where(ok, x + DH[kp, qp] * ps[rr + 1, k] * u[qp], x)
+    for q in range(N):
+        r = kpq[q][:, None]        # kpq[q, k]
+        ok = (r >= 0) & (r < N)
+        rr = np.where(ok, r, 0)
+        x = np.where(ok, x + DE[k, q] * ps[kp, rr + 1] * u[q], x)
+    return x * ii * hbar
+
+
 __all__ = ["calc_mveh", "bg_renorm", "carrier_renorm", "vn1n2", "vehint", "chi1dqw", "eps1dqw",
-           "calchw", "pl_spectrum", "c0"]
+           "calchw", "pl_spectrum", "offdiag_deph1", "offdiag_dephasing1", "c0"]

@@ -145,6 +145,9 @@ SIGNATURES = {
     "psa_mod_plspectrum": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_dbl, c_ptr, c_ptr,
                                           c_dbl, c_dbl, c_dbl, c_dbl, c_ptr, c_ptr, c_ptr]),
     "psa_mod_vehint": (ctypes.c_int, [c_dbl, c_ptr, c_i32, c_dbl, c_dbl, c_dbl, c_ptr, c_ptr]),
+    "psa_mod_offdiag1": (ctypes.c_int, [c_i32, c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl,
+                                        c_dbl, c_dbl, c_ptr, c_ptr]),
+    "psa_mod_offdiag1_apply": (ctypes.c_int, [c_i32, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_dbl, c_ptr, c_ptr]),
 }
 
 _lib = None

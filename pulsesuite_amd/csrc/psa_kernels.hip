@@ -3578,6 +3578,29 @@ int psa_mod_plspectrum(int32_t N, int32_t Nhw, const double *ne, const double *n
   return PSA_OK;
 }
 
+int psa_mod_offdiag1(int32_t N, int32_t hole, const double *ne0, const double *nh0, const double *VC, const double *Ee,
+                     const double *Eh, const int32_t *kpq, const int32_t *kmq, double gss, double gso, double hbar,
+                     double *D, void *stream) {
+  if (N < 1 || N > 4096 || hole < 0 || hole > 1 || !ne0 || !nh0 || !VC || !Ee || !Eh || !kpq || !kmq || !D ||
+      !(hbar > 0.0))
+    return fail(PSA_E_INVALID, "psa_mod_offdiag1: bad arguments");
+  hipLaunchKernelGGL(mod::k_od1_tables, dim3(mod_blocks((long)N * N)), dim3(mod::RED_T), 0, (hipStream_t)stream, N,
+                     hole, reinterpret_cast<const cd *>(ne0), reinterpret_cast<const cd *>(nh0), VC, Ee, Eh, kpq, kmq,
+                     gss, gso, hbar, D);
+  PSA_LAUNCH_CHECK("k_od1_tables");
+  return PSA_OK;
+}
+
+int psa_mod_offdiag1_apply(int32_t N, const double *DE, const double *DH, const double *p, const int32_t *kpq,
+                           const double *ky, double hbar, double *x, void *stream) {
+  if (N < 1 || N > 4096 || !DE || !DH || !p || !kpq || !ky || !x || !(hbar > 0.0))
+    return fail(PSA_E_INVALID, "psa_mod_offdiag1_apply: bad arguments");
+  hipLaunchKernelGGL(mod::k_od1_apply, dim3(mod_blocks((long)N * N)), dim3(mod::RED_T), 0, (hipStream_t)stream, N,
+                     DE, DH, reinterpret_cast<const cd *>(p), kpq, ky, hbar, reinterpret_cast<cd *>(x));
+  PSA_LAUNCH_CHECK("k_od1_apply");
+  return PSA_OK;
+}
+
 int psa_mod_vehint(double dky, const double *y, int32_t Ny, double alphae, double alphah, double Delta0, double *out,
                    void *stream) {
   if (Ny < 4 || !y || !out || !(alphae > 0.0) || !(alphah > 0.0))

@@ -796,5 +796,84 @@ __global__ __launch_bounds__(RED_T) void k_vehint(double dky, const double *y, i
   if (threadIdx.x == 0) out[0] = acc;
 }
 
+
+// CalcOffDiagDeph_E (hole 0) / CalcOffDiagDeph_H (hole 1) (dephasing.py:
+// 107-190, 627-685), the version-1 tables: thread per (k, q), t = k N + q,
+//   D[k][q] = pi/hbar (sum_k1 Vss(k1, a)^2 L(Es[a] + Es[k] - Es[k1] - Es[b], hbar gss)
+//             (ns[a] ns[k] (1 - ns[k1]) + (1 - ns[a]) (1 - ns[k]) ns[k1])
+//           + sum_k1 Vso^2 L(Eo[c] + Es[k] - Eo[k1] - Es[b], hbar gso)
+//             (no[c] (1 - no[k1]) ns[k] + (1 - no[c]) no[k1] (1 - ns[k])))
+// with b = kpq[k][q], a = kpq[k1][q], c = kmq[k1][q] (0-based; an index
+// outside [0, N) drops the term), n = |n0|, Vso = Veh at (k, b) for electrons
+// and (k1, c) for holes; the same-carrier sum first, both over k1 ascending.
+__global__ __launch_bounds__(RED_T) void k_od1_tables(int N, int hole, const cd *ne0, const cd *nh0, const double *VC,
+                                                      const double *Ee, const double *Eh, const int32_t *kpq,
+                                                      const int32_t *kmq, double gss, double gso, double hbar,
+                                                      double *D) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * RED_T + threadIdx.x;
+  if (t >= (long)N * N) return;
+  const int q = (int)(t % N), k = (int)(t / N);
+  const cd *ns = hole ? nh0 : ne0, *no = hole ? ne0 : nh0;
+  const double *Es = hole ? Eh : Ee, *Eo = hole ? Ee : Eh;
+  const int css = hole ? 2 : 1;
+  auto V = [&](int a, int b, int c) { return VC[((long)a * N + b) * 3 + c]; };
+  auto lrtz = [](double a, double b) { return (b / PI) / (a * a + b * b); };
+  double d = 0.0;
+  const int b = kpq[(long)k * N + q];
+  if (b >= 0 && b < N) {
+    const double nk = cabs_(ns[k]), hs = hbar * gss, ho = hbar * gso;
+    for (int k1 = 0; k1 < N; ++k1) {
+      const int a = kpq[(long)k1 * N + q];
+      if (a < 0 || a >= N) continue;
+      const double Ed = ((Es[a] + Es[k]) - Es[k1]) - Es[b];
+      const double v = V(k1, a, css), na = cabs_(ns[a]), n1 = cabs_(ns[k1]);
+      const double occ = (na * nk) * (1.0 - n1) + ((1.0 - na) * (1.0 - nk)) * n1;
+      d = d + ((v * v) * lrtz(Ed, hs)) * occ;
+    }
+    for (int k1 = 0; k1 < N; ++k1) {
+      const int c = kmq[(long)k1 * N + q];
+      if (c < 0 || c >= N) continue;
+      const double Ed = ((Eo[c] + Es[k]) - Eo[k1]) - Es[b];
+      const double v = hole ? V(k1, c, 0) : V(k, b, 0), nc = cabs_(no[c]), n1 = cabs_(no[k1]);
+      const double occ = (nc * (1.0 - n1)) * nk + ((1.0 - nc) * n1) * (1.0 - nk);
+      d = d + ((v * v) * lrtz(Ed, ho)) * occ;
+    }
+  }
+  D[t] = (d * PI) / hbar;
+}
+
+// OffDiagDephasing (dephasing.py:193-218, 687-712), the version-1
+// contraction: thread per (k, kp), t = k N + kp,
+//   x[k][kp] = i hbar (sum_qp DH[kp][qp] P[kpq[qp][kp]][k - 1] u[qp]
+//                      + sum_q DE[k][q] P[kp - 1][kpq[q][k]] u[q]),
+// u = |ky| / (|ky| + 1e-10), P[i][-1] = P[-1][j] = 0: the reference reads its
+// one-padded copies of p and p^T with the unshifted column index.
+__global__ __launch_bounds__(RED_T) void k_od1_apply(int N, const double *DE, const double *DH, const cd *p,
+                                                     const int32_t *kpq, const double *ky, double hbar, cd *x) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * RED_T + threadIdx.x;
+  if (t >= (long)N * N) return;
+  const int kp = (int)(t % N), k = (int)(t / N);
+  cd acc{0.0, 0.0};
+  for (int qp = 0; qp < N; ++qp) {
+    const int r = kpq[(long)qp * N + kp];
+    if (r < 0 || r >= N) continue;
+    const cd v = k >= 1 ? p[(long)r * N + (k - 1)] : cd{0.0, 0.0};
+    const double dh = DH[(long)kp * N + qp], u = fabs(ky[qp]) / (fabs(ky[qp]) + 1e-10);
+    acc.re = acc.re + (dh * v.re) * u;
+    acc.im = acc.im + (dh * v.im) * u;
+  }
+  for (int q = 0; q < N; ++q) {
+    const int r = kpq[(long)q * N + k];
+    if (r < 0 || r >= N) continue;
+    const cd v = kp >= 1 ? p[(long)(kp - 1) * N + r] : cd{0.0, 0.0};
+    const double de = DE[(long)k * N + q], u = fabs(ky[q]) / (fabs(ky[q]) + 1e-10);
+    acc.re = acc.re + (de * v.re) * u;
+    acc.im = acc.im + (de * v.im) * u;
+  }
+  x[t] = cd{-acc.im * hbar, acc.re * hbar};
+}
+
 }  // namespace mod
 }  // namespace psa

@@ -10,9 +10,11 @@ with the per-step functions on the GPU.
   ``OffDiagDephasing2`` (``psa_mod_offdiag2``; the latter also appends the
   max/min line of each table, as the reference does).
 * ``Vxx2``, ``WriteDephasing`` (``dephasing.py:362-378, 417-437``).
-
-Not provided (not on the SBE step, which uses the version-2 functions):
-``CalcOffDiagDeph_E/H`` and ``OffDiagDephasing``.
+* off the SBE step (which uses the version-2 functions), also on the
+  device: ``CalcOffDiagDeph_E`` / ``_H`` and ``OffDiagDephasing``
+  (``dephasing.py:107-218, 627-712``; ``psa_mod_offdiag1``,
+  ``psa_mod_offdiag1_apply``), with the reference's one-index shift of the
+  second index of p in the contraction.
 """
 
 import os
@@ -150,3 +152,51 @@ def WriteDephasing(ky, gamE, gamH, w, xxx):
         with open(fn, "w", encoding="utf-8") as f:
             for zi, di in zip(ky, g):
                 f.write(f"{np.float32(zi)} {np.float32(di)}\n")
+
+
+def _od1_tables(hole, ne0, nh0, Ee, Eh, gss, gso, VC, dev):
+    N = len(ne0)
+    if _k_p_q is None:
+        raise RuntimeError("InitializeDephasing must be called first")
+    if np.shape(VC) != (N, N, 3) or len(nh0) != N or np.shape(_k_p_q) != (N, N):
+        raise ValueError("ne0, nh0 need N entries, VC shape (N, N, 3), the index tables (N, N)")
+    lib = capi.load()
+    ned, nhd = to_dev(ne0, C128, dev), to_dev(nh0, C128, dev)
+    vcd, Ed, Hd = to_dev(VC, F64, dev), to_dev(Ee, F64, dev), to_dev(Eh, F64, dev)
+    kp = torch.as_tensor(np.ascontiguousarray(_k_p_q, dtype=np.int32), device=dev)
+    km = torch.as_tensor(np.ascontiguousarray(_k_m_q, dtype=np.int32), device=dev)
+    D = torch.empty(N, N, dtype=F64, device=dev)
+    capi.check(lib.psa_mod_offdiag1(N, int(hole), ned.data_ptr(), nhd.data_ptr(), vcd.data_ptr(), Ed.data_ptr(),
+                                    Hd.data_ptr(), kp.data_ptr(), km.data_ptr(), float(gss), float(gso), hbar,
+                                    D.data_ptr(), stream(dev)))
+    return D, kp, (ned, nhd, vcd, Ed, Hd, km)
+
+
+def CalcOffDiagDeph_E(ne0, nh0, ky, Ee0, Eh0, gee, geh, VC):
+    """dephasing.py:627-655 (_CalcOffDiagDeph_E_jit 107-147): the version-1
+    electron table D[k, q], returned as a new (Nk, Nk) array."""
+    return to_host(_od1_tables(0, ne0, nh0, Ee0, Eh0, gee, geh, VC, device())[0])
+
+
+def CalcOffDiagDeph_H(ne0, nh0, ky, Ee0, Eh0, ghh, geh, VC):
+    """dephasing.py:657-685 (_CalcOffDiagDeph_H_jit 150-190): the version-1
+    hole table D[k, q], returned as a new (Nk, Nk) array."""
+    return to_host(_od1_tables(1, ne0, nh0, Ee0, Eh0, ghh, geh, VC, device())[0])
+
+
+def OffDiagDephasing(ne, nh, p, ky, Ee, Eh, g, VC, x):
+    """dephasing.py:687-712 (_OffDiagDephasing_jit 193-218): x = i hbar (the
+    Dh contraction over qp + the De contraction over q) with g = (gee, ghh,
+    geh), written into x."""
+    N = len(ky)
+    if np.shape(p) != (N, N) or np.shape(x) != (N, N):
+        raise ValueError("p and x need shape (Nk, Nk)")
+    dev = device()
+    DE, kp, keep = _od1_tables(0, ne, nh, Ee, Eh, g[0], g[2], VC, dev)
+    DH, _, keep2 = _od1_tables(1, ne, nh, Ee, Eh, g[1], g[2], VC, dev)
+    pd, kyd = to_dev(p, C128, dev), to_dev(ky, F64, dev)
+    xd = torch.empty(N, N, dtype=C128, device=dev)
+    capi.check(capi.load().psa_mod_offdiag1_apply(N, DE.data_ptr(), DH.data_ptr(), pd.data_ptr(), kp.data_ptr(),
+                                                  kyd.data_ptr(), hbar, xd.data_ptr(), stream(dev)))
+    x[...] = to_host(xd)
+    del keep, keep2  # the inputs of the queued launches lived until here

@@ -599,6 +599,19 @@ def extras_case(nk=16, seed=17, nx=64, drr=10e-9):
             V = [np.zeros((n, n), dtype=complex) for _ in range(4)]
             qwoptics.GetVn1n2(s.kr, rcv, Hcc, Hhh, Hcv, *V)
             out.update(rcv=rcv, Hcc=Hcc, Hhh=Hhh, Hcv=Hcv, Vcc=V[0], Vvv=V[1], Vcv=V[2], Vvc=V[3])
+            # version-1 off-diagonal dephasing (dephasing.py:107-225, 627-712) on
+            # the solver's index tables, with complex occupations and a random p
+            from pulsesuite.PSTD3D import dephasing
+            ge1, gh1, geh1 = 3.0e12, 4.5e12, 2.2e12
+            od_ne = nep
+            od_nh = nhp
+            od_p = crand(n, n) * 0.05
+            DE = dephasing.CalcOffDiagDeph_E(od_ne, od_nh, s.kr, s.Ee, s.Eh, ge1, geh1, VC)
+            DH = dephasing.CalcOffDiagDeph_H(od_ne, od_nh, s.kr, s.Ee, s.Eh, gh1, geh1, VC)
+            X1 = np.zeros((n, n), dtype=complex)
+            dephasing.OffDiagDephasing(od_ne, od_nh, od_p, s.kr, s.Ee, s.Eh, [ge1, gh1, geh1], VC, X1)
+            out.update(od_g=np.array([ge1, gh1, geh1]), od_ne=od_ne, od_nh=od_nh, od_p=od_p, od_DE=DE,
+                       od_DH=DH, od_x=X1, od_kpq=dephasing._instance._k_p_q, od_kmq=dephasing._instance._k_m_q)
             # file writers (qwoptics._firsttime starts True: printITReal2's first file has two columns)
             f1 = crand(8)
             y8 = np.linspace(-1e-7, 1e-7, 8)

@@ -9,7 +9,8 @@ psa_mod_vehint, psa_vint_table), on the golden run's own grid:
   numpy's pairwise sums, device exp/atan); Vint within 1e-13 and Vehint
   within 1e-12 (the reference's complex CIK01 K0);
 * against the oracle at N = 48 on random inputs, the same bars;
-* outputs written into strided views."""
+* outputs written into strided views;
+* the version-1 off-diagonal dephasing functions."""
 
 import numpy as np
 import pytest
@@ -120,3 +121,41 @@ def test_extras_against_oracle_n48():
     emission.PLSpectrum(nep, nhp, g.Ee, g.Eh, prm.gap, g.gam_eh, VCs, hw, 2e11, PLS)
     assert np.abs(want).max() > 10 * 1e-8
     assert rel(PLS, want) < 1e-12
+
+
+def test_offdiag_dephasing_v1_matches_reference_and_oracle(z):
+    """CalcOffDiagDeph_E/H and OffDiagDephasing (the version-1 functions,
+    psa_mod_offdiag1 / _apply) on the golden run's index tables: the tables
+    and the contraction within 1e-14 of the reference's values (sequential
+    sums in the reference's order; |n| by the device's hypot against
+    libm's); again against the oracle on random occupations at N = 48."""
+    from pulsesuite_amd import dephasing
+    from tests.helpers import wire_length
+    print("[modx] off-diagonal dephasing v1", flush=True)
+    prm, g = _init(float(z["L"]))
+    dephasing.InitializeDephasing(g.kr, prm.me, prm.mh)
+    assert np.array_equal(dephasing._k_p_q, z["od_kpq"]) and np.array_equal(dephasing._k_m_q, z["od_kmq"])
+    gg = z["od_g"]
+    DE = dephasing.CalcOffDiagDeph_E(z["od_ne"], z["od_nh"], z["kr"], z["Ee"], z["Eh"], gg[0], gg[2], z["fVC"])
+    DH = dephasing.CalcOffDiagDeph_H(z["od_ne"], z["od_nh"], z["kr"], z["Ee"], z["Eh"], gg[1], gg[2], z["fVC"])
+    assert rel(DE, z["od_DE"]) < 1e-14
+    assert rel(DH, z["od_DH"]) < 1e-14
+    x = np.zeros((len(z["kr"]), len(z["kr"])), dtype=complex)
+    dephasing.OffDiagDephasing(z["od_ne"], z["od_nh"], z["od_p"], z["kr"], z["Ee"], z["Eh"], list(gg), z["fVC"], x)
+    assert rel(x, z["od_x"]) < 1e-14
+    prm, g = _init(wire_length(48))
+    dephasing.InitializeDephasing(g.kr, prm.me, prm.mh)
+    N = g.Nk
+    rng = np.random.default_rng(11)
+    ne = rng.uniform(0, 0.6, N) * np.exp(1j * rng.uniform(0, 6, N))
+    nh = rng.uniform(0, 0.6, N) * np.exp(1j * rng.uniform(0, 6, N))
+    VC = rng.uniform(0.1, 1.0, (N, N, 3)) * 1e-21
+    p = (rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))) * 0.05
+    gg = (3e12, 4e12, 2e12)
+    kpq, kmq = dephasing._k_p_q, dephasing._k_m_q
+    for hole, fn, gs in ((False, dephasing.CalcOffDiagDeph_E, gg[0]), (True, dephasing.CalcOffDiagDeph_H, gg[1])):
+        got = fn(ne, nh, g.kr, g.Ee, g.Eh, gs, gg[2], VC)
+        assert rel(got, M.offdiag_deph1(ne, nh, VC, g.Ee, g.Eh, kpq, kmq, gs, gg[2], hole)) < 1e-14, hole
+    x = np.zeros((N, N), dtype=complex)
+    dephasing.OffDiagDephasing(ne, nh, p, g.kr, g.Ee, g.Eh, gg, VC, x)
+    assert rel(x, M.offdiag_dephasing1(ne, nh, p, g.kr, g.Ee, g.Eh, gg, VC, kpq, kmq)) < 1e-14

@@ -12,6 +12,10 @@
   over the delta of k + qp = kp + q, GetVn1n2 on diagonal Hamiltonians
   has a closed form and Vvc = Vcv^H, Vehint against scipy.special.k0,
   PLSpectrum with ne nh = 0 only rescales PLS;
+* the version-1 off-diagonal dephasing (CalcOffDiagDeph_E/H,
+  OffDiagDephasing) against the reference's values, the contraction with
+  its one-index shift of p's second index, and against a direct loop over
+  the reference's own formula at a second size;
 * the host-side drop-in functions (GetChi1Dqw, GetEps1Dqw, QWChi1,
   Calchw) and the file writers (printITReal2, printITReal, printIT3D,
   WriteSBESolns, WritePLSpectrum, WritePropFields) against the
@@ -202,3 +206,64 @@ def test_file_writers_match_reference_files(z, tmp_path):
     finally:
         os.chdir(cwd)
         SBEs._prop_first[0] = True
+
+
+def test_offdiag_dephasing_v1_against_reference(z):
+    g = z["od_g"]
+    DE = M.offdiag_deph1(z["od_ne"], z["od_nh"], z["fVC"], z["Ee"], z["Eh"], z["od_kpq"], z["od_kmq"], g[0], g[2],
+                         False)
+    DH = M.offdiag_deph1(z["od_ne"], z["od_nh"], z["fVC"], z["Ee"], z["Eh"], z["od_kpq"], z["od_kmq"], g[1], g[2],
+                         True)
+    assert np.array_equal(DE, z["od_DE"])
+    assert rel(DH, z["od_DH"]) < 1e-15  # two entries one ulp apart
+    x = M.offdiag_dephasing1(z["od_ne"], z["od_nh"], z["od_p"], z["kr"], z["Ee"], z["Eh"], g, z["fVC"], z["od_kpq"],
+                             z["od_kmq"])
+    assert np.array_equal(x, z["od_x"])
+    # the shift is what the reference does: the unshifted contraction is far off
+    ps = z["od_p"]
+    assert np.abs(x[:, 0]).max() > 0 and np.abs(x[0, :]).max() > 0
+    assert rel(M.offdiag_dephasing1(z["od_ne"], z["od_nh"], np.roll(ps, 1, axis=1), z["kr"], z["Ee"], z["Eh"], g,
+                                    z["fVC"], z["od_kpq"], z["od_kmq"]), x) > 1e-3
+
+
+def test_offdiag_tables_v1_brute_force():
+    """The table restatement against a plain loop over the reference's
+    formula (dephasing.py:107-190) on random inputs with tables that have
+    out-of-range entries."""
+    from scipy.constants import hbar
+    rng = np.random.default_rng(8)
+    N = 7
+    ne = rng.uniform(0, 0.6, N) * np.exp(1j * rng.uniform(0, 6, N))
+    nh = rng.uniform(0, 0.6, N) * np.exp(1j * rng.uniform(0, 6, N))
+    VC = rng.uniform(0.1, 1.0, (N, N, 3)) * 1e-21
+    Ee = np.sort(rng.uniform(0, 1e-20, N))
+    Eh = np.sort(rng.uniform(0, 1e-21, N))
+    kpq = rng.integers(-2, N + 2, (N, N)).astype(np.int32)
+    kmq = rng.integers(-2, N + 2, (N, N)).astype(np.int32)
+    for hole in (False, True):
+        gss, gso = (4e12, 2e12) if hole else (3e12, 2e12)
+        ns, no = (np.abs(nh), np.abs(ne)) if hole else (np.abs(ne), np.abs(nh))
+        Es, Eo = (Eh, Ee) if hole else (Ee, Eh)
+        Vs = VC[:, :, 2 if hole else 1]
+        want = np.zeros((N, N))
+        L = lambda a, b: (b / np.pi) / (a * a + b * b)  # noqa: E731
+        for k in range(N):
+            for q in range(N):
+                b = kpq[k, q]
+                if not 0 <= b < N:
+                    continue
+                acc = 0.0
+                for k1 in range(N):
+                    a = kpq[k1, q]
+                    if 0 <= a < N:
+                        acc += Vs[k1, a] ** 2 * L(Es[a] + Es[k] - Es[k1] - Es[b], hbar * gss) * (
+                            ns[a] * ns[k] * (1 - ns[k1]) + (1 - ns[a]) * (1 - ns[k]) * ns[k1])
+                for k1 in range(N):
+                    c = kmq[k1, q]
+                    if 0 <= c < N:
+                        v = VC[k1, c, 0] if hole else VC[k, b, 0]
+                        acc += v ** 2 * L(Eo[c] + Es[k] - Eo[k1] - Es[b], hbar * gso) * (
+                            no[c] * (1 - no[k1]) * ns[k] + (1 - no[c]) * no[k1] * (1 - ns[k]))
+                want[k, q] = acc * np.pi / hbar
+        got = M.offdiag_deph1(ne, nh, VC, Ee, Eh, kpq, kmq, gss, gso, hole)
+        assert rel(got, want) < 1e-14, hole
