@@ -9,7 +9,7 @@ with the per-step functions on the GPU.
   (``psa_mod_gamma``), ``CalcOffDiagDeph_E2`` / ``_H2`` and
   ``OffDiagDephasing2`` (``psa_mod_offdiag2``; the latter also appends the
   max/min line of each table, as the reference does).
-* ``Vxx2``, ``WriteDephasing`` (``dephasing.py:362-378, 417-437``).
+* ``Vxx2``, ``printGam``, ``WriteDephasing`` (``dephasing.py:362-378, 417-437``).
 * off the SBE step (which uses the version-2 functions), also on the
   device: ``CalcOffDiagDeph_E`` / ``_H`` and ``OffDiagDephasing``
   (``dephasing.py:107-218, 627-712``; ``psa_mod_offdiag1``,
@@ -143,15 +143,21 @@ def OffDiagDephasing2(ne, nh, p, ky, Ee, Eh, g, VC, t, x):
             f.flush()
 
 
+def printGam(Dx, z, n, file):
+    """dephasing.py:417-427: z and Dx per line as float32,
+    dataQW/<file><n:05d>.dat."""
+    fn = f"dataQW/{file}{n:05d}.dat"
+    os.makedirs(os.path.dirname(fn), exist_ok=True)
+    with open(fn, "w", encoding="utf-8") as f:
+        for zi, di in zip(z, Dx):
+            f.write(f"{np.float32(zi)} {np.float32(di)}\n")
+
+
 def WriteDephasing(ky, gamE, gamH, w, xxx):
     """dephasing.py:417-437: dataQW/Wire/Ge|Gh/G?.<ww>.k.<xxxxx>.dat."""
     wire = f"{w:02d}"
     for name, g in ((f"Wire/Ge/Ge.{wire}.k.", gamE), (f"Wire/Gh/Gh.{wire}.k.", gamH)):
-        fn = f"dataQW/{name}{xxx:05d}.dat"
-        os.makedirs(os.path.dirname(fn), exist_ok=True)
-        with open(fn, "w", encoding="utf-8") as f:
-            for zi, di in zip(ky, g):
-                f.write(f"{np.float32(zi)} {np.float32(di)}\n")
+        printGam(g, ky, xxx, name)
 
 
 def _od1_tables(hole, ne0, nh0, Ee, Eh, gss, gso, VC, dev):
