@@ -271,17 +271,25 @@ struct SplitK {
   }
 };
 
+// S <= SPLITK_MAX: every split's partial is loaded before the first add
+// (one round of memory latency), then they are added in split order.
+constexpr int SPLITK_MAX = 8;
 template <class P>
 __global__ __launch_bounds__(256) void splitk_reduce(P p, const double *scr, int S, int M, int ncol) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= (long)M * ncol) return;
   const int r = (int)(i / ncol), c = (int)(i % ncol);
+  double2 v[SPLITK_MAX];
+#pragma unroll
+  for (int s = 0; s < SPLITK_MAX; ++s)
+    if (s < S) v[s] = *reinterpret_cast<const double2 *>(scr + (((long)s * M + r) * ncol + c) * 2);
   double re = 0.0, im = 0.0;
-  for (int s = 0; s < S; ++s) {
-    const double *o = scr + (((long)s * M + r) * ncol + c) * 2;
-    re += o[0];
-    im += o[1];
-  }
+#pragma unroll
+  for (int s = 0; s < SPLITK_MAX; ++s)
+    if (s < S) {
+      re += v[s].x;
+      im += v[s].y;
+    }
   p.store(0, r, c, re, im);
 }
 

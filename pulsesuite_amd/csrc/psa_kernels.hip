@@ -1891,7 +1891,7 @@ int launch_gemm_splitk(const P &p, int M, int ncol, int K, hipStream_t s, const 
       cus = 256;
   }
   const int tiles = ((M + GBM - 1) / GBM) * ((ncol + GBN - 1) / GBN);
-  int S = min(8, max(1, (2 * cus + tiles - 1) / tiles));
+  int S = min(SPLITK_MAX, max(1, (2 * cus + tiles - 1) / tiles));
   S = min(S, (K + GBK - 1) / GBK);
   while (S > 1 && (long)S * M * ncol * 2 > g.kscr_len) --S;
   if (S <= 1 || !g.kscr) return launch_gemm(p, PSA_PREC_FP64, M, ncol, 1, s, what);
