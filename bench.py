@@ -31,6 +31,19 @@ BF16_PEAK_TFLOPS = 2500.0  # dense
 # WRITE_SIZE runs, scaled by a 1 GiB copy's counts (x2.0, x1.0); the runs and
 # the calibration are in profiles/sbe_step_64w_nk256.md.
 RHS_TRAFFIC_BYTES = {"fp64": 2.79e9, "bf16x3": 2.08e9}
+HBM_PEAK_GBS = 8000.0
+
+
+def rhs_min_bytes(mode, nw, n):
+    """Bytes the timed RHS launch must move at least, each operand read and
+    each result written once: p, C, D (16 B each) and the older level X1
+    (16 B each) read, the three Hamiltonians read (complex64 in the split-BF16
+    mode, complex128 otherwise), the two real off-diagonal dephasing tables
+    (N x (2N - 1) doubles each) read, the new level and the averaged X1 that
+    the deferred leapfrog average writes back (16 B each) written."""
+    h = 8 if mode == "bf16x3" else 16
+    per = 3 * 16 + 3 * 16 + 3 * h + 3 * 16 + 3 * 16
+    return nw * (per * n * n + 2 * 8 * n * (2 * n - 1))
 
 
 def wire_length(nk, me_rel=0.067, gap_ev=1.5):
@@ -541,6 +554,7 @@ def main():
     t_rhs = r["t_rhs"]
     flops_rhs = args.wires * 104.0 * N**3
     achieved = flops_rhs / t_rhs / 1e12
+    min_bytes = float(rhs_min_bytes(best, args.wires, N))
     peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16x3": BF16_PEAK_TFLOPS}[best]
     res["reported_mode"] = best
     if not args.no_rk4:
@@ -623,7 +637,13 @@ def main():
                      "traffic": (RHS_TRAFFIC_BYTES.get(best)
                                  if (args.wires, args.nk) == (64, 256) else None),
                      "avg_launch_ms": t_rhs * 1e3,
-                     "flops_per_launch": flops_rhs},
+                     "flops_per_launch": flops_rhs,
+                     # the same launch against HBM: its minimum bytes over its time
+                     "hbm": {"min_bytes_per_launch": min_bytes,
+                             "achieved": min_bytes / t_rhs / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": min_bytes / t_rhs / 1e9 / HBM_PEAK_GBS,
+                             "flop_per_byte": flops_rhs / min_bytes,
+                             "ridge_flop_per_byte": peak * 1e12 / (HBM_PEAK_GBS * 1e9)}},
         "cpu_baseline": cpu,
         "serving": serving,
         "extra": res,
