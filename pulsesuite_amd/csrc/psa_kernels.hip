@@ -1913,11 +1913,12 @@ int launch_gemm_splitk(const P &p, int M, int ncol, int K, hipStream_t s, const 
 
 // The split-bf16 operand planes of wires [w0, w0 + nw) (sp::k_split, and
 // sp::k_split_band for the off-diagonal dephasing bands).
-// the operand planes of p, C, D (unless carried) and of the Hamiltonians
+// the operand planes of p, C, D (unless carried; the RHS kernel splits the
+// Hamiltonians itself as it stages their slabs)
 int split_main(const psa_group &g, int w0, int nw, hipStream_t s) {
   if (!g.spl) return fail(PSA_E_INVALID, "bf16x3 mode needs the spl operand buffer");
   const int nt = (g.N + 63) / 64;
-  hipLaunchKernelGGL(sp::k_split, dim3(nt, nt, 6 * nw), dim3(256), 0, s, g, w0);
+  hipLaunchKernelGGL(sp::k_split, dim3(nt, nt, 3 * nw), dim3(256), 0, s, g, w0);
   PSA_LAUNCH_CHECK("sp::k_split");
   return PSA_OK;
 }

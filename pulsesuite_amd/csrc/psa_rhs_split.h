@@ -5,7 +5,8 @@
 //   f = float(x), hi = bf16(f), lo = bf16(f - float(hi))
 // (v_cvt_pk_bf16_f32, round to nearest even; |x - hi - lo| <= ~2^-17 |x|),
 // for the real and the imaginary part: four bf16 planes per operand slot
-// (k_split).  A real product a b is a_hi b_hi + a_hi b_lo + a_lo b_hi on
+// (k_split for p, C, D; the Hamiltonians are split in registers while their
+// slabs are staged, copy_h).  A real product a b is a_hi b_hi + a_hi b_lo + a_lo b_hi on
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation (the lo*lo term, ~2^-16
 // relative, is dropped); a complex product is four real ones, 12 MFMAs per
 // 16 x 16 x 32 block.  The diagonal parts, the epilogue and the state stay
@@ -18,6 +19,10 @@
 // dephasing (rows x, k = m; purely imaginary, real planes zero).  An A operand A(m, k)
 // reads slot rows m, a B operand B(k, n) slot rows n, both k-contiguous
 // as the MFMA lanes take them (A[l%16][8(l/16)+e], B[8(l/16)+e][l%16]).
+// The Hamiltonian slots 6-11 are not kept in the plane buffer: the RHS
+// kernel reads the complex64 Hamiltonians (8 bytes per value, as many as
+// the four planes) and splits each slab in registers as it stages it
+// (copy_h), into the same LDS layout.
 //
 // Layout: a slot is padded to NR = 128 ceil(N/128) rows and NK = 32
 // ceil(N/32) k (zeros) and stored as [row block of 64][k block of 32]
@@ -60,30 +65,26 @@ __device__ __forceinline__ void split_pair(double x0, double x1, uint32_t &hi, u
 }
 
 // ---------------------------------------------------------------------------
-// k_split: block (64 x 64 tile of the stored FP64 buffer, matrix, wire).
-// Stored buffers: p, C, D as the matrix; Heh, Heo, Hho as its transpose.
-// The tile is read coalesced along the buffer's rows (two rows per thread
-// and pass), split, and kept in LDS as row pairs per column (dword = rows
+// k_split: block (64 x 64 tile of the matrix, matrix p / C / D, wire).
+// The tile is read coalesced along the rows (two rows per thread and
+// pass), split, and kept in LDS as row pairs per column (dword = rows
 // 2r, 2r+1; column stride 33 dwords), from which both slots are written
 // with 16-byte stores of 8 consecutive k.  Values past N (the padding) are
 // written as zeros; groups of 8 k at or past NK are not written.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_split(psa_group g, int w0) {
   __shared__ uint32_t t[4][64][33];  // [plane][column j][row pair]
-  const int N = g.N, NK = nk(N), mat = blockIdx.z % 6, w = w0 + blockIdx.z / 6;
+  const int N = g.N, NK = nk(N), mat = blockIdx.z % 3, w = w0 + blockIdx.z / 3;
   const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64, tid = threadIdx.x;
   const long o = mat_off(g, w);
-  // p, C, D planes already hold this level (written by the previous step's
-  // transport): nothing to do for them (block-uniform, before any barrier)
-  if (mat < 3 && g.wtag && g.wtag[2 * w] == (uint64_t)(g.pB + o)) return;
-  // p, C, D complex128; the Hamiltonians complex64 in this mode
-  const double *X = mat == 0 ? g.pB : mat == 1 ? g.CB : mat == 2 ? g.DB
-                    : mat == 3 ? g.Heh : mat == 4 ? g.Heo : g.Hho;
+  // the planes already hold this level (written by the previous step's
+  // transport): nothing to do (block-uniform, before any barrier)
+  if (g.wtag && g.wtag[2 * w] == (uint64_t)(g.pB + o)) return;
+  const double *X = mat == 0 ? g.pB : mat == 1 ? g.CB : g.DB;
   const bool zd = (mat == 1 || mat == 2);
-  const bool stored_t = mat >= 3;  // the buffer holds the transpose of the matrix
   uint16_t *spl = reinterpret_cast<uint16_t *>(g.spl);
-  // slot receiving the buffer as stored, and the one receiving its transpose
-  const int s_same = 2 * mat + (stored_t ? 1 : 0), s_tr = 2 * mat + (stored_t ? 0 : 1);
+  // slot of the matrix, and of its transpose
+  const int s_same = 2 * mat, s_tr = 2 * mat + 1;
   {
     const int j = tid & 63, gj = c0 + j;
     cd v[8][2];
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void k_split(psa_group g, int w0) {
       for (int r = 0; r < 2; ++r) {
         const int gi = r0 + 2 * rp + r;
         const long ij = (long)min(gi, N - 1) * N + min(gj, N - 1);
-        v[e][r] = mat < 3 ? ld(X + o, ij) : ld_h32(X, o, ij);
+        v[e][r] = ld(X + o, ij);
         if (gi >= N || gj >= N || (zd && gi == gj)) v[e][r] = cd{0.0, 0.0};
       }
     }
@@ -252,6 +253,81 @@ __device__ __forceinline__ void copy_slab(uint16_t (*dst)[4 * 64 * 32], const ui
   }
 }
 
+// Stage the slab (k block kb) of Hamiltonian slot `slot` (6-11: matrix
+// slot / 2 = 3 Heh, 4 Hee_o, 5 Hhh_o, orientation slot & 1), rows of the
+// 128-row tile at x0, from the complex64 buffer (stored transposed:
+// buffer[k2][k1] = H[k1][k2], so orientation 1 reads the buffer as stored
+// and orientation 0 its transpose), split as k_split splits, into dst[2]
+// [chunk] in the chunk layout.  The four waves of the copying group take
+// 512 items (tile row x, 8-k group kg), two per lane: for orientation 0
+// the lanes of a wave read 64 neighbouring columns of each buffer row (8
+// loads of 8 bytes per item); for orientation 1 with N even, 16 lanes read
+// a row's 32 k as 16-byte pairs (N odd: four lanes per row, 8-byte loads).
+// Values past N are zeros.
+__device__ __forceinline__ void copy_h(uint16_t (*dst)[4 * 64 * 32], const psa_group &g, int N, int w, int slot,
+                                       int x0, int kb, int wave) {
+  const int lane = threadIdx.x & 63, mat = slot >> 1;
+  const bool asis = (slot & 1) != 0;
+  const float2 *H = reinterpret_cast<const float2 *>(
+      reinterpret_cast<const float *>(mat == 3 ? g.Heh : (mat == 4 ? g.Heo : g.Hho)) + mat_off(g, w));
+  if (asis && (N & 1) == 0) {
+    // even N: 16 lanes per row, each a k pair (k, k + 1) as one 16-byte
+    // load, four whole 256-byte row segments per instruction; a pair leaves
+    // as one dword per plane
+    const float4 *H4 = reinterpret_cast<const float4 *>(H);
+    float4 u[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int row = x0 + wave * 32 + 4 * j + (lane >> 4), k = kb * 32 + 2 * (lane & 15);
+      u[j] = H4[((long)min(row, N - 1) * N + min(k, N - 2)) >> 1];
+      if (row >= N || k >= N) u[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int x = wave * 32 + 4 * j + (lane >> 4), kl = 2 * (lane & 15);
+      uint32_t pl[4];
+      split_pair(u[j].x, u[j].z, pl[0], pl[1]);
+      split_pair(u[j].y, u[j].w, pl[2], pl[3]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) *reinterpret_cast<uint32_t *>(&dst[x >> 6][in_chunk(q, x, kl)]) = pl[q];
+    }
+    return;
+  }
+  float2 v[2][8];
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int i = it * 256 + wave * 64 + lane;
+    const int x = asis ? i >> 2 : i & 127, kg = asis ? i & 3 : i >> 7;
+    const int row = x0 + x, k0 = kb * 32 + 8 * kg;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = k0 + q;
+      const bool ok = row < N && k < N;
+      const long ij = asis ? (long)min(row, N - 1) * N + min(k, N - 1) : (long)min(k, N - 1) * N + min(row, N - 1);
+      v[it][q] = H[ij];
+      if (!ok) v[it][q] = make_float2(0.f, 0.f);
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int i = it * 256 + wave * 64 + lane;
+    const int x = asis ? i >> 2 : i & 127, kg = asis ? i & 3 : i >> 7;
+    u32x4 pl[4];
+#pragma unroll
+    for (int q = 0; q < 8; q += 2) {
+      uint32_t rh, rl, ih, il;
+      split_pair(v[it][q].x, v[it][q + 1].x, rh, rl);
+      split_pair(v[it][q].y, v[it][q + 1].y, ih, il);
+      pl[0][q >> 1] = rh;
+      pl[1][q >> 1] = rl;
+      pl[2][q >> 1] = ih;
+      pl[3][q >> 1] = il;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) *reinterpret_cast<u32x4 *>(&dst[x >> 6][in_chunk(p, x, 8 * kg)]) = pl[p];
+  }
+}
+
 __device__ __forceinline__ bf16x8 fread(const uint16_t *half, int pl, int row, int kc) {
   return __builtin_bit_cast(
       bf16x8, *reinterpret_cast<const u32x4 *>(&half[pl * 2048 + row * 32 + ((kc ^ ((row >> 1) & 3)) << 3)]));
@@ -392,13 +468,18 @@ __device__ __forceinline__ void pp_tile(const RhsProb &p, int b, int m0, int n0,
       aI[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
     }
   // slab s of this tile into stage st by the calling group (its four waves)
+  // (the direct copies of the plane slots are issued before the register
+  // loads of a Hamiltonian slot)
   auto copy = [&](Stage &st, int s) {
     const Term tm = term(e, s / KS);
     const int kb = s % KS;
+    const bool ha = tm.sa >= 6 && tm.sa < 12, hb = tm.sb >= 6 && tm.sb < 12;
     if (tm.sa >= 12) copy_slab<true>(st.a, spl, N, w, tm.sa, m0, kb, wv4);
-    else copy_slab<false>(st.a, spl, N, w, tm.sa, m0, kb, wv4);
+    else if (!ha) copy_slab<false>(st.a, spl, N, w, tm.sa, m0, kb, wv4);
     if (tm.sb >= 12) copy_slab<true>(st.b, spl, N, w, tm.sb, n0, kb, wv4);
-    else copy_slab<false>(st.b, spl, N, w, tm.sb, n0, kb, wv4);
+    else if (!hb) copy_slab<false>(st.b, spl, N, w, tm.sb, n0, kb, wv4);
+    if (ha) copy_h(st.a, p.g, N, w, tm.sa, m0, kb, wv4);
+    if (hb) copy_h(st.b, p.g, N, w, tm.sb, n0, kb, wv4);
   };
   if (grp == 0) copy(stg[0], 0);
   __builtin_amdgcn_s_waitcnt(0);
