@@ -30,7 +30,7 @@ BF16_PEAK_TFLOPS = 2500.0  # dense
 # HBM bytes per RHS launch at 64 wires x Nk = 256 from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE runs, scaled by a 1 GiB copy's counts (x2.0, x1.0); the runs and
 # the calibration are in profiles/sbe_step_64w_nk256.md.
-RHS_TRAFFIC_BYTES = {"fp64": 2.79e9, "bf16x3": 2.08e9}
+RHS_TRAFFIC_BYTES = {"fp64": 2.79e9, "bf16x3": 1.74e9}
 HBM_PEAK_GBS = 8000.0
 
 
