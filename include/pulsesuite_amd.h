@@ -129,7 +129,9 @@ typedef struct psa_group {
                                          NR = 128 ceil(N/128), NK = 32 ceil(N/32); slot = 2 matrix +
                                          transposed (p, C_o, D_o, Heh, Hee_o, Hhh_o), then the
                                          dephasing bands Dh, De; planes re_hi, re_lo, im_hi, im_lo;
-                                         the 8-k groups of a row XOR-swizzled by (row >> 1) & 3 */
+                                         the 8-k groups of a row XOR-swizzled by (row >> 1) & 3.
+                                         Slots 6-11 (the Hamiltonians) stay unused: the RHS kernel
+                                         splits the complex64 Hamiltonians as it stages them */
   double *kscr;                       /* optional scratch for split-K partial sums of the source and
                                          propagation-grid products (NULL: no split); kscr_len doubles,
                                          8 x 2 x max(Nr x 4 Nw, (2N-1) x 3 Nw, bnx x 5 Nw) uses all splits */
